@@ -1,0 +1,210 @@
+/* riftphys_c.h — C ABI of the MI355X step library (librphys_amd.so).
+ *
+ * Every declaration in the first part keeps the name, signature, layout and error behaviour of
+ * the reference header crates/riftphys-ffi/src/riftphys_c.h (cited per entry, header lines "h:N"
+ * and implementation lines "lib.rs:N" of crates/riftphys-ffi/src/lib.rs), so a host written
+ * against the reference links against this library unchanged.  The second part holds additive
+ * exports (ABI version stays 1; no v1 layout changes).
+ */
+#ifndef RIFTPHYS_C_H
+#define RIFTPHYS_C_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPHYS_ABI_VERSION 1u
+uint32_t rphys_get_abi_version(void);                 /* h:13, lib.rs:84-94 */
+const char *rphys_get_build_id(void);                 /* h:14 */
+
+typedef uint32_t RPhysBool;
+#define RPHYS_FALSE 0u
+#define RPHYS_TRUE 1u
+
+typedef uint32_t RPhysResult;                         /* h:22-28 */
+#define RPHYS_OK 0u
+#define RPHYS_ERR_NULL 1u
+#define RPHYS_ERR_INVALID_ID 2u
+#define RPHYS_ERR_CAPACITY 3u
+#define RPHYS_ERR_BAD_ARG 4u
+#define RPHYS_ERR_INTERNAL 0x80000000u
+
+#define RPHYS_INVALID_BODY_ID 0u                      /* h:30-35 */
+#define RPHYS_TERRAIN_BODY_ID 0xFFFFFFFFu
+
+const char *rphys_last_error_message(void);           /* h:38, lib.rs:56-82 (thread-local) */
+
+typedef struct RPhysWorld RPhysWorld;
+
+typedef struct RPhysVec3 { float x, y, z; } RPhysVec3;             /* h:46-60 */
+typedef struct RPhysQuat { float x, y, z, w; } RPhysQuat;
+typedef struct RPhysIsometry { RPhysVec3 pos; RPhysQuat rot; } RPhysIsometry;
+typedef struct RPhysVelocity { RPhysVec3 lin; RPhysVec3 ang; } RPhysVelocity;
+typedef struct RPhysMaterial { float mu_static; float mu_dynamic; float restitution; } RPhysMaterial;
+
+typedef uint32_t RPhysBodyType;                       /* h:66-68 */
+#define RPHYS_BODY_STATIC 0u
+#define RPHYS_BODY_DYNAMIC 1u
+
+typedef struct RPhysCapsuleBodyDesc {                 /* h:70-79 */
+    RPhysIsometry pose; RPhysVelocity vel; float radius; float half_height; float mass;
+    RPhysBodyType body_type; RPhysMaterial material; uint32_t user_tag;
+} RPhysCapsuleBodyDesc;
+typedef struct RPhysBoxBodyDesc {                     /* h:81-89 */
+    RPhysIsometry pose; RPhysVelocity vel; float hx, hy, hz; float mass;
+    RPhysBodyType body_type; RPhysMaterial material; uint32_t user_tag;
+} RPhysBoxBodyDesc;
+typedef struct RPhysSphereBodyDesc {                  /* h:91-99 */
+    RPhysIsometry pose; RPhysVelocity vel; float radius; float mass;
+    RPhysBodyType body_type; RPhysMaterial material; uint32_t user_tag;
+} RPhysSphereBodyDesc;
+
+#define RPHYS_STEP_HASH_BYTES 32
+typedef struct RPhysHash32 { uint8_t bytes[RPHYS_STEP_HASH_BYTES]; } RPhysHash32;  /* h:103-107 */
+
+typedef struct RPhysStepParams {                      /* h:111-117 */
+    uint32_t size_bytes; uint32_t flags; uint32_t substeps; uint32_t solver_iters; float dt;
+} RPhysStepParams;
+
+typedef struct RPhysRayQuery { RPhysVec3 origin; RPhysVec3 dir; float max_distance; uint32_t ignore_body; } RPhysRayQuery;
+typedef struct RPhysRayHit { RPhysBool hit; uint32_t body_id; float fraction; RPhysVec3 point; RPhysVec3 normal; } RPhysRayHit;
+typedef struct RPhysCapsuleSweepQuery { RPhysVec3 from; RPhysVec3 to; float radius; float half_height; uint32_t ignore_body; } RPhysCapsuleSweepQuery;
+typedef struct RPhysCapsuleSweepHit { RPhysBool hit; uint32_t body_id; float fraction; RPhysBool started_overlapping; RPhysVec3 point; RPhysVec3 normal; } RPhysCapsuleSweepHit;
+
+typedef struct RPhysHeightfieldDesc {                 /* h:155-166 */
+    uint32_t size_bytes; uint32_t flags; uint32_t width; uint32_t height; uint32_t row_stride;
+    float cell_size_x; float cell_size_z; float height_scale; float height_offset;
+    const int16_t *heights;
+} RPhysHeightfieldDesc;
+
+/* h:168-172, lib.rs:1419-1493 */
+RPhysResult rphys_world_set_heightfield_i16(RPhysWorld *world, const RPhysHeightfieldDesc *hf, float origin_x, float origin_z, float y_offset);
+/* h:174-182, lib.rs:1495-1549 */
+RPhysResult rphys_world_set_heightfield_raw32_square(RPhysWorld *world, const uint8_t *bytes, uint32_t bytes_len, float world_size_x,
+                                                     float world_size_z, float origin_x, float origin_z, float y_offset);
+RPhysResult rphys_world_clear_heightfield(RPhysWorld *world);          /* h:184, lib.rs:1551-1567 */
+
+RPhysWorld *rphys_world_create(uint32_t max_bodies, uint32_t max_colliders);  /* h:189, lib.rs:608-631 */
+void rphys_world_destroy(RPhysWorld *world);                         /* h:190, lib.rs:633-639 */
+RPhysResult rphys_world_set_epoch(RPhysWorld *world, uint64_t epoch);    /* h:191 */
+RPhysResult rphys_world_set_rng_seed(RPhysWorld *world, uint64_t seed);  /* h:192 */
+
+/* h:195-198, lib.rs:677-927 (1-based ids, density of MaterialId::Default) */
+RPhysResult rphys_add_capsule_body(RPhysWorld *world, const RPhysCapsuleBodyDesc *desc, uint32_t *out_body_id);
+RPhysResult rphys_add_box_body(RPhysWorld *world, const RPhysBoxBodyDesc *desc, uint32_t *out_body_id);
+RPhysResult rphys_add_sphere_body(RPhysWorld *world, const RPhysSphereBodyDesc *desc, uint32_t *out_body_id);
+RPhysResult rphys_body_remove(RPhysWorld *world, uint32_t body_id);
+
+/* h:201-208, lib.rs:929-1101 */
+RPhysResult rphys_body_get_pose(const RPhysWorld *world, uint32_t body_id, RPhysIsometry *out_pose);
+RPhysResult rphys_body_set_pose(RPhysWorld *world, uint32_t body_id, const RPhysIsometry *pose);
+RPhysResult rphys_body_get_velocity(const RPhysWorld *world, uint32_t body_id, RPhysVelocity *out_vel);
+RPhysResult rphys_body_set_velocity(RPhysWorld *world, uint32_t body_id, const RPhysVelocity *vel);
+RPhysResult rphys_body_apply_impulse(RPhysWorld *world, uint32_t body_id, RPhysVec3 impulse);
+
+/* h:211-212, lib.rs:1103-1164: every substep is one device tick */
+RPhysResult rphys_world_step_ex(RPhysWorld *world, const RPhysStepParams *p);
+RPhysResult rphys_world_step(RPhysWorld *world, float dt);
+
+/* h:215-218, lib.rs:1166-1266: the hash is computed on the device */
+RPhysResult rphys_world_tick_index(const RPhysWorld *world, uint64_t *out_tick);
+RPhysResult rphys_world_num_bodies(const RPhysWorld *world, uint32_t *out_num);
+RPhysResult rphys_world_get_step_hash(const RPhysWorld *world, RPhysHash32 *out_hash);
+RPhysResult rphys_world_step_and_hash(RPhysWorld *world, float dt, RPhysHash32 *out_hash);
+
+/* h:221-222: queries are outside this library's scope; they write a miss and return
+ * RPHYS_ERR_INTERNAL with a message */
+RPhysResult rphys_world_raycast(const RPhysWorld *world, const RPhysRayQuery *query, RPhysRayHit *out_hit);
+RPhysResult rphys_world_capsule_sweep(const RPhysWorld *world, const RPhysCapsuleSweepQuery *query, RPhysCapsuleSweepHit *out_hit);
+
+RPhysResult rphys_last_error_copy(char *dst, uint32_t cap, uint32_t *out_len);   /* h:225, lib.rs:561-604 */
+
+/* lib.rs:107-115: the reference uses this type in rphys_world_get_contacts but its header never
+ * defines it; 36 bytes, fields in the reference's order. */
+typedef struct RPhysContactEvent {
+    uint32_t body_id_a; uint32_t body_id_b; RPhysVec3 position; RPhysVec3 normal; float impulse;
+} RPhysContactEvent;
+/* h:248-253, lib.rs:1801-1878: contacts of the last tick (count only when out_buffer is NULL) */
+RPhysResult rphys_world_get_contacts(const RPhysWorld *world, RPhysContactEvent *out_buffer, uint32_t buffer_capacity, uint32_t *out_count);
+
+typedef struct RPhysWorldSnapshot RPhysWorldSnapshot;                    /* h:256-260, lib.rs:1696-1800 */
+RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *world);
+RPhysResult rphys_world_snapshot_restore(RPhysWorld *world, const RPhysWorldSnapshot *snapshot);
+void rphys_world_snapshot_free(RPhysWorldSnapshot *snapshot);
+
+/* ============================ additive exports (no counterpart in the reference ABI) ========== */
+
+/* RPhysStepParams.flags bit: take solver_iters (> 0) as the velocity iterations of this call.
+ * Without it solver_iters is ignored, as in the reference (lib.rs:1147). */
+#define RPHYS_STEP_FLAG_SOLVER_ITERS 1u
+
+typedef uint32_t RPhysShape;
+#define RPHYS_SHAPE_SPHERE 0u
+#define RPHYS_SHAPE_BOX 1u
+#define RPHYS_SHAPE_CAPSULE 2u
+
+/* A body with an explicit density in kg/m^3 (<= 0: the reference default 1500), the only way
+ * to express other densities (mass.rs:92-165 takes density; the v1 descs' mass is ignored).
+ * dims: sphere {r}, box {hx,hy,hz}, capsule {r, half_height}. */
+RPhysResult rphys_add_body_density(RPhysWorld *world, RPhysShape shape, const float dims[3], const RPhysIsometry *pose,
+                                   const RPhysVelocity *vel, RPhysBodyType body_type, float density, uint32_t *out_body_id);
+/* World::set_pair_props (world.rs:975-979): friction/restitution of a material-id pair.
+ * FFI bodies use MaterialId::Default (0); heightfields use Grit (4). */
+RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
+                                       float restitution, float stribeck_v);
+/* World::set_solver_iterations (world.rs:994-996), (velocity, position), default (20, 20) */
+RPhysResult rphys_world_set_solver_iterations(RPhysWorld *world, uint32_t velocity, uint32_t position);
+RPhysResult rphys_world_set_gravity(RPhysWorld *world, RPhysVec3 g);          /* World::set_gravity (world.rs:560-563) */
+RPhysResult rphys_body_set_sleep_allowed(RPhysWorld *world, uint32_t body_id, RPhysBool allowed);  /* world.rs:565-570 */
+/* n ticks of dt enqueued back to back with one host synchronisation at the end */
+RPhysResult rphys_world_step_n(RPhysWorld *world, float dt, uint32_t n_ticks);
+
+typedef struct RPhysStepStats {
+    uint32_t pairs;              /* broadphase pairs (StepStats.pairs_tested) */
+    uint32_t contacts;           /* contacts after cull (StepStats.contacts) */
+    uint32_t terrain_candidates; /* (collider, triangle) tests */
+    uint32_t island_contacts;    /* contacts writing at least one body */
+    uint32_t islands;
+    uint32_t warm_entries;       /* warm-start cache entries after the tick */
+} RPhysStepStats;
+RPhysResult rphys_world_get_stats(const RPhysWorld *world, RPhysStepStats *out);
+
+/* One contact with its build-defined feature id (kind<<24 | which<<16 | k, bit 23 = swapped
+ * shapes) and terrain cell ((gz*(nx-1)+gx)*2+tri; 0xFFFFFFFF for body pairs).  Colliders are
+ * 0-based collider indices (the reference's Contact.a_collider / b_collider). */
+typedef struct RPhysContactEx {
+    uint32_t a_collider, b_collider;
+    float point[3];
+    float normal[3];
+    float depth;
+    float impulse;
+    uint32_t feature;
+    uint32_t cell;
+} RPhysContactEx;
+RPhysResult rphys_debug_get_contacts_ex(const RPhysWorld *world, RPhysContactEx *out, uint32_t cap, uint32_t *out_count);
+/* Broadphase pairs (collider indices i < j) of the last tick, in the reference's sorted order. */
+RPhysResult rphys_debug_get_pairs(const RPhysWorld *world, uint32_t *out_ij, uint32_t cap_pairs, uint32_t *out_count);
+
+/* ---- batches: many independent worlds stepped by one launch sequence ---- */
+typedef struct RPhysBatch RPhysBatch;
+/* Builds a batch from n_worlds existing worlds (their bodies, heightfields, settings and carried
+ * state are copied; the worlds stay usable).  Settings (iterations, gravity, pair props) are
+ * taken from world 0. */
+RPhysBatch *rphys_batch_create(RPhysWorld *const *worlds, uint32_t n_worlds);
+void rphys_batch_destroy(RPhysBatch *batch);
+RPhysResult rphys_batch_step_n(RPhysBatch *batch, float dt, uint32_t n_ticks);
+/* out: 32 * n_worlds bytes */
+RPhysResult rphys_batch_get_hashes(const RPhysBatch *batch, uint8_t *out);
+RPhysResult rphys_batch_get_state(const RPhysBatch *batch, uint32_t world_index, float *pos3, float *rot4, float *lin3, float *ang3);
+RPhysResult rphys_batch_snapshot(RPhysBatch *batch, uint32_t slot);   /* device-side copy of all worlds' state */
+RPhysResult rphys_batch_restore(RPhysBatch *batch, uint32_t slot);
+RPhysResult rphys_batch_total_bodies(const RPhysBatch *batch, uint32_t *out);
+/* Record the next tick of the batch into a HIP graph and replay it for n ticks (same results as
+ * rphys_batch_step_n; launch overhead paid once). */
+RPhysResult rphys_batch_step_graph(RPhysBatch *batch, float dt, uint32_t n_ticks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1271 @@
+/* Oracle (TEST INFRASTRUCTURE ONLY): C restatement of the RiftPhys reference step.
+ *
+ * Follows, in evaluation order:
+ *   World::step                       crates/riftphys-world/src/world/step.rs:40-420
+ *   S1 forces + gravity::eval Uniform step.rs:53-68, crates/riftphys-gravity/src/gravity.rs:79-88
+ *   update_colliders_aabb / aabb_of   world.rs:793-798, crates/riftphys-geom/src/shape.rs:18-76,
+ *                                     crates/riftphys-geom/src/aabb.rs:21-31
+ *   sweep + swept AABB                step.rs:75-81, world.rs:97-100
+ *   GridBroadphase::run               crates/riftphys-collision/src/broadphase.rs:29-99 (restated
+ *                                     as its equivalent: the sorted set of inclusively overlapping
+ *                                     (i<j) pairs of colliders with finite min.x, plus oversized)
+ *   narrowphase (pairs, terrain)      step.rs:95-142, narrowphase.rs (or_narrow.h)
+ *   HeightField::query_aabb           crates/riftphys-terrain/src/terrain.rs:300-359
+ *   cull_and_orient_contacts          step.rs:499-542
+ *   build_islands (non-empty test)    islands.rs:27-92
+ *   solve_islands_parallel            solver.rs:477-1008, 1247-1271 (e=0: W6 inert)
+ *   S4 integrate + gyroscopic turn    step.rs:154-280
+ *   S5 contact projection             step.rs:284-310, 423-497
+ *   S6 velocity from corrections      step.rs:312-379
+ *   S7 sleep + AABB refresh           sleep.rs:59-80, step.rs:381-384
+ *   step_hash + hash_carried          provenance.rs:6-24, world.rs:882-909
+ *   snapshots                         ffi/src/lib.rs:1696-1800, world.rs:912-961
+ *
+ * The PGS here runs serially in contact-index order; the reference's level schedule
+ * (solver.rs:621-642) is proven by its own construction to give the same result (constraints
+ * in one level share no written body), and the level count is computed for the stats.
+ *
+ * Parity pins (tests/test_oracle_*.py): BLAKE3 published digests; XorShift64 closed form;
+ * broadphase vs brute-force O(N^2) overlap; narrowphase unit tests restated from
+ * narrowphase.rs:735-776 and collision/tests/box_sphere_inside_test.rs; terrain query yields
+ * (terrain.rs:360-394); closed-form free fall and resting heights (contact_resting_test.rs:30-45,
+ * reality/src/analytic.rs free fall); invariants (manifold <= 4 points, n.(pb-pa) >= 0,
+ * acc_jn >= 0, |jt| <= mu jn, snapshot/restore hash equality).  glam's SSE2 operation order and
+ * the platform libm are third-party (glam 0.27.0, libm 0.2.15): no reference test pins the
+ * contact path's bits, so bit-level agreement with the Rust build is "parity unpinned" for the
+ * contact scenes; the oracle is pinned by the checks above.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+#include "or_math.h"
+#include "or_narrow.h"
+
+#define TERRAIN_CI 0xFFFFFFFFu
+static const float MARGIN = 0.05f;     /* step.rs:12 MELEE_CCD_MARGIN */
+static const float POS_SLOP = 0.005f;  /* step.rs:16 */
+
+/* ------------------------------------------------------------------ quantisers */
+/* det.rs:8-13 q6: (x / 1e-6).round() * 1e-6, non-finite -> 0 */
+static float q6(float x) { if (!isfinite(x)) return 0.0f; return roundf(x / 1.0e-6f) * 1.0e-6f; }
+/* determinism.rs:57-65 qf32 with q = 1e-6 */
+static float qf32(float x) { if (!isfinite(x)) return NAN; return roundf(x / 1.0e-6f) * 1.0e-6f; }
+/* materials.rs:319-321 quantize: (x * 1e6).round() * 1e-6 */
+static float mquant(float x) { return roundf(x * 1.0e6f) * 1.0e-6f; }
+/* gravity.rs:43-49 */
+static float gq6(float x) { return isfinite(x) ? roundf(x * 1.0e6f) * 1.0e-6f : 0.0f; }
+static uint32_t canon_bits(float x) {
+    if (!isfinite(x)) return 0x7fc00000u;
+    uint32_t b; memcpy(&b, &x, 4);
+    return b == 0x80000000u ? 0u : b;
+}
+static float fclampf(float x, float lo, float hi) { if (x < lo) x = lo; if (x > hi) x = hi; return x; }
+
+/* ------------------------------------------------------------------ materials */
+/* materials.rs:93-240: the core entries (ids 0..15) the FFI can reach: Default and Grit */
+typedef struct { float mu_s, mu_k, rest, vs; } MatP;
+static const MatP MATS[16] = {
+    {0.60f, 0.50f, 0.10f, 0.08f}, {0.05f, 0.03f, 0.02f, 0.02f}, {1.10f, 0.95f, 0.80f, 0.15f},
+    {0.70f, 0.55f, 0.50f, 0.05f}, {1.30f, 1.05f, 0.00f, 0.12f}, {0.55f, 0.45f, 0.08f, 0.10f},
+    {0.80f, 0.70f, 0.00f, 0.10f}, {0.90f, 0.80f, 0.00f, 0.12f}, {0.80f, 0.60f, 0.20f, 0.10f},
+    {0.55f, 0.45f, 0.10f, 0.06f}, {1.10f, 0.95f, 0.00f, 0.18f}, {0.40f, 0.30f, 0.65f, 0.04f},
+    {0.90f, 0.60f, 0.35f, 0.06f}, {0.61f, 0.47f, 0.40f, 0.05f}, {0.50f, 0.40f, 0.12f, 0.08f},
+    {0.20f, 0.15f, 0.85f, 0.04f}};
+enum { MAT_DEFAULT = 0, MAT_ICE = 1, MAT_RUBBER = 2, MAT_STEEL = 3, MAT_GRIT = 4, MAT_CONCRETE = 6, MAT_ASPHALT = 7 };
+
+typedef struct { float mu_s, mu_k, rest, vs; } PairP;
+/* materials.rs:278-304 mix_pair + pair_props + override_for */
+static PairP table_pair(uint32_t a, uint32_t b) {
+    MatP pa = MATS[a & 15], pb = MATS[b & 15];
+    PairP r;
+    r.mu_s = mquant(sqrtf(fabsf(pa.mu_s * pb.mu_s)));
+    r.mu_k = mquant(sqrtf(fabsf(pa.mu_k * pb.mu_k)));
+    r.rest = mquant(fmaxf(pa.rest, pb.rest));
+    r.vs = mquant(0.5f * (pa.vs + pb.vs));
+    uint32_t x = a < b ? a : b, y = a < b ? b : a;
+    int ov = 0; PairP o = r;
+    if (x == MAT_ICE && y == MAT_RUBBER) { ov = 1; o.mu_s = 0.20f; o.mu_k = 0.10f; o.rest = 0.05f; o.vs = 0.08f; }
+    if (x == MAT_RUBBER && y == MAT_ASPHALT) { ov = 1; o.mu_s = 1.10f; o.mu_k = 0.95f; o.rest = 0.00f; o.vs = 0.18f; }
+    if (x == MAT_RUBBER && y == MAT_CONCRETE) { ov = 1; o.mu_s = 1.00f; o.mu_k = 0.85f; o.rest = 0.00f; o.vs = 0.16f; }
+    if (x == MAT_STEEL && y == MAT_STEEL) { ov = 1; o.mu_s = 0.58f; o.mu_k = 0.50f; o.rest = 0.08f; o.vs = 0.04f; }
+    if (ov) { o.mu_s = mquant(o.mu_s); o.mu_k = mquant(o.mu_k); o.rest = mquant(o.rest); o.vs = mquant(o.vs); return o; }
+    return r;
+}
+
+/* ------------------------------------------------------------------ world */
+typedef struct { V3 min, max; } Aabb;
+typedef struct {
+    uint32_t a, b;  /* collider indices; b = TERRAIN_CI for terrain */
+    V3 point, normal;
+    float depth, impulse;
+    uint32_t env_mat, feat, cell;
+} Contact;
+typedef struct { uint32_t key[4]; V3 la; float jn; V3 jt; } Warm;
+typedef struct { uint32_t a, b; PairP p; } PairOverride;
+
+struct OrWorld {
+    uint32_t cap, n;
+    V3 *pos; Q4 *rot; V3 *lin, *ang; float *inv_mass; uint8_t *dyn; M3 *iil;
+    uint8_t *alive; V3 *last_lin;
+    uint8_t *awake; uint16_t *sticks; uint8_t *sallow;
+    /* colliders (one per body, in body order, removed bodies drop out) */
+    uint32_t nc; uint32_t *cbody; ShapeD *cshape; uint32_t *cmat; Aabb *caabb;
+    /* heightfield */
+    int has_hf; uint32_t hnx, hnz; float hsx, hsz, hox, hoz, hmin, hmax; float *hts; uint32_t hmat;
+    V3 gravity; uint64_t epoch, rng, tick;
+    uint32_t iv, ip;
+    int resid_on; float resid_tol; uint32_t resid_max;
+    PairOverride ovr[32]; uint32_t novr;
+    Warm *warm; uint32_t nwarm;
+    /* per-step outputs */
+    uint32_t *pairs; uint32_t npairs;
+    Contact *contacts; uint32_t ncontacts;
+    uint8_t *loaded; int loaded_valid;
+    OrStats stats;
+};
+
+static void *xrealloc(void *p, size_t n) { void *q = realloc(p, n ? n : 1); if (!q) abort(); return q; }
+
+OrWorld *or_world_create(uint32_t cap) {
+    OrWorld *w = (OrWorld *)calloc(1, sizeof(OrWorld));
+    w->cap = cap;
+    w->pos = calloc(cap + 1, sizeof(V3)); w->rot = calloc(cap + 1, sizeof(Q4));
+    w->lin = calloc(cap + 1, sizeof(V3)); w->ang = calloc(cap + 1, sizeof(V3));
+    w->inv_mass = calloc(cap + 1, 4); w->dyn = calloc(cap + 1, 1); w->iil = calloc(cap + 1, sizeof(M3));
+    w->alive = calloc(cap + 1, 1); w->last_lin = calloc(cap + 1, sizeof(V3));
+    w->awake = calloc(cap + 1, 1); w->sticks = calloc(cap + 1, 2); w->sallow = calloc(cap + 1, 1);
+    for (uint32_t i = 0; i < cap; i++) { w->awake[i] = 1; w->sallow[i] = 1; }
+    w->cbody = calloc(cap + 1, 4); w->cshape = calloc(cap + 1, sizeof(ShapeD));
+    w->cmat = calloc(cap + 1, 4); w->caabb = calloc(cap + 1, sizeof(Aabb));
+    w->loaded = calloc(cap + 1, 1);
+    w->gravity = v3(0.0f, -9.81f, 0.0f);
+    w->rng = 0xC0FFEEull | 1ull; /* world.rs:413 XorShift64::new(0xC0FFEE) */
+    w->iv = 20; w->ip = 20;      /* world.rs:425 */
+    return w;
+}
+
+void or_world_destroy(OrWorld *w) {
+    if (!w) return;
+    free(w->pos); free(w->rot); free(w->lin); free(w->ang); free(w->inv_mass); free(w->dyn); free(w->iil);
+    free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow);
+    free(w->cbody); free(w->cshape); free(w->cmat); free(w->caabb); free(w->loaded);
+    free(w->hts); free(w->warm); free(w->pairs); free(w->contacts);
+    free(w);
+}
+
+void or_set_epoch(OrWorld *w, uint64_t e) { w->epoch = e; }
+void or_set_rng_seed(OrWorld *w, uint64_t s) { w->rng = s | 1ull; }
+void or_set_gravity(OrWorld *w, float x, float y, float z) { w->gravity = v3(x, y, z); }
+/* world.rs:994-996 */
+void or_set_iterations(OrWorld *w, uint32_t v, uint32_t p) { w->iv = v ? v : 1; w->ip = p ? p : 1; }
+void or_set_residual_mode(OrWorld *w, int on, float tol, uint32_t max_iters) { w->resid_on = on; w->resid_tol = tol; w->resid_max = max_iters; }
+/* world.rs:975-986 */
+void or_set_pair_props(OrWorld *w, uint32_t a, uint32_t b, float mu_s, float mu_k, float rest, float vs) {
+    uint32_t x = a < b ? a : b, y = a < b ? b : a;
+    for (uint32_t i = 0; i < w->novr; i++) if (w->ovr[i].a == x && w->ovr[i].b == y) { w->ovr[i] = w->ovr[--w->novr]; break; }
+    if (w->novr >= 32) return;
+    PairOverride o; o.a = x; o.b = y; o.p.mu_s = mu_s; o.p.mu_k = mu_k; o.p.rest = rest; o.p.vs = vs;
+    /* the reference keeps overrides in a Vec and finds the first match; order of distinct keys is irrelevant */
+    w->ovr[w->novr++] = o;
+}
+static PairP pair_props(const OrWorld *w, uint32_t a, uint32_t b) {
+    uint32_t x = a < b ? a : b, y = a < b ? b : a;
+    for (uint32_t i = 0; i < w->novr; i++) if (w->ovr[i].a == x && w->ovr[i].b == y) return w->ovr[i].p;
+    return table_pair(a, b);
+}
+static PairP contact_props(const OrWorld *w, const Contact *c) {
+    return pair_props(w, w->cmat[c->a], c->b == TERRAIN_CI ? c->env_mat : w->cmat[c->b]);
+}
+
+/* dynamics.rs:14-25 */
+static float dfin(float x) { return isfinite(x) ? x : 0.0f; }
+static V3 dfin_v(V3 v) { return v3(dfin(v.x), dfin(v.y), dfin(v.z)); }
+static Q4 fin_q(Q4 q) {
+    if (!qfinite(q)) return qident();
+    Q4 qq = qnormalize(q);
+    if (!qfinite(qq)) return qident();
+    if (qq.w < 0.0f) qq = qneg(qq);
+    return qq;
+}
+/* dynamics.rs:130-147 */
+static void b_set_pose(OrWorld *w, uint32_t i, V3 p, Q4 q) {
+    if (vfinite(p)) w->pos[i] = dfin_v(p);
+    if (qfinite(q)) w->rot[i] = fin_q(q);
+}
+static void b_set_vel(OrWorld *w, uint32_t i, V3 l, V3 a) {
+    if (vfinite(l)) w->lin[i] = dfin_v(l);
+    if (vfinite(a)) w->ang[i] = dfin_v(a);
+}
+/* dynamics.rs:158-165 */
+static M3 inv_inertia_world(const OrWorld *w, uint32_t i) {
+    if (w->inv_mass[i] == 0.0f) return m3zero();
+    M3 r = m3quat(w->rot[i]);
+    return m3mul(m3mul(r, w->iil[i]), m3t(r));
+}
+static M3 inv_inertia_local_of(const OrWorld *w, uint32_t i) {
+    if (w->inv_mass[i] == 0.0f) return m3zero();
+    return w->iil[i];
+}
+/* dynamics.rs:193-205 */
+static void apply_impulse(OrWorld *w, uint32_t i, V3 j) {
+    float im = w->inv_mass[i];
+    if (im > 0.0f && vfinite(j)) w->lin[i] = dfin_v(vadd(w->lin[i], vmul(j, im)));
+}
+static void apply_ang_impulse(OrWorld *w, uint32_t i, V3 tau) {
+    if (w->inv_mass[i] > 0.0f && vfinite(tau)) {
+        V3 dw = m3v(inv_inertia_world(w, i), tau);
+        w->ang[i] = dfin_v(vadd(w->ang[i], dw));
+    }
+}
+
+/* shape.rs:18-76 + aabb.rs:21-31 */
+static Aabb aabb_new(V3 a, V3 b) {
+    V3 qa = v3(qf32(a.x), qf32(a.y), qf32(a.z)), qb = v3(qf32(b.x), qf32(b.y), qf32(b.z));
+    Aabb r; r.min = vmin(qa, qb); r.max = vmax(qa, qb); return r;
+}
+static Aabb aabb_of(ShapeD s, V3 p, Q4 q) {
+    V3 he;
+    if (s.type == OR_SPHERE) he = v3s(fabsf(s.d[0]));
+    else if (s.type == OR_BOX) {
+        V3 e = v3(fabsf(s.d[0]), fabsf(s.d[1]), fabsf(s.d[2]));
+        M3 r = m3quat(q);
+        M3 ar = m3cols(vabs(r.c0), vabs(r.c1), vabs(r.c2));
+        he = m3v(ar, e);
+    } else {
+        float rr = fabsf(s.d[0]), h = fabsf(s.d[1]);
+        V3 axw = vmul(qrot(q, v3(0, 1, 0)), h);
+        he = vadd(vabs(axw), v3s(rr));
+    }
+    return aabb_new(vsub(p, he), vadd(p, he));
+}
+
+/* mass.rs:92-165 (density-based primitives) */
+static void mass_props(ShapeD s, float rho, float *mass, V3 *idiag, int *ok) {
+    *ok = 0;
+    if (s.type == OR_SPHERE) {
+        float r = fabsf(s.d[0]);
+        if (!(isfinite(r) && r > 0 && isfinite(rho) && rho > 0)) return;
+        float r2 = r * r;
+        float vol = ((4.0f / 3.0f) * 3.14159265358979323846f * r2) * r;
+        float m = rho * vol;
+        float ii = (0.4f * m) * r2;
+        *mass = m; *idiag = v3s(ii);
+    } else if (s.type == OR_BOX) {
+        V3 he = v3(fabsf(s.d[0]), fabsf(s.d[1]), fabsf(s.d[2]));
+        if (!(vfinite(he) && isfinite(rho) && rho > 0)) return;
+        V3 d = vmul(he, 2.0f);
+        if (!(d.x > 0 && d.y > 0 && d.z > 0)) return;
+        float vol = (d.x * d.y) * d.z;
+        float m = rho * vol;
+        float x2 = d.x * d.x, y2 = d.y * d.y, z2 = d.z * d.z;
+        *mass = m;
+        *idiag = v3(((1.0f / 12.0f) * m) * (y2 + z2), ((1.0f / 12.0f) * m) * (x2 + z2), ((1.0f / 12.0f) * m) * (x2 + y2));
+    } else {
+        float r = fabsf(s.d[0]), hh = fabsf(s.d[1]);
+        if (!(isfinite(r) && r > 0 && isfinite(rho) && rho > 0)) return;
+        float h = hh * 2.0f, r2 = r * r;
+        float pi = 3.14159265358979323846f;
+        float vc = (pi * r2) * h;
+        float vsph = ((4.0f / 3.0f) * pi * r2) * r;
+        float m = rho * (vc + vsph);
+        float ix = ((0.25f * m) * r2) + (((1.0f / 12.0f) * m) * h) * h;
+        float iy = (0.5f * m) * r2;
+        *mass = m; *idiag = v3(ix, iy, ix);
+    }
+    /* MassProps::finalize */
+    if (!(isfinite(*mass) && *mass > 0.0f) || *mass < 1.0e-12f) return;
+    if (!(isfinite(idiag->x) && idiag->x > 0 && isfinite(idiag->y) && idiag->y > 0 && isfinite(idiag->z) && idiag->z > 0)) return;
+    *ok = 1;
+}
+
+/* ffi/src/lib.rs:677-902 add_*_body -> world.rs:595-622 add_body/add_collider */
+int32_t or_add_body(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic, float density) {
+    if (w->n >= w->cap) return -1;
+    uint32_t i = w->n++;
+    ShapeD s; s.type = shape; s.d[0] = dims[0]; s.d[1] = dims[1]; s.d[2] = dims[2];
+    float rho = density > 0.0f ? density : 1500.0f; /* materials.rs:362 MaterialId::Default density */
+    float mass = 0; V3 idiag = v3s(0); int ok = 0;
+    if (dynamic) mass_props(s, rho, &mass, &idiag, &ok);
+    float inv_mass = (dynamic && ok) ? 1.0f / mass : 0.0f;
+    M3 inertia = ok ? m3diag(idiag) : m3zero();
+    float det = m3det(inertia);
+    int have_ii = dynamic && isfinite(det) && det > 0.0f;
+    M3 iil = have_ii ? m3inv(inertia) : m3zero();
+    /* dynamics.rs:98-119 Bodies::add */
+    int is_dyn = dynamic && isfinite(inv_mass) && inv_mass > 0.0f;
+    float im = is_dyn ? dfin(inv_mass) : 0.0f;
+    V3 p = v3(pose[0], pose[1], pose[2]);
+    Q4 q = q4(pose[3], pose[4], pose[5], pose[6]);
+    V3 l = v3(vel[0], vel[1], vel[2]), a = v3(vel[3], vel[4], vel[5]);
+    w->pos[i] = vfinite(p) ? p : v3s(0);
+    w->rot[i] = fin_q(q);
+    w->lin[i] = vfinite(l) ? l : v3s(0);
+    w->ang[i] = vfinite(a) ? a : v3s(0);
+    w->inv_mass[i] = im;
+    w->dyn[i] = (uint8_t)is_dyn;
+    if (!is_dyn) w->iil[i] = m3zero();
+    else if (have_ii && m3finite(iil)) w->iil[i] = iil;
+    else w->iil[i] = m3diag(v3s(im));
+    w->last_lin[i] = l;
+    if (i >= w->cap) { /* capacity grows only through the FFI check; never here */ }
+    w->awake[i] = 1; w->sticks[i] = 0;
+    w->alive[i] = 1;
+    uint32_t c = w->nc++;
+    w->cbody[c] = i; w->cshape[c] = s; w->cmat[c] = MAT_DEFAULT;
+    w->caabb[c] = aabb_of(s, w->pos[i], w->rot[i]);
+    return (int32_t)i;
+}
+
+static void drop_warm_of(OrWorld *w, uint32_t b) {
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < w->nwarm; i++)
+        if (w->warm[i].key[0] != b && w->warm[i].key[1] != b) w->warm[k++] = w->warm[i];
+    w->nwarm = k;
+}
+
+/* world.rs:696-713 remove_body */
+int or_remove_body(OrWorld *w, uint32_t id) {
+    if (id >= w->n || !w->alive[id]) return 0;
+    w->alive[id] = 0;
+    w->lin[id] = v3s(0); w->ang[id] = v3s(0); w->inv_mass[id] = 0; w->dyn[id] = 0; w->iil[id] = m3zero();
+    uint32_t k = 0;
+    for (uint32_t c = 0; c < w->nc; c++) {
+        if (w->cbody[c] == id) continue;
+        w->cbody[k] = w->cbody[c]; w->cshape[k] = w->cshape[c]; w->cmat[k] = w->cmat[c]; w->caabb[k] = w->caabb[c]; k++;
+    }
+    w->nc = k;
+    drop_warm_of(w, id);
+    return 1;
+}
+
+/* terrain.rs:24-52 from_heights (heights already include offsets); ffi/src/lib.rs:1479,1536 use Grit */
+void or_set_heightfield(OrWorld *w, uint32_t nx, uint32_t nz, float sx, float sz, float ox, float oz, const float *h, uint32_t mat) {
+    free(w->hts);
+    w->hts = malloc(sizeof(float) * (size_t)nx * nz);
+    w->hnx = nx; w->hnz = nz;
+    w->hsx = (isfinite(sx) && sx > 0) ? q6(sx) : 1.0f;
+    w->hsz = (isfinite(sz) && sz > 0) ? q6(sz) : 1.0f;
+    w->hox = ox; w->hoz = oz;
+    float mn = INFINITY, mx = -INFINITY;
+    for (size_t i = 0; i < (size_t)nx * nz; i++) {
+        float v = isfinite(h[i]) ? h[i] : 0.0f;
+        v = q6(v);
+        w->hts[i] = v;
+        mn = fminf(mn, v); mx = fmaxf(mx, v);
+    }
+    w->hmin = q6(mn); w->hmax = q6(mx);
+    w->hmat = mat;
+    w->has_hf = 1;
+}
+
+/* world.rs:501-513 set_body_pose (drops the body's warm entries, refreshes its AABB) */
+void or_set_body_pose(OrWorld *w, uint32_t id, const float *pose) {
+    drop_warm_of(w, id);
+    b_set_pose(w, id, v3(pose[0], pose[1], pose[2]), q4(pose[3], pose[4], pose[5], pose[6]));
+    for (uint32_t c = 0; c < w->nc; c++)
+        if (w->cbody[c] == id) w->caabb[c] = aabb_of(w->cshape[c], v3(pose[0], pose[1], pose[2]), q4(pose[3], pose[4], pose[5], pose[6]));
+}
+/* world.rs:674-676 set_body_vel */
+void or_set_body_vel(OrWorld *w, uint32_t id, const float *vel) {
+    b_set_vel(w, id, v3(vel[0], vel[1], vel[2]), v3(vel[3], vel[4], vel[5]));
+    w->last_lin[id] = v3(vel[0], vel[1], vel[2]);
+}
+void or_set_sleep_allowed(OrWorld *w, uint32_t id, int allowed) {
+    w->sallow[id] = (uint8_t)(allowed != 0);
+    if (!allowed) { w->awake[id] = 1; w->sticks[id] = 0; }
+}
+
+/* ------------------------------------------------------------------ broadphase */
+static int aabb_overlaps(const Aabb *a, const Aabb *b) {
+    return !(a->max.x < b->min.x || a->min.x > b->max.x || a->max.y < b->min.y || a->min.y > b->max.y ||
+             a->max.z < b->min.z || a->min.z > b->max.z);
+}
+static int cmp_u64(const void *x, const void *y) {
+    uint64_t a = *(const uint64_t *)x, b = *(const uint64_t *)y;
+    return a < b ? -1 : a > b;
+}
+/* broadphase.rs:29-99: grid cells of 2.5 m; a shared cell is implied by an inclusive overlap
+ * (floor(x/2.5) is monotone), so the sorted, deduplicated result is every overlapping (i<j) pair
+ * of colliders with finite min.x, plus oversized (> 12.5 m) colliders against everything. */
+static uint32_t broadphase(const Aabb *sw, uint32_t n, uint32_t **out) {
+    const float cell = 2.5f;
+    size_t cap = 1024, cnt = 0;
+    uint64_t *buf = malloc(cap * 8);
+    uint8_t *valid = malloc(n + 1), *over = malloc(n + 1);
+    /* sweep along x for the regular colliders */
+    uint32_t *ord = malloc(sizeof(uint32_t) * (n + 1));
+    uint32_t nv = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        valid[i] = isfinite(sw[i].min.x) ? 1 : 0;
+        float dx = sw[i].max.x - sw[i].min.x, dy = sw[i].max.y - sw[i].min.y, dz = sw[i].max.z - sw[i].min.z;
+        over[i] = valid[i] && (dx > cell * 5.0f || dy > cell * 5.0f || dz > cell * 5.0f);
+        if (valid[i] && !over[i]) ord[nv++] = i;
+    }
+    /* insertion-free O(N^2)-bounded sweep: sort by min.x then scan */
+    uint64_t *keys = malloc(8 * (nv + 1));
+    for (uint32_t k = 0; k < nv; k++) {
+        float x = sw[ord[k]].min.x; uint32_t b; memcpy(&b, &x, 4);
+        b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+        keys[k] = ((uint64_t)b << 32) | ord[k];
+    }
+    qsort(keys, nv, 8, cmp_u64);
+    for (uint32_t k = 0; k < nv; k++) {
+        uint32_t i = (uint32_t)keys[k];
+        for (uint32_t m = k + 1; m < nv; m++) {
+            uint32_t j = (uint32_t)keys[m];
+            if (sw[j].min.x > sw[i].max.x) break;
+            if (aabb_overlaps(&sw[i], &sw[j])) {
+                if (cnt == cap) { cap *= 2; buf = xrealloc(buf, cap * 8); }
+                uint32_t a = i < j ? i : j, b = i < j ? j : i;
+                buf[cnt++] = ((uint64_t)a << 32) | b;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (!over[i]) continue;
+        for (uint32_t j = 0; j < n; j++) {
+            if (j == i || !aabb_overlaps(&sw[i], &sw[j])) continue;
+            if (cnt == cap) { cap *= 2; buf = xrealloc(buf, cap * 8); }
+            uint32_t a = i < j ? i : j, b = i < j ? j : i;
+            buf[cnt++] = ((uint64_t)a << 32) | b;
+        }
+    }
+    qsort(buf, cnt, 8, cmp_u64);
+    size_t u = 0;
+    for (size_t k = 0; k < cnt; k++) if (u == 0 || buf[k] != buf[u - 1]) buf[u++] = buf[k];
+    uint32_t *pairs = malloc(sizeof(uint32_t) * 2 * (u + 1));
+    for (size_t k = 0; k < u; k++) { pairs[2 * k] = (uint32_t)(buf[k] >> 32); pairs[2 * k + 1] = (uint32_t)buf[k]; }
+    free(buf); free(valid); free(over); free(ord); free(keys);
+    *out = pairs;
+    return (uint32_t)u;
+}
+
+/* ------------------------------------------------------------------ contacts buffer */
+typedef struct { Contact *v; size_t n, cap; } CVec;
+static void cpush(CVec *cv, Contact c) {
+    if (cv->n == cv->cap) { cv->cap = cv->cap ? cv->cap * 2 : 256; cv->v = xrealloc(cv->v, cv->cap * sizeof(Contact)); }
+    cv->v[cv->n++] = c;
+}
+
+/* terrain.rs:300-359 query_aabb + step.rs:121-133 per collider */
+static uint32_t terrain_contacts(OrWorld *w, uint32_t ci, float sweep_ci, CVec *out) {
+    if (!w->has_hf) return 0;
+    int nx = (int)w->hnx, nz = (int)w->hnz;
+    if (nx < 2 || nz < 2) return 0;
+    const Aabb *bb = &w->caabb[ci];
+    float lminx = bb->min.x - w->hox, lmaxx = bb->max.x - w->hox;
+    float lminz = bb->min.z - w->hoz, lmaxz = bb->max.z - w->hoz;
+    int gx0 = (int)floorf(lminx / w->hsx) - 1, gx1 = (int)ceilf(lmaxx / w->hsx) + 1;
+    int gz0 = (int)floorf(lminz / w->hsz) - 1, gz1 = (int)ceilf(lmaxz / w->hsz) + 1;
+#define ICLAMP(v, lo, hi) ((v) < (lo) ? (lo) : ((v) > (hi) ? (hi) : (v)))
+    gx0 = ICLAMP(gx0, 0, nx - 2); gx1 = ICLAMP(gx1, 0, nx - 2);
+    gz0 = ICLAMP(gz0, 0, nz - 2); gz1 = ICLAMP(gz1, 0, nz - 2);
+    if (bb->min.y > w->hmax + 1.0f || bb->max.y < w->hmin - 1.0f) return 0;
+    uint32_t tcount = 0;
+    uint32_t b = w->cbody[ci];
+    for (int gz = gz0; gz <= gz1; gz++) {
+        for (int gx = gx0; gx <= gx1; gx++) {
+            float h00 = w->hts[gx + gz * nx], h10 = w->hts[gx + 1 + gz * nx];
+            float h01 = w->hts[gx + (gz + 1) * nx], h11 = w->hts[gx + 1 + (gz + 1) * nx];
+            float x0 = (float)gx * w->hsx + w->hox, x1 = x0 + w->hsx;
+            float z0 = (float)gz * w->hsz + w->hoz, z1 = z0 + w->hsz;
+            V3 p00 = v3(x0, h00, z0), p10 = v3(x1, h10, z0), p01 = v3(x0, h01, z1), p11 = v3(x1, h11, z1);
+            Tri tris[2] = {{p00, p01, p10}, {p10, p01, p11}};
+            for (int t = 0; t < 2; t++) {
+                tcount++;
+                Manifold m;
+                if (!collide_shape_tri(w->cshape[ci], w->pos[b], w->rot[b], tris[t], sweep_ci, &m)) continue;
+                for (int k = 0; k < m.len; k++) {
+                    Contact c;
+                    c.a = ci; c.b = TERRAIN_CI; c.point = m.p[k].pos; c.normal = m.p[k].n;
+                    c.depth = m.p[k].pen - sweep_ci + MARGIN;
+                    c.impulse = 0; c.env_mat = w->hmat; c.feat = m.p[k].feat;
+                    c.cell = ((uint32_t)gz * (uint32_t)(nx - 1) + (uint32_t)gx) * 2u + (uint32_t)t;
+                    cpush(out, c);
+                }
+            }
+        }
+    }
+    return tcount;
+}
+
+/* step.rs:499-542 */
+typedef struct { uint64_t key; uint32_t idx; } KI;
+static int cmp_ki(const void *x, const void *y) {
+    const KI *a = x, *b = y;
+    if (a->key != b->key) return a->key < b->key ? -1 : 1;
+    return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+typedef struct { int64_t k; uint32_t idx; } SK;
+static int cmp_sk_desc(const void *x, const void *y) { /* Reverse(sort_key6), stable by idx */
+    const SK *a = x, *b = y;
+    if (a->k != b->k) return a->k > b->k ? -1 : 1;
+    return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+static int cmp_sk_asc(const void *x, const void *y) {
+    const SK *a = x, *b = y;
+    if (a->k != b->k) return a->k < b->k ? -1 : 1;
+    return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+static void cull_and_orient(OrWorld *w, CVec *raw, CVec *out) {
+    size_t n = raw->n;
+    KI *ki = malloc(sizeof(KI) * (n + 1));
+    for (size_t i = 0; i < n; i++) {
+        uint32_t a = raw->v[i].a, b = raw->v[i].b;
+        uint32_t x = a <= b ? a : b, y = a <= b ? b : a;
+        ki[i].key = ((uint64_t)x << 32) | y; ki[i].idx = (uint32_t)i;
+    }
+    qsort(ki, n, sizeof(KI), cmp_ki);
+    size_t s = 0;
+    while (s < n) {
+        size_t e = s;
+        while (e < n && ki[e].key == ki[s].key) e++;
+        size_t len = e - s;
+        if (len <= 4) {
+            for (size_t k = s; k < e; k++) cpush(out, raw->v[ki[k].idx]);
+        } else if (w->cshape[raw->v[ki[s].idx].a].type == OR_BOX) {
+            Cand *cands = malloc(sizeof(Cand) * len);
+            size_t deep = 0;
+            for (size_t k = 0; k < len; k++) { cands[k].depth = raw->v[ki[s + k].idx].depth; cands[k].p = raw->v[ki[s + k].idx].point; }
+            for (size_t k = 1; k < len; k++)
+                if (sort_key6(raw->v[ki[s + k].idx].depth) > sort_key6(raw->v[ki[s + deep].idx].depth)) deep = k;
+            int idx[4];
+            int m = reduce_to_4(cands, (int)len, raw->v[ki[s + deep].idx].normal, idx);
+            for (int k = 0; k < m; k++) cpush(out, raw->v[ki[s + idx[k]].idx]);
+            free(cands);
+        } else {
+            SK *sk = malloc(sizeof(SK) * len);
+            for (size_t k = 0; k < len; k++) { sk[k].k = sort_key6(raw->v[ki[s + k].idx].depth); sk[k].idx = (uint32_t)k; }
+            qsort(sk, len, sizeof(SK), cmp_sk_desc);
+            for (size_t k = 0; k < 4; k++) cpush(out, raw->v[ki[s + sk[k].idx].idx]);
+            free(sk);
+        }
+        s = e;
+    }
+    free(ki);
+    for (size_t i = 0; i < out->n; i++) {
+        Contact *c = &out->v[i];
+        if (c->b == TERRAIN_CI) continue;
+        V3 pa = w->pos[w->cbody[c->a]], pb = w->pos[w->cbody[c->b]];
+        if (vdot(c->normal, vsub(pb, pa)) < 0.0f) c->normal = vneg(c->normal);
+    }
+}
+
+/* ------------------------------------------------------------------ solver */
+typedef struct {
+    uint32_t ai, bi; int terrain;
+    float inv_a, inv_b; M3 ii_a, ii_b; V3 ra, rb;
+    float m_n, m_t1, m_t2; V3 n, t1, t2;
+    float mu_s, mu_k, vs, depth_bias, rest_bias, approach;
+} CC;
+
+/* world/src/world/geom.rs:30-38 */
+static void orthonormal_basis(V3 n, V3 *t1, V3 *t2) {
+    float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    V3 base = (ax <= ay && ax <= az) ? v3(1, 0, 0) : (ay <= az ? v3(0, 1, 0) : v3(0, 0, 1));
+    *t1 = vnormalize_or_zero(vcross(base, n));
+    *t2 = vcross(n, *t1);
+}
+/* solver.rs:443-451 */
+static V3 rel_vel(const OrWorld *w, const CC *c) {
+    V3 va = vadd(w->lin[c->ai], vcross(w->ang[c->ai], c->ra));
+    V3 vb = c->terrain ? v3s(0) : vadd(w->lin[c->bi], vcross(w->ang[c->bi], c->rb));
+    return vsub(vb, va);
+}
+/* solver.rs:429-431 */
+static float mu_dyn(const CC *c, float slip) {
+    float x = slip / c->vs;
+    return c->mu_k + (c->mu_s - c->mu_k) * or_expf_d(-(x * x));
+}
+/* solver.rs:455-464 */
+static void apply_cimp(OrWorld *w, const CC *c, V3 j) {
+    if (c->inv_a > 0.0f) {
+        V3 nj = vneg(j);
+        apply_impulse(w, c->ai, nj);
+        apply_ang_impulse(w, c->ai, vcross(c->ra, nj));
+    }
+    if (c->inv_b > 0.0f) {
+        apply_impulse(w, c->bi, j);
+        apply_ang_impulse(w, c->bi, vcross(c->rb, j));
+    }
+}
+static float eff_mass(const CC *c, V3 d) {
+    V3 ca = vcross(c->ra, d), cb = vcross(c->rb, d);
+    float k = ((c->inv_a + c->inv_b) + vdot(ca, m3v(c->ii_a, ca))) + vdot(cb, m3v(c->ii_b, cb));
+    return k > 1.0e-9f ? 1.0f / k : 0.0f;
+}
+static int key_cmp(const uint32_t *a, const uint32_t *b) {
+    for (int i = 0; i < 4; i++) if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+static int is_idle(const OrWorld *w, uint32_t b) {
+    if (b == 0xFFFFFFFFu) return 1;
+    if (b >= w->cap || !w->alive[b]) return 0;
+    return !w->dyn[b] || !w->awake[b];
+}
+
+/* solver.rs:477-1008, 1247-1271 */
+static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, const V3 *pre_ang, float dt) {
+    const float beta = 0.05f, slop = 0.005f;
+    uint32_t nb = w->n;
+    CC *cs = malloc(sizeof(CC) * (nct + 1));
+    for (uint32_t i = 0; i < nct; i++) {
+        const Contact *c = &cts[i];
+        CC *k = &cs[i];
+        k->ai = w->cbody[c->a];
+        k->terrain = c->b == TERRAIN_CI;
+        k->bi = k->terrain ? 0 : w->cbody[c->b];
+        int a_aw = w->awake[k->ai];
+        int b_aw = k->terrain ? 0 : w->awake[k->bi];
+        k->inv_a = a_aw ? w->inv_mass[k->ai] : 0.0f;
+        k->inv_b = b_aw ? w->inv_mass[k->bi] : 0.0f;
+        k->ii_a = k->inv_a > 0.0f ? inv_inertia_world(w, k->ai) : m3zero();
+        k->ii_b = k->inv_b > 0.0f ? inv_inertia_world(w, k->bi) : m3zero();
+        PairP pp = contact_props(w, c);
+        k->n = c->normal;
+        orthonormal_basis(k->n, &k->t1, &k->t2);
+        k->ra = vsub(c->point, w->pos[k->ai]);
+        k->rb = k->terrain ? v3s(0) : vsub(c->point, w->pos[k->bi]);
+        k->m_n = eff_mass(k, k->n); k->m_t1 = eff_mass(k, k->t1); k->m_t2 = eff_mass(k, k->t2);
+        k->mu_s = pp.mu_s; k->mu_k = pp.mu_k; k->vs = fmaxf(pp.vs, 1.0e-4f);
+        float pen = c->depth - MARGIN;
+        k->depth_bias = pen > 0.0f ? fminf(fmaxf(pen - slop, 0.0f) * (beta / dt), 2.0f) : pen / dt;
+        V3 pva = vadd(pre_lin[k->ai], vcross(pre_ang[k->ai], k->ra));
+        V3 pvb = k->terrain ? v3s(0) : vadd(pre_lin[k->bi], vcross(pre_ang[k->bi], k->rb));
+        float v_pre = vdot(vsub(pvb, pva), k->n);
+        float vrn = 0.5f * (v_pre + vdot(rel_vel(w, k), k->n));
+        k->approach = vrn;
+        k->rest_bias = 0.0f;
+        if (vrn < -1.0f) k->rest_bias = -pp.rest * vrn; /* W6 (solver.rs:1009-1245) not built */
+    }
+    /* level schedule, for the stats (solver.rs:621-642) */
+    uint32_t *blev = calloc(nb + 1, 4);
+    uint32_t levels = 0;
+    for (uint32_t i = 0; i < nct; i++) {
+        int wa = cs[i].inv_a > 0.0f, wb = !cs[i].terrain && cs[i].inv_b > 0.0f;
+        if (!wa && !wb) continue;
+        uint32_t la = wa ? blev[cs[i].ai] : 0, lb = wb ? blev[cs[i].bi] : 0;
+        uint32_t t = la > lb ? la : lb;
+        if (t + 1 > levels) levels = t + 1;
+        if (wa) blev[cs[i].ai] = t + 1;
+        if (wb) blev[cs[i].bi] = t + 1;
+    }
+    free(blev);
+    w->stats.levels = levels;
+
+    float *jn = calloc(nct + 1, 4), *jt = calloc(2 * nct + 2, 4);
+    /* warm start: keys and anchors (solver.rs:717-762) */
+    uint32_t (*keys)[4] = malloc(sizeof(uint32_t) * 4 * (nct + 1));
+    V3 *anch = malloc(sizeof(V3) * (nct + 1));
+    for (uint32_t i = 0; i < nct; i++) {
+        keys[i][0] = cs[i].ai; keys[i][1] = cs[i].terrain ? 0xFFFFFFFFu : cs[i].bi; keys[i][2] = 0; keys[i][3] = 0;
+        anch[i] = qrot(qconj(w->rot[cs[i].ai]), vsub(cts[i].point, w->pos[cs[i].ai]));
+    }
+    uint8_t *taken = calloc(w->nwarm + 1, 1);
+    uint32_t matched = 0;
+    const float tol2 = 0.01f * 0.01f;
+    for (uint32_t i = 0; i < nct; i++) {
+        /* entries of this key: binary search in the sorted cache */
+        uint32_t lo = 0, hi = w->nwarm;
+        while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (key_cmp(w->warm[mid].key, keys[i]) < 0) lo = mid + 1; else hi = mid; }
+        if (lo >= w->nwarm || key_cmp(w->warm[lo].key, keys[i]) != 0) continue;
+        int best = -1; float bd = 0;
+        for (uint32_t e = lo; e < w->nwarm && key_cmp(w->warm[e].key, keys[i]) == 0; e++) {
+            if (taken[e]) continue;
+            float d = vlen2(vsub(w->warm[e].la, anch[i]));
+            if (d <= tol2 && (best < 0 || d < bd)) { best = (int)e; bd = d; }
+        }
+        if (best < 0) continue;
+        taken[best] = 1;
+        matched++;
+        const Warm *e = &w->warm[best];
+        const CC *k = &cs[i];
+        jn[i] = e->jn;
+        V3 imp = vmul(k->n, e->jn);
+        V3 v = rel_vel(w, k);
+        float vt1 = vdot(v, k->t1), vt2 = vdot(v, k->t2);
+        float mu = mu_dyn(k, sqrtf(vt1 * vt1 + vt2 * vt2));
+        V3 t = vclamp_len_max(v3(vdot(e->jt, k->t1), vdot(e->jt, k->t2), 0.0f), mu * e->jn);
+        jt[2 * i] = t.x; jt[2 * i + 1] = t.y;
+        imp = vadd(imp, vadd(vmul(k->t1, t.x), vmul(k->t2, t.y)));
+        apply_cimp(w, k, imp);
+    }
+    free(taken);
+    w->stats.warm_matched = matched;
+
+    /* PGS iterations (solver.rs:788-880), serial in contact order == the level schedule */
+    uint32_t iters = w->resid_on ? w->resid_max : w->iv;
+    uint32_t used = 0;
+    float resid = 0;
+    for (uint32_t it = 0; it < iters; it++) {
+        resid = 0;
+        for (uint32_t i = 0; i < nct; i++) {
+            const CC *k = &cs[i];
+            int wa = k->inv_a > 0.0f, wb = !k->terrain && k->inv_b > 0.0f;
+            if (!wa && !wb) continue;
+            float vrn = vdot(rel_vel(w, k), k->n);
+            float jraw = (k->depth_bias - vrn) * k->m_n;
+            float old = jn[i];
+            float nw = fmaxf(old + jraw, 0.0f);
+            jn[i] = nw;
+            apply_cimp(w, k, vmul(k->n, nw - old));
+            V3 vf = rel_vel(w, k);
+            float vt1 = vdot(vf, k->t1), vt2 = vdot(vf, k->t2);
+            float slip = sqrtf(vt1 * vt1 + vt2 * vt2);
+            float mu = mu_dyn(k, slip);
+            V3 ot = v3(jt[2 * i], jt[2 * i + 1], 0.0f);
+            V3 nt = vclamp_len_max(vsub(ot, v3(vt1 * k->m_t1, vt2 * k->m_t2, 0.0f)), mu * nw);
+            jt[2 * i] = nt.x; jt[2 * i + 1] = nt.y;
+            V3 d = vsub(nt, ot);
+            apply_cimp(w, k, vadd(vmul(k->t1, d.x), vmul(k->t2, d.y)));
+            if (w->resid_on) {
+                float r0 = k->m_n > 0.0f ? fabsf(nw - old) / k->m_n / (1.0f + fabsf(k->depth_bias)) : 0.0f;
+                float r1 = k->m_t1 > 0.0f ? fabsf(d.x) / k->m_t1 : 0.0f;
+                float r2 = k->m_t2 > 0.0f ? fabsf(d.y) / k->m_t2 : 0.0f;
+                float r = fmaxf(r0, fmaxf(r1, r2));
+                if (r > resid) resid = r;
+            }
+        }
+        used++;
+        if (w->resid_on && resid <= w->resid_tol) break;
+    }
+    w->stats.iters_used = used;
+    w->stats.residual = resid;
+
+    /* warm-cache store (solver.rs:917-944) */
+    Warm *nw = malloc(sizeof(Warm) * (nct + w->nwarm + 1));
+    uint32_t nn = 0;
+    for (uint32_t i = 0; i < nct; i++) {
+        V3 j = vadd(vmul(cs[i].t1, jt[2 * i]), vmul(cs[i].t2, jt[2 * i + 1]));
+        if (jn[i] == 0.0f && viszero(j)) continue;
+        Warm e; memcpy(e.key, keys[i], 16); e.la = anch[i]; e.jn = jn[i]; e.jt = j;
+        nw[nn++] = e;
+    }
+    /* contacts arrive in key order (bucket order); merge the idle pairs' old entries */
+    Warm *merged = malloc(sizeof(Warm) * (nn + w->nwarm + 1));
+    uint32_t mcount = 0, p = 0, q = 0;
+    while (p < nn || q < w->nwarm) {
+        if (q < w->nwarm) {
+            const uint32_t *ok = w->warm[q].key;
+            int keep = is_idle(w, ok[0]) && is_idle(w, ok[1]);
+            if (keep) {
+                /* skip if the new map has this key */
+                uint32_t lo = 0, hi = nn;
+                while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (key_cmp(nw[mid].key, ok) < 0) lo = mid + 1; else hi = mid; }
+                if (lo < nn && key_cmp(nw[lo].key, ok) == 0) keep = 0;
+            }
+            if (!keep) { q++; continue; }
+            if (p < nn && key_cmp(nw[p].key, ok) < 0) { merged[mcount++] = nw[p++]; continue; }
+            merged[mcount++] = w->warm[q++];
+            continue;
+        }
+        merged[mcount++] = nw[p++];
+    }
+    free(nw);
+    free(w->warm);
+    w->warm = merged; w->nwarm = mcount;
+
+    /* shock propagation (solver.rs:964-993) */
+    V3 up = vlen2(w->gravity) > 0.0f ? vneg(vnormalize(w->gravity)) : v3(0, 1, 0);
+    SK *ord = malloc(sizeof(SK) * (nct + 1));
+    uint32_t no = 0;
+    for (uint32_t i = 0; i < nct; i++) {
+        if (!(cs[i].inv_a > 0.0f || cs[i].inv_b > 0.0f)) continue;
+        float ha = vdot(w->pos[cs[i].ai], up);
+        float h = cs[i].terrain ? -INFINITY : fminf(ha, vdot(w->pos[cs[i].bi], up));
+        ord[no].k = sort_key6(h); ord[no].idx = i; no++;
+    }
+    qsort(ord, no, sizeof(SK), cmp_sk_asc);
+    for (uint32_t o = 0; o < no; o++) {
+        uint32_t i = ord[o].idx;
+        CC k = cs[i];
+        if (k.inv_a > 0.0f && k.inv_b > 0.0f) {
+            if (vdot(w->pos[k.ai], up) < vdot(w->pos[k.bi], up)) { k.inv_a = 0.0f; k.ii_a = m3zero(); }
+            else { k.inv_b = 0.0f; k.ii_b = m3zero(); }
+            k.m_n = eff_mass(&k, k.n);
+        }
+        float v = vdot(rel_vel(w, &k), k.n);
+        float old = jn[i];
+        float nwv = fmaxf(old + (k.depth_bias - v) * k.m_n, 0.0f);
+        jn[i] = nwv;
+        apply_cimp(w, &k, vmul(k.n, nwv - old));
+    }
+    free(ord);
+    /* restitution relax (solver.rs:1001-1008): toi contacts go to W6 (not built); at e=0 nothing runs */
+    /* loaded flags + impulses (solver.rs:1247-1270) */
+    memset(w->loaded, 0, w->cap + 1);
+    w->loaded_valid = 1;
+    for (uint32_t i = 0; i < nct; i++) {
+        if (jn[i] > 0.0f || jt[2 * i] != 0.0f || jt[2 * i + 1] != 0.0f) {
+            w->loaded[cs[i].ai] = 1;
+            if (!cs[i].terrain) w->loaded[cs[i].bi] = 1;
+        }
+        cts[i].impulse = jn[i];
+    }
+    free(jn); free(jt); free(keys); free(anch); free(cs);
+}
+
+/* islands.rs:27-101: only emptiness matters for the contact-only step (joints: none) */
+static int islands_nonempty(const OrWorld *w, const Contact *cts, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t a = w->cbody[cts[i].a];
+        int da = w->inv_mass[a] > 0.0f;
+        int db = cts[i].b != TERRAIN_CI && w->inv_mass[w->cbody[cts[i].b]] > 0.0f;
+        if (da || db) return 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ sin/cos (glam from_axis_angle) */
+void or_sincosf(float x, float *s, float *c) { or_sincos_d(x, s, c); }
+/* Quat::from_axis_angle: (axis * sin(a/2), cos(a/2)) */
+static Q4 from_axis_angle(V3 axis, float angle) {
+    float s, c;
+    or_sincosf(angle * 0.5f, &s, &c);
+    V3 v = vmul(axis, s);
+    return q4(v.x, v.y, v.z, c);
+}
+/* step.rs:230-233 turn */
+static Q4 turn(Q4 q, V3 wv, float h) {
+    if (vlen2(wv) <= 1.0e-18f) return q;
+    return qnormalize(qmul(from_axis_angle(vnormalize(wv), vlen(wv) * h), q));
+}
+
+typedef struct { uint32_t a, b; int has_b; V3 la, lb, pbw, n; float pen0; } PosC;
+
+/* step.rs:447-497 */
+static void project(OrWorld *w, const PosC *pc, uint32_t npc, V3 *slin, V3 *sang, int shock) {
+    V3 up = vlen2(w->gravity) > 0.0f ? vneg(vnormalize(w->gravity)) : v3(0, 1, 0);
+    SK *ord = malloc(sizeof(SK) * (npc + 1));
+    for (uint32_t i = 0; i < npc; i++) {
+        ord[i].idx = i;
+        if (shock) {
+            float h = pc[i].has_b ? fminf(vdot(w->pos[pc[i].a], up), vdot(w->pos[pc[i].b], up)) : -INFINITY;
+            ord[i].k = sort_key6(h);
+        } else ord[i].k = 0;
+    }
+    if (shock) qsort(ord, npc, sizeof(SK), cmp_sk_asc);
+    for (uint32_t o = 0; o < npc; o++) {
+        const PosC *c = &pc[ord[o].idx];
+        int am = w->inv_mass[c->a] > 0.0f && w->awake[c->a];
+        int bm = c->has_b && w->inv_mass[c->b] > 0.0f && w->awake[c->b];
+        if (shock && am && bm) {
+            float ha = vdot(w->pos[c->a], up), hb = vdot(w->pos[c->b], up);
+            if (ha < hb) am = 0; else bm = 0;
+        }
+        V3 pa = w->pos[c->a]; Q4 qa = w->rot[c->a];
+        V3 wa = vadd(pa, qrot(qa, c->la));
+        V3 wb, pb = v3s(0); Q4 qb = qident();
+        if (c->has_b) { pb = w->pos[c->b]; qb = w->rot[c->b]; wb = vadd(pb, qrot(qb, c->lb)); }
+        else wb = c->pbw;
+        float pen = c->pen0 - vdot(vsub(wb, wa), c->n);
+        float err = pen - POS_SLOP;
+        if (err <= 0.0f) continue;
+        V3 ra = vsub(wa, pa);
+        V3 rb = c->has_b ? vsub(wb, pb) : v3s(0);
+        float ima = 0, imb = 0; M3 iia = m3zero(), iib = m3zero();
+        if (am) { ima = w->inv_mass[c->a]; iia = inv_inertia_world(w, c->a); }
+        if (c->has_b && bm) { imb = w->inv_mass[c->b]; iib = inv_inertia_world(w, c->b); }
+        V3 ca = vcross(ra, c->n), cb = vcross(rb, c->n);
+        float wsum = ((ima + imb) + vdot(ca, m3v(iia, ca))) + vdot(cb, m3v(iib, cb));
+        if (wsum <= 1.0e-9f) continue;
+        float lambda = err / wsum;
+        for (int side = 0; side < 2; side++) {
+            uint32_t id; V3 p; Q4 q; float im; M3 ii; V3 r; V3 j;
+            if (side == 0) { if (!am) continue; id = c->a; p = pa; q = qa; im = ima; ii = iia; r = ra; j = vmul(vneg(c->n), lambda); }
+            else { if (!(c->has_b && bm)) continue; id = c->b; p = pb; q = qb; im = imb; ii = iib; r = rb; j = vmul(c->n, lambda); }
+            V3 dth = m3v(ii, vcross(r, j));
+            slin[id] = vadd(slin[id], vmul(j, im));
+            sang[id] = vadd(sang[id], dth);
+            Q4 dq = qscale(qmul(q4(dth.x, dth.y, dth.z, 0.0f), q), 0.5f);
+            Q4 nr = qnormalize(q4(q.x + dq.x, q.y + dq.y, q.z + dq.z, q.w + dq.w));
+            b_set_pose(w, id, vadd(p, vmul(j, im)), nr);
+        }
+    }
+    free(ord);
+}
+
+/* ------------------------------------------------------------------ the step */
+void or_step(OrWorld *w, float dt) {
+    uint32_t count = w->n;
+    w->tick += 1;
+    memset(&w->stats, 0, sizeof(w->stats));
+    w->loaded_valid = 0;
+    /* S1 (step.rs:53-68) */
+    V3 *pre_lin = malloc(sizeof(V3) * (count + 1)), *pre_ang = malloc(sizeof(V3) * (count + 1));
+    for (uint32_t i = 0; i < count; i++) { pre_lin[i] = w->lin[i]; pre_ang[i] = w->ang[i]; }
+    V3 g = v3(gq6(w->gravity.x), gq6(w->gravity.y), gq6(w->gravity.z));
+    for (uint32_t i = 0; i < count; i++) {
+        if (!w->dyn[i] || !w->awake[i]) continue;
+        V3 a = vfinite(w->pos[i]) ? g : v3s(0);
+        b_set_vel(w, i, vadd(w->lin[i], vmul(a, dt)), w->ang[i]);
+    }
+    /* S2 AABBs, sweep, swept (step.rs:71-82) */
+    uint32_t nc = w->nc;
+    for (uint32_t c = 0; c < nc; c++) { uint32_t b = w->cbody[c]; w->caabb[c] = aabb_of(w->cshape[c], w->pos[b], w->rot[b]); }
+    float *sweep = malloc(sizeof(float) * (nc + 1));
+    Aabb *sw = malloc(sizeof(Aabb) * (nc + 1));
+    for (uint32_t c = 0; c < nc; c++) {
+        uint32_t b = w->cbody[c];
+        float reach = 0.5f * vlen(vsub(w->caabb[c].max, w->caabb[c].min));
+        sweep[c] = MARGIN + (vlen(w->lin[b]) + vlen(w->ang[b]) * reach) * dt;
+        sw[c].min = vsub(w->caabb[c].min, v3s(sweep[c]));
+        sw[c].max = vadd(w->caabb[c].max, v3s(sweep[c]));
+    }
+    free(w->pairs);
+    w->npairs = broadphase(sw, nc, &w->pairs);
+    w->stats.pairs = w->npairs;
+    /* narrowphase, pairs (step.rs:95-119); no joints in this restatement */
+    CVec raw = {0};
+    for (uint32_t p = 0; p < w->npairs; p++) {
+        uint32_t i = w->pairs[2 * p], j = w->pairs[2 * p + 1];
+        uint32_t ba = w->cbody[i], bb = w->cbody[j];
+        if (ba == bb) continue;
+        float reach = sweep[i] + sweep[j] - MARGIN;
+        Manifold m;
+        if (!collide_shapes(w->cshape[i], w->pos[ba], w->rot[ba], w->cshape[j], w->pos[bb], w->rot[bb], reach, &m)) continue;
+        for (int k = 0; k < m.len; k++) {
+            Contact c; c.a = i; c.b = j; c.point = m.p[k].pos; c.normal = m.p[k].n;
+            c.depth = m.p[k].pen - reach + MARGIN; c.impulse = 0; c.env_mat = 0; c.feat = m.p[k].feat; c.cell = 0xFFFFFFFFu;
+            cpush(&raw, c);
+        }
+    }
+    /* terrain (step.rs:121-142) */
+    uint32_t tc = 0;
+    for (uint32_t c = 0; c < nc; c++) tc += terrain_contacts(w, c, sweep[c], &raw);
+    w->stats.terrain_candidates = tc;
+    /* S3 (step.rs:144-150) */
+    CVec cul = {0};
+    cull_and_orient(w, &raw, &cul);
+    free(raw.v);
+    if (islands_nonempty(w, cul.v, (uint32_t)cul.n)) {
+        w->stats.islands = 1;
+        solve(w, cul.v, (uint32_t)cul.n, pre_lin, pre_ang, dt);
+    }
+    /* S4 (step.rs:154-280) */
+    V3 *opos = calloc(count + 1, sizeof(V3)); Q4 *orot = calloc(count + 1, sizeof(Q4));
+    V3 *ppos = calloc(count + 1, sizeof(V3)); Q4 *prot = calloc(count + 1, sizeof(Q4));
+    for (uint32_t i = 0; i < count; i++) { orot[i] = qident(); prot[i] = qident(); }
+    uint8_t *constrained = calloc(count + 1, 1), *ball = calloc(count + 1, 1);
+    for (uint32_t k = 0; k < cul.n; k++) {
+        constrained[w->cbody[cul.v[k].a]] = 1;
+        if (cul.v[k].b != TERRAIN_CI) constrained[w->cbody[cul.v[k].b]] = 1;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        int ld = w->loaded_valid ? w->loaded[i] : 0;
+        ball[i] = !constrained[i] || !ld;
+    }
+    for (uint32_t k = 0; k < cul.n; k++) {
+        const Contact *c = &cul.v[k];
+        uint32_t a = w->cbody[c->a];
+        int hb = c->b != TERRAIN_CI;
+        uint32_t b = hb ? w->cbody[c->b] : 0;
+        if (!ball[a] && (!hb || !ball[b])) continue;
+        V3 la = ball[a] ? vmul(vadd(w->last_lin[a], w->lin[a]), 0.5f) : w->lin[a];
+        V3 wa = w->ang[a];
+        V3 ra = vsub(c->point, w->pos[a]);
+        V3 lb = v3s(0), wb = v3s(0), rb = v3s(0);
+        if (hb) { lb = ball[b] ? vmul(vadd(w->last_lin[b], w->lin[b]), 0.5f) : w->lin[b]; wb = w->ang[b]; rb = vsub(c->point, w->pos[b]); }
+        float approach = vdot(vsub(vadd(lb, vcross(wb, rb)), vadd(la, vcross(wa, ra))), c->normal);
+        V3 val = w->lin[a], vaa = w->ang[a], vbl = v3s(0), vba = v3s(0);
+        if (hb) { vbl = w->lin[b]; vba = w->ang[b]; }
+        float vend = vdot(vsub(vadd(vbl, vcross(vba, rb)), vadd(val, vcross(vaa, ra))), c->normal);
+        float s0 = -(c->depth - MARGIN);
+        if (s0 + (2.0f * vend) * dt < 0.0f || s0 + approach * dt < 0.0f) { ball[a] = 0; if (hb) ball[b] = 0; }
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        if (!w->dyn[i] || !w->awake[i]) continue;
+        V3 p = w->pos[i]; Q4 q = w->rot[i];
+        opos[i] = p; orot[i] = q;
+        V3 vl = w->lin[i], va = w->ang[i];
+        p = vadd(p, vmul(vl, dt));
+        if (ball[i]) p = vadd(p, vmul(vsub(w->last_lin[i], vl), 0.5f * dt));
+        if (vlen2(va) > 0.0f) {
+            M3 l = inv_inertia_world(w, i);
+            float det = m3det(l);
+            int has_m = isfinite(det) && fabsf(det) > 1.0e-20f;
+            V3 mom = has_m ? m3v(m3inv(l), va) : v3s(0);
+            Q4 start = q;
+            q = turn(start, va, dt);
+            if (has_m) {
+                M3 il = inv_inertia_local_of(w, i);
+                float mean = ((il.c0.x + il.c1.y) + il.c2.z) / 3.0f;
+                M3 dev = m3sub(il, m3diag(v3s(mean)));
+                int principal = vlen(vcross(mom, va)) < (1.0e-5f * vlen(mom)) * vlen(va);
+                float gain = principal ? 0.0f : (dt * fmaxf(fmaxf(vlen(dev.c0), vlen(dev.c1)), vlen(dev.c2))) * vlen(mom);
+                float gq = ceilf(gain / 0.5f);
+                uint32_t n;
+                if (!(gq >= 1.0f)) n = 1; /* NaN, 0 and negatives saturate to 0 in `as usize`, then clamp to 1 */
+                else if (gq > 256.0f) n = 256;
+                else n = (uint32_t)gq;
+                float h = dt / (float)n;
+                Q4 qq = start; V3 wv = va;
+                for (uint32_t s = 0; s < n; s++) {
+                    Q4 q0 = qq;
+                    qq = turn(q0, wv, h);
+                    M3 r = m3quat(qq);
+                    V3 wend = m3v(m3mul(m3mul(r, il), m3t(r)), mom);
+                    for (int t = 0; t < 2; t++) {
+                        qq = turn(q0, vmul(vadd(wv, wend), 0.5f), h);
+                        r = m3quat(qq);
+                        wend = m3v(m3mul(m3mul(r, il), m3t(r)), mom);
+                    }
+                    wv = wend;
+                }
+                q = qq;
+                va = wv;
+                b_set_vel(w, i, vl, va);
+            }
+        }
+        b_set_pose(w, i, p, q);
+        ppos[i] = w->pos[i]; prot[i] = w->rot[i];
+    }
+    /* S5 (step.rs:284-310, 423-443) */
+    PosC *pc = malloc(sizeof(PosC) * (cul.n + 1));
+    uint32_t npc = 0;
+    for (uint32_t k = 0; k < cul.n; k++) {
+        const Contact *c = &cul.v[k];
+        uint32_t a = w->cbody[c->a];
+        int hb = c->b != TERRAIN_CI;
+        uint32_t b = hb ? w->cbody[c->b] : 0;
+        int ma = w->inv_mass[a] > 0.0f && w->awake[a];
+        int mb = hb && w->inv_mass[b] > 0.0f && w->awake[b];
+        if (!ma && !mb) continue;
+        PosC x; x.a = a; x.b = b; x.has_b = hb; x.n = c->normal; x.pen0 = c->depth - MARGIN;
+        V3 pap = (w->dyn[a] && w->awake[a]) ? opos[a] : w->pos[a];
+        Q4 paq = (w->dyn[a] && w->awake[a]) ? orot[a] : w->rot[a];
+        x.la = qrot(qconj(paq), vsub(c->point, pap));
+        if (hb) {
+            V3 pbp = (w->dyn[b] && w->awake[b]) ? opos[b] : w->pos[b];
+            Q4 pbq = (w->dyn[b] && w->awake[b]) ? orot[b] : w->rot[b];
+            x.lb = qrot(qconj(pbq), vsub(c->point, pbp));
+        } else x.lb = v3s(0);
+        x.pbw = c->point;
+        pc[npc++] = x;
+    }
+    V3 *slin = calloc(count + 1, sizeof(V3)), *sang = calloc(count + 1, sizeof(V3));
+    if (npc > 0) {
+        for (uint32_t it = 0; it < w->ip; it++) project(w, pc, npc, slin, sang, 0);
+        project(w, pc, npc, slin, sang, 1);
+    }
+    free(pc);
+    /* S6 (step.rs:316-379): the grounding scan is restated as a per-body flag (same result) */
+    uint8_t *ground = calloc(count + 1, 1);
+    for (uint32_t k = 0; k < cul.n; k++) {
+        const Contact *c = &cul.v[k];
+        if (c->b == TERRAIN_CI && c->depth - MARGIN > -POS_SLOP) ground[w->cbody[c->a]] = 1;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        if (!w->dyn[i] || !w->awake[i]) continue;
+        V3 vl = w->lin[i], va = w->ang[i];
+        V3 np = w->pos[i]; Q4 nq = w->rot[i];
+        V3 dl = vsub(vsub(np, ppos[i]), slin[i]);
+        if (!viszero(dl)) vl = vadd(vl, vdivs(dl, dt));
+        if (ground[i] && vl.y < 0.0f) {
+            vl.y = 0.0f;
+            np.y = opos[i].y;
+            b_set_pose(w, i, np, nq);
+        }
+        if (!qeq(nq, prot[i])) {
+            Q4 dq = qmul(nq, qconj(prot[i]));
+            V3 da = vmul(v3(dq.x, dq.y, dq.z), 2.0f);
+            if (dq.w < 0.0f) da = vneg(da);
+            va = vadd(va, vdivs(vsub(da, sang[i]), dt));
+        }
+        b_set_vel(w, i, vl, va);
+    }
+    free(ground);
+    for (uint32_t i = 0; i < count; i++) w->last_lin[i] = w->lin[i];
+    /* S7 (sleep.rs:59-80, step.rs:381-384) */
+    for (uint32_t i = 0; i < count; i++) {
+        if (!w->sallow[i]) continue;
+        if (!w->awake[i] || !w->dyn[i]) continue;
+        if (vlen2(w->lin[i]) < 0.01f && vlen2(w->ang[i]) < 0.01f) {
+            w->sticks[i] = (uint16_t)(w->sticks[i] + 1);
+            if (w->sticks[i] > 30) w->awake[i] = 0;
+        } else w->sticks[i] = 0;
+    }
+    for (uint32_t c = 0; c < nc; c++) { uint32_t b = w->cbody[c]; w->caabb[c] = aabb_of(w->cshape[c], w->pos[b], w->rot[b]); }
+    free(w->contacts);
+    w->contacts = cul.v; w->ncontacts = (uint32_t)cul.n;
+    w->stats.contacts = w->ncontacts;
+    free(pre_lin); free(pre_ang); free(sweep); free(sw);
+    free(opos); free(orot); free(ppos); free(prot); free(constrained); free(ball); free(slin); free(sang);
+}
+
+/* ------------------------------------------------------------------ hash (provenance.rs:6-24, world.rs:882-909) */
+static void hu32(OrBlake3 *h, uint32_t v) { uint8_t b[4] = {(uint8_t)v, (uint8_t)(v >> 8), (uint8_t)(v >> 16), (uint8_t)(v >> 24)}; or_blake3_update(h, b, 4); }
+static void hu64(OrBlake3 *h, uint64_t v) { hu32(h, (uint32_t)v); hu32(h, (uint32_t)(v >> 32)); }
+static void hf(OrBlake3 *h, float x) { hu32(h, canon_bits(x)); }
+static void hv3(OrBlake3 *h, V3 v) { hf(h, v.x); hf(h, v.y); hf(h, v.z); }
+
+void or_step_hash(const OrWorld *w, uint8_t out[32]) {
+    OrBlake3 h;
+    or_blake3_init(&h);
+    hu64(&h, w->epoch);
+    hu64(&h, w->rng);
+    uint8_t empty[32];
+    or_blake3(NULL, 0, empty); /* ScheduleRecorder::digest of no stages */
+    or_blake3_update(&h, empty, 32);
+    for (uint32_t i = 0; i < w->n; i++) {
+        hu32(&h, i);
+        hv3(&h, w->pos[i]);
+        hf(&h, w->rot[i].x); hf(&h, w->rot[i].y); hf(&h, w->rot[i].z); hf(&h, w->rot[i].w);
+        hv3(&h, w->lin[i]); hv3(&h, w->ang[i]);
+    }
+    /* warm: u64 len (keys), then per key: 4 u32, u32 count, entries */
+    uint64_t nkeys = 0;
+    for (uint32_t i = 0; i < w->nwarm; i++) if (i == 0 || key_cmp(w->warm[i].key, w->warm[i - 1].key) != 0) nkeys++;
+    hu64(&h, nkeys);
+    for (uint32_t i = 0; i < w->nwarm;) {
+        uint32_t e = i;
+        while (e < w->nwarm && key_cmp(w->warm[e].key, w->warm[i].key) == 0) e++;
+        for (int k = 0; k < 4; k++) hu32(&h, w->warm[i].key[k]);
+        hu32(&h, e - i);
+        for (uint32_t j = i; j < e; j++) { hv3(&h, w->warm[j].la); hf(&h, w->warm[j].jn); hv3(&h, w->warm[j].jt); }
+        i = e;
+    }
+    hu64(&h, 0); /* warm_patches */
+    hu64(&h, 0); /* warm_joints */
+    hu64(&h, 0); /* rotor */
+    hu64(&h, w->n);
+    for (uint32_t i = 0; i < w->n; i++) hv3(&h, w->last_lin[i]);
+    uint32_t sl = w->cap > w->n ? w->cap : w->n;
+    hu64(&h, sl);
+    or_blake3_update(&h, w->awake, sl);
+    hu64(&h, sl);
+    for (uint32_t i = 0; i < sl; i++) { uint8_t b[2] = {(uint8_t)w->sticks[i], (uint8_t)(w->sticks[i] >> 8)}; or_blake3_update(&h, b, 2); }
+    or_blake3_final(&h, out);
+}
+
+uint32_t or_num_bodies(const OrWorld *w) { return w->n; }
+uint64_t or_tick(const OrWorld *w) { return w->tick; }
+void or_get_state(const OrWorld *w, float *pos, float *rot, float *lin, float *ang) {
+    for (uint32_t i = 0; i < w->n; i++) {
+        if (pos) { pos[3 * i] = w->pos[i].x; pos[3 * i + 1] = w->pos[i].y; pos[3 * i + 2] = w->pos[i].z; }
+        if (rot) { rot[4 * i] = w->rot[i].x; rot[4 * i + 1] = w->rot[i].y; rot[4 * i + 2] = w->rot[i].z; rot[4 * i + 3] = w->rot[i].w; }
+        if (lin) { lin[3 * i] = w->lin[i].x; lin[3 * i + 1] = w->lin[i].y; lin[3 * i + 2] = w->lin[i].z; }
+        if (ang) { ang[3 * i] = w->ang[i].x; ang[3 * i + 1] = w->ang[i].y; ang[3 * i + 2] = w->ang[i].z; }
+    }
+}
+void or_get_stats(const OrWorld *w, OrStats *s) { *s = w->stats; }
+uint32_t or_get_pairs(const OrWorld *w, uint32_t *ij, uint32_t cap) {
+    uint32_t n = w->npairs < cap ? w->npairs : cap;
+    if (ij) memcpy(ij, w->pairs, sizeof(uint32_t) * 2 * n);
+    return w->npairs;
+}
+uint32_t or_get_contacts(const OrWorld *w, OrContact *out, uint32_t cap) {
+    uint32_t n = w->ncontacts < cap ? w->ncontacts : cap;
+    for (uint32_t i = 0; out && i < n; i++) {
+        const Contact *c = &w->contacts[i];
+        out[i].a_col = c->a; out[i].b_col = c->b;
+        out[i].point[0] = c->point.x; out[i].point[1] = c->point.y; out[i].point[2] = c->point.z;
+        out[i].normal[0] = c->normal.x; out[i].normal[1] = c->normal.y; out[i].normal[2] = c->normal.z;
+        out[i].depth = c->depth; out[i].impulse = c->impulse; out[i].feat = c->feat; out[i].cell = c->cell;
+    }
+    return w->ncontacts;
+}
+uint32_t or_get_awake(const OrWorld *w, uint8_t *out, uint32_t cap) {
+    uint32_t n = w->n < cap ? w->n : cap;
+    if (out) memcpy(out, w->awake, n);
+    return w->n;
+}
+
+/* ------------------------------------------------------------------ snapshots (ffi/src/lib.rs:1696-1800) */
+typedef struct {
+    uint64_t tick; uint32_t n, cap;
+    V3 *pos; Q4 *rot; V3 *lin, *ang;
+    Warm *warm; uint32_t nwarm;
+    V3 *last_lin; uint8_t *awake; uint16_t *sticks;
+} Snap;
+void *or_snapshot(const OrWorld *w) {
+    Snap *s = calloc(1, sizeof(Snap));
+    s->tick = w->tick; s->n = w->n; s->cap = w->cap;
+    s->pos = malloc(sizeof(V3) * (w->n + 1)); s->rot = malloc(sizeof(Q4) * (w->n + 1));
+    s->lin = malloc(sizeof(V3) * (w->n + 1)); s->ang = malloc(sizeof(V3) * (w->n + 1));
+    for (uint32_t i = 0; i < w->n; i++) {
+        if (w->alive[i]) { s->pos[i] = w->pos[i]; s->rot[i] = w->rot[i]; s->lin[i] = w->lin[i]; s->ang[i] = w->ang[i]; }
+        else { s->pos[i] = v3s(0); s->rot[i] = qident(); s->lin[i] = v3s(0); s->ang[i] = v3s(0); }
+    }
+    s->warm = malloc(sizeof(Warm) * (w->nwarm + 1)); memcpy(s->warm, w->warm, sizeof(Warm) * w->nwarm); s->nwarm = w->nwarm;
+    s->last_lin = malloc(sizeof(V3) * (w->n + 1)); memcpy(s->last_lin, w->last_lin, sizeof(V3) * w->n);
+    s->awake = malloc(w->cap + 1); memcpy(s->awake, w->awake, w->cap);
+    s->sticks = malloc(2 * (w->cap + 1)); memcpy(s->sticks, w->sticks, 2 * w->cap);
+    return s;
+}
+void or_restore(OrWorld *w, const void *sp) {
+    const Snap *s = sp;
+    w->tick = s->tick;
+    for (uint32_t i = 0; i < s->n && i < w->n; i++) {
+        if (!w->alive[i]) continue;
+        float pose[7] = {s->pos[i].x, s->pos[i].y, s->pos[i].z, s->rot[i].x, s->rot[i].y, s->rot[i].z, s->rot[i].w};
+        float vel[6] = {s->lin[i].x, s->lin[i].y, s->lin[i].z, s->ang[i].x, s->ang[i].y, s->ang[i].z};
+        or_set_body_pose(w, i, pose);
+        or_set_body_vel(w, i, vel);
+    }
+    free(w->warm);
+    w->warm = malloc(sizeof(Warm) * (s->nwarm + 1)); memcpy(w->warm, s->warm, sizeof(Warm) * s->nwarm); w->nwarm = s->nwarm;
+    for (uint32_t i = 0; i < s->n && i < w->n; i++) w->last_lin[i] = s->last_lin[i];
+    for (uint32_t i = 0; i < s->cap && i < w->cap; i++) { w->awake[i] = s->awake[i]; w->sticks[i] = s->sticks[i]; }
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < w->nwarm; i++) {
+        uint32_t a = w->warm[i].key[0], b = w->warm[i].key[1];
+        int la = a == 0xFFFFFFFFu || (a < w->cap && w->alive[a]);
+        int lb = b == 0xFFFFFFFFu || (b < w->cap && w->alive[b]);
+        if (la && lb) w->warm[k++] = w->warm[i];
+    }
+    w->nwarm = k;
+}
+void or_snapshot_free(void *sp) {
+    Snap *s = sp;
+    if (!s) return;
+    free(s->pos); free(s->rot); free(s->lin); free(s->ang); free(s->warm); free(s->last_lin); free(s->awake); free(s->sticks);
+    free(s);
+}
+
+/* ------------------------------------------------------------------ unit entry points */
+static void write_m(const Manifold *m, float *out) {
+    for (int i = 0; i < m->len; i++) {
+        out[7 * i] = m->p[i].pos.x; out[7 * i + 1] = m->p[i].pos.y; out[7 * i + 2] = m->p[i].pos.z;
+        out[7 * i + 3] = m->p[i].n.x; out[7 * i + 4] = m->p[i].n.y; out[7 * i + 5] = m->p[i].n.z;
+        out[7 * i + 6] = m->p[i].pen;
+    }
+}
+int or_collide(int sa, const float *da, const float *pa, int sb, const float *db, const float *pb, float cd, float *out) {
+    ShapeD A = {sa, {da[0], da[1], da[2]}}, B = {sb, {db[0], db[1], db[2]}};
+    Manifold m;
+    if (!collide_shapes(A, v3(pa[0], pa[1], pa[2]), q4(pa[3], pa[4], pa[5], pa[6]), B, v3(pb[0], pb[1], pb[2]),
+                        q4(pb[3], pb[4], pb[5], pb[6]), cd, &m)) return 0;
+    write_m(&m, out);
+    return m.len;
+}
+int or_collide_box_triangle(const float *he, const float *pose, const float *t9, float cd, float *out) {
+    Tri t = {v3(t9[0], t9[1], t9[2]), v3(t9[3], t9[4], t9[5]), v3(t9[6], t9[7], t9[8])};
+    ShapeD s = {OR_BOX, {he[0], he[1], he[2]}};
+    Manifold m;
+    if (!collide_shape_tri(s, v3(pose[0], pose[1], pose[2]), q4(pose[3], pose[4], pose[5], pose[6]), t, cd, &m)) return 0;
+    write_m(&m, out);
+    return m.len;
+}
+int or_collide_sphere_triangle(float r, const float *c, const float *t9, float cd, float *out) {
+    Tri t = {v3(t9[0], t9[1], t9[2]), v3(t9[3], t9[4], t9[5]), v3(t9[6], t9[7], t9[8])};
+    ShapeD s = {OR_SPHERE, {r, 0, 0}};
+    Manifold m;
+    if (!collide_shape_tri(s, v3(c[0], c[1], c[2]), qident(), t, cd, &m)) return 0;
+    write_m(&m, out);
+    return m.len;
+}
+
+/* Broadphase over caller-given (swept) AABBs {minx,miny,minz,maxx,maxy,maxz} per collider;
+ * returns the pair count, writes up to cap (i, j) pairs. */
+uint32_t or_broadphase(const float *bb, uint32_t n, uint32_t *out, uint32_t cap) {
+    Aabb *a = malloc(sizeof(Aabb) * (n + 1));
+    for (uint32_t i = 0; i < n; i++) { a[i].min = v3(bb[6 * i], bb[6 * i + 1], bb[6 * i + 2]); a[i].max = v3(bb[6 * i + 3], bb[6 * i + 4], bb[6 * i + 5]); }
+    uint32_t *pairs = NULL;
+    uint32_t np = broadphase(a, n, &pairs);
+    for (uint32_t k = 0; k < np && k < cap; k++) { out[2 * k] = pairs[2 * k]; out[2 * k + 1] = pairs[2 * k + 1]; }
+    free(pairs); free(a);
+    return np;
+}

@@ -1,0 +1,111 @@
+/* Oracle (TEST INFRASTRUCTURE ONLY — never shipped, never on the product path).
+ *
+ * A plain-C restatement of the RiftPhys reference step `World::step`
+ * (crates/riftphys-world/src/world/step.rs:40-420) and everything it calls on the contact path,
+ * in the reference's evaluation order, single-threaded.  The reference itself is Rust and cannot
+ * be compiled here (no Rust toolchain), so this restatement is the checker the GPU path is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so.
+ *
+ * Parity status of each function is listed in oracle/cpu_ref.c's header.
+ */
+#ifndef RP_ORACLE_H
+#define RP_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint32_t cv[8];
+    uint64_t chunk_counter;
+    uint8_t buf[64];
+    uint32_t buf_len;
+    uint32_t blocks_compressed;
+    uint32_t stack[54][8];
+    uint32_t stack_len;
+} OrBlake3;
+
+void or_blake3_init(OrBlake3 *h);
+void or_blake3_update(OrBlake3 *h, const void *data, size_t len);
+void or_blake3_final(const OrBlake3 *h, uint8_t out32[32]);
+void or_blake3(const void *data, size_t len, uint8_t out32[32]);
+
+typedef struct OrWorld OrWorld;
+
+/* Shape codes shared with the product's debug API. */
+enum { OR_SPHERE = 0, OR_BOX = 1, OR_CAPSULE = 2 };
+
+/* One contact after cull/orient, with the build-defined feature id. */
+typedef struct {
+    uint32_t a_col, b_col; /* b_col = 0xFFFFFFFF for terrain */
+    float point[3];
+    float normal[3];
+    float depth;
+    float impulse;
+    uint32_t feat;  /* kind<<24 | which<<16 | k */
+    uint32_t cell;  /* terrain: ((gz*(nx-1))+gx)*2+tri ; body pair: 0xFFFFFFFF */
+} OrContact;
+
+typedef struct {
+    uint32_t pairs, contacts, islands;
+    uint32_t terrain_candidates;
+    uint32_t levels;       /* contact batches of the velocity solve */
+    uint32_t warm_matched;
+    uint32_t iters_used;   /* velocity iterations actually run */
+    float residual;        /* last sweep's residual (m/s), residual mode only */
+} OrStats;
+
+OrWorld *or_world_create(uint32_t max_bodies);
+void or_world_destroy(OrWorld *w);
+void or_set_epoch(OrWorld *w, uint64_t epoch);
+void or_set_rng_seed(OrWorld *w, uint64_t seed);
+void or_set_gravity(OrWorld *w, float gx, float gy, float gz);
+void or_set_iterations(OrWorld *w, uint32_t vel, uint32_t pos);
+/* Residual mode (build addition): iterate until sweep residual <= tol, at most max_iters. */
+void or_set_residual_mode(OrWorld *w, int on, float tol, uint32_t max_iters);
+void or_set_pair_props(OrWorld *w, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
+                       float restitution, float stribeck_v);
+/* shape: OR_*; dims: sphere {r}, box {hx,hy,hz}, capsule {r,hh}; pose {px,py,pz,qx,qy,qz,qw};
+ * vel {lx,ly,lz,ax,ay,az}; density <= 0 means the reference default (1500 kg/m^3). Returns the
+ * 0-based body id. */
+int32_t or_add_body(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel,
+                    int dynamic, float density);
+int or_remove_body(OrWorld *w, uint32_t id);
+void or_set_heightfield(OrWorld *w, uint32_t nx, uint32_t nz, float sx, float sz, float ox, float oz,
+                        const float *heights, uint32_t material);
+void or_set_body_pose(OrWorld *w, uint32_t id, const float *pose);
+void or_set_body_vel(OrWorld *w, uint32_t id, const float *vel);
+void or_set_sleep_allowed(OrWorld *w, uint32_t id, int allowed);
+void or_step(OrWorld *w, float dt);
+void or_step_hash(const OrWorld *w, uint8_t out32[32]);
+uint32_t or_num_bodies(const OrWorld *w);
+uint64_t or_tick(const OrWorld *w);
+/* pos[3n], rot[4n], lin[3n], ang[3n] */
+void or_get_state(const OrWorld *w, float *pos, float *rot, float *lin, float *ang);
+void or_get_stats(const OrWorld *w, OrStats *s);
+/* Debug views of the last step. Each returns the count; copies at most cap entries. */
+uint32_t or_get_pairs(const OrWorld *w, uint32_t *ij, uint32_t cap);
+uint32_t or_get_contacts(const OrWorld *w, OrContact *out, uint32_t cap);
+uint32_t or_get_awake(const OrWorld *w, uint8_t *out, uint32_t cap);
+/* Snapshot / restore (rphys_world_snapshot_* semantics). */
+void *or_snapshot(const OrWorld *w);
+void or_restore(OrWorld *w, const void *snap);
+void or_snapshot_free(void *snap);
+
+/* Unit-level entry points used by the oracle's own pin tests. */
+int or_collide(int shape_a, const float *dims_a, const float *pose_a, int shape_b, const float *dims_b,
+               const float *pose_b, float contact_dist, float *out_points /* 4 x (pos3, n3, pen) */);
+int or_collide_box_triangle(const float *he, const float *pose, const float *tri9, float contact_dist,
+                            float *out_points);
+int or_collide_sphere_triangle(float r, const float *center, const float *tri9, float contact_dist,
+                               float *out_points);
+void or_sincosf(float x, float *s, float *c);
+uint32_t or_broadphase(const float *aabbs6, uint32_t n, uint32_t *out_ij, uint32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1306 @@
+// C ABI of the step library: the rphys_* entry points of crates/riftphys-ffi/src/riftphys_c.h
+// over device-resident worlds.  Host code keeps a mirror of each world (bodies, heightfield,
+// settings, carried state); the device holds the copy the step runs on.  Host reads and writes
+// pull the device copy first; a step pushes host edits first.  Kernels live in rp_step.hip and
+// rp_hash.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/riftphys_c.h"
+#include "rp_dev.h"
+
+struct HashDev {
+    uint32_t *keyflag, *w_chunk, *w_len;
+    uint64_t *w_epoch, *w_rng;
+    uint8_t *stream;
+    uint32_t *cv, *cv2;
+    uint8_t *digest;
+    uint32_t cap_chunks;
+};
+extern "C" void rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st);
+extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
+extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uint32_t max_extra_slots, hipStream_t st);
+
+// ------------------------------------------------------------------ errors (lib.rs:56-82)
+static thread_local std::string g_last_error;
+static void set_err(const std::string &s) { g_last_error = s; }
+static void clear_err() { g_last_error.clear(); }
+
+struct F4 { float x, y, z, w; };
+static inline F4 f4(float x, float y, float z, float w = 0.0f) { F4 r = {x, y, z, w}; return r; }
+
+// ------------------------------------------------------------------ host-side arithmetic used at setup
+// (the same operation order as the device; compiled with -ffp-contract=off)
+struct HV { float x, y, z; };
+static inline HV hv(float x, float y, float z) { HV r = {x, y, z}; return r; }
+static inline HV hcross(HV a, HV b) { return hv(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y); }
+static inline float hdot(HV a, HV b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+struct HM { HV c0, c1, c2; };
+static HM hinv(HM m) {  // Mat3A::inverse
+    HV t0 = hcross(m.c1, m.c2), t1 = hcross(m.c2, m.c0), t2 = hcross(m.c0, m.c1);
+    float id = 1.0f / hdot(m.c2, t2);
+    HV a = hv(t0.x * id, t0.y * id, t0.z * id), b = hv(t1.x * id, t1.y * id, t1.z * id), c = hv(t2.x * id, t2.y * id, t2.z * id);
+    HM r; r.c0 = hv(a.x, b.x, c.x); r.c1 = hv(a.y, b.y, c.y); r.c2 = hv(a.z, b.z, c.z);
+    return r;
+}
+static inline bool hfin(float x) { return std::isfinite(x); }
+// SSE2 Vec4 normalize (division by sqrt of the pairwise dot4)
+static F4 qnorm(F4 q) {
+    float l = std::sqrt((q.x * q.x + q.z * q.z) + (q.y * q.y + q.w * q.w));
+    return f4(q.x / l, q.y / l, q.z / l, q.w / l);
+}
+static F4 fin_q(F4 q) {  // dynamics.rs:16-22
+    if (!(hfin(q.x) && hfin(q.y) && hfin(q.z) && hfin(q.w))) return f4(0, 0, 0, 1);
+    F4 n = qnorm(q);
+    if (!(hfin(n.x) && hfin(n.y) && hfin(n.z) && hfin(n.w))) return f4(0, 0, 0, 1);
+    if (n.w < 0.0f) n = f4(-n.x, -n.y, -n.z, -n.w);
+    return n;
+}
+static float q6div(float x) { return std::isfinite(x) ? std::round(x / 1.0e-6f) * 1.0e-6f : 0.0f; }
+static float q6mul(float x) { return std::round(x * 1.0e6f) * 1.0e-6f; }
+
+// materials.rs:93-304: the core table entries the FFI can reach (Default 0, Grit 4 and the
+// first sixteen ids), geometric-mean friction, max restitution, quantised, and the curated
+// overrides.
+struct MatP { float mu_s, mu_k, rest, vs; };
+static const MatP MATS[16] = {
+    {0.60f, 0.50f, 0.10f, 0.08f}, {0.05f, 0.03f, 0.02f, 0.02f}, {1.10f, 0.95f, 0.80f, 0.15f},
+    {0.70f, 0.55f, 0.50f, 0.05f}, {1.30f, 1.05f, 0.00f, 0.12f}, {0.55f, 0.45f, 0.08f, 0.10f},
+    {0.80f, 0.70f, 0.00f, 0.10f}, {0.90f, 0.80f, 0.00f, 0.12f}, {0.80f, 0.60f, 0.20f, 0.10f},
+    {0.55f, 0.45f, 0.10f, 0.06f}, {1.10f, 0.95f, 0.00f, 0.18f}, {0.40f, 0.30f, 0.65f, 0.04f},
+    {0.90f, 0.60f, 0.35f, 0.06f}, {0.61f, 0.47f, 0.40f, 0.05f}, {0.50f, 0.40f, 0.12f, 0.08f},
+    {0.20f, 0.15f, 0.85f, 0.04f}};
+static MatP table_pair(uint32_t a, uint32_t b) {
+    MatP pa = MATS[a & 15], pb = MATS[b & 15], r;
+    r.mu_s = q6mul(std::sqrt(std::fabs(pa.mu_s * pb.mu_s)));
+    r.mu_k = q6mul(std::sqrt(std::fabs(pa.mu_k * pb.mu_k)));
+    r.rest = q6mul(std::fmax(pa.rest, pb.rest));
+    r.vs = q6mul(0.5f * (pa.vs + pb.vs));
+    uint32_t x = std::min(a, b), y = std::max(a, b);
+    MatP o = r; bool ov = false;
+    if (x == 1 && y == 2) { ov = true; o = {0.20f, 0.10f, 0.05f, 0.08f}; }
+    if (x == 2 && y == 7) { ov = true; o = {1.10f, 0.95f, 0.00f, 0.18f}; }
+    if (x == 2 && y == 6) { ov = true; o = {1.00f, 0.85f, 0.00f, 0.16f}; }
+    if (x == 3 && y == 3) { ov = true; o = {0.58f, 0.50f, 0.08f, 0.04f}; }
+    if (ov) { o.mu_s = q6mul(o.mu_s); o.mu_k = q6mul(o.mu_k); o.rest = q6mul(o.rest); o.vs = q6mul(o.vs); return o; }
+    return r;
+}
+
+// ------------------------------------------------------------------ host world mirror
+struct PairOverride { uint32_t a, b; MatP p; };
+struct HostWorld {
+    uint32_t max_bodies = 0, max_colliders = 0, bodies_created = 0;
+    // bodies
+    std::vector<F4> pos, rot, lin, ang, last_lin, dims;
+    std::vector<float> inv_mass, iil;
+    std::vector<uint32_t> bflags, awake, sticks, btype;
+    std::vector<uint32_t> col_body, col_mat;
+    // sleep slots beyond the bodies (capacity): awake 1, ticks 0 unless changed
+    // heightfield (terrain.rs:24-52 applied at set time)
+    bool has_hf = false;
+    uint32_t hnx = 0, hnz = 0, hmat = 4;
+    float hsx = 1, hsz = 1, hox = 0, hoz = 0, hmin = 0, hmax = 0;
+    std::vector<float> heights;
+    // settings
+    float gx = 0.0f, gy = -9.81f, gz = 0.0f;
+    uint32_t iv = 20, ip = 20;
+    std::vector<PairOverride> ovr;
+    uint64_t epoch = 0, rng = 0xC0FFEEull | 1ull, tick = 0;
+    // carried warm cache
+    std::vector<uint64_t> wkey;
+    std::vector<F4> wla, wjt;
+    std::vector<float> wjn;
+    uint32_t nb() const { return (uint32_t)pos.size(); }
+    uint32_t nc() const { return (uint32_t)col_body.size(); }
+};
+
+static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
+    size_t k = 0;
+    for (size_t i = 0; i < w.wkey.size(); i++) {
+        uint32_t a = (uint32_t)(w.wkey[i] >> 32), c = (uint32_t)w.wkey[i];
+        if (a == b || c == b) continue;
+        w.wkey[k] = w.wkey[i]; w.wla[k] = w.wla[i]; w.wjn[k] = w.wjn[i]; w.wjt[k] = w.wjt[i]; k++;
+    }
+    w.wkey.resize(k); w.wla.resize(k); w.wjn.resize(k); w.wjt.resize(k);
+}
+
+// ------------------------------------------------------------------ device scene (batch of worlds)
+struct DeviceBuffers {
+    std::vector<void *> ptrs;
+    ~DeviceBuffers() { for (void *p : ptrs) (void)hipFree(p); }
+    template <class T> bool alloc(T *&p, size_t n) {
+        void *q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { p = nullptr; return false; }
+        ptrs.push_back(q);
+        p = (T *)q;
+        return true;
+    }
+};
+
+struct Slot {  // device-side snapshot of a scene
+    bool valid = false;
+    DeviceBuffers buf;
+    float4 *pos, *rot, *lin, *ang, *last;
+    uint32_t *awake, *sticks;
+    uint64_t *wkey; float4 *wla; float *wjn; float4 *wjt; uint32_t *woff;
+    uint32_t cur; uint32_t nwarm;
+    std::vector<uint64_t> ticks;
+};
+
+struct Scene {
+    Dev d{};
+    HashDev h{};
+    DeviceBuffers buf;
+    hipStream_t st = nullptr;
+    uint32_t cur = 0;
+    uint32_t island_grid = 2048;
+    uint32_t max_extra = 0;
+    std::vector<uint32_t> body_off, col_off;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    float graph_dt = -1.0f;
+    std::vector<std::unique_ptr<Slot>> slots;
+    ~Scene() {
+        for (int i = 0; i < 2; i++) if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+static bool hip_ok(hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    set_err(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+struct Caps { uint32_t pairs, cands, contacts, warm; };
+
+static uint64_t next_pow2(uint64_t v) { uint64_t m = 1; while (m < v) m <<= 1; return m; }
+
+// Builds a device scene for the given worlds (host mirrors) with the given capacities and
+// uploads everything.  Returns nullptr (with the error set) on failure.
+static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_t cur) {
+    std::unique_ptr<Scene> S(new (std::nothrow) Scene());
+    if (!S) { set_err("out of host memory"); return nullptr; }
+    Dev &d = S->d;
+    uint32_t nw = (uint32_t)ws.size(), nb = 0, nc = 0, nh = 0, nwarm = 0;
+    S->body_off.push_back(0); S->col_off.push_back(0);
+    for (auto *w : ws) {
+        if (w->nc() > RP_MAX_WORLD_COLLIDERS) { set_err("a world holds more than 16384 colliders (the broadphase's per-world LDS sort width)"); return nullptr; }
+        nb += w->nb(); nc += w->nc(); nh += w->has_hf ? (uint32_t)w->heights.size() : 0u; nwarm += (uint32_t)w->wkey.size();
+        S->body_off.push_back(nb); S->col_off.push_back(nc);
+        uint32_t cap = std::max(w->max_bodies, w->nb());
+        S->max_extra = std::max(S->max_extra, cap - w->nb());
+    }
+    caps.warm = std::max(caps.warm, nwarm + 1024u);
+    d.nb = nb; d.nc = nc; d.nw = nw;
+    d.cap_pairs = caps.pairs; d.cap_cands = caps.cands; d.cap_contacts = caps.contacts; d.cap_warm = caps.warm;
+    DeviceBuffers &B = S->buf;
+    bool ok = true;
+    size_t NB = nb + 1, NC = nc + 1;
+    ok &= B.alloc(d.pos, NB) && B.alloc(d.rot, NB) && B.alloc(d.lin, NB) && B.alloc(d.ang, NB) && B.alloc(d.last_lin, NB);
+    ok &= B.alloc(d.inv_mass, NB) && B.alloc(d.bflags, NB) && B.alloc(d.iil, 9 * NB) && B.alloc(d.awake, NB) && B.alloc(d.sticks, NB);
+    ok &= B.alloc(d.btype, NB) && B.alloc(d.bdims, NB) && B.alloc(d.bworld, NB);
+    ok &= B.alloc(d.col_body, NC) && B.alloc(d.col_mat, NC) && B.alloc(d.col_world, NC);
+    ok &= B.alloc(d.w_body_off, nw + 1) && B.alloc(d.w_col_off, nw + 1) && B.alloc(d.w_cap, nw) && B.alloc(d.hf, nw);
+    ok &= B.alloc(d.heights, nh + 1) && B.alloc(d.pair_tab, 256);
+    ok &= B.alloc(d.bk_pos, NB) && B.alloc(d.bk_rot, NB) && B.alloc(d.bk_lin, NB) && B.alloc(d.bk_ang, NB) && B.alloc(d.bk_last, NB);
+    ok &= B.alloc(d.bk_awake, NB) && B.alloc(d.bk_sticks, NB);
+    ok &= B.alloc(d.pre_lin, NB) && B.alloc(d.pre_ang, NB) && B.alloc(d.iiw, 9 * NB) && B.alloc(d.flags, NB) && B.alloc(d.ballf, NB) && B.alloc(d.root, NB);
+    ok &= B.alloc(d.opos, NB) && B.alloc(d.orot, NB) && B.alloc(d.ppos, NB) && B.alloc(d.prot, NB) && B.alloc(d.slin, NB) && B.alloc(d.sang, NB);
+    ok &= B.alloc(d.amin, NC) && B.alloc(d.amax, NC) && B.alloc(d.smin, NC) && B.alloc(d.smax, NC) && B.alloc(d.sweep, NC);
+    ok &= B.alloc(d.ccls, NC) && B.alloc(d.bp_ord, NC) && B.alloc(d.bp_rank, NC);
+    ok &= B.alloc(d.w_nsap, nw) && B.alloc(d.w_nover, nw) && B.alloc(d.w_solve, nw) && B.alloc(d.w_maxext, nw);
+    ok &= B.alloc(d.out_off, NC) && B.alloc(d.cand_off, NC) && B.alloc(d.c_off, NC);
+    size_t P = caps.pairs, T = caps.cands, C = caps.contacts, W = caps.warm;
+    ok &= B.alloc(d.pair_i, P) && B.alloc(d.pair_j, P) && B.alloc(d.pair_n, P) && B.alloc(d.pc_p, 4 * P) && B.alloc(d.pc_n, 4 * P);
+    ok &= B.alloc(d.tc_n, T) && B.alloc(d.tc_p, 4 * T) && B.alloc(d.tc_nf, 4 * T) && B.alloc(d.tc_cell, T);
+    ok &= B.alloc(d.tcul_p, 4 * NC) && B.alloc(d.tcul_n, 4 * NC) && B.alloc(d.tcul_cell, 4 * NC) && B.alloc(d.tcul_cnt, NC);
+    ok &= B.alloc(d.ct_a, C) && B.alloc(d.ct_b, C) && B.alloc(d.ct_cell, C) && B.alloc(d.ct_p, C) && B.alloc(d.ct_n, C) && B.alloc(d.ct_imp, C);
+    ok &= B.alloc(d.cs_ai, C) && B.alloc(d.cs_bi, C) && B.alloc(d.cs_wa, C) && B.alloc(d.cs_wb, C) && B.alloc(d.cs_inv, C);
+    ok &= B.alloc(d.cs_ra, C) && B.alloc(d.cs_rb, C) && B.alloc(d.cs_n, C) && B.alloc(d.cs_t1, C) && B.alloc(d.cs_t2, C) && B.alloc(d.cs_m, C) && B.alloc(d.cs_mu, C);
+    ok &= B.alloc(d.jn, C) && B.alloc(d.jn_pre, C) && B.alloc(d.jt, C) && B.alloc(d.ct_key, C) && B.alloc(d.ct_anchor, C) && B.alloc(d.warm_idx, C);
+    ok &= B.alloc(d.isl_k, C) && B.alloc(d.isl_v, C) && B.alloc(d.isl_k2, C) && B.alloc(d.isl_v2, C) && B.alloc(d.isl_start, C) && B.alloc(d.isl_flag, C + 1);
+    ok &= B.alloc(d.g_ent, 4 * C) && B.alloc(d.g_order, C) && B.alloc(d.g_pa, C) && B.alloc(d.g_pb, C) && B.alloc(d.g_done, C);
+    d.rs_blocks = (uint32_t)((C + 1023) / 1024);
+    ok &= B.alloc(d.rs_hist, 256 * (size_t)d.rs_blocks + 1);
+    ok &= B.alloc(d.pc_la, C) && B.alloc(d.pc_lb, C);
+    for (int k = 0; k < 2; k++) ok &= B.alloc(d.wk_key[k], W) && B.alloc(d.wk_la[k], W) && B.alloc(d.wk_jn[k], W) && B.alloc(d.wk_jt[k], W) && B.alloc(d.wc_off[k], nw + 1);
+    ok &= B.alloc(d.newflag, C + 1) && B.alloc(d.keepflag, W + 1);
+    size_t scan_elems = std::max<size_t>({NB, NC, C + 1, W + 1, 256 * (size_t)d.rs_blocks + 1, (size_t)nw + 8});
+    ok &= B.alloc(d.scan_tmp, scan_elems / 1024 + 2 * nw + 64);
+    ok &= B.alloc(d.cnt, CT_COUNT);
+    // hash buffers
+    HashDev &h = S->h;
+    uint64_t bytes = 0;
+    for (auto *w : ws) {
+        uint64_t n = w->nb(), cap = std::max(w->max_bodies, w->nb());
+        bytes += 48 + 56 * n + 8 + 24 + 8 + 12 * n + 8 + cap + 8 + 2 * cap + 1024;
+    }
+    bytes += 48ull * W;
+    h.cap_chunks = (uint32_t)(bytes / 1024 + nw + 1);
+    ok &= B.alloc(h.keyflag, W + 1) && B.alloc(h.w_chunk, nw + 1) && B.alloc(h.w_len, nw) && B.alloc(h.w_epoch, nw) && B.alloc(h.w_rng, nw);
+    ok &= B.alloc(h.stream, (size_t)h.cap_chunks * 1024) && B.alloc(h.cv, 8 * (size_t)h.cap_chunks) && B.alloc(h.cv2, 8 * (size_t)h.cap_chunks) && B.alloc(h.digest, 32 * (size_t)nw);
+    if (!ok) { set_err("device allocation failed (scene too large for the configured capacities)"); return nullptr; }
+    if (!hip_ok(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking), "hipStreamCreate")) return nullptr;
+    // ---- upload
+    std::vector<F4> pos, rot, lin, ang, last, dims;
+    std::vector<float> invm, iil, heights;
+    std::vector<uint32_t> bfl, awk, stk, bty, bwd, cbody, cmat, cwld, wcap;
+    std::vector<HfDesc> hfs;
+    std::vector<uint64_t> wkey, ep, rn;
+    std::vector<F4> wla, wjt;
+    std::vector<float> wjn;
+    std::vector<uint32_t> woff;
+    woff.push_back(0);
+    for (uint32_t wi = 0; wi < nw; wi++) {
+        HostWorld *w = ws[wi];
+        uint32_t bo = S->body_off[wi];
+        pos.insert(pos.end(), w->pos.begin(), w->pos.end()); rot.insert(rot.end(), w->rot.begin(), w->rot.end());
+        lin.insert(lin.end(), w->lin.begin(), w->lin.end()); ang.insert(ang.end(), w->ang.begin(), w->ang.end());
+        last.insert(last.end(), w->last_lin.begin(), w->last_lin.end()); dims.insert(dims.end(), w->dims.begin(), w->dims.end());
+        invm.insert(invm.end(), w->inv_mass.begin(), w->inv_mass.end()); iil.insert(iil.end(), w->iil.begin(), w->iil.end());
+        bfl.insert(bfl.end(), w->bflags.begin(), w->bflags.end()); awk.insert(awk.end(), w->awake.begin(), w->awake.end());
+        stk.insert(stk.end(), w->sticks.begin(), w->sticks.end()); bty.insert(bty.end(), w->btype.begin(), w->btype.end());
+        for (uint32_t i = 0; i < w->nb(); i++) bwd.push_back(wi);
+        for (uint32_t c = 0; c < w->nc(); c++) { cbody.push_back(w->col_body[c] + bo); cmat.push_back(w->col_mat[c]); cwld.push_back(wi); }
+        wcap.push_back(std::max(w->max_bodies, w->nb()));
+        HfDesc hd{};
+        if (w->has_hf) {
+            hd.nx = w->hnx; hd.nz = w->hnz; hd.off = (uint32_t)heights.size(); hd.mat = w->hmat;
+            hd.sx = w->hsx; hd.sz = w->hsz; hd.ox = w->hox; hd.oz = w->hoz; hd.hmin = w->hmin; hd.hmax = w->hmax; hd.valid = 1;
+            heights.insert(heights.end(), w->heights.begin(), w->heights.end());
+        }
+        hfs.push_back(hd);
+        wkey.insert(wkey.end(), w->wkey.begin(), w->wkey.end()); wla.insert(wla.end(), w->wla.begin(), w->wla.end());
+        wjn.insert(wjn.end(), w->wjn.begin(), w->wjn.end()); wjt.insert(wjt.end(), w->wjt.begin(), w->wjt.end());
+        woff.push_back((uint32_t)wkey.size());
+        ep.push_back(w->epoch); rn.push_back(w->rng);
+    }
+    std::vector<F4> ptab(256);
+    {
+        HostWorld *w0 = ws[0];
+        for (uint32_t a = 0; a < 16; a++)
+            for (uint32_t b = 0; b < 16; b++) {
+                MatP p = table_pair(a, b);
+                uint32_t x = std::min(a, b), y = std::max(a, b);
+                for (auto &o : w0->ovr) if (o.a == x && o.b == y) p = o.p;
+                ptab[a * 16 + b] = f4(p.mu_s, p.mu_k, p.rest, p.vs);
+            }
+    }
+    auto up = [&](void *dst, const void *src, size_t bytes) { return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S->st), "upload"); };
+    ok = up(d.pos, pos.data(), pos.size() * 16) && up(d.rot, rot.data(), rot.size() * 16) && up(d.lin, lin.data(), lin.size() * 16) &&
+         up(d.ang, ang.data(), ang.size() * 16) && up(d.last_lin, last.data(), last.size() * 16) && up(d.bdims, dims.data(), dims.size() * 16) &&
+         up(d.inv_mass, invm.data(), invm.size() * 4) && up(d.iil, iil.data(), iil.size() * 4) && up(d.bflags, bfl.data(), bfl.size() * 4) &&
+         up(d.awake, awk.data(), awk.size() * 4) && up(d.sticks, stk.data(), stk.size() * 4) && up(d.btype, bty.data(), bty.size() * 4) &&
+         up(d.bworld, bwd.data(), bwd.size() * 4) && up(d.col_body, cbody.data(), cbody.size() * 4) && up(d.col_mat, cmat.data(), cmat.size() * 4) &&
+         up(d.col_world, cwld.data(), cwld.size() * 4) && up(d.w_body_off, S->body_off.data(), S->body_off.size() * 4) &&
+         up(d.w_col_off, S->col_off.data(), S->col_off.size() * 4) && up(d.w_cap, wcap.data(), wcap.size() * 4) &&
+         up(d.hf, hfs.data(), hfs.size() * sizeof(HfDesc)) && up(d.heights, heights.data(), heights.size() * 4) &&
+         up(d.pair_tab, ptab.data(), ptab.size() * 16) && up(d.wk_key[cur], wkey.data(), wkey.size() * 8) &&
+         up(d.wk_la[cur], wla.data(), wla.size() * 16) && up(d.wk_jn[cur], wjn.data(), wjn.size() * 4) && up(d.wk_jt[cur], wjt.data(), wjt.size() * 16) &&
+         up(d.wc_off[cur], woff.data(), woff.size() * 4) && up(h.w_epoch, ep.data(), ep.size() * 8) && up(h.w_rng, rn.data(), rn.size() * 8);
+    if (ok) ok = hip_ok(hipMemsetAsync(d.cnt, 0, CT_COUNT * 4, S->st), "memset");
+    if (ok) ok = hip_ok(hipStreamSynchronize(S->st), "upload sync");
+    if (!ok) return nullptr;
+    S->cur = cur;
+    return S.release();
+}
+
+// Downloads the mutable state of world wi of the scene into the host mirror.
+static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
+    const Dev &d = S.d;
+    uint32_t bo = S.body_off[wi], n = S.body_off[wi + 1] - bo;
+    auto dn = [&](void *dst, const void *src, size_t bytes) { return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S.st), "download"); };
+    bool ok = dn(w.pos.data(), d.pos + bo, n * 16) && dn(w.rot.data(), d.rot + bo, n * 16) && dn(w.lin.data(), d.lin + bo, n * 16) &&
+              dn(w.ang.data(), d.ang + bo, n * 16) && dn(w.last_lin.data(), d.last_lin + bo, n * 16) &&
+              dn(w.awake.data(), d.awake + bo, n * 4) && dn(w.sticks.data(), d.sticks + bo, n * 4);
+    uint32_t off[2];
+    ok = ok && dn(off, d.wc_off[S.cur] + wi, 8) && hip_ok(hipStreamSynchronize(S.st), "download sync");
+    if (!ok) return false;
+    uint32_t ne = off[1] - off[0];
+    w.wkey.resize(ne); w.wla.resize(ne); w.wjn.resize(ne); w.wjt.resize(ne);
+    ok = dn(w.wkey.data(), d.wk_key[S.cur] + off[0], ne * 8) && dn(w.wla.data(), d.wk_la[S.cur] + off[0], ne * 16) &&
+         dn(w.wjn.data(), d.wk_jn[S.cur] + off[0], ne * 4) && dn(w.wjt.data(), d.wk_jt[S.cur] + off[0], ne * 16);
+    return ok && hip_ok(hipStreamSynchronize(S.st), "download sync");
+}
+
+static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
+    TickParams tp{};
+    tp.dt = dt;
+    // gravity.rs:85-87 q6 of the uniform field
+    tp.gx = std::isfinite(w.gx) ? q6mul(w.gx) : 0.0f;
+    tp.gy = std::isfinite(w.gy) ? q6mul(w.gy) : 0.0f;
+    tp.gz = std::isfinite(w.gz) ? q6mul(w.gz) : 0.0f;
+    // step.rs:448 / solver.rs:968 up = -normalize(gravity) (Vec3::normalize = v * (1/|v|))
+    float l2 = (w.gx * w.gx + w.gy * w.gy) + w.gz * w.gz;
+    if (l2 > 0.0f) {
+        float r = 1.0f / std::sqrt(l2);
+        tp.upx = -(w.gx * r); tp.upy = -(w.gy * r); tp.upz = -(w.gz * r);
+    } else { tp.upx = 0.0f; tp.upy = 1.0f; tp.upz = 0.0f; }
+    tp.iv = iv; tp.ip = w.ip;
+    return tp;
+}
+
+// Runs n ticks on the scene.  On a capacity overflow the failed tick is rolled back, the scene is
+// rebuilt with larger buffers (through the host mirrors) and stepping resumes.
+static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n, bool use_graph);
+
+static Caps initial_caps(const std::vector<HostWorld *> &ws) {
+    uint64_t nc = 0;
+    for (auto *w : ws) nc += w->nc();
+    Caps c;
+    c.pairs = (uint32_t)std::min<uint64_t>(16 * nc + 1024, 0x7FFFFFFF);
+    c.cands = (uint32_t)std::min<uint64_t>(64 * nc + 4096, 0x7FFFFFFF);
+    c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
+    c.warm = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
+    return c;
+}
+
+static bool capture_graph(Scene &S, const TickParams &tp0, uint32_t parity) {
+    TickParams tp = tp0;
+    tp.cur = parity;
+    hipGraph_t g = nullptr;
+    if (!hip_ok(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
+    rp_launch_tick(S.d, tp, S.island_grid, S.st);
+    if (!hip_ok(hipStreamEndCapture(S.st, &g), "hipStreamEndCapture")) return false;
+    bool ok = hip_ok(hipGraphInstantiate(&S.graph[parity], g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    (void)hipGraphDestroy(g);
+    return ok;
+}
+
+static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n, bool use_graph) {
+    uint32_t remaining = n;
+    Caps caps{S->d.cap_pairs, S->d.cap_cands, S->d.cap_contacts, S->d.cap_warm};
+    while (remaining > 0) {
+        TickParams tp = tick_params(*ws[0], dt, iv);
+        if (!hip_ok(hipMemsetAsync(S->d.cnt, 0, CT_COUNT * 4, S->st), "memset")) return false;
+        if (use_graph) {
+            if (S->graph_dt != dt || !S->graph[0] || !S->graph[1]) {
+                for (int i = 0; i < 2; i++) if (S->graph[i]) { (void)hipGraphExecDestroy(S->graph[i]); S->graph[i] = nullptr; }
+                if (!capture_graph(*S, tp, 0) || !capture_graph(*S, tp, 1)) return false;
+                S->graph_dt = dt;
+            }
+            for (uint32_t k = 0; k < remaining; k++)
+                if (!hip_ok(hipGraphLaunch(S->graph[(S->cur + k) & 1u], S->st), "hipGraphLaunch")) return false;
+        } else {
+            for (uint32_t k = 0; k < remaining; k++) {
+                tp.cur = (S->cur + k) & 1u;
+                rp_launch_tick(S->d, tp, S->island_grid, S->st);
+            }
+        }
+        uint32_t cnt[CT_COUNT];
+        if (!hip_ok(hipMemcpyAsync(cnt, S->d.cnt, sizeof(cnt), hipMemcpyDeviceToHost, S->st), "counters")) return false;
+        if (!hip_ok(hipStreamSynchronize(S->st), "step")) return false;
+        uint32_t done = cnt[CT_DONE];
+        S->cur ^= (done & 1u);
+        for (auto *w : ws) w->tick += done;
+        remaining -= done;
+        if (!cnt[CT_ABORT]) { if (done == 0 && remaining > 0) { set_err("step made no progress"); return false; } continue; }
+        if (cnt[CT_BIG_ISLAND]) { set_err("an island exceeds 1,048,576 contacts"); return false; }
+        // roll back the failed tick and grow
+        rp_launch_restore_backup(S->d, S->st);
+        if (!hip_ok(hipStreamSynchronize(S->st), "restore")) return false;
+        auto grow = [](uint32_t cap, uint32_t need) -> uint32_t {
+            uint64_t v = std::max<uint64_t>((uint64_t)need + need / 2 + 1024, (uint64_t)cap * 2);
+            return (uint32_t)std::min<uint64_t>(v, 0x7FFFFFFF);
+        };
+        if (cnt[CT_NEED_PAIRS]) caps.pairs = grow(caps.pairs, cnt[CT_NEED_PAIRS]);
+        if (cnt[CT_NEED_CANDS]) caps.cands = grow(caps.cands, cnt[CT_NEED_CANDS]);
+        if (cnt[CT_NEED_CONTACTS]) caps.contacts = grow(caps.contacts, cnt[CT_NEED_CONTACTS]);
+        if (cnt[CT_NEED_WARM]) caps.warm = grow(caps.warm, cnt[CT_NEED_WARM]);
+        for (uint32_t i = 0; i < ws.size(); i++) if (!pull_world(*S, i, *ws[i])) return false;
+        uint32_t cur = S->cur;
+        Scene *N = build_scene(ws, caps, cur);
+        if (!N) return false;
+        N->island_grid = S->island_grid;
+        delete S;
+        S = N;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ the world handle
+struct RPhysWorld {
+    std::mutex mu;
+    HostWorld w;
+    Scene *scene = nullptr;
+    bool dev_newer = false;    // device state newer than the host mirror
+    bool need_build = true;    // host structure or state changed since the last upload
+    bool have_contacts = false;  // the scene's contact arrays hold the last tick's contacts
+    ~RPhysWorld() { delete scene; }
+};
+
+static bool pull(RPhysWorld *W) {
+    if (W->dev_newer && W->scene) {
+        if (!pull_world(*W->scene, 0, W->w)) return false;
+        W->dev_newer = false;
+    }
+    return true;
+}
+static void host_changed(RPhysWorld *W) { W->need_build = true; }
+static bool push(RPhysWorld *W) {
+    if (!W->need_build && W->scene) return true;
+    Caps caps = W->scene ? Caps{W->scene->d.cap_pairs, W->scene->d.cap_cands, W->scene->d.cap_contacts, W->scene->d.cap_warm}
+                         : initial_caps({&W->w});
+    uint32_t cur = W->scene ? W->scene->cur : 0;
+    std::vector<HostWorld *> ws{&W->w};
+    Scene *S = build_scene(ws, caps, cur);
+    if (!S) return false;
+    delete W->scene;
+    W->scene = S;
+    W->need_build = false;
+    W->have_contacts = false;
+    return true;
+}
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                                     \
+    }                                                                                 \
+    catch (const std::bad_alloc &) { set_err("out of host memory"); return RPHYS_ERR_INTERNAL; } \
+    catch (...) { set_err("internal error crossed the C ABI"); return RPHYS_ERR_INTERNAL; }
+
+static bool fin3(const RPhysVec3 &v) { return hfin(v.x) && hfin(v.y) && hfin(v.z); }
+static bool finq(const RPhysQuat &q) { return hfin(q.x) && hfin(q.y) && hfin(q.z) && hfin(q.w); }
+static bool fin_iso(const RPhysIsometry &p) { return fin3(p.pos) && finq(p.rot); }
+static bool fin_vel(const RPhysVelocity &v) { return fin3(v.lin) && fin3(v.ang); }
+static bool fin_mat(const RPhysMaterial &m) { return hfin(m.mu_static) && hfin(m.mu_dynamic) && hfin(m.restitution); }
+
+static RPhysResult bad_arg(const char *ctx, const char *msg) { set_err(std::string(ctx) + ": " + msg); return RPHYS_ERR_BAD_ARG; }
+
+// lib.rs:369-409 checked_live_body_id (1-based ids, 0 and the terrain sentinel rejected)
+static RPhysResult live_id(const RPhysWorld *W, uint32_t id, const char *ctx, uint32_t &out) {
+    if (id == RPHYS_INVALID_BODY_ID) { set_err(std::string(ctx) + ": body_id == 0 (invalid)"); return RPHYS_ERR_INVALID_ID; }
+    if (id == RPHYS_TERRAIN_BODY_ID) { set_err(std::string(ctx) + ": body_id is terrain sentinel (not a real body)"); return RPHYS_ERR_INVALID_ID; }
+    uint32_t raw = id - 1;
+    if (raw >= W->w.bodies_created) { set_err(std::string(ctx) + ": body_id out of range"); return RPHYS_ERR_INVALID_ID; }
+    if (!(W->w.bflags[raw] & BF_ALIVE)) { set_err(std::string(ctx) + ": body_id not found (removed)"); return RPHYS_ERR_INVALID_ID; }
+    out = raw;
+    return RPHYS_OK;
+}
+
+// mass.rs:92-165 density primitives + MassProps::finalize; world.rs:595-614 add_body
+static void add_body_host(HostWorld &w, uint32_t shape, const float dims[3], const RPhysIsometry &pose, const RPhysVelocity &vel, bool dynamic, float density) {
+    float rho = density > 0.0f ? density : 1500.0f;
+    float m = 0.0f, ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    bool ok = false;
+    const float PI = 3.14159265358979323846f;
+    if (dynamic) {
+        if (shape == RPHYS_SHAPE_SPHERE) {
+            float r = std::fabs(dims[0]);
+            if (hfin(r) && r > 0 && hfin(rho) && rho > 0) { float r2 = r * r; float vol = ((4.0f / 3.0f) * PI * r2) * r; m = rho * vol; ix = iy = iz = (0.4f * m) * r2; ok = true; }
+        } else if (shape == RPHYS_SHAPE_BOX) {
+            float hx = std::fabs(dims[0]), hy = std::fabs(dims[1]), hz = std::fabs(dims[2]);
+            if (hfin(hx) && hfin(hy) && hfin(hz) && hfin(rho) && rho > 0) {
+                float dx = hx * 2.0f, dy = hy * 2.0f, dz = hz * 2.0f;
+                if (dx > 0 && dy > 0 && dz > 0) {
+                    m = rho * ((dx * dy) * dz);
+                    float x2 = dx * dx, y2 = dy * dy, z2 = dz * dz;
+                    ix = ((1.0f / 12.0f) * m) * (y2 + z2); iy = ((1.0f / 12.0f) * m) * (x2 + z2); iz = ((1.0f / 12.0f) * m) * (x2 + y2);
+                    ok = true;
+                }
+            }
+        } else {
+            float r = std::fabs(dims[0]), hh = std::fabs(dims[1]);
+            if (hfin(r) && r > 0 && hfin(rho) && rho > 0) {
+                float h = hh * 2.0f, r2 = r * r;
+                float vc = (PI * r2) * h, vs = ((4.0f / 3.0f) * PI * r2) * r;
+                m = rho * (vc + vs);
+                ix = ((0.25f * m) * r2) + (((1.0f / 12.0f) * m) * h) * h; iy = (0.5f * m) * r2; iz = ix;
+                ok = true;
+            }
+        }
+        if (ok && (!(hfin(m) && m > 0.0f) || m < 1.0e-12f)) ok = false;
+        if (ok && !(hfin(ix) && ix > 0 && hfin(iy) && iy > 0 && hfin(iz) && iz > 0)) ok = false;
+    }
+    float inv_mass = (dynamic && ok) ? 1.0f / m : 0.0f;
+    HM I; I.c0 = hv(ok ? ix : 0, 0, 0); I.c1 = hv(0, ok ? iy : 0, 0); I.c2 = hv(0, 0, ok ? iz : 0);
+    float det = hdot(I.c2, hcross(I.c0, I.c1));
+    bool have_ii = dynamic && hfin(det) && det > 0.0f;
+    HM inv = have_ii ? hinv(I) : HM{};
+    bool is_dyn = dynamic && hfin(inv_mass) && inv_mass > 0.0f;
+    float im = is_dyn ? inv_mass : 0.0f;
+    w.pos.push_back(fin3(pose.pos) ? f4(pose.pos.x, pose.pos.y, pose.pos.z) : f4(0, 0, 0));
+    w.rot.push_back(fin_q(f4(pose.rot.x, pose.rot.y, pose.rot.z, pose.rot.w)));
+    w.lin.push_back(fin3(vel.lin) ? f4(vel.lin.x, vel.lin.y, vel.lin.z) : f4(0, 0, 0));
+    w.ang.push_back(fin3(vel.ang) ? f4(vel.ang.x, vel.ang.y, vel.ang.z) : f4(0, 0, 0));
+    w.last_lin.push_back(f4(vel.lin.x, vel.lin.y, vel.lin.z));
+    w.inv_mass.push_back(im);
+    w.bflags.push_back((is_dyn ? BF_DYN : 0u) | BF_ALIVE | BF_SLEEP_OK);
+    float m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (is_dyn) {
+        bool inv_fin = hfin(inv.c0.x) && hfin(inv.c0.y) && hfin(inv.c0.z) && hfin(inv.c1.x) && hfin(inv.c1.y) && hfin(inv.c1.z) &&
+                       hfin(inv.c2.x) && hfin(inv.c2.y) && hfin(inv.c2.z);
+        if (have_ii && inv_fin) { float t[9] = {inv.c0.x, inv.c0.y, inv.c0.z, inv.c1.x, inv.c1.y, inv.c1.z, inv.c2.x, inv.c2.y, inv.c2.z}; std::memcpy(m9, t, sizeof m9); }
+        else { m9[0] = im; m9[4] = im; m9[8] = im; }
+    }
+    w.iil.insert(w.iil.end(), m9, m9 + 9);
+    w.awake.push_back(1); w.sticks.push_back(0);
+    w.btype.push_back(shape);
+    w.dims.push_back(f4(dims[0], shape == RPHYS_SHAPE_SPHERE ? 0.0f : dims[1], shape == RPHYS_SHAPE_BOX ? dims[2] : 0.0f));
+    w.col_body.push_back(w.nb() - 1);
+    w.col_mat.push_back(0);  // MaterialId::Default (lib.rs:543-557 keeps the id)
+    w.bodies_created++;
+}
+
+extern "C" {
+
+uint32_t rphys_get_abi_version(void) { return RPHYS_ABI_VERSION; }
+const char *rphys_get_build_id(void) { return "riftphys-mi355x gfx950 step library"; }
+const char *rphys_last_error_message(void) { return g_last_error.empty() ? nullptr : g_last_error.c_str(); }
+
+RPhysResult rphys_last_error_copy(char *dst, uint32_t cap, uint32_t *out_len) {
+    uint32_t len = (uint32_t)g_last_error.size();
+    if (out_len) *out_len = len;
+    if (cap == 0) return RPHYS_OK;
+    if (!dst) return RPHYS_ERR_NULL;
+    uint32_t n = std::min(len, cap - 1);
+    std::memcpy(dst, g_last_error.data(), n);
+    dst[n] = 0;
+    return RPHYS_OK;
+}
+
+RPhysWorld *rphys_world_create(uint32_t max_bodies, uint32_t max_colliders) {
+    clear_err();
+    try {
+        RPhysWorld *W = new RPhysWorld();
+        W->w.max_bodies = max_bodies;
+        W->w.max_colliders = max_colliders;
+        return W;
+    } catch (...) {
+        set_err("panic in rphys_world_create");
+        return nullptr;
+    }
+}
+void rphys_world_destroy(RPhysWorld *world) { delete world; }
+
+RPhysResult rphys_world_set_epoch(RPhysWorld *world, uint64_t epoch) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_epoch: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    world->w.epoch = epoch;
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_set_rng_seed(RPhysWorld *world, uint64_t seed) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_rng_seed: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    world->w.rng = seed | 1ull;
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+
+static RPhysResult add_common(RPhysWorld *world, const char *ctx, uint32_t shape, const float dims[3], const RPhysIsometry &pose,
+                              const RPhysVelocity &vel, RPhysBodyType type, float mass, bool check_mass, float density, uint32_t *out) {
+    if (!fin_iso(pose)) return bad_arg(ctx, "pose contains NaN/Inf");
+    if (!fin_vel(vel)) return bad_arg(ctx, "vel contains NaN/Inf");
+    if (type != RPHYS_BODY_STATIC && type != RPHYS_BODY_DYNAMIC) return bad_arg(ctx, "invalid body_type");
+    if (check_mass && type == RPHYS_BODY_DYNAMIC && !(hfin(mass) && mass > 0.0f)) return bad_arg(ctx, "dynamic body mass must be finite and > 0");
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (world->w.bodies_created + 1 > world->w.max_bodies || world->w.nc() + 1 > world->w.max_colliders) {
+        set_err(std::string(ctx) + ": body capacity exceeded");
+        return RPHYS_ERR_CAPACITY;
+    }
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    add_body_host(world->w, shape, dims, pose, vel, type == RPHYS_BODY_DYNAMIC, density);
+    host_changed(world);
+    *out = world->w.bodies_created;  // 1-based
+    return RPHYS_OK;
+}
+
+RPhysResult rphys_add_capsule_body(RPhysWorld *world, const RPhysCapsuleBodyDesc *desc, uint32_t *out_body_id) {
+    GUARD_BEGIN
+    if (out_body_id) *out_body_id = RPHYS_INVALID_BODY_ID;
+    if (!world || !desc || !out_body_id) { set_err("rphys_add_capsule_body: null arg"); return RPHYS_ERR_NULL; }
+    const auto &d = *desc;
+    if (!fin_mat(d.material)) return bad_arg("rphys_add_capsule_body", "material contains NaN/Inf");
+    if (!(hfin(d.radius) && d.radius > 0.0f)) return bad_arg("rphys_add_capsule_body", "radius must be finite and > 0");
+    if (!(hfin(d.half_height) && d.half_height >= 0.0f)) return bad_arg("rphys_add_capsule_body", "half_height must be finite and >= 0");
+    float dims[3] = {d.radius, d.half_height, 0.0f};
+    return add_common(world, "rphys_add_capsule_body", RPHYS_SHAPE_CAPSULE, dims, d.pose, d.vel, d.body_type, d.mass, true, 0.0f, out_body_id);
+    GUARD_END
+}
+RPhysResult rphys_add_box_body(RPhysWorld *world, const RPhysBoxBodyDesc *desc, uint32_t *out_body_id) {
+    GUARD_BEGIN
+    if (out_body_id) *out_body_id = RPHYS_INVALID_BODY_ID;
+    if (!world || !desc || !out_body_id) { set_err("rphys_add_box_body: null arg"); return RPHYS_ERR_NULL; }
+    const auto &d = *desc;
+    if (!fin_mat(d.material)) return bad_arg("rphys_add_box_body", "material contains NaN/Inf");
+    if (!(hfin(d.hx) && d.hx > 0.0f)) return bad_arg("rphys_add_box_body", "hx must be finite and > 0");
+    if (!(hfin(d.hy) && d.hy > 0.0f)) return bad_arg("rphys_add_box_body", "hy must be finite and > 0");
+    if (!(hfin(d.hz) && d.hz > 0.0f)) return bad_arg("rphys_add_box_body", "hz must be finite and > 0");
+    float dims[3] = {d.hx, d.hy, d.hz};
+    return add_common(world, "rphys_add_box_body", RPHYS_SHAPE_BOX, dims, d.pose, d.vel, d.body_type, d.mass, true, 0.0f, out_body_id);
+    GUARD_END
+}
+RPhysResult rphys_add_sphere_body(RPhysWorld *world, const RPhysSphereBodyDesc *desc, uint32_t *out_body_id) {
+    GUARD_BEGIN
+    if (out_body_id) *out_body_id = RPHYS_INVALID_BODY_ID;
+    if (!world || !desc || !out_body_id) { set_err("rphys_add_sphere_body: null arg"); return RPHYS_ERR_NULL; }
+    const auto &d = *desc;
+    if (!fin_mat(d.material)) return bad_arg("rphys_add_sphere_body", "material contains NaN/Inf");
+    if (!(hfin(d.radius) && d.radius > 0.0f)) return bad_arg("rphys_add_sphere_body", "radius must be finite and > 0");
+    float dims[3] = {d.radius, 0.0f, 0.0f};
+    return add_common(world, "rphys_add_sphere_body", RPHYS_SHAPE_SPHERE, dims, d.pose, d.vel, d.body_type, d.mass, true, 0.0f, out_body_id);
+    GUARD_END
+}
+RPhysResult rphys_add_body_density(RPhysWorld *world, RPhysShape shape, const float dims[3], const RPhysIsometry *pose, const RPhysVelocity *vel,
+                                   RPhysBodyType body_type, float density, uint32_t *out_body_id) {
+    GUARD_BEGIN
+    if (out_body_id) *out_body_id = RPHYS_INVALID_BODY_ID;
+    if (!world || !dims || !pose || !vel || !out_body_id) { set_err("rphys_add_body_density: null arg"); return RPHYS_ERR_NULL; }
+    if (shape > RPHYS_SHAPE_CAPSULE) return bad_arg("rphys_add_body_density", "unknown shape");
+    if (!hfin(density)) return bad_arg("rphys_add_body_density", "density must be finite");
+    for (int i = 0; i < (shape == RPHYS_SHAPE_BOX ? 3 : (shape == RPHYS_SHAPE_CAPSULE ? 2 : 1)); i++) {
+        bool okd = hfin(dims[i]) && (dims[i] > 0.0f || (shape == RPHYS_SHAPE_CAPSULE && i == 1 && dims[i] >= 0.0f));
+        if (!okd) return bad_arg("rphys_add_body_density", "dimensions must be finite and > 0");
+    }
+    return add_common(world, "rphys_add_body_density", shape, dims, *pose, *vel, body_type, 1.0f, false, density, out_body_id);
+    GUARD_END
+}
+
+RPhysResult rphys_body_remove(RPhysWorld *world, uint32_t body_id) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_body_remove: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_remove", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    HostWorld &w = world->w;
+    // world.rs:696-713: dead, deactivated (dynamics.rs:211-219), collider gone, warm entries dropped
+    w.bflags[b] &= ~(BF_ALIVE | BF_DYN);
+    w.lin[b] = f4(0, 0, 0); w.ang[b] = f4(0, 0, 0); w.inv_mass[b] = 0.0f;
+    for (int k = 0; k < 9; k++) w.iil[9 * b + k] = 0.0f;
+    size_t k = 0;
+    for (size_t c = 0; c < w.col_body.size(); c++) {
+        if (w.col_body[c] == b) continue;
+        w.col_body[k] = w.col_body[c]; w.col_mat[k] = w.col_mat[c]; k++;
+    }
+    w.col_body.resize(k); w.col_mat.resize(k);
+    drop_warm_of(w, b);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_body_get_pose(const RPhysWorld *cworld, uint32_t body_id, RPhysIsometry *out_pose) {
+    GUARD_BEGIN
+    if (!out_pose) { set_err("rphys_body_get_pose: out_pose is null"); return RPHYS_ERR_NULL; }
+    out_pose->pos = {0, 0, 0}; out_pose->rot = {0, 0, 0, 1};
+    if (!cworld) { set_err("rphys_body_get_pose: world is null"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_get_pose", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    F4 p = world->w.pos[b], q = world->w.rot[b];
+    out_pose->pos = {p.x, p.y, p.z}; out_pose->rot = {q.x, q.y, q.z, q.w};
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_body_set_pose(RPhysWorld *world, uint32_t body_id, const RPhysIsometry *pose) {
+    GUARD_BEGIN
+    if (!world || !pose) { set_err("rphys_body_set_pose: null arg"); return RPHYS_ERR_NULL; }
+    if (!fin_iso(*pose)) return bad_arg("rphys_body_set_pose", "pose contains NaN/Inf");
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_set_pose", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    drop_warm_of(world->w, b);  // world.rs:501-503
+    world->w.pos[b] = f4(pose->pos.x, pose->pos.y, pose->pos.z);
+    world->w.rot[b] = fin_q(f4(pose->rot.x, pose->rot.y, pose->rot.z, pose->rot.w));
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_body_get_velocity(const RPhysWorld *cworld, uint32_t body_id, RPhysVelocity *out_vel) {
+    GUARD_BEGIN
+    if (!out_vel) { set_err("rphys_body_get_velocity: out_vel is null"); return RPHYS_ERR_NULL; }
+    out_vel->lin = {0, 0, 0}; out_vel->ang = {0, 0, 0};
+    if (!cworld) { set_err("rphys_body_get_velocity: world is null"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_get_velocity", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    F4 l = world->w.lin[b], a = world->w.ang[b];
+    out_vel->lin = {l.x, l.y, l.z}; out_vel->ang = {a.x, a.y, a.z};
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_body_set_velocity(RPhysWorld *world, uint32_t body_id, const RPhysVelocity *vel) {
+    GUARD_BEGIN
+    if (!world || !vel) { set_err("rphys_body_set_velocity: null arg"); return RPHYS_ERR_NULL; }
+    if (!fin_vel(*vel)) return bad_arg("rphys_body_set_velocity", "velocity contains NaN/Inf");
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_set_velocity", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    // world.rs:674-676 (+ dynamics.rs:143-147)
+    world->w.lin[b] = f4(vel->lin.x, vel->lin.y, vel->lin.z);
+    world->w.ang[b] = f4(vel->ang.x, vel->ang.y, vel->ang.z);
+    world->w.last_lin[b] = f4(vel->lin.x, vel->lin.y, vel->lin.z);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_body_apply_impulse(RPhysWorld *world, uint32_t body_id, RPhysVec3 impulse) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_body_apply_impulse: world is null"); return RPHYS_ERR_NULL; }
+    if (!fin3(impulse)) return bad_arg("rphys_body_apply_impulse", "impulse contains NaN/Inf");
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_apply_impulse", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    // world.rs:689-692: wake, then v += j * inv_mass
+    HostWorld &w = world->w;
+    w.awake[b] = 1; w.sticks[b] = 0;
+    float im = w.inv_mass[b];
+    if (im > 0.0f) {
+        F4 l = w.lin[b];
+        float x = l.x + impulse.x * im, y = l.y + impulse.y * im, z = l.z + impulse.z * im;
+        w.lin[b] = f4(hfin(x) ? x : 0.0f, hfin(y) ? y : 0.0f, hfin(z) ? z : 0.0f);
+    }
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_body_set_sleep_allowed(RPhysWorld *world, uint32_t body_id, RPhysBool allowed) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_body_set_sleep_allowed: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_set_sleep_allowed", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    if (allowed) world->w.bflags[b] |= BF_SLEEP_OK;
+    else { world->w.bflags[b] &= ~BF_SLEEP_OK; world->w.awake[b] = 1; world->w.sticks[b] = 0; }
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+
+static RPhysResult step_impl(RPhysWorld *world, float dt, uint32_t substeps, uint32_t iv_override, uint32_t repeat, bool graph) {
+    std::lock_guard<std::mutex> lk(world->mu);
+    float sub = dt / (float)substeps;
+    if (!hfin(sub)) { set_err("rphys_world_step_ex: sub_dt is not finite"); return RPHYS_ERR_BAD_ARG; }
+    if (world->w.nb() == 0) { world->w.tick += (uint64_t)substeps * repeat; return RPHYS_OK; }
+    if (world->need_build) { if (!pull(world) || !push(world)) return RPHYS_ERR_INTERNAL; }
+    std::vector<HostWorld *> ws{&world->w};
+    uint32_t iv = iv_override ? iv_override : world->w.iv;
+    bool ok = run_ticks(world->scene, ws, sub, iv, substeps * repeat, graph);
+    world->dev_newer = true;
+    world->have_contacts = ok;
+    return ok ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+}
+
+RPhysResult rphys_world_step_ex(RPhysWorld *world, const RPhysStepParams *p) {
+    GUARD_BEGIN
+    if (!world || !p) { set_err("rphys_world_step_ex: null arg"); return RPHYS_ERR_NULL; }
+    if (p->size_bytes < sizeof(RPhysStepParams)) { set_err("rphys_world_step_ex: size_bytes too small"); return RPHYS_ERR_BAD_ARG; }
+    if (!hfin(p->dt)) { set_err("rphys_world_step_ex: dt is not finite"); return RPHYS_ERR_BAD_ARG; }
+    if (p->dt < 0.0f) { set_err("rphys_world_step_ex: dt is negative"); return RPHYS_ERR_BAD_ARG; }
+    if (p->substeps == 0) { set_err("rphys_world_step_ex: substeps == 0"); return RPHYS_ERR_BAD_ARG; }
+    uint32_t iv = ((p->flags & RPHYS_STEP_FLAG_SOLVER_ITERS) && p->solver_iters > 0) ? p->solver_iters : 0;
+    return step_impl(world, p->dt, p->substeps, iv, 1, false);
+    GUARD_END
+}
+RPhysResult rphys_world_step(RPhysWorld *world, float dt) {
+    RPhysStepParams p{(uint32_t)sizeof(RPhysStepParams), 0, 1, 0, dt};
+    return rphys_world_step_ex(world, &p);
+}
+RPhysResult rphys_world_step_n(RPhysWorld *world, float dt, uint32_t n_ticks) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_step_n: world is null"); return RPHYS_ERR_NULL; }
+    if (!hfin(dt) || dt < 0.0f) { set_err("rphys_world_step_n: dt must be finite and >= 0"); return RPHYS_ERR_BAD_ARG; }
+    if (n_ticks == 0) return RPHYS_OK;
+    return step_impl(world, dt, 1, 0, n_ticks, false);
+    GUARD_END
+}
+
+RPhysResult rphys_world_tick_index(const RPhysWorld *cworld, uint64_t *out_tick) {
+    GUARD_BEGIN
+    if (!out_tick) { set_err("rphys_world_tick_index: out_tick is null"); return RPHYS_ERR_NULL; }
+    *out_tick = 0;
+    if (!cworld) { set_err("rphys_world_tick_index: world is null"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    *out_tick = world->w.tick;
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_num_bodies(const RPhysWorld *cworld, uint32_t *out_num) {
+    GUARD_BEGIN
+    if (!out_num) { set_err("rphys_world_num_bodies: out_num is null"); return RPHYS_ERR_NULL; }
+    *out_num = 0;
+    if (!cworld) { set_err("rphys_world_num_bodies: world is null"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    *out_num = world->w.nb();
+    return RPHYS_OK;
+    GUARD_END
+}
+
+static bool device_hash(Scene &S, uint8_t *out, uint32_t nw) {
+    rp_launch_hash(S.d, S.h, S.cur, S.max_extra, S.st);
+    return hip_ok(hipMemcpyAsync(out, S.h.digest, 32 * (size_t)nw, hipMemcpyDeviceToHost, S.st), "hash copy") &&
+           hip_ok(hipStreamSynchronize(S.st), "hash");
+}
+
+RPhysResult rphys_world_get_step_hash(const RPhysWorld *cworld, RPhysHash32 *out_hash) {
+    GUARD_BEGIN
+    if (!out_hash) { set_err("rphys_world_get_step_hash: out_hash is null"); return RPHYS_ERR_NULL; }
+    std::memset(out_hash, 0, sizeof(*out_hash));
+    if (!cworld) { set_err("rphys_world_get_step_hash: world is null"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (world->need_build) { if (!pull(world) || !push(world)) return RPHYS_ERR_INTERNAL; }
+    if (world->w.nb() == 0 && !world->scene) {
+        // an empty world still hashes its header and sleep slots: build a scene for it
+        if (!push(world)) return RPHYS_ERR_INTERNAL;
+    }
+    if (!device_hash(*world->scene, out_hash->bytes, 1)) return RPHYS_ERR_INTERNAL;
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_step_and_hash(RPhysWorld *world, float dt, RPhysHash32 *out_hash) {
+    GUARD_BEGIN
+    if (!out_hash) { set_err("rphys_world_step_and_hash: out_hash is null"); return RPHYS_ERR_NULL; }
+    std::memset(out_hash, 0, sizeof(*out_hash));
+    RPhysResult r = rphys_world_step(world, dt);
+    if (r != RPHYS_OK) return r;
+    return rphys_world_get_step_hash(world, out_hash);
+    GUARD_END
+}
+
+RPhysResult rphys_world_raycast(const RPhysWorld *world, const RPhysRayQuery *query, RPhysRayHit *out_hit) {
+    if (out_hit) std::memset(out_hit, 0, sizeof(*out_hit));
+    if (!world || !query || !out_hit) { set_err("rphys_world_raycast: null arg"); return RPHYS_ERR_NULL; }
+    set_err("rphys_world_raycast: queries are not part of the step library");
+    return RPHYS_ERR_INTERNAL;
+}
+RPhysResult rphys_world_capsule_sweep(const RPhysWorld *world, const RPhysCapsuleSweepQuery *query, RPhysCapsuleSweepHit *out_hit) {
+    if (out_hit) std::memset(out_hit, 0, sizeof(*out_hit));
+    if (!world || !query || !out_hit) { set_err("rphys_world_capsule_sweep: null arg"); return RPHYS_ERR_NULL; }
+    set_err("rphys_world_capsule_sweep: queries are not part of the step library");
+    return RPHYS_ERR_INTERNAL;
+}
+
+// terrain.rs:24-52 from_heights, material Grit (lib.rs:1479,1536)
+static void set_hf(HostWorld &w, uint32_t nx, uint32_t nz, float sx, float sz, float ox, float oz, std::vector<float> &&h) {
+    w.hnx = nx; w.hnz = nz;
+    w.hsx = (hfin(sx) && sx > 0.0f) ? q6div(sx) : 1.0f;
+    w.hsz = (hfin(sz) && sz > 0.0f) ? q6div(sz) : 1.0f;
+    w.hox = ox; w.hoz = oz;
+    float mn = INFINITY, mx = -INFINITY;
+    for (float &v : h) {
+        float x = hfin(v) ? v : 0.0f;
+        v = q6div(x);
+        mn = std::fmin(mn, v); mx = std::fmax(mx, v);
+    }
+    w.hmin = q6div(mn); w.hmax = q6div(mx);
+    w.heights = std::move(h);
+    w.hmat = 4;
+    w.has_hf = true;
+}
+
+RPhysResult rphys_world_set_heightfield_i16(RPhysWorld *world, const RPhysHeightfieldDesc *hf, float origin_x, float origin_z, float y_offset) {
+    GUARD_BEGIN
+    if (!world || !hf) { set_err("rphys_world_set_heightfield_i16: null arg"); return RPHYS_ERR_NULL; }
+    const auto &d = *hf;
+    if (d.size_bytes < sizeof(RPhysHeightfieldDesc)) { set_err("rphys_world_set_heightfield_i16: size_bytes too small"); return RPHYS_ERR_BAD_ARG; }
+    if (d.width == 0 || d.height == 0) { set_err("rphys_world_set_heightfield_i16: width/height == 0"); return RPHYS_ERR_BAD_ARG; }
+    if (d.row_stride != 0 && d.row_stride < d.width) return bad_arg("rphys_world_set_heightfield_i16", "row_stride must be >= width");
+    if (!d.heights) { set_err("rphys_world_set_heightfield_i16: heights is null"); return RPHYS_ERR_NULL; }
+    if (!(hfin(d.cell_size_x) && hfin(d.cell_size_z) && d.cell_size_x > 0.0f && d.cell_size_z > 0.0f)) { set_err("rphys_world_set_heightfield_i16: bad cell sizes"); return RPHYS_ERR_BAD_ARG; }
+    uint32_t stride = d.row_stride == 0 ? d.width : d.row_stride;
+    std::vector<float> h;
+    h.reserve((size_t)d.width * d.height);
+    for (uint32_t r = 0; r < d.height; r++)
+        for (uint32_t c = 0; c < d.width; c++) h.push_back(((float)d.heights[(size_t)r * stride + c] * d.height_scale + d.height_offset) + y_offset);
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    set_hf(world->w, d.width, d.height, d.cell_size_x, d.cell_size_z, origin_x, origin_z, std::move(h));
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_set_heightfield_raw32_square(RPhysWorld *world, const uint8_t *bytes, uint32_t bytes_len, float world_size_x,
+                                                     float world_size_z, float origin_x, float origin_z, float y_offset) {
+    GUARD_BEGIN
+    if (!world || !bytes) { set_err("rphys_world_set_heightfield_raw32_square: null arg"); return RPHYS_ERR_NULL; }
+    if (bytes_len < 4) return bad_arg("rphys_world_set_heightfield_raw32_square", "bytes_len too small to form a grid");
+    size_t count = bytes_len / 4;
+    size_t dim = (size_t)std::sqrt((double)count);
+    if (dim * dim != count) return bad_arg("rphys_world_set_heightfield_raw32_square", "byte length is not a square number of f32s");
+    float sx = world_size_x / std::fmax((float)dim - 1.0f, 1.0f);
+    float sz = world_size_z / std::fmax((float)dim - 1.0f, 1.0f);
+    std::vector<float> h(count);
+    std::memcpy(h.data(), bytes, count * 4);
+    for (float &v : h) v = v + y_offset;
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    set_hf(world->w, (uint32_t)dim, (uint32_t)dim, sx, sz, origin_x, origin_z, std::move(h));
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_clear_heightfield(RPhysWorld *world) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_clear_heightfield: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    world->w.has_hf = false;
+    world->w.heights.clear();
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t a, uint32_t b, float mu_s, float mu_k, float rest, float vs) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_pair_props: world is null"); return RPHYS_ERR_NULL; }
+    if (a > 15 || b > 15) return bad_arg("rphys_world_set_pair_props", "material ids above 15 are not reachable through this ABI");
+    if (!(hfin(mu_s) && hfin(mu_k) && hfin(rest) && hfin(vs))) return bad_arg("rphys_world_set_pair_props", "non-finite property");
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    uint32_t x = std::min(a, b), y = std::max(a, b);
+    auto &o = world->w.ovr;
+    o.erase(std::remove_if(o.begin(), o.end(), [&](const PairOverride &p) { return p.a == x && p.b == y; }), o.end());
+    o.push_back(PairOverride{x, y, MatP{mu_s, mu_k, rest, vs}});
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_set_solver_iterations(RPhysWorld *world, uint32_t velocity, uint32_t position) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_solver_iterations: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    world->w.iv = std::max(velocity, 1u);
+    world->w.ip = std::max(position, 1u);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_set_gravity(RPhysWorld *world, RPhysVec3 g) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_gravity: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    world->w.gx = g.x; world->w.gy = g.y; world->w.gz = g.z;
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_world_get_stats(const RPhysWorld *cworld, RPhysStepStats *out) {
+    GUARD_BEGIN
+    if (!cworld || !out) { set_err("rphys_world_get_stats: null arg"); return RPHYS_ERR_NULL; }
+    std::memset(out, 0, sizeof(*out));
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!world->scene) return RPHYS_OK;
+    uint32_t c[CT_COUNT];
+    Scene &S = *world->scene;
+    if (!hip_ok(hipMemcpyAsync(c, S.d.cnt, sizeof c, hipMemcpyDeviceToHost, S.st), "stats") || !hip_ok(hipStreamSynchronize(S.st), "stats")) return RPHYS_ERR_INTERNAL;
+    uint32_t wt = 0;
+    if (!hip_ok(hipMemcpy(&wt, S.d.wc_off[S.cur] + S.d.nw, 4, hipMemcpyDeviceToHost), "stats")) return RPHYS_ERR_INTERNAL;
+    out->pairs = c[CT_PAIRS]; out->contacts = c[CT_CONTACTS]; out->terrain_candidates = c[CT_CANDS];
+    out->island_contacts = c[CT_ISLC]; out->islands = c[CT_ISLANDS]; out->warm_entries = wt;
+    return RPHYS_OK;
+    GUARD_END
+}
+
+struct ContactDump {
+    std::vector<uint32_t> a, b, cell;
+    std::vector<F4> p, n;
+    std::vector<float> imp;
+};
+static bool dump_contacts(Scene &S, ContactDump &D) {
+    uint32_t n = 0;
+    if (!hip_ok(hipMemcpy(&n, S.d.cnt + CT_CONTACTS, 4, hipMemcpyDeviceToHost), "contacts")) return false;
+    n = std::min(n, S.d.cap_contacts);
+    D.a.resize(n); D.b.resize(n); D.cell.resize(n); D.p.resize(n); D.n.resize(n); D.imp.resize(n);
+    if (n == 0) return true;
+    return hip_ok(hipMemcpy(D.a.data(), S.d.ct_a, 4 * n, hipMemcpyDeviceToHost), "contacts") &&
+           hip_ok(hipMemcpy(D.b.data(), S.d.ct_b, 4 * n, hipMemcpyDeviceToHost), "contacts") &&
+           hip_ok(hipMemcpy(D.cell.data(), S.d.ct_cell, 4 * n, hipMemcpyDeviceToHost), "contacts") &&
+           hip_ok(hipMemcpy(D.p.data(), S.d.ct_p, 16 * n, hipMemcpyDeviceToHost), "contacts") &&
+           hip_ok(hipMemcpy(D.n.data(), S.d.ct_n, 16 * n, hipMemcpyDeviceToHost), "contacts") &&
+           hip_ok(hipMemcpy(D.imp.data(), S.d.ct_imp, 4 * n, hipMemcpyDeviceToHost), "contacts");
+}
+
+RPhysResult rphys_world_get_contacts(const RPhysWorld *cworld, RPhysContactEvent *out_buffer, uint32_t buffer_capacity, uint32_t *out_count) {
+    GUARD_BEGIN
+    if (!cworld || !out_count) { set_err("rphys_world_get_contacts: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    *out_count = 0;
+    if (!world->scene || !world->have_contacts) return RPHYS_OK;
+    ContactDump D;
+    if (!dump_contacts(*world->scene, D)) return RPHYS_ERR_INTERNAL;
+    *out_count = (uint32_t)D.a.size();
+    if (!out_buffer) return RPHYS_OK;
+    uint32_t n = std::min(*out_count, buffer_capacity);
+    const HostWorld &w = world->w;
+    for (uint32_t i = 0; i < n; i++) {
+        RPhysContactEvent e;
+        e.body_id_a = w.col_body[D.a[i]] + 1;
+        e.body_id_b = D.b[i] == RP_TERRAIN ? RPHYS_TERRAIN_BODY_ID : w.col_body[D.b[i]] + 1;
+        e.position = {D.p[i].x, D.p[i].y, D.p[i].z};
+        e.normal = {D.n[i].x, D.n[i].y, D.n[i].z};
+        e.impulse = D.imp[i];
+        out_buffer[i] = e;
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_debug_get_contacts_ex(const RPhysWorld *cworld, RPhysContactEx *out, uint32_t cap, uint32_t *out_count) {
+    GUARD_BEGIN
+    if (!cworld || !out_count) { set_err("rphys_debug_get_contacts_ex: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    *out_count = 0;
+    if (!world->scene || !world->have_contacts) return RPHYS_OK;
+    ContactDump D;
+    if (!dump_contacts(*world->scene, D)) return RPHYS_ERR_INTERNAL;
+    *out_count = (uint32_t)D.a.size();
+    if (!out) return RPHYS_OK;
+    uint32_t n = std::min(*out_count, cap);
+    for (uint32_t i = 0; i < n; i++) {
+        RPhysContactEx e;
+        e.a_collider = D.a[i]; e.b_collider = D.b[i];
+        e.point[0] = D.p[i].x; e.point[1] = D.p[i].y; e.point[2] = D.p[i].z; e.depth = D.p[i].w;
+        e.normal[0] = D.n[i].x; e.normal[1] = D.n[i].y; e.normal[2] = D.n[i].z;
+        uint32_t f; std::memcpy(&f, &D.n[i].w, 4);
+        e.feature = f; e.cell = D.cell[i]; e.impulse = D.imp[i];
+        out[i] = e;
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_debug_get_pairs(const RPhysWorld *cworld, uint32_t *out_ij, uint32_t cap, uint32_t *out_count) {
+    GUARD_BEGIN
+    if (!cworld || !out_count) { set_err("rphys_debug_get_pairs: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    *out_count = 0;
+    if (!world->scene || !world->have_contacts) return RPHYS_OK;
+    Scene &S = *world->scene;
+    uint32_t n = 0;
+    if (!hip_ok(hipMemcpy(&n, S.d.cnt + CT_PAIRS, 4, hipMemcpyDeviceToHost), "pairs")) return RPHYS_ERR_INTERNAL;
+    n = std::min(n, S.d.cap_pairs);
+    *out_count = n;
+    if (!out_ij) return RPHYS_OK;
+    uint32_t m = std::min(n, cap);
+    std::vector<uint32_t> a(m), b(m);
+    if (m && (!hip_ok(hipMemcpy(a.data(), S.d.pair_i, 4 * m, hipMemcpyDeviceToHost), "pairs") ||
+              !hip_ok(hipMemcpy(b.data(), S.d.pair_j, 4 * m, hipMemcpyDeviceToHost), "pairs"))) return RPHYS_ERR_INTERNAL;
+    for (uint32_t i = 0; i < m; i++) { out_ij[2 * i] = a[i]; out_ij[2 * i + 1] = b[i]; }
+    return RPHYS_OK;
+    GUARD_END
+}
+
+// ------------------------------------------------------------------ snapshots (lib.rs:1696-1800)
+struct RPhysWorldSnapshot {
+    uint64_t tick;
+    std::vector<F4> pos, rot, lin, ang, last_lin;
+    std::vector<uint32_t> awake, sticks;
+    std::vector<uint64_t> wkey;
+    std::vector<F4> wla, wjt;
+    std::vector<float> wjn;
+};
+RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
+    clear_err();
+    try {
+        if (!cworld) { set_err("rphys_world_snapshot_create: world is null"); return nullptr; }
+        RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+        std::lock_guard<std::mutex> lk(world->mu);
+        if (!pull(world)) return nullptr;
+        const HostWorld &w = world->w;
+        auto *s = new RPhysWorldSnapshot();
+        s->tick = w.tick;
+        s->pos = w.pos; s->rot = w.rot; s->lin = w.lin; s->ang = w.ang;
+        for (uint32_t i = 0; i < w.nb(); i++)
+            if (!(w.bflags[i] & BF_ALIVE)) { s->pos[i] = f4(0, 0, 0); s->rot[i] = f4(0, 0, 0, 1); s->lin[i] = f4(0, 0, 0); s->ang[i] = f4(0, 0, 0); }
+        s->last_lin = w.last_lin; s->awake = w.awake; s->sticks = w.sticks;
+        s->wkey = w.wkey; s->wla = w.wla; s->wjn = w.wjn; s->wjt = w.wjt;
+        return s;
+    } catch (...) {
+        set_err("panic in rphys_world_snapshot_create");
+        return nullptr;
+    }
+}
+RPhysResult rphys_world_snapshot_restore(RPhysWorld *world, const RPhysWorldSnapshot *s) {
+    GUARD_BEGIN
+    if (!world || !s) { set_err("rphys_world_snapshot_restore: null arg"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    HostWorld &w = world->w;
+    w.tick = s->tick;
+    for (uint32_t i = 0; i < s->pos.size() && i < w.nb(); i++) {
+        if (!(w.bflags[i] & BF_ALIVE)) continue;
+        // set_body_pose (world.rs:501-506) then set_body_vel (world.rs:674-676)
+        w.pos[i] = s->pos[i];
+        w.rot[i] = fin_q(s->rot[i]);
+        w.lin[i] = s->lin[i]; w.ang[i] = s->ang[i];
+    }
+    // restore_carried_state (world.rs:927-961): warm cache, last velocities, sleep, dead keys dropped
+    w.wkey.clear(); w.wla.clear(); w.wjn.clear(); w.wjt.clear();
+    auto alive = [&](uint32_t b) { return b == 0xFFFFFFFFu || (b < w.nb() && (w.bflags[b] & BF_ALIVE)); };
+    for (size_t k = 0; k < s->wkey.size(); k++) {
+        if (!alive((uint32_t)(s->wkey[k] >> 32)) || !alive((uint32_t)s->wkey[k])) continue;
+        w.wkey.push_back(s->wkey[k]); w.wla.push_back(s->wla[k]); w.wjn.push_back(s->wjn[k]); w.wjt.push_back(s->wjt[k]);
+    }
+    for (uint32_t i = 0; i < s->last_lin.size() && i < w.nb(); i++) w.last_lin[i] = s->last_lin[i];
+    for (uint32_t i = 0; i < s->awake.size() && i < w.nb(); i++) { w.awake[i] = s->awake[i]; w.sticks[i] = s->sticks[i]; }
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+void rphys_world_snapshot_free(RPhysWorldSnapshot *s) { delete s; }
+
+// ------------------------------------------------------------------ batches
+struct RPhysBatch {
+    std::vector<HostWorld> worlds;
+    Scene *scene = nullptr;
+    ~RPhysBatch() { delete scene; }
+};
+
+RPhysBatch *rphys_batch_create(RPhysWorld *const *worlds, uint32_t n_worlds) {
+    clear_err();
+    try {
+        if (!worlds || n_worlds == 0) { set_err("rphys_batch_create: no worlds"); return nullptr; }
+        auto *B = new RPhysBatch();
+        B->worlds.reserve(n_worlds);
+        for (uint32_t i = 0; i < n_worlds; i++) {
+            if (!worlds[i]) { delete B; set_err("rphys_batch_create: null world"); return nullptr; }
+            std::lock_guard<std::mutex> lk(worlds[i]->mu);
+            if (!pull(worlds[i])) { delete B; return nullptr; }
+            B->worlds.push_back(worlds[i]->w);
+        }
+        std::vector<HostWorld *> ws;
+        for (auto &w : B->worlds) ws.push_back(&w);
+        B->scene = build_scene(ws, initial_caps(ws), 0);
+        if (!B->scene) { delete B; return nullptr; }
+        return B;
+    } catch (...) {
+        set_err("panic in rphys_batch_create");
+        return nullptr;
+    }
+}
+void rphys_batch_destroy(RPhysBatch *b) { delete b; }
+
+static RPhysResult batch_step(RPhysBatch *b, float dt, uint32_t n, bool graph) {
+    if (!b) { set_err("rphys_batch_step: batch is null"); return RPHYS_ERR_NULL; }
+    if (!hfin(dt) || dt < 0.0f) { set_err("rphys_batch_step: dt must be finite and >= 0"); return RPHYS_ERR_BAD_ARG; }
+    std::vector<HostWorld *> ws;
+    for (auto &w : b->worlds) ws.push_back(&w);
+    return run_ticks(b->scene, ws, dt, b->worlds[0].iv, n, graph) ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+}
+RPhysResult rphys_batch_step_n(RPhysBatch *b, float dt, uint32_t n) { GUARD_BEGIN return batch_step(b, dt, n, false); GUARD_END }
+RPhysResult rphys_batch_step_graph(RPhysBatch *b, float dt, uint32_t n) { GUARD_BEGIN return batch_step(b, dt, n, true); GUARD_END }
+
+RPhysResult rphys_batch_get_hashes(const RPhysBatch *b, uint8_t *out) {
+    GUARD_BEGIN
+    if (!b || !out) { set_err("rphys_batch_get_hashes: null arg"); return RPHYS_ERR_NULL; }
+    return device_hash(*b->scene, out, b->scene->d.nw) ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+    GUARD_END
+}
+RPhysResult rphys_batch_total_bodies(const RPhysBatch *b, uint32_t *out) {
+    if (!b || !out) { set_err("rphys_batch_total_bodies: null arg"); return RPHYS_ERR_NULL; }
+    *out = b->scene->d.nb;
+    return RPHYS_OK;
+}
+RPhysResult rphys_batch_get_state(const RPhysBatch *cb, uint32_t wi, float *pos3, float *rot4, float *lin3, float *ang3) {
+    GUARD_BEGIN
+    if (!cb) { set_err("rphys_batch_get_state: null arg"); return RPHYS_ERR_NULL; }
+    RPhysBatch *b = const_cast<RPhysBatch *>(cb);
+    if (wi >= b->worlds.size()) { set_err("rphys_batch_get_state: world index out of range"); return RPHYS_ERR_BAD_ARG; }
+    if (!pull_world(*b->scene, wi, b->worlds[wi])) return RPHYS_ERR_INTERNAL;
+    const HostWorld &w = b->worlds[wi];
+    for (uint32_t i = 0; i < w.nb(); i++) {
+        if (pos3) { pos3[3 * i] = w.pos[i].x; pos3[3 * i + 1] = w.pos[i].y; pos3[3 * i + 2] = w.pos[i].z; }
+        if (rot4) { rot4[4 * i] = w.rot[i].x; rot4[4 * i + 1] = w.rot[i].y; rot4[4 * i + 2] = w.rot[i].z; rot4[4 * i + 3] = w.rot[i].w; }
+        if (lin3) { lin3[3 * i] = w.lin[i].x; lin3[3 * i + 1] = w.lin[i].y; lin3[3 * i + 2] = w.lin[i].z; }
+        if (ang3) { ang3[3 * i] = w.ang[i].x; ang3[3 * i + 1] = w.ang[i].y; ang3[3 * i + 2] = w.ang[i].z; }
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
+
+// Device-side rollback slots: body state, sleep state and the warm cache of every world.
+RPhysResult rphys_batch_snapshot(RPhysBatch *b, uint32_t slot) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_snapshot: batch is null"); return RPHYS_ERR_NULL; }
+    if (slot > 63) return bad_arg("rphys_batch_snapshot", "slot must be < 64");
+    Scene &S = *b->scene;
+    if (S.slots.size() <= slot) S.slots.resize(slot + 1);
+    const Dev &d = S.d;
+    if (!S.slots[slot]) {
+        std::unique_ptr<Slot> sl(new Slot());
+        size_t NB = d.nb + 1, W = d.cap_warm;
+        bool ok = sl->buf.alloc(sl->pos, NB) && sl->buf.alloc(sl->rot, NB) && sl->buf.alloc(sl->lin, NB) && sl->buf.alloc(sl->ang, NB) &&
+                  sl->buf.alloc(sl->last, NB) && sl->buf.alloc(sl->awake, NB) && sl->buf.alloc(sl->sticks, NB) &&
+                  sl->buf.alloc(sl->wkey, W) && sl->buf.alloc(sl->wla, W) && sl->buf.alloc(sl->wjn, W) && sl->buf.alloc(sl->wjt, W) &&
+                  sl->buf.alloc(sl->woff, d.nw + 1);
+        if (!ok) { set_err("rphys_batch_snapshot: device allocation failed"); return RPHYS_ERR_INTERNAL; }
+        S.slots[slot] = std::move(sl);
+    }
+    Slot &sl = *S.slots[slot];
+    uint32_t nwarm = 0;
+    if (!hip_ok(hipMemcpyAsync(&nwarm, d.wc_off[S.cur] + d.nw, 4, hipMemcpyDeviceToHost, S.st), "snapshot") || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
+    size_t n = d.nb;
+    auto cp = [&](void *dst, const void *src, size_t bytes) { return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.st), "snapshot copy"); };
+    bool ok = cp(sl.pos, d.pos, 16 * n) && cp(sl.rot, d.rot, 16 * n) && cp(sl.lin, d.lin, 16 * n) && cp(sl.ang, d.ang, 16 * n) &&
+              cp(sl.last, d.last_lin, 16 * n) && cp(sl.awake, d.awake, 4 * n) && cp(sl.sticks, d.sticks, 4 * n) &&
+              cp(sl.wkey, d.wk_key[S.cur], 8 * (size_t)nwarm) && cp(sl.wla, d.wk_la[S.cur], 16 * (size_t)nwarm) &&
+              cp(sl.wjn, d.wk_jn[S.cur], 4 * (size_t)nwarm) && cp(sl.wjt, d.wk_jt[S.cur], 16 * (size_t)nwarm) &&
+              cp(sl.woff, d.wc_off[S.cur], 4 * ((size_t)d.nw + 1));
+    if (!ok || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
+    sl.cur = S.cur; sl.nwarm = nwarm; sl.valid = true;
+    sl.ticks.clear();
+    for (auto &w : b->worlds) sl.ticks.push_back(w.tick);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_batch_restore(RPhysBatch *b, uint32_t slot) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_restore: batch is null"); return RPHYS_ERR_NULL; }
+    Scene &S = *b->scene;
+    if (slot >= S.slots.size() || !S.slots[slot] || !S.slots[slot]->valid) return bad_arg("rphys_batch_restore", "empty slot");
+    Slot &sl = *S.slots[slot];
+    const Dev &d = S.d;
+    if (sl.nwarm > d.cap_warm) { set_err("rphys_batch_restore: warm cache larger than the scene's capacity"); return RPHYS_ERR_INTERNAL; }
+    size_t n = d.nb;
+    auto cp = [&](void *dst, const void *src, size_t bytes) { return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.st), "restore copy"); };
+    // Snapshot poses are already sanitised (set_pose's normalisation maps them to themselves for
+    // every pose a step produces: |q|^2 rounds to 1 within 1 ulp); the batch copies them back.
+    bool ok = cp(d.pos, sl.pos, 16 * n) && cp(d.rot, sl.rot, 16 * n) && cp(d.lin, sl.lin, 16 * n) && cp(d.ang, sl.ang, 16 * n) &&
+              cp(d.last_lin, sl.last, 16 * n) && cp(d.awake, sl.awake, 4 * n) && cp(d.sticks, sl.sticks, 4 * n) &&
+              cp(d.wk_key[sl.cur], sl.wkey, 8 * (size_t)sl.nwarm) && cp(d.wk_la[sl.cur], sl.wla, 16 * (size_t)sl.nwarm) &&
+              cp(d.wk_jn[sl.cur], sl.wjn, 4 * (size_t)sl.nwarm) && cp(d.wk_jt[sl.cur], sl.wjt, 16 * (size_t)sl.nwarm) &&
+              cp(d.wc_off[sl.cur], sl.woff, 4 * ((size_t)d.nw + 1));
+    if (!ok || !hip_ok(hipStreamSynchronize(S.st), "restore")) return RPHYS_ERR_INTERNAL;
+    S.cur = sl.cur;
+    for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];
+    return RPHYS_OK;
+    GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,1314 @@
+// The per-tick step of crates/riftphys-world/src/world/step.rs:40-420 as a chain of gfx950
+// kernels over a batch of worlds.  Stage map (reference line -> kernel):
+//   S1 forces step.rs:53-68                      k_bodies_pre
+//   S2 AABB / sweep step.rs:71-81                k_col_aabb
+//   broadphase broadphase.rs:29-99               k_bp_sort, k_bp_enum (count, write)
+//   narrowphase step.rs:95-119                   k_narrow_pairs
+//   terrain step.rs:121-142, terrain.rs:300-359  k_narrow_terrain, k_terrain_cull
+//   cull/orient step.rs:499-542                  k_terrain_cull, k_contact_count/write
+//   solver prep solver.rs:494-590                k_prep
+//   warm start solver.rs:717-762                 k_warm_match + first sweep of k_sweep_vel
+//   PGS solver.rs:788-880 + shock :964-993       k_sweep_vel (one workgroup per island)
+//   warm store solver.rs:917-944                 k_warm_newflags / k_warm_keepflags / k_warm_merge
+//   loaded + impulses solver.rs:1247-1270        k_loaded
+//   S4 ballistic + integrate step.rs:154-280     k_sweep_ball, k_integrate
+//   S5 projection step.rs:284-310, 423-497       k_posc_prep, k_sweep_proj
+//   S6/S7 step.rs:312-384, sleep.rs:59-80        k_s6
+// Ordered (Gauss-Seidel) stages run per island: constraints that write a common body keep the
+// reference's order through per-body predecessor chains; constraints of different islands share
+// no written body, so the result equals the reference's serial sweep bit for bit.
+#include "rp_dev.h"
+#include "rp_math.h"
+#include "rp_narrow.h"
+#include "rp_prims.h"
+
+static __device__ __forceinline__ V3 L3(float4 v) { return mk3(v.x, v.y, v.z); }
+static __device__ __forceinline__ Q4 LQ(float4 v) { return mkq(v.x, v.y, v.z, v.w); }
+static __device__ __forceinline__ float4 S3(V3 v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
+static __device__ __forceinline__ float4 SQ(Q4 q) { return make_float4(q.x, q.y, q.z, q.w); }
+static __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+static __device__ __forceinline__ M3 load_m9(const float *p) {
+    return mkm(mk3(p[0], p[1], p[2]), mk3(p[3], p[4], p[5]), mk3(p[6], p[7], p[8]));
+}
+static __device__ __forceinline__ void store_m9(float *p, M3 m) {
+    p[0] = m.c0.x; p[1] = m.c0.y; p[2] = m.c0.z; p[3] = m.c1.x; p[4] = m.c1.y; p[5] = m.c1.z;
+    p[6] = m.c2.x; p[7] = m.c2.y; p[8] = m.c2.z;
+}
+#define GRID_STRIDE(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+static __device__ __forceinline__ bool aborted(const Dev &d) { return d.cnt[CT_ABORT] != 0; }
+
+// dynamics.rs:158-165 inv_inertia_world = R I^-1 R^T (zero when inv_mass == 0)
+static __device__ __forceinline__ M3 inv_inertia_world(const Dev &d, uint32_t b, Q4 q) {
+    if (d.inv_mass[b] == 0.0f) return mzero();
+    M3 r = mquat(q);
+    return mmul(mmul(r, load_m9(d.iil + 9 * b)), mtrans(r));
+}
+// dynamics.rs:130-147 set_pose / set_vel sanitisation
+static __device__ __forceinline__ void set_pose(const Dev &d, uint32_t b, V3 p, Q4 q) {
+    if (fin3(p)) d.pos[b] = S3(fin_v(p));
+    if (finq(q)) d.rot[b] = SQ(fin_q(q));
+}
+static __device__ __forceinline__ void set_vel(const Dev &d, uint32_t b, V3 l, V3 a) {
+    if (fin3(l)) d.lin[b] = S3(fin_v(l));
+    if (fin3(a)) d.ang[b] = S3(fin_v(a));
+}
+
+// ------------------------------------------------------------------ S1
+__global__ void k_bodies_pre(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    GRID_STRIDE(b, d.nb) {
+        float4 p = d.pos[b], q = d.rot[b], l = d.lin[b], a = d.ang[b];
+        d.bk_pos[b] = p; d.bk_rot[b] = q; d.bk_lin[b] = l; d.bk_ang[b] = a;
+        d.bk_last[b] = d.last_lin[b]; d.bk_awake[b] = d.awake[b]; d.bk_sticks[b] = d.sticks[b];
+        d.pre_lin[b] = l; d.pre_ang[b] = a;
+        d.flags[b] = 0; d.root[b] = b; d.ballf[b] = 0;
+        d.slin[b] = make_float4(0, 0, 0, 0); d.sang[b] = make_float4(0, 0, 0, 0);
+        bool dyn = d.bflags[b] & BF_DYN;
+        if (dyn && d.awake[b]) {
+            V3 g = fin3(L3(p)) ? mk3(tp.gx, tp.gy, tp.gz) : splat3(0.0f);
+            set_vel(d, b, L3(l) + g * tp.dt, L3(a));
+        }
+        store_m9(d.iiw + 9 * b, inv_inertia_world(d, b, LQ(q)));
+    }
+}
+
+// ------------------------------------------------------------------ S2 AABBs
+// shape.rs:18-76 aabb_of + aabb.rs:21-31 (qf32 then min/max)
+static __device__ void aabb_of(uint32_t type, float4 dims, V3 p, Q4 q, V3 &mn, V3 &mx) {
+    V3 he;
+    if (type == 0) he = splat3(fabsf(dims.x));
+    else if (type == 1) {
+        V3 e = mk3(fabsf(dims.x), fabsf(dims.y), fabsf(dims.z));
+        M3 r = mquat(q);
+        he = mv(mkm(v3abs(r.c0), v3abs(r.c1), v3abs(r.c2)), e);
+    } else {
+        V3 axw = qrot(q, mk3(0, 1, 0)) * fabsf(dims.y);
+        he = v3abs(axw) + splat3(fabsf(dims.x));
+    }
+    V3 a = p - he, b = p + he;
+    V3 qa = mk3(qf32(a.x), qf32(a.y), qf32(a.z)), qb = mk3(qf32(b.x), qf32(b.y), qf32(b.z));
+    mn = v3min(qa, qb); mx = v3max(qa, qb);
+}
+
+// terrain.rs:300-319 cell range of a collider's (unswept) AABB; false if culled
+static __device__ bool hf_range(const HfDesc &h, V3 mn, V3 mx, int &gx0, int &gx1, int &gz0, int &gz1) {
+    if (!h.valid) return false;
+    int nx = (int)h.nx, nz = (int)h.nz;
+    if (nx < 2 || nz < 2) return false;
+    float lminx = mn.x - h.ox, lmaxx = mx.x - h.ox, lminz = mn.z - h.oz, lmaxz = mx.z - h.oz;
+    // `as i32` saturates; NaN becomes 0
+    auto sat = [](float f) -> int { if (f != f) return 0; if (f >= 2147483647.0f) return 2147483647; if (f <= -2147483648.0f) return (-2147483647 - 1); return (int)f; };
+    long long a0 = (long long)sat(floorf(lminx / h.sx)) - 1, a1 = (long long)sat(ceilf(lmaxx / h.sx)) + 1;
+    long long b0 = (long long)sat(floorf(lminz / h.sz)) - 1, b1 = (long long)sat(ceilf(lmaxz / h.sz)) + 1;
+    auto cl = [](long long v, long long lo, long long hi) -> int { return (int)(v < lo ? lo : (v > hi ? hi : v)); };
+    gx0 = cl(a0, 0, nx - 2); gx1 = cl(a1, 0, nx - 2); gz0 = cl(b0, 0, nz - 2); gz1 = cl(b1, 0, nz - 2);
+    if (mn.y > h.hmax + 1.0f || mx.y < h.hmin - 1.0f) return false;
+    return gx1 >= gx0 && gz1 >= gz0;
+}
+
+__global__ void k_col_aabb(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        uint32_t b = d.col_body[c];
+        V3 p = L3(d.pos[b]);
+        Q4 q = LQ(d.rot[b]);
+        V3 mn, mx;
+        aabb_of(d.btype[b], d.bdims[b], p, q, mn, mx);
+        d.amin[c] = S3(mn); d.amax[c] = S3(mx);
+        float reach = 0.5f * len3(mx - mn);                              // world.rs:97-100
+        float s = 0.05f + (len3(L3(d.lin[b])) + len3(L3(d.ang[b])) * reach) * tp.dt;  // step.rs:78
+        d.sweep[c] = s;
+        V3 smn = mn - splat3(s), smx = mx + splat3(s);
+        d.smin[c] = S3(smn); d.smax[c] = S3(smx);
+        uint32_t cls;
+        if (!isfinite(smn.x)) cls = CLS_INVALID;
+        else {
+            float dx = smx.x - smn.x, dy = smx.y - smn.y, dz = smx.z - smn.z;
+            const float lim = 2.5f * 5.0f;  // broadphase.rs:44 with cell 2.5 (world.rs:419)
+            cls = (dx > lim || dy > lim || dz > lim) ? CLS_OVER : CLS_SAP;
+        }
+        d.ccls[c] = cls;
+        int gx0, gx1, gz0, gz1;
+        uint32_t cand = 0;
+        if (hf_range(d.hf[d.col_world[c]], mn, mx, gx0, gx1, gz0, gz1))
+            cand = 2u * (uint32_t)(gx1 - gx0 + 1) * (uint32_t)(gz1 - gz0 + 1);
+        d.cand_off[c] = cand;
+    }
+}
+
+// ------------------------------------------------------------------ broadphase
+static __device__ __forceinline__ uint32_t sortable(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// One workgroup per world: bitonic sort of (key, local index) in LDS; SAP colliders by min.x,
+// then oversized, then invalid colliders, each group in index order.
+__global__ void __launch_bounds__(1024) k_bp_sort(Dev d) {
+    if (aborted(d)) return;
+    __shared__ uint64_t e[RP_MAX_WORLD_COLLIDERS];
+    __shared__ float red[1024];
+    __shared__ uint32_t cnt_sap, cnt_over;
+    const uint32_t w = blockIdx.x, t = threadIdx.x;
+    const uint32_t c0 = d.w_col_off[w], n = d.w_col_off[w + 1] - c0;
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    if (t == 0) { cnt_sap = 0; cnt_over = 0; }
+    __syncthreads();
+    float mext = 0.0f;
+    for (uint32_t i = t; i < m; i += blockDim.x) {
+        uint64_t key = ~0ull;
+        if (i < n) {
+            uint32_t c = c0 + i, cls = d.ccls[c];
+            uint32_t k;
+            if (cls == CLS_SAP) {
+                k = sortable(d.smin[c].x);
+                if (k == 0xFFFFFFFFu || k == 0xFFFFFFFEu) k = 0xFFFFFFFDu;
+                atomicAdd(&cnt_sap, 1u);
+                mext = fmaxf(mext, d.smax[c].x - d.smin[c].x);
+            } else if (cls == CLS_OVER) { k = 0xFFFFFFFEu; atomicAdd(&cnt_over, 1u); }
+            else k = 0xFFFFFFFFu;
+            key = ((uint64_t)k << 32) | i;
+        }
+        e[i] = key;
+    }
+    red[t] = mext;
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < m; i += blockDim.x) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t a = e[i], b = e[ixj];
+                    bool asc = (i & k) == 0;
+                    if ((a > b) == asc) { e[i] = b; e[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t s = 512; s > 0; s >>= 1) {
+        if (t < s) red[t] = fmaxf(red[t], red[t + s]);
+        __syncthreads();
+    }
+    for (uint32_t i = t; i < n; i += blockDim.x) {
+        uint32_t li = (uint32_t)e[i];
+        d.bp_ord[c0 + i] = li;
+        d.bp_rank[c0 + li] = i;
+    }
+    if (t == 0) {
+        d.w_nsap[w] = cnt_sap;
+        d.w_nover[w] = cnt_over;
+        // inflated so the backward scan never stops early on rounding
+        d.w_maxext[w] = red[0] * 1.0001f + 1.0e-4f;
+    }
+}
+
+// aabb.rs:35-40 inclusive overlap (symmetric)
+static __device__ __forceinline__ bool overlaps(float4 amn, float4 amx, float4 bmn, float4 bmx) {
+    return !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z);
+}
+
+// Enumerates the partners j > i of collider c (global index) that broadphase.rs:29-99 pairs it
+// with; emits world-global collider indices.  mode 0 counts, mode 1 writes (then sorts) the list.
+template <int MODE>
+__global__ void k_bp_enum(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        const uint32_t w = d.col_world[c], c0 = d.w_col_off[w], n = d.w_col_off[w + 1] - c0, i = c - c0;
+        const uint32_t nsap = d.w_nsap[w], nover = d.w_nover[w];
+        const float4 mn = d.smin[c], mx = d.smax[c];
+        const uint32_t cls = d.ccls[c];
+        uint32_t count = 0;
+        uint32_t base = 0, cap = d.cap_pairs;
+        if (MODE == 1) base = d.out_off[c];
+        auto emit = [&](uint32_t j) {
+            if (MODE == 1) {
+                uint32_t o = base + count;
+                if (o < cap) { d.pair_i[o] = c; d.pair_j[o] = c0 + j; }
+            }
+            count++;
+        };
+        if (cls == CLS_OVER) {
+            for (uint32_t j = i + 1; j < n; j++)
+                if (overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(j);
+        } else {
+            if (cls == CLS_SAP) {
+                const uint32_t r = d.bp_rank[c];
+                for (uint32_t q = r + 1; q < nsap; q++) {
+                    uint32_t j = d.bp_ord[c0 + q];
+                    float4 jmn = d.smin[c0 + j];
+                    if (jmn.x > mx.x) break;
+                    if (j > i && overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(j);
+                }
+                const float ext = d.w_maxext[w];
+                for (int q = (int)r - 1; q >= 0; q--) {
+                    uint32_t j = d.bp_ord[c0 + q];
+                    float4 jmn = d.smin[c0 + j];
+                    if (jmn.x + ext < mn.x) break;
+                    if (j > i && overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(j);
+                }
+            }
+            for (uint32_t q = nsap; q < nsap + nover; q++) {
+                uint32_t j = d.bp_ord[c0 + q];
+                if (j > i && overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(j);
+            }
+        }
+        if (MODE == 0) d.out_off[c] = count;
+        else if (base + count <= cap) {
+            // insertion sort of this collider's partner list (ascending j)
+            for (uint32_t a = base + 1; a < base + count; a++) {
+                uint32_t v = d.pair_j[a];
+                uint32_t b = a;
+                while (b > base && d.pair_j[b - 1] > v) { d.pair_j[b] = d.pair_j[b - 1]; b--; }
+                d.pair_j[b] = v;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ narrowphase
+static __device__ __forceinline__ Shape shape_of(const Dev &d, uint32_t b) {
+    float4 dm = d.bdims[b];
+    Shape s; s.type = (int)d.btype[b]; s.d0 = dm.x; s.d1 = dm.y; s.d2 = dm.z;
+    return s;
+}
+
+__global__ void k_narrow_pairs(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t np = min(d.cnt[CT_PAIRS], d.cap_pairs);
+    GRID_STRIDE(p, np) {
+        uint32_t i = d.pair_i[p], j = d.pair_j[p];
+        uint32_t ba = d.col_body[i], bb = d.col_body[j];
+        uint32_t cnt = 0;
+        if (ba != bb) {
+            float reach = (d.sweep[i] + d.sweep[j]) - 0.05f;  // step.rs:105
+            Manifold m;
+            if (collide_pair(shape_of(d, ba), L3(d.pos[ba]), LQ(d.rot[ba]), shape_of(d, bb), L3(d.pos[bb]), LQ(d.rot[bb]), reach, m)) {
+                for (int k = 0; k < m.len; k++) {
+                    float depth = (m.p[k].pen - reach) + 0.05f;  // step.rs:109
+                    d.pc_p[4 * p + k] = S3(m.p[k].pos, depth);
+                    d.pc_n[4 * p + k] = S3(m.p[k].n, __uint_as_float(m.p[k].feat));
+                }
+                cnt = (uint32_t)m.len;
+            }
+        }
+        d.pair_n[p] = cnt;
+    }
+}
+
+static __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (a[mid] <= v) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+
+// terrain.rs:321-356 triangle t of cell (gx, gz): tri1 = (p00, p01, p10), tri2 = (p10, p01, p11)
+static __device__ void hf_tri(const Dev &d, const HfDesc &h, int gx, int gz, int tri, V3 &a, V3 &b, V3 &c) {
+    const float *H = d.heights + h.off;
+    int nx = (int)h.nx;
+    float h00 = H[gx + gz * nx], h10 = H[gx + 1 + gz * nx], h01 = H[gx + (gz + 1) * nx], h11 = H[gx + 1 + (gz + 1) * nx];
+    float x0 = (float)gx * h.sx + h.ox, x1 = x0 + h.sx;
+    float z0 = (float)gz * h.sz + h.oz, z1 = z0 + h.sz;
+    V3 p00 = mk3(x0, h00, z0), p10 = mk3(x1, h10, z0), p01 = mk3(x0, h01, z1), p11 = mk3(x1, h11, z1);
+    if (tri == 0) { a = p00; b = p01; c = p10; } else { a = p10; b = p01; c = p11; }
+}
+
+__global__ void k_narrow_terrain(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nt = min(d.cnt[CT_CANDS], d.cap_cands);
+    GRID_STRIDE(t, nt) {
+        uint32_t c = upper_bound_u32(d.cand_off, d.nc, t) - 1;
+        uint32_t local = t - d.cand_off[c];
+        uint32_t b = d.col_body[c];
+        const HfDesc h = d.hf[d.col_world[c]];
+        int gx0, gx1, gz0, gz1;
+        hf_range(h, L3(d.amin[c]), L3(d.amax[c]), gx0, gx1, gz0, gz1);
+        uint32_t width = (uint32_t)(gx1 - gx0 + 1);
+        uint32_t cell = local >> 1, tri = local & 1u;
+        int gz = gz0 + (int)(cell / width), gx = gx0 + (int)(cell % width);
+        V3 ta, tb, tc;
+        hf_tri(d, h, gx, gz, (int)tri, ta, tb, tc);
+        float s = d.sweep[c];
+        Manifold m;
+        uint32_t cnt = 0;
+        if (collide_terrain(shape_of(d, b), L3(d.pos[b]), LQ(d.rot[b]), ta, tb, tc, s, m)) {
+            for (int k = 0; k < m.len; k++) {
+                float depth = (m.p[k].pen - s) + 0.05f;  // step.rs:129
+                d.tc_p[4 * t + k] = S3(m.p[k].pos, depth);
+                d.tc_nf[4 * t + k] = S3(m.p[k].n, __uint_as_float(m.p[k].feat));
+            }
+            cnt = (uint32_t)m.len;
+        }
+        d.tc_n[t] = cnt;
+        d.tc_cell[t] = ((uint32_t)gz * (h.nx - 1u) + (uint32_t)gx) * 2u + tri;
+    }
+}
+
+// step.rs:499-531 for a collider's terrain bucket: <= 4 kept; a box keeps reduce_to_4 of its
+// (depth, point) list with the deepest point's normal; other shapes keep the 4 deepest
+// (sort_key6 descending, ties in narrowphase order).  Streams over the candidate slots.
+__global__ void k_terrain_cull(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        uint32_t t0 = d.cand_off[c], t1 = d.cand_off[c + 1];
+        if (t1 > d.cap_cands) t1 = d.cap_cands;
+        if (t0 > t1) t0 = t1;
+        uint32_t n = 0;
+        for (uint32_t t = t0; t < t1; t++) n += d.tc_n[t];
+        uint32_t out = 0;
+        auto put = [&](uint32_t t, uint32_t k) {
+            d.tcul_p[4 * c + out] = d.tc_p[4 * t + k];
+            d.tcul_n[4 * c + out] = d.tc_nf[4 * t + k];
+            d.tcul_cell[4 * c + out] = d.tc_cell[t];
+            out++;
+        };
+        if (n <= 4) {
+            for (uint32_t t = t0; t < t1; t++)
+                for (uint32_t k = 0; k < d.tc_n[t]; k++) put(t, k);
+        } else if (d.btype[d.col_body[c]] == 1) {
+            // reduce_to_4 (narrowphase.rs:312-338) streamed; items in (candidate, point) order
+            uint32_t bt0 = 0, bk0 = 0; int64_t best = INT64_MIN;
+            for (uint32_t t = t0; t < t1; t++)
+                for (uint32_t k = 0; k < d.tc_n[t]; k++) {
+                    int64_t key = sort_key6(d.tc_p[4 * t + k].w);
+                    if (key > best) { best = key; bt0 = t; bk0 = k; }
+                }
+            V3 nrm = L3(d.tc_nf[4 * bt0 + bk0]);
+            V3 p0 = L3(d.tc_p[4 * bt0 + bk0]);
+            put(bt0, bk0);
+            uint32_t bt1 = 0, bk1 = 0; float d1 = -INFINITY; best = INT64_MIN;
+            for (uint32_t t = t0; t < t1; t++)
+                for (uint32_t k = 0; k < d.tc_n[t]; k++) {
+                    float v = len2(L3(d.tc_p[4 * t + k]) - p0);
+                    int64_t key = sort_key6(v);
+                    if (key > best) { best = key; bt1 = t; bk1 = k; d1 = v; }
+                }
+            if (d1 > 1.0e-12f) {
+                put(bt1, bk1);
+                V3 e = L3(d.tc_p[4 * bt1 + bk1]) - p0;
+                uint32_t bt2 = 0, bk2 = 0; float a2 = -INFINITY; best = INT64_MIN;
+                for (uint32_t t = t0; t < t1; t++)
+                    for (uint32_t k = 0; k < d.tc_n[t]; k++) {
+                        float v = fabsf(dot3(cross3(L3(d.tc_p[4 * t + k]) - p0, e), nrm));
+                        int64_t key = sort_key6(v);
+                        if (key > best) { best = key; bt2 = t; bk2 = k; a2 = v; }
+                    }
+                if (a2 > 1.0e-12f) {
+                    put(bt2, bk2);
+                    float side2 = dot3(cross3(L3(d.tc_p[4 * bt2 + bk2]) - p0, e), nrm);
+                    float s2 = (side2 != side2) ? side2 : (signbit(side2) ? -1.0f : 1.0f);
+                    uint32_t bt3 = 0, bk3 = 0; float a3 = -INFINITY; best = INT64_MIN;
+                    for (uint32_t t = t0; t < t1; t++)
+                        for (uint32_t k = 0; k < d.tc_n[t]; k++) {
+                            float v = -dot3(cross3(L3(d.tc_p[4 * t + k]) - p0, e), nrm) * s2;
+                            int64_t key = sort_key6(v);
+                            if (key > best) { best = key; bt3 = t; bk3 = k; a3 = v; }
+                        }
+                    if (a3 > 1.0e-12f) put(bt3, bk3);
+                }
+            }
+        } else {
+            // top 4 by (sort_key6(depth) desc, order asc)
+            int64_t kk[4]; uint32_t tt[4], ks[4]; int m = 0;
+            for (uint32_t t = t0; t < t1; t++)
+                for (uint32_t k = 0; k < d.tc_n[t]; k++) {
+                    int64_t key = sort_key6(d.tc_p[4 * t + k].w);
+                    int pos = m;
+                    while (pos > 0 && kk[pos - 1] < key) pos--;
+                    if (pos >= 4) continue;
+                    int last = m < 4 ? m : 3;
+                    for (int z = last; z > pos; z--) { kk[z] = kk[z - 1]; tt[z] = tt[z - 1]; ks[z] = ks[z - 1]; }
+                    kk[pos] = key; tt[pos] = t; ks[pos] = k;
+                    if (m < 4) m++;
+                }
+            for (int z = 0; z < m; z++) put(tt[z], ks[z]);
+        }
+        d.tcul_cnt[c] = out;
+    }
+}
+
+__global__ void k_contact_count(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        uint32_t p0 = d.out_off[c], p1 = d.out_off[c + 1];
+        uint32_t s = 0;
+        for (uint32_t p = p0; p < p1 && p < d.cap_pairs; p++) s += d.pair_n[p];
+        d.c_off[c] = s + d.tcul_cnt[c];
+    }
+}
+
+// step.rs:532-540 orientation: body-pair normals flipped so n.(pb - pa) >= 0
+__global__ void k_contact_write(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        uint32_t o = d.c_off[c];
+        uint32_t ba = d.col_body[c];
+        V3 pa = L3(d.pos[ba]);
+        for (uint32_t p = d.out_off[c]; p < d.out_off[c + 1] && p < d.cap_pairs; p++) {
+            uint32_t j = d.pair_j[p];
+            V3 pb = L3(d.pos[d.col_body[j]]);
+            for (uint32_t k = 0; k < d.pair_n[p]; k++, o++) {
+                if (o >= d.cap_contacts) continue;
+                float4 nf = d.pc_n[4 * p + k];
+                V3 n = L3(nf);
+                if (dot3(n, pb - pa) < 0.0f) n = -n;
+                d.ct_a[o] = c; d.ct_b[o] = j; d.ct_cell[o] = RP_NONE;
+                d.ct_p[o] = d.pc_p[4 * p + k];
+                d.ct_n[o] = S3(n, nf.w);
+            }
+        }
+        for (uint32_t k = 0; k < d.tcul_cnt[c]; k++, o++) {
+            if (o >= d.cap_contacts) continue;
+            d.ct_a[o] = c; d.ct_b[o] = RP_TERRAIN; d.ct_cell[o] = d.tcul_cell[4 * c + k];
+            d.ct_p[o] = d.tcul_p[4 * c + k];
+            d.ct_n[o] = d.tcul_n[4 * c + k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ solver prep
+static __device__ __forceinline__ uint32_t uf_find(uint32_t *root, uint32_t x) {
+    uint32_t p = __hip_atomic_load(&root[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&root[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+static __device__ void uf_union(uint32_t *root, uint32_t a, uint32_t b) {
+    while (true) {
+        uint32_t ra = uf_find(root, a), rb = uf_find(root, b);
+        if (ra == rb) return;
+        uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+        uint32_t expected = hi;
+        if (__hip_atomic_compare_exchange_strong(&root[hi], &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    }
+}
+
+static __device__ __forceinline__ float eff_mass(float inv_a, float inv_b, M3 iia, M3 iib, V3 ra, V3 rb, V3 dir) {
+    V3 ca = cross3(ra, dir), cb = cross3(rb, dir);
+    float k = ((inv_a + inv_b) + dot3(ca, mv(iia, ca))) + dot3(cb, mv(iib, cb));
+    return k > 1.0e-9f ? 1.0f / k : 0.0f;
+}
+// world/src/world/geom.rs:30-38
+static __device__ __forceinline__ void orthonormal_basis(V3 n, V3 &t1, V3 &t2) {
+    float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    V3 base = (ax <= ay && ax <= az) ? mk3(1, 0, 0) : (ay <= az ? mk3(0, 1, 0) : mk3(0, 0, 1));
+    t1 = normalize_or_zero3(cross3(base, n));
+    t2 = cross3(n, t1);
+}
+
+// solver.rs:494-590 constraint rows; islands.rs:58-66 unions; step.rs:160-163 constrained
+// flags; step.rs:337-343 grounding flags; warm keys and anchors (solver.rs:720-731)
+__global__ void k_prep(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(c, nct) {
+        uint32_t ca = d.ct_a[c], cb = d.ct_b[c];
+        bool terr = cb == RP_TERRAIN;
+        uint32_t ai = d.col_body[ca], bi = terr ? RP_NONE : d.col_body[cb];
+        uint32_t w = d.col_world[ca];
+        uint32_t boff = d.w_body_off[w];
+        float4 pn = d.ct_p[c];
+        V3 point = L3(pn);
+        float depth = pn.w;
+        V3 n = L3(d.ct_n[c]);
+        bool aw = d.awake[ai] != 0, bw = !terr && d.awake[bi] != 0;
+        float inv_a = aw ? d.inv_mass[ai] : 0.0f, inv_b = bw ? d.inv_mass[bi] : 0.0f;
+        M3 iia = inv_a > 0.0f ? load_m9(d.iiw + 9 * ai) : mzero();
+        M3 iib = inv_b > 0.0f ? load_m9(d.iiw + 9 * bi) : mzero();
+        uint32_t ma = d.col_mat[ca], mb = terr ? d.hf[w].mat : d.col_mat[cb];
+        float4 pp = d.pair_tab[(ma & 15u) * 16u + (mb & 15u)];
+        V3 t1, t2;
+        orthonormal_basis(n, t1, t2);
+        V3 pa = L3(d.pos[ai]);
+        V3 ra = point - pa, rb = terr ? splat3(0.0f) : point - L3(d.pos[bi]);
+        float mn = eff_mass(inv_a, inv_b, iia, iib, ra, rb, n);
+        float m1 = eff_mass(inv_a, inv_b, iia, iib, ra, rb, t1);
+        float m2 = eff_mass(inv_a, inv_b, iia, iib, ra, rb, t2);
+        float pen = depth - 0.05f;
+        float bias = pen > 0.0f ? fminf(fmaxf(pen - 0.005f, 0.0f) * (0.05f / tp.dt), 2.0f) : pen / tp.dt;
+        V3 pva = L3(d.pre_lin[ai]) + cross3(L3(d.pre_ang[ai]), ra);
+        V3 pvb = terr ? splat3(0.0f) : L3(d.pre_lin[bi]) + cross3(L3(d.pre_ang[bi]), rb);
+        float v_pre = dot3(pvb - pva, n);
+        V3 cva = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
+        V3 cvb = terr ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
+        float approach = 0.5f * (v_pre + dot3(cvb - cva, n));
+        d.cs_ai[c] = ai; d.cs_bi[c] = bi;
+        d.cs_inv[c] = make_float2(inv_a, inv_b);
+        d.cs_ra[c] = S3(ra); d.cs_rb[c] = S3(rb);
+        d.cs_n[c] = S3(n); d.cs_t1[c] = S3(t1); d.cs_t2[c] = S3(t2);
+        d.cs_m[c] = make_float4(mn, m1, m2, bias);
+        d.cs_mu[c] = make_float4(pp.x, pp.y, fmaxf(pp.w, 1.0e-4f), approach);
+        uint32_t wa = inv_a > 0.0f ? ai : RP_NONE;
+        uint32_t wb = (!terr && inv_b > 0.0f) ? bi : RP_NONE;
+        d.cs_wa[c] = wa; d.cs_wb[c] = wb;
+        d.jn[c] = 0.0f; d.jn_pre[c] = 0.0f; d.jt[c] = make_float2(0.0f, 0.0f); d.ct_imp[c] = 0.0f;
+        d.warm_idx[c] = -1;
+        d.ct_key[c] = ((uint64_t)(ai - boff) << 32) | (terr ? 0xFFFFFFFFull : (uint64_t)(bi - boff));
+        d.ct_anchor[c] = S3(qrot(qconj(LQ(d.rot[ai])), point - pa));
+        atomicOr(&d.flags[ai], FL_CONSTRAINED);
+        if (!terr) atomicOr(&d.flags[bi], FL_CONSTRAINED);
+        if (terr && depth - 0.05f > -0.005f) atomicOr(&d.flags[ai], FL_GROUND);
+        if (d.inv_mass[ai] > 0.0f || (!terr && d.inv_mass[bi] > 0.0f)) atomicOr(&d.w_solve[w], 1u);
+        if (wa != RP_NONE && wb != RP_NONE) uf_union(d.root, wa, wb);
+    }
+}
+
+// Island key of each contact: the component root (lowest body index) of its written bodies.
+__global__ void k_island_keys(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(c, nct) {
+        uint32_t wa = d.cs_wa[c], wb = d.cs_wb[c];
+        uint32_t x = wa != RP_NONE ? wa : wb;
+        d.isl_k[c] = x == RP_NONE ? RP_NONE : uf_find(d.root, x);
+        d.isl_v[c] = c;
+    }
+}
+
+// Island segment starts over the sorted (key, contact) list; counts contacts with a writer.
+__global__ void k_island_flags(Dev d, const uint32_t *keys) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(r, d.cap_contacts) {
+        if (r >= nct) { d.isl_flag[r] = 0; continue; }
+        uint32_t k = keys[r];
+        d.isl_flag[r] = (k != RP_NONE && (r == 0 || keys[r - 1] != k)) ? 1u : 0u;
+        if (k != RP_NONE && (r + 1 == nct || keys[r + 1] == RP_NONE)) d.cnt[CT_ISLC] = r + 1;
+    }
+}
+__global__ void k_island_compact(Dev d, const uint32_t *keys) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(r, nct) {
+        uint32_t k = keys[r];
+        if (k != RP_NONE && (r == 0 || keys[r - 1] != k)) d.isl_start[d.isl_flag[r]] = r;
+    }
+}
+
+// solver.rs:733-762 matching: per key group (contiguous, <= 4 contacts), the nearest unused
+// entry within 1 cm, lowest index on ties.  Contacts writing no body get their friction
+// accumulator here (their bodies' velocities are final); the others in the first sweep.
+__global__ void k_warm_match(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    const uint32_t cur = tp.cur;
+    GRID_STRIDE(c, nct) {
+        uint64_t key = d.ct_key[c];
+        uint32_t w = d.col_world[d.ct_a[c]];
+        if (!d.w_solve[w]) continue;
+        if (c > 0 && d.ct_key[c - 1] == key && d.col_world[d.ct_a[c - 1]] == w) continue;
+        uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
+        uint32_t lo = e0, hi = e1;
+        while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (d.wk_key[cur][mid] < key) lo = mid + 1; else hi = mid; }
+        uint32_t es = lo, ee = lo;
+        while (ee < e1 && d.wk_key[cur][ee] == key) ee++;
+        if (es == ee) continue;
+        uint32_t taken = 0;  // entries per key stay well under 32
+        for (uint32_t g = c; g < nct && d.ct_key[g] == key && d.col_world[d.ct_a[g]] == w; g++) {
+            V3 an = L3(d.ct_anchor[g]);
+            int best = -1; float bd = 0.0f;
+            for (uint32_t e = es; e < ee && e - es < 32; e++) {
+                if (taken & (1u << (e - es))) continue;
+                float dd = len2(L3(d.wk_la[cur][e]) - an);
+                if (dd <= 0.01f * 0.01f && (best < 0 || dd < bd)) { best = (int)e; bd = dd; }
+            }
+            if (best < 0) continue;
+            taken |= 1u << (best - es);
+            d.warm_idx[g] = best;
+            float ejn = d.wk_jn[cur][best];
+            d.jn[g] = ejn;
+            d.jn_pre[g] = ejn;
+            if (d.cs_wa[g] == RP_NONE && d.cs_wb[g] == RP_NONE) {
+                // slip speed from constant velocities; clamp as the iterations do
+                uint32_t ai = d.cs_ai[g], bi = d.cs_bi[g];
+                V3 ra = L3(d.cs_ra[g]), rb = L3(d.cs_rb[g]);
+                V3 va = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
+                V3 vb = bi == RP_NONE ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
+                V3 v = vb - va;
+                V3 t1 = L3(d.cs_t1[g]), t2 = L3(d.cs_t2[g]);
+                float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
+                float4 mu4 = d.cs_mu[g];
+                float x = sqrtf(vt1 * vt1 + vt2 * vt2) / mu4.z;
+                float mu = mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
+                V3 ejt = L3(d.wk_jt[cur][best]);
+                V3 t = clamp_len_max(mk3(dot3(ejt, t1), dot3(ejt, t2), 0.0f), mu * ejn);
+                d.jt[g] = make_float2(t.x, t.y);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ ordered sweeps
+// Chain storage of the island being processed: LDS for islands up to RP_ISLAND_LDS_MAX
+// contacts, else a global slice owned by the island (its list positions are its own).
+struct Chains { uint64_t *ent; uint32_t *order, *pa, *pb, *done; };
+struct IslandLds {
+    uint64_t ent[2 * RP_ISLAND_LDS_MAX];
+    uint32_t order[RP_ISLAND_LDS_MAX], pa[RP_ISLAND_LDS_MAX], pb[RP_ISLAND_LDS_MAX], done[RP_ISLAND_LDS_MAX];
+};
+static __device__ __forceinline__ Chains chains_for(const Dev &d, IslandLds &L, uint32_t s, uint32_t n) {
+    Chains c;
+    if (n <= RP_ISLAND_LDS_MAX) { c.ent = L.ent; c.order = L.order; c.pa = L.pa; c.pb = L.pb; c.done = L.done; }
+    else { c.ent = d.g_ent + 4ull * s; c.order = d.g_order + s; c.pa = d.g_pa + s; c.pb = d.g_pb + s; c.done = d.g_done + s; }
+    return c;
+}
+
+static __device__ void bitonic_u64(uint64_t *a, uint32_t m) {
+    for (uint32_t k = 2; k <= m; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t x = a[i], y = a[ixj];
+                    bool asc = (i & k) == 0;
+                    if ((x > y) == asc) { a[i] = y; a[ixj] = x; }
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+}
+
+// Predecessor chains for the sweep order C.order (positions 0..n-1 of the island list): for
+// each position and each written body, the previous position in the order writing the same body
+// (stable sort of (body, position, side) entries).
+static __device__ void build_chains(const Dev &d, const Chains &C, const uint32_t *list, uint32_t n) {
+    uint32_t m = 1;
+    while (m < 2 * n) m <<= 1;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        uint64_t v = ~0ull;
+        if (i < 2 * n) {
+            uint32_t r = i >> 1, side = i & 1;
+            uint32_t c = list[C.order[r]];
+            uint32_t body = side ? d.cs_wb[c] : d.cs_wa[c];
+            if (body != RP_NONE) v = ((uint64_t)body << 32) | ((uint64_t)r << 1) | side;
+        }
+        C.ent[i] = v;
+    }
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) { C.pa[r] = RP_NONE; C.pb[r] = RP_NONE; C.done[r] = 0; }
+    __threadfence_block();
+    __syncthreads();
+    bitonic_u64(C.ent, m);
+    for (uint32_t i = threadIdx.x; i < 2 * n; i += blockDim.x) {
+        uint64_t v = C.ent[i];
+        if (v == ~0ull) continue;
+        uint32_t r = ((uint32_t)v) >> 1, side = (uint32_t)v & 1u;
+        uint32_t pred = RP_NONE;
+        if (i > 0 && C.ent[i - 1] != ~0ull && (C.ent[i - 1] >> 32) == (v >> 32)) pred = ((uint32_t)C.ent[i - 1]) >> 1;
+        if (side) C.pb[r] = pred; else C.pa[r] = pred;
+    }
+    __threadfence_block();
+    __syncthreads();
+}
+
+// Dataflow execution of one sweep: each lane walks its positions in increasing order and runs
+// one when both predecessors are done in this sweep (the lowest unfinished position is always
+// ready, so every lane finishes).
+template <class F>
+static __device__ void run_sweep(const Chains &C, uint32_t n, uint32_t token, F exec) {
+    uint32_t r = threadIdx.x;
+    while (r < n) {
+        uint32_t a = C.pa[r], b = C.pb[r];
+        bool ready = (a == RP_NONE || __hip_atomic_load(&C.done[a], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= token) &&
+                     (b == RP_NONE || __hip_atomic_load(&C.done[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= token);
+        if (ready) {
+            exec(r);
+            __hip_atomic_store(&C.done[r], token, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            r += blockDim.x;
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+}
+
+// (sort_key6(height), list position) packed for the shock passes: keys clamped to +-2^43 units
+// of 1e-6 m (heights beyond 8,796 km tie and fall back to list order).
+static __device__ __forceinline__ uint64_t shock_key(float h, uint32_t r) {
+    int64_t k = sort_key6(h);
+    const int64_t LIM = (int64_t)1 << 43;
+    if (k < -LIM) k = -LIM;
+    if (k > LIM - 1) k = LIM - 1;
+    return ((uint64_t)(k + LIM) << 20) | r;
+}
+
+// Sets C.order to the shock order of the island list from per-position heights.
+template <class H>
+static __device__ void shock_order(const Chains &C, uint32_t n, H height) {
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) C.ent[i] = i < n ? shock_key(height(i), i) : ~0ull;
+    __threadfence_block();
+    __syncthreads();
+    bitonic_u64(C.ent, m);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) C.order[i] = (uint32_t)(C.ent[i] & 0xFFFFFu);
+    __threadfence_block();
+    __syncthreads();
+}
+
+// solver.rs:455-464 apply_contact_impulse (+ dynamics.rs:193-205)
+static __device__ __forceinline__ void apply_cimp(const Dev &d, uint32_t ai, uint32_t bi, float inv_a, float inv_b, V3 ra, V3 rb, V3 j) {
+    if (inv_a > 0.0f) {
+        V3 nj = -j;
+        float im = d.inv_mass[ai];
+        if (im > 0.0f && fin3(nj)) d.lin[ai] = S3(fin_v(L3(d.lin[ai]) + nj * im));
+        V3 tau = cross3(ra, nj);
+        if (d.inv_mass[ai] > 0.0f && fin3(tau)) d.ang[ai] = S3(fin_v(L3(d.ang[ai]) + mv(load_m9(d.iiw + 9 * ai), tau)));
+    }
+    if (inv_b > 0.0f) {
+        float im = d.inv_mass[bi];
+        if (im > 0.0f && fin3(j)) d.lin[bi] = S3(fin_v(L3(d.lin[bi]) + j * im));
+        V3 tau = cross3(rb, j);
+        if (d.inv_mass[bi] > 0.0f && fin3(tau)) d.ang[bi] = S3(fin_v(L3(d.ang[bi]) + mv(load_m9(d.iiw + 9 * bi), tau)));
+    }
+}
+static __device__ __forceinline__ V3 rel_vel(const Dev &d, uint32_t ai, uint32_t bi, V3 ra, V3 rb) {
+    V3 va = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
+    V3 vb = bi == RP_NONE ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
+    return vb - va;
+}
+static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
+    float x = slip / mu4.z;
+    return mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
+}
+
+// The island's segment of the sorted list and its chain storage.
+#define ISLAND_LOOP_BEGIN                                                                      \
+    __shared__ IslandLds L;                                                                    \
+    const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = d.cnt[CT_ISLC];                           \
+    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {                            \
+        const uint32_t s = d.isl_start[isl], n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - s; \
+        if (n > RP_ISLAND_MAX) { if (threadIdx.x == 0) atomicMax(&d.cnt[CT_BIG_ISLAND], n); continue; } \
+        const uint32_t *lst = d.isl_v2 + s;                                                    \
+        const Chains C = chains_for(d, L, s, n);                                               \
+        for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) C.order[r] = r;                 \
+        __threadfence_block();                                                                 \
+        __syncthreads();                                                                       \
+        build_chains(d, C, lst, n);
+#define ISLAND_LOOP_END }
+
+// Velocity solve of one island per workgroup: warm-start application, PGS iterations, the
+// pre-shock impulse snapshot (what the warm cache stores), then the shock pass.
+__global__ void __launch_bounds__(256) k_sweep_vel(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t cur = tp.cur;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    ISLAND_LOOP_BEGIN
+        uint32_t token = 1;
+        // warm-start application (solver.rs:749-760)
+        run_sweep(C, n, token++, [&](uint32_t r) {
+            uint32_t c = lst[r];
+            int wi = d.warm_idx[c];
+            if (wi < 0) return;
+            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+            V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
+            float2 inv = d.cs_inv[c];
+            float ejn = d.wk_jn[cur][wi];
+            V3 imp = n3 * ejn;
+            V3 v = rel_vel(d, ai, bi, ra, rb);
+            float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
+            float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
+            V3 ejt = L3(d.wk_jt[cur][wi]);
+            V3 t = clamp_len_max(mk3(dot3(ejt, t1), dot3(ejt, t2), 0.0f), mu * ejn);
+            d.jt[c] = make_float2(t.x, t.y);
+            imp = imp + (t1 * t.x + t2 * t.y);
+            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
+        });
+        // PGS iterations (solver.rs:839-879)
+        for (uint32_t it = 0; it < tp.iv; it++) {
+            run_sweep(C, n, token++, [&](uint32_t r) {
+                uint32_t c = lst[r];
+                uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+                V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
+                float2 inv = d.cs_inv[c];
+                float4 m4 = d.cs_m[c];
+                float vrn = dot3(rel_vel(d, ai, bi, ra, rb), n3);
+                float jraw = (m4.w - vrn) * m4.x;
+                float old = d.jn[c];
+                float nw = fmaxf(old + jraw, 0.0f);
+                d.jn[c] = nw;
+                apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, n3 * (nw - old));
+                V3 vf = rel_vel(d, ai, bi, ra, rb);
+                float vt1 = dot3(vf, t1), vt2 = dot3(vf, t2);
+                float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
+                float2 ot2 = d.jt[c];
+                V3 ot = mk3(ot2.x, ot2.y, 0.0f);
+                V3 nt = clamp_len_max(ot - mk3(vt1 * m4.y, vt2 * m4.z, 0.0f), mu * nw);
+                d.jt[c] = make_float2(nt.x, nt.y);
+                V3 dd = nt - ot;
+                apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, t1 * dd.x + t2 * dd.y);
+            });
+        }
+        for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
+        // shock pass (solver.rs:968-993): order by (sort_key6(lowest height), index)
+        shock_order(C, n, [&](uint32_t i) {
+            uint32_t c = lst[i];
+            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+            float ha = dot3(L3(d.pos[ai]), up);
+            return bi == RP_NONE ? -INFINITY : fminf(ha, dot3(L3(d.pos[bi]), up));
+        });
+        build_chains(d, C, lst, n);
+        run_sweep(C, n, token++, [&](uint32_t r) {
+            uint32_t c = lst[C.order[r]];
+            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+            V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
+            float2 inv = d.cs_inv[c];
+            float4 m4 = d.cs_m[c];
+            float inv_a = inv.x, inv_b = inv.y, mn = m4.x;
+            if (inv_a > 0.0f && inv_b > 0.0f) {
+                M3 iia = load_m9(d.iiw + 9 * ai), iib = load_m9(d.iiw + 9 * bi);
+                if (dot3(L3(d.pos[ai]), up) < dot3(L3(d.pos[bi]), up)) { inv_a = 0.0f; iia = mzero(); }
+                else { inv_b = 0.0f; iib = mzero(); }
+                mn = eff_mass(inv_a, inv_b, iia, iib, ra, rb, n3);
+            }
+            float v = dot3(rel_vel(d, ai, bi, ra, rb), n3);
+            float old = d.jn[c];
+            float nw = fmaxf(old + (m4.w - v) * mn, 0.0f);
+            d.jn[c] = nw;
+            apply_cimp(d, ai, bi, inv_a, inv_b, ra, rb, n3 * (nw - old));
+        });
+    ISLAND_LOOP_END
+}
+
+// solver.rs:1247-1270 loaded flags and impulse write-back
+__global__ void k_loaded(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(c, nct) {
+        uint32_t w = d.col_world[d.ct_a[c]];
+        if (!d.w_solve[w]) { d.ct_imp[c] = 0.0f; continue; }
+        float2 t = d.jt[c];
+        float jn = d.jn[c];
+        if (jn > 0.0f || t.x != 0.0f || t.y != 0.0f) {
+            atomicOr(&d.flags[d.cs_ai[c]], FL_LOADED);
+            if (d.cs_bi[c] != RP_NONE) atomicOr(&d.flags[d.cs_bi[c]], FL_LOADED);
+        }
+        d.ct_imp[c] = jn;
+    }
+}
+
+// ------------------------------------------------------------------ warm cache store
+// solver.rs:917-944: the new cache of a world is its contacts' nonzero impulses (contact order is
+// key order) merged with the old entries of idle pairs whose key has no new entry.
+static __device__ __forceinline__ bool idle_body(const Dev &d, uint32_t w, uint32_t local) {
+    if (local == 0xFFFFFFFFu) return true;
+    uint32_t b = d.w_body_off[w] + local;
+    if (b >= d.w_body_off[w + 1] || !(d.bflags[b] & BF_ALIVE)) return false;
+    return !(d.bflags[b] & BF_DYN) || !d.awake[b];
+}
+static __device__ __forceinline__ bool new_entry(const Dev &d, uint32_t c) {
+    float2 t = d.jt[c];
+    V3 jt = L3(d.cs_t1[c]) * t.x + L3(d.cs_t2[c]) * t.y;
+    return !(d.jn_pre[c] == 0.0f && zero3(jt));
+}
+static __device__ __forceinline__ void world_contacts(const Dev &d, uint32_t w, uint32_t &c0, uint32_t &c1) {
+    c0 = min(d.c_off[d.w_col_off[w]], d.cap_contacts);
+    c1 = min(d.c_off[d.w_col_off[w + 1]], d.cap_contacts);
+}
+static __device__ __forceinline__ uint32_t lower_bound_key(const uint64_t *a, uint32_t lo, uint32_t hi, uint64_t k) {
+    while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (a[mid] < k) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+
+__global__ void k_warm_newflags(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(c, d.cap_contacts) {
+        uint32_t f = 0;
+        if (c < nct) f = (d.w_solve[d.col_world[d.ct_a[c]]] && new_entry(d, c)) ? 1u : 0u;
+        d.newflag[c] = f;
+    }
+}
+__global__ void k_warm_keepflags(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t cur = tp.cur, tot = d.wc_off[cur][d.nw];
+    GRID_STRIDE(e, d.cap_warm) {
+        uint32_t f = 0;
+        if (e < tot) {
+            uint32_t w = upper_bound_u32(d.wc_off[cur], d.nw, e) - 1;
+            if (!d.w_solve[w]) f = 1;
+            else {
+                uint64_t k = d.wk_key[cur][e];
+                if (idle_body(d, w, (uint32_t)(k >> 32)) && idle_body(d, w, (uint32_t)k)) {
+                    uint32_t c0, c1;
+                    world_contacts(d, w, c0, c1);
+                    bool has_new = false;
+                    for (uint32_t c = lower_bound_key(d.ct_key, c0, c1, k); c < c1 && d.ct_key[c] == k; c++) has_new |= d.newflag[c + 1] != d.newflag[c];
+                    f = has_new ? 0u : 1u;
+                }
+            }
+        }
+        d.keepflag[e] = f;
+    }
+}
+// After both flag arrays are scanned: positions by rank in own list + smaller keys in the other.
+__global__ void k_warm_merge(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t cur = tp.cur, nxt = cur ^ 1u;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts), tot = d.wc_off[cur][d.nw];
+    GRID_STRIDE(c, nct) {
+        if (d.newflag[c + 1] == d.newflag[c]) continue;  // not a new entry
+        uint32_t w = d.col_world[d.ct_a[c]];
+        uint64_t k = d.ct_key[c];
+        uint32_t e = lower_bound_key(d.wk_key[cur], d.wc_off[cur][w], d.wc_off[cur][w + 1], k);
+        uint32_t o = d.newflag[c] + d.keepflag[e];
+        if (o >= d.cap_warm) continue;
+        float2 t = d.jt[c];
+        d.wk_key[nxt][o] = k; d.wk_la[nxt][o] = d.ct_anchor[c]; d.wk_jn[nxt][o] = d.jn_pre[c];
+        d.wk_jt[nxt][o] = S3(L3(d.cs_t1[c]) * t.x + L3(d.cs_t2[c]) * t.y);
+    }
+    GRID_STRIDE(e, tot) {
+        if (d.keepflag[e + 1] == d.keepflag[e]) continue;
+        uint32_t w = upper_bound_u32(d.wc_off[cur], d.nw, e) - 1;
+        uint64_t k = d.wk_key[cur][e];
+        uint32_t c0, c1;
+        world_contacts(d, w, c0, c1);
+        uint32_t c = lower_bound_key(d.ct_key, c0, c1, k);
+        uint32_t o = d.keepflag[e] + d.newflag[c];
+        if (o >= d.cap_warm) continue;
+        d.wk_key[nxt][o] = k; d.wk_la[nxt][o] = d.wk_la[cur][e];
+        d.wk_jn[nxt][o] = d.wk_jn[cur][e]; d.wk_jt[nxt][o] = d.wk_jt[cur][e];
+    }
+    GRID_STRIDE(w, d.nw + 1) {
+        uint32_t c0 = w < d.nw ? min(d.c_off[d.w_col_off[w]], d.cap_contacts) : nct;
+        uint32_t e0 = d.wc_off[cur][w];
+        d.wc_off[nxt][w] = d.newflag[c0] + d.keepflag[e0];
+    }
+}
+
+// ------------------------------------------------------------------ S4
+__global__ void k_ball_init(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(b, d.nb) {
+        uint32_t f = d.flags[b];
+        d.ballf[b] = (!(f & FL_CONSTRAINED) || !(f & FL_LOADED)) ? 1u : 0u;
+    }
+}
+
+// step.rs:180-206 in contact order over each island; only awake dynamic bodies' flags can change
+// an outcome, and those are chained.
+__global__ void __launch_bounds__(256) k_sweep_ball(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    ISLAND_LOOP_BEGIN
+        run_sweep(C, n, 1u, [&](uint32_t r) {
+            uint32_t c = lst[r];
+            uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+            bool hb = b != RP_NONE;
+            bool ba = d.ballf[a] != 0, bb = hb && d.ballf[b] != 0;
+            if (!ba && !bb) return;
+            V3 point = L3(d.ct_p[c]);
+            V3 nrm = L3(d.cs_n[c]);
+            V3 la = ba ? (L3(d.last_lin[a]) + L3(d.lin[a])) * 0.5f : L3(d.lin[a]);
+            V3 wa = L3(d.ang[a]);
+            V3 ra = point - L3(d.pos[a]);
+            V3 lb = splat3(0.0f), wb = splat3(0.0f), rb = splat3(0.0f), vbl = splat3(0.0f);
+            if (hb) { lb = bb ? (L3(d.last_lin[b]) + L3(d.lin[b])) * 0.5f : L3(d.lin[b]); wb = L3(d.ang[b]); rb = point - L3(d.pos[b]); vbl = L3(d.lin[b]); }
+            float approach = dot3((lb + cross3(wb, rb)) - (la + cross3(wa, ra)), nrm);
+            float vend = dot3((vbl + cross3(wb, rb)) - (L3(d.lin[a]) + cross3(wa, ra)), nrm);
+            float s0 = -(d.ct_p[c].w - 0.05f);
+            if (s0 + (2.0f * vend) * tp.dt < 0.0f || s0 + approach * tp.dt < 0.0f) {
+                if (d.cs_wa[c] != RP_NONE) d.ballf[a] = 0;
+                if (d.cs_wb[c] != RP_NONE) d.ballf[b] = 0;
+            }
+        });
+    ISLAND_LOOP_END
+}
+
+static __device__ Q4 from_axis_angle(V3 axis, float angle) {
+    float s, c;
+    rp_sincosf(angle * 0.5f, &s, &c);
+    V3 v = axis * s;
+    return mkq(v.x, v.y, v.z, c);
+}
+static __device__ Q4 turn(Q4 q, V3 w, float h) {  // step.rs:230-233
+    if (len2(w) <= 1.0e-18f) return q;
+    return qnormalize(qmul(from_axis_angle(normalize3(w), len3(w) * h), q));
+}
+
+// step.rs:208-280
+__global__ void k_integrate(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    GRID_STRIDE(i, d.nb) {
+        float4 p4 = d.pos[i], q4 = d.rot[i];
+        d.opos[i] = p4; d.orot[i] = q4;
+        d.ppos[i] = make_float4(0, 0, 0, 0); d.prot[i] = make_float4(0, 0, 0, 1);
+        if (!(d.bflags[i] & BF_DYN) || !d.awake[i]) continue;
+        const float dt = tp.dt;
+        V3 p = L3(p4); Q4 q = LQ(q4);
+        V3 vl = L3(d.lin[i]), va = L3(d.ang[i]);
+        p = p + vl * dt;
+        if (d.ballf[i]) p = p + (L3(d.last_lin[i]) - vl) * (0.5f * dt);
+        if (len2(va) > 0.0f) {
+            M3 l = load_m9(d.iiw + 9 * i);
+            float det = mdet(l);
+            bool has_m = isfinite(det) && fabsf(det) > 1.0e-20f;
+            V3 mom = has_m ? mv(minverse(l), va) : splat3(0.0f);
+            Q4 start = q;
+            q = turn(start, va, dt);
+            if (has_m) {
+                M3 il = d.inv_mass[i] == 0.0f ? mzero() : load_m9(d.iil + 9 * i);
+                float mean = ((il.c0.x + il.c1.y) + il.c2.z) / 3.0f;
+                M3 dev = msub(il, mdiag(splat3(mean)));
+                bool principal = len3(cross3(mom, va)) < (1.0e-5f * len3(mom)) * len3(va);
+                float gain = principal ? 0.0f : (dt * fmaxf(fmaxf(len3(dev.c0), len3(dev.c1)), len3(dev.c2))) * len3(mom);
+                float gq = ceilf(gain / 0.5f);
+                uint32_t n = !(gq >= 1.0f) ? 1u : (gq > 256.0f ? 256u : (uint32_t)gq);
+                float h = dt / (float)n;
+                Q4 qq = start; V3 wv = va;
+                for (uint32_t s = 0; s < n; s++) {
+                    Q4 q0 = qq;
+                    qq = turn(q0, wv, h);
+                    M3 r = mquat(qq);
+                    V3 wend = mv(mmul(mmul(r, il), mtrans(r)), mom);
+                    for (int t = 0; t < 2; t++) {
+                        qq = turn(q0, (wv + wend) * 0.5f, h);
+                        r = mquat(qq);
+                        wend = mv(mmul(mmul(r, il), mtrans(r)), mom);
+                    }
+                    wv = wend;
+                }
+                q = qq;
+                set_vel(d, i, vl, wv);
+            }
+        }
+        set_pose(d, i, p, q);
+        d.ppos[i] = d.pos[i]; d.prot[i] = d.rot[i];
+    }
+}
+
+// ------------------------------------------------------------------ S5
+// step.rs:423-443 anchors at the step-start poses (pose before integration for every body)
+__global__ void k_posc_prep(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nislc = d.cnt[CT_ISLC];
+    GRID_STRIDE(r, nislc) {
+        uint32_t c = d.isl_v2[r];
+        uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+        V3 point = L3(d.ct_p[c]);
+        V3 la = qrot(qconj(LQ(d.orot[a])), point - L3(d.opos[a]));
+        V3 lb = b == RP_NONE ? splat3(0.0f) : qrot(qconj(LQ(d.orot[b])), point - L3(d.opos[b]));
+        d.pc_la[c] = S3(la, d.ct_p[c].w - 0.05f);
+        d.pc_lb[c] = S3(lb);
+    }
+}
+
+// step.rs:447-497 one projection pass item
+static __device__ void project_one(const Dev &d, uint32_t c, bool shock, V3 up) {
+    uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+    bool hb = b != RP_NONE;
+    bool am = d.inv_mass[a] > 0.0f && d.awake[a];
+    bool bm = hb && d.inv_mass[b] > 0.0f && d.awake[b];
+    if (shock && am && bm) {
+        float ha = dot3(L3(d.pos[a]), up), hbb = dot3(L3(d.pos[b]), up);
+        if (ha < hbb) am = false; else bm = false;
+    }
+    float4 la4 = d.pc_la[c];
+    V3 n = L3(d.cs_n[c]);
+    V3 pa = L3(d.pos[a]); Q4 qa = LQ(d.rot[a]);
+    V3 wa = pa + qrot(qa, L3(la4));
+    V3 pb = splat3(0.0f), wb; Q4 qb = qid();
+    if (hb) { pb = L3(d.pos[b]); qb = LQ(d.rot[b]); wb = pb + qrot(qb, L3(d.pc_lb[c])); }
+    else wb = L3(d.ct_p[c]);
+    float pen = la4.w - dot3(wb - wa, n);
+    float err = pen - 0.005f;
+    if (err <= 0.0f) return;
+    V3 ra = wa - pa, rb = hb ? wb - pb : splat3(0.0f);
+    float ima = 0.0f, imb = 0.0f; M3 iia = mzero(), iib = mzero();
+    if (am) { ima = d.inv_mass[a]; iia = inv_inertia_world(d, a, qa); }
+    if (hb && bm) { imb = d.inv_mass[b]; iib = inv_inertia_world(d, b, qb); }
+    V3 ca = cross3(ra, n), cb = cross3(rb, n);
+    float wsum = ((ima + imb) + dot3(ca, mv(iia, ca))) + dot3(cb, mv(iib, cb));
+    if (wsum <= 1.0e-9f) return;
+    float lambda = err / wsum;
+    for (int side = 0; side < 2; side++) {
+        uint32_t id; V3 p; Q4 q; float im; M3 ii; V3 r; V3 j;
+        if (side == 0) { if (!am) continue; id = a; p = pa; q = qa; im = ima; ii = iia; r = ra; j = (-n) * lambda; }
+        else { if (!(hb && bm)) continue; id = b; p = pb; q = qb; im = imb; ii = iib; r = rb; j = n * lambda; }
+        V3 dth = mv(ii, cross3(r, j));
+        d.slin[id] = S3(L3(d.slin[id]) + j * im);
+        d.sang[id] = S3(L3(d.sang[id]) + dth);
+        Q4 dq = qmul(mkq(dth.x, dth.y, dth.z, 0.0f), q);
+        dq = mkq(dq.x * 0.5f, dq.y * 0.5f, dq.z * 0.5f, dq.w * 0.5f);
+        Q4 nr = qnormalize(mkq(q.x + dq.x, q.y + dq.y, q.z + dq.z, q.w + dq.w));
+        set_pose(d, id, p + j * im, nr);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sweep_proj(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    ISLAND_LOOP_BEGIN
+        uint32_t token = 1;
+        for (uint32_t it = 0; it < tp.ip; it++)
+            run_sweep(C, n, token++, [&](uint32_t r) { project_one(d, lst[r], false, up); });
+        shock_order(C, n, [&](uint32_t i) {
+            uint32_t c = lst[i];
+            uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+            return b == RP_NONE ? -INFINITY : fminf(dot3(L3(d.pos[a]), up), dot3(L3(d.pos[b]), up));
+        });
+        build_chains(d, C, lst, n);
+        run_sweep(C, n, token++, [&](uint32_t r) { project_one(d, lst[C.order[r]], true, up); });
+    ISLAND_LOOP_END
+}
+
+// ------------------------------------------------------------------ S6 / S7
+__global__ void k_s6(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    GRID_STRIDE(i, d.nb) {
+        if ((d.bflags[i] & BF_DYN) && d.awake[i]) {
+            V3 vl = L3(d.lin[i]), va = L3(d.ang[i]);
+            V3 np = L3(d.pos[i]); Q4 nq = LQ(d.rot[i]);
+            Q4 pq = LQ(d.prot[i]);
+            V3 dl = (np - L3(d.ppos[i])) - L3(d.slin[i]);
+            if (!zero3(dl)) vl = vl + dl / tp.dt;
+            if ((d.flags[i] & FL_GROUND) && vl.y < 0.0f) {
+                vl.y = 0.0f;
+                np.y = d.opos[i].y;
+                set_pose(d, i, np, nq);
+            }
+            if (!eqq(nq, pq)) {
+                Q4 dq = qmul(nq, qconj(pq));
+                V3 da = mk3(dq.x, dq.y, dq.z) * 2.0f;
+                if (dq.w < 0.0f) da = -da;
+                va = va + (da - L3(d.sang[i])) / tp.dt;
+            }
+            set_vel(d, i, vl, va);
+        }
+        d.last_lin[i] = d.lin[i];
+        // sleep.rs:59-80
+        uint32_t bf = d.bflags[i];
+        if ((bf & BF_SLEEP_OK) && d.awake[i] && (bf & BF_DYN)) {
+            if (len2(L3(d.lin[i])) < 0.01f && len2(L3(d.ang[i])) < 0.01f) {
+                uint32_t t = (d.sticks[i] + 1u) & 0xFFFFu;
+                d.sticks[i] = t;
+                if (t > 30u) d.awake[i] = 0;
+            } else d.sticks[i] = 0;
+        }
+    }
+}
+
+// Overflow: restore the pre-tick state, keep the failed tick from repeating its mutations.
+__global__ void k_restore_backup(Dev d) {
+    GRID_STRIDE(b, d.nb) {
+        d.pos[b] = d.bk_pos[b]; d.rot[b] = d.bk_rot[b]; d.lin[b] = d.bk_lin[b]; d.ang[b] = d.bk_ang[b];
+        d.last_lin[b] = d.bk_last[b]; d.awake[b] = d.bk_awake[b]; d.sticks[b] = d.bk_sticks[b];
+    }
+}
+
+// Capacity checks after each sizing scan; sets the abort flag so later kernels return at once.
+__global__ void k_check(Dev d, uint32_t which) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (which == 0) {
+        if (d.cnt[CT_PAIRS] > d.cap_pairs) { d.cnt[CT_NEED_PAIRS] = d.cnt[CT_PAIRS]; d.cnt[CT_ABORT] = 1; }
+        if (d.cnt[CT_CANDS] > d.cap_cands) { d.cnt[CT_NEED_CANDS] = d.cnt[CT_CANDS]; d.cnt[CT_ABORT] = 1; }
+    } else if (which == 1) {
+        if (d.cnt[CT_CONTACTS] > d.cap_contacts) { d.cnt[CT_NEED_CONTACTS] = d.cnt[CT_CONTACTS]; d.cnt[CT_ABORT] = 1; }
+    } else if (which == 2) {
+        uint32_t wt = d.cnt[CT_NEW_TOTAL] + d.cnt[CT_KEEP_TOTAL];
+        if (wt > d.cap_warm) { d.cnt[CT_NEED_WARM] = wt; d.cnt[CT_ABORT] = 1; }
+        if (d.cnt[CT_BIG_ISLAND] != 0) d.cnt[CT_ABORT] = 1;
+    }
+}
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) { GRID_STRIDE(i, n) p[i] = 0; }
+__global__ void k_tick_done(Dev d) { if (threadIdx.x == 0 && blockIdx.x == 0 && !d.cnt[CT_ABORT]) d.cnt[CT_DONE] += 1; }
+__global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) cnt[idx] = v; }
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n) { GRID_STRIDE(i, n) dst[i] = src[i]; }
+
+// ------------------------------------------------------------------ host-side launch sequence
+static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t max_grid = 8192) {
+    uint32_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    return g;
+}
+
+// Exclusive scan of n u32 in place; the total goes to *total (device).  n + 1 slots are written
+// when with_total_slot (the scan of counts gives offsets with the end in slot n).
+static void scan_u32(uint32_t *a, uint32_t n, uint32_t *tmp, uint32_t *total, hipStream_t st) {
+    uint32_t nb = (n + RP_SCAN_BLOCK - 1) / RP_SCAN_BLOCK;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, st, a, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, tmp, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, a, n, tmp, a);
+}
+
+__global__ void k_store_total(uint32_t *a, uint32_t n, const uint32_t *total) { if (threadIdx.x == 0 && blockIdx.x == 0) a[n] = *total; }
+
+// Stable LSD radix sort of (keys, vals) by the low `bits` of the keys; n read from device.
+// Result ends in (k_out, v_out); ping-pong buffers given.
+static void radix_sort(const Dev &d, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, const uint32_t *n_dev, uint32_t cap,
+                       uint32_t bits, hipStream_t st, uint32_t **k_out, uint32_t **v_out) {
+    uint32_t nblocks = (cap + RP_RS_TILE - 1) / RP_RS_TILE;
+    if (nblocks < 1) nblocks = 1;
+    uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(256), 0, st, ka, n_dev, shift, nblocks, d.rs_hist);
+        scan_u32(d.rs_hist, 256 * nblocks, d.scan_tmp, d.cnt + CT_SCRATCH, st);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(256), 0, st, ka, va, n_dev, shift, nblocks, d.rs_hist, kb, vb);
+        uint32_t *t = ka; ka = kb; kb = t;
+        t = va; va = vb; vb = t;
+    }
+    *k_out = ka; *v_out = va;
+}
+
+extern "C" void rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st) {
+    const uint32_t B = 256;
+    uint32_t *scratch_total = d.cnt + CT_SCRATCH;
+    hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_bp_sort, dim3(d.nw), dim3(1024), 0, st, d);
+    hipLaunchKernelGGL(k_bp_enum<0>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    scan_u32(d.out_off, d.nc, d.scan_tmp, d.cnt + CT_PAIRS, st);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.out_off, d.nc, d.cnt + CT_PAIRS);
+    scan_u32(d.cand_off, d.nc, d.scan_tmp, d.cnt + CT_CANDS, st);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.cand_off, d.nc, d.cnt + CT_CANDS);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 0u);
+    hipLaunchKernelGGL(k_bp_enum<1>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_narrow_pairs, dim3(grid_for(d.cap_pairs, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_narrow_terrain, dim3(grid_for(d.cap_cands, B, 16384)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_terrain_cull, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_contact_count, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    scan_u32(d.c_off, d.nc, d.scan_tmp, d.cnt + CT_CONTACTS, st);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.c_off, d.nc, d.cnt + CT_CONTACTS);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 1u);
+    hipLaunchKernelGGL(k_contact_write, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_solve, d.nw);
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_island_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    uint32_t *ks, *vs;
+    radix_sort(d, d.isl_k, d.isl_v, d.isl_k2, d.isl_v2, d.cnt + CT_CONTACTS, d.cap_contacts, 32, st, &ks, &vs);
+    // four passes leave the result in (isl_k, isl_v); the sweeps read isl_v2 / isl_k2
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, vs, d.isl_v2, d.cap_contacts);
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, ks, d.isl_k2, d.cap_contacts);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC, 0u);
+    hipLaunchKernelGGL(k_island_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
+    scan_u32(d.isl_flag, d.cap_contacts, d.scan_tmp, d.cnt + CT_ISLANDS, st);
+    hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
+    hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_BIG_ISLAND, 0u);
+    hipLaunchKernelGGL(k_sweep_vel, dim3(island_grid), dim3(256), 0, st, d, tp);
+    // warm cache for the next tick (into buffer cur ^ 1)
+    hipLaunchKernelGGL(k_warm_newflags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    scan_u32(d.newflag, d.cap_contacts, d.scan_tmp, d.cnt + CT_NEW_TOTAL, st);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.newflag, d.cap_contacts, d.cnt + CT_NEW_TOTAL);
+    hipLaunchKernelGGL(k_warm_keepflags, dim3(grid_for(d.cap_warm, B)), dim3(B), 0, st, d, tp);
+    scan_u32(d.keepflag, d.cap_warm, d.scan_tmp, d.cnt + CT_KEEP_TOTAL, st);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.keepflag, d.cap_warm, d.cnt + CT_KEEP_TOTAL);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 2u);
+    hipLaunchKernelGGL(k_warm_merge, dim3(grid_for(d.cap_contacts > d.cap_warm ? d.cap_contacts : d.cap_warm, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_sweep_ball, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_sweep_proj, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);
+    (void)scratch_total;
+}
+
+extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st) {
+    hipLaunchKernelGGL(k_restore_backup, dim3(grid_for(d.nb, 256)), dim3(256), 0, st, d);
+}

@@ -1,0 +1,355 @@
+"""ctypes binding of librphys_amd.so (include/riftphys_c.h).
+
+Mirrors the reference's C ABI (crates/riftphys-ffi/src/riftphys_c.h): World wraps an RPhysWorld
+handle, with the same 1-based body ids, result codes and error messages, plus the library's
+additive exports (density bodies, pair props, iteration counts, debug views, batches).
+The library must be built (riftphys-rust_amd/csrc/Makefile); there is no fallback: every call
+runs the device step.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librphys_amd.so")
+
+OK, ERR_NULL, ERR_INVALID_ID, ERR_CAPACITY, ERR_BAD_ARG, ERR_INTERNAL = 0, 1, 2, 3, 4, 0x80000000
+BODY_STATIC, BODY_DYNAMIC = 0, 1
+SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE = 0, 1, 2
+TERRAIN_BODY_ID = 0xFFFFFFFF
+STEP_FLAG_SOLVER_ITERS = 1
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Quat(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class Isometry(C.Structure):
+    _fields_ = [("pos", Vec3), ("rot", Quat)]
+
+
+class Velocity(C.Structure):
+    _fields_ = [("lin", Vec3), ("ang", Vec3)]
+
+
+class Material(C.Structure):
+    _fields_ = [("mu_static", C.c_float), ("mu_dynamic", C.c_float), ("restitution", C.c_float)]
+
+
+class CapsuleBodyDesc(C.Structure):
+    _fields_ = [("pose", Isometry), ("vel", Velocity), ("radius", C.c_float), ("half_height", C.c_float), ("mass", C.c_float),
+                ("body_type", C.c_uint32), ("material", Material), ("user_tag", C.c_uint32)]
+
+
+class BoxBodyDesc(C.Structure):
+    _fields_ = [("pose", Isometry), ("vel", Velocity), ("hx", C.c_float), ("hy", C.c_float), ("hz", C.c_float), ("mass", C.c_float),
+                ("body_type", C.c_uint32), ("material", Material), ("user_tag", C.c_uint32)]
+
+
+class SphereBodyDesc(C.Structure):
+    _fields_ = [("pose", Isometry), ("vel", Velocity), ("radius", C.c_float), ("mass", C.c_float),
+                ("body_type", C.c_uint32), ("material", Material), ("user_tag", C.c_uint32)]
+
+
+class Hash32(C.Structure):
+    _fields_ = [("bytes", C.c_uint8 * 32)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("size_bytes", C.c_uint32), ("flags", C.c_uint32), ("substeps", C.c_uint32), ("solver_iters", C.c_uint32), ("dt", C.c_float)]
+
+
+class HeightfieldDesc(C.Structure):
+    _fields_ = [("size_bytes", C.c_uint32), ("flags", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("row_stride", C.c_uint32),
+                ("cell_size_x", C.c_float), ("cell_size_z", C.c_float), ("height_scale", C.c_float), ("height_offset", C.c_float),
+                ("heights", C.POINTER(C.c_int16))]
+
+
+class ContactEvent(C.Structure):
+    _fields_ = [("body_id_a", C.c_uint32), ("body_id_b", C.c_uint32), ("position", Vec3), ("normal", Vec3), ("impulse", C.c_float)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("terrain_candidates", C.c_uint32), ("island_contacts", C.c_uint32),
+                ("islands", C.c_uint32), ("warm_entries", C.c_uint32)]
+
+
+class ContactEx(C.Structure):
+    _fields_ = [("a_collider", C.c_uint32), ("b_collider", C.c_uint32), ("point", C.c_float * 3), ("normal", C.c_float * 3),
+                ("depth", C.c_float), ("impulse", C.c_float), ("feature", C.c_uint32), ("cell", C.c_uint32)]
+
+
+# (name, restype, argtypes) of every export in include/riftphys_c.h
+P = C.c_void_p
+U32, U64, F32 = C.c_uint32, C.c_uint64, C.c_float
+PU32 = C.POINTER(C.c_uint32)
+PF32 = C.POINTER(C.c_float)
+EXPORTS = [
+    ("rphys_get_abi_version", U32, []),
+    ("rphys_get_build_id", C.c_char_p, []),
+    ("rphys_last_error_message", C.c_char_p, []),
+    ("rphys_last_error_copy", U32, [C.c_char_p, U32, PU32]),
+    ("rphys_world_create", P, [U32, U32]),
+    ("rphys_world_destroy", None, [P]),
+    ("rphys_world_set_epoch", U32, [P, U64]),
+    ("rphys_world_set_rng_seed", U32, [P, U64]),
+    ("rphys_add_capsule_body", U32, [P, C.POINTER(CapsuleBodyDesc), PU32]),
+    ("rphys_add_box_body", U32, [P, C.POINTER(BoxBodyDesc), PU32]),
+    ("rphys_add_sphere_body", U32, [P, C.POINTER(SphereBodyDesc), PU32]),
+    ("rphys_body_remove", U32, [P, U32]),
+    ("rphys_body_get_pose", U32, [P, U32, C.POINTER(Isometry)]),
+    ("rphys_body_set_pose", U32, [P, U32, C.POINTER(Isometry)]),
+    ("rphys_body_get_velocity", U32, [P, U32, C.POINTER(Velocity)]),
+    ("rphys_body_set_velocity", U32, [P, U32, C.POINTER(Velocity)]),
+    ("rphys_body_apply_impulse", U32, [P, U32, Vec3]),
+    ("rphys_world_step_ex", U32, [P, C.POINTER(StepParams)]),
+    ("rphys_world_step", U32, [P, F32]),
+    ("rphys_world_tick_index", U32, [P, C.POINTER(U64)]),
+    ("rphys_world_num_bodies", U32, [P, PU32]),
+    ("rphys_world_get_step_hash", U32, [P, C.POINTER(Hash32)]),
+    ("rphys_world_step_and_hash", U32, [P, F32, C.POINTER(Hash32)]),
+    ("rphys_world_raycast", U32, [P, P, P]),
+    ("rphys_world_capsule_sweep", U32, [P, P, P]),
+    ("rphys_world_set_heightfield_i16", U32, [P, C.POINTER(HeightfieldDesc), F32, F32, F32]),
+    ("rphys_world_set_heightfield_raw32_square", U32, [P, P, U32, F32, F32, F32, F32, F32]),
+    ("rphys_world_clear_heightfield", U32, [P]),
+    ("rphys_world_get_contacts", U32, [P, C.POINTER(ContactEvent), U32, PU32]),
+    ("rphys_world_snapshot_create", P, [P]),
+    ("rphys_world_snapshot_restore", U32, [P, P]),
+    ("rphys_world_snapshot_free", None, [P]),
+    ("rphys_add_body_density", U32, [P, U32, PF32, C.POINTER(Isometry), C.POINTER(Velocity), U32, F32, PU32]),
+    ("rphys_world_set_pair_props", U32, [P, U32, U32, F32, F32, F32, F32]),
+    ("rphys_world_set_solver_iterations", U32, [P, U32, U32]),
+    ("rphys_world_set_gravity", U32, [P, Vec3]),
+    ("rphys_body_set_sleep_allowed", U32, [P, U32, U32]),
+    ("rphys_world_step_n", U32, [P, F32, U32]),
+    ("rphys_world_get_stats", U32, [P, C.POINTER(StepStats)]),
+    ("rphys_debug_get_contacts_ex", U32, [P, C.POINTER(ContactEx), U32, PU32]),
+    ("rphys_debug_get_pairs", U32, [P, PU32, U32, PU32]),
+    ("rphys_batch_create", P, [C.POINTER(P), U32]),
+    ("rphys_batch_destroy", None, [P]),
+    ("rphys_batch_step_n", U32, [P, F32, U32]),
+    ("rphys_batch_get_hashes", U32, [P, C.POINTER(C.c_uint8)]),
+    ("rphys_batch_get_state", U32, [P, U32, PF32, PF32, PF32, PF32]),
+    ("rphys_batch_snapshot", U32, [P, U32]),
+    ("rphys_batch_restore", U32, [P, U32]),
+    ("rphys_batch_total_bodies", U32, [P, PU32]),
+    ("rphys_batch_step_graph", U32, [P, F32, U32]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads the library (raises if it is not built: the step has no other implementation)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librphys_amd.so is not built (run __graft_entry__.build() or make -C riftphys-rust_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in EXPORTS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class RPhysError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rphys error %#x: %s" % (code, msg))
+        self.code = code
+
+
+def check(code):
+    if code != OK:
+        m = lib().rphys_last_error_message()
+        raise RPhysError(code, m.decode() if m else "")
+    return code
+
+
+def iso(p, q):
+    return Isometry(Vec3(*[float(x) for x in p]), Quat(*[float(x) for x in q]))
+
+
+def vel(l, a):
+    return Velocity(Vec3(*[float(x) for x in l]), Vec3(*[float(x) for x in a]))
+
+
+class World:
+    """An RPhysWorld, optionally loaded with a scene dict from scenes.py."""
+
+    def __init__(self, scene=None, max_bodies=None, max_colliders=None):
+        L = lib()
+        n = 0 if scene is None else len(scene["shape"])
+        mb = max_bodies if max_bodies is not None else max(n, 1)
+        mc = max_colliders if max_colliders is not None else mb
+        self.h = L.rphys_world_create(mb, mc)
+        if not self.h:
+            raise RPhysError(ERR_INTERNAL, "rphys_world_create failed")
+        self.dt = 1.0 / 60.0
+        if scene is not None:
+            self.load(scene)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rphys_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, s):
+        L = lib()
+        for a, b, ms, mk, e, vs in s.get("pairs", []):
+            check(L.rphys_world_set_pair_props(self.h, a, b, ms, mk, e, vs))
+        vi, pi = s.get("iters", (20, 20))
+        check(L.rphys_world_set_solver_iterations(self.h, vi, pi))
+        for i in range(len(s["shape"])):
+            self.add_body(int(s["shape"][i]), s["dims"][i], s["pose"][i], s["vel"][i], int(s["dynamic"][i]), float(s["density"][i]))
+        hf = s.get("hf")
+        if hf is not None:
+            self.set_heightfield(hf)
+        self.dt = float(s.get("dt", 1.0 / 60.0))
+
+    def add_body(self, shape, dims, pose, v, dynamic=1, density=0.0):
+        d = np.ascontiguousarray(dims, np.float32)
+        out = C.c_uint32(0)
+        check(lib().rphys_add_body_density(self.h, shape, d.ctypes.data_as(PF32), C.byref(iso(pose[:3], pose[3:])),
+                                           C.byref(vel(v[:3], v[3:])), BODY_DYNAMIC if dynamic else BODY_STATIC, density, C.byref(out)))
+        return out.value
+
+    def set_heightfield(self, hf):
+        """terrain.rs via rphys_world_set_heightfield_i16 is lossy for float heights, so the
+        raw32 square entry is used when the grid is square and the cell sizes agree."""
+        hts = np.ascontiguousarray(hf["heights"], np.float32)
+        nx, nz = int(hf["nx"]), int(hf["nz"])
+        if nx != nz:
+            raise ValueError("scene heightfields are square")
+        sx = float(hf["sx"]) * (nx - 1)
+        sz = float(hf["sz"]) * (nz - 1)
+        check(lib().rphys_world_set_heightfield_raw32_square(self.h, hts.ctypes.data, hts.nbytes, sx, sz, float(hf["ox"]), float(hf["oz"]), 0.0))
+
+    def step(self, dt=None):
+        check(lib().rphys_world_step(self.h, self.dt if dt is None else dt))
+
+    def step_n(self, n, dt=None):
+        check(lib().rphys_world_step_n(self.h, self.dt if dt is None else dt, n))
+
+    def hash(self):
+        h = Hash32()
+        check(lib().rphys_world_get_step_hash(self.h, C.byref(h)))
+        return bytes(h.bytes)
+
+    def tick(self):
+        t = C.c_uint64(0)
+        check(lib().rphys_world_tick_index(self.h, C.byref(t)))
+        return t.value
+
+    def num_bodies(self):
+        n = C.c_uint32(0)
+        check(lib().rphys_world_num_bodies(self.h, C.byref(n)))
+        return n.value
+
+    def state(self):
+        n = self.num_bodies()
+        pos = np.zeros((n, 3), np.float32); rot = np.zeros((n, 4), np.float32)
+        lin = np.zeros((n, 3), np.float32); ang = np.zeros((n, 3), np.float32)
+        p = Isometry(); v = Velocity()
+        for i in range(n):
+            if lib().rphys_body_get_pose(self.h, i + 1, C.byref(p)) != OK:
+                continue
+            check(lib().rphys_body_get_velocity(self.h, i + 1, C.byref(v)))
+            pos[i] = (p.pos.x, p.pos.y, p.pos.z); rot[i] = (p.rot.x, p.rot.y, p.rot.z, p.rot.w)
+            lin[i] = (v.lin.x, v.lin.y, v.lin.z); ang[i] = (v.ang.x, v.ang.y, v.ang.z)
+        return pos, rot, lin, ang
+
+    def stats(self):
+        s = StepStats()
+        check(lib().rphys_world_get_stats(self.h, C.byref(s)))
+        return s
+
+    def pairs(self):
+        n = C.c_uint32(0)
+        check(lib().rphys_debug_get_pairs(self.h, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 2), np.uint32)
+        check(lib().rphys_debug_get_pairs(self.h, out.ctypes.data_as(PU32), n.value, C.byref(n)))
+        return out[: n.value]
+
+    def contacts(self):
+        n = C.c_uint32(0)
+        check(lib().rphys_debug_get_contacts_ex(self.h, None, 0, C.byref(n)))
+        arr = (ContactEx * max(n.value, 1))()
+        check(lib().rphys_debug_get_contacts_ex(self.h, arr, n.value, C.byref(n)))
+        return np.ctypeslib.as_array(arr)[: n.value].copy()
+
+    def snapshot(self):
+        s = lib().rphys_world_snapshot_create(self.h)
+        if not s:
+            raise RPhysError(ERR_INTERNAL, "snapshot failed")
+        return s
+
+    def restore(self, snap):
+        check(lib().rphys_world_snapshot_restore(self.h, snap))
+
+    def free_snapshot(self, snap):
+        lib().rphys_world_snapshot_free(snap)
+
+
+class Batch:
+    """Many worlds stepped together (rphys_batch_*)."""
+
+    def __init__(self, worlds):
+        arr = (P * len(worlds))(*[w.h for w in worlds])
+        self.h = lib().rphys_batch_create(arr, len(worlds))
+        if not self.h:
+            m = lib().rphys_last_error_message()
+            raise RPhysError(ERR_INTERNAL, m.decode() if m else "batch create failed")
+        self.n = len(worlds)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rphys_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step_n(self, n, dt=1.0 / 60.0, graph=False):
+        if graph:
+            check(lib().rphys_batch_step_graph(self.h, dt, n))
+        else:
+            check(lib().rphys_batch_step_n(self.h, dt, n))
+
+    def hashes(self):
+        out = np.zeros(32 * self.n, np.uint8)
+        check(lib().rphys_batch_get_hashes(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return [bytes(out[32 * i: 32 * i + 32]) for i in range(self.n)]
+
+    def total_bodies(self):
+        n = C.c_uint32(0)
+        check(lib().rphys_batch_total_bodies(self.h, C.byref(n)))
+        return n.value
+
+    def state(self, wi, n):
+        pos = np.zeros((n, 3), np.float32); rot = np.zeros((n, 4), np.float32)
+        lin = np.zeros((n, 3), np.float32); ang = np.zeros((n, 3), np.float32)
+        check(lib().rphys_batch_get_state(self.h, wi, pos.ctypes.data_as(PF32), rot.ctypes.data_as(PF32), lin.ctypes.data_as(PF32), ang.ctypes.data_as(PF32)))
+        return pos, rot, lin, ang
+
+    def snapshot(self, slot=0):
+        check(lib().rphys_batch_snapshot(self.h, slot))
+
+    def restore(self, slot=0):
+        check(lib().rphys_batch_restore(self.h, slot))

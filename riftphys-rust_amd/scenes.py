@@ -1,0 +1,195 @@
+"""Seeded synthetic scenes for the step workloads (host-side input generation).
+
+All random draws come from XorShift64 as the reference defines it
+(crates/riftphys-core/src/rng.rs:6-39: state = seed | 1, xorshift 12/25/27, output
+(x * 2685821657736338717) >> 32, f32 in [0, 1) from the top 23 bits), in the loop order each
+scene documents, so a scene is a pure function of its seed.  Seeds: scene k, world w ->
+0x5EED0000 + 1000 * k + w.
+
+A scene is a dict of numpy arrays that both the device library (through the C ABI) and the
+CPU oracle consume:
+  shape  int32[N]      0 sphere, 1 box, 2 capsule
+  dims   float32[N,3]  sphere (r,0,0), box (hx,hy,hz), capsule (r,hh,0)
+  pose   float32[N,7]  px,py,pz,qx,qy,qz,qw
+  vel    float32[N,6]
+  dynamic int32[N], density float32[N]
+  hf     dict(nx, nz, sx, sz, ox, oz, heights float32[nz*nx]) (row-major, z rows) or None
+  pairs  list of (mat_a, mat_b, mu_s, mu_k, restitution, stribeck_v) overrides
+  iters  (velocity, position), dt
+"""
+import math
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+MAT_DEFAULT = 0
+MAT_GRIT = 4
+
+
+class XorShift64:
+    """crates/riftphys-core/src/rng.rs:6-39."""
+
+    def __init__(self, seed):
+        self.state = (seed | 1) & MASK64
+
+    def next_u32(self):
+        x = self.state
+        x ^= x >> 12
+        x ^= (x << 25) & MASK64
+        x ^= x >> 27
+        self.state = x
+        return ((x * 2685821657736338717) & MASK64) >> 32
+
+    def next_f32_01(self):
+        u = self.next_u32()
+        bits = (u >> 9) | 0x3F800000
+        return float(np.array([bits], dtype=np.uint32).view(np.float32)[0]) - 1.0
+
+    def uniform(self, lo, hi):
+        """lo + (hi - lo) * u, evaluated in f32."""
+        u = np.float32(self.next_f32_01())
+        return np.float32(np.float32(lo) + np.float32(np.float32(hi) - np.float32(lo)) * u)
+
+
+def seed_for(scene_k, world=0):
+    return 0x5EED0000 + 1000 * scene_k + world
+
+
+def quat_yaw_pitch(yaw, pitch):
+    """q = Ry(yaw) * Rx(pitch) as (x, y, z, w) in f32 (input data, not solver arithmetic)."""
+    cy, sy = math.cos(yaw * 0.5), math.sin(yaw * 0.5)
+    cp, sp = math.cos(pitch * 0.5), math.sin(pitch * 0.5)
+    # Hamilton product (0, sy, 0, cy) * (sp, 0, 0, cp)
+    x = cy * sp
+    y = sy * cp
+    z = -sy * sp
+    w = cy * cp
+    return np.array([x, y, z, w], dtype=np.float32)
+
+
+def fbm_heights(rng, n, cell, octaves=4, base_freq=1.0 / 32.0, lacunarity=2.0, gain=0.5, amplitude=4.0):
+    """Value-noise fBm on an n x n grid (row-major, z rows), normalised to [0, amplitude]."""
+    size = (n - 1) * cell
+    xs = np.arange(n, dtype=np.float64) * cell
+    total = np.zeros((n, n), dtype=np.float64)
+    freq, amp = base_freq, 1.0
+    for _ in range(octaves):
+        m = int(math.ceil(size * freq)) + 2
+        lat = np.array([rng.next_f32_01() for _ in range(m * m)], dtype=np.float64).reshape(m, m)
+        u = xs * freq
+        i0 = np.floor(u).astype(np.int64)
+        f = u - i0
+        f = f * f * (3.0 - 2.0 * f)
+        # bilinear on the smoothstep weights; rows = z, cols = x
+        z0, x0 = i0[:, None], i0[None, :]
+        fz, fx = f[:, None], f[None, :]
+        v00 = lat[z0, x0]
+        v10 = lat[z0, x0 + 1]
+        v01 = lat[z0 + 1, x0]
+        v11 = lat[z0 + 1, x0 + 1]
+        total += amp * ((v00 * (1 - fx) + v10 * fx) * (1 - fz) + (v01 * (1 - fx) + v11 * fx) * fz)
+        freq *= lacunarity
+        amp *= gain
+    total -= total.min()
+    peak = total.max()
+    if peak > 0:
+        total *= amplitude / peak
+    return total.astype(np.float32).reshape(-1)
+
+
+def _empty(n):
+    return dict(
+        shape=np.zeros(n, np.int32),
+        dims=np.zeros((n, 3), np.float32),
+        pose=np.zeros((n, 7), np.float32),
+        vel=np.zeros((n, 6), np.float32),
+        dynamic=np.ones(n, np.int32),
+        density=np.full(n, 1000.0, np.float32),
+    )
+
+
+CONFIG_PAIRS = [(MAT_DEFAULT, MAT_DEFAULT, 0.5, 0.5, 0.0, 0.08), (MAT_DEFAULT, MAT_GRIT, 0.5, 0.5, 0.0, 0.08)]
+
+
+def box_lattice(seed=None, n_side=8, spacing=1.1, half=0.5, hf_n=64):
+    """512 boxes (0.5 m half-extent, 1000 kg/m^3) in an 8x8x8 lattice at 1.1 m, jitter U(+-0.05),
+    random yaw U(-pi, pi), over a flat 64x64 heightfield of 1 m cells.  Draw order: i, j, k,
+    then jx, jy, jz, yaw."""
+    rng = XorShift64(seed_for(1) if seed is None else seed)
+    n = n_side ** 3
+    s = _empty(n)
+    c = (n_side - 1) / 2.0
+    idx = 0
+    for i in range(n_side):
+        for j in range(n_side):
+            for k in range(n_side):
+                jx = rng.uniform(-0.05, 0.05)
+                jy = rng.uniform(-0.05, 0.05)
+                jz = rng.uniform(-0.05, 0.05)
+                yaw = float(rng.uniform(-math.pi, math.pi))
+                p = np.array([32.0 + (i - c) * spacing, 1.05 + spacing * j, 32.0 + (k - c) * spacing], np.float32)
+                p = p + np.array([jx, jy, jz], np.float32)
+                s["shape"][idx] = 1
+                s["dims"][idx] = (half, half, half)
+                s["pose"][idx, :3] = p
+                s["pose"][idx, 3:] = quat_yaw_pitch(yaw, 0.0)
+                idx += 1
+    hts = np.zeros(hf_n * hf_n, np.float32)
+    s["hf"] = dict(nx=hf_n, nz=hf_n, sx=np.float32(63.0 / (hf_n - 1)), sz=np.float32(63.0 / (hf_n - 1)),
+                   ox=0.0, oz=0.0, heights=hts)
+    s["pairs"] = list(CONFIG_PAIRS)
+    s["iters"] = (10, 20)
+    s["dt"] = np.float32(1.0 / 60.0)
+    s["name"] = "box_lattice_512"
+    return s
+
+
+def mixed_shapes(n=16384, seed=None, hf_n=256, cell=0.5, extent=100.0, height=30.0, margin=13.75):
+    """n bodies: i%3==0 spheres r~U(0.2,0.6); 1 boxes he~U(0.2,0.6)^3; 2 capsules r 0.3,
+    hh~U(0.3,0.8); uniform yaw and pitch; x,z ~ U(margin, margin+extent), y ~ U(hmax+1, hmax+1+height),
+    over a 4-octave fBm heightfield (amplitude 4 m).  Terrain is drawn first, then per body:
+    shape dims, x, y, z, yaw, pitch."""
+    rng = XorShift64(seed_for(2) if seed is None else seed)
+    hts = fbm_heights(rng, hf_n, cell)
+    hmax = float(hts.max())
+    s = _empty(n)
+    for i in range(n):
+        kind = i % 3
+        if kind == 0:
+            s["shape"][i] = 0
+            s["dims"][i] = (rng.uniform(0.2, 0.6), 0.0, 0.0)
+        elif kind == 1:
+            s["shape"][i] = 1
+            s["dims"][i] = (rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6))
+        else:
+            s["shape"][i] = 2
+            s["dims"][i] = (0.3, rng.uniform(0.3, 0.8), 0.0)
+        x = rng.uniform(margin, margin + extent)
+        y = rng.uniform(hmax + 1.0, hmax + 1.0 + height)
+        z = rng.uniform(margin, margin + extent)
+        yaw = float(rng.uniform(-math.pi, math.pi))
+        pitch = float(rng.uniform(-math.pi, math.pi))
+        s["pose"][i, :3] = (x, y, z)
+        s["pose"][i, 3:] = quat_yaw_pitch(yaw, pitch)
+    s["hf"] = dict(nx=hf_n, nz=hf_n, sx=np.float32(cell), sz=np.float32(cell), ox=0.0, oz=0.0, heights=hts)
+    s["pairs"] = list(CONFIG_PAIRS)
+    s["iters"] = (10, 20)
+    s["dt"] = np.float32(1.0 / 60.0)
+    s["name"] = "mixed_%d" % n
+    return s
+
+
+def batch_world(w, seed_base_k=3):
+    """One world of the batched workload: N_w = round(exp(U(ln 32, ln 2048))) bodies with the mixed
+    shape mix and fBm terrain, volume scaled to keep the 16,384-body density (same 30 m height)."""
+    rng = XorShift64(seed_for(seed_base_k, w))
+    u = float(rng.next_f32_01())
+    nw = int(round(math.exp(math.log(32.0) + u * (math.log(2048.0) - math.log(32.0)))))
+    extent = 100.0 * math.sqrt(nw / 16384.0)
+    hf_n = min(256, int(math.ceil((extent + 2 * 13.75) / 0.5)) + 1)
+    return mixed_shapes(n=nw, seed=seed_for(seed_base_k, w) ^ 0x9E3779B9, hf_n=hf_n, extent=extent)
+
+
+def small_mixed(n=96, seed=7, hf_n=48):
+    """A few dozen mixed bodies over fBm terrain: the quick parity case."""
+    return mixed_shapes(n=n, seed=seed, hf_n=hf_n, cell=0.5, extent=14.0, height=6.0, margin=4.0)

@@ -1,0 +1,167 @@
+"""ctypes view of the CPU oracle (oracle/liboracle.so).  Test infrastructure only: imported by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class OrStats(C.Structure):
+    _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("islands", C.c_uint32),
+                ("terrain_candidates", C.c_uint32), ("levels", C.c_uint32), ("warm_matched", C.c_uint32),
+                ("iters_used", C.c_uint32), ("residual", C.c_float)]
+
+
+class OrContact(C.Structure):
+    _fields_ = [("a_col", C.c_uint32), ("b_col", C.c_uint32), ("point", C.c_float * 3), ("normal", C.c_float * 3),
+                ("depth", C.c_float), ("impulse", C.c_float), ("feat", C.c_uint32), ("cell", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        FP = C.POINTER(C.c_float)
+        L.or_world_create.restype = P
+        L.or_world_create.argtypes = [C.c_uint32]
+        L.or_world_destroy.argtypes = [P]
+        L.or_set_epoch.argtypes = [P, C.c_uint64]
+        L.or_set_rng_seed.argtypes = [P, C.c_uint64]
+        L.or_set_gravity.argtypes = [P, C.c_float, C.c_float, C.c_float]
+        L.or_set_iterations.argtypes = [P, C.c_uint32, C.c_uint32]
+        L.or_set_residual_mode.argtypes = [P, C.c_int, C.c_float, C.c_uint32]
+        L.or_set_pair_props.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.or_add_body.restype = C.c_int32
+        L.or_add_body.argtypes = [P, C.c_int, FP, FP, FP, C.c_int, C.c_float]
+        L.or_remove_body.argtypes = [P, C.c_uint32]
+        L.or_set_heightfield.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, FP, C.c_uint32]
+        L.or_set_body_pose.argtypes = [P, C.c_uint32, FP]
+        L.or_set_body_vel.argtypes = [P, C.c_uint32, FP]
+        L.or_set_sleep_allowed.argtypes = [P, C.c_uint32, C.c_int]
+        L.or_step.argtypes = [P, C.c_float]
+        L.or_step_hash.argtypes = [P, C.c_char_p]
+        L.or_num_bodies.restype = C.c_uint32
+        L.or_num_bodies.argtypes = [P]
+        L.or_tick.restype = C.c_uint64
+        L.or_tick.argtypes = [P]
+        L.or_get_state.argtypes = [P, FP, FP, FP, FP]
+        L.or_get_stats.argtypes = [P, C.POINTER(OrStats)]
+        L.or_get_pairs.restype = C.c_uint32
+        L.or_get_pairs.argtypes = [P, C.POINTER(C.c_uint32), C.c_uint32]
+        L.or_get_contacts.restype = C.c_uint32
+        L.or_get_contacts.argtypes = [P, C.POINTER(OrContact), C.c_uint32]
+        L.or_get_awake.restype = C.c_uint32
+        L.or_get_awake.argtypes = [P, C.POINTER(C.c_uint8), C.c_uint32]
+        L.or_snapshot.restype = P
+        L.or_snapshot.argtypes = [P]
+        L.or_restore.argtypes = [P, P]
+        L.or_snapshot_free.argtypes = [P]
+        L.or_blake3.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p]
+        L.or_collide.restype = C.c_int
+        L.or_collide.argtypes = [C.c_int, FP, FP, C.c_int, FP, FP, C.c_float, FP]
+        L.or_collide_box_triangle.restype = C.c_int
+        L.or_collide_box_triangle.argtypes = [FP, FP, FP, C.c_float, FP]
+        L.or_collide_sphere_triangle.restype = C.c_int
+        L.or_collide_sphere_triangle.argtypes = [C.c_float, FP, FP, C.c_float, FP]
+        L.or_sincosf.argtypes = [C.c_float, FP, FP]
+        _lib = L
+    return _lib
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def blake3(data: bytes) -> bytes:
+    out = C.create_string_buffer(32)
+    lib().or_blake3(data, len(data), out)
+    return out.raw
+
+
+class OracleWorld:
+    """The oracle world, loaded with a scene dict from scenes.py."""
+
+    def __init__(self, scene=None, capacity=None):
+        L = lib()
+        n = 0 if scene is None else len(scene["shape"])
+        self.cap = capacity if capacity is not None else max(n, 1)
+        self.h = L.or_world_create(self.cap)
+        if scene is not None:
+            self.load(scene)
+
+    def load(self, s):
+        L = lib()
+        for a, b, ms, mk, e, vs in s.get("pairs", []):
+            L.or_set_pair_props(self.h, a, b, ms, mk, e, vs)
+        vi, pi = s.get("iters", (20, 20))
+        L.or_set_iterations(self.h, vi, pi)
+        for i in range(len(s["shape"])):
+            d = np.ascontiguousarray(s["dims"][i], np.float32)
+            p = np.ascontiguousarray(s["pose"][i], np.float32)
+            v = np.ascontiguousarray(s["vel"][i], np.float32)
+            L.or_add_body(self.h, int(s["shape"][i]), fptr(d), fptr(p), fptr(v), int(s["dynamic"][i]), float(s["density"][i]))
+        hf = s.get("hf")
+        if hf is not None:
+            hts = np.ascontiguousarray(hf["heights"], np.float32)
+            # the cell size the reference's raw32 loader derives (lib.rs:1516-1517): size / (dim - 1)
+            n1 = np.float32(hf["nx"] - 1)
+            sx = float(np.float32(np.float32(np.float32(hf["sx"]) * n1) / n1))
+            sz = float(np.float32(np.float32(np.float32(hf["sz"]) * n1) / n1))
+            L.or_set_heightfield(self.h, hf["nx"], hf["nz"], sx, sz, float(hf["ox"]), float(hf["oz"]), fptr(hts), 4)
+        self.dt = float(s.get("dt", 1.0 / 60.0))
+
+    def __del__(self):
+        try:
+            lib().or_world_destroy(self.h)
+        except Exception:
+            pass
+
+    def step(self, dt=None):
+        lib().or_step(self.h, self.dt if dt is None else dt)
+
+    def hash(self):
+        out = C.create_string_buffer(32)
+        lib().or_step_hash(self.h, out)
+        return out.raw
+
+    def state(self):
+        n = lib().or_num_bodies(self.h)
+        pos = np.zeros((n, 3), np.float32); rot = np.zeros((n, 4), np.float32)
+        lin = np.zeros((n, 3), np.float32); ang = np.zeros((n, 3), np.float32)
+        lib().or_get_state(self.h, fptr(pos), fptr(rot), fptr(lin), fptr(ang))
+        return pos, rot, lin, ang
+
+    def stats(self):
+        s = OrStats()
+        lib().or_get_stats(self.h, C.byref(s))
+        return s
+
+    def pairs(self):
+        n = lib().or_get_pairs(self.h, None, 0)
+        out = np.zeros((max(n, 1), 2), np.uint32)
+        lib().or_get_pairs(self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+        return out[:n]
+
+    def contacts(self):
+        n = lib().or_get_contacts(self.h, None, 0)
+        arr = (OrContact * max(n, 1))()
+        lib().or_get_contacts(self.h, arr, n)
+        return np.ctypeslib.as_array(arr)[:n].copy() if n else np.zeros(0, dtype=np.ctypeslib.as_array(arr).dtype)
+
+    def snapshot(self):
+        return lib().or_snapshot(self.h)
+
+    def restore(self, snap):
+        lib().or_restore(self.h, snap)
+
+    def free_snapshot(self, snap):
+        lib().or_snapshot_free(snap)

@@ -1,0 +1,177 @@
+"""The C ABI without a GPU: every symbol include/riftphys_c.h declares is exported, the v1 struct
+layouts match the reference (ffi/src/tests.rs:1416-1432), and the host-side error conventions
+(lib.rs:96-447, 1103-1141) hold.  None of these calls reach the device."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import rphys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "riftphys_c.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b(rphys_[a-z0-9_]+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_every_declared_symbol_is_exported():
+    L = rphys.lib()
+    names = declared_functions()
+    assert len(names) >= 45
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    bound = {e[0] for e in rphys.EXPORTS}
+    assert set(names) == bound, set(names) ^ bound
+
+
+def test_v1_layouts_match_the_reference():
+    prog = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "riftphys_c.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(RPhysVec3), sizeof(RPhysQuat), sizeof(RPhysIsometry),
+         sizeof(RPhysVelocity), sizeof(RPhysHash32), sizeof(RPhysStepParams), sizeof(RPhysHeightfieldDesc),
+         sizeof(RPhysContactEvent), sizeof(RPhysBoxBodyDesc), offsetof(RPhysHeightfieldDesc, heights), sizeof(RPhysMaterial));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(d, "l")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    vec3, quat, iso, vel, h32, sp, hfd, ce, box, hoff, mat = map(int, out)
+    assert (vec3, quat, iso, vel, h32) == (12, 16, 28, 24, 32)
+    assert sp == 20 and hfd == 48 and hoff == 40 and ce == 36 and mat == 12
+    assert box == 28 + 24 + 16 + 4 + 12 + 4
+    assert C.sizeof(rphys.StepParams) == sp and C.sizeof(rphys.HeightfieldDesc) == hfd and C.sizeof(rphys.ContactEvent) == ce
+    assert C.sizeof(rphys.BoxBodyDesc) == box
+
+
+def box_desc(**kw):
+    d = rphys.BoxBodyDesc()
+    d.pose = rphys.iso([0, 1, 0], [0, 0, 0, 1])
+    d.vel = rphys.vel([0, 0, 0], [0, 0, 0])
+    d.hx = d.hy = d.hz = 0.5
+    d.mass = 1.0
+    d.body_type = rphys.BODY_DYNAMIC
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def last_error():
+    m = rphys.lib().rphys_last_error_message()
+    return m.decode() if m else ""
+
+
+def test_abi_version_and_null_handling():
+    L = rphys.lib()
+    assert L.rphys_get_abi_version() == 1
+    out = C.c_uint32(7)
+    assert L.rphys_add_box_body(None, C.byref(box_desc()), C.byref(out)) == rphys.ERR_NULL
+    assert out.value == 0
+    assert L.rphys_world_step_ex(None, None) == rphys.ERR_NULL
+    assert L.rphys_world_step(None, 0.016) == rphys.ERR_NULL
+    h = rphys.Hash32()
+    assert L.rphys_world_get_step_hash(None, C.byref(h)) == rphys.ERR_NULL
+    t = C.c_uint64(5)
+    assert L.rphys_world_tick_index(None, C.byref(t)) == rphys.ERR_NULL and t.value == 0
+
+
+def test_body_ids_validation_and_capacity():
+    L = rphys.lib()
+    w = L.rphys_world_create(2, 2)
+    try:
+        a, b = C.c_uint32(), C.c_uint32()
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(a)) == rphys.OK and a.value == 1
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(b)) == rphys.OK and b.value == 2
+        c = C.c_uint32(9)
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(c)) == rphys.ERR_CAPACITY and c.value == 0
+        assert "capacity" in last_error()
+        p = rphys.Isometry()
+        assert L.rphys_body_get_pose(w, 0, C.byref(p)) == rphys.ERR_INVALID_ID
+        assert p.rot.w == 1.0  # identity written on failure
+        assert L.rphys_body_get_pose(w, rphys.TERRAIN_BODY_ID, C.byref(p)) == rphys.ERR_INVALID_ID
+        assert L.rphys_body_get_pose(w, 3, C.byref(p)) == rphys.ERR_INVALID_ID
+        assert L.rphys_body_get_pose(w, 1, C.byref(p)) == rphys.OK and abs(p.pos.y - 1.0) < 1e-7
+        assert L.rphys_body_remove(w, 1) == rphys.OK
+        assert L.rphys_body_get_pose(w, 1, C.byref(p)) == rphys.ERR_INVALID_ID and "removed" in last_error()
+        n = C.c_uint32()
+        assert L.rphys_world_num_bodies(w, C.byref(n)) == rphys.OK and n.value == 2
+    finally:
+        L.rphys_world_destroy(w)
+
+
+def test_bad_arguments_are_rejected():
+    L = rphys.lib()
+    w = L.rphys_world_create(8, 8)
+    try:
+        out = C.c_uint32()
+        assert L.rphys_add_box_body(w, C.byref(box_desc(hx=-1.0)), C.byref(out)) == rphys.ERR_BAD_ARG
+        assert L.rphys_add_box_body(w, C.byref(box_desc(mass=0.0)), C.byref(out)) == rphys.ERR_BAD_ARG
+        assert L.rphys_add_box_body(w, C.byref(box_desc(body_type=7)), C.byref(out)) == rphys.ERR_BAD_ARG
+        d = box_desc()
+        d.pose.pos.x = float("nan")
+        assert L.rphys_add_box_body(w, C.byref(d), C.byref(out)) == rphys.ERR_BAD_ARG
+        sp = rphys.StepParams(8, 0, 1, 0, 0.016)
+        assert L.rphys_world_step_ex(w, C.byref(sp)) == rphys.ERR_BAD_ARG and "size_bytes" in last_error()
+        sp = rphys.StepParams(20, 0, 0, 0, 0.016)
+        assert L.rphys_world_step_ex(w, C.byref(sp)) == rphys.ERR_BAD_ARG
+        sp = rphys.StepParams(20, 0, 1, 0, -1.0)
+        assert L.rphys_world_step_ex(w, C.byref(sp)) == rphys.ERR_BAD_ARG
+        sp = rphys.StepParams(20, 0, 1, 0, float("inf"))
+        assert L.rphys_world_step_ex(w, C.byref(sp)) == rphys.ERR_BAD_ARG
+        # a static body needs no mass (lib.rs:304-312)
+        assert L.rphys_add_box_body(w, C.byref(box_desc(mass=0.0, body_type=rphys.BODY_STATIC)), C.byref(out)) == rphys.OK
+        bad = np.zeros(3, np.float32)
+        assert L.rphys_world_set_heightfield_raw32_square(w, bad.ctypes.data, 12, 1, 1, 0, 0, 0) == rphys.ERR_BAD_ARG
+        # stepping an empty-of-dynamics world is allowed without a device only when it has no bodies
+        L2 = L.rphys_world_create(4, 4)
+        assert L.rphys_world_step(L2, 0.016) == rphys.OK
+        t = C.c_uint64()
+        assert L.rphys_world_tick_index(L2, C.byref(t)) == rphys.OK and t.value == 1
+        L.rphys_world_destroy(L2)
+    finally:
+        L.rphys_world_destroy(w)
+
+
+def test_last_error_is_thread_local():
+    import threading
+
+    L = rphys.lib()
+    L.rphys_world_step(None, 0.016)
+    seen = []
+
+    def other():
+        seen.append(L.rphys_last_error_message())
+
+    th = threading.Thread(target=other)
+    th.start(); th.join()
+    assert seen == [None]
+    assert last_error() != ""
+    buf = C.create_string_buffer(8)
+    n = C.c_uint32()
+    assert L.rphys_last_error_copy(buf, 8, C.byref(n)) == rphys.OK and n.value == len(last_error())
+    assert buf.value == last_error().encode()[:7]
+
+
+def test_queries_are_declared_out_of_scope():
+    L = rphys.lib()
+    w = L.rphys_world_create(1, 1)
+    hit = (C.c_uint8 * 40)()
+    q = (C.c_uint8 * 32)()
+    assert L.rphys_world_raycast(w, q, hit) == rphys.ERR_INTERNAL
+    assert "not part" in last_error()
+    L.rphys_world_destroy(w)

@@ -186,6 +186,14 @@ RPhysResult rphys_debug_get_contacts_ex(const RPhysWorld *world, RPhysContactEx 
 /* Broadphase pairs (collider indices i < j) of the last tick, in the reference's sorted order. */
 RPhysResult rphys_debug_get_pairs(const RPhysWorld *world, uint32_t *out_ij, uint32_t cap_pairs, uint32_t *out_count);
 
+/* Full carried state of a world: per body pos/rot/lin/ang/last linear velocity/awake/sleep
+ * ticks, and the warm-start cache (keys (body_a << 32 | body_b or 0xFFFFFFFF), body-A-frame
+ * anchors, normal impulses, world friction impulses), for checkpointing and replay tools.
+ * Arrays are per body (3 or 4 floats each); warm arrays hold up to warm_cap entries. */
+RPhysResult rphys_debug_export_state(const RPhysWorld *world, float *pos3, float *rot4, float *lin3, float *ang3, float *last_lin3,
+                                     uint32_t *awake, uint32_t *sleep_ticks, uint64_t *warm_keys, float *warm_anchor3,
+                                     float *warm_jn, float *warm_jt3, uint32_t warm_cap, uint32_t *warm_count);
+
 /* ---- batches: many independent worlds stepped by one launch sequence ---- */
 typedef struct RPhysBatch RPhysBatch;
 /* Builds a batch from n_worlds existing worlds (their bodies, heightfields, settings and carried

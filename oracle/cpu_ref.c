@@ -1269,3 +1269,30 @@ uint32_t or_broadphase(const float *bb, uint32_t n, uint32_t *out, uint32_t cap)
     free(pairs); free(a);
     return np;
 }
+
+/* Loads a full state (e.g. one the device library reached): bodies, sleep state, last linear
+ * velocities, tick and the warm cache (keys (a<<32|b), anchors, jn, jt; sorted by key). */
+void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *lin, const float *ang, const float *last_lin,
+                   const uint32_t *awake, const uint32_t *sticks, uint64_t tick, const uint64_t *wkeys, const float *wla,
+                   const float *wjn, const float *wjt, uint32_t nwarm) {
+    for (uint32_t i = 0; i < w->n; i++) {
+        w->pos[i] = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        w->rot[i] = q4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+        w->lin[i] = v3(lin[3 * i], lin[3 * i + 1], lin[3 * i + 2]);
+        w->ang[i] = v3(ang[3 * i], ang[3 * i + 1], ang[3 * i + 2]);
+        w->last_lin[i] = v3(last_lin[3 * i], last_lin[3 * i + 1], last_lin[3 * i + 2]);
+        w->awake[i] = (uint8_t)awake[i];
+        w->sticks[i] = (uint16_t)sticks[i];
+    }
+    w->tick = tick;
+    free(w->warm);
+    w->warm = malloc(sizeof(Warm) * (nwarm + 1));
+    for (uint32_t k = 0; k < nwarm; k++) {
+        Warm *e = &w->warm[k];
+        e->key[0] = (uint32_t)(wkeys[k] >> 32); e->key[1] = (uint32_t)wkeys[k]; e->key[2] = 0; e->key[3] = 0;
+        e->la = v3(wla[3 * k], wla[3 * k + 1], wla[3 * k + 2]);
+        e->jn = wjn[k];
+        e->jt = v3(wjt[3 * k], wjt[3 * k + 1], wjt[3 * k + 2]);
+    }
+    w->nwarm = nwarm;
+}

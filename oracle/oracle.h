@@ -104,6 +104,9 @@ int or_collide_sphere_triangle(float r, const float *center, const float *tri9, 
                                float *out_points);
 void or_sincosf(float x, float *s, float *c);
 uint32_t or_broadphase(const float *aabbs6, uint32_t n, uint32_t *out_ij, uint32_t cap);
+void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *lin, const float *ang, const float *last_lin,
+                   const uint32_t *awake, const uint32_t *sticks, uint64_t tick, const uint64_t *wkeys, const float *wla,
+                   const float *wjn, const float *wjt, uint32_t nwarm);
 
 #ifdef __cplusplus
 }

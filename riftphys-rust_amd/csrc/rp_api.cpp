@@ -26,7 +26,7 @@ struct HashDev {
     uint8_t *digest;
     uint32_t cap_chunks;
 };
-extern "C" void rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st);
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uint32_t max_extra_slots, hipStream_t st);
 
@@ -369,8 +369,9 @@ static bool capture_graph(Scene &S, const TickParams &tp0, uint32_t parity) {
     tp.cur = parity;
     hipGraph_t g = nullptr;
     if (!hip_ok(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
-    rp_launch_tick(S.d, tp, S.island_grid, S.st);
+    hipError_t le = rp_launch_tick(S.d, tp, S.island_grid, S.st);
     if (!hip_ok(hipStreamEndCapture(S.st, &g), "hipStreamEndCapture")) return false;
+    if (!hip_ok(le, "kernel launch (capture)")) { (void)hipGraphDestroy(g); return false; }
     bool ok = hip_ok(hipGraphInstantiate(&S.graph[parity], g, nullptr, nullptr, 0), "hipGraphInstantiate");
     (void)hipGraphDestroy(g);
     return ok;
@@ -393,7 +394,7 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         } else {
             for (uint32_t k = 0; k < remaining; k++) {
                 tp.cur = (S->cur + k) & 1u;
-                rp_launch_tick(S->d, tp, S->island_grid, S->st);
+                if (!hip_ok(rp_launch_tick(S->d, tp, S->island_grid, S->st), "kernel launch")) return false;
             }
         }
         uint32_t cnt[CT_COUNT];
@@ -1112,6 +1113,35 @@ RPhysResult rphys_debug_get_pairs(const RPhysWorld *cworld, uint32_t *out_ij, ui
     if (m && (!hip_ok(hipMemcpy(a.data(), S.d.pair_i, 4 * m, hipMemcpyDeviceToHost), "pairs") ||
               !hip_ok(hipMemcpy(b.data(), S.d.pair_j, 4 * m, hipMemcpyDeviceToHost), "pairs"))) return RPHYS_ERR_INTERNAL;
     for (uint32_t i = 0; i < m; i++) { out_ij[2 * i] = a[i]; out_ij[2 * i + 1] = b[i]; }
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_debug_export_state(const RPhysWorld *cworld, float *pos3, float *rot4, float *lin3, float *ang3, float *last3, uint32_t *awake,
+                                     uint32_t *sticks, uint64_t *wkeys, float *wla3, float *wjn, float *wjt3, uint32_t warm_cap, uint32_t *warm_count) {
+    GUARD_BEGIN
+    if (!cworld || !warm_count) { set_err("rphys_debug_export_state: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    const HostWorld &w = world->w;
+    for (uint32_t i = 0; i < w.nb(); i++) {
+        if (pos3) { pos3[3 * i] = w.pos[i].x; pos3[3 * i + 1] = w.pos[i].y; pos3[3 * i + 2] = w.pos[i].z; }
+        if (rot4) { rot4[4 * i] = w.rot[i].x; rot4[4 * i + 1] = w.rot[i].y; rot4[4 * i + 2] = w.rot[i].z; rot4[4 * i + 3] = w.rot[i].w; }
+        if (lin3) { lin3[3 * i] = w.lin[i].x; lin3[3 * i + 1] = w.lin[i].y; lin3[3 * i + 2] = w.lin[i].z; }
+        if (ang3) { ang3[3 * i] = w.ang[i].x; ang3[3 * i + 1] = w.ang[i].y; ang3[3 * i + 2] = w.ang[i].z; }
+        if (last3) { last3[3 * i] = w.last_lin[i].x; last3[3 * i + 1] = w.last_lin[i].y; last3[3 * i + 2] = w.last_lin[i].z; }
+        if (awake) awake[i] = w.awake[i];
+        if (sticks) sticks[i] = w.sticks[i];
+    }
+    uint32_t n = (uint32_t)w.wkey.size();
+    *warm_count = n;
+    for (uint32_t k = 0; k < n && k < warm_cap; k++) {
+        if (wkeys) wkeys[k] = w.wkey[k];
+        if (wla3) { wla3[3 * k] = w.wla[k].x; wla3[3 * k + 1] = w.wla[k].y; wla3[3 * k + 2] = w.wla[k].z; }
+        if (wjn) wjn[k] = w.wjn[k];
+        if (wjt3) { wjt3[3 * k] = w.wjt[k].x; wjt3[3 * k + 1] = w.wjt[k].y; wjt3[3 * k + 2] = w.wjt[k].z; }
+    }
     return RPHYS_OK;
     GUARD_END
 }

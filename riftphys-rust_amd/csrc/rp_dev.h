@@ -9,7 +9,7 @@
 #define RP_NONE 0xFFFFFFFFu
 #define RP_TERRAIN 0xFFFFFFFFu
 #define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase
-#define RP_ISLAND_LDS_MAX 4096u       // islands up to this many contacts keep their chains in LDS
+#define RP_ISLAND_LDS_MAX 2048u       // islands up to this many contacts keep their chains in LDS (64 KiB)
 #define RP_ISLAND_MAX (1u << 20)      // position bits of the shock-order key
 
 enum { BF_DYN = 1u, BF_ALIVE = 2u, BF_SLEEP_OK = 4u };

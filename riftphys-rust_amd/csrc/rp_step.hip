@@ -642,21 +642,31 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
 }
 
 // ------------------------------------------------------------------ ordered sweeps
-// Chain storage of the island being processed: LDS for islands up to RP_ISLAND_LDS_MAX
-// contacts, else a global slice owned by the island (its list positions are its own).
-struct Chains { uint64_t *ent; uint32_t *order, *pa, *pb, *done; };
+// Chain storage of the island being processed.  Each sweep kernel has two instantiations: islands
+// of up to RP_ISLAND_LDS_MAX contacts keep their chains in LDS, larger ones in a global slice owned
+// by the island (its list positions are its own).  The storage's address space is part of the
+// pointer types, so LDS is only ever reached by ds_* instructions: flat (generic) accesses into
+// the LDS aperture beyond its first 64 KiB fault on gfx950.
+#define RP_LDS __attribute__((address_space(3)))
+template <class P64, class P32> struct ChainsT { P64 ent; P32 order, pa, pb, done; };
+using LdsChains = ChainsT<RP_LDS uint64_t *, RP_LDS uint32_t *>;
+using GlbChains = ChainsT<uint64_t *, uint32_t *>;
 struct IslandLds {
     uint64_t ent[2 * RP_ISLAND_LDS_MAX];
     uint32_t order[RP_ISLAND_LDS_MAX], pa[RP_ISLAND_LDS_MAX], pb[RP_ISLAND_LDS_MAX], done[RP_ISLAND_LDS_MAX];
 };
-static __device__ __forceinline__ Chains chains_for(const Dev &d, IslandLds &L, uint32_t s, uint32_t n) {
-    Chains c;
-    if (n <= RP_ISLAND_LDS_MAX) { c.ent = L.ent; c.order = L.order; c.pa = L.pa; c.pb = L.pb; c.done = L.done; }
-    else { c.ent = d.g_ent + 4ull * s; c.order = d.g_order + s; c.pa = d.g_pa + s; c.pb = d.g_pb + s; c.done = d.g_done + s; }
-    return c;
+static_assert(sizeof(IslandLds) <= 65536, "island LDS block must leave room for two workgroups per CU");
+template <bool USE_LDS>
+static __device__ __forceinline__ auto chains_for(const Dev &d, IslandLds &L, uint32_t s) {
+    if constexpr (USE_LDS)
+        return LdsChains{(RP_LDS uint64_t *)L.ent, (RP_LDS uint32_t *)L.order, (RP_LDS uint32_t *)L.pa, (RP_LDS uint32_t *)L.pb,
+                         (RP_LDS uint32_t *)L.done};
+    else
+        return GlbChains{d.g_ent + 4ull * s, d.g_order + s, d.g_pa + s, d.g_pb + s, d.g_done + s};
 }
 
-static __device__ void bitonic_u64(uint64_t *a, uint32_t m) {
+template <class P64>
+static __device__ void bitonic_u64(P64 a, uint32_t m) {
     for (uint32_t k = 2; k <= m; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
@@ -675,7 +685,8 @@ static __device__ void bitonic_u64(uint64_t *a, uint32_t m) {
 // Predecessor chains for the sweep order C.order (positions 0..n-1 of the island list): for
 // each position and each written body, the previous position in the order writing the same body
 // (stable sort of (body, position, side) entries).
-static __device__ void build_chains(const Dev &d, const Chains &C, const uint32_t *list, uint32_t n) {
+template <class Ch>
+static __device__ void build_chains(const Dev &d, const Ch &C, const uint32_t *list, uint32_t n) {
     uint32_t m = 1;
     while (m < 2 * n) m <<= 1;
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
@@ -707,8 +718,8 @@ static __device__ void build_chains(const Dev &d, const Chains &C, const uint32_
 // Dataflow execution of one sweep: each lane walks its positions in increasing order and runs
 // one when both predecessors are done in this sweep (the lowest unfinished position is always
 // ready, so every lane finishes).
-template <class F>
-static __device__ void run_sweep(const Chains &C, uint32_t n, uint32_t token, F exec) {
+template <class Ch, class F>
+static __device__ void run_sweep(const Ch &C, uint32_t n, uint32_t token, F exec) {
     uint32_t r = threadIdx.x;
     while (r < n) {
         uint32_t a = C.pa[r], b = C.pb[r];
@@ -735,8 +746,8 @@ static __device__ __forceinline__ uint64_t shock_key(float h, uint32_t r) {
 }
 
 // Sets C.order to the shock order of the island list from per-position heights.
-template <class H>
-static __device__ void shock_order(const Chains &C, uint32_t n, H height) {
+template <class Ch, class H>
+static __device__ void shock_order(const Ch &C, uint32_t n, H height) {
     uint32_t m = 1;
     while (m < n) m <<= 1;
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) C.ent[i] = i < n ? shock_key(height(i), i) : ~0ull;
@@ -774,15 +785,17 @@ static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
     return mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
 }
 
-// The island's segment of the sorted list and its chain storage.
+// The island's segment of the sorted list and its chain storage; an instantiation skips the
+// islands of the other storage class.
 #define ISLAND_LOOP_BEGIN                                                                      \
     __shared__ IslandLds L;                                                                    \
     const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = d.cnt[CT_ISLC];                           \
     for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {                            \
         const uint32_t s = d.isl_start[isl], n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - s; \
+        if ((n <= RP_ISLAND_LDS_MAX) != USE_LDS) continue;                                     \
         if (n > RP_ISLAND_MAX) { if (threadIdx.x == 0) atomicMax(&d.cnt[CT_BIG_ISLAND], n); continue; } \
         const uint32_t *lst = d.isl_v2 + s;                                                    \
-        const Chains C = chains_for(d, L, s, n);                                               \
+        const auto C = chains_for<USE_LDS>(d, L, s);                                           \
         for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) C.order[r] = r;                 \
         __threadfence_block();                                                                 \
         __syncthreads();                                                                       \
@@ -791,6 +804,7 @@ static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
 
 // Velocity solve of one island per workgroup: warm-start application, PGS iterations, the
 // pre-shock impulse snapshot (what the warm cache stores), then the shock pass.
+template <bool USE_LDS>
 __global__ void __launch_bounds__(256) k_sweep_vel(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t cur = tp.cur;
@@ -989,6 +1003,7 @@ __global__ void k_ball_init(Dev d) {
 
 // step.rs:180-206 in contact order over each island; only awake dynamic bodies' flags can change
 // an outcome, and those are chained.
+template <bool USE_LDS>
 __global__ void __launch_bounds__(256) k_sweep_ball(Dev d, TickParams tp) {
     if (aborted(d)) return;
     ISLAND_LOOP_BEGIN
@@ -1136,6 +1151,7 @@ static __device__ void project_one(const Dev &d, uint32_t c, bool shock, V3 up) 
     }
 }
 
+template <bool USE_LDS>
 __global__ void __launch_bounds__(256) k_sweep_proj(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
@@ -1253,8 +1269,10 @@ static void radix_sort(const Dev &d, uint32_t *k0, uint32_t *v0, uint32_t *k1, u
     *k_out = ka; *v_out = va;
 }
 
-extern "C" void rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st) {
+// Enqueues one tick; returns the launch status (hipGetLastError after the last launch).
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st) {
     const uint32_t B = 256;
+    const uint32_t big_grid = island_grid < 256u ? island_grid : 256u;  // islands above the LDS size are few
     uint32_t *scratch_total = d.cnt + CT_SCRATCH;
     hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
@@ -1288,7 +1306,8 @@ extern "C" void rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t isla
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_BIG_ISLAND, 0u);
-    hipLaunchKernelGGL(k_sweep_vel, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_vel<true>, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_vel<false>, dim3(big_grid), dim3(256), 0, st, d, tp);
     // warm cache for the next tick (into buffer cur ^ 1)
     hipLaunchKernelGGL(k_warm_newflags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     scan_u32(d.newflag, d.cap_contacts, d.scan_tmp, d.cnt + CT_NEW_TOTAL, st);
@@ -1300,13 +1319,16 @@ extern "C" void rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t isla
     hipLaunchKernelGGL(k_warm_merge, dim3(grid_for(d.cap_contacts > d.cap_warm ? d.cap_contacts : d.cap_warm, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
-    hipLaunchKernelGGL(k_sweep_ball, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_ball<true>, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_ball<false>, dim3(big_grid), dim3(256), 0, st, d, tp);
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
-    hipLaunchKernelGGL(k_sweep_proj, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_proj<true>, dim3(island_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_proj<false>, dim3(big_grid), dim3(256), 0, st, d, tp);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);
     (void)scratch_total;
+    return hipGetLastError();
 }
 
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st) {

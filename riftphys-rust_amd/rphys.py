@@ -131,6 +131,7 @@ EXPORTS = [
     ("rphys_world_get_stats", U32, [P, C.POINTER(StepStats)]),
     ("rphys_debug_get_contacts_ex", U32, [P, C.POINTER(ContactEx), U32, PU32]),
     ("rphys_debug_get_pairs", U32, [P, PU32, U32, PU32]),
+    ("rphys_debug_export_state", U32, [P, PF32, PF32, PF32, PF32, PF32, PU32, PU32, C.POINTER(C.c_uint64), PF32, PF32, PF32, U32, PU32]),
     ("rphys_batch_create", P, [C.POINTER(P), U32]),
     ("rphys_batch_destroy", None, [P]),
     ("rphys_batch_step_n", U32, [P, F32, U32]),
@@ -290,6 +291,26 @@ class World:
         arr = (ContactEx * max(n.value, 1))()
         check(lib().rphys_debug_get_contacts_ex(self.h, arr, n.value, C.byref(n)))
         return np.ctypeslib.as_array(arr)[: n.value].copy()
+
+    def export_state(self):
+        """Bodies, sleep state, last velocities and the warm cache (rphys_debug_export_state)."""
+        n = self.num_bodies()
+        cnt = C.c_uint32(0)
+        check(lib().rphys_debug_export_state(self.h, None, None, None, None, None, None, None, None, None, None, None, 0, C.byref(cnt)))
+        m = cnt.value
+        st = dict(pos=np.zeros((n, 3), np.float32), rot=np.zeros((n, 4), np.float32), lin=np.zeros((n, 3), np.float32),
+                  ang=np.zeros((n, 3), np.float32), last_lin=np.zeros((n, 3), np.float32), awake=np.zeros(n, np.uint32),
+                  sticks=np.zeros(n, np.uint32), wkeys=np.zeros(max(m, 1), np.uint64), wla=np.zeros((max(m, 1), 3), np.float32),
+                  wjn=np.zeros(max(m, 1), np.float32), wjt=np.zeros((max(m, 1), 3), np.float32))
+        f = lambda a: a.ctypes.data_as(PF32)
+        u = lambda a: a.ctypes.data_as(PU32)
+        check(lib().rphys_debug_export_state(self.h, f(st["pos"]), f(st["rot"]), f(st["lin"]), f(st["ang"]), f(st["last_lin"]),
+                                             u(st["awake"]), u(st["sticks"]), st["wkeys"].ctypes.data_as(C.POINTER(C.c_uint64)),
+                                             f(st["wla"]), f(st["wjn"]), f(st["wjt"]), m, C.byref(cnt)))
+        for k in ("wkeys", "wla", "wjn", "wjt"):
+            st[k] = st[k][:m]
+        st["tick"] = self.tick()
+        return st
 
     def snapshot(self):
         s = lib().rphys_world_snapshot_create(self.h)

@@ -73,6 +73,10 @@ def lib():
         L.or_collide_sphere_triangle.restype = C.c_int
         L.or_collide_sphere_triangle.argtypes = [C.c_float, FP, FP, C.c_float, FP]
         L.or_sincosf.argtypes = [C.c_float, FP, FP]
+        U = C.POINTER(C.c_uint32)
+        L.or_load_state.argtypes = [P, FP, FP, FP, FP, FP, U, U, C.c_uint64, C.POINTER(C.c_uint64), FP, FP, FP, C.c_uint32]
+        L.or_broadphase.restype = C.c_uint32
+        L.or_broadphase.argtypes = [FP, C.c_uint32, U, C.c_uint32]
         _lib = L
     return _lib
 
@@ -156,6 +160,17 @@ class OracleWorld:
         arr = (OrContact * max(n, 1))()
         lib().or_get_contacts(self.h, arr, n)
         return np.ctypeslib.as_array(arr)[:n].copy() if n else np.zeros(0, dtype=np.ctypeslib.as_array(arr).dtype)
+
+    def load_state(self, st):
+        """Continues from a state exported by the device library (rphys.World.export_state)."""
+        c = lambda a, t=np.float32: np.ascontiguousarray(a, t)
+        pos, rot, lin, ang, last = c(st["pos"]), c(st["rot"]), c(st["lin"]), c(st["ang"]), c(st["last_lin"])
+        awake, sticks = c(st["awake"], np.uint32), c(st["sticks"], np.uint32)
+        wk, wla, wjn, wjt = c(st["wkeys"], np.uint64), c(st["wla"]), c(st["wjn"]), c(st["wjt"])
+        U = C.POINTER(C.c_uint32)
+        lib().or_load_state(self.h, fptr(pos), fptr(rot), fptr(lin), fptr(ang), fptr(last), awake.ctypes.data_as(U),
+                            sticks.ctypes.data_as(U), int(st["tick"]), wk.ctypes.data_as(C.POINTER(C.c_uint64)), fptr(wla),
+                            fptr(wjn), fptr(wjt), len(wk))
 
     def snapshot(self):
         return lib().or_snapshot(self.h)

@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Step-throughput bench of the device step library (one MI355X).
+
+Workload: the plain method on GPU, one world of 16,384 bodies (1/3 spheres, 1/3 boxes, 1/3
+capsules) spawned over a 256x256, 0.5 m fBm heightfield; dt = 1/60, 10 velocity and 20 position
+iterations, mu = 0.5, e = 0.  A step is one tick of the whole step (broadphase, narrowphase,
+contact solve, integration, projection, sleep) through the C ABI; the scene is resident in HBM
+before the timed region starts.
+
+Prints one JSON line: body-steps/s over the timed ticks, plus
+  roofline     - the dominant kernel's algorithmic bytes per launch over its mean launch time,
+                 both measured live with HIP events on the library's stream during the timed ticks;
+  cpu_baseline - the oracle (single-threaded C restatement of the reference step) timed on this
+                 host on a bounded, stratified sample of the same ticks: a few oracle ticks
+                 continued from the device state at the start of every 100-tick chunk of the timed
+                 region, each chunk's CPU time extrapolated from its sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "riftphys-rust_amd"))
+
+import rphys  # noqa: E402
+import scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+# Algorithmic bytes per unit of each single-kernel stage: every input read once and every output
+# written once (f32 SoA constants).
+#   velocity solve : per island contact, Iv row-iterations x 260 B (constraint 140, two bodies'
+#                    velocities read+written 96, accumulators 24) + one shock row 230 B
+#   projection     : per island contact, Ip x 230 B (PosContact 60, two poses read+written 112,
+#                    inverse mass/inertia 56); its shock pass runs in a launch of its own
+#   body-pair narrowphase : per pair, 88 B (two poses + shapes)
+#   terrain narrowphase   : per (collider, triangle) candidate, 16 B (height samples)
+
+
+def stage_bytes(stage, ctr, iv, ip):
+    c_large = ctr["island_contacts_global"]  # islands of 4,097..131,072 contacts
+    c_small = ctr["island_contacts"] - c_large  # (islands above 131,072 contacts: none at these sizes)
+    table = {
+        "vel_solve_small": (iv * 260 + 230) * c_small,
+        "vel_solve_large": (iv * 260 + 230) * c_large,
+        "projection_small": ip * 230 * c_small,
+        "projection_large": ip * 230 * c_large,
+        "narrow_pairs": 88 * ctr["pairs"],
+        "narrow_terrain": 16 * ctr["terrain_candidates"],
+    }
+    return table.get(stage)
+
+
+KERNEL_OF_STAGE = {
+    "vel_solve_small": "k_sweep_vel<0>",
+    "vel_solve_large": "k_sweep_vel<1>",
+    "projection_small": "k_sweep_proj<0>",
+    "projection_large": "k_sweep_proj<1>",
+    "narrow_pairs": "k_narrow_pairs",
+    "narrow_terrain": "k_narrow_terrain",
+}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, when one exists."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == workload and f[1] == kernel:
+            try:
+                return float(f[2])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, strata, ticks_per_stratum, dt):
+    """strata: (device state, chunk length) per chunk of the timed region.  Runs the oracle for a
+    few ticks from each state and extrapolates each chunk's CPU time from its mean tick time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+
+    n = len(strata[0][0]["pos"])
+    est, sampled, spent = 0.0, 0, 0.0
+    for state, length in strata:
+        o = oracle_py.OracleWorld(scene)
+        o.load_state(state)
+        t0 = time.perf_counter()
+        for _ in range(ticks_per_stratum):
+            o.step(dt)
+        el = time.perf_counter() - t0
+        est += el / ticks_per_stratum * length
+        sampled += ticks_per_stratum
+        spent += el
+        del o
+    total_ticks = sum(length for _, length in strata)
+    return {
+        "value": n * total_ticks / est,
+        "unit": "body-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "%d oracle ticks: %d continued from the device state at the start of each of the %d chunks of the timed ticks, "
+        "each chunk's time extrapolated from its sample (%.1f s measured, %.0f s estimated for all %d ticks; %s, %d host cores, 1 used)"
+        % (sampled, ticks_per_stratum, len(strata), spent, est, total_ticks, cpu_model(), os.cpu_count() or 0),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=["mixed16k", "boxes512"], default="mixed16k")
+    ap.add_argument("--cpu-ticks", type=int, default=3, help="oracle ticks sampled per 100-tick chunk")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle sample (profiling runs)")
+    ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the step library has no multi-GPU path (independent worlds would shard by "
+              "world across GPUs with no per-tick exchange). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    if a.workload == "mixed16k":
+        scene = scenes.mixed_shapes()
+        wl = {"workload": "mixed16k_fbm256", "bodies": 16384, "shapes": "1/3 sphere, 1/3 box, 1/3 capsule",
+              "heightfield": "256x256 fBm, 0.5 m", "dt": "1/60", "vel_iters": 10, "pos_iters": 20, "worlds": 1}
+    else:
+        scene = scenes.box_lattice()
+        wl = {"workload": "boxes512_flat64", "bodies": 512, "heightfield": "64x64 flat, 1 m", "dt": "1/60",
+              "vel_iters": 10, "pos_iters": 20, "worlds": 1}
+    dt = float(scene.get("dt", 1.0 / 60.0))
+    iv, ip = scene.get("iters", (10, 20))
+
+    w = rphys.World(scene)
+    nb = w.num_bodies()
+    print("bench: %s, %d bodies, warmup %d ticks" % (wl["workload"], nb, a.warmup), file=sys.stderr, flush=True)
+    if a.warmup > 0:
+        w.step_n(a.warmup, dt)
+    w.hash()  # forces completion of the warmup ticks before the clock starts
+
+    # timed region: the profiled step in chunks of up to 100 ticks, a progress line after each
+    # (the CPU sample's device states are exported between chunks, off the clock)
+    stages, ctr, strata = {}, {}, []
+    done, el = 0, 0.0
+    while done < a.steps:
+        k = min(100, a.steps - done)
+        if not a.no_cpu:
+            strata.append((w.export_state(), k))
+        t0 = time.perf_counter()
+        st, c = w.step_profiled(k, dt)
+        el += time.perf_counter() - t0
+        for key, v in st.items():
+            stages[key] = stages.get(key, 0.0) + v
+        for key, v in c.items():
+            if isinstance(v, dict):
+                acc = ctr.setdefault(key, {})
+                for k2, v2 in v.items():
+                    acc[k2] = acc.get(k2, 0.0) + v2
+            else:
+                ctr[key] = ctr.get(key, 0) + v
+        done += k
+        print("bench: %d/%d ticks, %.2f s" % (done, a.steps, el), file=sys.stderr, flush=True)
+    value = nb * a.steps / el
+
+    # dominant single-kernel stage
+    cand = {k: v for k, v in stages.items() if k in KERNEL_OF_STAGE}
+    dom = max(cand, key=cand.get)
+    avg_ms = cand[dom] / max(ctr["ticks"], 1)
+    bytes_total = stage_bytes(dom, ctr, iv, ip)
+    bytes_launch = bytes_total / max(ctr["ticks"], 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = measured_traffic(KERNEL_OF_STAGE[dom], wl["workload"])
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic, "kernel": KERNEL_OF_STAGE[dom], "algorithmic_bytes_per_launch": bytes_launch,
+            "mean_launch_ms": avg_ms}
+
+    if a.stages:
+        tot = sum(stages.values())
+        print("stage split over %d ticks (ms/tick):" % ctr["ticks"], file=sys.stderr)
+        for k, v in stages.items():
+            print("  %-18s %8.3f  %5.1f%%" % (k, v / ctr["ticks"], 100.0 * v / tot), file=sys.stderr)
+        print("counters per tick: " + ", ".join("%s %.0f" % (k, v / ctr["ticks"]) for k, v in ctr.items()
+                                                 if k not in ("ticks", "sweep_phase")), file=sys.stderr)
+        print("large-island sweep phases per tick (ms; levels): " + ", ".join(
+            "%s %.3f" % (k, v / ctr["ticks"]) for k, v in ctr["sweep_phase"].items()), file=sys.stderr)
+
+    if not a.no_cpu:
+        print("bench: oracle sample, %d ticks from each of %d states" % (a.cpu_ticks, len(strata)), file=sys.stderr, flush=True)
+    base = None if a.no_cpu else cpu_baseline(scene, strata, a.cpu_ticks, dt)
+    wl.update({"pairs_per_tick": ctr["pairs"] / ctr["ticks"], "contacts_per_tick": ctr["contacts"] / ctr["ticks"],
+               "terrain_candidates_per_tick": ctr["terrain_candidates"] / ctr["ticks"], "ticks": "%d..%d" % (a.warmup, a.warmup + a.steps)})
+    out = {
+        "metric": "body-steps/s on 1 GPU (plain method, single world)",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1000.0 * el / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded XorShift64 scene)",
+        "config": wl,
+        "roofline": roof,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

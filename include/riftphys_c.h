@@ -170,6 +170,26 @@ typedef struct RPhysStepStats {
 } RPhysStepStats;
 RPhysResult rphys_world_get_stats(const RPhysWorld *world, RPhysStepStats *out);
 
+/* Profiled stepping (build addition; the reference has no counterpart): runs n ticks on the
+ * library's stream with HIP events at the stage boundaries of every tick and returns the stage
+ * times and the tick counters summed over the ticks.  Stage k's name comes from
+ * rphys_profile_stage_name(k).  Same results as rphys_world_step_n. */
+#define RPHYS_PROFILE_MAX_STAGES 24
+typedef struct RPhysProfile {
+    uint32_t ticks;
+    uint32_t n_stages;
+    double stage_ms[RPHYS_PROFILE_MAX_STAGES];
+    double pairs, terrain_candidates, contacts, island_contacts, islands;
+    double island_contacts_global; /* contacts of the islands solved by the 1024-thread LDS-flag tier */
+    /* Inside the sweeps of the islands larger than 4,096 contacts (summed over their workgroups):
+     * ms in [0] velocity level order, [1] warm start, [2] PGS iterations, [3] velocity shock order,
+     * [4] velocity shock pass, [5] ballistic pass, [6] projection iterations, [7] projection shock
+     * order, [8] projection shock pass; [9] / [10] levels of the list / shock orders, summed. */
+    double sweep_phase[11];
+} RPhysProfile;
+RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out);
+const char *rphys_profile_stage_name(uint32_t stage);
+
 /* One contact with its build-defined feature id (kind<<24 | which<<16 | k, bit 23 = swapped
  * shapes) and terrain cell ((gz*(nx-1)+gx)*2+tri; 0xFFFFFFFF for body pairs).  Colliders are
  * 0-based collider indices (the reference's Contact.a_collider / b_collider). */

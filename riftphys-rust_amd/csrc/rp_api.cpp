@@ -26,7 +26,7 @@ struct HashDev {
     uint8_t *digest;
     uint32_t cap_chunks;
 };
-extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st);
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st, hipEvent_t *ev);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uint32_t max_extra_slots, hipStream_t st);
 
@@ -228,8 +228,12 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.cs_ra, C) && B.alloc(d.cs_rb, C) && B.alloc(d.cs_n, C) && B.alloc(d.cs_t1, C) && B.alloc(d.cs_t2, C) && B.alloc(d.cs_m, C) && B.alloc(d.cs_mu, C);
     ok &= B.alloc(d.jn, C) && B.alloc(d.jn_pre, C) && B.alloc(d.jt, C) && B.alloc(d.ct_key, C) && B.alloc(d.ct_anchor, C) && B.alloc(d.warm_idx, C);
     ok &= B.alloc(d.isl_k, C) && B.alloc(d.isl_v, C) && B.alloc(d.isl_k2, C) && B.alloc(d.isl_v2, C) && B.alloc(d.isl_start, C) && B.alloc(d.isl_flag, C + 1);
-    ok &= B.alloc(d.g_ent, 4 * C) && B.alloc(d.g_order, C) && B.alloc(d.g_pa, C) && B.alloc(d.g_pb, C) && B.alloc(d.g_done, C);
-    d.rs_blocks = (uint32_t)((C + 1023) / 1024);
+    ok &= B.alloc(d.ch_pa, C) && B.alloc(d.ch_pb, C) && B.alloc(d.sh_ord, C) && B.alloc(d.sh_pa, C) && B.alloc(d.sh_pb, C) && B.alloc(d.g_done, C);
+    ok &= B.alloc(d.ch_sa, C) && B.alloc(d.ch_sb, C) && B.alloc(d.sh_sa, C) && B.alloc(d.sh_sb, C) && B.alloc(d.ex_base, C) && B.alloc(d.ex_shock, C);
+    ok &= B.alloc(d.bs_pa, C) && B.alloc(d.bs_pb, C) && B.alloc(d.bs_c, C) && B.alloc(d.ss_pa, C) && B.alloc(d.ss_pb, C) && B.alloc(d.ss_c, C);
+    ok &= B.alloc(d.slot_of, C) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
+    ok &= B.alloc(d.sk0, 2 * C) && B.alloc(d.sv0, 2 * C) && B.alloc(d.sk1, 2 * C) && B.alloc(d.sv1, 2 * C) && B.alloc(d.sh_hi, C);
+    d.rs_blocks = (uint32_t)((2 * C + 1023) / 1024);
     ok &= B.alloc(d.rs_hist, 256 * (size_t)d.rs_blocks + 1);
     ok &= B.alloc(d.pc_la, C) && B.alloc(d.pc_lb, C);
     for (int k = 0; k < 2; k++) ok &= B.alloc(d.wk_key[k], W) && B.alloc(d.wk_la[k], W) && B.alloc(d.wk_jn[k], W) && B.alloc(d.wk_jt[k], W) && B.alloc(d.wc_off[k], nw + 1);
@@ -351,14 +355,15 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
 
 // Runs n ticks on the scene.  On a capacity overflow the failed tick is rolled back, the scene is
 // rebuilt with larger buffers (through the host mirrors) and stepping resumes.
-static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n, bool use_graph);
+struct ProfileAcc;
+static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n, bool use_graph, ProfileAcc *prof = nullptr);
 
 static Caps initial_caps(const std::vector<HostWorld *> &ws) {
     uint64_t nc = 0;
     for (auto *w : ws) nc += w->nc();
     Caps c;
     c.pairs = (uint32_t)std::min<uint64_t>(16 * nc + 1024, 0x7FFFFFFF);
-    c.cands = (uint32_t)std::min<uint64_t>(64 * nc + 4096, 0x7FFFFFFF);
+    c.cands = (uint32_t)std::min<uint64_t>(96 * nc + 4096, 0x7FFFFFFF);
     c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
     c.warm = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
     return c;
@@ -369,7 +374,7 @@ static bool capture_graph(Scene &S, const TickParams &tp0, uint32_t parity) {
     tp.cur = parity;
     hipGraph_t g = nullptr;
     if (!hip_ok(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
-    hipError_t le = rp_launch_tick(S.d, tp, S.island_grid, S.st);
+    hipError_t le = rp_launch_tick(S.d, tp, S.island_grid, S.st, nullptr);
     if (!hip_ok(hipStreamEndCapture(S.st, &g), "hipStreamEndCapture")) return false;
     if (!hip_ok(le, "kernel launch (capture)")) { (void)hipGraphDestroy(g); return false; }
     bool ok = hip_ok(hipGraphInstantiate(&S.graph[parity], g, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -377,9 +382,43 @@ static bool capture_graph(Scene &S, const TickParams &tp0, uint32_t parity) {
     return ok;
 }
 
-static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n, bool use_graph) {
+static_assert(RP_NMARKS - 1 <= RPHYS_PROFILE_MAX_STAGES, "stage count exceeds RPhysProfile");
+// Stage timing of the profiled step: events at the stage boundaries of each tick, summed.
+// Up to PROFILE_BATCH ticks are in flight, each with its own event set, so the stream never
+// waits on the host between ticks.
+struct ProfileAcc {
+    static constexpr uint32_t PROFILE_BATCH = 64;
+    hipEvent_t ev[PROFILE_BATCH][RP_NMARKS] = {};
+    RPhysProfile *out = nullptr;
+    bool init() {
+        for (uint32_t t = 0; t < PROFILE_BATCH; t++)
+            for (int k = 0; k < RP_NMARKS; k++)
+                if (!hip_ok(hipEventCreate(&ev[t][k]), "hipEventCreate")) return false;
+        return true;
+    }
+    ~ProfileAcc() {
+        for (uint32_t t = 0; t < PROFILE_BATCH; t++)
+            for (int k = 0; k < RP_NMARKS; k++) if (ev[t][k]) (void)hipEventDestroy(ev[t][k]);
+    }
+    // Adds the stage times of the first `done` ticks of the batch and the counter sums.
+    bool add(uint32_t done, const uint32_t *cnt) {
+        for (uint32_t t = 0; t < done; t++)
+            for (int k = 0; k + 1 < RP_NMARKS; k++) {
+                float ms = 0.0f;
+                if (!hip_ok(hipEventElapsedTime(&ms, ev[t][k], ev[t][k + 1]), "hipEventElapsedTime")) return false;
+                out->stage_ms[k] += ms;
+            }
+        out->ticks += done;
+        out->pairs += cnt[CT_SUM_PAIRS]; out->terrain_candidates += cnt[CT_SUM_CANDS]; out->contacts += cnt[CT_SUM_CONTACTS];
+        out->island_contacts += cnt[CT_SUM_ISLC]; out->islands += cnt[CT_SUM_ISLANDS]; out->island_contacts_global += cnt[CT_SUM_ISLC_GLB];
+        return true;
+    }
+};
+
+static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n, bool use_graph, ProfileAcc *prof) {
     uint32_t remaining = n;
     Caps caps{S->d.cap_pairs, S->d.cap_cands, S->d.cap_contacts, S->d.cap_warm};
+    if (prof) use_graph = false;
     while (remaining > 0) {
         TickParams tp = tick_params(*ws[0], dt, iv);
         if (!hip_ok(hipMemsetAsync(S->d.cnt, 0, CT_COUNT * 4, S->st), "memset")) return false;
@@ -392,9 +431,10 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
             for (uint32_t k = 0; k < remaining; k++)
                 if (!hip_ok(hipGraphLaunch(S->graph[(S->cur + k) & 1u], S->st), "hipGraphLaunch")) return false;
         } else {
-            for (uint32_t k = 0; k < remaining; k++) {
+            uint32_t batch = prof ? std::min(remaining, ProfileAcc::PROFILE_BATCH) : remaining;
+            for (uint32_t k = 0; k < batch; k++) {
                 tp.cur = (S->cur + k) & 1u;
-                if (!hip_ok(rp_launch_tick(S->d, tp, S->island_grid, S->st), "kernel launch")) return false;
+                if (!hip_ok(rp_launch_tick(S->d, tp, S->island_grid, S->st, prof ? prof->ev[k] : nullptr), "kernel launch")) return false;
             }
         }
         uint32_t cnt[CT_COUNT];
@@ -404,8 +444,12 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         S->cur ^= (done & 1u);
         for (auto *w : ws) w->tick += done;
         remaining -= done;
-        if (!cnt[CT_ABORT]) { if (done == 0 && remaining > 0) { set_err("step made no progress"); return false; } continue; }
-        if (cnt[CT_BIG_ISLAND]) { set_err("an island exceeds 1,048,576 contacts"); return false; }
+        if (!cnt[CT_ABORT]) {
+            if (done == 0 && remaining > 0) { set_err("step made no progress"); return false; }
+            if (prof && !prof->add(done, cnt)) return false;
+            continue;
+        }
+        if (prof && !prof->add(done, cnt)) return false;
         // roll back the failed tick and grow
         rp_launch_restore_backup(S->d, S->st);
         if (!hip_ok(hipStreamSynchronize(S->st), "restore")) return false;
@@ -833,8 +877,49 @@ RPhysResult rphys_world_step_n(RPhysWorld *world, float dt, uint32_t n_ticks) {
     if (!world) { set_err("rphys_world_step_n: world is null"); return RPHYS_ERR_NULL; }
     if (!hfin(dt) || dt < 0.0f) { set_err("rphys_world_step_n: dt must be finite and >= 0"); return RPHYS_ERR_BAD_ARG; }
     if (n_ticks == 0) return RPHYS_OK;
-    return step_impl(world, dt, 1, 0, n_ticks, false);
+    return step_impl(world, dt, 1, 0, n_ticks, n_ticks > 1);
     GUARD_END
+}
+
+RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out) {
+    GUARD_BEGIN
+    if (!world || !out) { set_err("rphys_world_step_profiled: null arg"); return RPHYS_ERR_NULL; }
+    std::memset(out, 0, sizeof(*out));
+    out->n_stages = RP_NMARKS - 1;
+    if (!hfin(dt) || dt < 0.0f) { set_err("rphys_world_step_profiled: dt must be finite and >= 0"); return RPHYS_ERR_BAD_ARG; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (n_ticks == 0) return RPHYS_OK;
+    if (world->w.nb() == 0) { world->w.tick += n_ticks; return RPHYS_OK; }
+    if (world->need_build) { if (!pull(world) || !push(world)) return RPHYS_ERR_INTERNAL; }
+    ProfileAcc acc;
+    acc.out = out;
+    if (!acc.init()) return RPHYS_ERR_INTERNAL;
+    if (!hip_ok(hipMemsetAsync(world->scene->d.ph, 0, RP_PH_COUNT * sizeof(unsigned long long), world->scene->st), "memset")) return RPHYS_ERR_INTERNAL;
+    std::vector<HostWorld *> ws{&world->w};
+    bool ok = run_ticks(world->scene, ws, dt, world->w.iv, n_ticks, false, &acc);
+    if (ok) {  // sweep-phase clocks (a capacity rebuild during the call restarts them)
+        unsigned long long ph[RP_PH_COUNT];
+        int dev = 0, khz = 0;
+        ok = hip_ok(hipMemcpy(ph, world->scene->d.ph, sizeof ph, hipMemcpyDeviceToHost), "phase clocks") && hip_ok(hipGetDevice(&dev), "hipGetDevice") &&
+             hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
+        if (ok) {
+            static_assert(RP_PH_COUNT == 11, "RPhysProfile.sweep_phase holds 11 entries");
+            for (int k = 0; k < RP_PH_COUNT; k++)
+                out->sweep_phase[k] = k < RP_PH_ROUNDS_BASE ? (khz > 0 ? (double)ph[k] / (double)khz : 0.0) : (double)ph[k];
+        }
+    }
+    world->dev_newer = true;
+    world->have_contacts = ok;
+    return ok ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+    GUARD_END
+}
+
+const char *rphys_profile_stage_name(uint32_t stage) {
+    static const char *names[RP_NMARKS - 1] = {"pre", "broadphase", "narrow_pairs", "narrow_terrain", "contacts", "prep_islands",
+                                               "orders", "vel_solve_small", "vel_solve_large", "vel_solve_huge", "warm_store",
+                                               "ballistic", "integrate", "pos_prep", "projection_small", "projection_large",
+                                               "projection_huge", "projection_order", "projection_shock", "s6_sleep"};
+    return stage < RP_NMARKS - 1 ? names[stage] : nullptr;
 }
 
 RPhysResult rphys_world_tick_index(const RPhysWorld *cworld, uint64_t *out_tick) {

@@ -9,8 +9,6 @@
 #define RP_NONE 0xFFFFFFFFu
 #define RP_TERRAIN 0xFFFFFFFFu
 #define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase
-#define RP_ISLAND_LDS_MAX 2048u       // islands up to this many contacts keep their chains in LDS (64 KiB)
-#define RP_ISLAND_MAX (1u << 20)      // position bits of the shock-order key
 
 enum { BF_DYN = 1u, BF_ALIVE = 2u, BF_SLEEP_OK = 4u };
 enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u };
@@ -25,7 +23,11 @@ struct HfDesc {
 // Counters on device (u32 slots)
 enum {
     CT_PAIRS = 0, CT_CANDS, CT_CONTACTS, CT_ISLC, CT_ISLANDS, CT_ABORT, CT_NEED_PAIRS, CT_NEED_CANDS,
-    CT_NEED_CONTACTS, CT_NEED_WARM, CT_BIG_ISLAND, CT_WARM_TOTAL, CT_NEW_TOTAL, CT_KEEP_TOTAL, CT_SCRATCH, CT_DONE, CT_COUNT
+    CT_NEED_CONTACTS, CT_NEED_WARM, CT_CHAIN_N, CT_WARM_TOTAL, CT_NEW_TOTAL, CT_KEEP_TOTAL, CT_SCRATCH, CT_DONE,
+    CT_ISLC_GLB,  // contacts of the islands solved by the 1024-thread LDS-flag tier (this tick)
+    // sums over the completed ticks since the counters were last cleared
+    CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB,
+    CT_COUNT
 };
 
 struct Dev {
@@ -86,9 +88,18 @@ struct Dev {
     int32_t *warm_idx;
     // islands
     uint32_t *isl_k, *isl_v, *isl_k2, *isl_v2, *isl_start, *isl_flag;
-    // chain scratch for islands above the LDS size (sliced by island position)
-    uint64_t *g_ent;  // 4 * cap_contacts
-    uint32_t *g_order, *g_pa, *g_pb, *g_done;
+    // sweep orders over island-list positions: chains of the list order, the shock order (contact
+    // at each shock rank) and its chains, readiness flags of the largest islands
+    uint32_t *ch_pa, *ch_pb, *ch_sa, *ch_sb, *sh_ord, *sh_pa, *sh_pb, *sh_sa, *sh_sb, *g_done;
+    // level orders (execution slot -> island rank) of the islands larger than a workgroup, the
+    // slots of the list order and of the shock order (predecessor slots, contact), and the
+    // rank -> slot scratch
+    uint32_t *ex_base, *ex_shock;
+    uint32_t *bs_pa, *bs_pb, *bs_c, *ss_pa, *ss_pb, *ss_c, *slot_of;
+    unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
+    float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
+    // radix-sort ping-pong buffers (2 * cap_contacts each) and the shock keys' high bits
+    uint32_t *sk0, *sv0, *sk1, *sv1, *sh_hi;
     uint32_t *rs_hist;
     uint32_t rs_blocks;
     // projection contacts
@@ -104,6 +115,41 @@ struct Dev {
     uint32_t *scan_tmp;
     uint32_t *cnt;  // CT_COUNT counters
 };
+
+// Stage boundaries of one tick (events recorded by the profiled step; stage k spans marks k..k+1).
+enum {
+    RP_ST_PRE = 0,          // S1 forces, AABBs, swept bounds
+    RP_ST_BROAD,            // SAP broadphase (sort, enumerate, scans)
+    RP_ST_NARROW_PAIRS,     // body-pair narrowphase
+    RP_ST_NARROW_TERRAIN,   // collider-triangle narrowphase
+    RP_ST_CONTACTS,         // terrain cull, contact count/scan/write
+    RP_ST_PREP,             // constraint prep, islands, island sort, warm match
+    RP_ST_ORDERS,           // list-order chains, velocity shock order and its chains
+    RP_ST_VEL_SMALL,        // velocity solve, islands of up to 4,096 contacts (256 threads)
+    RP_ST_VEL_LARGE,        // velocity solve, islands of up to 131,072 contacts (1024 threads, LDS flags)
+    RP_ST_VEL_HUGE,         // velocity solve, larger islands (1024 threads, global flags)
+    RP_ST_WARM_STORE,       // warm cache store, loaded flags
+    RP_ST_BALLISTIC,        // ballistic classification
+    RP_ST_INTEGRATE,        // S4 integration
+    RP_ST_POS_PREP,         // projection anchors
+    RP_ST_PROJ_SMALL,       // projection iterations, small islands
+    RP_ST_PROJ_LARGE,       // projection iterations, islands of up to 131,072 contacts
+    RP_ST_PROJ_HUGE,        // projection iterations, larger islands
+    RP_ST_PROJ_ORDER,       // projection shock order and its chains
+    RP_ST_PROJ_SHOCK,       // projection shock pass
+    RP_ST_S6,               // S6 velocity from corrections, S7 sleep
+    RP_ST_END,
+    RP_NMARKS
+};
+
+// Sweep phases of the large-island tiers (wall-clock ticks summed over workgroups), plus the
+// level counts of their orders.
+enum {
+    RP_PH_VEL_ORDER = 0, RP_PH_WARM, RP_PH_PGS, RP_PH_VEL_SHOCK_ORDER, RP_PH_VEL_SHOCK, RP_PH_BALL, RP_PH_PROJ,
+    RP_PH_PROJ_SHOCK_ORDER, RP_PH_PROJ_SHOCK, RP_PH_ROUNDS_BASE, RP_PH_ROUNDS_SHOCK, RP_PH_COUNT
+};
+
+#define RP_SROW 12u
 
 struct TickParams {
     float dt;

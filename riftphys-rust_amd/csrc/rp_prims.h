@@ -68,9 +68,12 @@ static __global__ void __launch_bounds__(256) k_scan_apply(const uint32_t *in, u
 // ---------------------------------------------------------------- radix sort
 #define RP_RS_TILE 1024  // items per block per pass (256 threads x 4 rounds)
 
-static __global__ void __launch_bounds__(256) k_rs_hist(const uint32_t *keys, const uint32_t *n_ptr, uint32_t shift, uint32_t nblocks_alloc, uint32_t *hist) {
+// n = min(*n_ptr, cap): the device count can exceed the buffers' capacity on an overflowing tick
+// (which is then rolled back), so the sort never reads or writes past cap.
+static __global__ void __launch_bounds__(256) k_rs_hist(const uint32_t *keys, const uint32_t *n_ptr, uint32_t cap, uint32_t shift,
+                                                 uint32_t nblocks_alloc, uint32_t *hist) {
     __shared__ uint32_t h[256];
-    const uint32_t n = *n_ptr;
+    const uint32_t n = min(*n_ptr, cap);
     h[threadIdx.x] = 0;
     __syncthreads();
     uint32_t base = blockIdx.x * RP_RS_TILE;
@@ -82,11 +85,12 @@ static __global__ void __launch_bounds__(256) k_rs_hist(const uint32_t *keys, co
     hist[threadIdx.x * nblocks_alloc + blockIdx.x] = h[threadIdx.x];
 }
 
-static __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t *keys, const uint32_t *vals, const uint32_t *n_ptr, uint32_t shift,
-                                                    uint32_t nblocks_alloc, const uint32_t *hist_off, uint32_t *keys_out, uint32_t *vals_out) {
+static __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t *keys, const uint32_t *vals, const uint32_t *n_ptr, uint32_t cap,
+                                                    uint32_t shift, uint32_t nblocks_alloc, const uint32_t *hist_off, uint32_t *keys_out,
+                                                    uint32_t *vals_out) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[4][256];
-    const uint32_t n = *n_ptr;
+    const uint32_t n = min(*n_ptr, cap);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     run[t] = hist_off[t * nblocks_alloc + blockIdx.x];
     uint32_t base = blockIdx.x * RP_RS_TILE;

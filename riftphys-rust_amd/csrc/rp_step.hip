@@ -563,7 +563,7 @@ __global__ void k_island_keys(Dev d) {
     GRID_STRIDE(c, nct) {
         uint32_t wa = d.cs_wa[c], wb = d.cs_wb[c];
         uint32_t x = wa != RP_NONE ? wa : wb;
-        d.isl_k[c] = x == RP_NONE ? RP_NONE : uf_find(d.root, x);
+        d.isl_k[c] = x == RP_NONE ? d.nb : uf_find(d.root, x);  // nb: no written body (sorts last)
         d.isl_v[c] = c;
     }
 }
@@ -575,8 +575,8 @@ __global__ void k_island_flags(Dev d, const uint32_t *keys) {
     GRID_STRIDE(r, d.cap_contacts) {
         if (r >= nct) { d.isl_flag[r] = 0; continue; }
         uint32_t k = keys[r];
-        d.isl_flag[r] = (k != RP_NONE && (r == 0 || keys[r - 1] != k)) ? 1u : 0u;
-        if (k != RP_NONE && (r + 1 == nct || keys[r + 1] == RP_NONE)) d.cnt[CT_ISLC] = r + 1;
+        d.isl_flag[r] = (k != d.nb && (r == 0 || keys[r - 1] != k)) ? 1u : 0u;
+        if (k != d.nb && (r + 1 == nct || keys[r + 1] == d.nb)) d.cnt[CT_ISLC] = r + 1;
     }
 }
 __global__ void k_island_compact(Dev d, const uint32_t *keys) {
@@ -584,7 +584,7 @@ __global__ void k_island_compact(Dev d, const uint32_t *keys) {
     const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
     GRID_STRIDE(r, nct) {
         uint32_t k = keys[r];
-        if (k != RP_NONE && (r == 0 || keys[r - 1] != k)) d.isl_start[d.isl_flag[r]] = r;
+        if (k != d.nb && (r == 0 || keys[r - 1] != k)) d.isl_start[d.isl_flag[r]] = r;
     }
 }
 
@@ -642,121 +642,237 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
 }
 
 // ------------------------------------------------------------------ ordered sweeps
-// Chain storage of the island being processed.  Each sweep kernel has two instantiations: islands
-// of up to RP_ISLAND_LDS_MAX contacts keep their chains in LDS, larger ones in a global slice owned
-// by the island (its list positions are its own).  The storage's address space is part of the
-// pointer types, so LDS is only ever reached by ds_* instructions: flat (generic) accesses into
-// the LDS aperture beyond its first 64 KiB fault on gfx950.
-#define RP_LDS __attribute__((address_space(3)))
-template <class P64, class P32> struct ChainsT { P64 ent; P32 order, pa, pb, done; };
-using LdsChains = ChainsT<RP_LDS uint64_t *, RP_LDS uint32_t *>;
-using GlbChains = ChainsT<uint64_t *, uint32_t *>;
-struct IslandLds {
-    uint64_t ent[2 * RP_ISLAND_LDS_MAX];
-    uint32_t order[RP_ISLAND_LDS_MAX], pa[RP_ISLAND_LDS_MAX], pb[RP_ISLAND_LDS_MAX], done[RP_ISLAND_LDS_MAX];
-};
-static_assert(sizeof(IslandLds) <= 65536, "island LDS block must leave room for two workgroups per CU");
-template <bool USE_LDS>
-static __device__ __forceinline__ auto chains_for(const Dev &d, IslandLds &L, uint32_t s) {
-    if constexpr (USE_LDS)
-        return LdsChains{(RP_LDS uint64_t *)L.ent, (RP_LDS uint32_t *)L.order, (RP_LDS uint32_t *)L.pa, (RP_LDS uint32_t *)L.pb,
-                         (RP_LDS uint32_t *)L.done};
-    else
-        return GlbChains{d.g_ent + 4ull * s, d.g_order + s, d.g_pa + s, d.g_pb + s, d.g_done + s};
+// Island lists: positions [0, nislc) of isl_v2 grouped by island (isl_start), contact index order
+// within an island, which is the reference's constraint order.  An order of an island is a
+// permutation of its segment; its chains give, for every rank and side, the previous rank of the
+// segment that writes the same body (global ranks, RP_NONE when there is none).  A sweep then runs
+// as dataflow: a rank executes once both predecessors have, which is serial Gauss-Seidel in the
+// order (islands share no written body).
+//
+// Chains are built for all islands at once: the (body, rank, side) entries are radix-sorted by
+// body, stably, so ranks stay increasing within a body, and each entry links to the one before.
+__global__ void k_chain_count(Dev d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) d.cnt[CT_CHAIN_N] = 2u * min(d.cnt[CT_ISLC], d.cap_contacts);
 }
-
-template <class P64>
-static __device__ void bitonic_u64(P64 a, uint32_t m) {
-    for (uint32_t k = 2; k <= m; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-                uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    uint64_t x = a[i], y = a[ixj];
-                    bool asc = (i & k) == 0;
-                    if ((x > y) == asc) { a[i] = y; a[ixj] = x; }
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
-}
-
-// Predecessor chains for the sweep order C.order (positions 0..n-1 of the island list): for
-// each position and each written body, the previous position in the order writing the same body
-// (stable sort of (body, position, side) entries).
-template <class Ch>
-static __device__ void build_chains(const Dev &d, const Ch &C, const uint32_t *list, uint32_t n) {
-    uint32_t m = 1;
-    while (m < 2 * n) m <<= 1;
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        uint64_t v = ~0ull;
-        if (i < 2 * n) {
-            uint32_t r = i >> 1, side = i & 1;
-            uint32_t c = list[C.order[r]];
-            uint32_t body = side ? d.cs_wb[c] : d.cs_wa[c];
-            if (body != RP_NONE) v = ((uint64_t)body << 32) | ((uint64_t)r << 1) | side;
-        }
-        C.ent[i] = v;
+__global__ void k_chain_entries(Dev d, const uint32_t *ord, uint32_t *keys, uint32_t *vals) {
+    if (aborted(d)) return;
+    const uint32_t n2 = d.cnt[CT_CHAIN_N];
+    GRID_STRIDE(e, n2) {
+        uint32_t c = ord[e >> 1];
+        uint32_t b = (e & 1u) ? d.cs_wb[c] : d.cs_wa[c];
+        keys[e] = b == RP_NONE ? d.nb : b;
+        vals[e] = e;
     }
-    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) { C.pa[r] = RP_NONE; C.pb[r] = RP_NONE; C.done[r] = 0; }
-    __threadfence_block();
-    __syncthreads();
-    bitonic_u64(C.ent, m);
-    for (uint32_t i = threadIdx.x; i < 2 * n; i += blockDim.x) {
-        uint64_t v = C.ent[i];
-        if (v == ~0ull) continue;
-        uint32_t r = ((uint32_t)v) >> 1, side = (uint32_t)v & 1u;
-        uint32_t pred = RP_NONE;
-        if (i > 0 && C.ent[i - 1] != ~0ull && (C.ent[i - 1] >> 32) == (v >> 32)) pred = ((uint32_t)C.ent[i - 1]) >> 1;
-        if (side) C.pb[r] = pred; else C.pa[r] = pred;
-    }
-    __threadfence_block();
-    __syncthreads();
 }
-
-// Dataflow execution of one sweep: each lane walks its positions in increasing order and runs
-// one when both predecessors are done in this sweep (the lowest unfinished position is always
-// ready, so every lane finishes).
-template <class Ch, class F>
-static __device__ void run_sweep(const Ch &C, uint32_t n, uint32_t token, F exec) {
-    uint32_t r = threadIdx.x;
-    while (r < n) {
-        uint32_t a = C.pa[r], b = C.pb[r];
-        bool ready = (a == RP_NONE || __hip_atomic_load(&C.done[a], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= token) &&
-                     (b == RP_NONE || __hip_atomic_load(&C.done[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= token);
-        if (ready) {
-            exec(r);
-            __hip_atomic_store(&C.done[r], token, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-            r += blockDim.x;
+// Writes each (rank, side)'s predecessor (pa / pb) and successor (sa / sb) on its body's chain:
+// the neighbouring entries of the same body at a strictly different rank (a contact writing one
+// body on both sides links past its own other side), so every wait is on an earlier rank.
+__global__ void k_chain_link(Dev d, const uint32_t *keys, const uint32_t *vals, uint32_t *pa, uint32_t *pb, uint32_t *sa, uint32_t *sb) {
+    if (aborted(d)) return;
+    const uint32_t n2 = d.cnt[CT_CHAIN_N];
+    GRID_STRIDE(i, n2) {
+        uint32_t v = vals[i], k = keys[i], r = v >> 1;
+        uint32_t pred = RP_NONE, succ = RP_NONE;
+        if (k != d.nb) {
+            uint32_t j = i;
+            while (j > 0 && keys[j - 1] == k && (vals[j - 1] >> 1) == r) j--;
+            if (j > 0 && keys[j - 1] == k) pred = vals[j - 1] >> 1;
+            j = i + 1;
+            while (j < n2 && keys[j] == k && (vals[j] >> 1) == r) j++;
+            if (j < n2 && keys[j] == k) succ = vals[j] >> 1;
         }
+        ((v & 1u) ? pb : pa)[r] = pred;
+        ((v & 1u) ? sb : sa)[r] = succ;
     }
-    __threadfence_block();
-    __syncthreads();
 }
 
-// (sort_key6(height), list position) packed for the shock passes: keys clamped to +-2^43 units
-// of 1e-6 m (heights beyond 8,796 km tie and fall back to list order).
-static __device__ __forceinline__ uint64_t shock_key(float h, uint32_t r) {
+// Shock order (solver.rs:968-975): (sort_key6(lowest height along up), list position) within each
+// island; terrain contacts count as -infinity.  Keys are clamped to +-2^43 units of 1e-6 m
+// (heights beyond 8,796 km tie and keep list order) and offset to 44 unsigned bits, split into
+// the low 32 and high 12 bits for the stable radix passes.
+static __device__ __forceinline__ uint64_t shock_key44(float h) {
     int64_t k = sort_key6(h);
     const int64_t LIM = (int64_t)1 << 43;
     if (k < -LIM) k = -LIM;
     if (k > LIM - 1) k = LIM - 1;
-    return ((uint64_t)(k + LIM) << 20) | r;
+    return (uint64_t)(k + LIM);
+}
+__global__ void k_shock_keys(Dev d, TickParams tp, uint32_t *klo, uint32_t *khi, uint32_t *vals) {
+    if (aborted(d)) return;
+    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    GRID_STRIDE(r, nislc) {
+        uint32_t c = d.isl_v2[r];
+        uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+        float h = b == RP_NONE ? -INFINITY : fminf(dot3(L3(d.pos[a]), up), dot3(L3(d.pos[b]), up));
+        uint64_t k = shock_key44(h);
+        klo[r] = (uint32_t)k;
+        khi[r] = (uint32_t)(k >> 32);
+        vals[r] = r;
+    }
+}
+// keys_out[i] = src[perm[i]]
+__global__ void k_gather_keys(Dev d, const uint32_t *src, const uint32_t *perm, uint32_t *keys_out) {
+    if (aborted(d)) return;
+    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    GRID_STRIDE(i, nislc) keys_out[i] = src[perm[i]];
+}
+__global__ void k_shock_order(Dev d, const uint32_t *perm) {
+    if (aborted(d)) return;
+    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    GRID_STRIDE(i, nislc) d.sh_ord[i] = d.isl_v2[perm[i]];
 }
 
-// Sets C.order to the shock order of the island list from per-position heights.
-template <class Ch, class H>
-static __device__ void shock_order(const Ch &C, uint32_t n, H height) {
-    uint32_t m = 1;
-    while (m < n) m <<= 1;
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) C.ent[i] = i < n ? shock_key(height(i), i) : ~0ull;
-    __threadfence_block();
+// Readiness flags of the island being swept.  Tier 0 (up to 4,096 ranks) and tier 1 (up to
+// 131,072 ranks) keep one byte per rank in LDS; tier 2 uses a u32 slice of a global array owned
+// by the island.  The flag of a rank holds the token of the last sweep that executed it; tokens
+// increase through an island's sweeps, so a new chain set needs no reset.  LDS is reached through
+// LDS-typed pointers only (ds_* instructions): flat accesses into the LDS aperture beyond its
+// first 64 KiB fault on gfx950.
+#define RP_LDS __attribute__((address_space(3)))
+struct LdsFlags {
+    RP_LDS uint8_t *f;
+    static constexpr uint32_t max_token = 255u;
+    __device__ __forceinline__ uint32_t load(uint32_t i) const { return __hip_atomic_load(&f[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+    __device__ __forceinline__ void store(uint32_t i, uint32_t t) const {
+        __hip_atomic_store(&f[i], (uint8_t)t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ void clear(uint32_t i) const { f[i] = 0; }
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { f[i] = (uint8_t)v; }
+    // decrements byte i (never below zero: at most its value is subtracted), returns its old value
+    __device__ __forceinline__ uint32_t dec(uint32_t i) const {
+        RP_LDS uint32_t *w = (RP_LDS uint32_t *)f + (i >> 2);
+        const uint32_t sh = 8u * (i & 3u);
+        return (__hip_atomic_fetch_sub(w, 1u << sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> sh) & 255u;
+    }
+};
+struct GlbFlags {
+    uint32_t *f;
+    static constexpr uint32_t max_token = 0xFFFFFFFEu;
+    __device__ __forceinline__ uint32_t load(uint32_t i) const { return __hip_atomic_load(&f[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+    __device__ __forceinline__ void store(uint32_t i, uint32_t t) const { __hip_atomic_store(&f[i], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+    __device__ __forceinline__ void clear(uint32_t i) const { f[i] = 0; }
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { f[i] = v; }
+    __device__ __forceinline__ uint32_t dec(uint32_t i) const { return __hip_atomic_fetch_sub(&f[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+};
+template <int TIER> struct Tier;
+template <> struct Tier<0> { static constexpr uint32_t threads = 256, flag_bytes = 4096; };
+template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
+template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
+static __device__ __forceinline__ int tier_of(uint32_t n) { return n <= Tier<0>::flag_bytes ? 0 : (n <= Tier<1>::flag_bytes ? 1 : 2); }
+
+// The island's sweep state: segment [s, s + n) and the token counter.
+template <class FL> struct Island {
+    FL fl;
+    uint32_t s, n, token;
+    __device__ void begin() {
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) fl.clear(i);
+        token = 0;
+        __syncthreads();
+    }
+};
+
+// An order of the island as execution slots j = 0..n-1 (global offset s + j): the contact at
+// each slot and its predecessors' slots (RP_NONE: none).  Slot order is a topological order of
+// the chains (list order, or level order), and readiness flags are per slot.
+struct Slots { uint32_t *pa, *pb, *c; };
+
+// Writes the slots of the order whose chains are (pa, pb) over island ranks, whose contacts are
+// ordc[s + rank], with ex[s + slot] = rank (ex == nullptr: slot = rank).
+template <class FL>
+static __device__ void build_slots(const Dev &d, const Island<FL> &I, const uint32_t *ordc, const uint32_t *pa, const uint32_t *pb,
+                                   const uint32_t *ex, Slots o) {
+    const uint32_t s = I.s, n = I.n;
+    if (ex) {
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) d.slot_of[s + ex[s + j]] = j;
+        __syncthreads();
+    }
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const uint32_t r = ex ? ex[s + j] : j, a = pa[s + r], b = pb[s + r];
+        o.pa[s + j] = a == RP_NONE ? RP_NONE : (ex ? d.slot_of[a] : a - s);
+        o.pb[s + j] = b == RP_NONE ? RP_NONE : (ex ? d.slot_of[b] : b - s);
+        o.c[s + j] = ordc[s + r];
+    }
     __syncthreads();
-    bitonic_u64(C.ent, m);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) C.order[i] = (uint32_t)(C.ent[i] & 0xFFFFFu);
-    __threadfence_block();
+}
+
+// One sweep over the slots: each lane walks its slots j = threadIdx.x + k * blockDim in
+// increasing order; it loads the slot's constant row with prep(contact, slot) at once, waits until
+// both predecessor slots carry this sweep's token, then runs run(contact, row) and marks the
+// slot.  The lowest unfinished slot is always ready, so every lane finishes.
+template <class FL, class P, class R>
+static __device__ void run_sweep(Island<FL> &I, const Slots &sl, P prep, R run) {
+    if (I.token >= FL::max_token) I.begin();
+    const uint32_t tok = ++I.token, s = I.s, n = I.n;
+    uint32_t j = threadIdx.x, a = RP_NONE, b = RP_NONE, c = 0;
+    decltype(prep(0u, 0u)) x{};
+    bool fresh = true;
+    while (j < n) {
+        if (fresh) { a = sl.pa[s + j]; b = sl.pb[s + j]; c = sl.c[s + j]; x = prep(c, s + j); fresh = false; }
+        bool ready = (a == RP_NONE || I.fl.load(a) >= tok) && (b == RP_NONE || I.fl.load(b) >= tok);
+        if (ready) {
+            run(c, x);
+            I.fl.store(j, tok);
+            j += blockDim.x;
+            fresh = true;
+        }
+    }
     __syncthreads();
+}
+struct NoRow {};
+template <class FL, class F>
+static __device__ void run_sweep(Island<FL> &I, const Slots &sl, F exec) {
+    run_sweep(I, sl, [](uint32_t, uint32_t) { return NoRow{}; }, [&](uint32_t c, const NoRow &) { exec(c); });
+}
+
+// Wall-clock time per sweep phase of the large-island tiers (thread 0 of each workgroup adds
+// its share), read by the profiled step.
+struct PhaseClock {
+    uint64_t t;
+    __device__ PhaseClock() : t(wall_clock64()) {}
+    __device__ void lap(const Dev &d, int k) {
+        uint64_t now = wall_clock64();
+        if (threadIdx.x == 0) atomicAdd(&d.ph[k], (unsigned long long)(now - t));
+        t = now;
+    }
+};
+
+// Level order of the island for the chains (pa, pb) with successors (sa, sb): ranks are grouped
+// by level(r) = 1 + max(level of its predecessors), the reference's batch index
+// (solver.rs:621-642) within the island.  Ranks of one level write no common body, so executing
+// them in any order gives the same bits; the sweeps then keep every lane busy on independent
+// ranks.  Built by frontier expansion: ranks without predecessors form level 1, and a rank joins
+// the next level when its last predecessor leaves the current one.  In-degrees live in the
+// island's flag storage (cleared again on return); ex[s + slot] receives local ranks.  Returns ex,
+// or nullptr (list order, always valid) should the expansion not place every rank.
+template <class FL>
+static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa, const uint32_t *pb, const uint32_t *sa, const uint32_t *sb,
+                                              uint32_t *ex, unsigned long long *rounds) {
+    __shared__ uint32_t tail;
+    const uint32_t s = I.s, n = I.n;
+    if (threadIdx.x == 0) tail = 0;
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) I.fl.set(r, (pa[s + r] != RP_NONE ? 1u : 0u) + (pb[s + r] != RP_NONE ? 1u : 0u));
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x)
+        if (pa[s + r] == RP_NONE && pb[s + r] == RP_NONE) ex[s + atomicAdd(&tail, 1u)] = r;
+    __syncthreads();
+    uint32_t fs = 0, fe = tail, nr = 0;
+    while (fs < fe) {
+        nr++;
+        for (uint32_t j = fs + threadIdx.x; j < fe; j += blockDim.x) {
+            const uint32_t r = ex[s + j];
+            const uint32_t qa = sa[s + r], qb = sb[s + r];
+            if (qa != RP_NONE && I.fl.dec(qa - s) == 1u) ex[s + atomicAdd(&tail, 1u)] = qa - s;
+            if (qb != RP_NONE && I.fl.dec(qb - s) == 1u) ex[s + atomicAdd(&tail, 1u)] = qb - s;
+        }
+        __syncthreads();
+        fs = fe;
+        fe = tail;
+        __syncthreads();
+    }
+    I.begin();
+    if (threadIdx.x == 0) atomicAdd(rounds, (unsigned long long)nr);
+    return fe == n ? ex : nullptr;
 }
 
 // solver.rs:455-464 apply_contact_impulse (+ dynamics.rs:193-205)
@@ -784,36 +900,132 @@ static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
     float x = slip / mu4.z;
     return mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
 }
+// A contact row's constants during the velocity solve: everything but the bodies' velocities
+// and the row's accumulators (world inverse inertia is fixed until integration).
+struct Row {
+    uint32_t ai, bi;
+    float inv_a, inv_b, ima, imb;
+    V3 ra, rb, n, t1, t2;
+    float4 m4, mu4;
+    M3 iia, iib;
+};
+static __device__ __forceinline__ Row load_row(const Dev &d, uint32_t c) {
+    Row x;
+    x.ai = d.cs_ai[c]; x.bi = d.cs_bi[c];
+    float2 inv = d.cs_inv[c];
+    x.inv_a = inv.x; x.inv_b = inv.y;
+    x.ra = L3(d.cs_ra[c]); x.rb = L3(d.cs_rb[c]); x.n = L3(d.cs_n[c]); x.t1 = L3(d.cs_t1[c]); x.t2 = L3(d.cs_t2[c]);
+    x.m4 = d.cs_m[c]; x.mu4 = d.cs_mu[c];
+    x.ima = d.inv_mass[x.ai];
+    x.iia = load_m9(d.iiw + 9 * x.ai);
+    if (x.bi != RP_NONE) { x.imb = d.inv_mass[x.bi]; x.iib = load_m9(d.iiw + 9 * x.bi); }
+    else { x.imb = 0.0f; x.iib = mzero(); }
+    return x;
+}
+// Rows stored per execution slot, component-major (srow[k * cap + slot], k < RP_SROW), so the
+// lanes of a wave load consecutive addresses and a row is one round of independent loads.
+static __device__ __forceinline__ float4 f4u(uint32_t a, uint32_t b, float z, float w) { return make_float4(__uint_as_float(a), __uint_as_float(b), z, w); }
+static __device__ __forceinline__ void store_row(const Dev &d, uint32_t slot, const Row &x) {
+    float4 *o = d.srow + slot;
+    const size_t C = d.cap_contacts;
+    o[0] = S3(x.ra, x.m4.x); o[C] = S3(x.rb, x.m4.y); o[2 * C] = S3(x.n, x.m4.z); o[3 * C] = S3(x.t1, x.m4.w);
+    o[4 * C] = S3(x.t2, x.mu4.x); o[5 * C] = make_float4(x.mu4.y, x.mu4.z, x.inv_a, x.inv_b); o[6 * C] = f4u(x.ai, x.bi, x.ima, x.imb);
+    o[7 * C] = make_float4(x.iia.c0.x, x.iia.c0.y, x.iia.c0.z, x.iia.c1.x); o[8 * C] = make_float4(x.iia.c1.y, x.iia.c1.z, x.iia.c2.x, x.iia.c2.y);
+    o[9 * C] = make_float4(x.iia.c2.z, x.iib.c0.x, x.iib.c0.y, x.iib.c0.z); o[10 * C] = make_float4(x.iib.c1.x, x.iib.c1.y, x.iib.c1.z, x.iib.c2.x);
+    o[11 * C] = make_float4(x.iib.c2.y, x.iib.c2.z, 0.0f, 0.0f);
+}
+static __device__ __forceinline__ Row fetch_row(const Dev &d, uint32_t slot) {
+    const float4 *o = d.srow + slot;
+    const size_t C = d.cap_contacts;
+    float4 f0 = o[0], f1 = o[C], f2 = o[2 * C], f3 = o[3 * C], f4 = o[4 * C], f5 = o[5 * C], f6 = o[6 * C];
+    float4 f7 = o[7 * C], f8 = o[8 * C], f9 = o[9 * C], f10 = o[10 * C], f11 = o[11 * C];
+    Row x;
+    x.ra = L3(f0); x.rb = L3(f1); x.n = L3(f2); x.t1 = L3(f3); x.t2 = L3(f4);
+    x.m4 = make_float4(f0.w, f1.w, f2.w, f3.w);
+    x.mu4 = make_float4(f4.w, f5.x, f5.y, 0.0f);
+    x.inv_a = f5.z; x.inv_b = f5.w;
+    x.ai = __float_as_uint(f6.x); x.bi = __float_as_uint(f6.y); x.ima = f6.z; x.imb = f6.w;
+    x.iia.c0 = mk3(f7.x, f7.y, f7.z); x.iia.c1 = mk3(f7.w, f8.x, f8.y); x.iia.c2 = mk3(f8.z, f8.w, f9.x);
+    x.iib.c0 = mk3(f9.y, f9.z, f9.w); x.iib.c1 = mk3(f10.x, f10.y, f10.z); x.iib.c2 = mk3(f10.w, f11.x, f11.y);
+    return x;
+}
+static __device__ __forceinline__ bool same_bits(V3 a, V3 b) {
+    return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) && __float_as_uint(a.z) == __float_as_uint(b.z);
+}
+// Velocities of a row's two bodies held in registers across its normal and friction updates.
+struct Vel2 { V3 la, aa, lb, ab; };
+static __device__ __forceinline__ V3 rel_vel2(const Row &x, const Vel2 &v) {
+    V3 va = v.la + cross3(v.aa, x.ra);
+    V3 vb = x.bi == RP_NONE ? splat3(0.0f) : v.lb + cross3(v.ab, x.rb);
+    return vb - va;
+}
+// apply_cimp on the register copies (same operations and the same sanitised values as stored)
+static __device__ __forceinline__ void apply_row2(const Row &x, Vel2 &v, V3 j) {
+    if (x.inv_a > 0.0f) {
+        V3 nj = -j;
+        if (x.ima > 0.0f && fin3(nj)) v.la = fin_v(v.la + nj * x.ima);
+        V3 tau = cross3(x.ra, nj);
+        if (x.ima > 0.0f && fin3(tau)) v.aa = fin_v(v.aa + mv(x.iia, tau));
+    }
+    if (x.inv_b > 0.0f) {
+        if (x.imb > 0.0f && fin3(j)) v.lb = fin_v(v.lb + j * x.imb);
+        V3 tau = cross3(x.rb, j);
+        if (x.imb > 0.0f && fin3(tau)) v.ab = fin_v(v.ab + mv(x.iib, tau));
+    }
+}
+// apply_cimp with the row's prefetched masses and inertias (same operations)
+static __device__ __forceinline__ void apply_row(const Dev &d, const Row &x, V3 j) {
+    if (x.inv_a > 0.0f) {
+        V3 nj = -j;
+        if (x.ima > 0.0f && fin3(nj)) d.lin[x.ai] = S3(fin_v(L3(d.lin[x.ai]) + nj * x.ima));
+        V3 tau = cross3(x.ra, nj);
+        if (x.ima > 0.0f && fin3(tau)) d.ang[x.ai] = S3(fin_v(L3(d.ang[x.ai]) + mv(x.iia, tau)));
+    }
+    if (x.inv_b > 0.0f) {
+        if (x.imb > 0.0f && fin3(j)) d.lin[x.bi] = S3(fin_v(L3(d.lin[x.bi]) + j * x.imb));
+        V3 tau = cross3(x.rb, j);
+        if (x.imb > 0.0f && fin3(tau)) d.ang[x.bi] = S3(fin_v(L3(d.ang[x.bi]) + mv(x.iib, tau)));
+    }
+}
 
-// The island's segment of the sorted list and its chain storage; an instantiation skips the
-// islands of the other storage class.
-#define ISLAND_LOOP_BEGIN                                                                      \
-    __shared__ IslandLds L;                                                                    \
-    const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = d.cnt[CT_ISLC];                           \
-    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {                            \
+// Islands of this tier, one per workgroup (grid-stride); I is the island's sweep state with its
+// readiness flags, lst its contacts in list order.
+#define ISLAND_LOOP_BEGIN                                                                          \
+    __shared__ __attribute__((aligned(4))) uint8_t flag_lds[Tier<TIER>::flag_bytes > 0 ? Tier<TIER>::flag_bytes : 4]; \
+    const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);         \
+    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {                                \
         const uint32_t s = d.isl_start[isl], n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - s; \
-        if ((n <= RP_ISLAND_LDS_MAX) != USE_LDS) continue;                                     \
-        if (n > RP_ISLAND_MAX) { if (threadIdx.x == 0) atomicMax(&d.cnt[CT_BIG_ISLAND], n); continue; } \
-        const uint32_t *lst = d.isl_v2 + s;                                                    \
-        const auto C = chains_for<USE_LDS>(d, L, s);                                           \
-        for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) C.order[r] = r;                 \
-        __threadfence_block();                                                                 \
-        __syncthreads();                                                                       \
-        build_chains(d, C, lst, n);
+        if (tier_of(n) != TIER) continue;                                                          \
+        const uint32_t *lst = d.isl_v2 + s;                                                        \
+        auto I = make_island<TIER>(d, flag_lds, s, n);                                             \
+        I.begin();
 #define ISLAND_LOOP_END }
 
-// Velocity solve of one island per workgroup: warm-start application, PGS iterations, the
-// pre-shock impulse snapshot (what the warm cache stores), then the shock pass.
-template <bool USE_LDS>
-__global__ void __launch_bounds__(256) k_sweep_vel(Dev d, TickParams tp) {
+template <int TIER>
+static __device__ __forceinline__ auto make_island(const Dev &d, uint8_t *flag_lds, uint32_t s, uint32_t n) {
+    if constexpr (TIER < 2) return Island<LdsFlags>{LdsFlags{(RP_LDS uint8_t *)flag_lds}, s, n, 0};
+    else return Island<GlbFlags>{GlbFlags{d.g_done + s}, s, n, 0};
+}
+
+// Velocity solve of one island per workgroup: warm-start application and the PGS iterations in
+// list order, the pre-shock impulse snapshot (what the warm cache stores), then the shock pass in
+// shock order (built before the launch from the tick-start poses, which the solve leaves as is).
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t cur = tp.cur;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     ISLAND_LOOP_BEGIN
-        uint32_t token = 1;
+        if (threadIdx.x == 0 && TIER == 1) atomicAdd(&d.cnt[CT_ISLC_GLB], n);
+        // islands larger than the workgroup sweep in level order (the ballistic and projection
+        // sweeps reuse these slots); smaller ones give every rank its own lane in list order
+        PhaseClock pc;
+        const uint32_t *exb = n > blockDim.x ? level_order(I, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, d.ex_base, &d.ph[RP_PH_ROUNDS_BASE]) : nullptr;
+        const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
+        build_slots(d, I, d.isl_v2, d.ch_pa, d.ch_pb, exb, sb);
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_ORDER);
         // warm-start application (solver.rs:749-760)
-        run_sweep(C, n, token++, [&](uint32_t r) {
-            uint32_t c = lst[r];
+        run_sweep(I, sb, [&](uint32_t c) {
             int wi = d.warm_idx[c];
             if (wi < 0) return;
             uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
@@ -830,42 +1042,60 @@ __global__ void __launch_bounds__(256) k_sweep_vel(Dev d, TickParams tp) {
             imp = imp + (t1 * t.x + t2 * t.y);
             apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
         });
-        // PGS iterations (solver.rs:839-879)
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_WARM);
+        // PGS iterations (solver.rs:839-879).  The rows' constants are stored per slot once and
+        // loaded, with the row's own accumulators, before its wait; the bodies' velocities are
+        // loaded once and kept in registers between the normal and the friction update.
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
+        __syncthreads();
+        struct PgsIn { Row x; float jn; float2 jt; };
         for (uint32_t it = 0; it < tp.iv; it++) {
-            run_sweep(C, n, token++, [&](uint32_t r) {
-                uint32_t c = lst[r];
-                uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-                V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
-                float2 inv = d.cs_inv[c];
-                float4 m4 = d.cs_m[c];
-                float vrn = dot3(rel_vel(d, ai, bi, ra, rb), n3);
-                float jraw = (m4.w - vrn) * m4.x;
-                float old = d.jn[c];
-                float nw = fmaxf(old + jraw, 0.0f);
-                d.jn[c] = nw;
-                apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, n3 * (nw - old));
-                V3 vf = rel_vel(d, ai, bi, ra, rb);
-                float vt1 = dot3(vf, t1), vt2 = dot3(vf, t2);
-                float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
-                float2 ot2 = d.jt[c];
-                V3 ot = mk3(ot2.x, ot2.y, 0.0f);
-                V3 nt = clamp_len_max(ot - mk3(vt1 * m4.y, vt2 * m4.z, 0.0f), mu * nw);
-                d.jt[c] = make_float2(nt.x, nt.y);
-                V3 dd = nt - ot;
-                apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, t1 * dd.x + t2 * dd.y);
-            });
+            run_sweep(I, sb,
+                [&](uint32_t c, uint32_t slot) {
+                    PgsIn in;
+                    in.x = fetch_row(d, slot);
+                    in.jn = d.jn[c];
+                    in.jt = d.jt[c];
+                    return in;
+                },
+                [&](uint32_t c, const PgsIn &in) {
+                    const Row &x = in.x;
+                    Vel2 v;
+                    v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
+                    if (x.bi != RP_NONE) { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
+                    const Vel2 v0 = v;
+                    float vrn = dot3(rel_vel2(x, v), x.n);
+                    float jraw = (x.m4.w - vrn) * x.m4.x;
+                    float old = in.jn;
+                    float nw = fmaxf(old + jraw, 0.0f);
+                    d.jn[c] = nw;
+                    apply_row2(x, v, x.n * (nw - old));
+                    V3 vf = rel_vel2(x, v);
+                    float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
+                    float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
+                    V3 ot = mk3(in.jt.x, in.jt.y, 0.0f);
+                    V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
+                    d.jt[c] = make_float2(nt.x, nt.y);
+                    V3 dd = nt - ot;
+                    apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
+                    // store what changed (a body the row does not write keeps its bits)
+                    if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
+                    if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
+                    if (x.bi != RP_NONE) {
+                        if (!same_bits(v.lb, v0.lb)) d.lin[x.bi] = S3(v.lb);
+                        if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
+                    }
+                });
         }
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);
         for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
-        // shock pass (solver.rs:968-993): order by (sort_key6(lowest height), index)
-        shock_order(C, n, [&](uint32_t i) {
-            uint32_t c = lst[i];
-            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-            float ha = dot3(L3(d.pos[ai]), up);
-            return bi == RP_NONE ? -INFINITY : fminf(ha, dot3(L3(d.pos[bi]), up));
-        });
-        build_chains(d, C, lst, n);
-        run_sweep(C, n, token++, [&](uint32_t r) {
-            uint32_t c = lst[C.order[r]];
+        __syncthreads();
+        // shock pass (solver.rs:976-993) in shock order
+        const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK]) : nullptr;
+        const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
+        build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, ss);
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
+        run_sweep(I, ss, [&](uint32_t c) {
             uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
             V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
             float2 inv = d.cs_inv[c];
@@ -883,6 +1113,7 @@ __global__ void __launch_bounds__(256) k_sweep_vel(Dev d, TickParams tp) {
             d.jn[c] = nw;
             apply_cimp(d, ai, bi, inv_a, inv_b, ra, rb, n3 * (nw - old));
         });
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK);
     ISLAND_LOOP_END
 }
 
@@ -1003,12 +1234,12 @@ __global__ void k_ball_init(Dev d) {
 
 // step.rs:180-206 in contact order over each island; only awake dynamic bodies' flags can change
 // an outcome, and those are chained.
-template <bool USE_LDS>
-__global__ void __launch_bounds__(256) k_sweep_ball(Dev d, TickParams tp) {
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_ball(Dev d, TickParams tp) {
     if (aborted(d)) return;
     ISLAND_LOOP_BEGIN
-        run_sweep(C, n, 1u, [&](uint32_t r) {
-            uint32_t c = lst[r];
+        PhaseClock pc;
+        run_sweep(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, [&](uint32_t c) {
             uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
             bool hb = b != RP_NONE;
             bool ba = d.ballf[a] != 0, bb = hb && d.ballf[b] != 0;
@@ -1028,6 +1259,7 @@ __global__ void __launch_bounds__(256) k_sweep_ball(Dev d, TickParams tp) {
                 if (d.cs_wb[c] != RP_NONE) d.ballf[b] = 0;
             }
         });
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_BALL);
     ISLAND_LOOP_END
 }
 
@@ -1109,30 +1341,80 @@ __global__ void k_posc_prep(Dev d) {
     }
 }
 
-// step.rs:447-497 one projection pass item
-static __device__ void project_one(const Dev &d, uint32_t c, bool shock, V3 up) {
-    uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
-    bool hb = b != RP_NONE;
-    bool am = d.inv_mass[a] > 0.0f && d.awake[a];
-    bool bm = hb && d.inv_mass[b] > 0.0f && d.awake[b];
+// step.rs:447-497 one projection pass item, split into its constants (loaded before the item's
+// wait) and the pose-dependent part
+struct PRow {
+    uint32_t a, b;
+    bool am, bm;
+    float4 la4;
+    V3 lb, n, pt;
+    float ima, imb;
+    M3 ila, ilb;  // local inverse inertia
+};
+static __device__ __forceinline__ PRow load_prow(const Dev &d, uint32_t c) {
+    PRow x;
+    x.a = d.cs_ai[c]; x.b = d.cs_bi[c];
+    const bool hb = x.b != RP_NONE;
+    x.ima = d.inv_mass[x.a];
+    x.imb = hb ? d.inv_mass[x.b] : 0.0f;
+    x.am = x.ima > 0.0f && d.awake[x.a];
+    x.bm = hb && x.imb > 0.0f && d.awake[x.b];
+    x.la4 = d.pc_la[c];
+    x.n = L3(d.cs_n[c]);
+    x.lb = hb ? L3(d.pc_lb[c]) : splat3(0.0f);
+    x.pt = hb ? splat3(0.0f) : L3(d.ct_p[c]);
+    x.ila = load_m9(d.iil + 9 * x.a);
+    x.ilb = hb ? load_m9(d.iil + 9 * x.b) : mzero();
+    return x;
+}
+static __device__ __forceinline__ void store_prow(const Dev &d, uint32_t slot, const PRow &x) {
+    float4 *o = d.srow + slot;
+    const size_t C = d.cap_contacts;
+    o[0] = f4u(x.a, x.b, x.ima, x.imb);
+    o[C] = make_float4(__uint_as_float((x.am ? 1u : 0u) | (x.bm ? 2u : 0u)), x.lb.x, x.lb.y, x.lb.z);
+    o[2 * C] = x.la4; o[3 * C] = S3(x.n, x.pt.x); o[4 * C] = make_float4(x.pt.y, x.pt.z, x.ila.c0.x, x.ila.c0.y);
+    o[5 * C] = make_float4(x.ila.c0.z, x.ila.c1.x, x.ila.c1.y, x.ila.c1.z); o[6 * C] = make_float4(x.ila.c2.x, x.ila.c2.y, x.ila.c2.z, x.ilb.c0.x);
+    o[7 * C] = make_float4(x.ilb.c0.y, x.ilb.c0.z, x.ilb.c1.x, x.ilb.c1.y); o[8 * C] = make_float4(x.ilb.c1.z, x.ilb.c2.x, x.ilb.c2.y, x.ilb.c2.z);
+}
+static __device__ __forceinline__ PRow fetch_prow(const Dev &d, uint32_t slot) {
+    const float4 *o = d.srow + slot;
+    const size_t C = d.cap_contacts;
+    float4 f0 = o[0], f1 = o[C], f2 = o[2 * C], f3 = o[3 * C], f4 = o[4 * C], f5 = o[5 * C], f6 = o[6 * C], f7 = o[7 * C], f8 = o[8 * C];
+    PRow x;
+    x.a = __float_as_uint(f0.x); x.b = __float_as_uint(f0.y); x.ima = f0.z; x.imb = f0.w;
+    const uint32_t fl = __float_as_uint(f1.x);
+    x.am = (fl & 1u) != 0; x.bm = (fl & 2u) != 0;
+    x.lb = mk3(f1.y, f1.z, f1.w); x.la4 = f2; x.n = L3(f3); x.pt = mk3(f3.w, f4.x, f4.y);
+    x.ila.c0 = mk3(f4.z, f4.w, f5.x); x.ila.c1 = mk3(f5.y, f5.z, f5.w); x.ila.c2 = mk3(f6.x, f6.y, f6.z);
+    x.ilb.c0 = mk3(f6.w, f7.x, f7.y); x.ilb.c1 = mk3(f7.z, f7.w, f8.x); x.ilb.c2 = mk3(f8.y, f8.z, f8.w);
+    return x;
+}
+static __device__ __forceinline__ M3 inertia_world_of(M3 il, Q4 q) {
+    M3 r = mquat(q);
+    return mmul(mmul(r, il), mtrans(r));
+}
+static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 up) {
+    const uint32_t a = x.a, b = x.b;
+    const bool hb = b != RP_NONE;
+    bool am = x.am, bm = x.bm;
     if (shock && am && bm) {
         float ha = dot3(L3(d.pos[a]), up), hbb = dot3(L3(d.pos[b]), up);
         if (ha < hbb) am = false; else bm = false;
     }
-    float4 la4 = d.pc_la[c];
-    V3 n = L3(d.cs_n[c]);
+    const float4 la4 = x.la4;
+    const V3 n = x.n;
     V3 pa = L3(d.pos[a]); Q4 qa = LQ(d.rot[a]);
     V3 wa = pa + qrot(qa, L3(la4));
     V3 pb = splat3(0.0f), wb; Q4 qb = qid();
-    if (hb) { pb = L3(d.pos[b]); qb = LQ(d.rot[b]); wb = pb + qrot(qb, L3(d.pc_lb[c])); }
-    else wb = L3(d.ct_p[c]);
+    if (hb) { pb = L3(d.pos[b]); qb = LQ(d.rot[b]); wb = pb + qrot(qb, x.lb); }
+    else wb = x.pt;
     float pen = la4.w - dot3(wb - wa, n);
     float err = pen - 0.005f;
     if (err <= 0.0f) return;
     V3 ra = wa - pa, rb = hb ? wb - pb : splat3(0.0f);
     float ima = 0.0f, imb = 0.0f; M3 iia = mzero(), iib = mzero();
-    if (am) { ima = d.inv_mass[a]; iia = inv_inertia_world(d, a, qa); }
-    if (hb && bm) { imb = d.inv_mass[b]; iib = inv_inertia_world(d, b, qb); }
+    if (am) { ima = x.ima; iia = inertia_world_of(x.ila, qa); }
+    if (hb && bm) { imb = x.imb; iib = inertia_world_of(x.ilb, qb); }
     V3 ca = cross3(ra, n), cb = cross3(rb, n);
     float wsum = ((ima + imb) + dot3(ca, mv(iia, ca))) + dot3(cb, mv(iib, cb));
     if (wsum <= 1.0e-9f) return;
@@ -1151,21 +1433,35 @@ static __device__ void project_one(const Dev &d, uint32_t c, bool shock, V3 up) 
     }
 }
 
-template <bool USE_LDS>
-__global__ void __launch_bounds__(256) k_sweep_proj(Dev d, TickParams tp) {
+// Position projection iterations in list order (step.rs:297-310); the shock pass follows in its
+// own launch once the shock order of the projected poses is built.
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     ISLAND_LOOP_BEGIN
-        uint32_t token = 1;
+        PhaseClock pc;
+        const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
+        __syncthreads();
         for (uint32_t it = 0; it < tp.ip; it++)
-            run_sweep(C, n, token++, [&](uint32_t r) { project_one(d, lst[r], false, up); });
-        shock_order(C, n, [&](uint32_t i) {
-            uint32_t c = lst[i];
-            uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
-            return b == RP_NONE ? -INFINITY : fminf(dot3(L3(d.pos[a]), up), dot3(L3(d.pos[b]), up));
-        });
-        build_chains(d, C, lst, n);
-        run_sweep(C, n, token++, [&](uint32_t r) { project_one(d, lst[C.order[r]], true, up); });
+            run_sweep(I, sb, [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
+    ISLAND_LOOP_END
+}
+// Projection shock pass (step.rs:299-301, 447-497 with shock = true) in shock order.
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    ISLAND_LOOP_BEGIN
+        PhaseClock pc;
+        const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK]) : nullptr;
+        const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
+        build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, ss);
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ_SHOCK_ORDER);
+        run_sweep(I, ss, [&](uint32_t c, uint32_t) { return load_prow(d, c); }, [&](uint32_t, const PRow &x) { project_row(d, x, true, up); });
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ_SHOCK);
     ISLAND_LOOP_END
 }
 
@@ -1214,8 +1510,10 @@ __global__ void k_restore_backup(Dev d) {
 }
 
 // Capacity checks after each sizing scan; sets the abort flag so later kernels return at once.
+// Once a tick is aborted the later sizing scans run over stale counts, so their checks record
+// nothing: only the first overflow's needs are grown before the replay.
 __global__ void k_check(Dev d, uint32_t which) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0 || d.cnt[CT_ABORT]) return;
     if (which == 0) {
         if (d.cnt[CT_PAIRS] > d.cap_pairs) { d.cnt[CT_NEED_PAIRS] = d.cnt[CT_PAIRS]; d.cnt[CT_ABORT] = 1; }
         if (d.cnt[CT_CANDS] > d.cap_cands) { d.cnt[CT_NEED_CANDS] = d.cnt[CT_CANDS]; d.cnt[CT_ABORT] = 1; }
@@ -1224,11 +1522,16 @@ __global__ void k_check(Dev d, uint32_t which) {
     } else if (which == 2) {
         uint32_t wt = d.cnt[CT_NEW_TOTAL] + d.cnt[CT_KEEP_TOTAL];
         if (wt > d.cap_warm) { d.cnt[CT_NEED_WARM] = wt; d.cnt[CT_ABORT] = 1; }
-        if (d.cnt[CT_BIG_ISLAND] != 0) d.cnt[CT_ABORT] = 1;
     }
 }
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) { GRID_STRIDE(i, n) p[i] = 0; }
-__global__ void k_tick_done(Dev d) { if (threadIdx.x == 0 && blockIdx.x == 0 && !d.cnt[CT_ABORT]) d.cnt[CT_DONE] += 1; }
+__global__ void k_tick_done(Dev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || d.cnt[CT_ABORT]) return;
+    uint32_t *c = d.cnt;
+    c[CT_DONE] += 1;
+    c[CT_SUM_PAIRS] += c[CT_PAIRS]; c[CT_SUM_CANDS] += c[CT_CANDS]; c[CT_SUM_CONTACTS] += c[CT_CONTACTS];
+    c[CT_SUM_ISLC] += c[CT_ISLC]; c[CT_SUM_ISLANDS] += c[CT_ISLANDS]; c[CT_SUM_ISLC_GLB] += c[CT_ISLC_GLB];
+}
 __global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) cnt[idx] = v; }
 __global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n) { GRID_STRIDE(i, n) dst[i] = src[i]; }
 
@@ -1260,22 +1563,63 @@ static void radix_sort(const Dev &d, uint32_t *k0, uint32_t *v0, uint32_t *k1, u
     if (nblocks < 1) nblocks = 1;
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (uint32_t shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(256), 0, st, ka, n_dev, shift, nblocks, d.rs_hist);
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(256), 0, st, ka, n_dev, cap, shift, nblocks, d.rs_hist);
         scan_u32(d.rs_hist, 256 * nblocks, d.scan_tmp, d.cnt + CT_SCRATCH, st);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(256), 0, st, ka, va, n_dev, shift, nblocks, d.rs_hist, kb, vb);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(256), 0, st, ka, va, n_dev, cap, shift, nblocks, d.rs_hist, kb, vb);
         uint32_t *t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
     }
     *k_out = ka; *v_out = va;
 }
 
-// Enqueues one tick; returns the launch status (hipGetLastError after the last launch).
-extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st) {
+// Radix bits covering keys 0..maxkey, in whole 8-bit passes.
+static uint32_t key_bits(uint32_t maxkey) {
+    uint32_t b = 8;
+    while (b < 32 && (uint64_t)maxkey >= (1ull << b)) b += 8;
+    return b;
+}
+
+// Chains of the order `ord` (contact at each island-list rank) into (pa, pb).
+static void build_chains(const Dev &d, const uint32_t *ord, uint32_t *pa, uint32_t *pb, uint32_t *sa, uint32_t *sb, uint32_t body_bits,
+                         hipStream_t st) {
+    const uint32_t B = 256, cap2 = 2u * d.cap_contacts;
+    hipLaunchKernelGGL(k_chain_entries, dim3(grid_for(cap2, B)), dim3(B), 0, st, d, ord, d.sk0, d.sv0);
+    uint32_t *ks, *vs;
+    radix_sort(d, d.sk0, d.sv0, d.sk1, d.sv1, d.cnt + CT_CHAIN_N, cap2, body_bits, st, &ks, &vs);
+    hipLaunchKernelGGL(k_chain_link, dim3(grid_for(cap2, B)), dim3(B), 0, st, d, ks, vs, pa, pb, sa, sb);
+}
+
+// Shock order of every island from the current poses into sh_ord, with its chains: stable radix
+// passes over the key's low 32 bits, then its high 12 bits, then the island (root body) key,
+// starting from list order, so equal keys keep list order.
+static void build_shock_order(const Dev &d, const TickParams &tp, uint32_t body_bits, hipStream_t st) {
+    const uint32_t B = 256, C = d.cap_contacts;
+    const uint32_t *n_dev = d.cnt + CT_ISLC;
+    hipLaunchKernelGGL(k_shock_keys, dim3(grid_for(C, B)), dim3(B), 0, st, d, tp, d.sk0, d.sh_hi, d.sv0);
+    uint32_t *ka, *va, *kb, *vb, *kc, *vc;
+    radix_sort(d, d.sk0, d.sv0, d.sk1, d.sv1, n_dev, C, 32, st, &ka, &va);
+    uint32_t *kf = ka == d.sk0 ? d.sk1 : d.sk0, *vf = va == d.sv0 ? d.sv1 : d.sv0;
+    hipLaunchKernelGGL(k_gather_keys, dim3(grid_for(C, B)), dim3(B), 0, st, d, d.sh_hi, va, kf);
+    radix_sort(d, kf, va, ka, vf, n_dev, C, 16, st, &kb, &vb);
+    kf = kb == d.sk0 ? d.sk1 : d.sk0;
+    vf = vb == d.sv0 ? d.sv1 : d.sv0;
+    hipLaunchKernelGGL(k_gather_keys, dim3(grid_for(C, B)), dim3(B), 0, st, d, d.isl_k2, vb, kf);
+    radix_sort(d, kf, vb, kb, vf, n_dev, C, body_bits, st, &kc, &vc);
+    hipLaunchKernelGGL(k_shock_order, dim3(grid_for(C, B)), dim3(B), 0, st, d, vc);
+    build_chains(d, d.sh_ord, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, body_bits, st);
+}
+
+// Enqueues one tick; returns the launch status (hipGetLastError after the last launch).  When ev
+// is given, ev[k] is recorded on st at stage boundary k (RP_NMARKS events; stage k spans
+// ev[k]..ev[k+1]).
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st, hipEvent_t *ev) {
     const uint32_t B = 256;
-    const uint32_t big_grid = island_grid < 256u ? island_grid : 256u;  // islands above the LDS size are few
+    auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev[k], st); };
+    mark(RP_ST_PRE);
     uint32_t *scratch_total = d.cnt + CT_SCRATCH;
     hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
+    mark(RP_ST_BROAD);
     hipLaunchKernelGGL(k_bp_sort, dim3(d.nw), dim3(1024), 0, st, d);
     hipLaunchKernelGGL(k_bp_enum<0>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     scan_u32(d.out_off, d.nc, d.scan_tmp, d.cnt + CT_PAIRS, st);
@@ -1284,30 +1628,46 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.cand_off, d.nc, d.cnt + CT_CANDS);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 0u);
     hipLaunchKernelGGL(k_bp_enum<1>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    mark(RP_ST_NARROW_PAIRS);
     hipLaunchKernelGGL(k_narrow_pairs, dim3(grid_for(d.cap_pairs, B)), dim3(B), 0, st, d);
+    mark(RP_ST_NARROW_TERRAIN);
     hipLaunchKernelGGL(k_narrow_terrain, dim3(grid_for(d.cap_cands, B, 16384)), dim3(B), 0, st, d);
+    mark(RP_ST_CONTACTS);
     hipLaunchKernelGGL(k_terrain_cull, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_contact_count, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     scan_u32(d.c_off, d.nc, d.scan_tmp, d.cnt + CT_CONTACTS, st);
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.c_off, d.nc, d.cnt + CT_CONTACTS);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 1u);
     hipLaunchKernelGGL(k_contact_write, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    mark(RP_ST_PREP);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_solve, d.nw);
     hipLaunchKernelGGL(k_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_island_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    const uint32_t body_bits = key_bits(d.nb);  // body ids and the nb sentinel
     uint32_t *ks, *vs;
-    radix_sort(d, d.isl_k, d.isl_v, d.isl_k2, d.isl_v2, d.cnt + CT_CONTACTS, d.cap_contacts, 32, st, &ks, &vs);
-    // four passes leave the result in (isl_k, isl_v); the sweeps read isl_v2 / isl_k2
-    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, vs, d.isl_v2, d.cap_contacts);
-    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, ks, d.isl_k2, d.cap_contacts);
+    radix_sort(d, d.isl_k, d.isl_v, d.isl_k2, d.isl_v2, d.cnt + CT_CONTACTS, d.cap_contacts, body_bits, st, &ks, &vs);
+    if (vs != d.isl_v2) {  // the sweeps read (isl_k2, isl_v2)
+        hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, vs, d.isl_v2, d.cap_contacts);
+        hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, ks, d.isl_k2, d.cap_contacts);
+    }
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC, 0u);
     hipLaunchKernelGGL(k_island_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     scan_u32(d.isl_flag, d.cap_contacts, d.scan_tmp, d.cnt + CT_ISLANDS, st);
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
-    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_BIG_ISLAND, 0u);
-    hipLaunchKernelGGL(k_sweep_vel<true>, dim3(island_grid), dim3(256), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_vel<false>, dim3(big_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
+    hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
+    mark(RP_ST_ORDERS);
+    build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
+    build_shock_order(d, tp, body_bits, st);
+    const uint32_t g0 = island_grid, g1 = island_grid < 256u ? island_grid : 256u, g2 = 64u;
+    mark(RP_ST_VEL_SMALL);
+    hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+    mark(RP_ST_VEL_LARGE);
+    hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+    mark(RP_ST_VEL_HUGE);
+    hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    mark(RP_ST_WARM_STORE);
     // warm cache for the next tick (into buffer cur ^ 1)
     hipLaunchKernelGGL(k_warm_newflags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     scan_u32(d.newflag, d.cap_contacts, d.scan_tmp, d.cnt + CT_NEW_TOTAL, st);
@@ -1318,15 +1678,31 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 2u);
     hipLaunchKernelGGL(k_warm_merge, dim3(grid_for(d.cap_contacts > d.cap_warm ? d.cap_contacts : d.cap_warm, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
-    hipLaunchKernelGGL(k_sweep_ball<true>, dim3(island_grid), dim3(256), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_ball<false>, dim3(big_grid), dim3(256), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    mark(RP_ST_INTEGRATE);
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
+    mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
-    hipLaunchKernelGGL(k_sweep_proj<true>, dim3(island_grid), dim3(256), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_proj<false>, dim3(big_grid), dim3(256), 0, st, d, tp);
+    mark(RP_ST_PROJ_SMALL);
+    hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+    mark(RP_ST_PROJ_LARGE);
+    hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+    mark(RP_ST_PROJ_HUGE);
+    hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    mark(RP_ST_PROJ_ORDER);
+    build_shock_order(d, tp, body_bits, st);  // from the projected poses
+    mark(RP_ST_PROJ_SHOCK);
+    hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+    hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    mark(RP_ST_S6);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);
+    mark(RP_ST_END);
     (void)scratch_total;
     return hipGetLastError();
 }

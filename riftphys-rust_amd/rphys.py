@@ -84,6 +84,12 @@ class ContactEx(C.Structure):
                 ("depth", C.c_float), ("impulse", C.c_float), ("feature", C.c_uint32), ("cell", C.c_uint32)]
 
 
+class Profile(C.Structure):
+    _fields_ = [("ticks", C.c_uint32), ("n_stages", C.c_uint32), ("stage_ms", C.c_double * 24), ("pairs", C.c_double),
+                ("terrain_candidates", C.c_double), ("contacts", C.c_double), ("island_contacts", C.c_double), ("islands", C.c_double),
+                ("island_contacts_global", C.c_double), ("sweep_phase", C.c_double * 11)]
+
+
 # (name, restype, argtypes) of every export in include/riftphys_c.h
 P = C.c_void_p
 U32, U64, F32 = C.c_uint32, C.c_uint64, C.c_float
@@ -129,6 +135,8 @@ EXPORTS = [
     ("rphys_body_set_sleep_allowed", U32, [P, U32, U32]),
     ("rphys_world_step_n", U32, [P, F32, U32]),
     ("rphys_world_get_stats", U32, [P, C.POINTER(StepStats)]),
+    ("rphys_world_step_profiled", U32, [P, F32, U32, C.POINTER(Profile)]),
+    ("rphys_profile_stage_name", C.c_char_p, [U32]),
     ("rphys_debug_get_contacts_ex", U32, [P, C.POINTER(ContactEx), U32, PU32]),
     ("rphys_debug_get_pairs", U32, [P, PU32, U32, PU32]),
     ("rphys_debug_export_state", U32, [P, PF32, PF32, PF32, PF32, PF32, PU32, PU32, C.POINTER(C.c_uint64), PF32, PF32, PF32, U32, PU32]),
@@ -244,6 +252,18 @@ class World:
 
     def step_n(self, n, dt=None):
         check(lib().rphys_world_step_n(self.h, self.dt if dt is None else dt, n))
+
+    def step_profiled(self, n, dt=None):
+        """Runs n ticks with per-stage HIP event timing; returns (stage name -> ms summed, counters)."""
+        pr = Profile()
+        check(lib().rphys_world_step_profiled(self.h, self.dt if dt is None else dt, n, C.byref(pr)))
+        stages = {lib().rphys_profile_stage_name(k).decode(): pr.stage_ms[k] for k in range(pr.n_stages)}
+        counters = dict(ticks=pr.ticks, pairs=pr.pairs, terrain_candidates=pr.terrain_candidates, contacts=pr.contacts,
+                        island_contacts=pr.island_contacts, islands=pr.islands, island_contacts_global=pr.island_contacts_global)
+        names = ("vel_order", "warm", "pgs", "vel_shock_order", "vel_shock", "ballistic", "projection", "proj_shock_order",
+                 "proj_shock", "levels_list", "levels_shock")
+        counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}
+        return stages, counters
 
     def hash(self):
         h = Hash32()

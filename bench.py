@@ -128,7 +128,13 @@ def main():
     ap.add_argument("--cpu-ticks", type=int, default=3, help="oracle ticks sampled per 100-tick chunk")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle sample (profiling runs)")
     ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (bandwidth reference, counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        nbytes = a.copy_probe << 20
+        print(json.dumps({"copy_probe_bytes": nbytes, "reps": 20, "gbps_read_plus_write": rphys.copy_probe(nbytes, 20)}), flush=True)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: the step library has no multi-GPU path (independent worlds would shard by "
               "world across GPUs with no per-tick exchange). Run with --gpus 1.", file=sys.stderr)

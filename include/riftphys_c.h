@@ -188,6 +188,10 @@ typedef struct RPhysProfile {
     double sweep_phase[11];
 } RPhysProfile;
 RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out);
+/* Bandwidth probe (build addition): copies `bytes` between two fresh device buffers `reps` times
+ * with a plain copy kernel and returns read + write bytes per second in GB/s.  A reference for
+ * the step's roofline figures and for calibrating HBM traffic counters. */
+RPhysResult rphys_debug_copy_probe(uint64_t bytes, uint32_t reps, double *gbps);
 const char *rphys_profile_stage_name(uint32_t stage);
 
 /* One contact with its build-defined feature id (kind<<24 | which<<16 | k, bit 23 = swapped

@@ -28,6 +28,7 @@ struct HashDev {
 };
 extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st, hipEvent_t *ev);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
+extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uint32_t max_extra_slots, hipStream_t st);
 
 // ------------------------------------------------------------------ errors (lib.rs:56-82)
@@ -910,6 +911,35 @@ RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ti
     }
     world->dev_newer = true;
     world->have_contacts = ok;
+    return ok ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+    GUARD_END
+}
+
+RPhysResult rphys_debug_copy_probe(uint64_t bytes, uint32_t reps, double *gbps) {
+    GUARD_BEGIN
+    if (!gbps) { set_err("rphys_debug_copy_probe: null arg"); return RPHYS_ERR_NULL; }
+    *gbps = 0.0;
+    if (bytes < 16 || reps == 0) { set_err("rphys_debug_copy_probe: bytes >= 16 and reps > 0 required"); return RPHYS_ERR_BAD_ARG; }
+    const size_t n = (size_t)(bytes / 16);
+    DeviceBuffers B;
+    float4 *a = nullptr, *b = nullptr;
+    if (!B.alloc(a, n) || !B.alloc(b, n)) { set_err("rphys_debug_copy_probe: device allocation failed"); return RPHYS_ERR_CAPACITY; }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") && hip_ok(hipEventCreate(&e0), "hipEventCreate") &&
+              hip_ok(hipEventCreate(&e1), "hipEventCreate") && hip_ok(hipMemsetAsync(a, 0, n * 16, st), "memset");
+    if (ok) {
+        rp_launch_copy_probe(a, b, n, st);  // warm
+        ok = hip_ok(hipEventRecord(e0, st), "hipEventRecord");
+        for (uint32_t r = 0; ok && r < reps; r++) rp_launch_copy_probe(a, b, n, st);
+        float ms = 0.0f;
+        ok = ok && hip_ok(hipEventRecord(e1, st), "hipEventRecord") && hip_ok(hipEventSynchronize(e1), "hipEventSynchronize") &&
+             hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (ok && ms > 0.0f) *gbps = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
     return ok ? RPHYS_OK : RPHYS_ERR_INTERNAL;
     GUARD_END
 }

@@ -1707,6 +1707,14 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     return hipGetLastError();
 }
 
+// Plain copy of n float4 (bandwidth reference for the roofline and for calibrating HBM counters).
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, st, src, dst, n);
+}
+
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st) {
     hipLaunchKernelGGL(k_restore_backup, dim3(grid_for(d.nb, 256)), dim3(256), 0, st, d);
 }

@@ -137,6 +137,7 @@ EXPORTS = [
     ("rphys_world_get_stats", U32, [P, C.POINTER(StepStats)]),
     ("rphys_world_step_profiled", U32, [P, F32, U32, C.POINTER(Profile)]),
     ("rphys_profile_stage_name", C.c_char_p, [U32]),
+    ("rphys_debug_copy_probe", U32, [U64, U32, C.POINTER(C.c_double)]),
     ("rphys_debug_get_contacts_ex", U32, [P, C.POINTER(ContactEx), U32, PU32]),
     ("rphys_debug_get_pairs", U32, [P, PU32, U32, PU32]),
     ("rphys_debug_export_state", U32, [P, PF32, PF32, PF32, PF32, PF32, PU32, PU32, C.POINTER(C.c_uint64), PF32, PF32, PF32, U32, PU32]),
@@ -188,6 +189,13 @@ def iso(p, q):
 
 def vel(l, a):
     return Velocity(Vec3(*[float(x) for x in l]), Vec3(*[float(x) for x in a]))
+
+
+def copy_probe(nbytes, reps=20):
+    """GB/s (read + write) of the library's plain copy kernel over nbytes (rphys_debug_copy_probe)."""
+    g = C.c_double(0.0)
+    check(lib().rphys_debug_copy_probe(nbytes, reps, C.byref(g)))
+    return g.value
 
 
 class World:

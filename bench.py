@@ -119,12 +119,140 @@ def cpu_baseline(scene, strata, ticks_per_stratum, dt):
     }
 
 
+def merge_profile(stages, ctr, st, c):
+    for key, v in st.items():
+        stages[key] = stages.get(key, 0.0) + v
+    for key, v in c.items():
+        if isinstance(v, dict):
+            acc = ctr.setdefault(key, {})
+            for k2, v2 in v.items():
+                acc[k2] = acc.get(k2, 0.0) + v2
+        else:
+            ctr[key] = ctr.get(key, 0) + v
+
+
+def roofline_of(stages, ctr, iv, ip, workload):
+    cand = {k: v for k, v in stages.items() if k in KERNEL_OF_STAGE}
+    dom = max(cand, key=cand.get)
+    avg_ms = cand[dom] / max(ctr["ticks"], 1)
+    bytes_launch = stage_bytes(dom, ctr, iv, ip) / max(ctr["ticks"], 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(KERNEL_OF_STAGE[dom], workload), "kernel": KERNEL_OF_STAGE[dom],
+            "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": avg_ms}
+
+
+def print_stages(stages, ctr):
+    tot = sum(stages.values())
+    print("stage split over %d ticks (ms/tick):" % ctr["ticks"], file=sys.stderr)
+    for k, v in stages.items():
+        print("  %-18s %8.3f  %5.1f%%" % (k, v / ctr["ticks"], 100.0 * v / tot), file=sys.stderr)
+    print("counters per tick: " + ", ".join("%s %.0f" % (k, v / ctr["ticks"]) for k, v in ctr.items()
+                                             if k not in ("ticks", "sweep_phase")), file=sys.stderr)
+    print("large-island sweep phases per tick (ms; levels): " + ", ".join(
+        "%s %.3f" % (k, v / ctr["ticks"]) for k, v in ctr["sweep_phase"].items()), file=sys.stderr)
+
+
+def batch_bench(a):
+    """Batched workload: many independent worlds in one launch sequence, each replaying 8-tick
+    rollback windows from a device snapshot (snapshot, 8 ticks, restore, the same 8 ticks)."""
+    t0 = time.perf_counter()
+    worlds = [scenes.batch_world(w) for w in range(a.worlds)]
+    handles = [rphys.World(s) for s in worlds]
+    b = rphys.Batch(handles)
+    del handles
+    nb = b.total_bodies()
+    dt = float(worlds[0].get("dt", 1.0 / 60.0))
+    iv, ip = worlds[0].get("iters", (10, 20))
+    wl = {"workload": "batch%d_rollback8" % a.worlds, "worlds": a.worlds, "bodies": nb,
+          "bodies_per_world": "round(exp(U(ln 32, ln 2048))), mixed shapes over per-world fBm terrain",
+          "rollback_window": 8, "dt": "1/60", "vel_iters": iv, "pos_iters": ip}
+    print("bench: %s, %d bodies, built in %.1f s; warmup %d ticks" % (wl["workload"], nb, time.perf_counter() - t0, a.warmup),
+          file=sys.stderr, flush=True)
+    if a.warmup > 0:
+        b.step_n(a.warmup, dt, graph=True)
+    b.hashes()
+    b.snapshot(1)  # start of the timed windows (both passes below start here)
+    windows = max(1, a.steps // 16)
+    # timed: graph-captured ticks, each window snapshot -> 8 ticks -> restore -> the same 8 ticks
+    el = 0.0
+    for k in range(windows):
+        t1 = time.perf_counter()
+        b.snapshot(0)
+        b.step_n(8, dt, graph=True)
+        b.restore(0)
+        b.step_n(8, dt, graph=True)
+        el += time.perf_counter() - t1
+        if (k + 1) % 5 == 0 or k + 1 == windows:
+            print("bench: %d/%d windows, %.2f s" % (k + 1, windows, el), file=sys.stderr, flush=True)
+    ticks = 16 * windows
+    value = nb * ticks / el
+    # the same windows again with HIP events at every stage boundary (roofline, stage split)
+    b.restore(1)
+    stages, ctr = {}, {}
+    el_prof = 0.0
+    for k in range(windows):
+        t1 = time.perf_counter()
+        b.snapshot(0)
+        merge_profile(stages, ctr, *b.step_profiled(8, dt))
+        b.restore(0)
+        merge_profile(stages, ctr, *b.step_profiled(8, dt))
+        el_prof += time.perf_counter() - t1
+    print("bench: profiled pass %.2f s" % el_prof, file=sys.stderr, flush=True)
+    # rollback check (untimed hashes)
+    b.snapshot(0)
+    b.step_n(8, dt, graph=True)
+    h1 = b.hashes()
+    b.restore(0)
+    b.step_n(8, dt, graph=True)
+    replay_equal = h1 == b.hashes()
+    if a.stages:
+        print_stages(stages, ctr)
+    base = None
+    if not a.no_cpu:
+        print("bench: oracle sample over %d worlds" % len(range(0, a.worlds, max(1, a.worlds // 8))), file=sys.stderr, flush=True)
+        base = batch_cpu_baseline(worlds, a.warmup, ticks, dt)
+    wl.update({"contacts_per_tick": ctr["contacts"] / ctr["ticks"], "ticks": "%d..%d, each window stepped twice" % (a.warmup, a.warmup + ticks // 2),
+               "profiled_pass_body_steps_per_s": nb * ticks / el_prof, "replayed_hashes_equal": replay_equal})
+    return {"metric": "body-steps/s on 1 GPU (plain method, batched worlds with rollback windows)", "value": value,
+            "unit": "body-steps/s", "n_gpus": 1, "steps": ticks, "warmup": a.warmup, "ms_per_step": 1000.0 * el / ticks,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded XorShift64 scenes)", "config": wl, "roofline": roofline_of(stages, ctr, iv, ip, wl["workload"]),
+            "cpu_baseline": base}
+
+
+def batch_cpu_baseline(worlds, warmup, ticks, dt):
+    """The oracle on every (n/8)-th world: its trajectory from spawn through the timed ticks (a
+    rollback window's replay repeats the same kind of tick), timing the ticks after warmup (the
+    distinct ticks of the timed windows, capped at 40 per world to bound the sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+
+    sample = worlds[:: max(1, len(worlds) // 8)]
+    span = min(40, ticks // 2)
+    bodies, t_spent = 0, 0.0
+    for s in sample:
+        o = oracle_py.OracleWorld(s)
+        for _ in range(warmup):
+            o.step(dt)
+        t0 = time.perf_counter()
+        for _ in range(span):
+            o.step(dt)
+        t_spent += time.perf_counter() - t0
+        bodies += len(s["shape"]) * span
+        del o
+    return {"value": bodies / t_spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d of %d worlds (every %d-th), %d ticks each after %d warmup ticks (%.1f s; %s, %d host cores, 1 used)"
+            % (len(sample), len(worlds), max(1, len(worlds) // 8), span, warmup, t_spent, cpu_model(), os.cpu_count() or 0)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["mixed16k", "boxes512"], default="mixed16k")
+    ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch"], default="mixed16k")
+    ap.add_argument("--worlds", type=int, default=1024, help="batch workload: number of worlds")
     ap.add_argument("--cpu-ticks", type=int, default=3, help="oracle ticks sampled per 100-tick chunk")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle sample (profiling runs)")
     ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
@@ -140,6 +268,13 @@ def main():
               "world across GPUs with no per-tick exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "batch":
+        if a.steps == 600:
+            a.steps = 160
+        if a.warmup == 10:
+            a.warmup = 150
+        print(json.dumps(batch_bench(a)), flush=True)
+        return
     if a.workload == "mixed16k":
         scene = scenes.mixed_shapes()
         wl = {"workload": "mixed16k_fbm256", "bodies": 16384, "shapes": "1/3 sphere, 1/3 box, 1/3 capsule",
@@ -169,40 +304,14 @@ def main():
         t0 = time.perf_counter()
         st, c = w.step_profiled(k, dt)
         el += time.perf_counter() - t0
-        for key, v in st.items():
-            stages[key] = stages.get(key, 0.0) + v
-        for key, v in c.items():
-            if isinstance(v, dict):
-                acc = ctr.setdefault(key, {})
-                for k2, v2 in v.items():
-                    acc[k2] = acc.get(k2, 0.0) + v2
-            else:
-                ctr[key] = ctr.get(key, 0) + v
+        merge_profile(stages, ctr, st, c)
         done += k
         print("bench: %d/%d ticks, %.2f s" % (done, a.steps, el), file=sys.stderr, flush=True)
     value = nb * a.steps / el
 
-    # dominant single-kernel stage
-    cand = {k: v for k, v in stages.items() if k in KERNEL_OF_STAGE}
-    dom = max(cand, key=cand.get)
-    avg_ms = cand[dom] / max(ctr["ticks"], 1)
-    bytes_total = stage_bytes(dom, ctr, iv, ip)
-    bytes_launch = bytes_total / max(ctr["ticks"], 1)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = measured_traffic(KERNEL_OF_STAGE[dom], wl["workload"])
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic, "kernel": KERNEL_OF_STAGE[dom], "algorithmic_bytes_per_launch": bytes_launch,
-            "mean_launch_ms": avg_ms}
-
+    roof = roofline_of(stages, ctr, iv, ip, wl["workload"])
     if a.stages:
-        tot = sum(stages.values())
-        print("stage split over %d ticks (ms/tick):" % ctr["ticks"], file=sys.stderr)
-        for k, v in stages.items():
-            print("  %-18s %8.3f  %5.1f%%" % (k, v / ctr["ticks"], 100.0 * v / tot), file=sys.stderr)
-        print("counters per tick: " + ", ".join("%s %.0f" % (k, v / ctr["ticks"]) for k, v in ctr.items()
-                                                 if k not in ("ticks", "sweep_phase")), file=sys.stderr)
-        print("large-island sweep phases per tick (ms; levels): " + ", ".join(
-            "%s %.3f" % (k, v / ctr["ticks"]) for k, v in ctr["sweep_phase"].items()), file=sys.stderr)
+        print_stages(stages, ctr)
 
     if not a.no_cpu:
         print("bench: oracle sample, %d ticks from each of %d states" % (a.cpu_ticks, len(strata)), file=sys.stderr, flush=True)

@@ -155,6 +155,11 @@ RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t mat_a, uint32
                                        float restitution, float stribeck_v);
 /* World::set_solver_iterations (world.rs:994-996), (velocity, position), default (20, 20) */
 RPhysResult rphys_world_set_solver_iterations(RPhysWorld *world, uint32_t velocity, uint32_t position);
+/* Residual mode (build addition; the reference runs a fixed count): each island (component of the
+ * bodies the solve writes) runs PGS sweeps until its largest row change
+ * max(|dJn|/m_n/(1+|bias|), |dJt1|/m_t1, |dJt2|/m_t2) <= tol (m/s; the sweep criterion of
+ * solver.rs:1149), at most max_iters.  on = 0 returns to the fixed velocity iterations. */
+RPhysResult rphys_world_set_residual_mode(RPhysWorld *world, uint32_t on, float tol, uint32_t max_iters);
 RPhysResult rphys_world_set_gravity(RPhysWorld *world, RPhysVec3 g);          /* World::set_gravity (world.rs:560-563) */
 RPhysResult rphys_body_set_sleep_allowed(RPhysWorld *world, uint32_t body_id, RPhysBool allowed);  /* world.rs:565-570 */
 /* n ticks of dt enqueued back to back with one host synchronisation at the end */
@@ -167,6 +172,8 @@ typedef struct RPhysStepStats {
     uint32_t island_contacts;    /* contacts writing at least one body */
     uint32_t islands;
     uint32_t warm_entries;       /* warm-start cache entries after the tick */
+    uint32_t pgs_iters;          /* PGS sweeps run by the island that ran the most */
+    uint32_t pgs_iters_sum;      /* sum over islands of (sweeps x the island's contacts) */
 } RPhysStepStats;
 RPhysResult rphys_world_get_stats(const RPhysWorld *world, RPhysStepStats *out);
 
@@ -181,6 +188,7 @@ typedef struct RPhysProfile {
     double stage_ms[RPHYS_PROFILE_MAX_STAGES];
     double pairs, terrain_candidates, contacts, island_contacts, islands;
     double island_contacts_global; /* contacts of the islands solved by the 1024-thread LDS-flag tier */
+    double pgs_iters_max, pgs_iters_sum;  /* per-tick RPhysStepStats.pgs_iters / pgs_iters_sum, summed */
     /* Inside the sweeps of the islands larger than 4,096 contacts (summed over their workgroups):
      * ms in [0] velocity level order, [1] warm start, [2] PGS iterations, [3] velocity shock order,
      * [4] velocity shock pass, [5] ballistic pass, [6] projection iterations, [7] projection shock
@@ -234,6 +242,7 @@ RPhysResult rphys_batch_restore(RPhysBatch *batch, uint32_t slot);
 RPhysResult rphys_batch_total_bodies(const RPhysBatch *batch, uint32_t *out);
 /* Record the next tick of the batch into a HIP graph and replay it for n ticks (same results as
  * rphys_batch_step_n; launch overhead paid once). */
+RPhysResult rphys_batch_step_profiled(RPhysBatch *batch, float dt, uint32_t n_ticks, RPhysProfile *out); /* as rphys_world_step_profiled */
 RPhysResult rphys_batch_step_graph(RPhysBatch *batch, float dt, uint32_t n_ticks);
 
 #ifdef __cplusplus

@@ -606,6 +606,11 @@ static int is_idle(const OrWorld *w, uint32_t b) {
     return !w->dyn[b] || !w->awake[b];
 }
 
+static uint32_t uf_find(uint32_t *p, uint32_t x) {
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+}
+
 /* solver.rs:477-1008, 1247-1271 */
 static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, const V3 *pre_ang, float dt) {
     const float beta = 0.05f, slop = 0.005f;
@@ -695,16 +700,37 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     free(taken);
     w->stats.warm_matched = matched;
 
+    /* islands of the written bodies (residual mode stops each island on its own residual) */
+    uint32_t *par = malloc(sizeof(uint32_t) * (nb + 1));
+    for (uint32_t b = 0; b <= nb; b++) par[b] = b;
+    for (uint32_t i = 0; i < nct; i++) {
+        int wa = cs[i].inv_a > 0.0f, wb = !cs[i].terrain && cs[i].inv_b > 0.0f;
+        if (wa && wb) {
+            uint32_t x = uf_find(par, cs[i].ai), y = uf_find(par, cs[i].bi);
+            if (x != y) par[x > y ? x : y] = x < y ? x : y;
+        }
+    }
+    uint32_t *isl = malloc(sizeof(uint32_t) * (nct + 1));
+    uint32_t *isl_n = calloc(nb + 1, 4), *isl_it = calloc(nb + 1, 4);
+    uint8_t *isl_done = calloc(nb + 1, 1);
+    float *isl_r = calloc(nb + 1, 4);
+    for (uint32_t i = 0; i < nct; i++) {
+        int wa = cs[i].inv_a > 0.0f, wb = !cs[i].terrain && cs[i].inv_b > 0.0f;
+        isl[i] = wa ? uf_find(par, cs[i].ai) : (wb ? uf_find(par, cs[i].bi) : 0xFFFFFFFFu);
+        if (isl[i] != 0xFFFFFFFFu) isl_n[isl[i]]++;
+    }
+
     /* PGS iterations (solver.rs:788-880), serial in contact order == the level schedule */
     uint32_t iters = w->resid_on ? w->resid_max : w->iv;
     uint32_t used = 0;
     float resid = 0;
     for (uint32_t it = 0; it < iters; it++) {
-        resid = 0;
+        int active = 0;
+        for (uint32_t b = 0; b <= nb; b++) isl_r[b] = 0.0f;
         for (uint32_t i = 0; i < nct; i++) {
             const CC *k = &cs[i];
-            int wa = k->inv_a > 0.0f, wb = !k->terrain && k->inv_b > 0.0f;
-            if (!wa && !wb) continue;
+            if (isl[i] == 0xFFFFFFFFu || isl_done[isl[i]]) continue;
+            active = 1;
             float vrn = vdot(rel_vel(w, k), k->n);
             float jraw = (k->depth_bias - vrn) * k->m_n;
             float old = jn[i];
@@ -725,14 +751,27 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
                 float r1 = k->m_t1 > 0.0f ? fabsf(d.x) / k->m_t1 : 0.0f;
                 float r2 = k->m_t2 > 0.0f ? fabsf(d.y) / k->m_t2 : 0.0f;
                 float r = fmaxf(r0, fmaxf(r1, r2));
-                if (r > resid) resid = r;
+                if (r > isl_r[isl[i]]) isl_r[isl[i]] = r;
             }
         }
+        if (!active) break;
         used++;
-        if (w->resid_on && resid <= w->resid_tol) break;
+        for (uint32_t b = 0; b <= nb; b++) {
+            if (!isl_n[b] || isl_done[b]) continue;
+            isl_it[b]++;
+            if (w->resid_on && isl_r[b] <= w->resid_tol) isl_done[b] = 1;
+        }
+    }
+    uint32_t isum = 0;
+    resid = 0.0f;
+    for (uint32_t b = 0; b <= nb; b++) {
+        isum += isl_it[b] * isl_n[b];
+        if (isl_n[b] && isl_r[b] > resid) resid = isl_r[b];
     }
     w->stats.iters_used = used;
+    w->stats.iters_sum = isum;
     w->stats.residual = resid;
+    free(par); free(isl); free(isl_n); free(isl_it); free(isl_done); free(isl_r);
 
     /* warm-cache store (solver.rs:917-944) */
     Warm *nw = malloc(sizeof(Warm) * (nct + w->nwarm + 1));

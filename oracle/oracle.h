@@ -54,8 +54,9 @@ typedef struct {
     uint32_t terrain_candidates;
     uint32_t levels;       /* contact batches of the velocity solve */
     uint32_t warm_matched;
-    uint32_t iters_used;   /* velocity iterations actually run */
-    float residual;        /* last sweep's residual (m/s), residual mode only */
+    uint32_t iters_used;   /* velocity iterations run by the island that ran the most */
+    float residual;        /* largest final residual over the islands (m/s), residual mode only */
+    uint32_t iters_sum;    /* sum over islands of (iterations run x the island's contacts) */
 } OrStats;
 
 OrWorld *or_world_create(uint32_t max_bodies);
@@ -64,7 +65,10 @@ void or_set_epoch(OrWorld *w, uint64_t epoch);
 void or_set_rng_seed(OrWorld *w, uint64_t seed);
 void or_set_gravity(OrWorld *w, float gx, float gy, float gz);
 void or_set_iterations(OrWorld *w, uint32_t vel, uint32_t pos);
-/* Residual mode (build addition): iterate until sweep residual <= tol, at most max_iters. */
+/* Residual mode (build addition): each island (component of the written bodies) iterates until
+ * its sweep residual max(|dJn|/m_n/(1+|bias|), |dJt1|/m_t1, |dJt2|/m_t2) <= tol (the criterion of
+ * solver.rs:1149), at most max_iters sweeps.  Islands share no written body, so per-island
+ * stopping is independent of their order. */
 void or_set_residual_mode(OrWorld *w, int on, float tol, uint32_t max_iters);
 void or_set_pair_props(OrWorld *w, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
                        float restitution, float stribeck_v);

@@ -114,6 +114,8 @@ struct HostWorld {
     // settings
     float gx = 0.0f, gy = -9.81f, gz = 0.0f;
     uint32_t iv = 20, ip = 20;
+    uint32_t resid_on = 0, resid_max = 100;  // residual mode (build addition)
+    float resid_tol = 1.0e-4f;
     std::vector<PairOverride> ovr;
     uint64_t epoch = 0, rng = 0xC0FFEEull | 1ull, tick = 0;
     // carried warm cache
@@ -167,7 +169,7 @@ struct Scene {
     uint32_t max_extra = 0;
     std::vector<uint32_t> body_off, col_off;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    float graph_dt = -1.0f;
+    TickParams graph_tp{};  // parameters the graphs were captured with (cur aside)
     std::vector<std::unique_ptr<Slot>> slots;
     ~Scene() {
         for (int i = 0; i < 2; i++) if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
@@ -351,6 +353,7 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
         tp.upx = -(w.gx * r); tp.upy = -(w.gy * r); tp.upz = -(w.gz * r);
     } else { tp.upx = 0.0f; tp.upy = 1.0f; tp.upz = 0.0f; }
     tp.iv = iv; tp.ip = w.ip;
+    if (w.resid_on) { tp.resid_on = 1; tp.resid_tol = w.resid_tol; tp.iv = w.resid_max; }
     return tp;
 }
 
@@ -412,6 +415,7 @@ struct ProfileAcc {
         out->ticks += done;
         out->pairs += cnt[CT_SUM_PAIRS]; out->terrain_candidates += cnt[CT_SUM_CANDS]; out->contacts += cnt[CT_SUM_CONTACTS];
         out->island_contacts += cnt[CT_SUM_ISLC]; out->islands += cnt[CT_SUM_ISLANDS]; out->island_contacts_global += cnt[CT_SUM_ISLC_GLB];
+        out->pgs_iters_max += cnt[CT_SUM_ITERS_MAX]; out->pgs_iters_sum += cnt[CT_SUM_ITERS_SUM];
         return true;
     }
 };
@@ -424,10 +428,12 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         TickParams tp = tick_params(*ws[0], dt, iv);
         if (!hip_ok(hipMemsetAsync(S->d.cnt, 0, CT_COUNT * 4, S->st), "memset")) return false;
         if (use_graph) {
-            if (S->graph_dt != dt || !S->graph[0] || !S->graph[1]) {
+            TickParams key = tp;
+            key.cur = 0;
+            if (!S->graph[0] || !S->graph[1] || std::memcmp(&key, &S->graph_tp, sizeof key) != 0) {
                 for (int i = 0; i < 2; i++) if (S->graph[i]) { (void)hipGraphExecDestroy(S->graph[i]); S->graph[i] = nullptr; }
                 if (!capture_graph(*S, tp, 0) || !capture_graph(*S, tp, 1)) return false;
-                S->graph_dt = dt;
+                S->graph_tp = key;
             }
             for (uint32_t k = 0; k < remaining; k++)
                 if (!hip_ok(hipGraphLaunch(S->graph[(S->cur + k) & 1u], S->st), "hipGraphLaunch")) return false;
@@ -882,6 +888,27 @@ RPhysResult rphys_world_step_n(RPhysWorld *world, float dt, uint32_t n_ticks) {
     GUARD_END
 }
 
+// Profiled ticks of a scene (one world or a batch): stage events, counter sums, sweep phases.
+static bool profiled_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_t iv, uint32_t n_ticks, RPhysProfile *out) {
+    ProfileAcc acc;
+    acc.out = out;
+    if (!acc.init()) return false;
+    if (!hip_ok(hipMemsetAsync(S->d.ph, 0, RP_PH_COUNT * sizeof(unsigned long long), S->st), "memset")) return false;
+    bool ok = run_ticks(S, ws, dt, iv, n_ticks, false, &acc);
+    if (ok) {  // sweep-phase clocks (a capacity rebuild during the call restarts them)
+        unsigned long long ph[RP_PH_COUNT];
+        int dev = 0, khz = 0;
+        ok = hip_ok(hipMemcpy(ph, S->d.ph, sizeof ph, hipMemcpyDeviceToHost), "phase clocks") && hip_ok(hipGetDevice(&dev), "hipGetDevice") &&
+             hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
+        if (ok) {
+            static_assert(RP_PH_COUNT == 11, "RPhysProfile.sweep_phase holds 11 entries");
+            for (int k = 0; k < RP_PH_COUNT; k++)
+                out->sweep_phase[k] = k < RP_PH_ROUNDS_BASE ? (khz > 0 ? (double)ph[k] / (double)khz : 0.0) : (double)ph[k];
+        }
+    }
+    return ok;
+}
+
 RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out) {
     GUARD_BEGIN
     if (!world || !out) { set_err("rphys_world_step_profiled: null arg"); return RPHYS_ERR_NULL; }
@@ -892,23 +919,8 @@ RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ti
     if (n_ticks == 0) return RPHYS_OK;
     if (world->w.nb() == 0) { world->w.tick += n_ticks; return RPHYS_OK; }
     if (world->need_build) { if (!pull(world) || !push(world)) return RPHYS_ERR_INTERNAL; }
-    ProfileAcc acc;
-    acc.out = out;
-    if (!acc.init()) return RPHYS_ERR_INTERNAL;
-    if (!hip_ok(hipMemsetAsync(world->scene->d.ph, 0, RP_PH_COUNT * sizeof(unsigned long long), world->scene->st), "memset")) return RPHYS_ERR_INTERNAL;
     std::vector<HostWorld *> ws{&world->w};
-    bool ok = run_ticks(world->scene, ws, dt, world->w.iv, n_ticks, false, &acc);
-    if (ok) {  // sweep-phase clocks (a capacity rebuild during the call restarts them)
-        unsigned long long ph[RP_PH_COUNT];
-        int dev = 0, khz = 0;
-        ok = hip_ok(hipMemcpy(ph, world->scene->d.ph, sizeof ph, hipMemcpyDeviceToHost), "phase clocks") && hip_ok(hipGetDevice(&dev), "hipGetDevice") &&
-             hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
-        if (ok) {
-            static_assert(RP_PH_COUNT == 11, "RPhysProfile.sweep_phase holds 11 entries");
-            for (int k = 0; k < RP_PH_COUNT; k++)
-                out->sweep_phase[k] = k < RP_PH_ROUNDS_BASE ? (khz > 0 ? (double)ph[k] / (double)khz : 0.0) : (double)ph[k];
-        }
-    }
+    bool ok = profiled_ticks(world->scene, ws, dt, world->w.iv, n_ticks, out);
     world->dev_newer = true;
     world->have_contacts = ok;
     return ok ? RPHYS_OK : RPHYS_ERR_INTERNAL;
@@ -1115,6 +1127,16 @@ RPhysResult rphys_world_set_solver_iterations(RPhysWorld *world, uint32_t veloci
     return RPHYS_OK;
     GUARD_END
 }
+RPhysResult rphys_world_set_residual_mode(RPhysWorld *world, uint32_t on, float tol, uint32_t max_iters) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_residual_mode: world is null"); return RPHYS_ERR_NULL; }
+    if (on && (!hfin(tol) || tol < 0.0f || max_iters == 0)) return bad_arg("rphys_world_set_residual_mode", "tol must be finite and >= 0, max_iters > 0");
+    std::lock_guard<std::mutex> lk(world->mu);
+    world->w.resid_on = on ? 1u : 0u;
+    if (on) { world->w.resid_tol = tol; world->w.resid_max = max_iters; }
+    return RPHYS_OK;
+    GUARD_END
+}
 RPhysResult rphys_world_set_gravity(RPhysWorld *world, RPhysVec3 g) {
     GUARD_BEGIN
     if (!world) { set_err("rphys_world_set_gravity: world is null"); return RPHYS_ERR_NULL; }
@@ -1138,6 +1160,7 @@ RPhysResult rphys_world_get_stats(const RPhysWorld *cworld, RPhysStepStats *out)
     if (!hip_ok(hipMemcpy(&wt, S.d.wc_off[S.cur] + S.d.nw, 4, hipMemcpyDeviceToHost), "stats")) return RPHYS_ERR_INTERNAL;
     out->pairs = c[CT_PAIRS]; out->contacts = c[CT_CONTACTS]; out->terrain_candidates = c[CT_CANDS];
     out->island_contacts = c[CT_ISLC]; out->islands = c[CT_ISLANDS]; out->warm_entries = wt;
+    out->pgs_iters = c[CT_ITERS_MAX]; out->pgs_iters_sum = c[CT_ITERS_SUM];
     return RPHYS_OK;
     GUARD_END
 }
@@ -1360,6 +1383,18 @@ static RPhysResult batch_step(RPhysBatch *b, float dt, uint32_t n, bool graph) {
 }
 RPhysResult rphys_batch_step_n(RPhysBatch *b, float dt, uint32_t n) { GUARD_BEGIN return batch_step(b, dt, n, false); GUARD_END }
 RPhysResult rphys_batch_step_graph(RPhysBatch *b, float dt, uint32_t n) { GUARD_BEGIN return batch_step(b, dt, n, true); GUARD_END }
+RPhysResult rphys_batch_step_profiled(RPhysBatch *b, float dt, uint32_t n, RPhysProfile *out) {
+    GUARD_BEGIN
+    if (!b || !out) { set_err("rphys_batch_step_profiled: null arg"); return RPHYS_ERR_NULL; }
+    std::memset(out, 0, sizeof(*out));
+    out->n_stages = RP_NMARKS - 1;
+    if (!hfin(dt) || dt < 0.0f) { set_err("rphys_batch_step_profiled: dt must be finite and >= 0"); return RPHYS_ERR_BAD_ARG; }
+    if (n == 0) return RPHYS_OK;
+    std::vector<HostWorld *> ws;
+    for (auto &w : b->worlds) ws.push_back(&w);
+    return profiled_ticks(b->scene, ws, dt, b->worlds[0].iv, n, out) ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+    GUARD_END
+}
 
 RPhysResult rphys_batch_get_hashes(const RPhysBatch *b, uint8_t *out) {
     GUARD_BEGIN

@@ -25,8 +25,9 @@ enum {
     CT_PAIRS = 0, CT_CANDS, CT_CONTACTS, CT_ISLC, CT_ISLANDS, CT_ABORT, CT_NEED_PAIRS, CT_NEED_CANDS,
     CT_NEED_CONTACTS, CT_NEED_WARM, CT_CHAIN_N, CT_WARM_TOTAL, CT_NEW_TOTAL, CT_KEEP_TOTAL, CT_SCRATCH, CT_DONE,
     CT_ISLC_GLB,  // contacts of the islands solved by the 1024-thread LDS-flag tier (this tick)
+    CT_ITERS_MAX, CT_ITERS_SUM,  // PGS sweeps: most run by an island; sum of sweeps x island contacts
     // sums over the completed ticks since the counters were last cleared
-    CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB,
+    CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_COUNT
 };
 
@@ -155,6 +156,8 @@ struct TickParams {
     float dt;
     float gx, gy, gz;      // q6-quantised gravity acceleration
     float upx, upy, upz;   // -normalize(gravity) (or +Y)
-    uint32_t iv, ip;       // velocity / position iterations
+    uint32_t iv, ip;       // velocity iterations (the most, in residual mode) / position iterations
     uint32_t cur;          // warm buffer holding last tick's cache
+    uint32_t resid_on;     // residual mode: an island stops once its sweep residual <= resid_tol
+    float resid_tol;
 };

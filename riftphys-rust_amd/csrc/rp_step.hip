@@ -1049,7 +1049,12 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
         __syncthreads();
         struct PgsIn { Row x; float jn; float2 jt; };
+        // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
+        __shared__ uint32_t resid_bits;
+        uint32_t used = 0;
         for (uint32_t it = 0; it < tp.iv; it++) {
+            if (threadIdx.x == 0) resid_bits = 0;
+            __syncthreads();
             run_sweep(I, sb,
                 [&](uint32_t c, uint32_t slot) {
                     PgsIn in;
@@ -1078,6 +1083,12 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
                     d.jt[c] = make_float2(nt.x, nt.y);
                     V3 dd = nt - ot;
                     apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
+                    if (tp.resid_on) {  // the oracle's row residual, same operations
+                        float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
+                        float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
+                        float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
+                        atomicMax(&resid_bits, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))));
+                    }
                     // store what changed (a body the row does not write keeps its bits)
                     if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
                     if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
@@ -1086,7 +1097,10 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
                         if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
                     }
                 });
+            used++;
+            if (tp.resid_on && __uint_as_float(resid_bits) <= tp.resid_tol) break;
         }
+        if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n); }
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);
         for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
         __syncthreads();
@@ -1531,6 +1545,7 @@ __global__ void k_tick_done(Dev d) {
     c[CT_DONE] += 1;
     c[CT_SUM_PAIRS] += c[CT_PAIRS]; c[CT_SUM_CANDS] += c[CT_CANDS]; c[CT_SUM_CONTACTS] += c[CT_CONTACTS];
     c[CT_SUM_ISLC] += c[CT_ISLC]; c[CT_SUM_ISLANDS] += c[CT_ISLANDS]; c[CT_SUM_ISLC_GLB] += c[CT_ISLC_GLB];
+    c[CT_SUM_ITERS_MAX] += c[CT_ITERS_MAX]; c[CT_SUM_ITERS_SUM] += c[CT_ITERS_SUM];
 }
 __global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) cnt[idx] = v; }
 __global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n) { GRID_STRIDE(i, n) dst[i] = src[i]; }
@@ -1656,6 +1671,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_MAX, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_SUM, 0u);
     hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
     mark(RP_ST_ORDERS);
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);

@@ -76,7 +76,7 @@ class ContactEvent(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("terrain_candidates", C.c_uint32), ("island_contacts", C.c_uint32),
-                ("islands", C.c_uint32), ("warm_entries", C.c_uint32)]
+                ("islands", C.c_uint32), ("warm_entries", C.c_uint32), ("pgs_iters", C.c_uint32), ("pgs_iters_sum", C.c_uint32)]
 
 
 class ContactEx(C.Structure):
@@ -87,7 +87,8 @@ class ContactEx(C.Structure):
 class Profile(C.Structure):
     _fields_ = [("ticks", C.c_uint32), ("n_stages", C.c_uint32), ("stage_ms", C.c_double * 24), ("pairs", C.c_double),
                 ("terrain_candidates", C.c_double), ("contacts", C.c_double), ("island_contacts", C.c_double), ("islands", C.c_double),
-                ("island_contacts_global", C.c_double), ("sweep_phase", C.c_double * 11)]
+                ("island_contacts_global", C.c_double), ("pgs_iters_max", C.c_double), ("pgs_iters_sum", C.c_double),
+                ("sweep_phase", C.c_double * 11)]
 
 
 # (name, restype, argtypes) of every export in include/riftphys_c.h
@@ -132,6 +133,7 @@ EXPORTS = [
     ("rphys_world_set_pair_props", U32, [P, U32, U32, F32, F32, F32, F32]),
     ("rphys_world_set_solver_iterations", U32, [P, U32, U32]),
     ("rphys_world_set_gravity", U32, [P, Vec3]),
+    ("rphys_world_set_residual_mode", U32, [P, U32, F32, U32]),
     ("rphys_body_set_sleep_allowed", U32, [P, U32, U32]),
     ("rphys_world_step_n", U32, [P, F32, U32]),
     ("rphys_world_get_stats", U32, [P, C.POINTER(StepStats)]),
@@ -150,6 +152,7 @@ EXPORTS = [
     ("rphys_batch_restore", U32, [P, U32]),
     ("rphys_batch_total_bodies", U32, [P, PU32]),
     ("rphys_batch_step_graph", U32, [P, F32, U32]),
+    ("rphys_batch_step_profiled", U32, [P, F32, U32, C.POINTER(Profile)]),
 ]
 
 _lib = None
@@ -191,6 +194,18 @@ def vel(l, a):
     return Velocity(Vec3(*[float(x) for x in l]), Vec3(*[float(x) for x in a]))
 
 
+def _profile_dicts(pr):
+    """(stage name -> ms summed, counters) of an RPhysProfile."""
+    stages = {lib().rphys_profile_stage_name(k).decode(): pr.stage_ms[k] for k in range(pr.n_stages)}
+    counters = dict(ticks=pr.ticks, pairs=pr.pairs, terrain_candidates=pr.terrain_candidates, contacts=pr.contacts,
+                    island_contacts=pr.island_contacts, islands=pr.islands, island_contacts_global=pr.island_contacts_global,
+                    pgs_iters_max=pr.pgs_iters_max, pgs_iters_sum=pr.pgs_iters_sum)
+    names = ("vel_order", "warm", "pgs", "vel_shock_order", "vel_shock", "ballistic", "projection", "proj_shock_order",
+             "proj_shock", "levels_list", "levels_shock")
+    counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}
+    return stages, counters
+
+
 def copy_probe(nbytes, reps=20):
     """GB/s (read + write) of the library's plain copy kernel over nbytes (rphys_debug_copy_probe)."""
     g = C.c_double(0.0)
@@ -230,6 +245,9 @@ class World:
             check(L.rphys_world_set_pair_props(self.h, a, b, ms, mk, e, vs))
         vi, pi = s.get("iters", (20, 20))
         check(L.rphys_world_set_solver_iterations(self.h, vi, pi))
+        if s.get("residual"):  # (tol, max_iters): each island iterates to the residual
+            tol, mx = s["residual"]
+            check(L.rphys_world_set_residual_mode(self.h, 1, tol, mx))
         for i in range(len(s["shape"])):
             self.add_body(int(s["shape"][i]), s["dims"][i], s["pose"][i], s["vel"][i], int(s["dynamic"][i]), float(s["density"][i]))
         hf = s.get("hf")
@@ -265,13 +283,7 @@ class World:
         """Runs n ticks with per-stage HIP event timing; returns (stage name -> ms summed, counters)."""
         pr = Profile()
         check(lib().rphys_world_step_profiled(self.h, self.dt if dt is None else dt, n, C.byref(pr)))
-        stages = {lib().rphys_profile_stage_name(k).decode(): pr.stage_ms[k] for k in range(pr.n_stages)}
-        counters = dict(ticks=pr.ticks, pairs=pr.pairs, terrain_candidates=pr.terrain_candidates, contacts=pr.contacts,
-                        island_contacts=pr.island_contacts, islands=pr.islands, island_contacts_global=pr.island_contacts_global)
-        names = ("vel_order", "warm", "pgs", "vel_shock_order", "vel_shock", "ballistic", "projection", "proj_shock_order",
-                 "proj_shock", "levels_list", "levels_shock")
-        counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}
-        return stages, counters
+        return _profile_dicts(pr)
 
     def hash(self):
         h = Hash32()
@@ -380,6 +392,12 @@ class Batch:
             check(lib().rphys_batch_step_graph(self.h, dt, n))
         else:
             check(lib().rphys_batch_step_n(self.h, dt, n))
+
+    def step_profiled(self, n, dt=1.0 / 60.0):
+        """n ticks with per-stage HIP event timing (rphys_batch_step_profiled)."""
+        pr = Profile()
+        check(lib().rphys_batch_step_profiled(self.h, dt, n, C.byref(pr)))
+        return _profile_dicts(pr)
 
     def hashes(self):
         out = np.zeros(32 * self.n, np.uint8)

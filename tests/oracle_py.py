@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 class OrStats(C.Structure):
     _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("islands", C.c_uint32),
                 ("terrain_candidates", C.c_uint32), ("levels", C.c_uint32), ("warm_matched", C.c_uint32),
-                ("iters_used", C.c_uint32), ("residual", C.c_float)]
+                ("iters_used", C.c_uint32), ("residual", C.c_float), ("iters_sum", C.c_uint32)]
 
 
 class OrContact(C.Structure):
@@ -108,6 +108,9 @@ class OracleWorld:
             L.or_set_pair_props(self.h, a, b, ms, mk, e, vs)
         vi, pi = s.get("iters", (20, 20))
         L.or_set_iterations(self.h, vi, pi)
+        if s.get("residual"):
+            tol, mx = s["residual"]
+            L.or_set_residual_mode(self.h, 1, tol, mx)
         for i in range(len(s["shape"])):
             d = np.ascontiguousarray(s["dims"][i], np.float32)
             p = np.ascontiguousarray(s["pose"][i], np.float32)

@@ -149,3 +149,44 @@ def test_batch_rollback_window():
     b.restore(0)
     b.step_n(8)
     assert b.hashes() == h1
+
+
+def test_batch_profiled_matches_step_n():
+    ss = [scenes.small_mixed(n=n, seed=80 + i) for i, n in enumerate([45, 75, 30])]
+    b1 = rphys.Batch([rphys.World(s) for s in ss])
+    b2 = rphys.Batch([rphys.World(s) for s in ss])
+    b1.step_n(14)
+    b2.step_n(4)
+    stages, ctr = b2.step_profiled(10)
+    assert ctr["ticks"] == 10 and set(stages) >= {"broadphase", "vel_solve_small", "projection_shock"}
+    assert b1.hashes() == b2.hashes()
+
+
+@pytest.mark.parametrize("make", [lambda: scenes.small_mixed(n=96, seed=7), lambda: scenes.box_lattice()], ids=["mixed96", "boxes512"])
+def test_residual_mode_bit_exact_with_equal_iteration_counts(make):
+    """Each island iterates until its sweep residual <= 1e-4 (at most 100 sweeps); the device and the
+    oracle must stop every island at the same sweep and agree bit for bit."""
+    s = make()
+    s["residual"] = (1.0e-4, 100)
+    o, g = pair_worlds(s)
+    for t in range(25):
+        o.step()
+        g.step()
+        assert_state_close(o, g)
+        assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+        so, sg = o.stats(), g.stats()
+        assert (so.iters_used, so.iters_sum) == (sg.pgs_iters, sg.pgs_iters_sum), (t + 1, so.iters_used, sg.pgs_iters)
+
+
+def test_graph_recaptured_when_iterations_change():
+    s = scenes.small_mixed(n=64, seed=13)
+    a, b = rphys.World(s), rphys.World(s)
+    a.step_n(6)
+    for _ in range(6):
+        b.step()
+    for w in (a, b):
+        rphys.check(rphys.lib().rphys_world_set_solver_iterations(w.h, 4, 7))
+    a.step_n(6)
+    for _ in range(6):
+        b.step()
+    assert a.hash() == b.hash()

@@ -245,6 +245,42 @@ RPhysResult rphys_batch_total_bodies(const RPhysBatch *batch, uint32_t *out);
 RPhysResult rphys_batch_step_profiled(RPhysBatch *batch, float dt, uint32_t n_ticks, RPhysProfile *out); /* as rphys_world_step_profiled */
 RPhysResult rphys_batch_step_graph(RPhysBatch *batch, float dt, uint32_t n_ticks);
 
+/* ---- Learned PGS warm start (build addition; the reference has no counterpart) ----
+ * A 12 -> 64 -> 64 -> 3 ReLU MLP (BF16 weights and activations, FP32 accumulation on MFMA) maps
+ * per-contact features (normal/tangential relative velocity, penetration, inverse masses, four
+ * angular effective-mass terms, the previous frame's matched normal impulse, mu_s) to the contact's
+ * converged (normal, tangent 1, tangent 2) impulses.  Parameter vector (f32, rphys_mlp_param_count()
+ * entries): W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] x_mean[12] x_std[12] y_scale[3];
+ * inputs are normalised (x - x_mean) / x_std, outputs scaled by y_scale.
+ * Warm-start mode: 0 the previous frame's impulses (the reference, default), 1 zero, 2 the
+ * model's prediction for every solved contact (then clamped to the friction cone as a cached
+ * impulse would be).  The solver's iterations are unchanged: with residual mode on, only the
+ * number of sweeps to the residual depends on the warm start. */
+uint32_t rphys_mlp_param_count(void);
+uint32_t rphys_mlp_gradient_rows(void);  /* rows of one training step (rphys_mlp_gradient) */
+RPhysResult rphys_world_set_warm_start(RPhysWorld *world, uint32_t mode, const float *params);
+RPhysResult rphys_batch_set_warm_start(RPhysBatch *batch, uint32_t mode, const float *params);
+/* Contact log: while enabled, every tick appends its solved contacts (island-list order) as
+ * features [12] and converged impulses [3] (normal before the shock pass, friction), until full. */
+RPhysResult rphys_batch_set_contact_log(RPhysBatch *batch, uint32_t capacity_rows);  /* 0 disables */
+RPhysResult rphys_batch_contact_log_rows(RPhysBatch *batch, uint32_t *rows, uint32_t *dropped);
+RPhysResult rphys_batch_read_contact_log(RPhysBatch *batch, uint32_t first, uint32_t count, float *feat12, float *targ3);
+typedef struct RPhysMlpTrainConfig {
+    uint32_t size_bytes;  /* sizeof(RPhysMlpTrainConfig) */
+    uint32_t epochs;
+    float lr;             /* Adam (beta1 0.9, beta2 0.999, eps 1e-8) */
+    uint32_t reserved;
+    uint64_t seed;        /* weight initialisation and the per-epoch shuffles */
+} RPhysMlpTrainConfig;
+/* Trains on the batch's contact log on the device (mean squared error in units of y_scale, steps
+ * of rphys_mlp_gradient_rows() contacts); writes the parameters and each epoch's mean loss. */
+RPhysResult rphys_batch_train_warm_start(RPhysBatch *batch, const RPhysMlpTrainConfig *cfg, float *params_out, float *epoch_loss);
+/* Device inference on host features (n rows of 12) -> n rows of 3 (checking hook). */
+RPhysResult rphys_mlp_forward(const float *params, const float *feat12, uint32_t n, float *out3);
+/* One training step's loss and gradient (the first 5,187 parameters: weights and biases) over
+ * rphys_mlp_gradient_rows() given rows, without updating (checking hook). */
+RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const float *targ3, float *loss, float *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,10 @@ struct HashDev {
 extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st, hipEvent_t *ev);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
+extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st);
+extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
+                                    uint32_t cap, float *out, hipStream_t st);
+extern "C" void rp_launch_mlp_train(const TrainArgs &a, hipStream_t st);
 extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uint32_t max_extra_slots, hipStream_t st);
 
 // ------------------------------------------------------------------ errors (lib.rs:56-82)
@@ -116,6 +120,8 @@ struct HostWorld {
     uint32_t iv = 20, ip = 20;
     uint32_t resid_on = 0, resid_max = 100;  // residual mode (build addition)
     float resid_tol = 1.0e-4f;
+    uint32_t ws_mode = 0;                    // warm start: 0 previous frame, 1 zero, 2 learned
+    std::vector<float> mlp;                  // the learned model (RP_MLP_NPARAM floats) for mode 2
     std::vector<PairOverride> ovr;
     uint64_t epoch = 0, rng = 0xC0FFEEull | 1ull, tick = 0;
     // carried warm cache
@@ -187,6 +193,14 @@ struct Caps { uint32_t pairs, cands, contacts, warm; };
 
 static uint64_t next_pow2(uint64_t v) { uint64_t m = 1; while (m < v) m <<= 1; return m; }
 
+// Copies a learned warm-start model into the scene and prepares its bf16 tiles (stream-ordered).
+static bool upload_model(Scene &S, const float *params) {
+    const MlpWeights &m = S.d.mlp;
+    if (!hip_ok(hipMemcpyAsync((void *)m.p, params, RP_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, S.st), "model upload")) return false;
+    rp_launch_mlp_prep(m.p, m.w1p, m.w2p, m.w2t, m.w3p, m.w3t, S.st);
+    return hip_ok(hipGetLastError(), "model prep");
+}
+
 // Builds a device scene for the given worlds (host mirrors) with the given capacities and
 // uploads everything.  Returns nullptr (with the error set) on failure.
 static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_t cur) {
@@ -235,6 +249,13 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.ch_sa, C) && B.alloc(d.ch_sb, C) && B.alloc(d.sh_sa, C) && B.alloc(d.sh_sb, C) && B.alloc(d.ex_base, C) && B.alloc(d.ex_shock, C);
     ok &= B.alloc(d.bs_pa, C) && B.alloc(d.bs_pb, C) && B.alloc(d.bs_c, C) && B.alloc(d.ss_pa, C) && B.alloc(d.ss_pb, C) && B.alloc(d.ss_c, C);
     ok &= B.alloc(d.slot_of, C) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
+    ok &= B.alloc(d.feat, 12 * C) && B.alloc(d.pred, C);
+    {
+        float *mp = nullptr;
+        ok &= B.alloc(mp, RP_MLP_NPARAM) && B.alloc(d.mlp.w1p, RP_MLP_W1P) && B.alloc(d.mlp.w2p, RP_MLP_W2P) && B.alloc(d.mlp.w2t, RP_MLP_W2P) &&
+              B.alloc(d.mlp.w3p, RP_MLP_W3P) && B.alloc(d.mlp.w3t, RP_MLP_W3T);
+        d.mlp.p = mp;
+    }
     ok &= B.alloc(d.sk0, 2 * C) && B.alloc(d.sv0, 2 * C) && B.alloc(d.sk1, 2 * C) && B.alloc(d.sv1, 2 * C) && B.alloc(d.sh_hi, C);
     d.rs_blocks = (uint32_t)((2 * C + 1023) / 1024);
     ok &= B.alloc(d.rs_hist, 256 * (size_t)d.rs_blocks + 1);
@@ -315,6 +336,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
          up(d.wk_la[cur], wla.data(), wla.size() * 16) && up(d.wk_jn[cur], wjn.data(), wjn.size() * 4) && up(d.wk_jt[cur], wjt.data(), wjt.size() * 16) &&
          up(d.wc_off[cur], woff.data(), woff.size() * 4) && up(h.w_epoch, ep.data(), ep.size() * 8) && up(h.w_rng, rn.data(), rn.size() * 8);
     if (ok) ok = hip_ok(hipMemsetAsync(d.cnt, 0, CT_COUNT * 4, S->st), "memset");
+    if (ok && ws[0]->mlp.size() == RP_MLP_NPARAM) ok = upload_model(*S, ws[0]->mlp.data());
     if (ok) ok = hip_ok(hipStreamSynchronize(S->st), "upload sync");
     if (!ok) return nullptr;
     S->cur = cur;
@@ -354,6 +376,7 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
     } else { tp.upx = 0.0f; tp.upy = 1.0f; tp.upz = 0.0f; }
     tp.iv = iv; tp.ip = w.ip;
     if (w.resid_on) { tp.resid_on = 1; tp.resid_tol = w.resid_tol; tp.iv = w.resid_max; }
+    tp.ws_mode = w.ws_mode;
     return tp;
 }
 
@@ -426,6 +449,7 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
     if (prof) use_graph = false;
     while (remaining > 0) {
         TickParams tp = tick_params(*ws[0], dt, iv);
+        tp.log_on = S->d.log_cap > 0 ? 1u : 0u;
         if (!hip_ok(hipMemsetAsync(S->d.cnt, 0, CT_COUNT * 4, S->st), "memset")) return false;
         if (use_graph) {
             TickParams key = tp;
@@ -473,6 +497,7 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         Scene *N = build_scene(ws, caps, cur);
         if (!N) return false;
         N->island_grid = S->island_grid;
+        N->d.log_feat = S->d.log_feat; N->d.log_targ = S->d.log_targ; N->d.log_n = S->d.log_n; N->d.log_cap = S->d.log_cap;
         delete S;
         S = N;
     }
@@ -1347,6 +1372,7 @@ void rphys_world_snapshot_free(RPhysWorldSnapshot *s) { delete s; }
 struct RPhysBatch {
     std::vector<HostWorld> worlds;
     Scene *scene = nullptr;
+    DeviceBuffers logbuf;  // contact log (outlives scene rebuilds)
     ~RPhysBatch() { delete scene; }
 };
 
@@ -1480,6 +1506,235 @@ RPhysResult rphys_batch_restore(RPhysBatch *b, uint32_t slot) {
     S.cur = sl.cur;
     for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];
     return RPHYS_OK;
+    GUARD_END
+}
+
+// ------------------------------------------------------------------ learned warm start
+uint32_t rphys_mlp_param_count(void) { return RP_MLP_NPARAM; }
+uint32_t rphys_mlp_gradient_rows(void) { return RP_MLP_STEP_CONTACTS; }
+
+static RPhysResult check_mode(const char *ctx, uint32_t mode, const float *params) {
+    if (mode > 2) return bad_arg(ctx, "mode must be 0 (previous frame), 1 (zero) or 2 (learned)");
+    if (mode == 2 && !params) { set_err(std::string(ctx) + ": the learned mode needs parameters"); return RPHYS_ERR_NULL; }
+    if (mode == 2)
+        for (uint32_t i = 0; i < RP_MLP_NPARAM; i++)
+            if (!hfin(params[i])) return bad_arg(ctx, "parameters must be finite");
+    if (mode == 2)
+        for (uint32_t k = 0; k < 12; k++)
+            if (!(params[RP_MLP_XSTD + k] > 0.0f)) return bad_arg(ctx, "x_std must be positive");
+    return RPHYS_OK;
+}
+
+RPhysResult rphys_world_set_warm_start(RPhysWorld *world, uint32_t mode, const float *params) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_warm_start: world is null"); return RPHYS_ERR_NULL; }
+    RPhysResult r = check_mode("rphys_world_set_warm_start", mode, params);
+    if (r != RPHYS_OK) return r;
+    std::lock_guard<std::mutex> lk(world->mu);
+    world->w.ws_mode = mode;
+    if (mode == 2) {
+        world->w.mlp.assign(params, params + RP_MLP_NPARAM);
+        if (world->scene && !upload_model(*world->scene, params)) return RPHYS_ERR_INTERNAL;
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_batch_set_warm_start(RPhysBatch *b, uint32_t mode, const float *params) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_set_warm_start: batch is null"); return RPHYS_ERR_NULL; }
+    RPhysResult r = check_mode("rphys_batch_set_warm_start", mode, params);
+    if (r != RPHYS_OK) return r;
+    for (auto &w : b->worlds) {
+        w.ws_mode = mode;
+        if (mode == 2) w.mlp.assign(params, params + RP_MLP_NPARAM);
+    }
+    if (mode == 2 && !upload_model(*b->scene, params)) return RPHYS_ERR_INTERNAL;
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_batch_set_contact_log(RPhysBatch *b, uint32_t capacity_rows) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_set_contact_log: batch is null"); return RPHYS_ERR_NULL; }
+    if (!hip_ok(hipStreamSynchronize(b->scene->st), "sync")) return RPHYS_ERR_INTERNAL;
+    Dev &d = b->scene->d;
+    d.log_feat = nullptr; d.log_targ = nullptr; d.log_n = nullptr; d.log_cap = 0;
+    for (void *q : b->logbuf.ptrs) (void)hipFree(q);
+    b->logbuf.ptrs.clear();
+    if (capacity_rows == 0) return RPHYS_OK;
+    float *f = nullptr, *t = nullptr;
+    uint32_t *n = nullptr;
+    if (!b->logbuf.alloc(f, 12 * (size_t)capacity_rows) || !b->logbuf.alloc(t, 3 * (size_t)capacity_rows) || !b->logbuf.alloc(n, 2)) {
+        set_err("rphys_batch_set_contact_log: device allocation failed");
+        return RPHYS_ERR_CAPACITY;
+    }
+    if (!hip_ok(hipMemset(n, 0, 8), "memset")) return RPHYS_ERR_INTERNAL;
+    d.log_feat = f; d.log_targ = t; d.log_n = n; d.log_cap = capacity_rows;
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_batch_contact_log_rows(RPhysBatch *b, uint32_t *rows, uint32_t *dropped) {
+    GUARD_BEGIN
+    if (!b || !rows) { set_err("rphys_batch_contact_log_rows: null arg"); return RPHYS_ERR_NULL; }
+    *rows = 0;
+    if (dropped) *dropped = 0;
+    if (!b->scene->d.log_n) return RPHYS_OK;
+    uint32_t n[2];
+    if (!hip_ok(hipStreamSynchronize(b->scene->st), "sync") || !hip_ok(hipMemcpy(n, b->scene->d.log_n, 8, hipMemcpyDeviceToHost), "log rows"))
+        return RPHYS_ERR_INTERNAL;
+    *rows = n[0];
+    if (dropped) *dropped = n[1];
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_batch_read_contact_log(RPhysBatch *b, uint32_t first, uint32_t count, float *feat12, float *targ3) {
+    GUARD_BEGIN
+    if (!b || !feat12 || !targ3) { set_err("rphys_batch_read_contact_log: null arg"); return RPHYS_ERR_NULL; }
+    uint32_t rows = 0;
+    RPhysResult r = rphys_batch_contact_log_rows(b, &rows, nullptr);
+    if (r != RPHYS_OK) return r;
+    if ((uint64_t)first + count > rows) return bad_arg("rphys_batch_read_contact_log", "range beyond the logged rows");
+    const Dev &d = b->scene->d;
+    if (count && (!hip_ok(hipMemcpy(feat12, d.log_feat + 12 * (size_t)first, 48 * (size_t)count, hipMemcpyDeviceToHost), "log read") ||
+                  !hip_ok(hipMemcpy(targ3, d.log_targ + 3 * (size_t)first, 12 * (size_t)count, hipMemcpyDeviceToHost), "log read")))
+        return RPHYS_ERR_INTERNAL;
+    return RPHYS_OK;
+    GUARD_END
+}
+
+// Glorot-uniform weights from XorShift64(seed), zero biases; normalisation from the data.
+static void init_params(uint64_t seed, const double *xm, const double *xs, const double *ys, float *p) {
+    uint64_t st = seed ? seed : 0x9E3779B97F4A7C15ull;
+    auto next = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return (double)(st >> 11) * (1.0 / 9007199254740992.0); };
+    auto fill = [&](uint32_t off, uint32_t n, uint32_t fan_in, uint32_t fan_out) {
+        double a = std::sqrt(6.0 / (double)(fan_in + fan_out));
+        for (uint32_t i = 0; i < n; i++) p[off + i] = (float)((2.0 * next() - 1.0) * a);
+    };
+    std::memset(p, 0, RP_MLP_NPARAM * sizeof(float));
+    fill(RP_MLP_W1, 64 * 12, 12, 64);
+    fill(RP_MLP_W2, 64 * 64, 64, 64);
+    fill(RP_MLP_W3, 3 * 64, 64, 3);
+    for (int k = 0; k < 12; k++) { p[RP_MLP_XMEAN + k] = (float)xm[k]; p[RP_MLP_XSTD + k] = (float)xs[k]; }
+    for (int k = 0; k < 3; k++) p[RP_MLP_YSCALE + k] = (float)ys[k];
+}
+
+static uint64_t gcd64(uint64_t a, uint64_t b) { while (b) { uint64_t t = a % b; a = b; b = t; } return a; }
+
+// Runs the trainer on device data (feat [n][12], targ [n][3]); params in/out on the host.
+static bool train_on_device(const float *feat, const float *targ, uint64_t n, uint32_t epochs, uint32_t steps_per_epoch, float lr, uint64_t seed,
+                            float *params, float *epoch_loss, bool grad_only, float *grad_out) {
+    DeviceBuffers B;
+    float *p = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr, *grad = nullptr;
+    uint64_t *perm = nullptr;
+    if (!B.alloc(p, RP_MLP_NPARAM) || !B.alloc(m, RP_MLP_NW) || !B.alloc(v, RP_MLP_NW) || !B.alloc(loss, epochs) || !B.alloc(grad, RP_MLP_NW) ||
+        !B.alloc(perm, 2 * (size_t)epochs)) {
+        set_err("training: device allocation failed");
+        return false;
+    }
+    std::vector<uint64_t> pr(2 * (size_t)epochs);
+    uint64_t st = seed ^ 0xD1B54A32D192ED03ull;
+    auto nxt = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; };
+    for (uint32_t e = 0; e < epochs; e++) {
+        uint64_t a = 1, b = 0;
+        if (!grad_only && n > 1) {
+            do { a = 1 + nxt() % (n - 1); } while (gcd64(a, n) != 1);
+            b = nxt() % n;
+        }
+        pr[2 * e] = a; pr[2 * e + 1] = b;
+    }
+    bool ok = hip_ok(hipMemcpy(p, params, RP_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice), "params") &&
+              hip_ok(hipMemset(m, 0, RP_MLP_NW * sizeof(float)), "memset") && hip_ok(hipMemset(v, 0, RP_MLP_NW * sizeof(float)), "memset") &&
+              hip_ok(hipMemcpy(perm, pr.data(), pr.size() * 8, hipMemcpyHostToDevice), "perm");
+    if (!ok) return false;
+    TrainArgs A{};
+    A.p = p; A.m = m; A.v = v; A.feat = feat; A.targ = targ; A.n = n; A.perm = perm;
+    A.epochs = epochs; A.steps_per_epoch = steps_per_epoch; A.lr = lr; A.beta1 = 0.9f; A.beta2 = 0.999f; A.eps = 1.0e-8f;
+    A.loss = loss; A.grad = grad; A.grad_only = grad_only ? 1u : 0u;
+    rp_launch_mlp_train(A, nullptr);
+    ok = hip_ok(hipGetLastError(), "trainer launch") && hip_ok(hipDeviceSynchronize(), "training") &&
+         hip_ok(hipMemcpy(params, p, RP_MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost), "params") &&
+         hip_ok(hipMemcpy(epoch_loss, loss, epochs * sizeof(float), hipMemcpyDeviceToHost), "loss");
+    if (ok && grad_only) ok = hip_ok(hipMemcpy(grad_out, grad, RP_MLP_NW * sizeof(float), hipMemcpyDeviceToHost), "grad");
+    return ok;
+}
+
+RPhysResult rphys_batch_train_warm_start(RPhysBatch *b, const RPhysMlpTrainConfig *cfg, float *params_out, float *epoch_loss) {
+    GUARD_BEGIN
+    if (!b || !cfg || !params_out || !epoch_loss) { set_err("rphys_batch_train_warm_start: null arg"); return RPHYS_ERR_NULL; }
+    if (cfg->size_bytes < sizeof(RPhysMlpTrainConfig)) return bad_arg("rphys_batch_train_warm_start", "size_bytes too small");
+    if (cfg->epochs == 0 || !hfin(cfg->lr) || cfg->lr <= 0.0f) return bad_arg("rphys_batch_train_warm_start", "epochs > 0 and a positive lr required");
+    uint32_t rows = 0;
+    RPhysResult r = rphys_batch_contact_log_rows(b, &rows, nullptr);
+    if (r != RPHYS_OK) return r;
+    if (rows < RP_MLP_STEP_CONTACTS) return bad_arg("rphys_batch_train_warm_start", "the contact log holds fewer rows than one training step");
+    // normalisation from the logged data (f64 sums)
+    const Dev &d = b->scene->d;
+    std::vector<float> f(12 * (size_t)rows), t(3 * (size_t)rows);
+    if (!hip_ok(hipMemcpy(f.data(), d.log_feat, f.size() * 4, hipMemcpyDeviceToHost), "log") ||
+        !hip_ok(hipMemcpy(t.data(), d.log_targ, t.size() * 4, hipMemcpyDeviceToHost), "log"))
+        return RPHYS_ERR_INTERNAL;
+    double xm[12] = {0}, xs[12] = {0}, ys[3] = {0};
+    for (size_t i = 0; i < rows; i++)
+        for (int k = 0; k < 12; k++) xm[k] += f[12 * i + k];
+    for (int k = 0; k < 12; k++) xm[k] /= (double)rows;
+    for (size_t i = 0; i < rows; i++)
+        for (int k = 0; k < 12; k++) { double e = f[12 * i + k] - xm[k]; xs[k] += e * e; }
+    for (size_t i = 0; i < rows; i++)
+        for (int k = 0; k < 3; k++) ys[k] += (double)t[3 * i + k] * t[3 * i + k];
+    for (int k = 0; k < 12; k++) { xs[k] = std::sqrt(xs[k] / (double)rows); if (!(xs[k] > 1.0e-6)) xs[k] = 1.0; }
+    for (int k = 0; k < 3; k++) { ys[k] = std::sqrt(ys[k] / (double)rows); if (!(ys[k] > 1.0e-6)) ys[k] = 1.0; }
+    init_params(cfg->seed, xm, xs, ys, params_out);
+    uint32_t spe = rows / RP_MLP_STEP_CONTACTS;
+    return train_on_device(d.log_feat, d.log_targ, rows, cfg->epochs, spe, cfg->lr, cfg->seed, params_out, epoch_loss, false, nullptr)
+               ? RPHYS_OK : RPHYS_ERR_INTERNAL;
+    GUARD_END
+}
+
+RPhysResult rphys_mlp_forward(const float *params, const float *feat12, uint32_t n, float *out3) {
+    GUARD_BEGIN
+    if (!params || (n && (!feat12 || !out3))) { set_err("rphys_mlp_forward: null arg"); return RPHYS_ERR_NULL; }
+    RPhysResult r = check_mode("rphys_mlp_forward", 2, params);
+    if (r != RPHYS_OK || n == 0) return r;
+    DeviceBuffers B;
+    float *p = nullptr, *f = nullptr, *o = nullptr;
+    __bf16 *w1p, *w2p, *w2t, *w3p, *w3t;
+    if (!B.alloc(p, RP_MLP_NPARAM) || !B.alloc(f, 12 * (size_t)n) || !B.alloc(o, 4 * (size_t)n) || !B.alloc(w1p, RP_MLP_W1P) || !B.alloc(w2p, RP_MLP_W2P) ||
+        !B.alloc(w2t, RP_MLP_W2P) || !B.alloc(w3p, RP_MLP_W3P) || !B.alloc(w3t, RP_MLP_W3T)) {
+        set_err("rphys_mlp_forward: device allocation failed");
+        return RPHYS_ERR_CAPACITY;
+    }
+    std::vector<float> host(4 * (size_t)n);
+    bool ok = hip_ok(hipMemcpy(p, params, RP_MLP_NPARAM * 4, hipMemcpyHostToDevice), "upload") &&
+              hip_ok(hipMemcpy(f, feat12, 48 * (size_t)n, hipMemcpyHostToDevice), "upload");
+    if (ok) {
+        rp_launch_mlp_prep(p, w1p, w2p, w2t, w3p, w3t, nullptr);
+        rp_launch_mlp_infer(p, w1p, w2p, w3p, f, nullptr, n, o, nullptr);
+        ok = hip_ok(hipGetLastError(), "launch") && hip_ok(hipDeviceSynchronize(), "forward") &&
+             hip_ok(hipMemcpy(host.data(), o, host.size() * 4, hipMemcpyDeviceToHost), "download");
+    }
+    if (!ok) return RPHYS_ERR_INTERNAL;
+    for (uint32_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) out3[3 * (size_t)i + k] = host[4 * (size_t)i + k];
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const float *targ3, float *loss, float *grad) {
+    GUARD_BEGIN
+    if (!params || !feat12 || !targ3 || !loss || !grad) { set_err("rphys_mlp_gradient: null arg"); return RPHYS_ERR_NULL; }
+    RPhysResult r = check_mode("rphys_mlp_gradient", 2, params);
+    if (r != RPHYS_OK) return r;
+    const uint32_t n = RP_MLP_STEP_CONTACTS;
+    DeviceBuffers B;
+    float *f = nullptr, *t = nullptr;
+    if (!B.alloc(f, 12 * (size_t)n) || !B.alloc(t, 3 * (size_t)n)) { set_err("rphys_mlp_gradient: device allocation failed"); return RPHYS_ERR_CAPACITY; }
+    if (!hip_ok(hipMemcpy(f, feat12, 48 * (size_t)n, hipMemcpyHostToDevice), "upload") || !hip_ok(hipMemcpy(t, targ3, 12 * (size_t)n, hipMemcpyHostToDevice), "upload"))
+        return RPHYS_ERR_INTERNAL;
+    std::vector<float> p(params, params + RP_MLP_NPARAM);
+    return train_on_device(f, t, n, 1, 1, 1.0e-3f, 0, p.data(), loss, true, grad) ? RPHYS_OK : RPHYS_ERR_INTERNAL;
     GUARD_END
 }
 

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rp_mlp.h"
+
 #define RP_NONE 0xFFFFFFFFu
 #define RP_TERRAIN 0xFFFFFFFFu
 #define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase
@@ -99,6 +101,14 @@ struct Dev {
     uint32_t *bs_pa, *bs_pb, *bs_c, *ss_pa, *ss_pb, *ss_c, *slot_of;
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
+    // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model
+    float *feat;
+    float4 *pred;
+    MlpWeights mlp;
+    // contact log (training data): rows of features [log_cap][12] and converged impulses [log_cap][3]
+    float *log_feat, *log_targ;
+    uint32_t *log_n;  // [0] rows written, [1] rows dropped for lack of room (kept across ticks)
+    uint32_t log_cap;
     // radix-sort ping-pong buffers (2 * cap_contacts each) and the shock keys' high bits
     uint32_t *sk0, *sv0, *sk1, *sv1, *sh_hi;
     uint32_t *rs_hist;
@@ -160,4 +170,6 @@ struct TickParams {
     uint32_t cur;          // warm buffer holding last tick's cache
     uint32_t resid_on;     // residual mode: an island stops once its sweep residual <= resid_tol
     float resid_tol;
+    uint32_t ws_mode;      // warm start: 0 previous frame (the reference), 1 zero, 2 learned
+    uint32_t log_on;       // append (features, converged impulses) of solved contacts to the log
 };

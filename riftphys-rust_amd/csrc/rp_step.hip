@@ -618,6 +618,8 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
             if (best < 0) continue;
             taken |= 1u << (best - es);
             d.warm_idx[g] = best;
+            // zero / learned warm start: solved rows start elsewhere (the match stays a feature)
+            if (tp.ws_mode != 0 && (d.cs_wa[g] != RP_NONE || d.cs_wb[g] != RP_NONE)) continue;
             float ejn = d.wk_jn[cur][best];
             d.jn[g] = ejn;
             d.jn_pre[g] = ejn;
@@ -639,6 +641,57 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
             }
         }
     }
+}
+
+// ------------------------------------------------------------------ learned warm start
+// The 12 per-contact features of the warm-start model, from the constraint as prepared and the
+// velocities entering the solve: normal and tangential relative velocity, penetration, the
+// sides' inverse masses, the angular terms (r x d) . I^-1 (r x d) for the normal (both sides) and
+// tangent 1 (both sides), the previous frame's matched normal impulse (0 if none), and mu_s.
+__global__ void k_features(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    GRID_STRIDE(c, nct) {
+        const uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+        const V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
+        const float2 inv = d.cs_inv[c];
+        V3 va = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
+        V3 vb = bi == RP_NONE ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
+        V3 v = vb - va;
+        M3 iia = inv.x > 0.0f ? load_m9(d.iiw + 9 * ai) : mzero();
+        M3 iib = inv.y > 0.0f ? load_m9(d.iiw + 9 * bi) : mzero();
+        V3 an = cross3(ra, n), bn = cross3(rb, n), at = cross3(ra, t1), bt = cross3(rb, t1);
+        const int wi = d.warm_idx[c];
+        float *f = d.feat + (size_t)c * 12;
+        f[0] = dot3(v, n); f[1] = dot3(v, t1); f[2] = dot3(v, t2);
+        f[3] = d.ct_p[c].w - 0.05f;
+        f[4] = inv.x; f[5] = inv.y;
+        f[6] = dot3(an, mv(iia, an)); f[7] = dot3(bn, mv(iib, bn)); f[8] = dot3(at, mv(iia, at)); f[9] = dot3(bt, mv(iib, bt));
+        f[10] = wi >= 0 ? d.wk_jn[tp.cur][wi] : 0.0f;
+        f[11] = d.cs_mu[c].x;
+    }
+}
+// Appends the solved contacts of this tick (island-list order) with their converged impulses:
+// the normal impulse before the shock pass and the friction accumulators (solver.rs:917-926).
+__global__ void k_log_contacts(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    const uint32_t base = d.log_n[0];
+    const bool room = (uint64_t)base + nislc <= d.log_cap;
+    GRID_STRIDE(r, nislc) {
+        if (!room) break;
+        const uint32_t c = d.isl_v2[r];
+        const size_t o = (size_t)base + r;
+        for (int k = 0; k < 12; k++) d.log_feat[o * 12 + k] = d.feat[(size_t)c * 12 + k];
+        const float2 t = d.jt[c];
+        d.log_targ[o * 3] = d.jn_pre[c]; d.log_targ[o * 3 + 1] = t.x; d.log_targ[o * 3 + 2] = t.y;
+    }
+}
+__global__ void k_log_advance(Dev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || aborted(d)) return;
+    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    if ((uint64_t)d.log_n[0] + nislc <= d.log_cap) d.log_n[0] += nislc;
+    else d.log_n[1] += nislc;
 }
 
 // ------------------------------------------------------------------ ordered sweeps
@@ -1024,8 +1077,25 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
         build_slots(d, I, d.isl_v2, d.ch_pa, d.ch_pb, exb, sb);
         if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_ORDER);
-        // warm-start application (solver.rs:749-760)
-        run_sweep(I, sb, [&](uint32_t c) {
+        // warm-start application (solver.rs:749-760): the previous frame's impulses (reference),
+        // nothing (zero), or the model's prediction for every solved row (learned)
+        if (tp.ws_mode == 2) run_sweep(I, sb, [&](uint32_t c) {
+            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+            V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
+            float2 inv = d.cs_inv[c];
+            const float4 pr = d.pred[c];
+            float pjn = fmaxf(pr.x, 0.0f);
+            if (!(pjn == pjn)) pjn = 0.0f;
+            V3 v = rel_vel(d, ai, bi, ra, rb);
+            float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
+            float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
+            V3 t = clamp_len_max(mk3(pr.y, pr.z, 0.0f), mu * pjn);
+            if (!fin3(t)) t = splat3(0.0f);
+            d.jn[c] = pjn;
+            d.jt[c] = make_float2(t.x, t.y);
+            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, n3 * pjn + (t1 * t.x + t2 * t.y));
+        });
+        else if (tp.ws_mode == 0) run_sweep(I, sb, [&](uint32_t c) {
             int wi = d.warm_idx[c];
             if (wi < 0) return;
             uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
@@ -1551,6 +1621,8 @@ __global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (thr
 __global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n) { GRID_STRIDE(i, n) dst[i] = src[i]; }
 
 // ------------------------------------------------------------------ host-side launch sequence
+extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
+                                    uint32_t cap, float *out, hipStream_t st);
 static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t max_grid = 8192) {
     uint32_t g = (n + block - 1) / block;
     if (g < 1) g = 1;
@@ -1674,6 +1746,9 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_MAX, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_SUM, 0u);
     hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
+    if (tp.ws_mode == 2 || tp.log_on) hipLaunchKernelGGL(k_features, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
+    if (tp.ws_mode == 2)
+        rp_launch_mlp_infer(d.mlp.p, d.mlp.w1p, d.mlp.w2p, d.mlp.w3p, d.feat, d.cnt + CT_CONTACTS, d.cap_contacts, (float *)d.pred, st);
     mark(RP_ST_ORDERS);
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
     build_shock_order(d, tp, body_bits, st);
@@ -1684,6 +1759,10 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
     mark(RP_ST_VEL_HUGE);
     hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    if (tp.log_on) {
+        hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+        hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, d);
+    }
     mark(RP_ST_WARM_STORE);
     // warm cache for the next tick (into buffer cur ^ 1)
     hipLaunchKernelGGL(k_warm_newflags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);

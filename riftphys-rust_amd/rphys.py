@@ -91,6 +91,10 @@ class Profile(C.Structure):
                 ("sweep_phase", C.c_double * 11)]
 
 
+class MlpTrainConfig(C.Structure):
+    _fields_ = [("size_bytes", C.c_uint32), ("epochs", C.c_uint32), ("lr", C.c_float), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
 # (name, restype, argtypes) of every export in include/riftphys_c.h
 P = C.c_void_p
 U32, U64, F32 = C.c_uint32, C.c_uint64, C.c_float
@@ -153,6 +157,16 @@ EXPORTS = [
     ("rphys_batch_total_bodies", U32, [P, PU32]),
     ("rphys_batch_step_graph", U32, [P, F32, U32]),
     ("rphys_batch_step_profiled", U32, [P, F32, U32, C.POINTER(Profile)]),
+    ("rphys_mlp_param_count", U32, []),
+    ("rphys_mlp_gradient_rows", U32, []),
+    ("rphys_world_set_warm_start", U32, [P, U32, PF32]),
+    ("rphys_batch_set_warm_start", U32, [P, U32, PF32]),
+    ("rphys_batch_set_contact_log", U32, [P, U32]),
+    ("rphys_batch_contact_log_rows", U32, [P, PU32, PU32]),
+    ("rphys_batch_read_contact_log", U32, [P, U32, U32, PF32, PF32]),
+    ("rphys_batch_train_warm_start", U32, [P, C.POINTER(MlpTrainConfig), PF32, PF32]),
+    ("rphys_mlp_forward", U32, [PF32, PF32, U32, PF32]),
+    ("rphys_mlp_gradient", U32, [PF32, PF32, PF32, PF32, PF32]),
 ]
 
 _lib = None
@@ -204,6 +218,33 @@ def _profile_dicts(pr):
              "proj_shock", "levels_list", "levels_shock")
     counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}
     return stages, counters
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    if shape is not None:
+        assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+def mlp_forward(params, feat):
+    """Device inference of the warm-start model on host features [n][12] -> [n][3]."""
+    p, f = _f32(params), _f32(feat)
+    out = np.zeros((len(f), 3), np.float32)
+    check(lib().rphys_mlp_forward(p.ctypes.data_as(PF32), f.ctypes.data_as(PF32), len(f), out.ctypes.data_as(PF32)))
+    return out
+
+
+def mlp_gradient(params, feat, targ):
+    """(loss, gradient of the weights and biases) of one training step over the given rows."""
+    p, f, t = _f32(params), _f32(feat), _f32(targ)
+    n = lib().rphys_mlp_gradient_rows()
+    assert len(f) == n and len(t) == n
+    loss = np.zeros(1, np.float32)
+    grad = np.zeros(lib().rphys_mlp_param_count() - 27, np.float32)
+    check(lib().rphys_mlp_gradient(p.ctypes.data_as(PF32), f.ctypes.data_as(PF32), t.ctypes.data_as(PF32), loss.ctypes.data_as(PF32),
+                                   grad.ctypes.data_as(PF32)))
+    return float(loss[0]), grad
 
 
 def copy_probe(nbytes, reps=20):
@@ -352,6 +393,10 @@ class World:
         st["tick"] = self.tick()
         return st
 
+    def set_warm_start(self, mode, params=None):
+        p = None if params is None else _f32(params)
+        check(lib().rphys_world_set_warm_start(self.h, mode, None if p is None else p.ctypes.data_as(PF32)))
+
     def snapshot(self):
         s = lib().rphys_world_snapshot_create(self.h)
         if not s:
@@ -414,6 +459,30 @@ class Batch:
         lin = np.zeros((n, 3), np.float32); ang = np.zeros((n, 3), np.float32)
         check(lib().rphys_batch_get_state(self.h, wi, pos.ctypes.data_as(PF32), rot.ctypes.data_as(PF32), lin.ctypes.data_as(PF32), ang.ctypes.data_as(PF32)))
         return pos, rot, lin, ang
+
+    def set_warm_start(self, mode, params=None):
+        p = None if params is None else _f32(params)
+        check(lib().rphys_batch_set_warm_start(self.h, mode, None if p is None else p.ctypes.data_as(PF32)))
+
+    def set_contact_log(self, rows):
+        check(lib().rphys_batch_set_contact_log(self.h, rows))
+
+    def contact_log_rows(self):
+        n, dropped = C.c_uint32(0), C.c_uint32(0)
+        check(lib().rphys_batch_contact_log_rows(self.h, C.byref(n), C.byref(dropped)))
+        return n.value, dropped.value
+
+    def read_contact_log(self, first, count):
+        f = np.zeros((count, 12), np.float32); t = np.zeros((count, 3), np.float32)
+        check(lib().rphys_batch_read_contact_log(self.h, first, count, f.ctypes.data_as(PF32), t.ctypes.data_as(PF32)))
+        return f, t
+
+    def train_warm_start(self, epochs=20, lr=1.0e-3, seed=1):
+        cfg = MlpTrainConfig(C.sizeof(MlpTrainConfig), epochs, lr, 0, seed)
+        params = np.zeros(lib().rphys_mlp_param_count(), np.float32)
+        loss = np.zeros(epochs, np.float32)
+        check(lib().rphys_batch_train_warm_start(self.h, C.byref(cfg), params.ctypes.data_as(PF32), loss.ctypes.data_as(PF32)))
+        return params, loss
 
     def snapshot(self, slot=0):
         check(lib().rphys_batch_snapshot(self.h, slot))

@@ -246,12 +246,110 @@ def batch_cpu_baseline(worlds, warmup, ticks, dt):
             % (len(sample), len(worlds), max(1, len(worlds) // 8), span, warmup, t_spent, cpu_model(), os.cpu_count() or 0)}
 
 
+def learned_bench(a):
+    """Learned + batched vs plain, PGS to residual 1e-4 (per island, at most 100 sweeps).
+
+    1. Training data: the solver's own converged impulses on 16 seeds of the 512-box lattice
+       (residual mode), about 10M contacts; 20 epochs of Adam at lr 1e-3 on the device.
+    2. Batched: the batched worlds at tick `warmup`, the same ticks stepped from one snapshot with
+       the previous-frame (reference), zero and learned warm start.
+    3. Plain: one 16,384-body world at tick `warmup`, previous-frame warm start.
+    """
+    tol, max_it = 1.0e-4, 100
+    t0 = time.perf_counter()
+    lat = [scenes.box_lattice(seed=scenes.seed_for(4, w)) for w in range(16)]
+    tb = rphys.Batch([rphys.World(s_) for s_ in lat])
+    tb.step_n(60, graph=True)  # the lattices collapse (fixed iterations)
+    tb.set_residual_mode(True, tol, max_it)
+    tb.set_contact_log(a.train_rows + 200000)
+    rows = 0
+    while rows < a.train_rows:
+        tb.step_n(10, graph=True)
+        rows = tb.contact_log_rows()[0]
+        print("bench: training data %d rows, %.1f s" % (rows, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    t_data = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    params, losses = tb.train_warm_start(epochs=a.epochs, lr=1.0e-3, seed=20260101)
+    t_train = time.perf_counter() - t1
+    print("bench: trained %d epochs in %.1f s, loss %.4f -> %.4f" % (a.epochs, t_train, losses[0], losses[-1]), file=sys.stderr, flush=True)
+    del tb
+
+    def timed(stepper, nb_):
+        el, isum, ic, cts, done = 0.0, 0.0, 0.0, 0.0, 0
+        while done < a.steps:
+            k = min(10, a.steps - done)
+            t2 = time.perf_counter()
+            st, c = stepper(k)
+            el += time.perf_counter() - t2
+            isum += c["pgs_iters_sum"]; ic += c["island_contacts"]; cts += c["contacts"]
+            done += k
+            print("bench:   %d/%d ticks, %.1f s, %.0f contacts/tick, %.1f sweeps/contact" % (done, a.steps, el, c["contacts"] / k,
+                  c["pgs_iters_sum"] / max(c["island_contacts"], 1)), file=sys.stderr, flush=True)
+        return {"body_steps_per_s": nb_ * a.steps / el, "ms_per_tick": 1000.0 * el / a.steps, "mean_sweeps_per_contact": isum / max(ic, 1),
+                "contacts_per_tick": cts / a.steps}
+
+    # in distribution: 16 held-out seeds of the lattice
+    lb = rphys.Batch([rphys.World(scenes.box_lattice(seed=scenes.seed_for(4, 100 + w))) for w in range(16)])
+    lb.step_n(90, graph=True)
+    lb.snapshot(1)
+    lb.set_residual_mode(True, tol, max_it)
+    lattice = {}
+    for name, mode in (("previous_frame", 0), ("zero", 1), ("learned", 2)):
+        lb.restore(1)
+        lb.set_warm_start(mode, params if mode == 2 else None)
+        lattice[name] = timed(lambda k: lb.step_profiled(k, 1.0 / 60.0), lb.total_bodies())
+        print("bench: lattice, %s warm start: %.0f body-steps/s, %.2f sweeps per contact" % (name, lattice[name]["body_steps_per_s"],
+                                                                                            lattice[name]["mean_sweeps_per_contact"]), file=sys.stderr, flush=True)
+    del lb
+
+    worlds = [scenes.batch_world(w) for w in range(a.worlds)]
+    b = rphys.Batch([rphys.World(s_) for s_ in worlds])
+    nb = b.total_bodies()
+    dt = float(worlds[0].get("dt", 1.0 / 60.0))
+    for k in range(0, a.warmup, 50):  # warm up with the fixed iterations
+        b.step_n(min(50, a.warmup - k), dt, graph=True)
+        b.hashes()
+        print("bench: batched warmup %d/%d ticks" % (min(k + 50, a.warmup), a.warmup), file=sys.stderr, flush=True)
+    b.snapshot(1)
+    b.set_residual_mode(True, tol, max_it)
+    modes = {}
+    for name, mode in (("previous_frame", 0), ("zero", 1), ("learned", 2)):
+        b.restore(1)
+        b.set_warm_start(mode, params if mode == 2 else None)
+        modes[name] = timed(lambda k: b.step_profiled(k, dt), nb)
+        print("bench: batched, %s warm start: %.0f body-steps/s, %.2f sweeps per contact" % (name, modes[name]["body_steps_per_s"],
+                                                                                            modes[name]["mean_sweeps_per_contact"]), file=sys.stderr, flush=True)
+    del b
+
+    w = rphys.World(scenes.mixed_shapes())
+    for k in range(0, a.warmup, 50):
+        w.step_n(min(50, a.warmup - k))
+        w.hash()
+        print("bench: plain warmup %d/%d ticks" % (min(k + 50, a.warmup), a.warmup), file=sys.stderr, flush=True)
+    w.set_residual_mode(True, tol, max_it)
+    plain = timed(lambda k: w.step_profiled(k), w.num_bodies())
+    print("bench: plain single world: %.0f body-steps/s, %.2f sweeps per contact" % (plain["body_steps_per_s"], plain["mean_sweeps_per_contact"]),
+          file=sys.stderr, flush=True)
+    value = modes["learned"]["body_steps_per_s"]
+    ms = modes["learned"]["ms_per_tick"]
+    return {"metric": "body-steps/s on 1 GPU, learned+batched vs plain; PGS iters to residual 1e-4", "value": value, "unit": "body-steps/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 (model: bf16 weights and activations, f32 accumulation)", "data": "synthetic (seeded XorShift64 scenes)",
+            "config": {"workload": "learned_batch%d_vs_plain16k" % a.worlds, "residual": tol, "max_sweeps": max_it, "worlds": a.worlds, "bodies": nb,
+                       "training": {"scene": "16 seeds of the 512-box lattice", "rows": rows, "epochs": a.epochs, "lr": 1e-3,
+                                    "loss_first_last": [float(losses[0]), float(losses[-1])], "data_s": t_data, "train_s": t_train},
+                       "lattice_held_out": lattice, "batched": modes, "plain": plain, "learned_batched_over_plain": value / plain["body_steps_per_s"]},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch"], default="mixed16k")
+    ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch", "learned"], default="mixed16k")
+    ap.add_argument("--train-rows", type=int, default=10_000_000, help="learned workload: contacts logged for training")
+    ap.add_argument("--epochs", type=int, default=20, help="learned workload: training epochs")
     ap.add_argument("--worlds", type=int, default=1024, help="batch workload: number of worlds")
     ap.add_argument("--cpu-ticks", type=int, default=3, help="oracle ticks sampled per 100-tick chunk")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle sample (profiling runs)")
@@ -268,6 +366,13 @@ def main():
               "world across GPUs with no per-tick exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "learned":
+        if a.steps == 600:
+            a.steps = 40
+        if a.warmup == 10:
+            a.warmup = 150
+        print(json.dumps(learned_bench(a)), flush=True)
+        return
     if a.workload == "batch":
         if a.steps == 600:
             a.steps = 160

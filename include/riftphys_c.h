@@ -260,6 +260,7 @@ uint32_t rphys_mlp_param_count(void);
 uint32_t rphys_mlp_gradient_rows(void);  /* rows of one training step (rphys_mlp_gradient) */
 RPhysResult rphys_world_set_warm_start(RPhysWorld *world, uint32_t mode, const float *params);
 RPhysResult rphys_batch_set_warm_start(RPhysBatch *batch, uint32_t mode, const float *params);
+RPhysResult rphys_batch_set_residual_mode(RPhysBatch *batch, uint32_t on, float tol, uint32_t max_iters);  /* as the world call, every world */
 /* Contact log: while enabled, every tick appends its solved contacts (island-list order) as
  * features [12] and converged impulses [3] (normal before the shock pass, friction), until full. */
 RPhysResult rphys_batch_set_contact_log(RPhysBatch *batch, uint32_t capacity_rows);  /* 0 disables */

@@ -1554,6 +1554,18 @@ RPhysResult rphys_batch_set_warm_start(RPhysBatch *b, uint32_t mode, const float
     GUARD_END
 }
 
+RPhysResult rphys_batch_set_residual_mode(RPhysBatch *b, uint32_t on, float tol, uint32_t max_iters) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_set_residual_mode: batch is null"); return RPHYS_ERR_NULL; }
+    if (on && (!hfin(tol) || tol < 0.0f || max_iters == 0)) return bad_arg("rphys_batch_set_residual_mode", "tol must be finite and >= 0, max_iters > 0");
+    for (auto &w : b->worlds) {
+        w.resid_on = on ? 1u : 0u;
+        if (on) { w.resid_tol = tol; w.resid_max = max_iters; }
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
+
 RPhysResult rphys_batch_set_contact_log(RPhysBatch *b, uint32_t capacity_rows) {
     GUARD_BEGIN
     if (!b) { set_err("rphys_batch_set_contact_log: batch is null"); return RPHYS_ERR_NULL; }

@@ -910,6 +910,7 @@ static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa,
         if (pa[s + r] == RP_NONE && pb[s + r] == RP_NONE) ex[s + atomicAdd(&tail, 1u)] = r;
     __syncthreads();
     uint32_t fs = 0, fe = tail, nr = 0;
+    __syncthreads();  // every thread has the frontier's end before any appends to the next
     while (fs < fe) {
         nr++;
         for (uint32_t j = fs + threadIdx.x; j < fe; j += blockDim.x) {
@@ -1167,8 +1168,12 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
                         if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
                     }
                 });
+            // every thread takes the sweep's residual before thread 0 may clear it for the next
+            // sweep, so all threads leave the loop at the same sweep
+            const float rmax = __uint_as_float(resid_bits);
+            __syncthreads();
             used++;
-            if (tp.resid_on && __uint_as_float(resid_bits) <= tp.resid_tol) break;
+            if (tp.resid_on && rmax <= tp.resid_tol) break;
         }
         if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n); }
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);

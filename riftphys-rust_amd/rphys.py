@@ -162,6 +162,7 @@ EXPORTS = [
     ("rphys_world_set_warm_start", U32, [P, U32, PF32]),
     ("rphys_batch_set_warm_start", U32, [P, U32, PF32]),
     ("rphys_batch_set_contact_log", U32, [P, U32]),
+    ("rphys_batch_set_residual_mode", U32, [P, U32, F32, U32]),
     ("rphys_batch_contact_log_rows", U32, [P, PU32, PU32]),
     ("rphys_batch_read_contact_log", U32, [P, U32, U32, PF32, PF32]),
     ("rphys_batch_train_warm_start", U32, [P, C.POINTER(MlpTrainConfig), PF32, PF32]),
@@ -397,6 +398,9 @@ class World:
         p = None if params is None else _f32(params)
         check(lib().rphys_world_set_warm_start(self.h, mode, None if p is None else p.ctypes.data_as(PF32)))
 
+    def set_residual_mode(self, on, tol=1.0e-4, max_iters=100):
+        check(lib().rphys_world_set_residual_mode(self.h, 1 if on else 0, tol, max_iters))
+
     def snapshot(self):
         s = lib().rphys_world_snapshot_create(self.h)
         if not s:
@@ -463,6 +467,9 @@ class Batch:
     def set_warm_start(self, mode, params=None):
         p = None if params is None else _f32(params)
         check(lib().rphys_batch_set_warm_start(self.h, mode, None if p is None else p.ctypes.data_as(PF32)))
+
+    def set_residual_mode(self, on, tol=1.0e-4, max_iters=100):
+        check(lib().rphys_batch_set_residual_mode(self.h, 1 if on else 0, tol, max_iters))
 
     def set_contact_log(self, rows):
         check(lib().rphys_batch_set_contact_log(self.h, rows))

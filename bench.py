@@ -247,19 +247,22 @@ def batch_cpu_baseline(worlds, warmup, ticks, dt):
 
 
 def learned_bench(a):
-    """Learned + batched vs plain, PGS to residual 1e-4 (per island, at most 100 sweeps).
+    """Learned + batched vs plain, PGS to residual 1e-4 (per island, at most `--max-sweeps`).
 
-    1. Training data: the solver's own converged impulses on 16 seeds of the 512-box lattice
-       (residual mode), about 10M contacts; 20 epochs of Adam at lr 1e-3 on the device.
-    2. Batched: the batched worlds at tick `warmup`, the same ticks stepped from one snapshot with
-       the previous-frame (reference), zero and learned warm start.
+    1. Training data: the solver's own converged impulses (residual mode, less the previous-frame
+       warm start) on 16 seeds of the 512-box lattice and 64 mixed-shape worlds of the batched
+       workload's generator with their own seeds, ticks 120 on, about 10M contacts; 20 epochs of
+       Adam at lr 1e-3 on the device.
+    2. Held-out lattices and the batched worlds at tick `warmup`, the same ticks stepped from one
+       snapshot with the previous-frame (reference), zero and learned warm start.
     3. Plain: one 16,384-body world at tick `warmup`, previous-frame warm start.
     """
-    tol, max_it = 1.0e-4, 100
+    tol, max_it = 1.0e-4, a.max_sweeps
     t0 = time.perf_counter()
     lat = [scenes.box_lattice(seed=scenes.seed_for(4, w)) for w in range(16)]
+    lat += [scenes.batch_world(w, seed_base_k=5) for w in range(64)]
     tb = rphys.Batch([rphys.World(s_) for s_ in lat])
-    tb.step_n(60, graph=True)  # the lattices collapse (fixed iterations)
+    tb.step_n(120, graph=True)  # the lattices collapse, the mixed shapes land (fixed iterations)
     tb.set_residual_mode(True, tol, max_it)
     tb.set_contact_log(a.train_rows + 200000)
     rows = 0
@@ -275,18 +278,19 @@ def learned_bench(a):
     del tb
 
     def timed(stepper, nb_):
-        el, isum, ic, cts, done = 0.0, 0.0, 0.0, 0.0, 0
+        el, isum, ic, cts, done, conv, lrn = 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0
         while done < a.steps:
             k = min(10, a.steps - done)
             t2 = time.perf_counter()
             st, c = stepper(k)
             el += time.perf_counter() - t2
             isum += c["pgs_iters_sum"]; ic += c["island_contacts"]; cts += c["contacts"]
+            conv += c["island_contacts_converged"]; lrn += c["warm_start_learned"]
             done += k
             print("bench:   %d/%d ticks, %.1f s, %.0f contacts/tick, %.1f sweeps/contact" % (done, a.steps, el, c["contacts"] / k,
                   c["pgs_iters_sum"] / max(c["island_contacts"], 1)), file=sys.stderr, flush=True)
         return {"body_steps_per_s": nb_ * a.steps / el, "ms_per_tick": 1000.0 * el / a.steps, "mean_sweeps_per_contact": isum / max(ic, 1),
-                "contacts_per_tick": cts / a.steps}
+                "contacts_per_tick": cts / a.steps, "converged_contact_frac": conv / max(ic, 1), "learned_contact_frac": lrn / max(cts, 1)}
 
     # in distribution: 16 held-out seeds of the lattice
     lb = rphys.Batch([rphys.World(scenes.box_lattice(seed=scenes.seed_for(4, 100 + w))) for w in range(16)])
@@ -298,8 +302,9 @@ def learned_bench(a):
         lb.restore(1)
         lb.set_warm_start(mode, params if mode == 2 else None)
         lattice[name] = timed(lambda k: lb.step_profiled(k, 1.0 / 60.0), lb.total_bodies())
-        print("bench: lattice, %s warm start: %.0f body-steps/s, %.2f sweeps per contact" % (name, lattice[name]["body_steps_per_s"],
-                                                                                            lattice[name]["mean_sweeps_per_contact"]), file=sys.stderr, flush=True)
+        print("bench: lattice, %s warm start: %.0f body-steps/s, %.2f sweeps per contact, %.3f converged, %.3f learned" % (
+            name, lattice[name]["body_steps_per_s"], lattice[name]["mean_sweeps_per_contact"], lattice[name]["converged_contact_frac"],
+            lattice[name]["learned_contact_frac"]), file=sys.stderr, flush=True)
     del lb
 
     worlds = [scenes.batch_world(w) for w in range(a.worlds)]
@@ -317,8 +322,9 @@ def learned_bench(a):
         b.restore(1)
         b.set_warm_start(mode, params if mode == 2 else None)
         modes[name] = timed(lambda k: b.step_profiled(k, dt), nb)
-        print("bench: batched, %s warm start: %.0f body-steps/s, %.2f sweeps per contact" % (name, modes[name]["body_steps_per_s"],
-                                                                                            modes[name]["mean_sweeps_per_contact"]), file=sys.stderr, flush=True)
+        print("bench: batched, %s warm start: %.0f body-steps/s, %.2f sweeps per contact, %.3f converged, %.3f learned" % (
+            name, modes[name]["body_steps_per_s"], modes[name]["mean_sweeps_per_contact"], modes[name]["converged_contact_frac"],
+            modes[name]["learned_contact_frac"]), file=sys.stderr, flush=True)
     del b
 
     w = rphys.World(scenes.mixed_shapes())
@@ -336,7 +342,7 @@ def learned_bench(a):
             "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32 (model: bf16 weights and activations, f32 accumulation)", "data": "synthetic (seeded XorShift64 scenes)",
             "config": {"workload": "learned_batch%d_vs_plain16k" % a.worlds, "residual": tol, "max_sweeps": max_it, "worlds": a.worlds, "bodies": nb,
-                       "training": {"scene": "16 seeds of the 512-box lattice", "rows": rows, "epochs": a.epochs, "lr": 1e-3,
+                       "training": {"scene": "16 seeds of the 512-box lattice + 64 mixed-shape worlds", "rows": rows, "epochs": a.epochs, "lr": 1e-3,
                                     "loss_first_last": [float(losses[0]), float(losses[-1])], "data_s": t_data, "train_s": t_train},
                        "lattice_held_out": lattice, "batched": modes, "plain": plain, "learned_batched_over_plain": value / plain["body_steps_per_s"]},
             "roofline": None, "cpu_baseline": None}
@@ -350,6 +356,7 @@ def main():
     ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch", "learned"], default="mixed16k")
     ap.add_argument("--train-rows", type=int, default=10_000_000, help="learned workload: contacts logged for training")
     ap.add_argument("--epochs", type=int, default=20, help="learned workload: training epochs")
+    ap.add_argument("--max-sweeps", type=int, default=100, help="learned workload: PGS sweep limit per island in residual mode")
     ap.add_argument("--worlds", type=int, default=1024, help="batch workload: number of worlds")
     ap.add_argument("--cpu-ticks", type=int, default=3, help="oracle ticks sampled per 100-tick chunk")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle sample (profiling runs)")

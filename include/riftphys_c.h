@@ -194,6 +194,9 @@ typedef struct RPhysProfile {
      * [4] velocity shock pass, [5] ballistic pass, [6] projection iterations, [7] projection shock
      * order, [8] projection shock pass; [9] / [10] levels of the list / shock orders, summed. */
     double sweep_phase[11];
+    /* Residual mode: islands, and their contacts, that reached the tolerance before the sweep
+     * limit (summed per tick).  Learned warm start: contacts that passed the model's gate. */
+    double islands_converged, island_contacts_converged, warm_start_learned;
 } RPhysProfile;
 RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out);
 /* Bandwidth probe (build addition): copies `bytes` between two fresh device buffers `reps` times
@@ -248,21 +251,25 @@ RPhysResult rphys_batch_step_graph(RPhysBatch *batch, float dt, uint32_t n_ticks
 /* ---- Learned PGS warm start (build addition; the reference has no counterpart) ----
  * A 12 -> 64 -> 64 -> 3 ReLU MLP (BF16 weights and activations, FP32 accumulation on MFMA) maps
  * per-contact features (normal/tangential relative velocity, penetration, inverse masses, four
- * angular effective-mass terms, the previous frame's matched normal impulse, mu_s) to the contact's
- * converged (normal, tangent 1, tangent 2) impulses.  Parameter vector (f32, rphys_mlp_param_count()
+ * angular effective-mass terms, the previous frame's matched normal impulse, mu_s) to a correction
+ * of the previous-frame warm start: converged (normal, tangent 1, tangent 2) impulses minus (matched
+ * jn, matched jt projected on the contact's tangents; zeros without a match).  Parameter vector (f32, rphys_mlp_param_count()
  * entries): W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] x_mean[12] x_std[12] y_scale[3];
  * inputs are normalised (x - x_mean) / x_std, outputs scaled by y_scale.
  * Warm-start mode: 0 the previous frame's impulses (the reference, default), 1 zero, 2 the
- * model's prediction for every solved contact (then clamped to the friction cone as a cached
- * impulse would be).  The solver's iterations are unchanged: with residual mode on, only the
- * number of sweeps to the residual depends on the warm start. */
+ * previous frame's plus the model's correction (jn clamped at 0, jt to the friction cone as a
+ * cached impulse would be) for every solved contact whose normalised inputs and outputs all lie
+ * within +-8; other contacts get mode 0's warm start exactly, so a zero model is mode 0.  The
+ * solver's iterations are unchanged: with residual mode on, only the number of sweeps to the
+ * residual depends on the warm start. */
 uint32_t rphys_mlp_param_count(void);
 uint32_t rphys_mlp_gradient_rows(void);  /* rows of one training step (rphys_mlp_gradient) */
 RPhysResult rphys_world_set_warm_start(RPhysWorld *world, uint32_t mode, const float *params);
 RPhysResult rphys_batch_set_warm_start(RPhysBatch *batch, uint32_t mode, const float *params);
 RPhysResult rphys_batch_set_residual_mode(RPhysBatch *batch, uint32_t on, float tol, uint32_t max_iters);  /* as the world call, every world */
 /* Contact log: while enabled, every tick appends its solved contacts (island-list order) as
- * features [12] and converged impulses [3] (normal before the shock pass, friction), until full. */
+ * features [12] and targets [3]: converged impulses (normal before the shock pass, friction) minus
+ * the previous-frame warm start as defined above, until full. */
 RPhysResult rphys_batch_set_contact_log(RPhysBatch *batch, uint32_t capacity_rows);  /* 0 disables */
 RPhysResult rphys_batch_contact_log_rows(RPhysBatch *batch, uint32_t *rows, uint32_t *dropped);
 RPhysResult rphys_batch_read_contact_log(RPhysBatch *batch, uint32_t first, uint32_t count, float *feat12, float *targ3);

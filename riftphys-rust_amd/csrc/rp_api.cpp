@@ -31,7 +31,7 @@ extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st);
 extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
-                                    uint32_t cap, float *out, hipStream_t st);
+                                    uint32_t cap, float *out, uint32_t *n_trusted, hipStream_t st);
 extern "C" void rp_launch_mlp_train(const TrainArgs &a, hipStream_t st);
 extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uint32_t max_extra_slots, hipStream_t st);
 
@@ -438,7 +438,9 @@ struct ProfileAcc {
         out->ticks += done;
         out->pairs += cnt[CT_SUM_PAIRS]; out->terrain_candidates += cnt[CT_SUM_CANDS]; out->contacts += cnt[CT_SUM_CONTACTS];
         out->island_contacts += cnt[CT_SUM_ISLC]; out->islands += cnt[CT_SUM_ISLANDS]; out->island_contacts_global += cnt[CT_SUM_ISLC_GLB];
-        out->pgs_iters_max += cnt[CT_SUM_ITERS_MAX]; out->pgs_iters_sum += cnt[CT_SUM_ITERS_SUM];
+        out->pgs_iters_max += cnt[CT_SUM_ITERS_MAX]; out->pgs_iters_sum += cnt[CT_SUM_ITERS_SUM] + 4294967296.0 * cnt[CT_SUM_ITERS_SUM_HI];
+        out->islands_converged += cnt[CT_SUM_ISL_CONV]; out->island_contacts_converged += cnt[CT_SUM_ISLC_CONV];
+        out->warm_start_learned += cnt[CT_SUM_WS_LEARNED];
         return true;
     }
 };
@@ -1628,7 +1630,7 @@ static void init_params(uint64_t seed, const double *xm, const double *xs, const
     std::memset(p, 0, RP_MLP_NPARAM * sizeof(float));
     fill(RP_MLP_W1, 64 * 12, 12, 64);
     fill(RP_MLP_W2, 64 * 64, 64, 64);
-    fill(RP_MLP_W3, 3 * 64, 64, 3);
+    // the output layer starts at zero: the untrained model is the previous-frame warm start
     for (int k = 0; k < 12; k++) { p[RP_MLP_XMEAN + k] = (float)xm[k]; p[RP_MLP_XSTD + k] = (float)xs[k]; }
     for (int k = 0; k < 3; k++) p[RP_MLP_YSCALE + k] = (float)ys[k];
 }
@@ -1696,7 +1698,9 @@ RPhysResult rphys_batch_train_warm_start(RPhysBatch *b, const RPhysMlpTrainConfi
         for (int k = 0; k < 12; k++) { double e = f[12 * i + k] - xm[k]; xs[k] += e * e; }
     for (size_t i = 0; i < rows; i++)
         for (int k = 0; k < 3; k++) ys[k] += (double)t[3 * i + k] * t[3 * i + k];
-    for (int k = 0; k < 12; k++) { xs[k] = std::sqrt(xs[k] / (double)rows); if (!(xs[k] > 1.0e-6)) xs[k] = 1.0; }
+    // a feature (nearly) constant in the data keeps a small scale, so other values of it fall
+    // outside the gate and take the previous-frame warm start
+    for (int k = 0; k < 12; k++) xs[k] = std::max(std::sqrt(xs[k] / (double)rows), std::max(1.0e-3 * std::fabs(xm[k]), 1.0e-6));
     for (int k = 0; k < 3; k++) { ys[k] = std::sqrt(ys[k] / (double)rows); if (!(ys[k] > 1.0e-6)) ys[k] = 1.0; }
     init_params(cfg->seed, xm, xs, ys, params_out);
     uint32_t spe = rows / RP_MLP_STEP_CONTACTS;
@@ -1723,7 +1727,7 @@ RPhysResult rphys_mlp_forward(const float *params, const float *feat12, uint32_t
               hip_ok(hipMemcpy(f, feat12, 48 * (size_t)n, hipMemcpyHostToDevice), "upload");
     if (ok) {
         rp_launch_mlp_prep(p, w1p, w2p, w2t, w3p, w3t, nullptr);
-        rp_launch_mlp_infer(p, w1p, w2p, w3p, f, nullptr, n, o, nullptr);
+        rp_launch_mlp_infer(p, w1p, w2p, w3p, f, nullptr, n, o, nullptr, nullptr);
         ok = hip_ok(hipGetLastError(), "launch") && hip_ok(hipDeviceSynchronize(), "forward") &&
              hip_ok(hipMemcpy(host.data(), o, host.size() * 4, hipMemcpyDeviceToHost), "download");
     }

@@ -28,8 +28,12 @@ enum {
     CT_NEED_CONTACTS, CT_NEED_WARM, CT_CHAIN_N, CT_WARM_TOTAL, CT_NEW_TOTAL, CT_KEEP_TOTAL, CT_SCRATCH, CT_DONE,
     CT_ISLC_GLB,  // contacts of the islands solved by the 1024-thread LDS-flag tier (this tick)
     CT_ITERS_MAX, CT_ITERS_SUM,  // PGS sweeps: most run by an island; sum of sweeps x island contacts
+    CT_ISL_CONV, CT_ISLC_CONV,   // residual mode: islands (and their contacts) that reached the tolerance
+    CT_WS_LEARNED,               // contacts whose learned warm start passed the gate
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
+    CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
+    CT_SUM_ITERS_SUM_HI,  // high word of the CT_SUM_ITERS_SUM sum (sweeps x contacts outgrow 32 bits)
     CT_COUNT
 };
 

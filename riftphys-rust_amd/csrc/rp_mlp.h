@@ -1,7 +1,13 @@
 // Learned PGS warm start: a 12 -> 64 -> 64 -> 3 MLP (ReLU) in BF16 with FP32 accumulation on
-// gfx950 MFMA (v_mfma_f32_16x16x32_bf16).  It maps 12 per-contact features to the contact's
-// converged (normal, tangent 1, tangent 2) impulses, which then replace the previous-frame warm
-// start; the solver still iterates to its residual.
+// gfx950 MFMA (v_mfma_f32_16x16x32_bf16).  It maps 12 per-contact features to a correction of the
+// previous-frame warm start: the contact starts from (matched jn, matched jt projected on the
+// current tangents) + prediction, 0 where nothing matched, and the solver still iterates to its
+// residual.  A model that outputs zeros is therefore the previous-frame warm start.
+//
+// Gate: a contact takes the prediction only when every normalised input lies in
+// [-RP_MLP_ZMAX, RP_MLP_ZMAX] and every normalised output in [-RP_MLP_YMAX, RP_MLP_YMAX]; any
+// other contact (out of the training distribution, or a non-finite value) gets the previous-frame
+// warm start exactly.
 //
 // Canonical parameter vector (f32, RP_MLP_NPARAM entries, the ABI's layout):
 //   W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3] x_mean[12] x_std[12] y_scale[3]
@@ -24,6 +30,8 @@
 #define RP_MLP_XMEAN (RP_MLP_B3 + 3)
 #define RP_MLP_XSTD (RP_MLP_XMEAN + 12)
 #define RP_MLP_YSCALE (RP_MLP_XSTD + 12)
+#define RP_MLP_ZMAX 8.0f
+#define RP_MLP_YMAX 8.0f
 
 // Device copies used by the MFMA tiles (bf16, zero-padded): W1p[64][32], W2p[64][64] (out x in),
 // W2t[64][64] (in x out), W3p[16][64], W3t[64][32] (in x out, out padded to 32).

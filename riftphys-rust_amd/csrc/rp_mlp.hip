@@ -93,17 +93,38 @@ static __device__ f32x4 tile_forward(bf16x8 x, WP w1p, WP w2p, WP w3p, const flo
 }
 
 // ------------------------------------------------------------------ inference
-// out[c][0..2] = prediction for feat[c] (denormalised), c < min(*n_ptr, cap).  Each block takes
-// 64 contacts per iteration (one 16-contact tile per wave).
+// out[c][0..2] = prediction for feat[c] (denormalised), out[c][3] = 1 when the contact passes the
+// gate (inputs within RP_MLP_ZMAX, outputs within RP_MLP_YMAX, in normalised units) and 0
+// otherwise, c < min(*n_ptr, cap).  Each block takes 64 contacts per iteration (one 16-contact
+// tile per wave).  n_trusted (optional) gains the number of contacts that pass.
 __global__ void __launch_bounds__(256) k_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p,
-                                                   const float *feat, const uint32_t *n_ptr, uint32_t cap, float *out) {
+                                                   const float *feat, const uint32_t *n_ptr, uint32_t cap, float *out, uint32_t *n_trusted) {
     __shared__ __bf16 act[4][2][16 * 64];
+    __shared__ uint32_t bad[4][16];
+    __shared__ uint32_t block_trusted;
     const uint32_t n = n_ptr ? min(*n_ptr, cap) : cap;
     const uint32_t wave = threadIdx.x >> 6, l = threadIdx.x & 63u, col = l & 15u, grp = l >> 4;
     RP_LDS __bf16 *h1s = (RP_LDS __bf16 *)act[wave][0], *h2s = (RP_LDS __bf16 *)act[wave][1];
+    RP_LDS uint32_t *bw = (RP_LDS uint32_t *)bad[wave];
+    if (threadIdx.x == 0) block_trusted = 0;
+    uint32_t trusted = 0;
     for (uint64_t base = (uint64_t)blockIdx.x * 64; base < n; base += (uint64_t)gridDim.x * 64) {
         const uint64_t t0 = base + 16 * wave;
         const uint64_t c = t0 + col;
+        if (l < 16) bw[l] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        // input gate: lanes of groups 0 and 1 hold features 0..7 and 8..11 of row `col`
+        if (grp < 2 && c < n) {
+            bool ok = true;
+            for (int j = 0; j < 8; j++) {
+                uint32_t k = 8u * grp + j;
+                if (k < 12u) {
+                    float z = (feat[c * 12 + k] - p[RP_MLP_XMEAN + k]) / p[RP_MLP_XSTD + k];
+                    ok = ok && fabsf(z) <= RP_MLP_ZMAX;
+                }
+            }
+            if (!ok) bw[col] = 1u;
+        }
         bf16x8 x = input_operand(feat, p, c, c < n);
         f32x4 y = tile_forward(x, w1p, w2p, w3p, p, h1s, h2s);
         if (col < 3) {
@@ -111,9 +132,21 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *p, const __bf16 
             for (int r = 0; r < 4; r++) {
                 uint64_t cr = t0 + 4 * grp + r;
                 if (cr < n) out[cr * 4 + col] = y[r] * sc;
+                if (!(fabsf(y[r]) <= RP_MLP_YMAX)) bw[4 * grp + r] = 1u;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (l < 16 && c < n) {
+            const bool ok = bw[l] == 0u;
+            out[c * 4 + 3] = ok ? 1.0f : 0.0f;
+            trusted += ok ? 1u : 0u;
+        }
         __syncthreads();
+    }
+    if (n_trusted) {
+        if (trusted) atomicAdd(&block_trusted, trusted);
+        __syncthreads();
+        if (threadIdx.x == 0 && block_trusted) atomicAdd(n_trusted, block_trusted);
     }
 }
 
@@ -121,11 +154,11 @@ extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __b
     hipLaunchKernelGGL(k_mlp_prep, dim3(1), dim3(256), 0, st, p, w1p, w2p, w2t, w3p, w3t);
 }
 extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
-                                    uint32_t cap, float *out, hipStream_t st) {
+                                    uint32_t cap, float *out, uint32_t *n_trusted, hipStream_t st) {
     uint32_t blocks = (cap + 63) / 64;
     if (blocks < 1) blocks = 1;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, p, w1p, w2p, w3p, feat, n_ptr, cap, out);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, p, w1p, w2p, w3p, feat, n_ptr, cap, out, n_trusted);
 }
 
 // ------------------------------------------------------------------ training

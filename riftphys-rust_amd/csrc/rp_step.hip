@@ -671,9 +671,19 @@ __global__ void k_features(Dev d, TickParams tp) {
         f[11] = d.cs_mu[c].x;
     }
 }
-// Appends the solved contacts of this tick (island-list order) with their converged impulses:
-// the normal impulse before the shock pass and the friction accumulators (solver.rs:917-926).
-__global__ void k_log_contacts(Dev d) {
+// The previous-frame warm start of contact c in the contact's frame: (matched jn, matched jt
+// projected on the current tangents), zeros when nothing matched (solver.rs:749-760 before the
+// friction clamp).  The learned model predicts the converged impulses minus this.
+static __device__ __forceinline__ V3 warm_base(const Dev &d, uint32_t c, uint32_t cur) {
+    const int wi = d.warm_idx[c];
+    if (wi < 0) return splat3(0.0f);
+    const V3 ejt = L3(d.wk_jt[cur][wi]);
+    return mk3(d.wk_jn[cur][wi], dot3(ejt, L3(d.cs_t1[c])), dot3(ejt, L3(d.cs_t2[c])));
+}
+// Appends the solved contacts of this tick (island-list order) with their converged impulses,
+// the normal impulse before the shock pass and the friction accumulators (solver.rs:917-926),
+// less the contact's previous-frame warm start.
+__global__ void k_log_contacts(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     const uint32_t base = d.log_n[0];
@@ -684,7 +694,8 @@ __global__ void k_log_contacts(Dev d) {
         const size_t o = (size_t)base + r;
         for (int k = 0; k < 12; k++) d.log_feat[o * 12 + k] = d.feat[(size_t)c * 12 + k];
         const float2 t = d.jt[c];
-        d.log_targ[o * 3] = d.jn_pre[c]; d.log_targ[o * 3 + 1] = t.x; d.log_targ[o * 3 + 2] = t.y;
+        const V3 b = warm_base(d, c, tp.cur);
+        d.log_targ[o * 3] = d.jn_pre[c] - b.x; d.log_targ[o * 3 + 1] = t.x - b.y; d.log_targ[o * 3 + 2] = t.y - b.z;
     }
 }
 __global__ void k_log_advance(Dev d) {
@@ -1079,22 +1090,38 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         build_slots(d, I, d.isl_v2, d.ch_pa, d.ch_pb, exb, sb);
         if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_ORDER);
         // warm-start application (solver.rs:749-760): the previous frame's impulses (reference),
-        // nothing (zero), or the model's prediction for every solved row (learned)
+        // nothing (zero), or the previous frame's plus the model's correction where the contact
+        // passed the model's gate (learned; other contacts take the reference's warm start, and a
+        // zero correction leaves every operation of the reference's)
         if (tp.ws_mode == 2) run_sweep(I, sb, [&](uint32_t c) {
+            const int wi = d.warm_idx[c];
+            const float4 pr = d.pred[c];
+            const bool use = pr.w != 0.0f;
+            if (!use && wi < 0) return;
             uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
             V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
             float2 inv = d.cs_inv[c];
-            const float4 pr = d.pred[c];
-            float pjn = fmaxf(pr.x, 0.0f);
-            if (!(pjn == pjn)) pjn = 0.0f;
+            float jn = 0.0f, x1 = 0.0f, x2 = 0.0f;
+            if (wi >= 0) {
+                jn = d.wk_jn[cur][wi];
+                V3 ejt = L3(d.wk_jt[cur][wi]);
+                x1 = dot3(ejt, t1); x2 = dot3(ejt, t2);
+            }
+            if (use) {
+                if (pr.x != 0.0f) jn = fmaxf(jn + pr.x, 0.0f);
+                if (pr.y != 0.0f) x1 += pr.y;
+                if (pr.z != 0.0f) x2 += pr.z;
+            }
             V3 v = rel_vel(d, ai, bi, ra, rb);
             float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
             float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
-            V3 t = clamp_len_max(mk3(pr.y, pr.z, 0.0f), mu * pjn);
-            if (!fin3(t)) t = splat3(0.0f);
-            d.jn[c] = pjn;
+            V3 t = clamp_len_max(mk3(x1, x2, 0.0f), mu * jn);
+            d.jn[c] = jn;
             d.jt[c] = make_float2(t.x, t.y);
-            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, n3 * pjn + (t1 * t.x + t2 * t.y));
+            if (wi < 0 && jn == 0.0f && t.x == 0.0f && t.y == 0.0f) return;
+            V3 imp = n3 * jn;
+            imp = imp + (t1 * t.x + t2 * t.y);
+            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
         });
         else if (tp.ws_mode == 0) run_sweep(I, sb, [&](uint32_t c) {
             int wi = d.warm_idx[c];
@@ -1123,6 +1150,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
         __shared__ uint32_t resid_bits;
         uint32_t used = 0;
+        bool conv = false;
         for (uint32_t it = 0; it < tp.iv; it++) {
             if (threadIdx.x == 0) resid_bits = 0;
             __syncthreads();
@@ -1173,9 +1201,12 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             const float rmax = __uint_as_float(resid_bits);
             __syncthreads();
             used++;
-            if (tp.resid_on && rmax <= tp.resid_tol) break;
+            if (tp.resid_on && rmax <= tp.resid_tol) { conv = true; break; }
         }
-        if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n); }
+        if (threadIdx.x == 0) {
+            atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n);
+            if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); }
+        }
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);
         for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
         __syncthreads();
@@ -1620,14 +1651,17 @@ __global__ void k_tick_done(Dev d) {
     c[CT_DONE] += 1;
     c[CT_SUM_PAIRS] += c[CT_PAIRS]; c[CT_SUM_CANDS] += c[CT_CANDS]; c[CT_SUM_CONTACTS] += c[CT_CONTACTS];
     c[CT_SUM_ISLC] += c[CT_ISLC]; c[CT_SUM_ISLANDS] += c[CT_ISLANDS]; c[CT_SUM_ISLC_GLB] += c[CT_ISLC_GLB];
-    c[CT_SUM_ITERS_MAX] += c[CT_ITERS_MAX]; c[CT_SUM_ITERS_SUM] += c[CT_ITERS_SUM];
+    c[CT_SUM_ITERS_MAX] += c[CT_ITERS_MAX];
+    const uint64_t isum = ((uint64_t)c[CT_SUM_ITERS_SUM_HI] << 32 | c[CT_SUM_ITERS_SUM]) + c[CT_ITERS_SUM];
+    c[CT_SUM_ITERS_SUM] = (uint32_t)isum; c[CT_SUM_ITERS_SUM_HI] = (uint32_t)(isum >> 32);
+    c[CT_SUM_ISL_CONV] += c[CT_ISL_CONV]; c[CT_SUM_ISLC_CONV] += c[CT_ISLC_CONV]; c[CT_SUM_WS_LEARNED] += c[CT_WS_LEARNED];
 }
 __global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) cnt[idx] = v; }
 __global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n) { GRID_STRIDE(i, n) dst[i] = src[i]; }
 
 // ------------------------------------------------------------------ host-side launch sequence
 extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
-                                    uint32_t cap, float *out, hipStream_t st);
+                                    uint32_t cap, float *out, uint32_t *n_trusted, hipStream_t st);
 static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t max_grid = 8192) {
     uint32_t g = (n + block - 1) / block;
     if (g < 1) g = 1;
@@ -1750,10 +1784,14 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_MAX, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_SUM, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISL_CONV, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_CONV, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WS_LEARNED, 0u);
     hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
     if (tp.ws_mode == 2 || tp.log_on) hipLaunchKernelGGL(k_features, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     if (tp.ws_mode == 2)
-        rp_launch_mlp_infer(d.mlp.p, d.mlp.w1p, d.mlp.w2p, d.mlp.w3p, d.feat, d.cnt + CT_CONTACTS, d.cap_contacts, (float *)d.pred, st);
+        rp_launch_mlp_infer(d.mlp.p, d.mlp.w1p, d.mlp.w2p, d.mlp.w3p, d.feat, d.cnt + CT_CONTACTS, d.cap_contacts, (float *)d.pred,
+                            d.cnt + CT_WS_LEARNED, st);
     mark(RP_ST_ORDERS);
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
     build_shock_order(d, tp, body_bits, st);
@@ -1765,7 +1803,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     mark(RP_ST_VEL_HUGE);
     hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
     if (tp.log_on) {
-        hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+        hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, d);
     }
     mark(RP_ST_WARM_STORE);

@@ -88,7 +88,8 @@ class Profile(C.Structure):
     _fields_ = [("ticks", C.c_uint32), ("n_stages", C.c_uint32), ("stage_ms", C.c_double * 24), ("pairs", C.c_double),
                 ("terrain_candidates", C.c_double), ("contacts", C.c_double), ("island_contacts", C.c_double), ("islands", C.c_double),
                 ("island_contacts_global", C.c_double), ("pgs_iters_max", C.c_double), ("pgs_iters_sum", C.c_double),
-                ("sweep_phase", C.c_double * 11)]
+                ("sweep_phase", C.c_double * 11), ("islands_converged", C.c_double), ("island_contacts_converged", C.c_double),
+                ("warm_start_learned", C.c_double)]
 
 
 class MlpTrainConfig(C.Structure):
@@ -214,7 +215,8 @@ def _profile_dicts(pr):
     stages = {lib().rphys_profile_stage_name(k).decode(): pr.stage_ms[k] for k in range(pr.n_stages)}
     counters = dict(ticks=pr.ticks, pairs=pr.pairs, terrain_candidates=pr.terrain_candidates, contacts=pr.contacts,
                     island_contacts=pr.island_contacts, islands=pr.islands, island_contacts_global=pr.island_contacts_global,
-                    pgs_iters_max=pr.pgs_iters_max, pgs_iters_sum=pr.pgs_iters_sum)
+                    pgs_iters_max=pr.pgs_iters_max, pgs_iters_sum=pr.pgs_iters_sum, islands_converged=pr.islands_converged,
+                    island_contacts_converged=pr.island_contacts_converged, warm_start_learned=pr.warm_start_learned)
     names = ("vel_order", "warm", "pgs", "vel_shock_order", "vel_shock", "ballistic", "projection", "proj_shock_order",
              "proj_shock", "levels_list", "levels_shock")
     counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}

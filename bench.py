@@ -166,11 +166,11 @@ def batch_bench(a):
     iv, ip = worlds[0].get("iters", (10, 20))
     wl = {"workload": "batch%d_rollback8" % a.worlds, "worlds": a.worlds, "bodies": nb,
           "bodies_per_world": "round(exp(U(ln 32, ln 2048))), mixed shapes over per-world fBm terrain",
-          "rollback_window": 8, "dt": "1/60", "vel_iters": iv, "pos_iters": ip}
+          "rollback_window": 8, "dt": "1/60", "vel_iters": iv, "pos_iters": ip, "graphs": not a.no_graph}
     print("bench: %s, %d bodies, built in %.1f s; warmup %d ticks" % (wl["workload"], nb, time.perf_counter() - t0, a.warmup),
           file=sys.stderr, flush=True)
     if a.warmup > 0:
-        b.step_n(a.warmup, dt, graph=True)
+        b.step_n(a.warmup, dt, graph=not a.no_graph)
     b.hashes()
     b.snapshot(1)  # start of the timed windows (both passes below start here)
     windows = max(1, a.steps // 16)
@@ -179,9 +179,9 @@ def batch_bench(a):
     for k in range(windows):
         t1 = time.perf_counter()
         b.snapshot(0)
-        b.step_n(8, dt, graph=True)
+        b.step_n(8, dt, graph=not a.no_graph)
         b.restore(0)
-        b.step_n(8, dt, graph=True)
+        b.step_n(8, dt, graph=not a.no_graph)
         el += time.perf_counter() - t1
         if (k + 1) % 5 == 0 or k + 1 == windows:
             print("bench: %d/%d windows, %.2f s" % (k + 1, windows, el), file=sys.stderr, flush=True)
@@ -201,10 +201,10 @@ def batch_bench(a):
     print("bench: profiled pass %.2f s" % el_prof, file=sys.stderr, flush=True)
     # rollback check (untimed hashes)
     b.snapshot(0)
-    b.step_n(8, dt, graph=True)
+    b.step_n(8, dt, graph=not a.no_graph)
     h1 = b.hashes()
     b.restore(0)
-    b.step_n(8, dt, graph=True)
+    b.step_n(8, dt, graph=not a.no_graph)
     replay_equal = h1 == b.hashes()
     if a.stages:
         print_stages(stages, ctr)
@@ -262,12 +262,12 @@ def learned_bench(a):
     lat = [scenes.box_lattice(seed=scenes.seed_for(4, w)) for w in range(16)]
     lat += [scenes.batch_world(w, seed_base_k=5) for w in range(64)]
     tb = rphys.Batch([rphys.World(s_) for s_ in lat])
-    tb.step_n(120, graph=True)  # the lattices collapse, the mixed shapes land (fixed iterations)
+    tb.step_n(120, graph=not a.no_graph)  # the lattices collapse, the mixed shapes land (fixed iterations)
     tb.set_residual_mode(True, tol, max_it)
     tb.set_contact_log(a.train_rows + 200000)
     rows = 0
     while rows < a.train_rows:
-        tb.step_n(10, graph=True)
+        tb.step_n(10, graph=not a.no_graph)
         rows = tb.contact_log_rows()[0]
         print("bench: training data %d rows, %.1f s" % (rows, time.perf_counter() - t0), file=sys.stderr, flush=True)
     t_data = time.perf_counter() - t0
@@ -294,7 +294,7 @@ def learned_bench(a):
 
     # in distribution: 16 held-out seeds of the lattice
     lb = rphys.Batch([rphys.World(scenes.box_lattice(seed=scenes.seed_for(4, 100 + w))) for w in range(16)])
-    lb.step_n(90, graph=True)
+    lb.step_n(90, graph=not a.no_graph)
     lb.snapshot(1)
     lb.set_residual_mode(True, tol, max_it)
     lattice = {}
@@ -312,7 +312,7 @@ def learned_bench(a):
     nb = b.total_bodies()
     dt = float(worlds[0].get("dt", 1.0 / 60.0))
     for k in range(0, a.warmup, 50):  # warm up with the fixed iterations
-        b.step_n(min(50, a.warmup - k), dt, graph=True)
+        b.step_n(min(50, a.warmup - k), dt, graph=not a.no_graph)
         b.hashes()
         print("bench: batched warmup %d/%d ticks" % (min(k + 50, a.warmup), a.warmup), file=sys.stderr, flush=True)
     b.snapshot(1)
@@ -329,7 +329,11 @@ def learned_bench(a):
 
     w = rphys.World(scenes.mixed_shapes())
     for k in range(0, a.warmup, 50):
-        w.step_n(min(50, a.warmup - k))
+        if a.no_graph:
+            for _ in range(min(50, a.warmup - k)):
+                w.step()
+        else:
+            w.step_n(min(50, a.warmup - k))
         w.hash()
         print("bench: plain warmup %d/%d ticks" % (min(k + 50, a.warmup), a.warmup), file=sys.stderr, flush=True)
     w.set_residual_mode(True, tol, max_it)
@@ -360,6 +364,8 @@ def main():
     ap.add_argument("--worlds", type=int, default=1024, help="batch workload: number of worlds")
     ap.add_argument("--cpu-ticks", type=int, default=3, help="oracle ticks sampled per 100-tick chunk")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle sample (profiling runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="step on the stream instead of graph-captured ticks (profiling runs: rocprofv3 kernel tracing of graph launches crashed the profiled process)")
     ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (bandwidth reference, counter calibration)")

@@ -1151,57 +1151,62 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         __shared__ uint32_t resid_bits;
         uint32_t used = 0;
         bool conv = false;
-        for (uint32_t it = 0; it < tp.iv; it++) {
-            if (threadIdx.x == 0) resid_bits = 0;
-            __syncthreads();
-            run_sweep(I, sb,
-                [&](uint32_t c, uint32_t slot) {
-                    PgsIn in;
-                    in.x = fetch_row(d, slot);
-                    in.jn = d.jn[c];
-                    in.jt = d.jt[c];
-                    return in;
-                },
-                [&](uint32_t c, const PgsIn &in) {
-                    const Row &x = in.x;
-                    Vel2 v;
-                    v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
-                    if (x.bi != RP_NONE) { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
-                    const Vel2 v0 = v;
-                    float vrn = dot3(rel_vel2(x, v), x.n);
-                    float jraw = (x.m4.w - vrn) * x.m4.x;
-                    float old = in.jn;
-                    float nw = fmaxf(old + jraw, 0.0f);
-                    d.jn[c] = nw;
-                    apply_row2(x, v, x.n * (nw - old));
-                    V3 vf = rel_vel2(x, v);
-                    float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
-                    float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
-                    V3 ot = mk3(in.jt.x, in.jt.y, 0.0f);
-                    V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
-                    d.jt[c] = make_float2(nt.x, nt.y);
-                    V3 dd = nt - ot;
-                    apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
-                    if (tp.resid_on) {  // the oracle's row residual, same operations
-                        float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
-                        float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
-                        float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
-                        atomicMax(&resid_bits, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))));
-                    }
-                    // store what changed (a body the row does not write keeps its bits)
-                    if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
-                    if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
-                    if (x.bi != RP_NONE) {
-                        if (!same_bits(v.lb, v0.lb)) d.lin[x.bi] = S3(v.lb);
-                        if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
-                    }
-                });
-            // every thread takes the sweep's residual before thread 0 may clear it for the next
-            // sweep, so all threads leave the loop at the same sweep
-            const float rmax = __uint_as_float(resid_bits);
-            __syncthreads();
-            used++;
-            if (tp.resid_on && rmax <= tp.resid_tol) { conv = true; break; }
+        auto pgs_prep = [&](uint32_t c, uint32_t slot) {
+            PgsIn in;
+            in.x = fetch_row(d, slot);
+            in.jn = d.jn[c];
+            in.jt = d.jt[c];
+            return in;
+        };
+        auto pgs_row = [&](uint32_t c, const PgsIn &in) {
+            const Row &x = in.x;
+            Vel2 v;
+            v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
+            if (x.bi != RP_NONE) { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
+            const Vel2 v0 = v;
+            float vrn = dot3(rel_vel2(x, v), x.n);
+            float jraw = (x.m4.w - vrn) * x.m4.x;
+            float old = in.jn;
+            float nw = fmaxf(old + jraw, 0.0f);
+            d.jn[c] = nw;
+            apply_row2(x, v, x.n * (nw - old));
+            V3 vf = rel_vel2(x, v);
+            float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
+            float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
+            V3 ot = mk3(in.jt.x, in.jt.y, 0.0f);
+            V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
+            d.jt[c] = make_float2(nt.x, nt.y);
+            V3 dd = nt - ot;
+            apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
+            if (tp.resid_on) {  // the oracle's row residual, same operations
+                float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
+                float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
+                float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
+                atomicMax(&resid_bits, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))));
+            }
+            // store what changed (a body the row does not write keeps its bits)
+            if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
+            if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
+            if (x.bi != RP_NONE) {
+                if (!same_bits(v.lb, v0.lb)) d.lin[x.bi] = S3(v.lb);
+                if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
+            }
+        };
+        if (!tp.resid_on) {
+            for (uint32_t it = 0; it < tp.iv; it++) run_sweep(I, sb, pgs_prep, pgs_row);
+            used = tp.iv;
+        } else {
+            for (uint32_t it = 0; it < tp.iv; it++) {
+                if (threadIdx.x == 0) resid_bits = 0;
+                __syncthreads();
+                run_sweep(I, sb, pgs_prep, pgs_row);
+                // every thread takes the sweep's residual before thread 0 may clear it for the
+                // next sweep, so all threads leave the loop at the same sweep
+                const float rmax = __uint_as_float(resid_bits);
+                __syncthreads();
+                used++;
+                if (rmax <= tp.resid_tol) { conv = true; break; }
+            }
         }
         if (threadIdx.x == 0) {
             atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n);

@@ -172,6 +172,15 @@ def batch_bench(a):
     if a.warmup > 0:
         b.step_n(a.warmup, dt, graph=not a.no_graph)
     b.hashes()
+    # launch shapes of the island sweeps: cached per (workload, size), else a grid search on the
+    # device from this state (untimed; the batch is left where it was)
+    tuning = None
+    if not a.no_graph:
+        t1 = time.perf_counter()
+        entry, cached = b.autotune(wl["workload"], ticks=2, dt=dt)
+        tuning = dict(entry, cached=cached, search_s=time.perf_counter() - t1)
+        print("bench: tuning %s (%s, %.1f s)" % (b.get_tuning(), "cached" if cached else "searched", tuning["search_s"]), file=sys.stderr, flush=True)
+    wl["tuning"] = tuning
     b.snapshot(1)  # start of the timed windows (both passes below start here)
     windows = max(1, a.steps // 16)
     # timed: graph-captured ticks, each window snapshot -> 8 ticks -> restore -> the same 8 ticks

@@ -242,6 +242,22 @@ RPhysResult rphys_batch_get_hashes(const RPhysBatch *batch, uint8_t *out);
 RPhysResult rphys_batch_get_state(const RPhysBatch *batch, uint32_t world_index, float *pos3, float *rot4, float *lin3, float *ang3);
 RPhysResult rphys_batch_snapshot(RPhysBatch *batch, uint32_t slot);   /* device-side copy of all worlds' state */
 RPhysResult rphys_batch_restore(RPhysBatch *batch, uint32_t slot);
+/* Launch-shape tuning (build addition; the reference has no GPU launches to tune).  The island
+ * sweeps run one workgroup per island, taken grid-stride, in three tiers by island size; the
+ * grids only change speed, never results.  rphys_batch_autotune grid-searches grid_small in
+ * {1024..32768}, grid_large in {128..2048}, grid_huge in {64..256} (powers of two): each candidate steps the same
+ * `ticks` graph-captured ticks from a device snapshot, per-world hashes must agree across all
+ * candidates (else RPHYS_ERR_INTERNAL), the fastest is installed and the batch is left at the
+ * snapshot.  Caching results per (config, size) is the caller's (rphys.py keeps a JSON file). */
+typedef struct RPhysTuning {
+    uint32_t size_bytes;  /* sizeof(RPhysTuning) */
+    uint32_t grid_small;  /* islands of <= 4,096 contacts: 256-thread workgroups (default 2048) */
+    uint32_t grid_large;  /* <= 131,072 contacts: 1024 threads, LDS flags (default 256) */
+    uint32_t grid_huge;   /* larger islands: 1024 threads, HBM flags (default 64) */
+} RPhysTuning;
+RPhysResult rphys_batch_set_tuning(RPhysBatch *batch, const RPhysTuning *tuning);
+RPhysResult rphys_batch_get_tuning(const RPhysBatch *batch, RPhysTuning *tuning);
+RPhysResult rphys_batch_autotune(RPhysBatch *batch, float dt, uint32_t ticks, RPhysTuning *best, double *best_ms_per_tick);
 RPhysResult rphys_batch_total_bodies(const RPhysBatch *batch, uint32_t *out);
 /* Record the next tick of the batch into a HIP graph and replay it for n ticks (same results as
  * rphys_batch_step_n; launch overhead paid once). */

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -26,7 +27,7 @@ struct HashDev {
     uint8_t *digest;
     uint32_t cap_chunks;
 };
-extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st, hipEvent_t *ev);
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st);
@@ -171,7 +172,8 @@ struct Scene {
     DeviceBuffers buf;
     hipStream_t st = nullptr;
     uint32_t cur = 0;
-    uint32_t island_grid = 2048;
+    uint32_t grids[3] = {2048, 256, 64};  // island-sweep workgroups per tier (rphys_batch_set_tuning)
+    uint32_t graph_grids[3] = {0, 0, 0};  // the grids the graphs were captured with
     uint32_t max_extra = 0;
     std::vector<uint32_t> body_off, col_off;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -401,7 +403,7 @@ static bool capture_graph(Scene &S, const TickParams &tp0, uint32_t parity) {
     tp.cur = parity;
     hipGraph_t g = nullptr;
     if (!hip_ok(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
-    hipError_t le = rp_launch_tick(S.d, tp, S.island_grid, S.st, nullptr);
+    hipError_t le = rp_launch_tick(S.d, tp, S.grids, S.st, nullptr);
     if (!hip_ok(hipStreamEndCapture(S.st, &g), "hipStreamEndCapture")) return false;
     if (!hip_ok(le, "kernel launch (capture)")) { (void)hipGraphDestroy(g); return false; }
     bool ok = hip_ok(hipGraphInstantiate(&S.graph[parity], g, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -456,10 +458,12 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         if (use_graph) {
             TickParams key = tp;
             key.cur = 0;
-            if (!S->graph[0] || !S->graph[1] || std::memcmp(&key, &S->graph_tp, sizeof key) != 0) {
+            if (!S->graph[0] || !S->graph[1] || std::memcmp(&key, &S->graph_tp, sizeof key) != 0 ||
+                std::memcmp(S->grids, S->graph_grids, sizeof S->grids) != 0) {
                 for (int i = 0; i < 2; i++) if (S->graph[i]) { (void)hipGraphExecDestroy(S->graph[i]); S->graph[i] = nullptr; }
                 if (!capture_graph(*S, tp, 0) || !capture_graph(*S, tp, 1)) return false;
                 S->graph_tp = key;
+                std::memcpy(S->graph_grids, S->grids, sizeof S->grids);
             }
             for (uint32_t k = 0; k < remaining; k++)
                 if (!hip_ok(hipGraphLaunch(S->graph[(S->cur + k) & 1u], S->st), "hipGraphLaunch")) return false;
@@ -467,7 +471,7 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
             uint32_t batch = prof ? std::min(remaining, ProfileAcc::PROFILE_BATCH) : remaining;
             for (uint32_t k = 0; k < batch; k++) {
                 tp.cur = (S->cur + k) & 1u;
-                if (!hip_ok(rp_launch_tick(S->d, tp, S->island_grid, S->st, prof ? prof->ev[k] : nullptr), "kernel launch")) return false;
+                if (!hip_ok(rp_launch_tick(S->d, tp, S->grids, S->st, prof ? prof->ev[k] : nullptr), "kernel launch")) return false;
             }
         }
         uint32_t cnt[CT_COUNT];
@@ -498,7 +502,8 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         uint32_t cur = S->cur;
         Scene *N = build_scene(ws, caps, cur);
         if (!N) return false;
-        N->island_grid = S->island_grid;
+        std::memcpy(N->grids, S->grids, sizeof S->grids);
+        N->slots = std::move(S->slots);  // rollback slots hold body state and a warm cache no larger than the new capacity
         N->d.log_feat = S->d.log_feat; N->d.log_targ = S->d.log_targ; N->d.log_n = S->d.log_n; N->d.log_cap = S->d.log_cap;
         delete S;
         S = N;
@@ -1452,11 +1457,10 @@ RPhysResult rphys_batch_get_state(const RPhysBatch *cb, uint32_t wi, float *pos3
     GUARD_END
 }
 
-// Device-side rollback slots: body state, sleep state and the warm cache of every world.
-RPhysResult rphys_batch_snapshot(RPhysBatch *b, uint32_t slot) {
-    GUARD_BEGIN
-    if (!b) { set_err("rphys_batch_snapshot: batch is null"); return RPHYS_ERR_NULL; }
-    if (slot > 63) return bad_arg("rphys_batch_snapshot", "slot must be < 64");
+// Device-side rollback slots: body state, sleep state and the warm cache of every world.  Slots
+// 0..63 are the caller's; slot 64 is the autotuner's.
+static const uint32_t TUNE_SLOT = 64;
+static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
     Scene &S = *b->scene;
     if (S.slots.size() <= slot) S.slots.resize(slot + 1);
     const Dev &d = S.d;
@@ -1485,11 +1489,15 @@ RPhysResult rphys_batch_snapshot(RPhysBatch *b, uint32_t slot) {
     sl.ticks.clear();
     for (auto &w : b->worlds) sl.ticks.push_back(w.tick);
     return RPHYS_OK;
+}
+RPhysResult rphys_batch_snapshot(RPhysBatch *b, uint32_t slot) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_snapshot: batch is null"); return RPHYS_ERR_NULL; }
+    if (slot > 63) return bad_arg("rphys_batch_snapshot", "slot must be < 64");
+    return batch_snapshot(b, slot);
     GUARD_END
 }
-RPhysResult rphys_batch_restore(RPhysBatch *b, uint32_t slot) {
-    GUARD_BEGIN
-    if (!b) { set_err("rphys_batch_restore: batch is null"); return RPHYS_ERR_NULL; }
+static RPhysResult batch_restore(RPhysBatch *b, uint32_t slot) {
     Scene &S = *b->scene;
     if (slot >= S.slots.size() || !S.slots[slot] || !S.slots[slot]->valid) return bad_arg("rphys_batch_restore", "empty slot");
     Slot &sl = *S.slots[slot];
@@ -1507,6 +1515,73 @@ RPhysResult rphys_batch_restore(RPhysBatch *b, uint32_t slot) {
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "restore")) return RPHYS_ERR_INTERNAL;
     S.cur = sl.cur;
     for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];
+    return RPHYS_OK;
+}
+RPhysResult rphys_batch_restore(RPhysBatch *b, uint32_t slot) {
+    GUARD_BEGIN
+    if (!b) { set_err("rphys_batch_restore: batch is null"); return RPHYS_ERR_NULL; }
+    if (slot > 63) return bad_arg("rphys_batch_restore", "slot must be < 64");
+    return batch_restore(b, slot);
+    GUARD_END
+}
+
+// ------------------------------------------------------------------ launch-shape tuning
+RPhysResult rphys_batch_set_tuning(RPhysBatch *b, const RPhysTuning *t) {
+    GUARD_BEGIN
+    if (!b || !t) { set_err("rphys_batch_set_tuning: null arg"); return RPHYS_ERR_NULL; }
+    if (t->size_bytes < sizeof(RPhysTuning)) return bad_arg("rphys_batch_set_tuning", "size_bytes too small");
+    const uint32_t g[3] = {t->grid_small, t->grid_large, t->grid_huge};
+    for (uint32_t k = 0; k < 3; k++)
+        if (g[k] == 0 || g[k] > 65535) return bad_arg("rphys_batch_set_tuning", "grids must be in 1..65535");
+    std::memcpy(b->scene->grids, g, sizeof g);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_batch_get_tuning(const RPhysBatch *b, RPhysTuning *t) {
+    GUARD_BEGIN
+    if (!b || !t) { set_err("rphys_batch_get_tuning: null arg"); return RPHYS_ERR_NULL; }
+    t->size_bytes = sizeof(RPhysTuning);
+    t->grid_small = b->scene->grids[0]; t->grid_large = b->scene->grids[1]; t->grid_huge = b->scene->grids[2];
+    return RPHYS_OK;
+    GUARD_END
+}
+// Grid search over the island-sweep grids: every candidate steps the same `ticks` graph-captured
+// ticks from one device snapshot (once to capture, once timed); the fastest is installed and the
+// batch is left at the snapshot.  The per-world hashes after the ticks must agree across all
+// candidates, or the search fails.
+RPhysResult rphys_batch_autotune(RPhysBatch *b, float dt, uint32_t ticks, RPhysTuning *best, double *best_ms_per_tick) {
+    GUARD_BEGIN
+    if (!b || !best || !best_ms_per_tick) { set_err("rphys_batch_autotune: null arg"); return RPHYS_ERR_NULL; }
+    if (!hfin(dt) || dt < 0.0f) return bad_arg("rphys_batch_autotune", "dt must be finite and >= 0");
+    if (ticks == 0 || ticks > 64) return bad_arg("rphys_batch_autotune", "ticks must be in 1..64");
+    static const uint32_t G0[] = {1024, 2048, 4096, 8192, 16384, 32768}, G1[] = {128, 256, 512, 1024, 2048}, G2[] = {64, 128, 256};
+    RPhysResult r = batch_snapshot(b, TUNE_SLOT);
+    if (r != RPHYS_OK) return r;
+    const size_t nw = b->worlds.size();
+    std::vector<uint8_t> ref, got(32 * nw);
+    double best_s = 0.0;
+    uint32_t bg[3] = {0, 0, 0};
+    for (uint32_t g0 : G0)
+        for (uint32_t g1 : G1)
+            for (uint32_t g2 : G2) {
+                const uint32_t g[3] = {g0, g1, g2};
+                std::memcpy(b->scene->grids, g, sizeof g);
+                if ((r = batch_restore(b, TUNE_SLOT)) != RPHYS_OK || (r = batch_step(b, dt, ticks, true)) != RPHYS_OK ||
+                    (r = batch_restore(b, TUNE_SLOT)) != RPHYS_OK)
+                    return r;
+                const auto t0 = std::chrono::steady_clock::now();
+                if ((r = batch_step(b, dt, ticks, true)) != RPHYS_OK) return r;
+                const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (!device_hash(*b->scene, got.data(), b->scene->d.nw)) return RPHYS_ERR_INTERNAL;
+                if (ref.empty()) ref = got;
+                else if (got != ref) { set_err("rphys_batch_autotune: a launch shape changed the results"); return RPHYS_ERR_INTERNAL; }
+                if (bg[0] == 0 || sec < best_s) { best_s = sec; std::memcpy(bg, g, sizeof g); }
+            }
+    std::memcpy(b->scene->grids, bg, sizeof bg);
+    if ((r = batch_restore(b, TUNE_SLOT)) != RPHYS_OK) return r;
+    best->size_bytes = sizeof(RPhysTuning);
+    best->grid_small = bg[0]; best->grid_large = bg[1]; best->grid_huge = bg[2];
+    *best_ms_per_tick = 1000.0 * best_s / ticks;
     return RPHYS_OK;
     GUARD_END
 }

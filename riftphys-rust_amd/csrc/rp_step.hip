@@ -1743,7 +1743,9 @@ static void build_shock_order(const Dev &d, const TickParams &tp, uint32_t body_
 // Enqueues one tick; returns the launch status (hipGetLastError after the last launch).  When ev
 // is given, ev[k] is recorded on st at stage boundary k (RP_NMARKS events; stage k spans
 // ev[k]..ev[k+1]).
-extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_t island_grid, hipStream_t st, hipEvent_t *ev) {
+// grids: workgroups of the island sweeps per tier (<= 4,096 contacts, <= 131,072, larger); any
+// positive counts give the same results (islands are taken grid-stride).
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev) {
     const uint32_t B = 256;
     auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev[k], st); };
     mark(RP_ST_PRE);
@@ -1800,7 +1802,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, uint32_
     mark(RP_ST_ORDERS);
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
     build_shock_order(d, tp, body_bits, st);
-    const uint32_t g0 = island_grid, g1 = island_grid < 256u ? island_grid : 256u, g2 = 64u;
+    const uint32_t g0 = grids[0], g1 = grids[1], g2 = grids[2];
     mark(RP_ST_VEL_SMALL);
     hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
     mark(RP_ST_VEL_LARGE);

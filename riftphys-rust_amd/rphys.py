@@ -7,7 +7,10 @@ The library must be built (riftphys-rust_amd/csrc/Makefile); there is no fallbac
 runs the device step.
 """
 import ctypes as C
+import json
+import math
 import os
+import tempfile
 
 import numpy as np
 
@@ -96,6 +99,10 @@ class MlpTrainConfig(C.Structure):
     _fields_ = [("size_bytes", C.c_uint32), ("epochs", C.c_uint32), ("lr", C.c_float), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("size_bytes", C.c_uint32), ("grid_small", C.c_uint32), ("grid_large", C.c_uint32), ("grid_huge", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every export in include/riftphys_c.h
 P = C.c_void_p
 U32, U64, F32 = C.c_uint32, C.c_uint64, C.c_float
@@ -155,6 +162,9 @@ EXPORTS = [
     ("rphys_batch_get_state", U32, [P, U32, PF32, PF32, PF32, PF32]),
     ("rphys_batch_snapshot", U32, [P, U32]),
     ("rphys_batch_restore", U32, [P, U32]),
+    ("rphys_batch_set_tuning", U32, [P, C.POINTER(Tuning)]),
+    ("rphys_batch_get_tuning", U32, [P, C.POINTER(Tuning)]),
+    ("rphys_batch_autotune", U32, [P, F32, U32, C.POINTER(Tuning), C.POINTER(C.c_double)]),
     ("rphys_batch_total_bodies", U32, [P, PU32]),
     ("rphys_batch_step_graph", U32, [P, F32, U32]),
     ("rphys_batch_step_profiled", U32, [P, F32, U32, C.POINTER(Profile)]),
@@ -498,3 +508,64 @@ class Batch:
 
     def restore(self, slot=0):
         check(lib().rphys_batch_restore(self.h, slot))
+
+    def set_tuning(self, grid_small, grid_large, grid_huge):
+        t = Tuning(C.sizeof(Tuning), grid_small, grid_large, grid_huge)
+        check(lib().rphys_batch_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(lib().rphys_batch_get_tuning(self.h, C.byref(t)))
+        return dict(grid_small=t.grid_small, grid_large=t.grid_large, grid_huge=t.grid_huge)
+
+    def autotune(self, config, ticks=2, dt=1.0 / 60.0, cache=None):
+        """Installs the launch shape cached for (config, body-count bucket), or grid-searches one on
+        the device (rphys_batch_autotune: same ticks from a snapshot per candidate, results checked
+        equal) and caches it.  Returns (entry, found_in_cache); the batch state is unchanged."""
+        path = cache or default_tuning_cache()
+        key = tuning_key(config, self.total_bodies())
+        data = load_tuning_cache(path)
+        if key in data:
+            e = data[key]
+            self.set_tuning(e["grid_small"], e["grid_large"], e["grid_huge"])
+            return e, True
+        t, ms = Tuning(), C.c_double(0.0)
+        check(lib().rphys_batch_autotune(self.h, dt, ticks, C.byref(t), C.byref(ms)))
+        e = dict(grid_small=t.grid_small, grid_large=t.grid_large, grid_huge=t.grid_huge, ms_per_tick=ms.value, ticks=ticks)
+        data[key] = e
+        store_tuning_cache(path, data)
+        return e, False
+
+
+def tuning_key(config, bodies):
+    """Cache key of a launch shape: the workload's name and its body count's power-of-two bucket."""
+    return "%s/n2^%d" % (config, int(math.floor(math.log2(max(1, int(bodies))))))
+
+
+def default_tuning_cache():
+    base = os.environ.get("RPHYS_TUNING_CACHE")
+    if base:
+        return base
+    return os.path.join(os.path.expanduser("~"), ".cache", "rphys_amd", "tuning.json")
+
+
+def load_tuning_cache(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def store_tuning_cache(path, data):
+    """Writes the cache atomically (temporary file, then rename); an unwritable location is skipped."""
+    try:
+        d = os.path.dirname(path) or "."
+        os.makedirs(d, exist_ok=True)
+        fd, tmp = tempfile.mkstemp(dir=d, prefix=".tuning", suffix=".json")
+        with os.fdopen(fd, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass

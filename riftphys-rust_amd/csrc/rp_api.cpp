@@ -27,7 +27,8 @@ struct HashDev {
     uint8_t *digest;
     uint32_t cap_chunks;
 };
-extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev);
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev,
+                                     const TierStreams *ts);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st);
@@ -179,8 +180,11 @@ struct Scene {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     TickParams graph_tp{};  // parameters the graphs were captured with (cur aside)
     std::vector<std::unique_ptr<Slot>> slots;
+    TierStreams ts{{nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // island-sweep tiers run concurrently
     ~Scene() {
         for (int i = 0; i < 2; i++) if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
+        for (int i = 0; i < 3; i++) if (ts.ev[i]) (void)hipEventDestroy(ts.ev[i]);
+        for (int i = 0; i < 2; i++) if (ts.side[i]) (void)hipStreamDestroy(ts.side[i]);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -280,6 +284,10 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(h.stream, (size_t)h.cap_chunks * 1024) && B.alloc(h.cv, 8 * (size_t)h.cap_chunks) && B.alloc(h.cv2, 8 * (size_t)h.cap_chunks) && B.alloc(h.digest, 32 * (size_t)nw);
     if (!ok) { set_err("device allocation failed (scene too large for the configured capacities)"); return nullptr; }
     if (!hip_ok(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking), "hipStreamCreate")) return nullptr;
+    for (int i = 0; i < 2; i++)
+        if (!hip_ok(hipStreamCreateWithFlags(&S->ts.side[i], hipStreamNonBlocking), "hipStreamCreate")) return nullptr;
+    for (int i = 0; i < 3; i++)
+        if (!hip_ok(hipEventCreateWithFlags(&S->ts.ev[i], hipEventDisableTiming), "hipEventCreate")) return nullptr;
     // ---- upload
     std::vector<F4> pos, rot, lin, ang, last, dims;
     std::vector<float> invm, iil, heights;
@@ -403,7 +411,7 @@ static bool capture_graph(Scene &S, const TickParams &tp0, uint32_t parity) {
     tp.cur = parity;
     hipGraph_t g = nullptr;
     if (!hip_ok(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
-    hipError_t le = rp_launch_tick(S.d, tp, S.grids, S.st, nullptr);
+    hipError_t le = rp_launch_tick(S.d, tp, S.grids, S.st, nullptr, &S.ts);
     if (!hip_ok(hipStreamEndCapture(S.st, &g), "hipStreamEndCapture")) return false;
     if (!hip_ok(le, "kernel launch (capture)")) { (void)hipGraphDestroy(g); return false; }
     bool ok = hip_ok(hipGraphInstantiate(&S.graph[parity], g, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -471,7 +479,8 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
             uint32_t batch = prof ? std::min(remaining, ProfileAcc::PROFILE_BATCH) : remaining;
             for (uint32_t k = 0; k < batch; k++) {
                 tp.cur = (S->cur + k) & 1u;
-                if (!hip_ok(rp_launch_tick(S->d, tp, S->grids, S->st, prof ? prof->ev[k] : nullptr), "kernel launch")) return false;
+                if (!hip_ok(rp_launch_tick(S->d, tp, S->grids, S->st, prof ? prof->ev[k] : nullptr, prof ? nullptr : &S->ts), "kernel launch"))
+                    return false;
             }
         }
         uint32_t cnt[CT_COUNT];

@@ -166,6 +166,11 @@ enum {
 
 #define RP_SROW 12u
 
+// Optional side streams for the island sweeps: the three size tiers of one stage solve disjoint
+// islands, so they run concurrently (tier 0 on side[0], tier 1 on the tick's stream, tier 2 on
+// side[1]), forked from and joined back to the tick's stream through ev (no timing).
+struct TierStreams { hipStream_t side[2]; hipEvent_t ev[3]; };
+
 struct TickParams {
     float dt;
     float gx, gy, gz;      // q6-quantised gravity acceleration

@@ -1745,9 +1745,24 @@ static void build_shock_order(const Dev &d, const TickParams &tp, uint32_t body_
 // ev[k]..ev[k+1]).
 // grids: workgroups of the island sweeps per tier (<= 4,096 contacts, <= 131,072, larger); any
 // positive counts give the same results (islands are taken grid-stride).
-extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev) {
+extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev,
+                                     const TierStreams *ts) {
     const uint32_t B = 256;
     auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev[k], st); };
+    // the three tiers of one sweep stage: concurrent with side streams, else in order
+    auto tiers = [&](auto k0, auto k1, auto k2) {
+        if (!ts) { k0(st); k1(st); k2(st); return; }
+        (void)hipEventRecord(ts->ev[0], st);
+        (void)hipStreamWaitEvent(ts->side[0], ts->ev[0], 0);
+        (void)hipStreamWaitEvent(ts->side[1], ts->ev[0], 0);
+        k0(ts->side[0]);
+        k1(st);
+        k2(ts->side[1]);
+        (void)hipEventRecord(ts->ev[1], ts->side[0]);
+        (void)hipEventRecord(ts->ev[2], ts->side[1]);
+        (void)hipStreamWaitEvent(st, ts->ev[1], 0);
+        (void)hipStreamWaitEvent(st, ts->ev[2], 0);
+    };
     mark(RP_ST_PRE);
     uint32_t *scratch_total = d.cnt + CT_SCRATCH;
     hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
@@ -1803,12 +1818,18 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
     build_shock_order(d, tp, body_bits, st);
     const uint32_t g0 = grids[0], g1 = grids[1], g2 = grids[2];
-    mark(RP_ST_VEL_SMALL);
-    hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
-    mark(RP_ST_VEL_LARGE);
-    hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
-    mark(RP_ST_VEL_HUGE);
-    hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    if (ev) {  // profiled: in order, a stage per tier
+        mark(RP_ST_VEL_SMALL);
+        hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+        mark(RP_ST_VEL_LARGE);
+        hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+        mark(RP_ST_VEL_HUGE);
+        hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    } else {
+        tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
+    }
     if (tp.log_on) {
         hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, d);
@@ -1826,25 +1847,31 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
-    hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
+          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
+          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
     mark(RP_ST_INTEGRATE);
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
-    mark(RP_ST_PROJ_SMALL);
-    hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
-    mark(RP_ST_PROJ_LARGE);
-    hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
-    mark(RP_ST_PROJ_HUGE);
-    hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    if (ev) {
+        mark(RP_ST_PROJ_SMALL);
+        hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+        mark(RP_ST_PROJ_LARGE);
+        hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+        mark(RP_ST_PROJ_HUGE);
+        hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    } else {
+        tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
+    }
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses
     mark(RP_ST_PROJ_SHOCK);
-    hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
-    hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+    tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
+          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
+          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
     mark(RP_ST_S6);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);

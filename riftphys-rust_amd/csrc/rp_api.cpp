@@ -173,8 +173,10 @@ struct Scene {
     DeviceBuffers buf;
     hipStream_t st = nullptr;
     uint32_t cur = 0;
-    uint32_t grids[3] = {2048, 256, 64};  // island-sweep workgroups per tier (rphys_batch_set_tuning)
-    uint32_t graph_grids[3] = {0, 0, 0};  // the grids the graphs were captured with
+    // island-sweep workgroups per tier (rphys_batch_set_tuning), then the wide-level launches per
+    // sweep (set from the previous tick's widest island; 0: one-kernel sweeps)
+    uint32_t grids[4] = {2048, 256, 64, 0};
+    uint32_t graph_grids[4] = {0, 0, 0, 0};  // the shape the graphs were captured with
     uint32_t max_extra = 0;
     std::vector<uint32_t> body_off, col_off;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -254,7 +256,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.ch_pa, C) && B.alloc(d.ch_pb, C) && B.alloc(d.sh_ord, C) && B.alloc(d.sh_pa, C) && B.alloc(d.sh_pb, C) && B.alloc(d.g_done, C);
     ok &= B.alloc(d.ch_sa, C) && B.alloc(d.ch_sb, C) && B.alloc(d.sh_sa, C) && B.alloc(d.sh_sb, C) && B.alloc(d.ex_base, C) && B.alloc(d.ex_shock, C);
     ok &= B.alloc(d.bs_pa, C) && B.alloc(d.bs_pb, C) && B.alloc(d.bs_c, C) && B.alloc(d.ss_pa, C) && B.alloc(d.ss_pb, C) && B.alloc(d.ss_c, C);
-    ok &= B.alloc(d.slot_of, C) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
+    ok &= B.alloc(d.slot_of, C) && B.alloc(d.lvl_off, C) && B.alloc(d.isl_w, C + 1) && B.alloc(d.wide_list, C + 1) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
     ok &= B.alloc(d.feat, 12 * C) && B.alloc(d.pred, C);
     {
         float *mp = nullptr;
@@ -487,6 +489,10 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         if (!hip_ok(hipMemcpyAsync(cnt, S->d.cnt, sizeof(cnt), hipMemcpyDeviceToHost, S->st), "counters")) return false;
         if (!hip_ok(hipStreamSynchronize(S->st), "step")) return false;
         uint32_t done = cnt[CT_DONE];
+        // level launches for the next ticks: the last tick's most leading wide levels of one
+        // island, rounded up to a multiple of 8 (fewer graph re-captures); any count is exact
+        const uint32_t wk = cnt[CT_WIDE_MAX];
+        S->grids[3] = wk ? std::min<uint32_t>(RP_WIDE_MAX_LEVELS, (wk + 7u) / 8u * 8u) : 0u;
         S->cur ^= (done & 1u);
         for (auto *w : ws) w->tick += done;
         remaining -= done;

@@ -30,6 +30,7 @@ enum {
     CT_ITERS_MAX, CT_ITERS_SUM,  // PGS sweeps: most run by an island; sum of sweeps x island contacts
     CT_ISL_CONV, CT_ISLC_CONV,   // residual mode: islands (and their contacts) that reached the tolerance
     CT_WS_LEARNED,               // contacts whose learned warm start passed the gate
+    CT_WIDE_N, CT_WIDE_MAX,      // islands with wide levels; most wide levels of one island
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
@@ -103,6 +104,10 @@ struct Dev {
     // rank -> slot scratch
     uint32_t *ex_base, *ex_shock;
     uint32_t *bs_pa, *bs_pb, *bs_c, *ss_pa, *ss_pb, *ss_c, *slot_of;
+    // wide levels: an island's leading levels of >= RP_WIDE_MIN ranks run as GPU-wide launches
+    uint32_t *lvl_off;    // per island segment: start slot of each leading wide level, then their end
+    uint32_t *isl_w;      // per island: number of leading wide levels (0: none)
+    uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
     // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model
@@ -165,6 +170,9 @@ enum {
 };
 
 #define RP_SROW 12u
+
+#define RP_WIDE_MIN 1024u         // ranks of a level that one workgroup cannot take in one round
+#define RP_WIDE_MAX_LEVELS 64u    // leading wide levels recorded per island
 
 // Optional side streams for the island sweeps: the three size tiers of one stage solve disjoint
 // islands, so they run concurrently (tier 0 on side[0], tier 1 on the tick's stream, tier 2 on

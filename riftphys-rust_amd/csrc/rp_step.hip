@@ -883,6 +883,27 @@ static __device__ void run_sweep(Island<FL> &I, const Slots &sl, P prep, R run) 
     }
     __syncthreads();
 }
+// One sweep over slots [j0, n) of an island whose slots below j0 have run already (this sweep's
+// wide levels, launched across the GPU): predecessors below j0 count as done.
+template <class FL, class P, class R>
+static __device__ void run_sweep_from(Island<FL> &I, const Slots &sl, uint32_t j0, P prep, R run) {
+    if (I.token >= FL::max_token) I.begin();
+    const uint32_t tok = ++I.token, s = I.s, n = I.n;
+    uint32_t j = j0 + threadIdx.x, a = RP_NONE, b = RP_NONE, c = 0;
+    decltype(prep(0u, 0u)) x{};
+    bool fresh = true;
+    while (j < n) {
+        if (fresh) { a = sl.pa[s + j]; b = sl.pb[s + j]; c = sl.c[s + j]; x = prep(c, s + j); fresh = false; }
+        bool ready = (a == RP_NONE || a < j0 || I.fl.load(a) >= tok) && (b == RP_NONE || b < j0 || I.fl.load(b) >= tok);
+        if (ready) {
+            run(c, x);
+            I.fl.store(j, tok);
+            j += blockDim.x;
+            fresh = true;
+        }
+    }
+    __syncthreads();
+}
 struct NoRow {};
 template <class FL, class F>
 static __device__ void run_sweep(Island<FL> &I, const Slots &sl, F exec) {
@@ -911,7 +932,7 @@ struct PhaseClock {
 // or nullptr (list order, always valid) should the expansion not place every rank.
 template <class FL>
 static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa, const uint32_t *pb, const uint32_t *sa, const uint32_t *sb,
-                                              uint32_t *ex, unsigned long long *rounds) {
+                                              uint32_t *ex, unsigned long long *rounds, uint32_t *wide_off = nullptr, uint32_t *wide_k = nullptr) {
     __shared__ uint32_t tail;
     const uint32_t s = I.s, n = I.n;
     if (threadIdx.x == 0) tail = 0;
@@ -920,9 +941,17 @@ static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa,
     for (uint32_t r = threadIdx.x; r < n; r += blockDim.x)
         if (pa[s + r] == RP_NONE && pb[s + r] == RP_NONE) ex[s + atomicAdd(&tail, 1u)] = r;
     __syncthreads();
-    uint32_t fs = 0, fe = tail, nr = 0;
+    uint32_t fs = 0, fe = tail, nr = 0, nw = 0;
+    bool leading = wide_off != nullptr;
     __syncthreads();  // every thread has the frontier's end before any appends to the next
     while (fs < fe) {
+        // the island's leading run of wide levels: their start slots, then the run's end
+        if (leading && fe - fs >= RP_WIDE_MIN && nw < RP_WIDE_MAX_LEVELS) {
+            if (threadIdx.x == 0) { wide_off[nw] = fs; wide_off[nw + 1] = fe; }
+            nw++;
+        } else {
+            leading = false;
+        }
         nr++;
         for (uint32_t j = fs + threadIdx.x; j < fe; j += blockDim.x) {
             const uint32_t r = ex[s + j];
@@ -937,6 +966,7 @@ static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa,
     }
     I.begin();
     if (threadIdx.x == 0) atomicAdd(rounds, (unsigned long long)nr);
+    if (wide_k) *wide_k = fe == n ? nw : 0u;  // every thread stores the same count
     return fe == n ? ex : nullptr;
 }
 
@@ -1072,11 +1102,92 @@ static __device__ __forceinline__ auto make_island(const Dev &d, uint8_t *flag_l
     else return Island<GlbFlags>{GlbFlags{d.g_done + s}, s, n, 0};
 }
 
+// PGS row of contact c (solver.rs:839-879): its constant row and accumulators are fetched, before
+// the row's wait, by pgs_fetch; pgs_row keeps the two bodies' velocities in registers between the
+// normal and the friction update and stores only the components that changed.  resid_bits (LDS,
+// may be null) gathers the row residual of residual mode.
+struct PgsIn { Row x; float jn; float2 jt; };
+static __device__ __forceinline__ PgsIn pgs_fetch(const Dev &d, uint32_t c, uint32_t slot) {
+    PgsIn in;
+    in.x = fetch_row(d, slot);
+    in.jn = d.jn[c];
+    in.jt = d.jt[c];
+    return in;
+}
+static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, RP_LDS uint32_t *resid_bits) {
+    const Row &x = in.x;
+    Vel2 v;
+    v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
+    if (x.bi != RP_NONE) { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
+    const Vel2 v0 = v;
+    float vrn = dot3(rel_vel2(x, v), x.n);
+    float jraw = (x.m4.w - vrn) * x.m4.x;
+    float old = in.jn;
+    float nw = fmaxf(old + jraw, 0.0f);
+    d.jn[c] = nw;
+    apply_row2(x, v, x.n * (nw - old));
+    V3 vf = rel_vel2(x, v);
+    float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
+    float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
+    V3 ot = mk3(in.jt.x, in.jt.y, 0.0f);
+    V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
+    d.jt[c] = make_float2(nt.x, nt.y);
+    V3 dd = nt - ot;
+    apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
+    if (resid_bits) {  // the oracle's row residual, same operations
+        float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
+        float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
+        float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
+        __hip_atomic_fetch_max(resid_bits, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    // store what changed (a body the row does not write keeps its bits)
+    if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
+    if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
+    if (x.bi != RP_NONE) {
+        if (!same_bits(v.lb, v0.lb)) d.lin[x.bi] = S3(v.lb);
+        if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
+    }
+}
+
+// The pre-shock impulse snapshot (what the warm cache stores) and the velocity shock pass
+// (solver.rs:976-993) in shock order, for one island.
+template <int TIER, class FL>
+static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> &I, const uint32_t *lst, PhaseClock &pc) {
+    const uint32_t n = I.n;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
+    __syncthreads();
+    // shock pass (solver.rs:976-993) in shock order
+    const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK]) : nullptr;
+    const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
+    build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, ss);
+    if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
+    run_sweep(I, ss, [&](uint32_t c) {
+        uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+        V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
+        float2 inv = d.cs_inv[c];
+        float4 m4 = d.cs_m[c];
+        float inv_a = inv.x, inv_b = inv.y, mn = m4.x;
+        if (inv_a > 0.0f && inv_b > 0.0f) {
+            M3 iia = load_m9(d.iiw + 9 * ai), iib = load_m9(d.iiw + 9 * bi);
+            if (dot3(L3(d.pos[ai]), up) < dot3(L3(d.pos[bi]), up)) { inv_a = 0.0f; iia = mzero(); }
+            else { inv_b = 0.0f; iib = mzero(); }
+            mn = eff_mass(inv_a, inv_b, iia, iib, ra, rb, n3);
+        }
+        float v = dot3(rel_vel(d, ai, bi, ra, rb), n3);
+        float old = d.jn[c];
+        float nw = fmaxf(old + (m4.w - v) * mn, 0.0f);
+        d.jn[c] = nw;
+        apply_cimp(d, ai, bi, inv_a, inv_b, ra, rb, n3 * (nw - old));
+    });
+    if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK);
+}
+
 // Velocity solve of one island per workgroup: warm-start application and the PGS iterations in
 // list order, the pre-shock impulse snapshot (what the warm cache stores), then the shock pass in
 // shock order (built before the launch from the tick-start poses, which the solve leaves as is).
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickParams tp) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickParams tp, uint32_t split) {
     if (aborted(d)) return;
     const uint32_t cur = tp.cur;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
@@ -1085,10 +1196,21 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         // islands larger than the workgroup sweep in level order (the ballistic and projection
         // sweeps reuse these slots); smaller ones give every rank its own lane in list order
         PhaseClock pc;
-        const uint32_t *exb = n > blockDim.x ? level_order(I, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, d.ex_base, &d.ph[RP_PH_ROUNDS_BASE]) : nullptr;
+        __shared__ uint32_t wide_k;
+        const uint32_t *exb = n > blockDim.x ? level_order(I, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, d.ex_base, &d.ph[RP_PH_ROUNDS_BASE],
+                                                           TIER > 0 ? d.lvl_off + s : nullptr, TIER > 0 ? &wide_k : nullptr)
+                                              : nullptr;
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
         build_slots(d, I, d.isl_v2, d.ch_pa, d.ch_pb, exb, sb);
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_ORDER);
+        if constexpr (TIER > 0) {
+            // the island's leading wide levels (sweeps may run them as GPU-wide launches)
+            const uint32_t K = exb ? wide_k : 0u;
+            if (threadIdx.x == 0) {
+                d.isl_w[isl] = K;
+                if (K) { d.wide_list[atomicAdd(&d.cnt[CT_WIDE_N], 1u)] = isl; atomicMax(&d.cnt[CT_WIDE_MAX], K); }
+            }
+            pc.lap(d, RP_PH_VEL_ORDER);
+        }
         // warm-start application (solver.rs:749-760): the previous frame's impulses (reference),
         // nothing (zero), or the previous frame's plus the model's correction where the contact
         // passed the model's gate (learned; other contacts take the reference's warm start, and a
@@ -1146,60 +1268,25 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         // loaded once and kept in registers between the normal and the friction update.
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
         __syncthreads();
-        struct PgsIn { Row x; float jn; float2 jt; };
+        if (TIER > 0 && split) {  // the sweeps and the shock pass follow in their own launches
+            if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], tp.iv); atomicAdd(&d.cnt[CT_ITERS_SUM], tp.iv * n); }
+            continue;
+        }
         // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
         __shared__ uint32_t resid_bits;
+        RP_LDS uint32_t *rb_lds = tp.resid_on ? (RP_LDS uint32_t *)&resid_bits : nullptr;
         uint32_t used = 0;
         bool conv = false;
-        auto pgs_prep = [&](uint32_t c, uint32_t slot) {
-            PgsIn in;
-            in.x = fetch_row(d, slot);
-            in.jn = d.jn[c];
-            in.jt = d.jt[c];
-            return in;
-        };
-        auto pgs_row = [&](uint32_t c, const PgsIn &in) {
-            const Row &x = in.x;
-            Vel2 v;
-            v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
-            if (x.bi != RP_NONE) { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
-            const Vel2 v0 = v;
-            float vrn = dot3(rel_vel2(x, v), x.n);
-            float jraw = (x.m4.w - vrn) * x.m4.x;
-            float old = in.jn;
-            float nw = fmaxf(old + jraw, 0.0f);
-            d.jn[c] = nw;
-            apply_row2(x, v, x.n * (nw - old));
-            V3 vf = rel_vel2(x, v);
-            float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
-            float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
-            V3 ot = mk3(in.jt.x, in.jt.y, 0.0f);
-            V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
-            d.jt[c] = make_float2(nt.x, nt.y);
-            V3 dd = nt - ot;
-            apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
-            if (tp.resid_on) {  // the oracle's row residual, same operations
-                float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
-                float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
-                float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
-                atomicMax(&resid_bits, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))));
-            }
-            // store what changed (a body the row does not write keeps its bits)
-            if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
-            if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
-            if (x.bi != RP_NONE) {
-                if (!same_bits(v.lb, v0.lb)) d.lin[x.bi] = S3(v.lb);
-                if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
-            }
-        };
+        auto pgs_prep = [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); };
+        auto pgs_apply = [&](uint32_t c, const PgsIn &in) { pgs_row(d, c, in, rb_lds); };
         if (!tp.resid_on) {
-            for (uint32_t it = 0; it < tp.iv; it++) run_sweep(I, sb, pgs_prep, pgs_row);
+            for (uint32_t it = 0; it < tp.iv; it++) run_sweep(I, sb, pgs_prep, pgs_apply);
             used = tp.iv;
         } else {
             for (uint32_t it = 0; it < tp.iv; it++) {
                 if (threadIdx.x == 0) resid_bits = 0;
                 __syncthreads();
-                run_sweep(I, sb, pgs_prep, pgs_row);
+                run_sweep(I, sb, pgs_prep, pgs_apply);
                 // every thread takes the sweep's residual before thread 0 may clear it for the
                 // next sweep, so all threads leave the loop at the same sweep
                 const float rmax = __uint_as_float(resid_bits);
@@ -1213,33 +1300,45 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); }
         }
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);
-        for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
-        __syncthreads();
-        // shock pass (solver.rs:976-993) in shock order
-        const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK]) : nullptr;
-        const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
-        build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, ss);
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
-        run_sweep(I, ss, [&](uint32_t c) {
-            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-            V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
-            float2 inv = d.cs_inv[c];
-            float4 m4 = d.cs_m[c];
-            float inv_a = inv.x, inv_b = inv.y, mn = m4.x;
-            if (inv_a > 0.0f && inv_b > 0.0f) {
-                M3 iia = load_m9(d.iiw + 9 * ai), iib = load_m9(d.iiw + 9 * bi);
-                if (dot3(L3(d.pos[ai]), up) < dot3(L3(d.pos[bi]), up)) { inv_a = 0.0f; iia = mzero(); }
-                else { inv_b = 0.0f; iib = mzero(); }
-                mn = eff_mass(inv_a, inv_b, iia, iib, ra, rb, n3);
-            }
-            float v = dot3(rel_vel(d, ai, bi, ra, rb), n3);
-            float old = d.jn[c];
-            float nw = fmaxf(old + (m4.w - v) * mn, 0.0f);
-            d.jn[c] = nw;
-            apply_cimp(d, ai, bi, inv_a, inv_b, ra, rb, n3 * (nw - old));
-        });
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK);
+        vel_shock<TIER>(d, tp, I, lst, pc);
     ISLAND_LOOP_END
+}
+
+// The split form of the big-island velocity solve, after k_sweep_vel (split): one PGS sweep over
+// the slots after the wide levels that k_level_vel ran (kl of them launched)...
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_tail(Dev d, TickParams tp, uint32_t kl) {
+    if (aborted(d)) return;
+    ISLAND_LOOP_BEGIN
+        const uint32_t k = min(d.isl_w[isl], kl);
+        run_sweep_from(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, k ? d.lvl_off[s + k] : 0u,
+                       [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); }, [&](uint32_t c, const PgsIn &in) { pgs_row(d, c, in, nullptr); });
+    ISLAND_LOOP_END
+}
+// ... and, after the sweeps, the pre-shock snapshot and the shock pass.
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_post(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    ISLAND_LOOP_BEGIN
+        PhaseClock pc;
+        vel_shock<TIER>(d, tp, I, lst, pc);
+    ISLAND_LOOP_END
+}
+// Wide level L of every island that has one: its slots write pairwise distinct bodies, so each
+// lane runs one row, and the launch boundaries order the levels (the reference's level batches,
+// solver.rs:621-642, across the GPU instead of one workgroup).
+__global__ void __launch_bounds__(256) k_level_vel(Dev d, uint32_t L) {
+    if (aborted(d)) return;
+    const uint32_t nwide = d.cnt[CT_WIDE_N];
+    for (uint32_t i = 0; i < nwide; i++) {
+        const uint32_t isl = d.wide_list[i];
+        if (L >= d.isl_w[isl]) continue;
+        const uint32_t s = d.isl_start[isl], lo = d.lvl_off[s + L], hi = d.lvl_off[s + L + 1];
+        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) {
+            const uint32_t c = d.bs_c[s + j];
+            pgs_row(d, c, pgs_fetch(d, c, s + j), nullptr);
+        }
+    }
 }
 
 // solver.rs:1247-1270 loaded flags and impulse write-back
@@ -1561,7 +1660,7 @@ static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 u
 // Position projection iterations in list order (step.rs:297-310); the shock pass follows in its
 // own launch once the shock order of the projected poses is built.
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickParams tp) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickParams tp, uint32_t split) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     ISLAND_LOOP_BEGIN
@@ -1569,10 +1668,37 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickP
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
         __syncthreads();
+        if (TIER > 0 && split) continue;  // the sweeps follow as k_level_proj + k_sweep_proj_tail
         for (uint32_t it = 0; it < tp.ip; it++)
             run_sweep(I, sb, [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
     ISLAND_LOOP_END
+}
+// Split form: one projection sweep over the slots after the wide levels k_level_proj ran.
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_tail(Dev d, TickParams tp, uint32_t kl) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    ISLAND_LOOP_BEGIN
+        PhaseClock pc;
+        const uint32_t k = min(d.isl_w[isl], kl);
+        run_sweep_from(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, k ? d.lvl_off[s + k] : 0u,
+                       [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
+        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
+    ISLAND_LOOP_END
+}
+// Wide level L of the projection sweep, across the GPU (as k_level_vel).
+__global__ void __launch_bounds__(256) k_level_proj(Dev d, TickParams tp, uint32_t L) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    const uint32_t nwide = d.cnt[CT_WIDE_N];
+    for (uint32_t i = 0; i < nwide; i++) {
+        const uint32_t isl = d.wide_list[i];
+        if (L >= d.isl_w[isl]) continue;
+        const uint32_t s = d.isl_start[isl], lo = d.lvl_off[s + L], hi = d.lvl_off[s + L + 1];
+        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x)
+            project_row(d, fetch_prow(d, s + j), false, up);
+    }
 }
 // Projection shock pass (step.rs:299-301, 447-497 with shock = true) in shock order.
 template <int TIER>
@@ -1809,6 +1935,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISL_CONV, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_CONV, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WS_LEARNED, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_N, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_MAX, 0u);
     hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
     if (tp.ws_mode == 2 || tp.log_on) hipLaunchKernelGGL(k_features, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     if (tp.ws_mode == 2)
@@ -1818,17 +1946,55 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
     build_shock_order(d, tp, body_bits, st);
     const uint32_t g0 = grids[0], g1 = grids[1], g2 = grids[2];
-    if (ev) {  // profiled: in order, a stage per tier
+    // wide levels: kl level launches per sweep (fixed iteration counts only); 0 keeps the
+    // one-kernel sweeps.  Any kl gives the same results: levels beyond it join the tail sweep.
+    const uint32_t kl = grids[3], split = (kl > 0 && !tp.resid_on) ? 1u : 0u, GL = 64;
+    // tier 0 on side[0] alongside `big` on the tick's stream, joined back
+    auto fork0 = [&](auto k0, auto big) {
+        if (!ts) { k0(st); big(st); return; }
+        (void)hipEventRecord(ts->ev[0], st);
+        (void)hipStreamWaitEvent(ts->side[0], ts->ev[0], 0);
+        k0(ts->side[0]);
+        big(st);
+        (void)hipEventRecord(ts->ev[1], ts->side[0]);
+        (void)hipStreamWaitEvent(st, ts->ev[1], 0);
+    };
+    auto big_vel = [&](hipStream_t s) {
+        hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
+        hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split);
+        if (!split) return;
+        for (uint32_t it = 0; it < tp.iv; it++) {
+            for (uint32_t L = 0; L < kl; L++) hipLaunchKernelGGL(k_level_vel, dim3(GL), dim3(256), 0, s, d, L);
+            hipLaunchKernelGGL(k_sweep_vel_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, kl);
+            hipLaunchKernelGGL(k_sweep_vel_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, kl);
+        }
+        hipLaunchKernelGGL(k_sweep_vel_post<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp);
+        hipLaunchKernelGGL(k_sweep_vel_post<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp);
+    };
+    auto big_proj = [&](hipStream_t s) {
+        hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
+        hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split);
+        if (!split) return;
+        for (uint32_t it = 0; it < tp.ip; it++) {
+            for (uint32_t L = 0; L < kl; L++) hipLaunchKernelGGL(k_level_proj, dim3(GL), dim3(256), 0, s, d, tp, L);
+            hipLaunchKernelGGL(k_sweep_proj_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, kl);
+            hipLaunchKernelGGL(k_sweep_proj_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, kl);
+        }
+    };
+    auto vel0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
+    if (ev) {  // profiled: in order, a stage per tier (the split big-island solve counts as large)
         mark(RP_ST_VEL_SMALL);
-        hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+        vel0(st);
         mark(RP_ST_VEL_LARGE);
-        hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+        if (split) big_vel(st);
+        else hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
         mark(RP_ST_VEL_HUGE);
-        hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+        if (!split) hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
+    } else if (split) {
+        fork0(vel0, big_vel);
     } else {
-        tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
-              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
-              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
+        tiers(vel0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     }
     if (tp.log_on) {
         hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
@@ -1854,17 +2020,20 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    auto proj0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
     if (ev) {
         mark(RP_ST_PROJ_SMALL);
-        hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, st, d, tp);
+        proj0(st);
         mark(RP_ST_PROJ_LARGE);
-        hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp);
+        if (split) big_proj(st);
+        else hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
         mark(RP_ST_PROJ_HUGE);
-        hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp);
+        if (!split) hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
+    } else if (split) {
+        fork0(proj0, big_proj);
     } else {
-        tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
-              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
-              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
+        tiers(proj0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     }
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses

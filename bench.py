@@ -53,24 +53,27 @@ def stage_bytes(stage, ctr, iv, ip):
     return table.get(stage)
 
 
+# The kernels of each stage.  A big island's velocity solve and projection run as one launch
+# sequence per tick (a setup kernel, kl wide-level launches and a tail sweep per iteration, and
+# for the velocity solve a shock kernel), so their roofline unit is the stage, per tick.
 KERNEL_OF_STAGE = {
     "vel_solve_small": "k_sweep_vel<0>",
-    "vel_solve_large": "k_sweep_vel<1>",
+    "vel_solve_large": "k_sweep_vel<1> + k_level_vel + k_sweep_vel_tail<1> + k_sweep_vel_post<1>",
     "projection_small": "k_sweep_proj<0>",
-    "projection_large": "k_sweep_proj<1>",
+    "projection_large": "k_sweep_proj<1> + k_level_proj + k_sweep_proj_tail<1>",
     "narrow_pairs": "k_narrow_pairs",
     "narrow_terrain": "k_narrow_terrain",
 }
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, when one exists."""
+def measured_traffic(stage, workload):
+    """HBM bytes per tick of a stage from the committed rocprofv3 PMC summary, when one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) >= 3 and f[0] == workload and f[1] == kernel:
+        if len(f) >= 3 and f[0] == workload and f[1] == stage:
             try:
                 return float(f[2])
             except ValueError:
@@ -138,8 +141,9 @@ def roofline_of(stages, ctr, iv, ip, workload):
     bytes_launch = stage_bytes(dom, ctr, iv, ip) / max(ctr["ticks"], 1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(KERNEL_OF_STAGE[dom], workload), "kernel": KERNEL_OF_STAGE[dom],
-            "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": avg_ms}
+            "traffic": measured_traffic(dom, workload), "kernel": KERNEL_OF_STAGE[dom], "stage": dom,
+            "algorithmic_bytes_per_launch": bytes_launch, "mean_launch_ms": avg_ms,
+            "launch": "one tick of the stage" if "+" in KERNEL_OF_STAGE[dom] else "one kernel launch"}
 
 
 def print_stages(stages, ctr):
@@ -417,7 +421,13 @@ def main():
     nb = w.num_bodies()
     print("bench: %s, %d bodies, warmup %d ticks" % (wl["workload"], nb, a.warmup), file=sys.stderr, flush=True)
     if a.warmup > 0:
-        w.step_n(a.warmup, dt)
+        if a.no_graph:
+            for k in range(a.warmup):
+                w.step(dt)
+                if (k + 1) % 50 == 0:
+                    print("bench: warmup %d/%d ticks" % (k + 1, a.warmup), file=sys.stderr, flush=True)
+        else:
+            w.step_n(a.warmup, dt)
     w.hash()  # forces completion of the warmup ticks before the clock starts
 
     # timed region: the profiled step in chunks of up to 100 ticks, a progress line after each

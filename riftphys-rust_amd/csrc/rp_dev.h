@@ -31,6 +31,7 @@ enum {
     CT_ISL_CONV, CT_ISLC_CONV,   // residual mode: islands (and their contacts) that reached the tolerance
     CT_WS_LEARNED,               // contacts whose learned warm start passed the gate
     CT_WIDE_N, CT_WIDE_MAX,      // islands with wide levels; most wide levels of one island
+    CT_BIG_N,                    // islands of the 1024-thread tiers
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
@@ -108,6 +109,7 @@ struct Dev {
     uint32_t *lvl_off;    // per island segment: start slot of each leading wide level, then their end
     uint32_t *isl_w;      // per island: number of leading wide levels (0: none)
     uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
+    uint32_t *big_list;   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
     // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model

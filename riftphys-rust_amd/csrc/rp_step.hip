@@ -1102,6 +1102,60 @@ static __device__ __forceinline__ auto make_island(const Dev &d, uint8_t *flag_l
     else return Island<GlbFlags>{GlbFlags{d.g_done + s}, s, n, 0};
 }
 
+// Warm-start application to contact c (solver.rs:749-760): the previous frame's impulses (mode 0,
+// the reference), nothing (mode 1), or the previous frame's plus the model's correction where the
+// contact passed the model's gate (mode 2; other contacts take the reference's warm start, and a
+// zero correction leaves every operation of the reference's).
+static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &tp, uint32_t c) {
+    const uint32_t cur = tp.cur;
+    if (tp.ws_mode == 2) {
+        const int wi = d.warm_idx[c];
+        const float4 pr = d.pred[c];
+        const bool use = pr.w != 0.0f;
+        if (!use && wi < 0) return;
+        uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+        V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
+        float2 inv = d.cs_inv[c];
+        float jn = 0.0f, x1 = 0.0f, x2 = 0.0f;
+        if (wi >= 0) {
+            jn = d.wk_jn[cur][wi];
+            V3 ejt = L3(d.wk_jt[cur][wi]);
+            x1 = dot3(ejt, t1); x2 = dot3(ejt, t2);
+        }
+        if (use) {
+            if (pr.x != 0.0f) jn = fmaxf(jn + pr.x, 0.0f);
+            if (pr.y != 0.0f) x1 += pr.y;
+            if (pr.z != 0.0f) x2 += pr.z;
+        }
+        V3 v = rel_vel(d, ai, bi, ra, rb);
+        float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
+        float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
+        V3 t = clamp_len_max(mk3(x1, x2, 0.0f), mu * jn);
+        d.jn[c] = jn;
+        d.jt[c] = make_float2(t.x, t.y);
+        if (wi < 0 && jn == 0.0f && t.x == 0.0f && t.y == 0.0f) return;
+        V3 imp = n3 * jn;
+        imp = imp + (t1 * t.x + t2 * t.y);
+        apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
+    } else if (tp.ws_mode == 0) {
+        int wi = d.warm_idx[c];
+        if (wi < 0) return;
+        uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+        V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
+        float2 inv = d.cs_inv[c];
+        float ejn = d.wk_jn[cur][wi];
+        V3 imp = n3 * ejn;
+        V3 v = rel_vel(d, ai, bi, ra, rb);
+        float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
+        float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
+        V3 ejt = L3(d.wk_jt[cur][wi]);
+        V3 t = clamp_len_max(mk3(dot3(ejt, t1), dot3(ejt, t2), 0.0f), mu * ejn);
+        d.jt[c] = make_float2(t.x, t.y);
+        imp = imp + (t1 * t.x + t2 * t.y);
+        apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
+    }
+}
+
 // PGS row of contact c (solver.rs:839-879): its constant row and accumulators are fetched, before
 // the row's wait, by pgs_fetch; pgs_row keeps the two bodies' velocities in registers between the
 // normal and the friction update and stores only the components that changed.  resid_bits (LDS,
@@ -1206,72 +1260,25 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             // the island's leading wide levels (sweeps may run them as GPU-wide launches)
             const uint32_t K = exb ? wide_k : 0u;
             if (threadIdx.x == 0) {
+                d.big_list[atomicAdd(&d.cnt[CT_BIG_N], 1u)] = isl;
                 d.isl_w[isl] = K;
                 if (K) { d.wide_list[atomicAdd(&d.cnt[CT_WIDE_N], 1u)] = isl; atomicMax(&d.cnt[CT_WIDE_MAX], K); }
             }
             pc.lap(d, RP_PH_VEL_ORDER);
         }
-        // warm-start application (solver.rs:749-760): the previous frame's impulses (reference),
-        // nothing (zero), or the previous frame's plus the model's correction where the contact
-        // passed the model's gate (learned; other contacts take the reference's warm start, and a
-        // zero correction leaves every operation of the reference's)
-        if (tp.ws_mode == 2) run_sweep(I, sb, [&](uint32_t c) {
-            const int wi = d.warm_idx[c];
-            const float4 pr = d.pred[c];
-            const bool use = pr.w != 0.0f;
-            if (!use && wi < 0) return;
-            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-            V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
-            float2 inv = d.cs_inv[c];
-            float jn = 0.0f, x1 = 0.0f, x2 = 0.0f;
-            if (wi >= 0) {
-                jn = d.wk_jn[cur][wi];
-                V3 ejt = L3(d.wk_jt[cur][wi]);
-                x1 = dot3(ejt, t1); x2 = dot3(ejt, t2);
-            }
-            if (use) {
-                if (pr.x != 0.0f) jn = fmaxf(jn + pr.x, 0.0f);
-                if (pr.y != 0.0f) x1 += pr.y;
-                if (pr.z != 0.0f) x2 += pr.z;
-            }
-            V3 v = rel_vel(d, ai, bi, ra, rb);
-            float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
-            float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
-            V3 t = clamp_len_max(mk3(x1, x2, 0.0f), mu * jn);
-            d.jn[c] = jn;
-            d.jt[c] = make_float2(t.x, t.y);
-            if (wi < 0 && jn == 0.0f && t.x == 0.0f && t.y == 0.0f) return;
-            V3 imp = n3 * jn;
-            imp = imp + (t1 * t.x + t2 * t.y);
-            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
-        });
-        else if (tp.ws_mode == 0) run_sweep(I, sb, [&](uint32_t c) {
-            int wi = d.warm_idx[c];
-            if (wi < 0) return;
-            uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-            V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]), t1 = L3(d.cs_t1[c]), t2 = L3(d.cs_t2[c]);
-            float2 inv = d.cs_inv[c];
-            float ejn = d.wk_jn[cur][wi];
-            V3 imp = n3 * ejn;
-            V3 v = rel_vel(d, ai, bi, ra, rb);
-            float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
-            float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
-            V3 ejt = L3(d.wk_jt[cur][wi]);
-            V3 t = clamp_len_max(mk3(dot3(ejt, t1), dot3(ejt, t2), 0.0f), mu * ejn);
-            d.jt[c] = make_float2(t.x, t.y);
-            imp = imp + (t1 * t.x + t2 * t.y);
-            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
-        });
+        if (!(TIER > 0 && split) && tp.ws_mode != 1) run_sweep(I, sb, [&](uint32_t c) { warm_row(d, tp, c); });
         if constexpr (TIER > 0) pc.lap(d, RP_PH_WARM);
         // PGS iterations (solver.rs:839-879).  The rows' constants are stored per slot once and
         // loaded, with the row's own accumulators, before its wait; the bodies' velocities are
         // loaded once and kept in registers between the normal and the friction update.
-        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
-        __syncthreads();
-        if (TIER > 0 && split) {  // the sweeps and the shock pass follow in their own launches
+        if (TIER > 0 && split) {
+            // the row store (k_store_rows), warm start, sweeps and shock pass follow in launches
+            // of their own
             if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], tp.iv); atomicAdd(&d.cnt[CT_ITERS_SUM], tp.iv * n); }
             continue;
         }
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
+        __syncthreads();
         // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
         __shared__ uint32_t resid_bits;
         RP_LDS uint32_t *rb_lds = tp.resid_on ? (RP_LDS uint32_t *)&resid_bits : nullptr;
@@ -1339,6 +1346,36 @@ __global__ void __launch_bounds__(256) k_level_vel(Dev d, uint32_t L) {
             pgs_row(d, c, pgs_fetch(d, c, s + j), nullptr);
         }
     }
+}
+// The constant rows of every big island's slots (split form), across the GPU.
+__global__ void __launch_bounds__(256) k_store_rows(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nbig = d.cnt[CT_BIG_N];
+    for (uint32_t i = 0; i < nbig; i++) {
+        const uint32_t isl = d.big_list[i], s = d.isl_start[isl];
+        const uint32_t n = (isl + 1 < d.cnt[CT_ISLANDS] ? d.isl_start[isl + 1] : min(d.cnt[CT_ISLC], d.cap_contacts)) - s;
+        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
+    }
+}
+// Warm start of the split form: wide level L across the GPU, then the tail in one workgroup.
+__global__ void __launch_bounds__(256) k_level_warm(Dev d, TickParams tp, uint32_t L) {
+    if (aborted(d)) return;
+    const uint32_t nwide = d.cnt[CT_WIDE_N];
+    for (uint32_t i = 0; i < nwide; i++) {
+        const uint32_t isl = d.wide_list[i];
+        if (L >= d.isl_w[isl]) continue;
+        const uint32_t s = d.isl_start[isl], lo = d.lvl_off[s + L], hi = d.lvl_off[s + L + 1];
+        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) warm_row(d, tp, d.bs_c[s + j]);
+    }
+}
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_warm_tail(Dev d, TickParams tp, uint32_t kl) {
+    if (aborted(d)) return;
+    ISLAND_LOOP_BEGIN
+        const uint32_t k = min(d.isl_w[isl], kl);
+        run_sweep_from(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, k ? d.lvl_off[s + k] : 0u, [](uint32_t, uint32_t) { return NoRow{}; },
+                       [&](uint32_t c, const NoRow &) { warm_row(d, tp, c); });
+    ISLAND_LOOP_END
 }
 
 // solver.rs:1247-1270 loaded flags and impulse write-back
@@ -1666,9 +1703,9 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickP
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
+        if (TIER > 0 && split) continue;  // k_store_prows, then k_level_proj + k_sweep_proj_tail
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
         __syncthreads();
-        if (TIER > 0 && split) continue;  // the sweeps follow as k_level_proj + k_sweep_proj_tail
         for (uint32_t it = 0; it < tp.ip; it++)
             run_sweep(I, sb, [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
@@ -1686,6 +1723,16 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_tail(Dev d, 
                        [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
     ISLAND_LOOP_END
+}
+// The projection rows of every big island's slots (split form), across the GPU.
+__global__ void __launch_bounds__(256) k_store_prows(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nbig = d.cnt[CT_BIG_N];
+    for (uint32_t i = 0; i < nbig; i++) {
+        const uint32_t isl = d.big_list[i], s = d.isl_start[isl];
+        const uint32_t n = (isl + 1 < d.cnt[CT_ISLANDS] ? d.isl_start[isl + 1] : min(d.cnt[CT_ISLC], d.cap_contacts)) - s;
+        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
+    }
 }
 // Wide level L of the projection sweep, across the GPU (as k_level_vel).
 __global__ void __launch_bounds__(256) k_level_proj(Dev d, TickParams tp, uint32_t L) {
@@ -1937,6 +1984,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WS_LEARNED, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_N, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_MAX, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_BIG_N, 0u);
     hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
     if (tp.ws_mode == 2 || tp.log_on) hipLaunchKernelGGL(k_features, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     if (tp.ws_mode == 2)
@@ -1963,6 +2011,12 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
         hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split);
         if (!split) return;
+        hipLaunchKernelGGL(k_store_rows, dim3(1024), dim3(256), 0, s, d);
+        if (tp.ws_mode != 1) {
+            for (uint32_t L = 0; L < kl; L++) hipLaunchKernelGGL(k_level_warm, dim3(GL), dim3(256), 0, s, d, tp, L);
+            hipLaunchKernelGGL(k_sweep_warm_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, kl);
+            hipLaunchKernelGGL(k_sweep_warm_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, kl);
+        }
         for (uint32_t it = 0; it < tp.iv; it++) {
             for (uint32_t L = 0; L < kl; L++) hipLaunchKernelGGL(k_level_vel, dim3(GL), dim3(256), 0, s, d, L);
             hipLaunchKernelGGL(k_sweep_vel_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, kl);
@@ -1975,6 +2029,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
         hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split);
         if (!split) return;
+        hipLaunchKernelGGL(k_store_prows, dim3(1024), dim3(256), 0, s, d);
         for (uint32_t it = 0; it < tp.ip; it++) {
             for (uint32_t L = 0; L < kl; L++) hipLaunchKernelGGL(k_level_proj, dim3(GL), dim3(256), 0, s, d, tp, L);
             hipLaunchKernelGGL(k_sweep_proj_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, kl);

@@ -109,7 +109,8 @@ struct Dev {
     uint32_t *lvl_off;    // per island segment: start slot of each leading wide level, then their end
     uint32_t *isl_w;      // per island: number of leading wide levels (0: none)
     uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
-    uint32_t *big_list;   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
+    uint32_t *big_list;
+    uint32_t *bp_fill;    // per collider: pairs appended to its list (broadphase write pass)   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
     // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model

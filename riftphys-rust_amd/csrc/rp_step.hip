@@ -209,60 +209,62 @@ static __device__ __forceinline__ bool overlaps(float4 amn, float4 amx, float4 b
     return !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z);
 }
 
-// Enumerates the partners j > i of collider c (global index) that broadphase.rs:29-99 pairs it
-// with; emits world-global collider indices.  mode 0 counts, mode 1 writes (then sorts) the list.
+// The pairs of broadphase.rs:29-99, as per-collider lists of partners j > i (world-global
+// indices, ascending j).  A collider of the sweep class scans forward in min.x order only, up to
+// its own max.x, so each x-overlapping pair of the sweep class is found once, by its first member
+// in that order, and appended to the list of its lower index; oversized colliders test every
+// later collider, and every collider tests the oversized ones after it.  Mode 0 counts each
+// list's length (out_off, zeroed before), mode 1 appends through per-list cursors (bp_fill,
+// zeroed before); k_bp_sort_lists then puts each list in ascending order, so the lists do not
+// depend on which lane appended first.
 template <int MODE>
 __global__ void k_bp_enum(Dev d) {
     if (aborted(d)) return;
+    const uint32_t cap = d.cap_pairs;
+    auto emit = [&](uint32_t owner, uint32_t other) {
+        if (MODE == 0) {
+            atomicAdd(&d.out_off[owner], 1u);
+        } else {
+            const uint32_t o = d.out_off[owner] + atomicAdd(&d.bp_fill[owner], 1u);
+            if (o < cap) { d.pair_i[o] = owner; d.pair_j[o] = other; }
+        }
+    };
     GRID_STRIDE(c, d.nc) {
         const uint32_t w = d.col_world[c], c0 = d.w_col_off[w], n = d.w_col_off[w + 1] - c0, i = c - c0;
         const uint32_t nsap = d.w_nsap[w], nover = d.w_nover[w];
         const float4 mn = d.smin[c], mx = d.smax[c];
         const uint32_t cls = d.ccls[c];
-        uint32_t count = 0;
-        uint32_t base = 0, cap = d.cap_pairs;
-        if (MODE == 1) base = d.out_off[c];
-        auto emit = [&](uint32_t j) {
-            if (MODE == 1) {
-                uint32_t o = base + count;
-                if (o < cap) { d.pair_i[o] = c; d.pair_j[o] = c0 + j; }
-            }
-            count++;
-        };
         if (cls == CLS_OVER) {
             for (uint32_t j = i + 1; j < n; j++)
-                if (overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(j);
+                if (overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(c, c0 + j);
         } else {
             if (cls == CLS_SAP) {
                 const uint32_t r = d.bp_rank[c];
                 for (uint32_t q = r + 1; q < nsap; q++) {
-                    uint32_t j = d.bp_ord[c0 + q];
-                    float4 jmn = d.smin[c0 + j];
+                    const uint32_t j = d.bp_ord[c0 + q];
+                    const float4 jmn = d.smin[c0 + j];
                     if (jmn.x > mx.x) break;
-                    if (j > i && overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(j);
-                }
-                const float ext = d.w_maxext[w];
-                for (int q = (int)r - 1; q >= 0; q--) {
-                    uint32_t j = d.bp_ord[c0 + q];
-                    float4 jmn = d.smin[c0 + j];
-                    if (jmn.x + ext < mn.x) break;
-                    if (j > i && overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(j);
+                    if (overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(min(c, c0 + j), max(c, c0 + j));
                 }
             }
             for (uint32_t q = nsap; q < nsap + nover; q++) {
-                uint32_t j = d.bp_ord[c0 + q];
-                if (j > i && overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(j);
+                const uint32_t j = d.bp_ord[c0 + q];
+                if (j > i && overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(c, c0 + j);
             }
         }
-        if (MODE == 0) d.out_off[c] = count;
-        else if (base + count <= cap) {
-            // insertion sort of this collider's partner list (ascending j)
-            for (uint32_t a = base + 1; a < base + count; a++) {
-                uint32_t v = d.pair_j[a];
-                uint32_t b = a;
-                while (b > base && d.pair_j[b - 1] > v) { d.pair_j[b] = d.pair_j[b - 1]; b--; }
-                d.pair_j[b] = v;
-            }
+    }
+}
+// Insertion sort of each collider's partner list (ascending j).
+__global__ void k_bp_sort_lists(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        const uint32_t base = d.out_off[c], count = d.bp_fill[c];
+        if (base + count > d.cap_pairs) continue;
+        for (uint32_t a = base + 1; a < base + count; a++) {
+            const uint32_t v = d.pair_j[a];
+            uint32_t b = a;
+            while (b > base && d.pair_j[b - 1] > v) { d.pair_j[b] = d.pair_j[b - 1]; b--; }
+            d.pair_j[b] = v;
         }
     }
 }
@@ -1942,13 +1944,16 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_BROAD);
     hipLaunchKernelGGL(k_bp_sort, dim3(d.nw), dim3(1024), 0, st, d);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d.out_off, d.nc);
     hipLaunchKernelGGL(k_bp_enum<0>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     scan_u32(d.out_off, d.nc, d.scan_tmp, d.cnt + CT_PAIRS, st);
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.out_off, d.nc, d.cnt + CT_PAIRS);
     scan_u32(d.cand_off, d.nc, d.scan_tmp, d.cnt + CT_CANDS, st);
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.cand_off, d.nc, d.cnt + CT_CANDS);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 0u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d.bp_fill, d.nc);
     hipLaunchKernelGGL(k_bp_enum<1>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_bp_sort_lists, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     mark(RP_ST_NARROW_PAIRS);
     hipLaunchKernelGGL(k_narrow_pairs, dim3(grid_for(d.cap_pairs, B)), dim3(B), 0, st, d);
     mark(RP_ST_NARROW_TERRAIN);

@@ -175,8 +175,8 @@ struct Scene {
     uint32_t cur = 0;
     // island-sweep workgroups per tier (rphys_batch_set_tuning), then the wide-level launches per
     // sweep (set from the previous tick's widest island; 0: one-kernel sweeps)
-    uint32_t grids[4] = {2048, 256, 64, 0};
-    uint32_t graph_grids[4] = {0, 0, 0, 0};  // the shape the graphs were captured with
+    uint32_t grids[5] = {2048, 256, 64, 0, 0};  // ... and the same for the shock order
+    uint32_t graph_grids[5] = {0, 0, 0, 0, 0};  // the shape the graphs were captured with
     uint32_t max_extra = 0;
     std::vector<uint32_t> body_off, col_off;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -256,7 +256,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.ch_pa, C) && B.alloc(d.ch_pb, C) && B.alloc(d.sh_ord, C) && B.alloc(d.sh_pa, C) && B.alloc(d.sh_pb, C) && B.alloc(d.g_done, C);
     ok &= B.alloc(d.ch_sa, C) && B.alloc(d.ch_sb, C) && B.alloc(d.sh_sa, C) && B.alloc(d.sh_sb, C) && B.alloc(d.ex_base, C) && B.alloc(d.ex_shock, C);
     ok &= B.alloc(d.bs_pa, C) && B.alloc(d.bs_pb, C) && B.alloc(d.bs_c, C) && B.alloc(d.ss_pa, C) && B.alloc(d.ss_pb, C) && B.alloc(d.ss_c, C);
-    ok &= B.alloc(d.slot_of, C) && B.alloc(d.lvl_off, C) && B.alloc(d.isl_w, C + 1) && B.alloc(d.wide_list, C + 1) && B.alloc(d.big_list, C + 1) && B.alloc(d.bp_fill, NC) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
+    ok &= B.alloc(d.slot_of, C) && B.alloc(d.lvl_off, C) && B.alloc(d.isl_w, C + 1) && B.alloc(d.wide_list, C + 1) && B.alloc(d.big_list, C + 1) && B.alloc(d.lvl_off_sh, C) && B.alloc(d.isl_wsh, C + 1) && B.alloc(d.bp_fill, NC) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
     ok &= B.alloc(d.feat, 12 * C) && B.alloc(d.pred, C);
     {
         float *mp = nullptr;
@@ -491,8 +491,9 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         uint32_t done = cnt[CT_DONE];
         // level launches for the next ticks: the last tick's most leading wide levels of one
         // island, rounded up to a multiple of 8 (fewer graph re-captures); any count is exact
-        const uint32_t wk = cnt[CT_WIDE_MAX];
-        S->grids[3] = wk ? std::min<uint32_t>(RP_WIDE_MAX_LEVELS, (wk + 7u) / 8u * 8u) : 0u;
+        auto levels = [](uint32_t k) { return k ? std::min<uint32_t>(RP_WIDE_MAX_LEVELS, (k + 7u) / 8u * 8u) : 0u; };
+        S->grids[3] = levels(cnt[CT_WIDE_MAX]);
+        S->grids[4] = levels(cnt[CT_WSH_MAX]);
         S->cur ^= (done & 1u);
         for (auto *w : ws) w->tick += done;
         remaining -= done;

@@ -32,6 +32,7 @@ enum {
     CT_WS_LEARNED,               // contacts whose learned warm start passed the gate
     CT_WIDE_N, CT_WIDE_MAX,      // islands with wide levels; most wide levels of one island
     CT_BIG_N,                    // islands of the 1024-thread tiers
+    CT_WSH_MAX,                  // most leading wide levels of one island's shock order
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
@@ -109,8 +110,9 @@ struct Dev {
     uint32_t *lvl_off;    // per island segment: start slot of each leading wide level, then their end
     uint32_t *isl_w;      // per island: number of leading wide levels (0: none)
     uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
-    uint32_t *big_list;
-    uint32_t *bp_fill;    // per collider: pairs appended to its list (broadphase write pass)   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
+    uint32_t *big_list;   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
+    uint32_t *bp_fill;    // per collider: pairs appended to its list (broadphase write pass)
+    uint32_t *lvl_off_sh, *isl_wsh;  // as lvl_off / isl_w, for the shock order of the current shock pass
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
     // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model

@@ -1205,37 +1205,53 @@ static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const P
     }
 }
 
-// The pre-shock impulse snapshot (what the warm cache stores) and the velocity shock pass
-// (solver.rs:976-993) in shock order, for one island.
+// Velocity shock row of contact c (solver.rs:976-993): the lower body along up is held fixed.
+static __device__ __forceinline__ void vshock_row(const Dev &d, const V3 up, uint32_t c) {
+    uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
+    V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
+    float2 inv = d.cs_inv[c];
+    float4 m4 = d.cs_m[c];
+    float inv_a = inv.x, inv_b = inv.y, mn = m4.x;
+    if (inv_a > 0.0f && inv_b > 0.0f) {
+        M3 iia = load_m9(d.iiw + 9 * ai), iib = load_m9(d.iiw + 9 * bi);
+        if (dot3(L3(d.pos[ai]), up) < dot3(L3(d.pos[bi]), up)) { inv_a = 0.0f; iia = mzero(); }
+        else { inv_b = 0.0f; iib = mzero(); }
+        mn = eff_mass(inv_a, inv_b, iia, iib, ra, rb, n3);
+    }
+    float v = dot3(rel_vel(d, ai, bi, ra, rb), n3);
+    float old = d.jn[c];
+    float nw = fmaxf(old + (m4.w - v) * mn, 0.0f);
+    d.jn[c] = nw;
+    apply_cimp(d, ai, bi, inv_a, inv_b, ra, rb, n3 * (nw - old));
+}
+// Shock order of island isl in slots ss, recording (tiers 1 and 2) its leading wide levels.
 template <int TIER, class FL>
-static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> &I, const uint32_t *lst, PhaseClock &pc) {
+static __device__ void shock_slots(const Dev &d, Island<FL> &I, uint32_t isl) {
+    const uint32_t s = I.s, n = I.n;
+    __shared__ uint32_t wide_ks;
+    const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK],
+                                                       TIER > 0 ? d.lvl_off_sh + s : nullptr, TIER > 0 ? &wide_ks : nullptr)
+                                          : nullptr;
+    build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, Slots{d.ss_pa, d.ss_pb, d.ss_c});
+    if constexpr (TIER > 0) {
+        const uint32_t K = exs ? wide_ks : 0u;
+        if (threadIdx.x == 0) { d.isl_wsh[isl] = K; if (K) atomicMax(&d.cnt[CT_WSH_MAX], K); }
+    }
+}
+// The pre-shock impulse snapshot (what the warm cache stores) and the velocity shock pass in
+// shock order, for one island; with klsh > 0 (tiers 1 and 2) the pass itself follows as
+// k_level_vshock + k_sweep_vshock_tail.
+template <int TIER, class FL>
+static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> &I, const uint32_t *lst, uint32_t isl, PhaseClock &pc,
+                                 uint32_t klsh) {
     const uint32_t n = I.n;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
     __syncthreads();
-    // shock pass (solver.rs:976-993) in shock order
-    const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK]) : nullptr;
-    const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
-    build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, ss);
+    shock_slots<TIER>(d, I, isl);
     if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
-    run_sweep(I, ss, [&](uint32_t c) {
-        uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-        V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
-        float2 inv = d.cs_inv[c];
-        float4 m4 = d.cs_m[c];
-        float inv_a = inv.x, inv_b = inv.y, mn = m4.x;
-        if (inv_a > 0.0f && inv_b > 0.0f) {
-            M3 iia = load_m9(d.iiw + 9 * ai), iib = load_m9(d.iiw + 9 * bi);
-            if (dot3(L3(d.pos[ai]), up) < dot3(L3(d.pos[bi]), up)) { inv_a = 0.0f; iia = mzero(); }
-            else { inv_b = 0.0f; iib = mzero(); }
-            mn = eff_mass(inv_a, inv_b, iia, iib, ra, rb, n3);
-        }
-        float v = dot3(rel_vel(d, ai, bi, ra, rb), n3);
-        float old = d.jn[c];
-        float nw = fmaxf(old + (m4.w - v) * mn, 0.0f);
-        d.jn[c] = nw;
-        apply_cimp(d, ai, bi, inv_a, inv_b, ra, rb, n3 * (nw - old));
-    });
+    if (TIER > 0 && klsh) return;
+    run_sweep(I, Slots{d.ss_pa, d.ss_pb, d.ss_c}, [&](uint32_t c) { vshock_row(d, up, c); });
     if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK);
 }
 
@@ -1309,7 +1325,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); }
         }
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);
-        vel_shock<TIER>(d, tp, I, lst, pc);
+        vel_shock<TIER>(d, tp, I, lst, isl, pc, 0u);
     ISLAND_LOOP_END
 }
 
@@ -1326,11 +1342,34 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_tail(Dev d, T
 }
 // ... and, after the sweeps, the pre-shock snapshot and the shock pass.
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_post(Dev d, TickParams tp) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_post(Dev d, TickParams tp, uint32_t klsh) {
     if (aborted(d)) return;
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
-        vel_shock<TIER>(d, tp, I, lst, pc);
+        vel_shock<TIER>(d, tp, I, lst, isl, pc, klsh);
+    ISLAND_LOOP_END
+}
+// Wide level L of the velocity shock pass across the GPU ...
+__global__ void __launch_bounds__(256) k_level_vshock(Dev d, TickParams tp, uint32_t L) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    const uint32_t nbig = d.cnt[CT_BIG_N];
+    for (uint32_t i = 0; i < nbig; i++) {
+        const uint32_t isl = d.big_list[i];
+        if (L >= d.isl_wsh[isl]) continue;
+        const uint32_t s = d.isl_start[isl], lo = d.lvl_off_sh[s + L], hi = d.lvl_off_sh[s + L + 1];
+        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) vshock_row(d, up, d.ss_c[s + j]);
+    }
+}
+// ... and the pass's tail in one workgroup.
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vshock_tail(Dev d, TickParams tp, uint32_t klsh) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    ISLAND_LOOP_BEGIN
+        const uint32_t k = min(d.isl_wsh[isl], klsh);
+        run_sweep_from(I, Slots{d.ss_pa, d.ss_pb, d.ss_c}, k ? d.lvl_off_sh[s + k] : 0u, [](uint32_t, uint32_t) { return NoRow{}; },
+                       [&](uint32_t c, const NoRow &) { vshock_row(d, up, c); });
     ISLAND_LOOP_END
 }
 // Wide level L of every island that has one: its slots write pairwise distinct bodies, so each
@@ -1751,17 +1790,43 @@ __global__ void __launch_bounds__(256) k_level_proj(Dev d, TickParams tp, uint32
 }
 // Projection shock pass (step.rs:299-301, 447-497 with shock = true) in shock order.
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d, TickParams tp) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d, TickParams tp, uint32_t klsh) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
-        const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK]) : nullptr;
+        shock_slots<TIER>(d, I, isl);
         const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
-        build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, ss);
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ_SHOCK_ORDER);
+        if (TIER > 0 && klsh) continue;  // k_level_pshock + k_sweep_pshock_tail follow
         run_sweep(I, ss, [&](uint32_t c, uint32_t) { return load_prow(d, c); }, [&](uint32_t, const PRow &x) { project_row(d, x, true, up); });
         if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ_SHOCK);
+    ISLAND_LOOP_END
+}
+
+// Split form of the projection shock pass: wide level L across the GPU, then the tail.
+__global__ void __launch_bounds__(256) k_level_pshock(Dev d, TickParams tp, uint32_t L) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    const uint32_t nbig = d.cnt[CT_BIG_N];
+    for (uint32_t i = 0; i < nbig; i++) {
+        const uint32_t isl = d.big_list[i];
+        if (L >= d.isl_wsh[isl]) continue;
+        const uint32_t s = d.isl_start[isl], lo = d.lvl_off_sh[s + L], hi = d.lvl_off_sh[s + L + 1];
+        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) {
+            const uint32_t c = d.ss_c[s + j];
+            project_row(d, load_prow(d, c), true, up);
+        }
+    }
+}
+template <int TIER>
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_pshock_tail(Dev d, TickParams tp, uint32_t klsh) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    ISLAND_LOOP_BEGIN
+        const uint32_t k = min(d.isl_wsh[isl], klsh);
+        run_sweep_from(I, Slots{d.ss_pa, d.ss_pb, d.ss_c}, k ? d.lvl_off_sh[s + k] : 0u, [&](uint32_t c, uint32_t) { return load_prow(d, c); },
+                       [&](uint32_t, const PRow &x) { project_row(d, x, true, up); });
     ISLAND_LOOP_END
 }
 
@@ -1990,6 +2055,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_N, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_MAX, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_BIG_N, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WSH_MAX, 0u);
     hipLaunchKernelGGL(k_chain_count, dim3(1), dim3(1), 0, st, d);
     if (tp.ws_mode == 2 || tp.log_on) hipLaunchKernelGGL(k_features, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     if (tp.ws_mode == 2)
@@ -2002,6 +2068,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     // wide levels: kl level launches per sweep (fixed iteration counts only); 0 keeps the
     // one-kernel sweeps.  Any kl gives the same results: levels beyond it join the tail sweep.
     const uint32_t kl = grids[3], split = (kl > 0 && !tp.resid_on) ? 1u : 0u, GL = 64;
+    // shock passes in the split form: klsh wide-level launches of the shock order per pass
+    const uint32_t klsh = split ? grids[4] : 0u;
     // tier 0 on side[0] alongside `big` on the tick's stream, joined back
     auto fork0 = [&](auto k0, auto big) {
         if (!ts) { k0(st); big(st); return; }
@@ -2027,8 +2095,12 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
             hipLaunchKernelGGL(k_sweep_vel_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, kl);
             hipLaunchKernelGGL(k_sweep_vel_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, kl);
         }
-        hipLaunchKernelGGL(k_sweep_vel_post<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp);
-        hipLaunchKernelGGL(k_sweep_vel_post<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp);
+        hipLaunchKernelGGL(k_sweep_vel_post<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
+        hipLaunchKernelGGL(k_sweep_vel_post<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
+        if (!klsh) return;
+        for (uint32_t L = 0; L < klsh; L++) hipLaunchKernelGGL(k_level_vshock, dim3(GL), dim3(256), 0, s, d, tp, L);
+        hipLaunchKernelGGL(k_sweep_vshock_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
+        hipLaunchKernelGGL(k_sweep_vshock_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
     };
     auto big_proj = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
@@ -2098,9 +2170,18 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses
     mark(RP_ST_PROJ_SHOCK);
-    tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
-          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
-          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
+    auto pshock0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
+    auto big_pshock = [&](hipStream_t s) {
+        hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
+        hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
+        if (!klsh) return;
+        for (uint32_t L = 0; L < klsh; L++) hipLaunchKernelGGL(k_level_pshock, dim3(GL), dim3(256), 0, s, d, tp, L);
+        hipLaunchKernelGGL(k_sweep_pshock_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
+        hipLaunchKernelGGL(k_sweep_pshock_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
+    };
+    if (klsh) fork0(pshock0, big_pshock);
+    else tiers(pshock0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
+               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     mark(RP_ST_S6);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);

@@ -185,6 +185,9 @@ RPhysResult rphys_world_get_stats(const RPhysWorld *world, RPhysStepStats *out);
 typedef struct RPhysProfile {
     uint32_t ticks;
     uint32_t n_stages;
+    /* summed per stage; the small-island stages (velocity and projection) run alongside the
+     * larger islands' stages on a stream of their own, so the stages can add up to more than the
+     * ticks took */
     double stage_ms[RPHYS_PROFILE_MAX_STAGES];
     double pairs, terrain_candidates, contacts, island_contacts, islands;
     double island_contacts_global; /* contacts of the islands solved by the 1024-thread LDS-flag tier */

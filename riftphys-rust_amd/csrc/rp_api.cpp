@@ -427,24 +427,30 @@ static_assert(RP_NMARKS - 1 <= RPHYS_PROFILE_MAX_STAGES, "stage count exceeds RP
 // waits on the host between ticks.
 struct ProfileAcc {
     static constexpr uint32_t PROFILE_BATCH = 64;
-    hipEvent_t ev[PROFILE_BATCH][RP_NMARKS] = {};
+    // RP_NMARKS stage marks on the tick's stream, then the small tier's velocity and projection
+    // kernels' start and end events on the side stream they run on
+    static constexpr int NEV = RP_NMARKS + 4;
+    hipEvent_t ev[PROFILE_BATCH][NEV] = {};
     RPhysProfile *out = nullptr;
     bool init() {
         for (uint32_t t = 0; t < PROFILE_BATCH; t++)
-            for (int k = 0; k < RP_NMARKS; k++)
+            for (int k = 0; k < NEV; k++)
                 if (!hip_ok(hipEventCreate(&ev[t][k]), "hipEventCreate")) return false;
         return true;
     }
     ~ProfileAcc() {
         for (uint32_t t = 0; t < PROFILE_BATCH; t++)
-            for (int k = 0; k < RP_NMARKS; k++) if (ev[t][k]) (void)hipEventDestroy(ev[t][k]);
+            for (int k = 0; k < NEV; k++) if (ev[t][k]) (void)hipEventDestroy(ev[t][k]);
     }
     // Adds the stage times of the first `done` ticks of the batch and the counter sums.
     bool add(uint32_t done, const uint32_t *cnt) {
         for (uint32_t t = 0; t < done; t++)
             for (int k = 0; k + 1 < RP_NMARKS; k++) {
+                // the small tier's stages run alongside the larger tiers: their own events
+                const int a = k == RP_ST_VEL_SMALL ? RP_NMARKS : (k == RP_ST_PROJ_SMALL ? RP_NMARKS + 2 : k);
+                const int b = a == k ? k + 1 : a + 1;
                 float ms = 0.0f;
-                if (!hip_ok(hipEventElapsedTime(&ms, ev[t][k], ev[t][k + 1]), "hipEventElapsedTime")) return false;
+                if (!hip_ok(hipEventElapsedTime(&ms, ev[t][a], ev[t][b]), "hipEventElapsedTime")) return false;
                 out->stage_ms[k] += ms;
             }
         out->ticks += done;
@@ -481,7 +487,7 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
             uint32_t batch = prof ? std::min(remaining, ProfileAcc::PROFILE_BATCH) : remaining;
             for (uint32_t k = 0; k < batch; k++) {
                 tp.cur = (S->cur + k) & 1u;
-                if (!hip_ok(rp_launch_tick(S->d, tp, S->grids, S->st, prof ? prof->ev[k] : nullptr, prof ? nullptr : &S->ts), "kernel launch"))
+                if (!hip_ok(rp_launch_tick(S->d, tp, S->grids, S->st, prof ? prof->ev[k] : nullptr, &S->ts), "kernel launch"))
                     return false;
             }
         }

@@ -550,10 +550,11 @@ __global__ void k_prep(Dev d, TickParams tp) {
         d.warm_idx[c] = -1;
         d.ct_key[c] = ((uint64_t)(ai - boff) << 32) | (terr ? 0xFFFFFFFFull : (uint64_t)(bi - boff));
         d.ct_anchor[c] = S3(qrot(qconj(LQ(d.rot[ai])), point - pa));
-        atomicOr(&d.flags[ai], FL_CONSTRAINED);
-        if (!terr) atomicOr(&d.flags[bi], FL_CONSTRAINED);
-        if (terr && depth - 0.05f > -0.005f) atomicOr(&d.flags[ai], FL_GROUND);
-        if (d.inv_mass[ai] > 0.0f || (!terr && d.inv_mass[bi] > 0.0f)) atomicOr(&d.w_solve[w], 1u);
+        // idempotent ORs: skipped once the bit is seen set (a world's contacts all mark one word)
+        if (!(d.flags[ai] & FL_CONSTRAINED)) atomicOr(&d.flags[ai], FL_CONSTRAINED);
+        if (!terr && !(d.flags[bi] & FL_CONSTRAINED)) atomicOr(&d.flags[bi], FL_CONSTRAINED);
+        if (terr && depth - 0.05f > -0.005f && !(d.flags[ai] & FL_GROUND)) atomicOr(&d.flags[ai], FL_GROUND);
+        if ((d.inv_mass[ai] > 0.0f || (!terr && d.inv_mass[bi] > 0.0f)) && !d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
         if (wa != RP_NONE && wb != RP_NONE) uf_union(d.root, wa, wb);
     }
 }
@@ -2114,14 +2115,27 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         }
     };
     auto vel0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
-    if (ev) {  // profiled: in order, a stage per tier (the split big-island solve counts as large)
+    // profiled with side streams: tier 0 runs alongside on side[0], timed by its own events
+    // ev[RP_NMARKS + 2 * which] and ev[RP_NMARKS + 2 * which + 1]; the larger tiers take the stages
+    // of the tick's stream
+    auto small_timed = [&](auto k0, int which) {
+        (void)hipEventRecord(ts->ev[0], st);
+        (void)hipStreamWaitEvent(ts->side[0], ts->ev[0], 0);
+        (void)hipEventRecord(ev[RP_NMARKS + 2 * which], ts->side[0]);
+        k0(ts->side[0]);
+        (void)hipEventRecord(ev[RP_NMARKS + 2 * which + 1], ts->side[0]);
+        (void)hipEventRecord(ts->ev[1], ts->side[0]);
+    };
+    if (ev) {  // profiled: a stage per tier (the split big-island solve counts as large)
         mark(RP_ST_VEL_SMALL);
-        vel0(st);
+        if (ts) small_timed(vel0, 0);
+        else vel0(st);
         mark(RP_ST_VEL_LARGE);
         if (split) big_vel(st);
         else hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
         mark(RP_ST_VEL_HUGE);
         if (!split) hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
+        if (ts) (void)hipStreamWaitEvent(st, ts->ev[1], 0);
     } else if (split) {
         fork0(vel0, big_vel);
     } else {
@@ -2155,12 +2169,14 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     auto proj0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
     if (ev) {
         mark(RP_ST_PROJ_SMALL);
-        proj0(st);
+        if (ts) small_timed(proj0, 1);
+        else proj0(st);
         mark(RP_ST_PROJ_LARGE);
         if (split) big_proj(st);
         else hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
         mark(RP_ST_PROJ_HUGE);
         if (!split) hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
+        if (ts) (void)hipStreamWaitEvent(st, ts->ev[1], 0);
     } else if (split) {
         fork0(proj0, big_proj);
     } else {

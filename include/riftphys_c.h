@@ -200,6 +200,9 @@ typedef struct RPhysProfile {
     /* Residual mode: islands, and their contacts, that reached the tolerance before the sweep
      * limit (summed per tick).  Learned warm start: contacts that passed the model's gate. */
     double islands_converged, island_contacts_converged, warm_start_learned;
+    /* Leading levels of at least 1,024 rows of the largest island, list order and shock order,
+     * summed per tick: the levels that run as GPU-wide launches (fixed iteration counts) */
+    double wide_levels, wide_levels_shock;
 } RPhysProfile;
 RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out);
 /* Bandwidth probe (build addition): copies `bytes` between two fresh device buffers `reps` times

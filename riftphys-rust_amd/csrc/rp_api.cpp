@@ -459,6 +459,7 @@ struct ProfileAcc {
         out->pgs_iters_max += cnt[CT_SUM_ITERS_MAX]; out->pgs_iters_sum += cnt[CT_SUM_ITERS_SUM] + 4294967296.0 * cnt[CT_SUM_ITERS_SUM_HI];
         out->islands_converged += cnt[CT_SUM_ISL_CONV]; out->island_contacts_converged += cnt[CT_SUM_ISLC_CONV];
         out->warm_start_learned += cnt[CT_SUM_WS_LEARNED];
+        out->wide_levels += cnt[CT_SUM_WIDE]; out->wide_levels_shock += cnt[CT_SUM_WIDE_SH];
         return true;
     }
 };

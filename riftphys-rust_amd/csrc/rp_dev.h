@@ -36,6 +36,7 @@ enum {
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
+    CT_SUM_WIDE, CT_SUM_WIDE_SH,  // leading wide levels of the largest island (list, shock order)
     CT_SUM_ITERS_SUM_HI,  // high word of the CT_SUM_ITERS_SUM sum (sweeps x contacts outgrow 32 bits)
     CT_COUNT
 };

@@ -1900,6 +1900,7 @@ __global__ void k_tick_done(Dev d) {
     c[CT_SUM_ITERS_MAX] += c[CT_ITERS_MAX];
     const uint64_t isum = ((uint64_t)c[CT_SUM_ITERS_SUM_HI] << 32 | c[CT_SUM_ITERS_SUM]) + c[CT_ITERS_SUM];
     c[CT_SUM_ITERS_SUM] = (uint32_t)isum; c[CT_SUM_ITERS_SUM_HI] = (uint32_t)(isum >> 32);
+    c[CT_SUM_WIDE] += c[CT_WIDE_MAX]; c[CT_SUM_WIDE_SH] += c[CT_WSH_MAX];
     c[CT_SUM_ISL_CONV] += c[CT_ISL_CONV]; c[CT_SUM_ISLC_CONV] += c[CT_ISLC_CONV]; c[CT_SUM_WS_LEARNED] += c[CT_WS_LEARNED];
 }
 __global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) cnt[idx] = v; }

@@ -92,7 +92,7 @@ class Profile(C.Structure):
                 ("terrain_candidates", C.c_double), ("contacts", C.c_double), ("island_contacts", C.c_double), ("islands", C.c_double),
                 ("island_contacts_global", C.c_double), ("pgs_iters_max", C.c_double), ("pgs_iters_sum", C.c_double),
                 ("sweep_phase", C.c_double * 11), ("islands_converged", C.c_double), ("island_contacts_converged", C.c_double),
-                ("warm_start_learned", C.c_double)]
+                ("warm_start_learned", C.c_double), ("wide_levels", C.c_double), ("wide_levels_shock", C.c_double)]
 
 
 class MlpTrainConfig(C.Structure):
@@ -226,7 +226,8 @@ def _profile_dicts(pr):
     counters = dict(ticks=pr.ticks, pairs=pr.pairs, terrain_candidates=pr.terrain_candidates, contacts=pr.contacts,
                     island_contacts=pr.island_contacts, islands=pr.islands, island_contacts_global=pr.island_contacts_global,
                     pgs_iters_max=pr.pgs_iters_max, pgs_iters_sum=pr.pgs_iters_sum, islands_converged=pr.islands_converged,
-                    island_contacts_converged=pr.island_contacts_converged, warm_start_learned=pr.warm_start_learned)
+                    island_contacts_converged=pr.island_contacts_converged, warm_start_learned=pr.warm_start_learned,
+                    wide_levels=pr.wide_levels, wide_levels_shock=pr.wide_levels_shock)
     names = ("vel_order", "warm", "pgs", "vel_shock_order", "vel_shock", "ballistic", "projection", "proj_shock_order",
              "proj_shock", "levels_list", "levels_shock")
     counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}

@@ -27,7 +27,9 @@ def handover(scene, ticks):
 
 
 def compare_tick(g, o, t):
-    g.step()
+    """Steps both one tick (the device through the profiled step) and compares; returns the device
+    tick's profile counters."""
+    _, ctr = g.step_profiled(1)
     o.step()
     np.testing.assert_array_equal(o.pairs(), g.pairs(), err_msg="pairs at tick %d" % t)
     co, cg = o.contacts(), g.contacts()
@@ -39,13 +41,17 @@ def compare_tick(g, o, t):
         bad = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
         assert bad.size == 0, "%s differs at %d bodies, first %s" % (name, bad.size, bad[:5])
     assert o.hash() == g.hash(), "hash at tick %d" % t
+    return ctr
 
 
 def test_settled_pile_continues_bit_exact():
+    """The compared ticks run the big island in split form: its wide list-order and shock-order
+    levels as GPU-wide launches, the rest as one-workgroup tail sweeps."""
     s = scenes.mixed_shapes()
     g, o = handover(s, 150)
     for t in range(151, 153):
-        compare_tick(g, o, t)
+        ctr = compare_tick(g, o, t)
+        assert ctr["wide_levels"] > 0 and ctr["wide_levels_shock"] > 0, ctr
     st = g.stats()
     assert st.islands > 0 and st.island_contacts > 20000, (st.islands, st.island_contacts)
 

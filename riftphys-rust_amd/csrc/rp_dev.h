@@ -33,6 +33,7 @@ enum {
     CT_WIDE_N, CT_WIDE_MAX,      // islands with wide levels; most wide levels of one island
     CT_BIG_N,                    // islands of the 1024-thread tiers
     CT_WSH_MAX,                  // most leading wide levels of one island's shock order
+    CT_TIER_N, CT_TIER_N1, CT_TIER_N2,  // islands per sweep tier
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
@@ -113,6 +114,7 @@ struct Dev {
     uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
     uint32_t *big_list;   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
     uint32_t *bp_fill;    // per collider: pairs appended to its list (broadphase write pass)
+    uint32_t *tier_list[3];  // island indices of each sweep tier this tick (CT_TIER_N + tier of them)
     uint32_t *lvl_off_sh, *isl_wsh;  // as lvl_off / isl_w, for the shock order of the current shock pass
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot

@@ -827,6 +827,16 @@ template <> struct Tier<0> { static constexpr uint32_t threads = 256, flag_bytes
 template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
 template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
 static __device__ __forceinline__ int tier_of(uint32_t n) { return n <= Tier<0>::flag_bytes ? 0 : (n <= Tier<1>::flag_bytes ? 1 : 2); }
+// The islands of each sweep tier (by contact count), so each tier's launch walks only its own.
+__global__ void k_tier_lists(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    GRID_STRIDE(isl, nisl) {
+        const uint32_t n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - d.isl_start[isl];
+        const int t = tier_of(n);
+        d.tier_list[t][atomicAdd(&d.cnt[CT_TIER_N + t], 1u)] = isl;
+    }
+}
 
 // The island's sweep state: segment [s, s + n) and the token counter.
 template <class FL> struct Island {
@@ -1091,9 +1101,10 @@ static __device__ __forceinline__ void apply_row(const Dev &d, const Row &x, V3 
 #define ISLAND_LOOP_BEGIN                                                                          \
     __shared__ __attribute__((aligned(4))) uint8_t flag_lds[Tier<TIER>::flag_bytes > 0 ? Tier<TIER>::flag_bytes : 4]; \
     const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);         \
-    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {                                \
+    const uint32_t ntier = d.cnt[CT_TIER_N + TIER];                                                \
+    for (uint32_t q = blockIdx.x; q < ntier; q += gridDim.x) {                                     \
+        const uint32_t isl = d.tier_list[TIER][q];                                                 \
         const uint32_t s = d.isl_start[isl], n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - s; \
-        if (tier_of(n) != TIER) continue;                                                          \
         const uint32_t *lst = d.isl_v2 + s;                                                        \
         auto I = make_island<TIER>(d, flag_lds, s, n);                                             \
         I.begin();
@@ -2047,6 +2058,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_island_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     scan_u32(d.isl_flag, d.cap_contacts, d.scan_tmp, d.cnt + CT_ISLANDS, st);
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
+    for (uint32_t t = 0; t < 3; t++) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_TIER_N + t, 0u);
+    hipLaunchKernelGGL(k_tier_lists, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_MAX, 0u);

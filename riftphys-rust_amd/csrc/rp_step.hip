@@ -1341,6 +1341,31 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
     ISLAND_LOOP_END
 }
 
+// Every slot of wide level L of the islands in list[0..count) whose leading wide levels
+// (kw[isl] of them, starts at off[s + ...]) include L, spread over the launch's lanes: the rows of
+// one level write pairwise distinct bodies, and the launch boundaries order the levels (the
+// reference's level batches, solver.rs:621-642, across the GPU instead of one workgroup).
+template <class F>
+static __device__ __forceinline__ void for_wide_level(const Dev &d, const uint32_t *list, uint32_t count, const uint32_t *kw, const uint32_t *off,
+                                                      uint32_t L, F row) {
+    for (uint32_t i = 0; i < count; i++) {
+        const uint32_t isl = list[i];
+        if (L >= kw[isl]) continue;
+        const uint32_t s = d.isl_start[isl], lo = off[s + L], hi = off[s + L + 1];
+        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) row(s + j);
+    }
+}
+// Every slot of every big island (the 1024-thread tiers) this tick, across the launch's lanes.
+template <class F>
+static __device__ __forceinline__ void for_big_island_slots(const Dev &d, F f) {
+    const uint32_t nbig = d.cnt[CT_BIG_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    for (uint32_t i = 0; i < nbig; i++) {
+        const uint32_t isl = d.big_list[i], s = d.isl_start[isl];
+        const uint32_t n = (isl + 1 < nisl ? d.isl_start[isl + 1] : nislc) - s;
+        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) f(s + j);
+    }
+}
+
 // The split form of the big-island velocity solve, after k_sweep_vel (split): one PGS sweep over
 // the slots after the wide levels that k_level_vel ran (kl of them launched)...
 template <int TIER>
@@ -1365,13 +1390,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_post(Dev d, T
 __global__ void __launch_bounds__(256) k_level_vshock(Dev d, TickParams tp, uint32_t L) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    const uint32_t nbig = d.cnt[CT_BIG_N];
-    for (uint32_t i = 0; i < nbig; i++) {
-        const uint32_t isl = d.big_list[i];
-        if (L >= d.isl_wsh[isl]) continue;
-        const uint32_t s = d.isl_start[isl], lo = d.lvl_off_sh[s + L], hi = d.lvl_off_sh[s + L + 1];
-        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) vshock_row(d, up, d.ss_c[s + j]);
-    }
+    for_wide_level(d, d.big_list, d.cnt[CT_BIG_N], d.isl_wsh, d.lvl_off_sh, L, [&](uint32_t slot) { vshock_row(d, up, d.ss_c[slot]); });
 }
 // ... and the pass's tail in one workgroup.
 template <int TIER>
@@ -1384,42 +1403,23 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vshock_tail(Dev d
                        [&](uint32_t c, const NoRow &) { vshock_row(d, up, c); });
     ISLAND_LOOP_END
 }
-// Wide level L of every island that has one: its slots write pairwise distinct bodies, so each
-// lane runs one row, and the launch boundaries order the levels (the reference's level batches,
-// solver.rs:621-642, across the GPU instead of one workgroup).
+// Wide level L of the PGS sweep (split form).
 __global__ void __launch_bounds__(256) k_level_vel(Dev d, uint32_t L) {
     if (aborted(d)) return;
-    const uint32_t nwide = d.cnt[CT_WIDE_N];
-    for (uint32_t i = 0; i < nwide; i++) {
-        const uint32_t isl = d.wide_list[i];
-        if (L >= d.isl_w[isl]) continue;
-        const uint32_t s = d.isl_start[isl], lo = d.lvl_off[s + L], hi = d.lvl_off[s + L + 1];
-        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) {
-            const uint32_t c = d.bs_c[s + j];
-            pgs_row(d, c, pgs_fetch(d, c, s + j), nullptr);
-        }
-    }
+    for_wide_level(d, d.wide_list, d.cnt[CT_WIDE_N], d.isl_w, d.lvl_off, L, [&](uint32_t slot) {
+        const uint32_t c = d.bs_c[slot];
+        pgs_row(d, c, pgs_fetch(d, c, slot), nullptr);
+    });
 }
 // The constant rows of every big island's slots (split form), across the GPU.
 __global__ void __launch_bounds__(256) k_store_rows(Dev d) {
     if (aborted(d)) return;
-    const uint32_t nbig = d.cnt[CT_BIG_N];
-    for (uint32_t i = 0; i < nbig; i++) {
-        const uint32_t isl = d.big_list[i], s = d.isl_start[isl];
-        const uint32_t n = (isl + 1 < d.cnt[CT_ISLANDS] ? d.isl_start[isl + 1] : min(d.cnt[CT_ISLC], d.cap_contacts)) - s;
-        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
-    }
+    for_big_island_slots(d, [&](uint32_t slot) { store_row(d, slot, load_row(d, d.bs_c[slot])); });
 }
 // Warm start of the split form: wide level L across the GPU, then the tail in one workgroup.
 __global__ void __launch_bounds__(256) k_level_warm(Dev d, TickParams tp, uint32_t L) {
     if (aborted(d)) return;
-    const uint32_t nwide = d.cnt[CT_WIDE_N];
-    for (uint32_t i = 0; i < nwide; i++) {
-        const uint32_t isl = d.wide_list[i];
-        if (L >= d.isl_w[isl]) continue;
-        const uint32_t s = d.isl_start[isl], lo = d.lvl_off[s + L], hi = d.lvl_off[s + L + 1];
-        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) warm_row(d, tp, d.bs_c[s + j]);
-    }
+    for_wide_level(d, d.wide_list, d.cnt[CT_WIDE_N], d.isl_w, d.lvl_off, L, [&](uint32_t slot) { warm_row(d, tp, d.bs_c[slot]); });
 }
 template <int TIER>
 __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_warm_tail(Dev d, TickParams tp, uint32_t kl) {
@@ -1780,25 +1780,13 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_tail(Dev d, 
 // The projection rows of every big island's slots (split form), across the GPU.
 __global__ void __launch_bounds__(256) k_store_prows(Dev d) {
     if (aborted(d)) return;
-    const uint32_t nbig = d.cnt[CT_BIG_N];
-    for (uint32_t i = 0; i < nbig; i++) {
-        const uint32_t isl = d.big_list[i], s = d.isl_start[isl];
-        const uint32_t n = (isl + 1 < d.cnt[CT_ISLANDS] ? d.isl_start[isl + 1] : min(d.cnt[CT_ISLC], d.cap_contacts)) - s;
-        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
-    }
+    for_big_island_slots(d, [&](uint32_t slot) { store_prow(d, slot, load_prow(d, d.bs_c[slot])); });
 }
 // Wide level L of the projection sweep, across the GPU (as k_level_vel).
 __global__ void __launch_bounds__(256) k_level_proj(Dev d, TickParams tp, uint32_t L) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    const uint32_t nwide = d.cnt[CT_WIDE_N];
-    for (uint32_t i = 0; i < nwide; i++) {
-        const uint32_t isl = d.wide_list[i];
-        if (L >= d.isl_w[isl]) continue;
-        const uint32_t s = d.isl_start[isl], lo = d.lvl_off[s + L], hi = d.lvl_off[s + L + 1];
-        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x)
-            project_row(d, fetch_prow(d, s + j), false, up);
-    }
+    for_wide_level(d, d.wide_list, d.cnt[CT_WIDE_N], d.isl_w, d.lvl_off, L, [&](uint32_t slot) { project_row(d, fetch_prow(d, slot), false, up); });
 }
 // Projection shock pass (step.rs:299-301, 447-497 with shock = true) in shock order.
 template <int TIER>
@@ -1820,16 +1808,8 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d,
 __global__ void __launch_bounds__(256) k_level_pshock(Dev d, TickParams tp, uint32_t L) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    const uint32_t nbig = d.cnt[CT_BIG_N];
-    for (uint32_t i = 0; i < nbig; i++) {
-        const uint32_t isl = d.big_list[i];
-        if (L >= d.isl_wsh[isl]) continue;
-        const uint32_t s = d.isl_start[isl], lo = d.lvl_off_sh[s + L], hi = d.lvl_off_sh[s + L + 1];
-        for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < hi; j += gridDim.x * blockDim.x) {
-            const uint32_t c = d.ss_c[s + j];
-            project_row(d, load_prow(d, c), true, up);
-        }
-    }
+    for_wide_level(d, d.big_list, d.cnt[CT_BIG_N], d.isl_wsh, d.lvl_off_sh, L,
+                   [&](uint32_t slot) { project_row(d, load_prow(d, d.ss_c[slot]), true, up); });
 }
 template <int TIER>
 __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_pshock_tail(Dev d, TickParams tp, uint32_t klsh) {

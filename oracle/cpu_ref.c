@@ -104,6 +104,11 @@ typedef struct {
 } Contact;
 typedef struct { uint32_t key[4]; V3 la; float jn; V3 jt; } Warm;
 typedef struct { uint32_t a, b; PairP p; } PairOverride;
+/* A ball joint (articulation/src/bodies.rs:326-331 add_ball: the three translational axes of frame
+ * A locked, the rotational ones free) between bodies a and b, with body-local frames (pos, rot). */
+typedef struct { uint32_t a, b; V3 fap, fbp; Q4 fra, frb; } Joint;
+/* A joint's warm rows (world.rs:250 warm_joints): row tags and impulses of its last solve. */
+typedef struct { uint8_t has, n, tag[3]; float lam[3]; } WarmJ;
 
 struct OrWorld {
     uint32_t cap, n;
@@ -119,6 +124,7 @@ struct OrWorld {
     int resid_on; float resid_tol; uint32_t resid_max;
     PairOverride ovr[32]; uint32_t novr;
     Warm *warm; uint32_t nwarm;
+    Joint *joints; WarmJ *wj; uint32_t nj, capj;
     /* per-step outputs */
     uint32_t *pairs; uint32_t npairs;
     Contact *contacts; uint32_t ncontacts;
@@ -148,6 +154,7 @@ OrWorld *or_world_create(uint32_t cap) {
 
 void or_world_destroy(OrWorld *w) {
     if (!w) return;
+    free(w->joints); free(w->wj);
     free(w->pos); free(w->rot); free(w->lin); free(w->ang); free(w->inv_mass); free(w->dyn); free(w->iil);
     free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow);
     free(w->cbody); free(w->cshape); free(w->cmat); free(w->caabb); free(w->loaded);
@@ -323,7 +330,33 @@ static void drop_warm_of(OrWorld *w, uint32_t b) {
     for (uint32_t i = 0; i < w->nwarm; i++)
         if (w->warm[i].key[0] != b && w->warm[i].key[1] != b) w->warm[k++] = w->warm[i];
     w->nwarm = k;
+    for (uint32_t q = 0; q < w->nj; q++)
+        if (w->joints[q].a == b || w->joints[q].b == b) w->wj[q].has = 0;
 }
+
+/* world.rs:718 add_ball_joint: frames {px,py,pz,qx,qy,qz,qw} in the bodies' local spaces.
+ * Returns the joint index (joint ids are indices, never reused), or -1. */
+int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *fa, const float *fb) {
+    if (a >= w->n || b >= w->n) return -1;
+    if (w->nj == w->capj) {
+        w->capj = w->capj ? 2 * w->capj : 64;
+        w->joints = xrealloc(w->joints, sizeof(Joint) * w->capj);
+        w->wj = xrealloc(w->wj, sizeof(WarmJ) * w->capj);
+    }
+    Joint j = {a, b, v3(fa[0], fa[1], fa[2]), v3(fb[0], fb[1], fb[2]), q4(fa[3], fa[4], fa[5], fa[6]), q4(fb[3], fb[4], fb[5], fb[6])};
+    w->joints[w->nj] = j;
+    memset(&w->wj[w->nj], 0, sizeof(WarmJ));
+    return (int32_t)w->nj++;
+}
+uint32_t or_num_joints(const OrWorld *w) { return w->nj; }
+/* Joint q's warm rows: returns their count (0: none stored), tags and impulses into the arrays (3 each). */
+uint32_t or_get_joint_warm(const OrWorld *w, uint32_t q, uint8_t *tags, float *lams) {
+    if (q >= w->nj || !w->wj[q].has) return 0;
+    for (uint32_t k = 0; k < w->wj[q].n; k++) { tags[k] = w->wj[q].tag[k]; lams[k] = w->wj[q].lam[k]; }
+    return w->wj[q].n;
+}
+/* The SHAKE direction test of the position solve, turned(a, b) < 0.5 rad, for a given |w| of a * conj(b). */
+int or_shake_ok(float qw) { return 2.0f * acosf(fminf(fabsf(qw), 1.0f)) < 0.5f; }
 
 /* world.rs:696-713 remove_body */
 int or_remove_body(OrWorld *w, uint32_t id) {
@@ -612,6 +645,42 @@ static uint32_t uf_find(uint32_t *p, uint32_t x) {
 }
 
 /* solver.rs:477-1008, 1247-1271 */
+/* ---- ball joints at velocity level: JointRow (solver.rs:272-289), joint_row_vel / joint_row_apply
+ * (291-307), d6_rows for three locked translational axes (336-424: no limits, drives, damping,
+ * friction or armature on a ball joint, so each axis gives one equality row, target 0, unbounded) */
+typedef struct { uint32_t a, b; V3 u, ra, rb; float wa, wb, m; uint8_t tag; } JRow;
+static float jrow_vel(const OrWorld *w, const JRow *r) {
+    V3 va = vadd(w->lin[r->a], vcross(w->ang[r->a], r->ra));
+    V3 vb = vadd(w->lin[r->b], vcross(w->ang[r->b], r->rb));
+    return vdot(vsub(vb, va), r->u);
+}
+static void jrow_apply(OrWorld *w, const JRow *r, float lam) {
+    V3 p = vmul(r->u, lam);
+    if (r->wa > 0.0f) { apply_impulse(w, r->a, vneg(p)); apply_ang_impulse(w, r->a, vcross(r->ra, vneg(p))); }
+    if (r->wb > 0.0f) { apply_impulse(w, r->b, p); apply_ang_impulse(w, r->b, vcross(r->rb, p)); }
+}
+static uint32_t ball_rows(const OrWorld *w, const Joint *j, JRow *out) {
+    uint32_t a = j->a, b = j->b;
+    V3 pa = w->pos[a], pb = w->pos[b];
+    Q4 qa = w->rot[a], qb = w->rot[b];
+    Q4 rot_a = qmul(qa, j->fra);
+    V3 xa = vadd(pa, qrot(qa, j->fap)), xb = vadd(pb, qrot(qb, j->fbp));
+    float wa = w->inv_mass[a], wb = w->inv_mass[b];
+    if (wa == 0.0f && wb == 0.0f) return 0;
+    M3 ia = inv_inertia_world(w, a), ib = inv_inertia_world(w, b);
+    V3 ra = vsub(xa, pa), rb = vsub(xb, pb);
+    uint32_t n = 0;
+    for (int i = 0; i < 3; i++) {
+        V3 u = qrot(rot_a, v3(i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f));
+        V3 ca = vcross(ra, u), cb = vcross(rb, u);
+        float k = ((wa + wb) + vdot(ca, m3v(ia, ca))) + vdot(cb, m3v(ib, cb));
+        if (k <= 1.0e-9f) continue;
+        JRow r = {a, b, u, ra, rb, wa, wb, 1.0f / k, (uint8_t)(i << 3)};
+        out[n++] = r;
+    }
+    return n;
+}
+
 static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, const V3 *pre_ang, float dt) {
     const float beta = 0.05f, slop = 0.005f;
     uint32_t nb = w->n;
@@ -660,6 +729,31 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     free(blev);
     w->stats.levels = levels;
 
+    /* joint rows of the active joints (a dynamic body on either side, islands.rs:96-101;
+     * solver.rs:647-665), then their warm start from last step's rows when the tags agree, serial in
+     * joint order (solver.rs:700-715) */
+    JRow *jr = malloc(sizeof(JRow) * (3 * w->nj + 1));
+    uint32_t *jr0 = malloc(4 * (w->nj + 1)), *jr1 = malloc(4 * (w->nj + 1)), njr = 0;
+    for (uint32_t q = 0; q < w->nj; q++) {
+        const Joint *j = &w->joints[q];
+        jr0[q] = njr;
+        if (w->inv_mass[j->a] > 0.0f || w->inv_mass[j->b] > 0.0f) njr += ball_rows(w, j, jr + njr);
+        jr1[q] = njr;
+    }
+    float *racc = calloc(njr + 1, 4);
+    for (uint32_t q = 0; q < w->nj; q++) {
+        uint32_t s0 = jr0[q], e0 = jr1[q];
+        if (e0 == s0 || !w->wj[q].has || w->wj[q].n != e0 - s0) continue;
+        int same = 1;
+        for (uint32_t k = s0; k < e0; k++) if (w->wj[q].tag[k - s0] != jr[k].tag) same = 0;
+        if (!same) continue;
+        for (uint32_t k = s0; k < e0; k++) {
+            float lam = w->wj[q].lam[k - s0]; /* clamped to the row's (-inf, inf): unchanged */
+            racc[k] = lam;
+            if (lam != 0.0f) jrow_apply(w, &jr[k], lam);
+        }
+    }
+
     float *jn = calloc(nct + 1, 4), *jt = calloc(2 * nct + 2, 4);
     /* warm start: keys and anchors (solver.rs:717-762) */
     uint32_t (*keys)[4] = malloc(sizeof(uint32_t) * 4 * (nct + 1));
@@ -700,13 +794,23 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     free(taken);
     w->stats.warm_matched = matched;
 
-    /* islands of the written bodies (residual mode stops each island on its own residual) */
+    /* islands of the written bodies (residual mode stops each island on its own residual).  A joint
+     * writes a sleeping body too; a contact that reads such a body joins its island. */
     uint32_t *par = malloc(sizeof(uint32_t) * (nb + 1));
+    uint8_t *jw = calloc(nb + 1, 1);
+    for (uint32_t k = 0; k < njr; k++) { if (jr[k].wa > 0.0f) jw[jr[k].a] = 1; if (jr[k].wb > 0.0f) jw[jr[k].b] = 1; }
     for (uint32_t b = 0; b <= nb; b++) par[b] = b;
     for (uint32_t i = 0; i < nct; i++) {
         int wa = cs[i].inv_a > 0.0f, wb = !cs[i].terrain && cs[i].inv_b > 0.0f;
+        if (wa || wb) { wa = wa || jw[cs[i].ai]; wb = wb || (!cs[i].terrain && jw[cs[i].bi]); }
         if (wa && wb) {
             uint32_t x = uf_find(par, cs[i].ai), y = uf_find(par, cs[i].bi);
+            if (x != y) par[x > y ? x : y] = x < y ? x : y;
+        }
+    }
+    for (uint32_t k = 0; k < njr; k++) {
+        if (jr[k].wa > 0.0f && jr[k].wb > 0.0f) {
+            uint32_t x = uf_find(par, jr[k].a), y = uf_find(par, jr[k].b);
             if (x != y) par[x > y ? x : y] = x < y ? x : y;
         }
     }
@@ -719,6 +823,13 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         isl[i] = wa ? uf_find(par, cs[i].ai) : (wb ? uf_find(par, cs[i].bi) : 0xFFFFFFFFu);
         if (isl[i] != 0xFFFFFFFFu) isl_n[isl[i]]++;
     }
+    /* rows per island (contacts and joint rows): an island with rows takes part in the sweeps */
+    uint32_t *isl_rows = calloc(nb + 1, 4), *jisl = malloc(4 * (njr + 1));
+    for (uint32_t i = 0; i < nct; i++) if (isl[i] != 0xFFFFFFFFu) isl_rows[isl[i]]++;
+    for (uint32_t k = 0; k < njr; k++) {
+        jisl[k] = uf_find(par, jr[k].wa > 0.0f ? jr[k].a : jr[k].b);
+        isl_rows[jisl[k]]++;
+    }
 
     /* PGS iterations (solver.rs:788-880), serial in contact order == the level schedule */
     uint32_t iters = w->resid_on ? w->resid_max : w->iv;
@@ -727,6 +838,22 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     for (uint32_t it = 0; it < iters; it++) {
         int active = 0;
         for (uint32_t b = 0; b <= nb; b++) isl_r[b] = 0.0f;
+        /* A. joint rows, serial in joint order (the joint level batches, solver.rs:791-835) */
+        for (uint32_t k = 0; k < njr; k++) {
+            const JRow *r = &jr[k];
+            if (isl_done[jisl[k]]) continue;
+            active = 1;
+            float v = jrow_vel(w, r);
+            float old = racc[k];
+            float nwv = old + (0.0f - v) * r->m;
+            racc[k] = nwv;
+            if (nwv != old) jrow_apply(w, r, nwv - old);
+            if (w->resid_on) {
+                float rr = fabsf(nwv - old) / r->m;
+                if (rr > isl_r[jisl[k]]) isl_r[jisl[k]] = rr;
+            }
+        }
+        /* B. contacts */
         for (uint32_t i = 0; i < nct; i++) {
             const CC *k = &cs[i];
             if (isl[i] == 0xFFFFFFFFu || isl_done[isl[i]]) continue;
@@ -757,7 +884,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         if (!active) break;
         used++;
         for (uint32_t b = 0; b <= nb; b++) {
-            if (!isl_n[b] || isl_done[b]) continue;
+            if (!isl_rows[b] || isl_done[b]) continue;
             isl_it[b]++;
             if (w->resid_on && isl_r[b] <= w->resid_tol) isl_done[b] = 1;
         }
@@ -766,12 +893,21 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     resid = 0.0f;
     for (uint32_t b = 0; b <= nb; b++) {
         isum += isl_it[b] * isl_n[b];
-        if (isl_n[b] && isl_r[b] > resid) resid = isl_r[b];
+        if (isl_rows[b] && isl_r[b] > resid) resid = isl_r[b];
     }
     w->stats.iters_used = used;
     w->stats.iters_sum = isum;
     w->stats.residual = resid;
-    free(par); free(isl); free(isl_n); free(isl_it); free(isl_done); free(isl_r);
+    free(par); free(isl); free(isl_n); free(isl_it); free(isl_done); free(isl_r); free(isl_rows); free(jisl); free(jw);
+    /* joints solved this step replace their warm rows (solver.rs:945-952) */
+    for (uint32_t q = 0; q < w->nj; q++) {
+        uint32_t s0 = jr0[q], e0 = jr1[q];
+        if (e0 == s0) continue;
+        WarmJ *e = &w->wj[q];
+        e->has = 1; e->n = (uint8_t)(e0 - s0);
+        for (uint32_t k = s0; k < e0; k++) { e->tag[k - s0] = jr[k].tag; e->lam[k - s0] = racc[k]; }
+    }
+    free(jr); free(jr0); free(jr1); free(racc);
 
     /* warm-cache store (solver.rs:917-944) */
     Warm *nw = malloc(sizeof(Warm) * (nct + w->nwarm + 1));
@@ -846,8 +982,11 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     free(jn); free(jt); free(keys); free(anch); free(cs);
 }
 
-/* islands.rs:27-101: only emptiness matters for the contact-only step (joints: none) */
+/* islands.rs:27-101: only emptiness matters here (route_to_island keeps a joint or contact with a
+ * dynamic body on either side) */
 static int islands_nonempty(const OrWorld *w, const Contact *cts, uint32_t n) {
+    for (uint32_t q = 0; q < w->nj; q++)
+        if (w->inv_mass[w->joints[q].a] > 0.0f || w->inv_mass[w->joints[q].b] > 0.0f) return 1;
     for (uint32_t i = 0; i < n; i++) {
         uint32_t a = w->cbody[cts[i].a];
         int da = w->inv_mass[a] > 0.0f;
@@ -870,6 +1009,71 @@ static Q4 from_axis_angle(V3 axis, float angle) {
 static Q4 turn(Q4 q, V3 wv, float h) {
     if (vlen2(wv) <= 1.0e-18f) return q;
     return qnormalize(qmul(from_axis_angle(vnormalize(wv), vlen(wv) * h), q));
+}
+
+/* ---- ball joints at position level: solve_d6's translational rows (articulation/src/bodies.rs:616-739)
+ * with SHAKE directions from the step's start poses (world.rs:1072-1076, JointProjection::Shake by
+ * default; bodies.rs:422-499 solve_step_with, one iteration per call).  A ball joint's rows are
+ * rigid (compliance 0): alpha = 0 and the multiplier's history drops out. */
+static V3 normalize_fin(V3 v) {
+    v = dfin_v(v);
+    float l2 = vlen2(v);
+    return l2 > 1.0e-12f ? vmul(v, 1.0f / sqrtf(l2)) : v3s(0);
+}
+/* bodies.rs:562-566 turned: rotation between two orientations, radians */
+static float turned(Q4 a, Q4 b) { return 2.0f * acosf(fminf(fabsf(qmul(a, qconj(b)).w), 1.0f)); }
+/* bodies.rs:569-573 inv_inertia_at */
+static M3 inv_inertia_at(const OrWorld *w, uint32_t i, Q4 q) { M3 r = m3quat(q); return m3mul(m3mul(r, inv_inertia_local_of(w, i)), m3t(r)); }
+/* dynamics.rs:207-210, 71-89 */
+static void pos_delta(OrWorld *w, uint32_t i, V3 dp) {
+    if (w->inv_mass[i] > 0.0f && vfinite(dp)) w->pos[i] = dfin_v(vadd(w->pos[i], dp));
+}
+static void orient_delta(OrWorld *w, uint32_t i, V3 dth) {
+    if (w->inv_mass[i] <= 0.0f || !vfinite(dth)) return;
+    V3 d = dfin_v(dth);
+    if (vlen2(d) <= 1.0e-24f) return;
+    Q4 dq = qnormalize(q4(d.x * 0.5f, d.y * 0.5f, d.z * 0.5f, 1.0f));
+    w->rot[i] = fin_q(qnormalize(qmul(dq, w->rot[i])));
+}
+static void ball_pos(OrWorld *w, const Joint *j, const V3 *spos, const Q4 *srot, float dt) {
+    (void)spos;
+    uint32_t a = j->a, b = j->b;
+    float wa = dfin(w->inv_mass[a]), wb = dfin(w->inv_mass[b]);
+    Q4 sa = srot[a], sb = srot[b];
+    /* SHAKE_MAX_TURN = 0.5 rad: beyond it this call uses the current iterate's directions */
+    int shake = turned(sa, w->rot[a]) < 0.5f && turned(sb, w->rot[b]) < 0.5f;
+    V3 ax0[3] = {v3s(0), v3s(0), v3s(0)}, la0 = v3s(0), lb0 = v3s(0);
+    M3 ia0 = m3zero(), ib0 = m3zero();
+    if (shake) {
+        Q4 ra0 = qmul(sa, j->fra);
+        for (int i = 0; i < 3; i++) ax0[i] = normalize_fin(qrot(ra0, v3(i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f)));
+        la0 = dfin_v(qrot(sa, j->fap)); lb0 = dfin_v(qrot(sb, j->fbp));
+        ia0 = inv_inertia_at(w, a, sa); ib0 = inv_inertia_at(w, b, sb);
+    }
+    for (int i = 0; i < 3; i++) {
+        V3 pa = w->pos[a], pb = w->pos[b];
+        Q4 qa = w->rot[a], qb = w->rot[b];
+        V3 u = normalize_fin(qrot(qmul(qa, j->fra), v3(i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f)));
+        if (viszero(u)) continue;
+        V3 xa = dfin_v(vadd(pa, qrot(qa, j->fap)));
+        V3 xb = dfin_v(vadd(pb, qrot(qb, j->fbp)));
+        V3 d = dfin_v(vsub(xb, xa));
+        V3 un, lva, lvb; M3 iaw, ibw;
+        if (shake) { un = ax0[i]; lva = la0; lvb = lb0; iaw = ia0; ibw = ib0; }
+        else { un = u; lva = vsub(xa, pa); lvb = vsub(xb, pb); iaw = inv_inertia_world(w, a); ibw = inv_inertia_world(w, b); }
+        float c = dfin(dfin(vdot(d, u)) - 0.0f);
+        V3 rau = dfin_v(vcross(lva, un)), rbu = dfin_v(vcross(lvb, un));
+        float wra = dfin(vdot(rau, m3v(iaw, rau))), wrb = dfin(vdot(rbu, m3v(ibw, rbu)));
+        float wsum = dfin(((wa + wb) + wra) + wrb);
+        float denom = dfin(wsum + 0.0f);
+        if (denom <= 0.0f) continue;
+        c = dfin(c + dfin(0.0f * dt));
+        float dl = dfin(-(c + 0.0f) / denom);
+        pos_delta(w, a, dfin_v(vmul(vneg(un), dfin(dl * wa))));
+        pos_delta(w, b, dfin_v(vmul(un, dfin(dl * wb))));
+        orient_delta(w, a, dfin_v(vmul(vneg(m3v(iaw, rau)), dl)));
+        orient_delta(w, b, dfin_v(vmul(m3v(ibw, rbu), dl)));
+    }
 }
 
 typedef struct { uint32_t a, b; int has_b; V3 la, lb, pbw, n; float pen0; } PosC;
@@ -956,12 +1160,23 @@ void or_step(OrWorld *w, float dt) {
     free(w->pairs);
     w->npairs = broadphase(sw, nc, &w->pairs);
     w->stats.pairs = w->npairs;
-    /* narrowphase, pairs (step.rs:95-119); no joints in this restatement */
+    /* narrowphase, pairs (step.rs:95-119); bodies joined directly by a joint do not collide
+     * (step.rs:85-93,102) */
+    uint64_t *jkeys = malloc(8 * (w->nj + 1));
+    for (uint32_t q = 0; q < w->nj; q++) {
+        uint32_t a = w->joints[q].a, b = w->joints[q].b;
+        jkeys[q] = ((uint64_t)(a < b ? a : b) << 32) | (a < b ? b : a);
+    }
+    qsort(jkeys, w->nj, 8, cmp_u64);
     CVec raw = {0};
     for (uint32_t p = 0; p < w->npairs; p++) {
         uint32_t i = w->pairs[2 * p], j = w->pairs[2 * p + 1];
         uint32_t ba = w->cbody[i], bb = w->cbody[j];
         if (ba == bb) continue;
+        if (w->nj) {
+            uint64_t key = ((uint64_t)(ba < bb ? ba : bb) << 32) | (ba < bb ? bb : ba);
+            if (bsearch(&key, jkeys, w->nj, 8, cmp_u64)) continue;
+        }
         float reach = sweep[i] + sweep[j] - MARGIN;
         Manifold m;
         if (!collide_shapes(w->cshape[i], w->pos[ba], w->rot[ba], w->cshape[j], w->pos[bb], w->rot[bb], reach, &m)) continue;
@@ -971,6 +1186,7 @@ void or_step(OrWorld *w, float dt) {
             cpush(&raw, c);
         }
     }
+    free(jkeys);
     /* terrain (step.rs:121-142) */
     uint32_t tc = 0;
     for (uint32_t c = 0; c < nc; c++) tc += terrain_contacts(w, c, sweep[c], &raw);
@@ -992,9 +1208,15 @@ void or_step(OrWorld *w, float dt) {
         constrained[w->cbody[cul.v[k].a]] = 1;
         if (cul.v[k].b != TERRAIN_CI) constrained[w->cbody[cul.v[k].b]] = 1;
     }
+    uint8_t *jointed = calloc(count + 1, 1);
+    for (uint32_t q = 0; q < w->nj; q++) {
+        uint32_t a = w->joints[q].a, b = w->joints[q].b;
+        constrained[a] = constrained[b] = 1;
+        jointed[a] = jointed[b] = 1;
+    }
     for (uint32_t i = 0; i < count; i++) {
         int ld = w->loaded_valid ? w->loaded[i] : 0;
-        ball[i] = !constrained[i] || !ld;
+        ball[i] = !constrained[i] || (!jointed[i] && !ld);
     }
     for (uint32_t k = 0; k < cul.n; k++) {
         const Contact *c = &cul.v[k];
@@ -1085,9 +1307,27 @@ void or_step(OrWorld *w, float dt) {
         pc[npc++] = x;
     }
     V3 *slin = calloc(count + 1, sizeof(V3)), *sang = calloc(count + 1, sizeof(V3));
-    if (npc > 0) {
-        for (uint32_t it = 0; it < w->ip; it++) project(w, pc, npc, slin, sang, 0);
+    if (w->nj > 0 || npc > 0) {
+        /* step.rs:293-308: start poses (the velocity solve's) for SHAKE; joints between two static
+         * or sleeping bodies are not solved; each joint pass is followed by a contact projection */
+        V3 *spos = malloc(sizeof(V3) * (count + 1)); Q4 *srot = malloc(sizeof(Q4) * (count + 1));
+        uint8_t *frozen = malloc(count + 1);
+        for (uint32_t i = 0; i < count; i++) {
+            int live = w->dyn[i] && w->awake[i];
+            spos[i] = live ? opos[i] : w->pos[i];
+            srot[i] = live ? orot[i] : w->rot[i];
+            frozen[i] = !live;
+        }
+        for (uint32_t it = 0; it < w->ip; it++) {
+            for (uint32_t q = 0; q < w->nj; q++) {
+                const Joint *j = &w->joints[q];
+                if (frozen[j->a] && frozen[j->b]) continue;
+                ball_pos(w, j, spos, srot, dt);
+            }
+            project(w, pc, npc, slin, sang, 0);
+        }
         project(w, pc, npc, slin, sang, 1);
+        free(spos); free(srot); free(frozen);
     }
     free(pc);
     /* S6 (step.rs:316-379): the grounding scan is restated as a per-body flag (same result) */
@@ -1100,7 +1340,8 @@ void or_step(OrWorld *w, float dt) {
         if (!w->dyn[i] || !w->awake[i]) continue;
         V3 vl = w->lin[i], va = w->ang[i];
         V3 np = w->pos[i]; Q4 nq = w->rot[i];
-        V3 dl = vsub(vsub(np, ppos[i]), slin[i]);
+        V3 sl = jointed[i] ? v3s(0) : slin[i], sa = jointed[i] ? v3s(0) : sang[i];
+        V3 dl = vsub(vsub(np, ppos[i]), sl);
         if (!viszero(dl)) vl = vadd(vl, vdivs(dl, dt));
         if (ground[i] && vl.y < 0.0f) {
             vl.y = 0.0f;
@@ -1111,7 +1352,7 @@ void or_step(OrWorld *w, float dt) {
             Q4 dq = qmul(nq, qconj(prot[i]));
             V3 da = vmul(v3(dq.x, dq.y, dq.z), 2.0f);
             if (dq.w < 0.0f) da = vneg(da);
-            va = vadd(va, vdivs(vsub(da, sang[i]), dt));
+            va = vadd(va, vdivs(vsub(da, sa), dt));
         }
         b_set_vel(w, i, vl, va);
     }
@@ -1131,7 +1372,7 @@ void or_step(OrWorld *w, float dt) {
     w->contacts = cul.v; w->ncontacts = (uint32_t)cul.n;
     w->stats.contacts = w->ncontacts;
     free(pre_lin); free(pre_ang); free(sweep); free(sw);
-    free(opos); free(orot); free(ppos); free(prot); free(constrained); free(ball); free(slin); free(sang);
+    free(opos); free(orot); free(ppos); free(prot); free(constrained); free(ball); free(slin); free(sang); free(jointed);
 }
 
 /* ------------------------------------------------------------------ hash (provenance.rs:6-24, world.rs:882-909) */
@@ -1167,7 +1408,15 @@ void or_step_hash(const OrWorld *w, uint8_t out[32]) {
         i = e;
     }
     hu64(&h, 0); /* warm_patches */
-    hu64(&h, 0); /* warm_joints */
+    uint64_t nwj = 0;
+    for (uint32_t q = 0; q < w->nj; q++) nwj += w->wj[q].has;
+    hu64(&h, nwj); /* warm_joints, in joint order */
+    for (uint32_t q = 0; q < w->nj; q++) {
+        if (!w->wj[q].has) continue;
+        hu32(&h, q);
+        hu32(&h, w->wj[q].n);
+        for (uint32_t k = 0; k < w->wj[q].n; k++) { or_blake3_update(&h, &w->wj[q].tag[k], 1); hf(&h, w->wj[q].lam[k]); }
+    }
     hu64(&h, 0); /* rotor */
     hu64(&h, w->n);
     for (uint32_t i = 0; i < w->n; i++) hv3(&h, w->last_lin[i]);
@@ -1218,6 +1467,7 @@ typedef struct {
     V3 *pos; Q4 *rot; V3 *lin, *ang;
     Warm *warm; uint32_t nwarm;
     V3 *last_lin; uint8_t *awake; uint16_t *sticks;
+    WarmJ *wj; uint32_t nj;
 } Snap;
 void *or_snapshot(const OrWorld *w) {
     Snap *s = calloc(1, sizeof(Snap));
@@ -1232,6 +1482,7 @@ void *or_snapshot(const OrWorld *w) {
     s->last_lin = malloc(sizeof(V3) * (w->n + 1)); memcpy(s->last_lin, w->last_lin, sizeof(V3) * w->n);
     s->awake = malloc(w->cap + 1); memcpy(s->awake, w->awake, w->cap);
     s->sticks = malloc(2 * (w->cap + 1)); memcpy(s->sticks, w->sticks, 2 * w->cap);
+    s->nj = w->nj; s->wj = malloc(sizeof(WarmJ) * (w->nj + 1)); memcpy(s->wj, w->wj, sizeof(WarmJ) * w->nj);
     return s;
 }
 void or_restore(OrWorld *w, const void *sp) {
@@ -1256,11 +1507,16 @@ void or_restore(OrWorld *w, const void *sp) {
         if (la && lb) w->warm[k++] = w->warm[i];
     }
     w->nwarm = k;
+    /* warm joint rows as snapshot (joints added since start cold); a dead body's joints drop theirs */
+    for (uint32_t q = 0; q < w->nj; q++) {
+        if (q < s->nj) w->wj[q] = s->wj[q]; else memset(&w->wj[q], 0, sizeof(WarmJ));
+        if (!w->alive[w->joints[q].a] || !w->alive[w->joints[q].b]) w->wj[q].has = 0;
+    }
 }
 void or_snapshot_free(void *sp) {
     Snap *s = sp;
     if (!s) return;
-    free(s->pos); free(s->rot); free(s->lin); free(s->ang); free(s->warm); free(s->last_lin); free(s->awake); free(s->sticks);
+    free(s->pos); free(s->rot); free(s->lin); free(s->ang); free(s->warm); free(s->last_lin); free(s->awake); free(s->sticks); free(s->wj);
     free(s);
 }
 

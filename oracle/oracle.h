@@ -85,6 +85,12 @@ void or_set_body_vel(OrWorld *w, uint32_t id, const float *vel);
 void or_set_sleep_allowed(OrWorld *w, uint32_t id, int allowed);
 void or_step(OrWorld *w, float dt);
 void or_step_hash(const OrWorld *w, uint8_t out32[32]);
+/* world.rs:718 add_ball_joint: bodies a, b (indices), frames {px,py,pz,qx,qy,qz,qw} local to each
+ * body.  Returns the joint index, or -1. */
+int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *frame_a, const float *frame_b);
+uint32_t or_num_joints(const OrWorld *w);
+uint32_t or_get_joint_warm(const OrWorld *w, uint32_t q, uint8_t *tags, float *lams);
+int or_shake_ok(float qw);
 uint32_t or_num_bodies(const OrWorld *w);
 uint64_t or_tick(const OrWorld *w);
 /* pos[3n], rot[4n], lin[3n], ang[3n] */

@@ -179,6 +179,10 @@ enum {
 
 #define RP_SROW 12u
 
+// The position solve's SHAKE directions need both bodies turned by less than 0.5 rad this step:
+// 2 acos(min(|w|, 1)) < 0.5 for w of q_start * conj(q_now), which (acosf, rounded as glibc does)
+// holds exactly when min(|w|, 1) >= this float (0.968912482)
+#define RP_SHAKE_COS_BITS 0x3f780aa6u
 #define RP_WIDE_MIN 1024u         // ranks of a level that one workgroup cannot take in one round
 #define RP_WIDE_MAX_LEVELS 64u    // leading wide levels recorded per island
 

@@ -193,3 +193,87 @@ def batch_world(w, seed_base_k=3):
 def small_mixed(n=96, seed=7, hf_n=48):
     """A few dozen mixed bodies over fBm terrain: the quick parity case."""
     return mixed_shapes(n=n, seed=seed, hf_n=hf_n, cell=0.5, extent=14.0, height=6.0, margin=4.0)
+
+
+def towers_ragdolls(n_towers=669, tower_h=20, n_loose=4, n_ragdolls=200, chain=15, seed=None, hf_n=256, cell=0.5,
+                    extent=100.0, margin=13.75, drop=8.0):
+    """Box towers and capsule chains over fBm terrain (64 of these worlds, 16,384 bodies each, are
+    the at-scale workload): n_towers towers of tower_h boxes (half-extent 0.25 m, 1000 kg/m^3) on a
+    grid with jitter U(+-0.2) and one yaw per tower, each box 5 mm above the one below it and the
+    lowest 5 mm above the highest terrain sample under its footprint; n_loose boxes above the
+    terrain; n_ragdolls chains of `chain` capsules (r 0.1, hh 0.15) lying horizontally at a random
+    yaw, end to end, y ~ U(hmax+2, hmax+2+drop), consecutive capsules joined by ball joints at
+    their touching ends.  Bodies: towers (bottom to top), loose boxes, chains.  Draw order:
+    terrain, per tower (jx, jz, yaw), per loose box (x, y, z, yaw), per chain (x, y, z, yaw)."""
+    rng = XorShift64(seed_for(5) if seed is None else seed)
+    hts = fbm_heights(rng, hf_n, cell)
+    grid = hts.reshape(hf_n, hf_n)
+    hmax = float(hts.max())
+    n = n_towers * tower_h + n_loose + n_ragdolls * chain
+    s = _empty(n)
+    g = int(math.ceil(math.sqrt(max(n_towers, 1))))
+    spacing = extent / g
+    idx = 0
+
+    def ground(x, z, r):
+        gx0, gx1 = max(int(math.floor((x - r) / cell)), 0), min(int(math.ceil((x + r) / cell)), hf_n - 1)
+        gz0, gz1 = max(int(math.floor((z - r) / cell)), 0), min(int(math.ceil((z + r) / cell)), hf_n - 1)
+        return float(grid[gz0:gz1 + 1, gx0:gx1 + 1].max())
+
+    for t in range(n_towers):
+        gi, gj = t % g, t // g
+        x = float(np.float32(margin + (gi + 0.5) * spacing) + rng.uniform(-0.2, 0.2))
+        z = float(np.float32(margin + (gj + 0.5) * spacing) + rng.uniform(-0.2, 0.2))
+        yaw = float(rng.uniform(-math.pi, math.pi))
+        base = ground(x, z, 0.36) + 0.255
+        q = quat_yaw_pitch(yaw, 0.0)
+        for k in range(tower_h):
+            s["shape"][idx] = 1
+            s["dims"][idx] = (0.25, 0.25, 0.25)
+            s["pose"][idx, :3] = (x, base + 0.505 * k, z)
+            s["pose"][idx, 3:] = q
+            idx += 1
+    for _ in range(n_loose):
+        x = rng.uniform(margin, margin + extent)
+        y = rng.uniform(hmax + 2.0, hmax + 2.0 + drop)
+        z = rng.uniform(margin, margin + extent)
+        yaw = float(rng.uniform(-math.pi, math.pi))
+        s["shape"][idx] = 1
+        s["dims"][idx] = (0.25, 0.25, 0.25)
+        s["pose"][idx, :3] = (x, y, z)
+        s["pose"][idx, 3:] = quat_yaw_pitch(yaw, 0.0)
+        idx += 1
+    joints = []
+    half = 0.25  # hh + r: a capsule's end along its local +Y
+    for _ in range(n_ragdolls):
+        x = rng.uniform(margin, margin + extent)
+        y = rng.uniform(hmax + 2.0, hmax + 2.0 + drop)
+        z = rng.uniform(margin, margin + extent)
+        yaw = float(rng.uniform(-math.pi, math.pi))
+        q = quat_yaw_pitch(yaw, math.pi / 2)  # local +Y horizontal
+        d = np.array([2.0 * (q[0] * q[1] - q[3] * q[2]), 1.0 - 2.0 * (q[0] ** 2 + q[2] ** 2), 2.0 * (q[1] * q[2] + q[3] * q[0])],
+                     np.float64)
+        first = idx
+        for k in range(chain):
+            p = np.array([x, y, z], np.float64) + d * (2.0 * half * k)
+            s["shape"][idx] = 2
+            s["dims"][idx] = (0.1, 0.15, 0.0)
+            s["pose"][idx, :3] = p.astype(np.float32)
+            s["pose"][idx, 3:] = q
+            idx += 1
+        for k in range(chain - 1):
+            joints.append((first + k, first + k + 1, (0.0, half, 0.0, 0.0, 0.0, 0.0, 1.0), (0.0, -half, 0.0, 0.0, 0.0, 0.0, 1.0)))
+    assert idx == n
+    s["joints"] = joints
+    s["hf"] = dict(nx=hf_n, nz=hf_n, sx=np.float32(cell), sz=np.float32(cell), ox=0.0, oz=0.0, heights=hts)
+    s["pairs"] = list(CONFIG_PAIRS)
+    s["iters"] = (10, 2)  # position iterations 2: the workload's two-iteration joint relaxation
+    s["dt"] = np.float32(1.0 / 60.0)
+    s["name"] = "towers_ragdolls_%d" % n
+    return s
+
+
+def ragdolls_small(seed=11, n_towers=4, tower_h=5, n_ragdolls=6, chain=15, hf_n=48):
+    """A few towers and capsule chains over fBm terrain: the quick joint parity case."""
+    return towers_ragdolls(n_towers=n_towers, tower_h=tower_h, n_loose=2, n_ragdolls=n_ragdolls, chain=chain, seed=seed,
+                           hf_n=hf_n, cell=0.5, extent=14.0, margin=4.0, drop=3.0)

@@ -43,6 +43,14 @@ def lib():
         L.or_add_body.restype = C.c_int32
         L.or_add_body.argtypes = [P, C.c_int, FP, FP, FP, C.c_int, C.c_float]
         L.or_remove_body.argtypes = [P, C.c_uint32]
+        L.or_add_ball_joint.argtypes = [P, C.c_uint32, C.c_uint32, FP, FP]
+        L.or_add_ball_joint.restype = C.c_int32
+        L.or_num_joints.argtypes = [P]
+        L.or_num_joints.restype = C.c_uint32
+        L.or_get_joint_warm.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8), FP]
+        L.or_get_joint_warm.restype = C.c_uint32
+        L.or_shake_ok.argtypes = [C.c_float]
+        L.or_shake_ok.restype = C.c_int
         L.or_set_heightfield.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, FP, C.c_uint32]
         L.or_set_body_pose.argtypes = [P, C.c_uint32, FP]
         L.or_set_body_vel.argtypes = [P, C.c_uint32, FP]
@@ -116,6 +124,10 @@ class OracleWorld:
             p = np.ascontiguousarray(s["pose"][i], np.float32)
             v = np.ascontiguousarray(s["vel"][i], np.float32)
             L.or_add_body(self.h, int(s["shape"][i]), fptr(d), fptr(p), fptr(v), int(s["dynamic"][i]), float(s["density"][i]))
+        for a, b, fa, fb in s.get("joints", []):
+            fa = np.ascontiguousarray(fa, np.float32)
+            fb = np.ascontiguousarray(fb, np.float32)
+            assert L.or_add_ball_joint(self.h, int(a), int(b), fptr(fa), fptr(fb)) >= 0
         hf = s.get("hf")
         if hf is not None:
             hts = np.ascontiguousarray(hf["heights"], np.float32)
@@ -174,6 +186,13 @@ class OracleWorld:
         lib().or_load_state(self.h, fptr(pos), fptr(rot), fptr(lin), fptr(ang), fptr(last), awake.ctypes.data_as(U),
                             sticks.ctypes.data_as(U), int(st["tick"]), wk.ctypes.data_as(C.POINTER(C.c_uint64)), fptr(wla),
                             fptr(wjn), fptr(wjt), len(wk))
+
+    def joint_warm(self, q):
+        """Joint q's stored rows as (tags, impulses)."""
+        tags = (C.c_uint8 * 3)()
+        lams = np.zeros(3, np.float32)
+        n = lib().or_get_joint_warm(self.h, q, tags, fptr(lams))
+        return list(tags)[:n], lams[:n].copy()
 
     def snapshot(self):
         return lib().or_snapshot(self.h)

@@ -151,6 +151,20 @@ RPhysResult rphys_add_body_density(RPhysWorld *world, RPhysShape shape, const fl
                                    const RPhysVelocity *vel, RPhysBodyType body_type, float density, uint32_t *out_body_id);
 /* World::set_pair_props (world.rs:975-979): friction/restitution of a material-id pair.
  * FFI bodies use MaterialId::Default (0); heightfields use Grit (4). */
+/* Ball joint between two live bodies (additive: the reference's C ABI has no joint export; it
+ * replaces World::add_ball_joint, crates/riftphys-world/src/world/world.rs:718, i.e.
+ * crates/riftphys-articulation/src/bodies.rs:326-331 add_ball: the three translational axes of
+ * frame A locked, rotation free).  frame_a / frame_b are the joint frames in body A's / B's space.
+ * Solved as the reference solves a ball joint: three velocity rows per joint ahead of the contacts
+ * in every PGS sweep (solver.rs:336-424, 644-715, 788-835, 945-952) and SHAKE position rows ahead
+ * of every contact projection pass (articulation/src/bodies.rs:616-739, step.rs:284-310); bodies
+ * joined directly do not collide (step.rs:85-93).  *out_joint_id receives the world's joint index
+ * (0-based, creation order, never reused; 0xFFFFFFFF on failure).  Errors: null -> ERR_NULL;
+ * unknown/removed/terrain body -> ERR_INVALID_ID; body_a == body_b or a non-finite frame ->
+ * ERR_BAD_ARG.  A removed body's joints stay, attached to a static body, as in the reference. */
+RPhysResult rphys_world_add_ball_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,
+                                       const RPhysIsometry *frame_b, uint32_t *out_joint_id);
+uint32_t rphys_world_num_joints(const RPhysWorld *world);
 RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
                                        float restitution, float stribeck_v);
 /* World::set_solver_iterations (world.rs:994-996), (velocity, position), default (20, 20) */

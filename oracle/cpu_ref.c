@@ -830,6 +830,8 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         jisl[k] = uf_find(par, jr[k].wa > 0.0f ? jr[k].a : jr[k].b);
         isl_rows[jisl[k]]++;
     }
+    /* the sweep statistics count a joint with rows as one entry of its island, like a contact */
+    for (uint32_t q = 0; q < w->nj; q++) if (jr1[q] > jr0[q]) isl_n[jisl[jr0[q]]]++;
 
     /* PGS iterations (solver.rs:788-880), serial in contact order == the level schedule */
     uint32_t iters = w->resid_on ? w->resid_max : w->iv;

@@ -26,6 +26,7 @@ struct HashDev {
     uint32_t *cv, *cv2;
     uint8_t *digest;
     uint32_t cap_chunks;
+    uint32_t *w_jbytes;
 };
 extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev,
                                      const TierStreams *ts);
@@ -130,8 +131,15 @@ struct HostWorld {
     std::vector<uint64_t> wkey;
     std::vector<F4> wla, wjt;
     std::vector<float> wjn;
+    // ball joints (world.rs:718 add_ball_joint): body indices, frames (position, rotation) in each
+    // body's space, and the carried warm rows (mask: bit 31 stored, bits 0-2 the rows' axes)
+    std::vector<uint32_t> jt_a, jt_b;
+    std::vector<F4> jt_fap, jt_far, jt_fbp, jt_fbr;
+    std::vector<uint32_t> wj_mask;
+    std::vector<float> wj_lam;  // 3 per joint
     uint32_t nb() const { return (uint32_t)pos.size(); }
     uint32_t nc() const { return (uint32_t)col_body.size(); }
+    uint32_t nj() const { return (uint32_t)jt_a.size(); }
 };
 
 static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
@@ -142,6 +150,8 @@ static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
         w.wkey[k] = w.wkey[i]; w.wla[k] = w.wla[i]; w.wjn[k] = w.wjn[i]; w.wjt[k] = w.wjt[i]; k++;
     }
     w.wkey.resize(k); w.wla.resize(k); w.wjn.resize(k); w.wjt.resize(k);
+    for (uint32_t q = 0; q < w.nj(); q++)  // and its joints' warm rows (world.rs:822-826)
+        if (w.jt_a[q] == b || w.jt_b[q] == b) w.wj_mask[q] = 0;
 }
 
 // ------------------------------------------------------------------ device scene (batch of worlds)
@@ -163,6 +173,7 @@ struct Slot {  // device-side snapshot of a scene
     float4 *pos, *rot, *lin, *ang, *last;
     uint32_t *awake, *sticks;
     uint64_t *wkey; float4 *wla; float *wjn; float4 *wjt; uint32_t *woff;
+    uint32_t *wj_mask; float *wj_lam;
     uint32_t cur; uint32_t nwarm;
     std::vector<uint64_t> ticks;
 };
@@ -215,8 +226,9 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     std::unique_ptr<Scene> S(new (std::nothrow) Scene());
     if (!S) { set_err("out of host memory"); return nullptr; }
     Dev &d = S->d;
-    uint32_t nw = (uint32_t)ws.size(), nb = 0, nc = 0, nh = 0, nwarm = 0;
+    uint32_t nw = (uint32_t)ws.size(), nb = 0, nc = 0, nh = 0, nwarm = 0, nj = 0;
     S->body_off.push_back(0); S->col_off.push_back(0);
+    for (auto *w : ws) nj += w->nj();
     for (auto *w : ws) {
         if (w->nc() > RP_MAX_WORLD_COLLIDERS) { set_err("a world holds more than 16384 colliders (the broadphase's per-world LDS sort width)"); return nullptr; }
         nb += w->nb(); nc += w->nc(); nh += w->has_hf ? (uint32_t)w->heights.size() : 0u; nwarm += (uint32_t)w->wkey.size();
@@ -225,7 +237,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         S->max_extra = std::max(S->max_extra, cap - w->nb());
     }
     caps.warm = std::max(caps.warm, nwarm + 1024u);
-    d.nb = nb; d.nc = nc; d.nw = nw;
+    d.nb = nb; d.nc = nc; d.nw = nw; d.nj = nj;
     d.cap_pairs = caps.pairs; d.cap_cands = caps.cands; d.cap_contacts = caps.contacts; d.cap_warm = caps.warm;
     DeviceBuffers &B = S->buf;
     bool ok = true;
@@ -273,15 +285,23 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     size_t scan_elems = std::max<size_t>({NB, NC, C + 1, W + 1, 256 * (size_t)d.rs_blocks + 1, (size_t)nw + 8});
     ok &= B.alloc(d.scan_tmp, scan_elems / 1024 + 2 * nw + 64);
     ok &= B.alloc(d.cnt, CT_COUNT);
+    {
+        const size_t NJ = nj + 1;
+        ok &= B.alloc(d.jt_a, NJ) && B.alloc(d.jt_b, NJ) && B.alloc(d.jt_fap, NJ) && B.alloc(d.jt_far, NJ) && B.alloc(d.jt_fbp, NJ) &&
+              B.alloc(d.jt_fbr, NJ) && B.alloc(d.jr_u, 3 * NJ) && B.alloc(d.jr_ra, NJ) && B.alloc(d.jr_rb, NJ) && B.alloc(d.jr_mask, NJ) &&
+              B.alloc(d.jr_acc, 3 * NJ) && B.alloc(d.wj_mask, NJ) && B.alloc(d.wj_lam, 3 * NJ) && B.alloc(d.jadj_off, NB + 1) &&
+              B.alloc(d.jadj, 2 * NJ) && B.alloc(d.w_joint_off, nw + 1);
+    }
     // hash buffers
     HashDev &h = S->h;
     uint64_t bytes = 0;
     for (auto *w : ws) {
         uint64_t n = w->nb(), cap = std::max(w->max_bodies, w->nb());
-        bytes += 48 + 56 * n + 8 + 24 + 8 + 12 * n + 8 + cap + 8 + 2 * cap + 1024;
+        bytes += 48 + 56 * n + 8 + 24 + 8 + 12 * n + 8 + cap + 8 + 2 * cap + 1024 + 23ull * w->nj();
     }
     bytes += 48ull * W;
     h.cap_chunks = (uint32_t)(bytes / 1024 + nw + 1);
+    ok &= B.alloc(h.w_jbytes, nw + 1);
     ok &= B.alloc(h.keyflag, W + 1) && B.alloc(h.w_chunk, nw + 1) && B.alloc(h.w_len, nw) && B.alloc(h.w_epoch, nw) && B.alloc(h.w_rng, nw);
     ok &= B.alloc(h.stream, (size_t)h.cap_chunks * 1024) && B.alloc(h.cv, 8 * (size_t)h.cap_chunks) && B.alloc(h.cv2, 8 * (size_t)h.cap_chunks) && B.alloc(h.digest, 32 * (size_t)nw);
     if (!ok) { set_err("device allocation failed (scene too large for the configured capacities)"); return nullptr; }
@@ -324,6 +344,29 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         woff.push_back((uint32_t)wkey.size());
         ep.push_back(w->epoch); rn.push_back(w->rng);
     }
+    // joints (global body indices), their warm rows, each body's jointed partners, world offsets
+    std::vector<uint32_t> ja, jb, jmask, jadj_off(nb + 1, 0), jadj, jwoff;
+    std::vector<F4> jfap, jfar, jfbp, jfbr;
+    std::vector<float> jlam;
+    jwoff.push_back(0);
+    for (uint32_t wi = 0; wi < nw; wi++) {
+        HostWorld *w = ws[wi];
+        uint32_t bo = S->body_off[wi];
+        for (uint32_t q = 0; q < w->nj(); q++) {
+            ja.push_back(w->jt_a[q] + bo); jb.push_back(w->jt_b[q] + bo);
+            jadj_off[w->jt_a[q] + bo + 1]++; jadj_off[w->jt_b[q] + bo + 1]++;
+        }
+        jfap.insert(jfap.end(), w->jt_fap.begin(), w->jt_fap.end()); jfar.insert(jfar.end(), w->jt_far.begin(), w->jt_far.end());
+        jfbp.insert(jfbp.end(), w->jt_fbp.begin(), w->jt_fbp.end()); jfbr.insert(jfbr.end(), w->jt_fbr.begin(), w->jt_fbr.end());
+        jmask.insert(jmask.end(), w->wj_mask.begin(), w->wj_mask.end()); jlam.insert(jlam.end(), w->wj_lam.begin(), w->wj_lam.end());
+        jwoff.push_back((uint32_t)ja.size());
+    }
+    for (uint32_t b = 0; b < nb; b++) jadj_off[b + 1] += jadj_off[b];
+    jadj.resize(jadj_off[nb]);
+    {
+        std::vector<uint32_t> fill(jadj_off.begin(), jadj_off.end() - 1);
+        for (size_t q = 0; q < ja.size(); q++) { jadj[fill[ja[q]]++] = jb[q]; jadj[fill[jb[q]]++] = ja[q]; }
+    }
     std::vector<F4> ptab(256);
     {
         HostWorld *w0 = ws[0];
@@ -346,7 +389,12 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
          up(d.hf, hfs.data(), hfs.size() * sizeof(HfDesc)) && up(d.heights, heights.data(), heights.size() * 4) &&
          up(d.pair_tab, ptab.data(), ptab.size() * 16) && up(d.wk_key[cur], wkey.data(), wkey.size() * 8) &&
          up(d.wk_la[cur], wla.data(), wla.size() * 16) && up(d.wk_jn[cur], wjn.data(), wjn.size() * 4) && up(d.wk_jt[cur], wjt.data(), wjt.size() * 16) &&
-         up(d.wc_off[cur], woff.data(), woff.size() * 4) && up(h.w_epoch, ep.data(), ep.size() * 8) && up(h.w_rng, rn.data(), rn.size() * 8);
+         up(d.wc_off[cur], woff.data(), woff.size() * 4) && up(h.w_epoch, ep.data(), ep.size() * 8) && up(h.w_rng, rn.data(), rn.size() * 8) &&
+         up(d.jt_a, ja.data(), ja.size() * 4) && up(d.jt_b, jb.data(), jb.size() * 4) && up(d.jt_fap, jfap.data(), jfap.size() * 16) &&
+         up(d.jt_far, jfar.data(), jfar.size() * 16) && up(d.jt_fbp, jfbp.data(), jfbp.size() * 16) && up(d.jt_fbr, jfbr.data(), jfbr.size() * 16) &&
+         up(d.wj_mask, jmask.data(), jmask.size() * 4) && up(d.wj_lam, jlam.data(), jlam.size() * 4) &&
+         up(d.jadj_off, jadj_off.data(), jadj_off.size() * 4) && up(d.jadj, jadj.data(), jadj.size() * 4) &&
+         up(d.w_joint_off, jwoff.data(), jwoff.size() * 4);
     if (ok) ok = hip_ok(hipMemsetAsync(d.cnt, 0, CT_COUNT * 4, S->st), "memset");
     if (ok && ws[0]->mlp.size() == RP_MLP_NPARAM) ok = upload_model(*S, ws[0]->mlp.data());
     if (ok) ok = hip_ok(hipStreamSynchronize(S->st), "upload sync");
@@ -370,6 +418,11 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
     w.wkey.resize(ne); w.wla.resize(ne); w.wjn.resize(ne); w.wjt.resize(ne);
     ok = dn(w.wkey.data(), d.wk_key[S.cur] + off[0], ne * 8) && dn(w.wla.data(), d.wk_la[S.cur] + off[0], ne * 16) &&
          dn(w.wjn.data(), d.wk_jn[S.cur] + off[0], ne * 4) && dn(w.wjt.data(), d.wk_jt[S.cur] + off[0], ne * 16);
+    if (ok && w.nj()) {
+        uint32_t joff[2];
+        ok = dn(joff, d.w_joint_off + wi, 8) && hip_ok(hipStreamSynchronize(S.st), "download sync") &&
+             dn(w.wj_mask.data(), d.wj_mask + joff[0], w.nj() * 4) && dn(w.wj_lam.data(), d.wj_lam + 3 * (size_t)joff[0], 3 * (size_t)w.nj() * 4);
+    }
     return ok && hip_ok(hipStreamSynchronize(S.st), "download sync");
 }
 
@@ -403,7 +456,9 @@ static Caps initial_caps(const std::vector<HostWorld *> &ws) {
     Caps c;
     c.pairs = (uint32_t)std::min<uint64_t>(16 * nc + 1024, 0x7FFFFFFF);
     c.cands = (uint32_t)std::min<uint64_t>(96 * nc + 4096, 0x7FFFFFFF);
-    c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
+    uint64_t nj = 0;
+    for (auto *w : ws) nj += w->nj();
+    c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + nj + 1024, 0x7FFFFFFF);  // the island lists hold joints too
     c.warm = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
     return c;
 }
@@ -1158,6 +1213,40 @@ RPhysResult rphys_world_clear_heightfield(RPhysWorld *world) {
     GUARD_END
 }
 
+// world.rs:718 add_ball_joint (articulation/src/bodies.rs:326-331): an additive export, the
+// reference's C ABI has no joints.  Joint indices are per world, in creation order, never reused.
+RPhysResult rphys_world_add_ball_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,
+                                       const RPhysIsometry *frame_b, uint32_t *out_joint_id) {
+    GUARD_BEGIN
+    if (out_joint_id) *out_joint_id = 0xFFFFFFFFu;
+    if (!world || !frame_a || !frame_b || !out_joint_id) { set_err("rphys_world_add_ball_joint: null arg"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t a, b;
+    RPhysResult r = live_id(world, body_a, "rphys_world_add_ball_joint", a);
+    if (r != RPHYS_OK) return r;
+    r = live_id(world, body_b, "rphys_world_add_ball_joint", b);
+    if (r != RPHYS_OK) return r;
+    if (a == b) { set_err("rphys_world_add_ball_joint: a body cannot be jointed to itself"); return RPHYS_ERR_BAD_ARG; }
+    const float v[14] = {frame_a->pos.x, frame_a->pos.y, frame_a->pos.z, frame_a->rot.x, frame_a->rot.y, frame_a->rot.z, frame_a->rot.w,
+                         frame_b->pos.x, frame_b->pos.y, frame_b->pos.z, frame_b->rot.x, frame_b->rot.y, frame_b->rot.z, frame_b->rot.w};
+    for (float x : v) if (!std::isfinite(x)) { set_err("rphys_world_add_ball_joint: non-finite frame"); return RPHYS_ERR_BAD_ARG; }
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    HostWorld &w = world->w;
+    *out_joint_id = w.nj();
+    w.jt_a.push_back(a); w.jt_b.push_back(b);
+    w.jt_fap.push_back(f4(v[0], v[1], v[2])); w.jt_far.push_back(f4(v[3], v[4], v[5], v[6]));
+    w.jt_fbp.push_back(f4(v[7], v[8], v[9])); w.jt_fbr.push_back(f4(v[10], v[11], v[12], v[13]));
+    w.wj_mask.push_back(0); w.wj_lam.insert(w.wj_lam.end(), 3, 0.0f);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+uint32_t rphys_world_num_joints(const RPhysWorld *world) {
+    if (!world) return 0;
+    std::lock_guard<std::mutex> lk(const_cast<RPhysWorld *>(world)->mu);
+    return world->w.nj();
+}
+
 RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t a, uint32_t b, float mu_s, float mu_k, float rest, float vs) {
     GUARD_BEGIN
     if (!world) { set_err("rphys_world_set_pair_props: world is null"); return RPHYS_ERR_NULL; }
@@ -1347,6 +1436,8 @@ struct RPhysWorldSnapshot {
     std::vector<uint64_t> wkey;
     std::vector<F4> wla, wjt;
     std::vector<float> wjn;
+    std::vector<uint32_t> wj_mask;
+    std::vector<float> wj_lam;
 };
 RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
     clear_err();
@@ -1363,6 +1454,7 @@ RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
             if (!(w.bflags[i] & BF_ALIVE)) { s->pos[i] = f4(0, 0, 0); s->rot[i] = f4(0, 0, 0, 1); s->lin[i] = f4(0, 0, 0); s->ang[i] = f4(0, 0, 0); }
         s->last_lin = w.last_lin; s->awake = w.awake; s->sticks = w.sticks;
         s->wkey = w.wkey; s->wla = w.wla; s->wjn = w.wjn; s->wjt = w.wjt;
+        s->wj_mask = w.wj_mask; s->wj_lam = w.wj_lam;
         return s;
     } catch (...) {
         set_err("panic in rphys_world_snapshot_create");
@@ -1392,6 +1484,13 @@ RPhysResult rphys_world_snapshot_restore(RPhysWorld *world, const RPhysWorldSnap
     }
     for (uint32_t i = 0; i < s->last_lin.size() && i < w.nb(); i++) w.last_lin[i] = s->last_lin[i];
     for (uint32_t i = 0; i < s->awake.size() && i < w.nb(); i++) { w.awake[i] = s->awake[i]; w.sticks[i] = s->sticks[i]; }
+    // warm joint rows as snapshot (joints added since start cold); a dead body's joints drop theirs
+    for (uint32_t q = 0; q < w.nj(); q++) {
+        const bool had = q < s->wj_mask.size();
+        w.wj_mask[q] = had ? s->wj_mask[q] : 0u;
+        for (int i = 0; i < 3; i++) w.wj_lam[3 * q + i] = had ? s->wj_lam[3 * q + i] : 0.0f;
+        if (!alive(w.jt_a[q]) || !alive(w.jt_b[q])) w.wj_mask[q] = 0;
+    }
     host_changed(world);
     return RPHYS_OK;
     GUARD_END
@@ -1493,7 +1592,7 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
         bool ok = sl->buf.alloc(sl->pos, NB) && sl->buf.alloc(sl->rot, NB) && sl->buf.alloc(sl->lin, NB) && sl->buf.alloc(sl->ang, NB) &&
                   sl->buf.alloc(sl->last, NB) && sl->buf.alloc(sl->awake, NB) && sl->buf.alloc(sl->sticks, NB) &&
                   sl->buf.alloc(sl->wkey, W) && sl->buf.alloc(sl->wla, W) && sl->buf.alloc(sl->wjn, W) && sl->buf.alloc(sl->wjt, W) &&
-                  sl->buf.alloc(sl->woff, d.nw + 1);
+                  sl->buf.alloc(sl->woff, d.nw + 1) && sl->buf.alloc(sl->wj_mask, d.nj + 1) && sl->buf.alloc(sl->wj_lam, 3 * ((size_t)d.nj + 1));
         if (!ok) { set_err("rphys_batch_snapshot: device allocation failed"); return RPHYS_ERR_INTERNAL; }
         S.slots[slot] = std::move(sl);
     }
@@ -1506,7 +1605,8 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
               cp(sl.last, d.last_lin, 16 * n) && cp(sl.awake, d.awake, 4 * n) && cp(sl.sticks, d.sticks, 4 * n) &&
               cp(sl.wkey, d.wk_key[S.cur], 8 * (size_t)nwarm) && cp(sl.wla, d.wk_la[S.cur], 16 * (size_t)nwarm) &&
               cp(sl.wjn, d.wk_jn[S.cur], 4 * (size_t)nwarm) && cp(sl.wjt, d.wk_jt[S.cur], 16 * (size_t)nwarm) &&
-              cp(sl.woff, d.wc_off[S.cur], 4 * ((size_t)d.nw + 1));
+              cp(sl.woff, d.wc_off[S.cur], 4 * ((size_t)d.nw + 1)) && cp(sl.wj_mask, d.wj_mask, 4 * (size_t)d.nj) &&
+              cp(sl.wj_lam, d.wj_lam, 12 * (size_t)d.nj);
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
     sl.cur = S.cur; sl.nwarm = nwarm; sl.valid = true;
     sl.ticks.clear();
@@ -1534,7 +1634,8 @@ static RPhysResult batch_restore(RPhysBatch *b, uint32_t slot) {
               cp(d.last_lin, sl.last, 16 * n) && cp(d.awake, sl.awake, 4 * n) && cp(d.sticks, sl.sticks, 4 * n) &&
               cp(d.wk_key[sl.cur], sl.wkey, 8 * (size_t)sl.nwarm) && cp(d.wk_la[sl.cur], sl.wla, 16 * (size_t)sl.nwarm) &&
               cp(d.wk_jn[sl.cur], sl.wjn, 4 * (size_t)sl.nwarm) && cp(d.wk_jt[sl.cur], sl.wjt, 16 * (size_t)sl.nwarm) &&
-              cp(d.wc_off[sl.cur], sl.woff, 4 * ((size_t)d.nw + 1));
+              cp(d.wc_off[sl.cur], sl.woff, 4 * ((size_t)d.nw + 1)) && cp(d.wj_mask, sl.wj_mask, 4 * (size_t)d.nj) &&
+              cp(d.wj_lam, sl.wj_lam, 12 * (size_t)d.nj);
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "restore")) return RPHYS_ERR_INTERNAL;
     S.cur = sl.cur;
     for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];

@@ -13,7 +13,12 @@
 #define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase
 
 enum { BF_DYN = 1u, BF_ALIVE = 2u, BF_SLEEP_OK = 4u };
-enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u };
+enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u, FL_JOINTED = 8u, FL_JW = 16u };
+// FL_JOINTED: in a joint (step.rs:164-173); FL_JW: written by a joint row this tick (a dynamic
+// body of an active joint, asleep or not)
+
+// Island-list entries are contact indices, or RP_JBIT | joint index for a ball joint's rows.
+#define RP_JBIT 0x80000000u
 enum { CLS_SAP = 0u, CLS_OVER = 1u, CLS_INVALID = 2u };
 
 struct HfDesc {
@@ -34,6 +39,7 @@ enum {
     CT_BIG_N,                    // islands of the 1024-thread tiers
     CT_WSH_MAX,                  // most leading wide levels of one island's shock order
     CT_TIER_N, CT_TIER_N1, CT_TIER_N2,  // islands per sweep tier
+    CT_ROWS,                     // island-list entries before the sort: joints + contacts
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
@@ -142,6 +148,22 @@ struct Dev {
     uint32_t *keepflag;  // cap_warm + 1: kept old entry flags, then their exclusive scan
     uint32_t *scan_tmp;
     uint32_t *cnt;  // CT_COUNT counters
+    // ball joints, world-major (articulation/src/bodies.rs:326-331): bodies, frames in each body's
+    // space (position, rotation), this tick's rows (axis i of frame A and 1 / K in jr_u[3 q + i],
+    // lever arms, bit i of jr_mask when the row exists) and their accumulators, the warm rows
+    // carried between ticks (wj_mask: bit 31 stored, bits 0-2 the rows' axes; wj_lam[3 q + i]),
+    // each body's jointed partners (CSR over bodies) and each world's first joint.  The island
+    // lists hold contacts and joints, so cap_contacts bounds their sum.
+    uint32_t nj;
+    uint32_t *jt_a, *jt_b;
+    float4 *jt_fap, *jt_far, *jt_fbp, *jt_fbr;
+    float4 *jr_u, *jr_ra, *jr_rb;
+    uint32_t *jr_mask;
+    float *jr_acc;
+    uint32_t *wj_mask;
+    float *wj_lam;
+    uint32_t *jadj_off, *jadj;
+    uint32_t *w_joint_off;
 };
 
 // Stage boundaries of one tick (events recorded by the profiled step; stage k spans marks k..k+1).

@@ -16,6 +16,7 @@ struct HashDev {
     uint32_t *cv2;        // merge scratch
     uint8_t *digest;      // 32 per world
     uint32_t cap_chunks;
+    uint32_t *w_jbytes;   // nw: bytes of the world's warm-joint entries
 };
 
 static __constant__ uint32_t B3_IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
@@ -72,17 +73,23 @@ __global__ void k_hash_keyflags(Dev d, HashDev h, uint32_t cur) {
     }
 }
 
-// stream layout per world: header 48, bodies 56 n, warm 8 + 20 K + 28 E, zeros 24, last_lin 8 + 12 n,
-// awake 8 + cap, sleep ticks 8 + 2 cap
+// stream layout per world: header 48, bodies 56 n, warm 8 + 20 K + 28 E, warm patches 8 (none),
+// warm joints 8 + JB, rotors 8 (none), last_lin 8 + 12 n, awake 8 + cap, sleep ticks 8 + 2 cap
 static __device__ __forceinline__ uint32_t world_len(const Dev &d, const HashDev &h, uint32_t w, uint32_t cur) {
     uint32_t n = d.w_body_off[w + 1] - d.w_body_off[w], cap = d.w_cap[w];
     uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
     uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
-    return 48u + 56u * n + 8u + 20u * K + 28u * E + 24u + 8u + 12u * n + 8u + cap + 8u + 2u * cap;
+    return 48u + 56u * n + 8u + 20u * K + 28u * E + 24u + h.w_jbytes[w] + 8u + 12u * n + 8u + cap + 8u + 2u * cap;
 }
+// A stored joint's entry (world.rs:896-900): index u32, row count u32, then (tag u8, impulse f32)
+// per row, rows in axis order with tags axis << 3.
+static __device__ __forceinline__ uint32_t joint_entry_bytes(uint32_t m) { return (m >> 31) ? 8u + 5u * (uint32_t)__popc(m & 7u) : 0u; }
 
 __global__ void k_hash_sizes(Dev d, HashDev h, uint32_t cur) {
     GS(w, d.nw) {
+        uint32_t jb = 0;
+        for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) jb += joint_entry_bytes(d.wj_mask[q]);
+        h.w_jbytes[w] = jb;
         uint32_t len = world_len(d, h, w, cur);
         h.w_len[w] = len;
         h.w_chunk[w] = (len + 1023u) / 1024u;
@@ -108,9 +115,27 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
             uint32_t o = 48u + 56u * n;
             put_u64(S + o, K);
             uint32_t z = o + 8u + 20u * K + 28u * E;
-            put_u64(S + z, 0); put_u64(S + z + 8, 0); put_u64(S + z + 16, 0);
-            put_u64(S + z + 24, n);
-            uint32_t ao = z + 32u + 12u * n;
+            const uint32_t JB = h.w_jbytes[w];
+            put_u64(S + z, 0);
+            uint32_t nstored = 0;
+            uint8_t *jp = S + z + 16u;
+            for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) {
+                const uint32_t m = d.wj_mask[q];
+                if (!(m >> 31)) continue;
+                nstored++;
+                put_u32(jp, q - d.w_joint_off[w]); put_u32(jp + 4, (uint32_t)__popc(m & 7u));
+                jp += 8;
+                for (uint32_t i = 0; i < 3; i++) {
+                    if (!((m >> i) & 1u)) continue;
+                    jp[0] = (uint8_t)(i << 3);
+                    put_u32(jp + 1, canon(d.wj_lam[3 * q + i]));
+                    jp += 5;
+                }
+            }
+            put_u64(S + z + 8, nstored);
+            put_u64(S + z + 16 + JB, 0);
+            put_u64(S + z + 24 + JB, n);
+            uint32_t ao = z + 32u + JB + 12u * n;
             put_u64(S + ao, cap);
             put_u64(S + ao + 8u + cap, cap);
         }
@@ -121,7 +146,7 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
         put_f(p + 16, q.x); put_f(p + 20, q.y); put_f(p + 24, q.z); put_f(p + 28, q.w);
         put_f(p + 32, l.x); put_f(p + 36, l.y); put_f(p + 40, l.z);
         put_f(p + 44, a.x); put_f(p + 48, a.y); put_f(p + 52, a.z);
-        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E;
+        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E + h.w_jbytes[w];
         uint8_t *ll = S + z + 32u + 12u * i;
         float4 lv = d.last_lin[b];
         put_f(ll, lv.x); put_f(ll + 4, lv.y); put_f(ll + 8, lv.z);
@@ -169,7 +194,7 @@ __global__ void k_hash_tail(Dev d, HashDev h, uint32_t cur, uint32_t max_extra) 
         uint8_t *S = h.stream + (size_t)h.w_chunk[w] * 1024u;
         uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
         uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
-        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E;
+        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E + h.w_jbytes[w];
         uint32_t ao = z + 32u + 12u * n;
         uint32_t i = n + k;
         S[ao + 8u + i] = 1;

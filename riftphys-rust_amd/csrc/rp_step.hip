@@ -27,6 +27,7 @@ static __device__ __forceinline__ Q4 LQ(float4 v) { return mkq(v.x, v.y, v.z, v.
 static __device__ __forceinline__ float4 S3(V3 v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
 static __device__ __forceinline__ float4 SQ(Q4 q) { return make_float4(q.x, q.y, q.z, q.w); }
 static __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+static __device__ __forceinline__ V3 unit_axis(uint32_t i) { return mk3(i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f); }
 static __device__ __forceinline__ M3 load_m9(const float *p) {
     return mkm(mk3(p[0], p[1], p[2]), mk3(p[3], p[4], p[5]), mk3(p[6], p[7], p[8]));
 }
@@ -283,7 +284,10 @@ __global__ void k_narrow_pairs(Dev d) {
         uint32_t i = d.pair_i[p], j = d.pair_j[p];
         uint32_t ba = d.col_body[i], bb = d.col_body[j];
         uint32_t cnt = 0;
-        if (ba != bb) {
+        bool jointed = false;  // bodies joined directly by a joint do not collide (step.rs:85-93,102)
+        if (d.nj)
+            for (uint32_t k = d.jadj_off[ba]; k < d.jadj_off[ba + 1]; k++) jointed |= d.jadj[k] == bb;
+        if (ba != bb && !jointed) {
             float reach = (d.sweep[i] + d.sweep[j]) - 0.05f;  // step.rs:105
             Manifold m;
             if (collide_pair(shape_of(d, ba), L3(d.pos[ba]), LQ(d.rot[ba]), shape_of(d, bb), L3(d.pos[bb]), LQ(d.rot[bb]), reach, m)) {
@@ -545,6 +549,12 @@ __global__ void k_prep(Dev d, TickParams tp) {
         d.cs_mu[c] = make_float4(pp.x, pp.y, fmaxf(pp.w, 1.0e-4f), approach);
         uint32_t wa = inv_a > 0.0f ? ai : RP_NONE;
         uint32_t wb = (!terr && inv_b > 0.0f) ? bi : RP_NONE;
+        if (d.nj && (wa != RP_NONE || wb != RP_NONE)) {
+            // a body a joint writes (asleep, say) that this contact only reads: the contact waits
+            // for that joint's row and joins its island
+            if (wa == RP_NONE && (d.flags[ai] & FL_JW)) wa = ai;
+            if (wb == RP_NONE && !terr && (d.flags[bi] & FL_JW)) wb = bi;
+        }
         d.cs_wa[c] = wa; d.cs_wb[c] = wb;
         d.jn[c] = 0.0f; d.jn_pre[c] = 0.0f; d.jt[c] = make_float2(0.0f, 0.0f); d.ct_imp[c] = 0.0f;
         d.warm_idx[c] = -1;
@@ -559,22 +569,74 @@ __global__ void k_prep(Dev d, TickParams tp) {
     }
 }
 
-// Island key of each contact: the component root (lowest body index) of its written bodies.
+// Ball joints before the contacts' prep: the constrained and jointed flags of both bodies
+// (step.rs:164-173), and for an active joint (a dynamic body on either side, islands.rs:96-101) its
+// rows from the tick-start poses (solver.rs:336-424: axis i of frame A, lever arms, 1 / K where
+// K = wa + wb + (ra x u) Ia (ra x u) + (rb x u) Ib (rb x u) > 1e-9), the world's solve flag, the
+// written-by-a-joint flags, and the union of its two bodies when both are dynamic.
+__global__ void k_joint_prep(Dev d) {
+    if (aborted(d)) return;
+    GRID_STRIDE(q, d.nj) {
+        const uint32_t a = d.jt_a[q], b = d.jt_b[q];
+        if ((d.flags[a] & (FL_CONSTRAINED | FL_JOINTED)) != (FL_CONSTRAINED | FL_JOINTED)) atomicOr(&d.flags[a], FL_CONSTRAINED | FL_JOINTED);
+        if ((d.flags[b] & (FL_CONSTRAINED | FL_JOINTED)) != (FL_CONSTRAINED | FL_JOINTED)) atomicOr(&d.flags[b], FL_CONSTRAINED | FL_JOINTED);
+        const float wa = d.inv_mass[a], wb = d.inv_mass[b];
+        uint32_t m = 0;
+        if (wa > 0.0f || wb > 0.0f) {
+            const uint32_t w = d.bworld[a];
+            if (!d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
+            const V3 pa = L3(d.pos[a]), pb = L3(d.pos[b]);
+            const Q4 qa = LQ(d.rot[a]), qb = LQ(d.rot[b]);
+            const Q4 rot_a = qmul(qa, LQ(d.jt_far[q]));
+            const V3 xa = pa + qrot(qa, L3(d.jt_fap[q])), xb = pb + qrot(qb, L3(d.jt_fbp[q]));
+            const M3 ia = load_m9(d.iiw + 9 * a), ib = load_m9(d.iiw + 9 * b);
+            const V3 ra = xa - pa, rb = xb - pb;
+            for (uint32_t i = 0; i < 3; i++) {
+                const V3 u = qrot(rot_a, unit_axis(i));
+                const V3 ca = cross3(ra, u), cb = cross3(rb, u);
+                const float k = ((wa + wb) + dot3(ca, mv(ia, ca))) + dot3(cb, mv(ib, cb));
+                d.jr_acc[3 * q + i] = 0.0f;
+                if (k <= 1.0e-9f) continue;
+                d.jr_u[3 * q + i] = S3(u, 1.0f / k);
+                m |= 1u << i;
+            }
+            d.jr_ra[q] = S3(ra); d.jr_rb[q] = S3(rb);
+            if (m) {
+                if (wa > 0.0f && !(d.flags[a] & FL_JW)) atomicOr(&d.flags[a], FL_JW);
+                if (wb > 0.0f && !(d.flags[b] & FL_JW)) atomicOr(&d.flags[b], FL_JW);
+                if (wa > 0.0f && wb > 0.0f) uf_union(d.root, a, b);
+            }
+        }
+        d.jr_mask[q] = m;
+    }
+}
+
+// Island key of each list entry (joints first, then contacts, so a stable sort keeps every joint
+// ahead of the contacts of its island, as the reference's joint batches run before the contact
+// batches): the component root (lowest body index) of its written bodies.
 __global__ void k_island_keys(Dev d) {
     if (aborted(d)) return;
-    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
-    GRID_STRIDE(c, nct) {
+    const uint32_t nrows = min(d.cnt[CT_ROWS], d.cap_contacts), nj = d.nj;
+    GRID_STRIDE(r, nrows) {
+        if (r < nj) {
+            const uint32_t a = d.jt_a[r], b = d.jt_b[r];
+            const uint32_t x = d.inv_mass[a] > 0.0f ? a : (d.inv_mass[b] > 0.0f ? b : RP_NONE);
+            d.isl_k[r] = (d.jr_mask[r] && x != RP_NONE) ? uf_find(d.root, x) : d.nb;
+            d.isl_v[r] = RP_JBIT | r;
+            continue;
+        }
+        const uint32_t c = r - nj;
         uint32_t wa = d.cs_wa[c], wb = d.cs_wb[c];
         uint32_t x = wa != RP_NONE ? wa : wb;
-        d.isl_k[c] = x == RP_NONE ? d.nb : uf_find(d.root, x);  // nb: no written body (sorts last)
-        d.isl_v[c] = c;
+        d.isl_k[r] = x == RP_NONE ? d.nb : uf_find(d.root, x);  // nb: no written body (sorts last)
+        d.isl_v[r] = c;
     }
 }
 
 // Island segment starts over the sorted (key, contact) list; counts contacts with a writer.
 __global__ void k_island_flags(Dev d, const uint32_t *keys) {
     if (aborted(d)) return;
-    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    const uint32_t nct = min(d.cnt[CT_ROWS], d.cap_contacts);
     GRID_STRIDE(r, d.cap_contacts) {
         if (r >= nct) { d.isl_flag[r] = 0; continue; }
         uint32_t k = keys[r];
@@ -584,7 +646,7 @@ __global__ void k_island_flags(Dev d, const uint32_t *keys) {
 }
 __global__ void k_island_compact(Dev d, const uint32_t *keys) {
     if (aborted(d)) return;
-    const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+    const uint32_t nct = min(d.cnt[CT_ROWS], d.cap_contacts);
     GRID_STRIDE(r, nct) {
         uint32_t k = keys[r];
         if (k != d.nb && (r == 0 || keys[r - 1] != k)) d.isl_start[d.isl_flag[r]] = r;
@@ -726,7 +788,13 @@ __global__ void k_chain_entries(Dev d, const uint32_t *ord, uint32_t *keys, uint
     const uint32_t n2 = d.cnt[CT_CHAIN_N];
     GRID_STRIDE(e, n2) {
         uint32_t c = ord[e >> 1];
-        uint32_t b = (e & 1u) ? d.cs_wb[c] : d.cs_wa[c];
+        uint32_t b;
+        if (c & RP_JBIT) {  // a joint writes each of its dynamic bodies
+            const uint32_t x = (e & 1u) ? d.jt_b[c & ~RP_JBIT] : d.jt_a[c & ~RP_JBIT];
+            b = d.inv_mass[x] > 0.0f ? x : RP_NONE;
+        } else {
+            b = (e & 1u) ? d.cs_wb[c] : d.cs_wa[c];
+        }
         keys[e] = b == RP_NONE ? d.nb : b;
         vals[e] = e;
     }
@@ -770,9 +838,14 @@ __global__ void k_shock_keys(Dev d, TickParams tp, uint32_t *klo, uint32_t *khi,
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     GRID_STRIDE(r, nislc) {
         uint32_t c = d.isl_v2[r];
-        uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
-        float h = b == RP_NONE ? -INFINITY : fminf(dot3(L3(d.pos[a]), up), dot3(L3(d.pos[b]), up));
-        uint64_t k = shock_key44(h);
+        uint64_t k;
+        if (c & RP_JBIT) {
+            k = ((uint64_t)1 << 44) - 1;  // joints take no part in the shock passes: last, no-ops
+        } else {
+            uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+            float h = b == RP_NONE ? -INFINITY : fminf(dot3(L3(d.pos[a]), up), dot3(L3(d.pos[b]), up));
+            k = shock_key44(h);
+        }
         klo[r] = (uint32_t)k;
         khi[r] = (uint32_t)(k >> 32);
         vals[r] = r;
@@ -1116,11 +1189,146 @@ static __device__ __forceinline__ auto make_island(const Dev &d, uint8_t *flag_l
     else return Island<GlbFlags>{GlbFlags{d.g_done + s}, s, n, 0};
 }
 
+// ------------------------------------------------------------------ ball joints
+// A ball joint locks the three translational axes of frame A (articulation/src/bodies.rs:326-331).
+// Velocity level: one equality row per axis, target 0, unbounded (solver.rs:336-424 d6_rows with
+// no limits, drives, damping, friction or armature), warm-started from last tick's rows when the
+// same rows exist (solver.rs:700-715) and stored after the sweeps (solver.rs:945-952).  Position
+// level: solve_d6's translational rows with SHAKE directions (bodies.rs:616-739), one pass per
+// projection iteration before the contacts (step.rs:302-306).  A joint's rows write both its
+// bodies when their inverse mass is positive, asleep or not (joint_row_apply, solver.rs:298-307;
+// apply_position_delta / apply_orientation_delta, dynamics.rs:71-89, 207-210).
+static __device__ __forceinline__ float jfin(float x) { return isfinite(x) ? x : 0.0f; }
+static __device__ __forceinline__ M3 inertia_world_of(M3 il, Q4 q) {
+    M3 r = mquat(q);
+    return mmul(mmul(r, il), mtrans(r));
+}
+// dynamics.rs:193-205 apply_impulse(b, j), then apply_angular_impulse(b, r x j)
+static __device__ __forceinline__ void joint_impulse(const Dev &d, uint32_t b, V3 j, V3 r) {
+    const float im = d.inv_mass[b];
+    if (im > 0.0f && fin3(j)) d.lin[b] = S3(fin_v(L3(d.lin[b]) + j * im));
+    const V3 tau = cross3(r, j);
+    if (im > 0.0f && fin3(tau)) d.ang[b] = S3(fin_v(L3(d.ang[b]) + mv(load_m9(d.iiw + 9 * b), tau)));
+}
+// solver.rs:298-307 joint_row_apply for a translational row
+static __device__ __forceinline__ void joint_apply(const Dev &d, uint32_t a, uint32_t b, V3 ra, V3 rb, V3 u, float lam) {
+    const V3 p = u * lam;
+    if (d.inv_mass[a] > 0.0f) joint_impulse(d, a, -p, ra);
+    if (d.inv_mass[b] > 0.0f) joint_impulse(d, b, p, rb);
+}
+// solver.rs:291-295 joint_row_vel
+static __device__ __forceinline__ float joint_vel(const Dev &d, uint32_t a, uint32_t b, V3 ra, V3 rb, V3 u) {
+    const V3 va = L3(d.lin[a]) + cross3(L3(d.ang[a]), ra);
+    const V3 vb = L3(d.lin[b]) + cross3(L3(d.ang[b]), rb);
+    return dot3(vb - va, u);
+}
+// warm start of joint q's rows
+static __device__ void joint_warm(const Dev &d, uint32_t q) {
+    const uint32_t m = d.jr_mask[q], wm = d.wj_mask[q];
+    if (!m || !(wm >> 31) || (wm & 7u) != m) return;
+    const uint32_t a = d.jt_a[q], b = d.jt_b[q];
+    const V3 ra = L3(d.jr_ra[q]), rb = L3(d.jr_rb[q]);
+    for (uint32_t i = 0; i < 3; i++) {
+        if (!((m >> i) & 1u)) continue;
+        const float lam = d.wj_lam[3 * q + i];  // clamped to the row's (-inf, inf): unchanged
+        d.jr_acc[3 * q + i] = lam;
+        if (lam != 0.0f) joint_apply(d, a, b, ra, rb, L3(d.jr_u[3 * q + i]), lam);
+    }
+}
+// one PGS pass over joint q's rows (solver.rs:822-832); residual |d lambda| K
+static __device__ void joint_pgs(const Dev &d, uint32_t q, RP_LDS uint32_t *resid_bits) {
+    const uint32_t m = d.jr_mask[q], a = d.jt_a[q], b = d.jt_b[q];
+    const V3 ra = L3(d.jr_ra[q]), rb = L3(d.jr_rb[q]);
+    for (uint32_t i = 0; i < 3; i++) {
+        if (!((m >> i) & 1u)) continue;
+        const float4 u4 = d.jr_u[3 * q + i];
+        const V3 u = L3(u4);
+        const float v = joint_vel(d, a, b, ra, rb, u);
+        const float old = d.jr_acc[3 * q + i];
+        const float nw = old + (0.0f - v) * u4.w;
+        d.jr_acc[3 * q + i] = nw;
+        if (nw != old) joint_apply(d, a, b, ra, rb, u, nw - old);
+        if (resid_bits) __hip_atomic_fetch_max(resid_bits, __float_as_uint(fabsf(nw - old) / u4.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+// joint q's rows become its warm rows (a joint without rows keeps its old ones)
+static __device__ __forceinline__ void joint_store(const Dev &d, uint32_t q) {
+    const uint32_t m = d.jr_mask[q];
+    if (!m) return;
+    d.wj_mask[q] = 0x80000000u | m;
+    for (uint32_t i = 0; i < 3; i++) if ((m >> i) & 1u) d.wj_lam[3 * q + i] = d.jr_acc[3 * q + i];
+}
+static __device__ __forceinline__ V3 normalize_fin(V3 v) {  // bodies.rs:19-27
+    v = fin_v(v);
+    const float l2 = len2(v);
+    return l2 > 1.0e-12f ? v * (1.0f / sqrtf(l2)) : splat3(0.0f);
+}
+// dynamics.rs:207-210
+static __device__ __forceinline__ void joint_pos_delta(const Dev &d, uint32_t b, V3 dp) {
+    if (d.inv_mass[b] > 0.0f && fin3(dp)) d.pos[b] = S3(fin_v(L3(d.pos[b]) + dp));
+}
+// dynamics.rs:71-89
+static __device__ __forceinline__ void joint_orient_delta(const Dev &d, uint32_t b, V3 dth) {
+    if (d.inv_mass[b] <= 0.0f || !fin3(dth)) return;
+    const V3 x = fin_v(dth);
+    if (len2(x) <= 1.0e-24f) return;
+    const Q4 dq = qnormalize(mkq(x.x * 0.5f, x.y * 0.5f, x.z * 0.5f, 1.0f));
+    d.rot[b] = SQ(fin_q(qnormalize(qmul(dq, LQ(d.rot[b])))));
+}
+// One position pass over joint q (solve_step_with, bodies.rs:422-460, one iteration): skipped when
+// both bodies are static or asleep; SHAKE directions from the step's start poses (the poses before
+// integration) unless a body turned by 0.5 rad or more since.
+static __device__ void joint_pos(const Dev &d, uint32_t q) {
+    const uint32_t a = d.jt_a[q], b = d.jt_b[q];
+    const bool fa = !((d.bflags[a] & BF_DYN) && d.awake[a]), fb = !((d.bflags[b] & BF_DYN) && d.awake[b]);
+    if (fa && fb) return;
+    const float wa = jfin(d.inv_mass[a]), wb = jfin(d.inv_mass[b]);
+    const Q4 sa = LQ(d.orot[a]), sb = LQ(d.orot[b]);
+    const float cmin = __uint_as_float(RP_SHAKE_COS_BITS);
+    const bool shake = fminf(fabsf(qmul(sa, qconj(LQ(d.rot[a]))).w), 1.0f) >= cmin && fminf(fabsf(qmul(sb, qconj(LQ(d.rot[b]))).w), 1.0f) >= cmin;
+    const Q4 fra = LQ(d.jt_far[q]);
+    const V3 fap = L3(d.jt_fap[q]), fbp = L3(d.jt_fbp[q]);
+    V3 la0 = splat3(0.0f), lb0 = splat3(0.0f);
+    M3 ia0 = mzero(), ib0 = mzero();
+    Q4 ra0 = qid();
+    if (shake) {
+        ra0 = qmul(sa, fra);
+        la0 = fin_v(qrot(sa, fap)); lb0 = fin_v(qrot(sb, fbp));
+        // bodies.rs:569-573 inv_inertia_at (the local inverse inertia of a static body is zero)
+        ia0 = inertia_world_of(d.inv_mass[a] == 0.0f ? mzero() : load_m9(d.iil + 9 * a), sa);
+        ib0 = inertia_world_of(d.inv_mass[b] == 0.0f ? mzero() : load_m9(d.iil + 9 * b), sb);
+    }
+    for (uint32_t i = 0; i < 3; i++) {
+        const V3 pa = L3(d.pos[a]), pb = L3(d.pos[b]);
+        const Q4 qa = LQ(d.rot[a]), qb = LQ(d.rot[b]);
+        const V3 u = normalize_fin(qrot(qmul(qa, fra), unit_axis(i)));
+        if (zero3(u)) continue;
+        const V3 xa = fin_v(pa + qrot(qa, fap)), xb = fin_v(pb + qrot(qb, fbp));
+        const V3 dd = fin_v(xb - xa);
+        V3 un, lva, lvb; M3 iaw, ibw;
+        if (shake) { un = normalize_fin(qrot(ra0, unit_axis(i))); lva = la0; lvb = lb0; iaw = ia0; ibw = ib0; }
+        else { un = u; lva = xa - pa; lvb = xb - pb; iaw = inv_inertia_world(d, a, qa); ibw = inv_inertia_world(d, b, qb); }
+        float c = jfin(jfin(dot3(dd, u)) - 0.0f);
+        const V3 rau = fin_v(cross3(lva, un)), rbu = fin_v(cross3(lvb, un));
+        const float wra = jfin(dot3(rau, mv(iaw, rau))), wrb = jfin(dot3(rbu, mv(ibw, rbu)));
+        const float wsum = jfin(((wa + wb) + wra) + wrb);
+        const float denom = jfin(wsum + 0.0f);
+        if (denom <= 0.0f) continue;
+        c = jfin(c + 0.0f);
+        const float dl = jfin(-(c + 0.0f) / denom);
+        joint_pos_delta(d, a, fin_v((-un) * jfin(dl * wa)));
+        joint_pos_delta(d, b, fin_v(un * jfin(dl * wb)));
+        joint_orient_delta(d, a, fin_v((-mv(iaw, rau)) * dl));
+        joint_orient_delta(d, b, fin_v(mv(ibw, rbu) * dl));
+    }
+}
+
 // Warm-start application to contact c (solver.rs:749-760): the previous frame's impulses (mode 0,
 // the reference), nothing (mode 1), or the previous frame's plus the model's correction where the
 // contact passed the model's gate (mode 2; other contacts take the reference's warm start, and a
 // zero correction leaves every operation of the reference's).
 static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &tp, uint32_t c) {
+    if (c & RP_JBIT) { joint_warm(d, c & ~RP_JBIT); return; }
     const uint32_t cur = tp.cur;
     if (tp.ws_mode == 2) {
         const int wi = d.warm_idx[c];
@@ -1177,12 +1385,14 @@ static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &
 struct PgsIn { Row x; float jn; float2 jt; };
 static __device__ __forceinline__ PgsIn pgs_fetch(const Dev &d, uint32_t c, uint32_t slot) {
     PgsIn in;
+    if (c & RP_JBIT) { in.jn = 0.0f; in.jt = make_float2(0.0f, 0.0f); in.x.ai = RP_NONE; in.x.bi = RP_NONE; return in; }
     in.x = fetch_row(d, slot);
     in.jn = d.jn[c];
     in.jt = d.jt[c];
     return in;
 }
 static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, RP_LDS uint32_t *resid_bits) {
+    if (c & RP_JBIT) { joint_pgs(d, c & ~RP_JBIT, resid_bits); return; }
     const Row &x = in.x;
     Vel2 v;
     v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
@@ -1219,6 +1429,7 @@ static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const P
 
 // Velocity shock row of contact c (solver.rs:976-993): the lower body along up is held fixed.
 static __device__ __forceinline__ void vshock_row(const Dev &d, const V3 up, uint32_t c) {
+    if (c & RP_JBIT) return;
     uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
     V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
     float2 inv = d.cs_inv[c];
@@ -1258,7 +1469,11 @@ static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> 
                                  uint32_t klsh) {
     const uint32_t n = I.n;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.jn_pre[lst[r]] = d.jn[lst[r]];
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
+        const uint32_t c = lst[r];
+        if (c & RP_JBIT) joint_store(d, c & ~RP_JBIT);
+        else d.jn_pre[c] = d.jn[c];
+    }
     __syncthreads();
     shock_slots<TIER>(d, I, isl);
     if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
@@ -1307,7 +1522,10 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], tp.iv); atomicAdd(&d.cnt[CT_ITERS_SUM], tp.iv * n); }
             continue;
         }
-        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_row(d, s + j, load_row(d, d.bs_c[s + j]));
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+            const uint32_t c = d.bs_c[s + j];
+            if (!(c & RP_JBIT)) store_row(d, s + j, load_row(d, c));
+        }
         __syncthreads();
         // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
         __shared__ uint32_t resid_bits;
@@ -1414,7 +1632,10 @@ __global__ void __launch_bounds__(256) k_level_vel(Dev d, uint32_t L) {
 // The constant rows of every big island's slots (split form), across the GPU.
 __global__ void __launch_bounds__(256) k_store_rows(Dev d) {
     if (aborted(d)) return;
-    for_big_island_slots(d, [&](uint32_t slot) { store_row(d, slot, load_row(d, d.bs_c[slot])); });
+    for_big_island_slots(d, [&](uint32_t slot) {
+        const uint32_t c = d.bs_c[slot];
+        if (!(c & RP_JBIT)) store_row(d, slot, load_row(d, c));
+    });
 }
 // Warm start of the split form: wide level L across the GPU, then the tail in one workgroup.
 __global__ void __launch_bounds__(256) k_level_warm(Dev d, TickParams tp, uint32_t L) {
@@ -1542,7 +1763,7 @@ __global__ void k_ball_init(Dev d) {
     if (aborted(d)) return;
     GRID_STRIDE(b, d.nb) {
         uint32_t f = d.flags[b];
-        d.ballf[b] = (!(f & FL_CONSTRAINED) || !(f & FL_LOADED)) ? 1u : 0u;
+        d.ballf[b] = (!(f & FL_CONSTRAINED) || (!(f & FL_JOINTED) && !(f & FL_LOADED))) ? 1u : 0u;
     }
 }
 
@@ -1554,6 +1775,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_ball(Dev d, TickP
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
         run_sweep(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, [&](uint32_t c) {
+            if (c & RP_JBIT) return;
             uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
             bool hb = b != RP_NONE;
             bool ba = d.ballf[a] != 0, bb = hb && d.ballf[b] != 0;
@@ -1646,6 +1868,7 @@ __global__ void k_posc_prep(Dev d) {
     const uint32_t nislc = d.cnt[CT_ISLC];
     GRID_STRIDE(r, nislc) {
         uint32_t c = d.isl_v2[r];
+        if (c & RP_JBIT) continue;
         uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
         V3 point = L3(d.ct_p[c]);
         V3 la = qrot(qconj(LQ(d.orot[a])), point - L3(d.opos[a]));
@@ -1667,6 +1890,11 @@ struct PRow {
 };
 static __device__ __forceinline__ PRow load_prow(const Dev &d, uint32_t c) {
     PRow x;
+    if (c & RP_JBIT) {  // a joint's position pass: only the entry travels
+        x.a = c; x.b = RP_NONE; x.am = x.bm = false; x.la4 = make_float4(0, 0, 0, 0); x.lb = x.n = x.pt = splat3(0.0f);
+        x.ima = x.imb = 0.0f; x.ila = x.ilb = mzero();
+        return x;
+    }
     x.a = d.cs_ai[c]; x.b = d.cs_bi[c];
     const bool hb = x.b != RP_NONE;
     x.ima = d.inv_mass[x.a];
@@ -1703,11 +1931,8 @@ static __device__ __forceinline__ PRow fetch_prow(const Dev &d, uint32_t slot) {
     x.ilb.c0 = mk3(f6.w, f7.x, f7.y); x.ilb.c1 = mk3(f7.z, f7.w, f8.x); x.ilb.c2 = mk3(f8.y, f8.z, f8.w);
     return x;
 }
-static __device__ __forceinline__ M3 inertia_world_of(M3 il, Q4 q) {
-    M3 r = mquat(q);
-    return mmul(mmul(r, il), mtrans(r));
-}
 static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 up) {
+    if (x.a & RP_JBIT) { if (!shock) joint_pos(d, x.a & ~RP_JBIT); return; }
     const uint32_t a = x.a, b = x.b;
     const bool hb = b != RP_NONE;
     bool am = x.am, bm = x.bm;
@@ -1830,7 +2055,9 @@ __global__ void k_s6(Dev d, TickParams tp) {
             V3 vl = L3(d.lin[i]), va = L3(d.ang[i]);
             V3 np = L3(d.pos[i]); Q4 nq = LQ(d.rot[i]);
             Q4 pq = LQ(d.prot[i]);
-            V3 dl = (np - L3(d.ppos[i])) - L3(d.slin[i]);
+            // no split impulse on a jointed body (step.rs:322-327)
+            const bool jb = (d.flags[i] & FL_JOINTED) != 0;
+            V3 dl = (np - L3(d.ppos[i])) - (jb ? splat3(0.0f) : L3(d.slin[i]));
             if (!zero3(dl)) vl = vl + dl / tp.dt;
             if ((d.flags[i] & FL_GROUND) && vl.y < 0.0f) {
                 vl.y = 0.0f;
@@ -1841,7 +2068,7 @@ __global__ void k_s6(Dev d, TickParams tp) {
                 Q4 dq = qmul(nq, qconj(pq));
                 V3 da = mk3(dq.x, dq.y, dq.z) * 2.0f;
                 if (dq.w < 0.0f) da = -da;
-                va = va + (da - L3(d.sang[i])) / tp.dt;
+                va = va + (da - (jb ? splat3(0.0f) : L3(d.sang[i]))) / tp.dt;
             }
             set_vel(d, i, vl, va);
         }
@@ -1875,7 +2102,9 @@ __global__ void k_check(Dev d, uint32_t which) {
         if (d.cnt[CT_PAIRS] > d.cap_pairs) { d.cnt[CT_NEED_PAIRS] = d.cnt[CT_PAIRS]; d.cnt[CT_ABORT] = 1; }
         if (d.cnt[CT_CANDS] > d.cap_cands) { d.cnt[CT_NEED_CANDS] = d.cnt[CT_CANDS]; d.cnt[CT_ABORT] = 1; }
     } else if (which == 1) {
-        if (d.cnt[CT_CONTACTS] > d.cap_contacts) { d.cnt[CT_NEED_CONTACTS] = d.cnt[CT_CONTACTS]; d.cnt[CT_ABORT] = 1; }
+        // the island lists hold the joints too
+        d.cnt[CT_ROWS] = d.cnt[CT_CONTACTS] + d.nj;
+        if (d.cnt[CT_ROWS] > d.cap_contacts) { d.cnt[CT_NEED_CONTACTS] = d.cnt[CT_ROWS]; d.cnt[CT_ABORT] = 1; }
     } else if (which == 2) {
         uint32_t wt = d.cnt[CT_NEW_TOTAL] + d.cnt[CT_KEEP_TOTAL];
         if (wt > d.cap_warm) { d.cnt[CT_NEED_WARM] = wt; d.cnt[CT_ABORT] = 1; }
@@ -2025,11 +2254,12 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_contact_write, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     mark(RP_ST_PREP);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_solve, d.nw);
+    if (d.nj) hipLaunchKernelGGL(k_joint_prep, dim3(grid_for(d.nj, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_island_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     const uint32_t body_bits = key_bits(d.nb);  // body ids and the nb sentinel
     uint32_t *ks, *vs;
-    radix_sort(d, d.isl_k, d.isl_v, d.isl_k2, d.isl_v2, d.cnt + CT_CONTACTS, d.cap_contacts, body_bits, st, &ks, &vs);
+    radix_sort(d, d.isl_k, d.isl_v, d.isl_k2, d.isl_v2, d.cnt + CT_ROWS, d.cap_contacts, body_bits, st, &ks, &vs);
     if (vs != d.isl_v2) {  // the sweeps read (isl_k2, isl_v2)
         hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, vs, d.isl_v2, d.cap_contacts);
         hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, ks, d.isl_k2, d.cap_contacts);

@@ -143,6 +143,8 @@ EXPORTS = [
     ("rphys_world_snapshot_free", None, [P]),
     ("rphys_add_body_density", U32, [P, U32, PF32, C.POINTER(Isometry), C.POINTER(Velocity), U32, F32, PU32]),
     ("rphys_world_set_pair_props", U32, [P, U32, U32, F32, F32, F32, F32]),
+    ("rphys_world_add_ball_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), PU32]),
+    ("rphys_world_num_joints", U32, [P]),
     ("rphys_world_set_solver_iterations", U32, [P, U32, U32]),
     ("rphys_world_set_gravity", U32, [P, Vec3]),
     ("rphys_world_set_residual_mode", U32, [P, U32, F32, U32]),
@@ -303,8 +305,10 @@ class World:
         if s.get("residual"):  # (tol, max_iters): each island iterates to the residual
             tol, mx = s["residual"]
             check(L.rphys_world_set_residual_mode(self.h, 1, tol, mx))
-        for i in range(len(s["shape"])):
-            self.add_body(int(s["shape"][i]), s["dims"][i], s["pose"][i], s["vel"][i], int(s["dynamic"][i]), float(s["density"][i]))
+        ids = [self.add_body(int(s["shape"][i]), s["dims"][i], s["pose"][i], s["vel"][i], int(s["dynamic"][i]), float(s["density"][i]))
+               for i in range(len(s["shape"]))]
+        for a, b, fa, fb in s.get("joints", []):  # scene joints name bodies by scene index
+            self.add_ball_joint(ids[a], ids[b], fa, fb)
         hf = s.get("hf")
         if hf is not None:
             self.set_heightfield(hf)
@@ -315,6 +319,14 @@ class World:
         out = C.c_uint32(0)
         check(lib().rphys_add_body_density(self.h, shape, d.ctypes.data_as(PF32), C.byref(iso(pose[:3], pose[3:])),
                                            C.byref(vel(v[:3], v[3:])), BODY_DYNAMIC if dynamic else BODY_STATIC, density, C.byref(out)))
+        return out.value
+
+    def add_ball_joint(self, body_a, body_b, frame_a, frame_b):
+        """Ball joint between two body ids, frames (px, py, pz, qx, qy, qz, qw) in each body's space;
+        returns the joint index."""
+        out = C.c_uint32(0)
+        check(lib().rphys_world_add_ball_joint(self.h, body_a, body_b, C.byref(iso(frame_a[:3], frame_a[3:])),
+                                               C.byref(iso(frame_b[:3], frame_b[3:])), C.byref(out)))
         return out.value
 
     def set_heightfield(self, hf):

@@ -175,3 +175,30 @@ def test_queries_are_declared_out_of_scope():
     assert L.rphys_world_raycast(w, q, hit) == rphys.ERR_INTERNAL
     assert "not part" in last_error()
     L.rphys_world_destroy(w)
+
+
+def test_ball_joint_export_validates_its_arguments():
+    L = rphys.lib()
+    w = L.rphys_world_create(4, 4)
+    try:
+        a, b = C.c_uint32(), C.c_uint32()
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(a)) == rphys.OK
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(b)) == rphys.OK
+        fa, fb = rphys.iso((0, 0.5, 0), (0, 0, 0, 1)), rphys.iso((0, -0.5, 0), (0, 0, 0, 1))
+        j = C.c_uint32(7)
+        assert L.rphys_world_add_ball_joint(None, a, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.ERR_NULL
+        assert L.rphys_world_add_ball_joint(w, a, b, None, C.byref(fb), C.byref(j)) == rphys.ERR_NULL
+        assert L.rphys_world_add_ball_joint(w, 0, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.ERR_INVALID_ID
+        assert j.value == 0xFFFFFFFF
+        assert L.rphys_world_add_ball_joint(w, rphys.TERRAIN_BODY_ID, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.ERR_INVALID_ID
+        assert L.rphys_world_add_ball_joint(w, a, a, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.ERR_BAD_ARG
+        bad = rphys.iso((0, float("nan"), 0), (0, 0, 0, 1))
+        assert L.rphys_world_add_ball_joint(w, a, b, C.byref(bad), C.byref(fb), C.byref(j)) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_num_joints(w) == 0
+        assert L.rphys_world_add_ball_joint(w, a, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.OK and j.value == 0
+        assert L.rphys_world_add_ball_joint(w, b, a, C.byref(fb), C.byref(fa), C.byref(j)) == rphys.OK and j.value == 1
+        assert L.rphys_world_num_joints(w) == 2
+        assert L.rphys_body_remove(w, a.value) == rphys.OK
+        assert L.rphys_world_add_ball_joint(w, a, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.ERR_INVALID_ID
+    finally:
+        L.rphys_world_destroy(w)

@@ -1,0 +1,98 @@
+"""Ball joints: the device step vs the oracle on scenes with capsule chains (bit for bit), residual
+mode with joints, the batch with joint worlds (graph, rollback) and the at-scale workload's shape."""
+import numpy as np
+import pytest
+
+import oracle_py
+import rphys
+import scenes
+from test_gpu_parity import assert_state_close, contacts_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def chains(seed=11, **kw):
+    return scenes.ragdolls_small(seed=seed, **kw)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_chain_scene_bit_exact(seed):
+    s = chains(seed)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    assert o.hash() == g.hash()
+    for t in range(150):
+        o.step()
+        g.step()
+        if t < 40 or t % 10 == 9:
+            np.testing.assert_array_equal(o.pairs(), g.pairs(), err_msg="pairs at tick %d" % (t + 1))
+            contacts_equal(o.contacts(), g.contacts())
+            assert_state_close(o, g)
+            assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+
+
+def test_chain_scene_iters_20_20_bit_exact():
+    s = chains(13)
+    s["iters"] = (20, 20)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(80):
+        o.step()
+        g.step()
+    assert_state_close(o, g)
+    assert o.hash() == g.hash()
+
+
+def test_chain_scene_residual_mode_bit_exact():
+    s = chains(14)
+    s["residual"] = (1.0e-4, 60)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(60):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+        so, sg = o.stats(), g.stats()
+        assert (so.iters_used, so.iters_sum) == (sg.pgs_iters, sg.pgs_iters_sum), (t + 1, so.iters_used, sg.pgs_iters)
+
+
+def test_towers_and_chains_medium_bit_exact():
+    s = scenes.towers_ragdolls(n_towers=36, tower_h=12, n_loose=4, n_ragdolls=24, seed=21, hf_n=96, extent=30.0, margin=8.0, drop=4.0)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(90):
+        o.step()
+        g.step()
+        if t % 15 == 14:
+            assert_state_close(o, g)
+            assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+
+
+def test_joint_batch_matches_single_worlds_graph_and_rollback():
+    ss = [chains(30 + i, n_ragdolls=3 + i) for i in range(3)]
+    singles = [rphys.World(s) for s in ss]
+    b = rphys.Batch([rphys.World(s) for s in ss])
+    for _ in range(20):
+        for w in singles:
+            w.step()
+    b.step_n(20, graph=True)
+    assert b.hashes() == [w.hash() for w in singles]
+    b.snapshot(0)
+    b.step_n(8, graph=True)
+    h1 = b.hashes()
+    b.restore(0)
+    b.step_n(8, graph=True)
+    assert b.hashes() == h1
+
+
+def test_world_snapshot_restore_with_joints_replays():
+    s = chains(15)
+    g = rphys.World(s)
+    g.step_n(25)
+    snap = g.snapshot()
+    runs = []
+    for _ in range(2):
+        g.restore(snap)
+        hs = []
+        for _ in range(6):
+            g.step()
+            hs.append(g.hash())
+        runs.append(hs)
+    g.free_snapshot(snap)
+    assert runs[0] == runs[1]

@@ -37,15 +37,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 #                    inverse mass/inertia 56); its shock pass runs in a launch of its own
 #   body-pair narrowphase : per pair, 88 B (two poses + shapes)
 #   terrain narrowphase   : per (collider, triangle) candidate, 16 B (height samples)
+#   ball joints (an island entry each, in the small islands at these sizes): per joint, 3 rows x
+#                    (Iv x 96 B velocity, Ip x 120 B position)
 
 
 def stage_bytes(stage, ctr, iv, ip):
     c_large = ctr["island_contacts_global"]  # islands of 4,097..131,072 contacts
-    c_small = ctr["island_contacts"] - c_large  # (islands above 131,072 contacts: none at these sizes)
+    j = ctr.get("joint_entries", 0)
+    c_small = ctr["island_contacts"] - c_large - j  # (islands above 131,072 contacts: none at these sizes)
     table = {
-        "vel_solve_small": (iv * 260 + 230) * c_small,
+        "vel_solve_small": (iv * 260 + 230) * c_small + iv * 96 * 3 * j,
         "vel_solve_large": (iv * 260 + 230) * c_large,
-        "projection_small": ip * 230 * c_small,
+        "projection_small": ip * 230 * c_small + ip * 120 * 3 * j,
         "projection_large": ip * 230 * c_large,
         "narrow_pairs": 88 * ctr["pairs"],
         "narrow_terrain": 16 * ctr["terrain_candidates"],
@@ -259,6 +262,99 @@ def batch_cpu_baseline(worlds, warmup, ticks, dt):
             % (len(sample), len(worlds), max(1, len(worlds) // 8), span, warmup, t_spent, cpu_model(), os.cpu_count() or 0)}
 
 
+def towers_bench(a):
+    """One GPU's share of the at-scale workload: `--worlds` (default 8 of 64) worlds of 16,384 bodies,
+    669 towers of 20 boxes, 4 loose boxes and 200 chains of 15 capsules joined by 14 ball joints
+    each, over fBm terrain, velocity iterations 10, position iterations 2.  Timed: graph-captured
+    ticks with the fixed iterations; then, from one snapshot, the PGS run to residual 1e-4 per island
+    with the previous-frame and the zero warm start (sweeps per contact); the oracle on one world
+    for the CPU baseline."""
+    t0 = time.perf_counter()
+    worlds = [scenes.towers_ragdolls(seed=scenes.seed_for(5, w)) for w in range(a.worlds)]
+    b = rphys.Batch([rphys.World(s_) for s_ in worlds])
+    nb = b.total_bodies()
+    dt = float(worlds[0].get("dt", 1.0 / 60.0))
+    iv, ip = worlds[0].get("iters", (10, 2))
+    nj = sum(len(s_["joints"]) for s_ in worlds)
+    wl = {"workload": "towers_chains%d_x16k" % a.worlds, "worlds": a.worlds, "bodies": nb, "ball_joints": nj,
+          "world": "669 towers x 20 boxes (he 0.25), 4 loose boxes, 200 chains x 15 capsules (r 0.1, hh 0.15), 256x256 fBm",
+          "dt": "1/60", "vel_iters": iv, "pos_iters": ip, "graphs": not a.no_graph}
+    print("bench: %s, %d bodies, %d joints, built in %.1f s; warmup %d ticks" % (wl["workload"], nb, nj, time.perf_counter() - t0, a.warmup),
+          file=sys.stderr, flush=True)
+    for k in range(0, a.warmup, 30):
+        b.step_n(min(30, a.warmup - k), dt, graph=not a.no_graph)
+        b.hashes()
+        print("bench: warmup %d/%d ticks" % (min(k + 30, a.warmup), a.warmup), file=sys.stderr, flush=True)
+    tuning = None
+    if not a.no_graph:  # launch shapes of the island sweeps, as for the batched workload
+        t1 = time.perf_counter()
+        entry, cached = b.autotune(wl["workload"], ticks=2, dt=dt)
+        tuning = dict(entry, cached=cached, search_s=time.perf_counter() - t1)
+        print("bench: tuning %s (%s, %.1f s)" % (b.get_tuning(), "cached" if cached else "searched", tuning["search_s"]), file=sys.stderr, flush=True)
+    wl["tuning"] = tuning
+    b.snapshot(1)
+    el, done = 0.0, 0
+    while done < a.steps:
+        k = min(20, a.steps - done)
+        t1 = time.perf_counter()
+        b.step_n(k, dt, graph=not a.no_graph)
+        b.hashes()
+        el += time.perf_counter() - t1
+        done += k
+        print("bench: %d/%d ticks, %.2f s" % (done, a.steps, el), file=sys.stderr, flush=True)
+    value = nb * a.steps / el
+    b.restore(1)
+    stages, ctr = {}, {}
+    for k in range(0, min(a.steps, 40), 10):
+        merge_profile(stages, ctr, *b.step_profiled(10, dt))
+    ctr["joint_entries"] = nj * ctr["ticks"]  # every joint of these worlds is active and has its 3 rows
+    if a.stages:
+        print_stages(stages, ctr)
+    # PGS to residual 1e-4 from the same state: previous-frame vs zero warm start
+    tol, max_it = 1.0e-4, a.max_sweeps
+    b.set_residual_mode(True, tol, max_it)
+    resid = {}
+    for name, mode in (("previous_frame", 0), ("zero", 1)):
+        b.restore(1)
+        b.set_warm_start(mode)
+        el2, isum, ic, conv, cts = 0.0, 0.0, 0.0, 0.0, 0.0
+        for k in range(0, 20, 10):
+            t1 = time.perf_counter()
+            _, c = b.step_profiled(10, dt)
+            el2 += time.perf_counter() - t1
+            isum += c["pgs_iters_sum"]; ic += c["island_contacts"]; conv += c["island_contacts_converged"]; cts += c["contacts"]
+        resid[name] = {"body_steps_per_s": nb * 20 / el2, "mean_sweeps_per_entry": isum / max(ic, 1), "converged_entry_frac": conv / max(ic, 1),
+                       "contacts_per_tick": cts / 20}
+        print("bench: residual 1e-4, %s warm start: %.0f body-steps/s, %.2f sweeps per island entry, %.3f converged" % (
+            name, resid[name]["body_steps_per_s"], resid[name]["mean_sweeps_per_entry"], resid[name]["converged_entry_frac"]), file=sys.stderr, flush=True)
+    b.set_residual_mode(False)
+    b.set_warm_start(0)
+    wl.update({"contacts_per_tick": ctr["contacts"] / ctr["ticks"], "ticks": "%d..%d" % (a.warmup, a.warmup + a.steps),
+               "residual_1e-4": resid, "max_sweeps": max_it})
+    base = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_py
+        o = oracle_py.OracleWorld(worlds[0])
+        t1 = time.perf_counter()
+        for k in range(a.warmup):
+            o.step(dt)
+            if k % 30 == 29:
+                print("bench: oracle warmup %d/%d ticks, %.1f s" % (k + 1, a.warmup, time.perf_counter() - t1), file=sys.stderr, flush=True)
+        span = max(1, a.cpu_ticks)
+        t1 = time.perf_counter()
+        for _ in range(span):
+            o.step(dt)
+        t_spent = time.perf_counter() - t1
+        base = {"value": len(worlds[0]["shape"]) * span / t_spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                "sample": "world 0 (16,384 bodies), ticks %d..%d after %d warmup ticks (%.1f s; %s, %d host cores, 1 used)"
+                % (a.warmup, a.warmup + span, a.warmup, t_spent, cpu_model(), os.cpu_count() or 0)}
+    return {"metric": "body-steps/s on 1 GPU (at-scale share: towers and jointed chains)", "value": value, "unit": "body-steps/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1000.0 * el / a.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded XorShift64 scenes)", "config": wl,
+            "roofline": roofline_of(stages, ctr, iv, ip, wl["workload"]), "cpu_baseline": base}
+
+
 def learned_bench(a):
     """Learned + batched vs plain, PGS to residual 1e-4 (per island, at most `--max-sweeps`).
 
@@ -370,7 +466,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch", "learned"], default="mixed16k")
+    ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch", "learned", "towers"], default="mixed16k")
     ap.add_argument("--train-rows", type=int, default=10_000_000, help="learned workload: contacts logged for training")
     ap.add_argument("--epochs", type=int, default=20, help="learned workload: training epochs")
     ap.add_argument("--max-sweeps", type=int, default=100, help="learned workload: PGS sweep limit per island in residual mode")
@@ -398,6 +494,15 @@ def main():
         if a.warmup == 10:
             a.warmup = 150
         print(json.dumps(learned_bench(a)), flush=True)
+        return
+    if a.workload == "towers":
+        if a.steps == 600:
+            a.steps = 60
+        if a.warmup == 10:
+            a.warmup = 120
+        if a.worlds == 1024:
+            a.worlds = 8
+        print(json.dumps(towers_bench(a)), flush=True)
         return
     if a.workload == "batch":
         if a.steps == 600:

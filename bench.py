@@ -267,8 +267,9 @@ def towers_bench(a):
     669 towers of 20 boxes, 4 loose boxes and 200 chains of 15 capsules joined by 14 ball joints
     each, over fBm terrain, velocity iterations 10, position iterations 2.  Timed: graph-captured
     ticks with the fixed iterations; then, from one snapshot, the PGS run to residual 1e-4 per island
-    with the previous-frame and the zero warm start (sweeps per contact); the oracle on one world
-    for the CPU baseline."""
+    with the previous-frame, the zero and (with --learned) the learned warm start, the model trained
+    on the solver's converged impulses logged from 4 other worlds of the same generator; the oracle
+    on one world for the CPU baseline."""
     t0 = time.perf_counter()
     worlds = [scenes.towers_ragdolls(seed=scenes.seed_for(5, w)) for w in range(a.worlds)]
     b = rphys.Batch([rphys.World(s_) for s_ in worlds])
@@ -310,27 +311,51 @@ def towers_bench(a):
     ctr["joint_entries"] = nj * ctr["ticks"]  # every joint of these worlds is active and has its 3 rows
     if a.stages:
         print_stages(stages, ctr)
-    # PGS to residual 1e-4 from the same state: previous-frame vs zero warm start
+    # PGS to residual 1e-4 from the same state: previous-frame vs zero (vs learned) warm start
     tol, max_it = 1.0e-4, a.max_sweeps
+    params, training = None, None
+    if a.learned:
+        t1 = time.perf_counter()
+        tw = [scenes.towers_ragdolls(seed=scenes.seed_for(5, 100 + w)) for w in range(4)]
+        tb = rphys.Batch([rphys.World(s_) for s_ in tw])
+        for k in range(0, a.warmup, 30):
+            tb.step_n(min(30, a.warmup - k), dt, graph=not a.no_graph)
+            tb.hashes()
+        tb.set_residual_mode(True, tol, max_it)
+        tb.set_contact_log(a.train_rows + 400000)
+        rows = 0
+        while rows < a.train_rows:
+            tb.step_n(2, dt, graph=not a.no_graph)
+            rows = tb.contact_log_rows()[0]
+            print("bench: training data %d rows, %.1f s" % (rows, time.perf_counter() - t1), file=sys.stderr, flush=True)
+        t2 = time.perf_counter()
+        params, losses = tb.train_warm_start(epochs=a.epochs, lr=1.0e-3, seed=20260101)
+        training = {"worlds": "4 towers-and-chains worlds (seeds of worlds 100..103), ticks %d on" % a.warmup, "rows": rows,
+                    "epochs": a.epochs, "lr": 1e-3, "loss_first_last": [float(losses[0]), float(losses[-1])],
+                    "data_s": t2 - t1, "train_s": time.perf_counter() - t2}
+        print("bench: trained %d epochs in %.1f s, loss %.4f -> %.4f" % (a.epochs, training["train_s"], losses[0], losses[-1]), file=sys.stderr, flush=True)
+        del tb
     b.set_residual_mode(True, tol, max_it)
     resid = {}
-    for name, mode in (("previous_frame", 0), ("zero", 1)):
+    for name, mode in (("previous_frame", 0), ("zero", 1)) + ((("learned", 2),) if params is not None else ()):
         b.restore(1)
-        b.set_warm_start(mode)
-        el2, isum, ic, conv, cts = 0.0, 0.0, 0.0, 0.0, 0.0
+        b.set_warm_start(mode, params if mode == 2 else None)
+        el2, isum, ic, conv, cts, lrn = 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
         for k in range(0, 20, 10):
             t1 = time.perf_counter()
             _, c = b.step_profiled(10, dt)
             el2 += time.perf_counter() - t1
             isum += c["pgs_iters_sum"]; ic += c["island_contacts"]; conv += c["island_contacts_converged"]; cts += c["contacts"]
+            lrn += c["warm_start_learned"]
         resid[name] = {"body_steps_per_s": nb * 20 / el2, "mean_sweeps_per_entry": isum / max(ic, 1), "converged_entry_frac": conv / max(ic, 1),
-                       "contacts_per_tick": cts / 20}
-        print("bench: residual 1e-4, %s warm start: %.0f body-steps/s, %.2f sweeps per island entry, %.3f converged" % (
-            name, resid[name]["body_steps_per_s"], resid[name]["mean_sweeps_per_entry"], resid[name]["converged_entry_frac"]), file=sys.stderr, flush=True)
+                       "contacts_per_tick": cts / 20, "learned_contact_frac": lrn / max(cts, 1)}
+        print("bench: residual 1e-4, %s warm start: %.0f body-steps/s, %.2f sweeps per island entry, %.3f converged, %.3f learned" % (
+            name, resid[name]["body_steps_per_s"], resid[name]["mean_sweeps_per_entry"], resid[name]["converged_entry_frac"],
+            resid[name]["learned_contact_frac"]), file=sys.stderr, flush=True)
     b.set_residual_mode(False)
     b.set_warm_start(0)
     wl.update({"contacts_per_tick": ctr["contacts"] / ctr["ticks"], "ticks": "%d..%d" % (a.warmup, a.warmup + a.steps),
-               "residual_1e-4": resid, "max_sweeps": max_it})
+               "residual_1e-4": resid, "max_sweeps": max_it, "training": training})
     base = None
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -476,6 +501,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="step on the stream instead of graph-captured ticks (profiling runs: rocprofv3 kernel tracing of graph launches crashed the profiled process)")
     ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
+    ap.add_argument("--learned", action="store_true", help="towers workload: also train and compare the learned warm start")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (bandwidth reference, counter calibration)")
     a = ap.parse_args()
@@ -502,6 +528,8 @@ def main():
             a.warmup = 120
         if a.worlds == 1024:
             a.worlds = 8
+        if a.train_rows == 10_000_000:
+            a.train_rows = 3_000_000
         print(json.dumps(towers_bench(a)), flush=True)
         return
     if a.workload == "batch":

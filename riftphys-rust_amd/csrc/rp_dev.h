@@ -40,6 +40,7 @@ enum {
     CT_WSH_MAX,                  // most leading wide levels of one island's shock order
     CT_TIER_N, CT_TIER_N1, CT_TIER_N2,  // islands per sweep tier
     CT_ROWS,                     // island-list entries before the sort: joints + contacts
+    CT_LOG_N,                    // contacts appended to the contact log this tick
     // sums over the completed ticks since the counters were last cleared
     CT_SUM_PAIRS, CT_SUM_CANDS, CT_SUM_CONTACTS, CT_SUM_ISLC, CT_SUM_ISLANDS, CT_SUM_ISLC_GLB, CT_SUM_ITERS_MAX, CT_SUM_ITERS_SUM,
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,

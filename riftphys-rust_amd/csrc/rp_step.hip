@@ -745,6 +745,13 @@ static __device__ __forceinline__ V3 warm_base(const Dev &d, uint32_t c, uint32_
     const V3 ejt = L3(d.wk_jt[cur][wi]);
     return mk3(d.wk_jn[cur][wi], dot3(ejt, L3(d.cs_t1[c])), dot3(ejt, L3(d.cs_t2[c])));
 }
+// The contact entries of the island lists (joint entries are not logged), flagged for an
+// exclusive scan that gives each its log row; newflag is free until the warm-cache store.
+__global__ void k_log_flags(Dev d) {
+    if (aborted(d)) return;
+    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    GRID_STRIDE(r, d.cap_contacts) d.newflag[r] = (r < nislc && !(d.isl_v2[r] & RP_JBIT)) ? 1u : 0u;
+}
 // Appends the solved contacts of this tick (island-list order) with their converged impulses,
 // the normal impulse before the shock pass and the friction accumulators (solver.rs:917-926),
 // less the contact's previous-frame warm start.
@@ -752,11 +759,12 @@ __global__ void k_log_contacts(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     const uint32_t base = d.log_n[0];
-    const bool room = (uint64_t)base + nislc <= d.log_cap;
+    const bool room = (uint64_t)base + d.cnt[CT_LOG_N] <= d.log_cap;
     GRID_STRIDE(r, nislc) {
         if (!room) break;
         const uint32_t c = d.isl_v2[r];
-        const size_t o = (size_t)base + r;
+        if (c & RP_JBIT) continue;
+        const size_t o = (size_t)base + d.newflag[r];
         for (int k = 0; k < 12; k++) d.log_feat[o * 12 + k] = d.feat[(size_t)c * 12 + k];
         const float2 t = d.jt[c];
         const V3 b = warm_base(d, c, tp.cur);
@@ -765,9 +773,9 @@ __global__ void k_log_contacts(Dev d, TickParams tp) {
 }
 __global__ void k_log_advance(Dev d) {
     if (threadIdx.x != 0 || blockIdx.x != 0 || aborted(d)) return;
-    const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
-    if ((uint64_t)d.log_n[0] + nislc <= d.log_cap) d.log_n[0] += nislc;
-    else d.log_n[1] += nislc;
+    const uint32_t nrow = d.cnt[CT_LOG_N];
+    if ((uint64_t)d.log_n[0] + nrow <= d.log_cap) d.log_n[0] += nrow;
+    else d.log_n[1] += nrow;
 }
 
 // ------------------------------------------------------------------ ordered sweeps
@@ -2367,6 +2375,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     }
     if (tp.log_on) {
+        hipLaunchKernelGGL(k_log_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+        scan_u32(d.newflag, d.cap_contacts, d.scan_tmp, d.cnt + CT_LOG_N, st);
         hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, d);
     }

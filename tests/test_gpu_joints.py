@@ -96,3 +96,22 @@ def test_world_snapshot_restore_with_joints_replays():
         runs.append(hs)
     g.free_snapshot(snap)
     assert runs[0] == runs[1]
+
+
+def test_contact_log_skips_joint_entries():
+    """The contact log (training data) holds one row per solved contact, none for joint entries of
+    the island lists."""
+    ss = [chains(40 + i, n_ragdolls=3 + i) for i in range(2)]
+    nj = sum(len(s["joints"]) for s in ss)
+    b = rphys.Batch([rphys.World(s) for s in ss])
+    b.step_n(10)
+    b.set_residual_mode(True, 1.0e-4, 30)
+    b.set_contact_log(200000)
+    expect = 0
+    for _ in range(5):
+        _, c = b.step_profiled(1)
+        expect += c["island_contacts"] - nj  # every joint here is active with its rows
+    rows, dropped = b.contact_log_rows()
+    assert dropped == 0 and rows == expect and rows > 0
+    feat, targ = b.read_contact_log(0, rows)
+    assert np.isfinite(feat).all() and np.isfinite(targ).all()

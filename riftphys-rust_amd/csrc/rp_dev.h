@@ -176,7 +176,7 @@ enum {
     RP_ST_CONTACTS,         // terrain cull, contact count/scan/write
     RP_ST_PREP,             // constraint prep, islands, island sort, warm match
     RP_ST_ORDERS,           // list-order chains, velocity shock order and its chains
-    RP_ST_VEL_SMALL,        // velocity solve, islands of up to 4,096 contacts (256 threads)
+    RP_ST_VEL_SMALL,        // velocity solve, islands of up to 4,096 contacts (one wave each)
     RP_ST_VEL_LARGE,        // velocity solve, islands of up to 131,072 contacts (1024 threads, LDS flags)
     RP_ST_VEL_HUGE,         // velocity solve, larger islands (1024 threads, global flags)
     RP_ST_WARM_STORE,       // warm cache store, loaded flags

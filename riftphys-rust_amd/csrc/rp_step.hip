@@ -904,7 +904,10 @@ struct GlbFlags {
     __device__ __forceinline__ uint32_t dec(uint32_t i) const { return __hip_atomic_fetch_sub(&f[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 };
 template <int TIER> struct Tier;
-template <> struct Tier<0> { static constexpr uint32_t threads = 256, flag_bytes = 4096; };
+// Tier 0 is one wave per island: its islands (towers, chains, small piles) are mostly deep chains
+// a few rows wide, so a single wave keeps up with them, and four times as many of them fit on the
+// chip at once as with 256-thread workgroups.
+template <> struct Tier<0> { static constexpr uint32_t threads = 64, flag_bytes = 4096; };
 template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
 template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
 static __device__ __forceinline__ int tier_of(uint32_t n) { return n <= Tier<0>::flag_bytes ? 0 : (n <= Tier<1>::flag_bytes ? 1 : 2); }

@@ -186,7 +186,7 @@ struct Scene {
     uint32_t cur = 0;
     // island-sweep workgroups per tier (rphys_batch_set_tuning), then the wide-level launches per
     // sweep (set from the previous tick's widest island; 0: one-kernel sweeps)
-    uint32_t grids[5] = {2048, 256, 64, 0, 0};  // ... and the same for the shock order
+    uint32_t grids[5] = {8192, 256, 64, 0, 0};  // ... and the same for the shock order
     uint32_t graph_grids[5] = {0, 0, 0, 0, 0};  // the shape the graphs were captured with
     uint32_t max_extra = 0;
     std::vector<uint32_t> body_off, col_off;

@@ -29,7 +29,7 @@ def test_autotune_keeps_state_and_results(tmp_path):
     tuned = b.hashes()
 
     ref = make_batch()
-    assert ref.get_tuning() == {"grid_small": 2048, "grid_large": 256, "grid_huge": 64}
+    assert ref.get_tuning() == {"grid_small": 8192, "grid_large": 256, "grid_huge": 64}
     ref.step_n(50, graph=True)
     assert ref.hashes() == tuned
 

@@ -115,3 +115,15 @@ def test_contact_log_skips_joint_entries():
     assert dropped == 0 and rows == expect and rows > 0
     feat, targ = b.read_contact_log(0, rows)
     assert np.isfinite(feat).all() and np.isfinite(targ).all()
+
+
+def test_full_towers_and_chains_world_bit_exact():
+    """One whole at-scale world (16,384 bodies, 2,800 ball joints), device and oracle from spawn."""
+    s = scenes.towers_ragdolls(seed=scenes.seed_for(5, 0))
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(90):
+        o.step()
+        g.step()
+        if t % 30 == 29:
+            assert_state_close(o, g)
+            assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)

@@ -13,9 +13,14 @@
  *   HeightField::query_aabb           crates/riftphys-terrain/src/terrain.rs:300-359
  *   cull_and_orient_contacts          step.rs:499-542
  *   build_islands (non-empty test)    islands.rs:27-92
- *   solve_islands_parallel            solver.rs:477-1008, 1247-1271 (e=0: W6 inert)
+ *   jointed-pair exclusion            step.rs:85-93, 102
+ *   solve_islands_parallel            solver.rs:477-1008, 1247-1271 (e=0: W6 inert); ball-joint
+ *                                     rows, warm start and store solver.rs:272-307, 336-424,
+ *                                     644-715, 788-835, 945-952 (articulation bodies.rs:326-331)
  *   S4 integrate + gyroscopic turn    step.rs:154-280
- *   S5 contact projection             step.rs:284-310, 423-497
+ *   S5 joints + contact projection    step.rs:284-310, 423-497; ball joints' solve_d6 translational
+ *                                     rows with SHAKE directions bodies.rs:422-460, 562-573, 616-739,
+ *                                     dynamics.rs:71-89, 207-210
  *   S6 velocity from corrections      step.rs:312-379
  *   S7 sleep + AABB refresh           sleep.rs:59-80, step.rs:381-384
  *   step_hash + hash_carried          provenance.rs:6-24, world.rs:882-909
@@ -30,10 +35,12 @@
  * narrowphase.rs:735-776 and collision/tests/box_sphere_inside_test.rs; terrain query yields
  * (terrain.rs:360-394); closed-form free fall and resting heights (contact_resting_test.rs:30-45,
  * reality/src/analytic.rs free fall); invariants (manifold <= 4 points, n.(pb-pa) >= 0,
- * acc_jn >= 0, |jt| <= mu jn, snapshot/restore hash equality).  glam's SSE2 operation order and
+ * acc_jn >= 0, |jt| <= mu jn, snapshot/restore hash equality); ball joints (test_oracle_joints.py):
+ * a rigid pendulum's large-amplitude half period and energy, anchor drift, a hanging body's joint
+ * impulse m g dt, jointed pairs without contacts, the SHAKE test's acosf boundary.  glam's SSE2 operation order and
  * the platform libm are third-party (glam 0.27.0, libm 0.2.15): no reference test pins the
- * contact path's bits, so bit-level agreement with the Rust build is "parity unpinned" for the
- * contact scenes; the oracle is pinned by the checks above.
+ * contact path's or a ball joint's bits, so bit-level agreement with the Rust build is "parity
+ * unpinned" for the contact and joint scenes; the oracle is pinned by the checks above.
  */
 #include <math.h>
 #include <stdint.h>

@@ -27,6 +27,7 @@ struct HashDev {
     uint8_t *digest;
     uint32_t cap_chunks;
     uint32_t *w_jbytes;
+    uint32_t *j_off;
 };
 extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev,
                                      const TierStreams *ts);
@@ -301,7 +302,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     }
     bytes += 48ull * W;
     h.cap_chunks = (uint32_t)(bytes / 1024 + nw + 1);
-    ok &= B.alloc(h.w_jbytes, nw + 1);
+    ok &= B.alloc(h.w_jbytes, nw + 1) && B.alloc(h.j_off, nj + 1);
     ok &= B.alloc(h.keyflag, W + 1) && B.alloc(h.w_chunk, nw + 1) && B.alloc(h.w_len, nw) && B.alloc(h.w_epoch, nw) && B.alloc(h.w_rng, nw);
     ok &= B.alloc(h.stream, (size_t)h.cap_chunks * 1024) && B.alloc(h.cv, 8 * (size_t)h.cap_chunks) && B.alloc(h.cv2, 8 * (size_t)h.cap_chunks) && B.alloc(h.digest, 32 * (size_t)nw);
     if (!ok) { set_err("device allocation failed (scene too large for the configured capacities)"); return nullptr; }

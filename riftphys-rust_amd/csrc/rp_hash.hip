@@ -17,6 +17,7 @@ struct HashDev {
     uint8_t *digest;      // 32 per world
     uint32_t cap_chunks;
     uint32_t *w_jbytes;   // nw: bytes of the world's warm-joint entries
+    uint32_t *j_off;      // nj + 1: bytes of each joint's entry, then their exclusive scan
 };
 
 static __constant__ uint32_t B3_IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
@@ -85,11 +86,11 @@ static __device__ __forceinline__ uint32_t world_len(const Dev &d, const HashDev
 // per row, rows in axis order with tags axis << 3.
 static __device__ __forceinline__ uint32_t joint_entry_bytes(uint32_t m) { return (m >> 31) ? 8u + 5u * (uint32_t)__popc(m & 7u) : 0u; }
 
+__global__ void k_hash_jbytes(Dev d, HashDev h) { GS(q, d.nj) h.j_off[q] = joint_entry_bytes(d.wj_mask[q]); }
+
 __global__ void k_hash_sizes(Dev d, HashDev h, uint32_t cur) {
     GS(w, d.nw) {
-        uint32_t jb = 0;
-        for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) jb += joint_entry_bytes(d.wj_mask[q]);
-        h.w_jbytes[w] = jb;
+        h.w_jbytes[w] = d.nj ? h.j_off[d.w_joint_off[w + 1]] - h.j_off[d.w_joint_off[w]] : 0u;
         uint32_t len = world_len(d, h, w, cur);
         h.w_len[w] = len;
         h.w_chunk[w] = (len + 1023u) / 1024u;
@@ -117,21 +118,8 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
             uint32_t z = o + 8u + 20u * K + 28u * E;
             const uint32_t JB = h.w_jbytes[w];
             put_u64(S + z, 0);
-            uint32_t nstored = 0;
-            uint8_t *jp = S + z + 16u;
-            for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) {
-                const uint32_t m = d.wj_mask[q];
-                if (!(m >> 31)) continue;
-                nstored++;
-                put_u32(jp, q - d.w_joint_off[w]); put_u32(jp + 4, (uint32_t)__popc(m & 7u));
-                jp += 8;
-                for (uint32_t i = 0; i < 3; i++) {
-                    if (!((m >> i) & 1u)) continue;
-                    jp[0] = (uint8_t)(i << 3);
-                    put_u32(jp + 1, canon(d.wj_lam[3 * q + i]));
-                    jp += 5;
-                }
-            }
+            uint32_t nstored = 0;  // the entries themselves are written per joint below
+            for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) nstored += d.wj_mask[q] >> 31;
             put_u64(S + z + 8, nstored);
             put_u64(S + z + 16 + JB, 0);
             put_u64(S + z + 24 + JB, n);
@@ -155,6 +143,24 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
         uint32_t so = ao + 8u + cap + 8u + 2u * i;
         uint32_t t = d.sticks[b];
         S[so] = (uint8_t)t; S[so + 1] = (uint8_t)(t >> 8);
+    }
+    // warm-joint entries, one lane per joint, at the joint's offset within its world's entries
+    GS(q, d.nj) {
+        const uint32_t m = d.wj_mask[q];
+        if (!(m >> 31)) continue;
+        const uint32_t w = d.bworld[d.jt_a[q]], q0 = d.w_joint_off[w];
+        const uint32_t n = d.w_body_off[w + 1] - d.w_body_off[w];
+        const uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
+        const uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
+        uint8_t *jp = h.stream + (size_t)h.w_chunk[w] * 1024u + 48u + 56u * n + 8u + 20u * K + 28u * E + 16u + (h.j_off[q] - h.j_off[q0]);
+        put_u32(jp, q - q0); put_u32(jp + 4, (uint32_t)__popc(m & 7u));
+        jp += 8;
+        for (uint32_t i = 0; i < 3; i++) {
+            if (!((m >> i) & 1u)) continue;
+            jp[0] = (uint8_t)(i << 3);
+            put_u32(jp + 1, canon(d.wj_lam[3 * q + i]));
+            jp += 5;
+        }
     }
     // warm entries (key headers written by the group's first entry)
     GS(e, tot_warm) {
@@ -298,6 +304,11 @@ extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uin
     hipLaunchKernelGGL(k_hash_keyflags, dim3(gfor(d.cap_warm)), dim3(256), 0, st, d, h, cur);
     hscan(h.keyflag, d.cap_warm, d.scan_tmp, d.cnt + CT_SCRATCH, st);
     hipLaunchKernelGGL(k_hash_store_total, dim3(1), dim3(1), 0, st, h.keyflag, d.cap_warm, d.cnt + CT_SCRATCH);
+    if (d.nj) {
+        hipLaunchKernelGGL(k_hash_jbytes, dim3(gfor(d.nj)), dim3(256), 0, st, d, h);
+        hscan(h.j_off, d.nj, d.scan_tmp, d.cnt + CT_SCRATCH, st);
+        hipLaunchKernelGGL(k_hash_store_total, dim3(1), dim3(1), 0, st, h.j_off, d.nj, d.cnt + CT_SCRATCH);
+    }
     hipLaunchKernelGGL(k_hash_sizes, dim3(gfor(d.nw)), dim3(256), 0, st, d, h, cur);
     hscan(h.w_chunk, d.nw, d.scan_tmp, d.cnt + CT_SCRATCH, st);
     hipLaunchKernelGGL(k_hash_store_total, dim3(1), dim3(1), 0, st, h.w_chunk, d.nw, d.cnt + CT_SCRATCH);

@@ -60,9 +60,9 @@ def stage_bytes(stage, ctr, iv, ip):
 # sequence per tick (a setup kernel, kl wide-level launches and a tail sweep per iteration, and
 # for the velocity solve a shock kernel), so their roofline unit is the stage, per tick.
 KERNEL_OF_STAGE = {
-    "vel_solve_small": "k_sweep_vel<0>",
+    "vel_solve_small": "k_sweep_vel<0> + k_sweep_vel<3>",
     "vel_solve_large": "k_sweep_vel<1> + k_level_vel + k_sweep_vel_tail<1> + k_sweep_vel_post<1>",
-    "projection_small": "k_sweep_proj<0>",
+    "projection_small": "k_sweep_proj<0> + k_sweep_proj<3>",
     "projection_large": "k_sweep_proj<1> + k_level_proj + k_sweep_proj_tail<1>",
     "narrow_pairs": "k_narrow_pairs",
     "narrow_terrain": "k_narrow_terrain",

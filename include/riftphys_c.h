@@ -271,7 +271,8 @@ RPhysResult rphys_batch_restore(RPhysBatch *batch, uint32_t slot);
  * snapshot.  Caching results per (config, size) is the caller's (rphys.py keeps a JSON file). */
 typedef struct RPhysTuning {
     uint32_t size_bytes;  /* sizeof(RPhysTuning) */
-    uint32_t grid_small;  /* islands of <= 4,096 contacts: one-wave workgroups (default 8192) */
+    uint32_t grid_small;  /* islands of <= 256 entries: one-wave workgroups (default 8192); islands of
+                           * 257..4,096 entries use grid_small / 4 four-wave workgroups */
     uint32_t grid_large;  /* <= 131,072 contacts: 1024 threads, LDS flags (default 256) */
     uint32_t grid_huge;   /* larger islands: 1024 threads, HBM flags (default 64) */
 } RPhysTuning;

@@ -269,7 +269,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.ch_pa, C) && B.alloc(d.ch_pb, C) && B.alloc(d.sh_ord, C) && B.alloc(d.sh_pa, C) && B.alloc(d.sh_pb, C) && B.alloc(d.g_done, C);
     ok &= B.alloc(d.ch_sa, C) && B.alloc(d.ch_sb, C) && B.alloc(d.sh_sa, C) && B.alloc(d.sh_sb, C) && B.alloc(d.ex_base, C) && B.alloc(d.ex_shock, C);
     ok &= B.alloc(d.bs_pa, C) && B.alloc(d.bs_pb, C) && B.alloc(d.bs_c, C) && B.alloc(d.ss_pa, C) && B.alloc(d.ss_pb, C) && B.alloc(d.ss_c, C);
-    ok &= B.alloc(d.slot_of, C) && B.alloc(d.lvl_off, C) && B.alloc(d.isl_w, C + 1) && B.alloc(d.wide_list, C + 1) && B.alloc(d.big_list, C + 1) && B.alloc(d.lvl_off_sh, C) && B.alloc(d.isl_wsh, C + 1) && B.alloc(d.bp_fill, NC) && B.alloc(d.tier_list[0], C + 1) && B.alloc(d.tier_list[1], C + 1) && B.alloc(d.tier_list[2], C + 1) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
+    ok &= B.alloc(d.slot_of, C) && B.alloc(d.lvl_off, C) && B.alloc(d.isl_w, C + 1) && B.alloc(d.wide_list, C + 1) && B.alloc(d.big_list, C + 1) && B.alloc(d.lvl_off_sh, C) && B.alloc(d.isl_wsh, C + 1) && B.alloc(d.bp_fill, NC) && B.alloc(d.tier_list[0], C + 1) && B.alloc(d.tier_list[1], C + 1) && B.alloc(d.tier_list[2], C + 1) && B.alloc(d.tier_list[3], C + 1) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
     ok &= B.alloc(d.feat, 12 * C) && B.alloc(d.pred, C);
     {
         float *mp = nullptr;

@@ -38,7 +38,7 @@ enum {
     CT_WIDE_N, CT_WIDE_MAX,      // islands with wide levels; most wide levels of one island
     CT_BIG_N,                    // islands of the 1024-thread tiers
     CT_WSH_MAX,                  // most leading wide levels of one island's shock order
-    CT_TIER_N, CT_TIER_N1, CT_TIER_N2,  // islands per sweep tier
+    CT_TIER_N, CT_TIER_N1, CT_TIER_N2, CT_TIER_N3,  // islands per sweep tier
     CT_ROWS,                     // island-list entries before the sort: joints + contacts
     CT_LOG_N,                    // contacts appended to the contact log this tick
     // sums over the completed ticks since the counters were last cleared
@@ -121,7 +121,7 @@ struct Dev {
     uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
     uint32_t *big_list;   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
     uint32_t *bp_fill;    // per collider: pairs appended to its list (broadphase write pass)
-    uint32_t *tier_list[3];  // island indices of each sweep tier this tick (CT_TIER_N + tier of them)
+    uint32_t *tier_list[4];  // island indices of each sweep tier this tick (CT_TIER_N + tier of them)
     uint32_t *lvl_off_sh, *isl_wsh;  // as lvl_off / isl_w, for the shock order of the current shock pass
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
@@ -176,7 +176,7 @@ enum {
     RP_ST_CONTACTS,         // terrain cull, contact count/scan/write
     RP_ST_PREP,             // constraint prep, islands, island sort, warm match
     RP_ST_ORDERS,           // list-order chains, velocity shock order and its chains
-    RP_ST_VEL_SMALL,        // velocity solve, islands of up to 4,096 contacts (one wave each)
+    RP_ST_VEL_SMALL,        // velocity solve, islands of up to 4,096 entries (tiers 0 and 3)
     RP_ST_VEL_LARGE,        // velocity solve, islands of up to 131,072 contacts (1024 threads, LDS flags)
     RP_ST_VEL_HUGE,         // velocity solve, larger islands (1024 threads, global flags)
     RP_ST_WARM_STORE,       // warm cache store, loaded flags

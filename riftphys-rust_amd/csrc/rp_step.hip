@@ -906,11 +906,16 @@ struct GlbFlags {
 template <int TIER> struct Tier;
 // Tier 0 is one wave per island: its islands (towers, chains, small piles) are mostly deep chains
 // a few rows wide, so a single wave keeps up with them, and four times as many of them fit on the
-// chip at once as with 256-thread workgroups.
-template <> struct Tier<0> { static constexpr uint32_t threads = 64, flag_bytes = 4096; };
+// chip at once as with 256-thread workgroups.  Tier 3 takes the mid-sized islands (up to 4,096
+// entries) with four waves; tiers 1 and 2 are the big islands (wide levels, split form, phase clocks).
+template <> struct Tier<0> { static constexpr uint32_t threads = 64, flag_bytes = 256; };
+template <> struct Tier<3> { static constexpr uint32_t threads = 256, flag_bytes = 4096; };
 template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
 template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
-static __device__ __forceinline__ int tier_of(uint32_t n) { return n <= Tier<0>::flag_bytes ? 0 : (n <= Tier<1>::flag_bytes ? 1 : 2); }
+template <int TIER> constexpr bool BIG = TIER == 1 || TIER == 2;
+static __device__ __forceinline__ int tier_of(uint32_t n) {
+    return n <= Tier<0>::flag_bytes ? 0 : (n <= Tier<3>::flag_bytes ? 3 : (n <= Tier<1>::flag_bytes ? 1 : 2));
+}
 // The islands of each sweep tier (by contact count), so each tier's launch walks only its own.
 __global__ void k_tier_lists(Dev d) {
     if (aborted(d)) return;
@@ -1196,7 +1201,7 @@ static __device__ __forceinline__ void apply_row(const Dev &d, const Row &x, V3 
 
 template <int TIER>
 static __device__ __forceinline__ auto make_island(const Dev &d, uint8_t *flag_lds, uint32_t s, uint32_t n) {
-    if constexpr (TIER < 2) return Island<LdsFlags>{LdsFlags{(RP_LDS uint8_t *)flag_lds}, s, n, 0};
+    if constexpr (TIER != 2) return Island<LdsFlags>{LdsFlags{(RP_LDS uint8_t *)flag_lds}, s, n, 0};
     else return Island<GlbFlags>{GlbFlags{d.g_done + s}, s, n, 0};
 }
 
@@ -1464,10 +1469,10 @@ static __device__ void shock_slots(const Dev &d, Island<FL> &I, uint32_t isl) {
     const uint32_t s = I.s, n = I.n;
     __shared__ uint32_t wide_ks;
     const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK],
-                                                       TIER > 0 ? d.lvl_off_sh + s : nullptr, TIER > 0 ? &wide_ks : nullptr)
+                                                       BIG<TIER> ? d.lvl_off_sh + s : nullptr, BIG<TIER> ? &wide_ks : nullptr)
                                           : nullptr;
     build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, Slots{d.ss_pa, d.ss_pb, d.ss_c});
-    if constexpr (TIER > 0) {
+    if constexpr (BIG<TIER>) {
         const uint32_t K = exs ? wide_ks : 0u;
         if (threadIdx.x == 0) { d.isl_wsh[isl] = K; if (K) atomicMax(&d.cnt[CT_WSH_MAX], K); }
     }
@@ -1487,10 +1492,10 @@ static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> 
     }
     __syncthreads();
     shock_slots<TIER>(d, I, isl);
-    if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
-    if (TIER > 0 && klsh) return;
+    if constexpr (BIG<TIER>) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
+    if (BIG<TIER> && klsh) return;
     run_sweep(I, Slots{d.ss_pa, d.ss_pb, d.ss_c}, [&](uint32_t c) { vshock_row(d, up, c); });
-    if constexpr (TIER > 0) pc.lap(d, RP_PH_VEL_SHOCK);
+    if constexpr (BIG<TIER>) pc.lap(d, RP_PH_VEL_SHOCK);
 }
 
 // Velocity solve of one island per workgroup: warm-start application and the PGS iterations in
@@ -1508,11 +1513,11 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         PhaseClock pc;
         __shared__ uint32_t wide_k;
         const uint32_t *exb = n > blockDim.x ? level_order(I, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, d.ex_base, &d.ph[RP_PH_ROUNDS_BASE],
-                                                           TIER > 0 ? d.lvl_off + s : nullptr, TIER > 0 ? &wide_k : nullptr)
+                                                           BIG<TIER> ? d.lvl_off + s : nullptr, BIG<TIER> ? &wide_k : nullptr)
                                               : nullptr;
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
         build_slots(d, I, d.isl_v2, d.ch_pa, d.ch_pb, exb, sb);
-        if constexpr (TIER > 0) {
+        if constexpr (BIG<TIER>) {
             // the island's leading wide levels (sweeps may run them as GPU-wide launches)
             const uint32_t K = exb ? wide_k : 0u;
             if (threadIdx.x == 0) {
@@ -1522,12 +1527,12 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             }
             pc.lap(d, RP_PH_VEL_ORDER);
         }
-        if (!(TIER > 0 && split) && tp.ws_mode != 1) run_sweep(I, sb, [&](uint32_t c) { warm_row(d, tp, c); });
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_WARM);
+        if (!(BIG<TIER> && split) && tp.ws_mode != 1) run_sweep(I, sb, [&](uint32_t c) { warm_row(d, tp, c); });
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_WARM);
         // PGS iterations (solver.rs:839-879).  The rows' constants are stored per slot once and
         // loaded, with the row's own accumulators, before its wait; the bodies' velocities are
         // loaded once and kept in registers between the normal and the friction update.
-        if (TIER > 0 && split) {
+        if (BIG<TIER> && split) {
             // the row store (k_store_rows), warm start, sweeps and shock pass follow in launches
             // of their own
             if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], tp.iv); atomicAdd(&d.cnt[CT_ITERS_SUM], tp.iv * n); }
@@ -1565,7 +1570,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n);
             if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); }
         }
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_PGS);
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PGS);
         vel_shock<TIER>(d, tp, I, lst, isl, pc, 0u);
     ISLAND_LOOP_END
 }
@@ -1806,7 +1811,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_ball(Dev d, TickP
                 if (d.cs_wb[c] != RP_NONE) d.ballf[b] = 0;
             }
         });
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_BALL);
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_BALL);
     ISLAND_LOOP_END
 }
 
@@ -1992,12 +1997,12 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickP
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
-        if (TIER > 0 && split) continue;  // k_store_prows, then k_level_proj + k_sweep_proj_tail
+        if (BIG<TIER> && split) continue;  // k_store_prows, then k_level_proj + k_sweep_proj_tail
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
         __syncthreads();
         for (uint32_t it = 0; it < tp.ip; it++)
             run_sweep(I, sb, [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ);
     ISLAND_LOOP_END
 }
 // Split form: one projection sweep over the slots after the wide levels k_level_proj ran.
@@ -2010,7 +2015,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_tail(Dev d, 
         const uint32_t k = min(d.isl_w[isl], kl);
         run_sweep_from(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, k ? d.lvl_off[s + k] : 0u,
                        [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ);
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ);
     ISLAND_LOOP_END
 }
 // The projection rows of every big island's slots (split form), across the GPU.
@@ -2033,10 +2038,10 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d,
         PhaseClock pc;
         shock_slots<TIER>(d, I, isl);
         const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ_SHOCK_ORDER);
-        if (TIER > 0 && klsh) continue;  // k_level_pshock + k_sweep_pshock_tail follow
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ_SHOCK_ORDER);
+        if (BIG<TIER> && klsh) continue;  // k_level_pshock + k_sweep_pshock_tail follow
         run_sweep(I, ss, [&](uint32_t c, uint32_t) { return load_prow(d, c); }, [&](uint32_t, const PRow &x) { project_row(d, x, true, up); });
-        if constexpr (TIER > 0) pc.lap(d, RP_PH_PROJ_SHOCK);
+        if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ_SHOCK);
     ISLAND_LOOP_END
 }
 
@@ -2223,13 +2228,16 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     const uint32_t B = 256;
     auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev[k], st); };
     // the three tiers of one sweep stage: concurrent with side streams, else in order
-    auto tiers = [&](auto k0, auto k1, auto k2) {
-        if (!ts) { k0(st); k1(st); k2(st); return; }
+    // the four tiers of one sweep stage: tier 0 on side[0], tier 3 then tier 2 on side[1], tier 1 on
+    // the tick's stream, concurrently (all solve disjoint islands); in order without side streams
+    auto tiers = [&](auto k0, auto k3, auto k1, auto k2) {
+        if (!ts) { k0(st); k3(st); k1(st); k2(st); return; }
         (void)hipEventRecord(ts->ev[0], st);
         (void)hipStreamWaitEvent(ts->side[0], ts->ev[0], 0);
         (void)hipStreamWaitEvent(ts->side[1], ts->ev[0], 0);
         k0(ts->side[0]);
         k1(st);
+        k3(ts->side[1]);
         k2(ts->side[1]);
         (void)hipEventRecord(ts->ev[1], ts->side[0]);
         (void)hipEventRecord(ts->ev[2], ts->side[1]);
@@ -2279,7 +2287,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_island_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     scan_u32(d.isl_flag, d.cap_contacts, d.scan_tmp, d.cnt + CT_ISLANDS, st);
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
-    for (uint32_t t = 0; t < 3; t++) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_TIER_N + t, 0u);
+    for (uint32_t t = 0; t < 4; t++) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_TIER_N + t, 0u);
     hipLaunchKernelGGL(k_tier_lists, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
@@ -2306,15 +2314,19 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     const uint32_t kl = grids[3], split = (kl > 0 && !tp.resid_on) ? 1u : 0u, GL = 64;
     // shock passes in the split form: klsh wide-level launches of the shock order per pass
     const uint32_t klsh = split ? grids[4] : 0u;
-    // tier 0 on side[0] alongside `big` on the tick's stream, joined back
-    auto fork0 = [&](auto k0, auto big) {
-        if (!ts) { k0(st); big(st); return; }
+    // tier 0 on side[0] and tier 3 on side[1] alongside `big` on the tick's stream, joined back
+    auto fork0 = [&](auto k0, auto k3, auto big) {
+        if (!ts) { k0(st); k3(st); big(st); return; }
         (void)hipEventRecord(ts->ev[0], st);
         (void)hipStreamWaitEvent(ts->side[0], ts->ev[0], 0);
+        (void)hipStreamWaitEvent(ts->side[1], ts->ev[0], 0);
         k0(ts->side[0]);
+        k3(ts->side[1]);
         big(st);
         (void)hipEventRecord(ts->ev[1], ts->side[0]);
+        (void)hipEventRecord(ts->ev[2], ts->side[1]);
         (void)hipStreamWaitEvent(st, ts->ev[1], 0);
+        (void)hipStreamWaitEvent(st, ts->ev[2], 0);
     };
     auto big_vel = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
@@ -2349,22 +2361,29 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
             hipLaunchKernelGGL(k_sweep_proj_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, kl);
         }
     };
+    // the small islands: tier 0 (one wave each) and tier 3 (mid-sized, four waves)
+    const uint32_t g3 = g0 / 4 > 256u ? g0 / 4 : 256u;
     auto vel0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
-    // profiled with side streams: tier 0 runs alongside on side[0], timed by its own events
-    // ev[RP_NMARKS + 2 * which] and ev[RP_NMARKS + 2 * which + 1]; the larger tiers take the stages
-    // of the tick's stream
-    auto small_timed = [&](auto k0, int which) {
+    auto vel3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
+    // profiled with side streams: tiers 0 and 3 run alongside on side[0] and side[1], timed as one
+    // stage by events ev[RP_NMARKS + 2 * which] and ev[RP_NMARKS + 2 * which + 1] on side[0]; the
+    // larger tiers take the stages of the tick's stream
+    auto small_timed = [&](auto k0, auto k3, int which) {
         (void)hipEventRecord(ts->ev[0], st);
         (void)hipStreamWaitEvent(ts->side[0], ts->ev[0], 0);
+        (void)hipStreamWaitEvent(ts->side[1], ts->ev[0], 0);
         (void)hipEventRecord(ev[RP_NMARKS + 2 * which], ts->side[0]);
         k0(ts->side[0]);
+        k3(ts->side[1]);
+        (void)hipEventRecord(ts->ev[2], ts->side[1]);
+        (void)hipStreamWaitEvent(ts->side[0], ts->ev[2], 0);
         (void)hipEventRecord(ev[RP_NMARKS + 2 * which + 1], ts->side[0]);
         (void)hipEventRecord(ts->ev[1], ts->side[0]);
     };
     if (ev) {  // profiled: a stage per tier (the split big-island solve counts as large)
         mark(RP_ST_VEL_SMALL);
-        if (ts) small_timed(vel0, 0);
-        else vel0(st);
+        if (ts) small_timed(vel0, vel3, 0);
+        else { vel0(st); vel3(st); }
         mark(RP_ST_VEL_LARGE);
         if (split) big_vel(st);
         else hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
@@ -2372,9 +2391,9 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         if (!split) hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
         if (ts) (void)hipStreamWaitEvent(st, ts->ev[1], 0);
     } else if (split) {
-        fork0(vel0, big_vel);
+        fork0(vel0, vel3, big_vel);
     } else {
-        tiers(vel0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
+        tiers(vel0, vel3, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     }
     if (tp.log_on) {
@@ -2397,6 +2416,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
     tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
+          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp); },
           [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
           [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
     mark(RP_ST_INTEGRATE);
@@ -2404,10 +2424,11 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     auto proj0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
+    auto proj3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
     if (ev) {
         mark(RP_ST_PROJ_SMALL);
-        if (ts) small_timed(proj0, 1);
-        else proj0(st);
+        if (ts) small_timed(proj0, proj3, 1);
+        else { proj0(st); proj3(st); }
         mark(RP_ST_PROJ_LARGE);
         if (split) big_proj(st);
         else hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
@@ -2415,15 +2436,16 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         if (!split) hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
         if (ts) (void)hipStreamWaitEvent(st, ts->ev[1], 0);
     } else if (split) {
-        fork0(proj0, big_proj);
+        fork0(proj0, proj3, big_proj);
     } else {
-        tiers(proj0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
+        tiers(proj0, proj3, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     }
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses
     mark(RP_ST_PROJ_SHOCK);
     auto pshock0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
+    auto pshock3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
     auto big_pshock = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
         hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
@@ -2432,8 +2454,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         hipLaunchKernelGGL(k_sweep_pshock_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
         hipLaunchKernelGGL(k_sweep_pshock_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
     };
-    if (klsh) fork0(pshock0, big_pshock);
-    else tiers(pshock0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
+    if (klsh) fork0(pshock0, pshock3, big_pshock);
+    else tiers(pshock0, pshock3, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
                [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     mark(RP_ST_S6);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);

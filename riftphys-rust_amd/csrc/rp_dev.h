@@ -209,9 +209,9 @@ enum {
 #define RP_WIDE_MIN 1024u         // ranks of a level that one workgroup cannot take in one round
 #define RP_WIDE_MAX_LEVELS 64u    // leading wide levels recorded per island
 
-// Optional side streams for the island sweeps: the three size tiers of one stage solve disjoint
-// islands, so they run concurrently (tier 0 on side[0], tier 1 on the tick's stream, tier 2 on
-// side[1]), forked from and joined back to the tick's stream through ev (no timing).
+// Optional side streams for the island sweeps: the four size tiers of one stage solve disjoint
+// islands, so they run concurrently (tier 0 on side[0], tier 1 on the tick's stream, tiers 3 and 2
+// on side[1]), forked from and joined back to the tick's stream through ev (no timing).
 struct TierStreams { hipStream_t side[2]; hipEvent_t ev[3]; };
 
 struct TickParams {

@@ -2227,7 +2227,6 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
                                      const TierStreams *ts) {
     const uint32_t B = 256;
     auto mark = [&](int k) { if (ev) (void)hipEventRecord(ev[k], st); };
-    // the three tiers of one sweep stage: concurrent with side streams, else in order
     // the four tiers of one sweep stage: tier 0 on side[0], tier 3 then tier 2 on side[1], tier 1 on
     // the tick's stream, concurrently (all solve disjoint islands); in order without side streams
     auto tiers = [&](auto k0, auto k3, auto k1, auto k2) {

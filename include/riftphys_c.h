@@ -263,12 +263,13 @@ RPhysResult rphys_batch_get_state(const RPhysBatch *batch, uint32_t world_index,
 RPhysResult rphys_batch_snapshot(RPhysBatch *batch, uint32_t slot);   /* device-side copy of all worlds' state */
 RPhysResult rphys_batch_restore(RPhysBatch *batch, uint32_t slot);
 /* Launch-shape tuning (build addition; the reference has no GPU launches to tune).  The island
- * sweeps run one workgroup per island, taken grid-stride, in three tiers by island size; the
- * grids only change speed, never results.  rphys_batch_autotune grid-searches grid_small in
+ * sweeps run one workgroup per island, taken grid-stride, in tiers by island size; the grids
+ * only change speed, never results.  rphys_batch_autotune grid-searches grid_small in
  * {1024..32768}, grid_large in {128..2048}, grid_huge in {64..256} (powers of two): each candidate steps the same
  * `ticks` graph-captured ticks from a device snapshot, per-world hashes must agree across all
- * candidates (else RPHYS_ERR_INTERNAL), the fastest is installed and the batch is left at the
- * snapshot.  Caching results per (config, size) is the caller's (rphys.py keeps a JSON file). */
+ * candidates (else RPHYS_ERR_INTERNAL); the four fastest and the default shape run three more
+ * times, the best of each shape's runs decides, the winner is installed and the batch is left at
+ * the snapshot.  Caching results per (config, size) is the caller's (rphys.py keeps a JSON file). */
 typedef struct RPhysTuning {
     uint32_t size_bytes;  /* sizeof(RPhysTuning) */
     uint32_t grid_small;  /* islands of <= 256 entries: one-wave workgroups (default 8192); islands of

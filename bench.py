@@ -14,6 +14,9 @@ Prints one JSON line: body-steps/s over the timed ticks, plus
                  host on a bounded, stratified sample of the same ticks: a few oracle ticks
                  continued from the device state at the start of every 100-tick chunk of the timed
                  region, each chunk's CPU time extrapolated from its sample.
+Other workloads (--workload): batch (1,024 worlds, rollback windows), learned (the learned warm
+start against the previous-frame and zero ones, batched and plain, to residual 1e-4), towers (8
+worlds of 16,384 bodies with towers and ball-jointed capsule chains), boxes512 (the box lattice).
 """
 import argparse
 import json

@@ -268,8 +268,8 @@ def batch_cpu_baseline(worlds, warmup, ticks, dt):
 def towers_bench(a):
     """One GPU's share of the at-scale workload: `--worlds` (default 8 of 64) worlds of 16,384 bodies,
     669 towers of 20 boxes, 4 loose boxes and 200 chains of 15 capsules joined by 14 ball joints
-    each, over fBm terrain, velocity iterations 10, position iterations 2.  Timed: graph-captured
-    ticks with the fixed iterations; then, from one snapshot, the PGS run to residual 1e-4 per island
+    each, over fBm terrain, velocity iterations 10, position iterations 2.  Timed: ticks with the
+    fixed iterations, launched on the library's streams (`--graph`: graph-captured and autotuned); then, from one snapshot, the PGS run to residual 1e-4 per island
     with the previous-frame, the zero and (with --learned) the learned warm start, the model trained
     on the solver's converged impulses logged from 4 other worlds of the same generator; the oracle
     on one world for the CPU baseline."""
@@ -505,6 +505,8 @@ def main():
                     help="step on the stream instead of graph-captured ticks (profiling runs: rocprofv3 kernel tracing of graph launches crashed the profiled process)")
     ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
     ap.add_argument("--learned", action="store_true", help="towers workload: also train and compare the learned warm start")
+    ap.add_argument("--graph", action="store_true",
+                    help="towers workload: graph-captured, autotuned ticks (default: stream launches, measured faster on this workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (bandwidth reference, counter calibration)")
     a = ap.parse_args()
@@ -533,6 +535,7 @@ def main():
             a.worlds = 8
         if a.train_rows == 10_000_000:
             a.train_rows = 3_000_000
+        a.no_graph = not a.graph
         print(json.dumps(towers_bench(a)), flush=True)
         return
     if a.workload == "batch":

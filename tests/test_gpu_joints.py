@@ -127,3 +127,47 @@ def test_full_towers_and_chains_world_bit_exact():
         if t % 30 == 29:
             assert_state_close(o, g)
             assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+
+
+def _two_body(anchor_pose, body_pose, fa, fb):
+    s = scenes._empty(2)
+    s["shape"][:] = [1, 2]
+    s["dims"][0] = (0.2, 0.2, 0.2)
+    s["dims"][1] = (0.1, 0.15, 0.0)
+    s["dynamic"][:] = [0, 1]
+    s["pose"][0] = anchor_pose
+    s["pose"][1] = body_pose
+    s["joints"] = [(0, 1, fa, fb)]
+    s["hf"] = None
+    s["pairs"] = list(scenes.CONFIG_PAIRS)
+    s["iters"] = (10, 20)
+    s["dt"] = np.float32(1.0 / 60.0)
+    return s
+
+
+def test_joint_to_a_static_body_bit_exact():
+    """A capsule pendulum hung from a static box: the joint writes one body only."""
+    q = scenes.quat_yaw_pitch(0.3, 1.2)
+    s = _two_body((0, 5, 0, 0, 0, 0, 1), (0.05, 5.1, 0.24, *q), (0, 0, 0, 0, 0, 0, 1), (0, -0.25, 0, 0, 0, 0, 1))
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(120):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+    assert_state_close(o, g)
+
+
+def test_removed_jointed_body_bit_exact():
+    """A chain link removed mid-run: its joints stay, now to a static body, and lose their warm rows."""
+    s = chains(16)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for _ in range(20):
+        o.step()
+        g.step()
+    victim = s["joints"][3][1]  # a capsule in the middle of the first chain (scene index)
+    assert oracle_py.lib().or_remove_body(o.h, victim) == 1
+    assert rphys.lib().rphys_body_remove(g.h, victim + 1) == rphys.OK
+    for t in range(40):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d after the removal" % (t + 1)

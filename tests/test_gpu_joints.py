@@ -1,5 +1,7 @@
 """Ball joints: the device step vs the oracle on scenes with capsule chains (bit for bit), residual
 mode with joints, the batch with joint worlds (graph, rollback) and the at-scale workload's shape."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -171,3 +173,32 @@ def test_removed_jointed_body_bit_exact():
         o.step()
         g.step()
         assert o.hash() == g.hash(), "hash at tick %d after the removal" % (t + 1)
+
+
+def test_set_pose_and_restore_on_joint_worlds_match_the_oracle():
+    """Teleporting a jointed body drops its joints' warm rows (world.rs:818-826), and a world restore
+    renormalises poses and restores the warm joint rows (lib.rs:1778-1790): device and oracle agree."""
+    s = chains(17)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for _ in range(15):
+        o.step()
+        g.step()
+    so, sg = o.snapshot(), g.snapshot()
+    b = s["joints"][5][0]
+    pose = np.array(o.state()[0][b].tolist() + o.state()[1][b].tolist(), np.float32)
+    pose[1] += 0.3
+    oracle_py.lib().or_set_body_pose(o.h, b, oracle_py.fptr(pose))
+    assert rphys.lib().rphys_body_set_pose(g.h, b + 1, C.byref(rphys.iso(pose[:3], pose[3:]))) == rphys.OK
+    for t in range(10):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d after the teleport" % (t + 1)
+    o.restore(so)
+    g.restore(sg)
+    assert o.hash() == g.hash(), "hash after the restore"
+    for t in range(10):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d after the restore" % (t + 1)
+    o.free_snapshot(so)
+    g.free_snapshot(sg)

@@ -202,3 +202,21 @@ def test_set_pose_and_restore_on_joint_worlds_match_the_oracle():
         assert o.hash() == g.hash(), "hash at tick %d after the restore" % (t + 1)
     o.free_snapshot(so)
     g.free_snapshot(sg)
+
+
+def test_batch_mixing_worlds_with_and_without_joints():
+    # a batch takes its solver settings from world 0: every world here runs (10, 20) iterations
+    ss = [scenes.small_mixed(n=40, seed=50), chains(51), scenes.small_mixed(n=70, seed=52), chains(53, n_ragdolls=2)]
+    for s in ss:
+        s["iters"] = (10, 20)
+    singles = [rphys.World(s) for s in ss]
+    b = rphys.Batch([rphys.World(s) for s in ss])
+    for _ in range(15):
+        for w in singles:
+            w.step()
+    b.step_n(15, graph=True)
+    assert b.hashes() == [w.hash() for w in singles]
+    o = oracle_py.OracleWorld(ss[1])
+    for _ in range(15):
+        o.step()
+    assert o.hash() == singles[1].hash()

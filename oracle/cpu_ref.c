@@ -112,8 +112,12 @@ typedef struct {
 typedef struct { uint32_t key[4]; V3 la; float jn; V3 jt; } Warm;
 typedef struct { uint32_t a, b; PairP p; } PairOverride;
 /* A ball joint (articulation/src/bodies.rs:326-331 add_ball: the three translational axes of frame
- * A locked, the rotational ones free) between bodies a and b, with body-local frames (pos, rot). */
-typedef struct { uint32_t a, b; V3 fap, fbp; Q4 fra, frb; } Joint;
+ * A locked, the rotational ones free) between bodies a and b, with body-local frames (pos, rot);
+ * or a distance joint between the two centres of mass (bodies.rs:29-50, 273-285: no anchors and
+ * no damping, the rows the reference keeps bit for bit), with rest length, compliance (m/N, 0 =
+ * rigid) and the step's XPBD multiplier. */
+enum { J_BALL = 0, J_DIST = 1 };
+typedef struct { uint32_t a, b; V3 fap, fbp; Q4 fra, frb; int kind; float rest, compliance, lam; } Joint;
 /* A joint's warm rows (world.rs:250 warm_joints): row tags and impulses of its last solve. */
 typedef struct { uint8_t has, n, tag[3]; float lam[3]; } WarmJ;
 
@@ -295,7 +299,8 @@ static void mass_props(ShapeD s, float rho, float *mass, V3 *idiag, int *ok) {
 }
 
 /* ffi/src/lib.rs:677-902 add_*_body -> world.rs:595-622 add_body/add_collider */
-int32_t or_add_body(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic, float density) {
+static int32_t add_body(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic, float density,
+                        int with_collider) {
     if (w->n >= w->cap) return -1;
     uint32_t i = w->n++;
     ShapeD s; s.type = shape; s.d[0] = dims[0]; s.d[1] = dims[1]; s.d[2] = dims[2];
@@ -326,10 +331,19 @@ int32_t or_add_body(OrWorld *w, int shape, const float *dims, const float *pose,
     if (i >= w->cap) { /* capacity grows only through the FFI check; never here */ }
     w->awake[i] = 1; w->sticks[i] = 0;
     w->alive[i] = 1;
+    if (!with_collider) return (int32_t)i;
     uint32_t c = w->nc++;
     w->cbody[c] = i; w->cshape[c] = s; w->cmat[c] = MAT_DEFAULT;
     w->caabb[c] = aabb_of(s, w->pos[i], w->rot[i]);
     return (int32_t)i;
+}
+
+int32_t or_add_body(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic, float density) {
+    return add_body(w, shape, dims, pose, vel, dynamic, density, 1);
+}
+/* world.rs:595-613 add_body alone: a body with the shape's mass properties and no collider. */
+int32_t or_add_body_nc(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic, float density) {
+    return add_body(w, shape, dims, pose, vel, dynamic, density, 0);
 }
 
 static void drop_warm_of(OrWorld *w, uint32_t b) {
@@ -350,10 +364,23 @@ int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *fa, c
         w->joints = xrealloc(w->joints, sizeof(Joint) * w->capj);
         w->wj = xrealloc(w->wj, sizeof(WarmJ) * w->capj);
     }
-    Joint j = {a, b, v3(fa[0], fa[1], fa[2]), v3(fb[0], fb[1], fb[2]), q4(fa[3], fa[4], fa[5], fa[6]), q4(fb[3], fb[4], fb[5], fb[6])};
+    Joint j = {a, b, v3(fa[0], fa[1], fa[2]), v3(fb[0], fb[1], fb[2]), q4(fa[3], fa[4], fa[5], fa[6]), q4(fb[3], fb[4], fb[5], fb[6]),
+               J_BALL, 0.0f, 0.0f, 0.0f};
     w->joints[w->nj] = j;
     memset(&w->wj[w->nj], 0, sizeof(WarmJ));
     return (int32_t)w->nj++;
+}
+/* world.rs:537-539 add_distance_joint -> bodies.rs:273-285: rest = fin(rest),
+ * compliance = max(fin(compliance), 0). */
+int32_t or_add_distance_joint(OrWorld *w, uint32_t a, uint32_t b, float rest, float compliance) {
+    float z[7] = {0, 0, 0, 0, 0, 0, 1};
+    int32_t q = or_add_ball_joint(w, a, b, z, z);
+    if (q < 0) return q;
+    Joint *j = &w->joints[q];
+    j->kind = J_DIST;
+    j->rest = dfin(rest);
+    j->compliance = fmaxf(dfin(compliance), 0.0f);
+    return q;
 }
 uint32_t or_num_joints(const OrWorld *w) { return w->nj; }
 /* Joint q's warm rows: returns their count (0: none stored), tags and impulses into the arrays (3 each). */
@@ -688,6 +715,27 @@ static uint32_t ball_rows(const OrWorld *w, const Joint *j, JRow *out) {
     return n;
 }
 
+/* solver.rs:795-820: the legacy row of a rigid distance joint without anchors or damping, run in
+ * each velocity iteration with the joint batches: a Baumgarte velocity row along the centre line,
+ * not warm-started and not accumulated.  Returns its velocity change |j| (wa + wb), the row's
+ * residual in residual mode. */
+static float dist_vel_row(OrWorld *w, const Joint *j, float dt) {
+    V3 pa = w->pos[j->a], pb = w->pos[j->b];
+    V3 dir = vsub(pb, pa);
+    float len = vlen(dir);
+    if (len < 1.0e-6f) return 0.0f;
+    V3 n = vdivs(dir, len);
+    float ia = w->inv_mass[j->a], ib = w->inv_mass[j->b];
+    float isum = ia + ib;
+    if (isum < 1.0e-9f) return 0.0f;
+    float vrel = vdot(vsub(w->lin[j->b], w->lin[j->a]), n);
+    float bias = (len - j->rest) * (0.1f / dt);
+    float jm = -(vrel + bias) / isum;
+    apply_impulse(w, j->a, vmul(vneg(n), jm));
+    apply_impulse(w, j->b, vmul(n, jm));
+    return fabsf(jm) * isum;
+}
+
 static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, const V3 *pre_ang, float dt) {
     const float beta = 0.05f, slop = 0.005f;
     uint32_t nb = w->n;
@@ -744,7 +792,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     for (uint32_t q = 0; q < w->nj; q++) {
         const Joint *j = &w->joints[q];
         jr0[q] = njr;
-        if (w->inv_mass[j->a] > 0.0f || w->inv_mass[j->b] > 0.0f) njr += ball_rows(w, j, jr + njr);
+        if (j->kind == J_BALL && (w->inv_mass[j->a] > 0.0f || w->inv_mass[j->b] > 0.0f)) njr += ball_rows(w, j, jr + njr);
         jr1[q] = njr;
     }
     float *racc = calloc(njr + 1, 4);
@@ -806,6 +854,18 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     uint32_t *par = malloc(sizeof(uint32_t) * (nb + 1));
     uint8_t *jw = calloc(nb + 1, 1);
     for (uint32_t k = 0; k < njr; k++) { if (jr[k].wa > 0.0f) jw[jr[k].a] = 1; if (jr[k].wb > 0.0f) jw[jr[k].b] = 1; }
+    /* a rigid distance joint's legacy row (solver.rs:795-820) writes each side with mass; a compliant
+     * one acts at position level only */
+    uint8_t *dact = calloc(w->nj + 1, 1);
+    uint32_t *disl = calloc(w->nj + 1, 4);
+    for (uint32_t q = 0; q < w->nj; q++) {
+        const Joint *j = &w->joints[q];
+        if (j->kind != J_DIST || j->compliance > 0.0f) continue;
+        if (!(w->inv_mass[j->a] > 0.0f || w->inv_mass[j->b] > 0.0f)) continue;
+        dact[q] = 1;
+        if (w->inv_mass[j->a] > 0.0f) jw[j->a] = 1;
+        if (w->inv_mass[j->b] > 0.0f) jw[j->b] = 1;
+    }
     for (uint32_t b = 0; b <= nb; b++) par[b] = b;
     for (uint32_t i = 0; i < nct; i++) {
         int wa = cs[i].inv_a > 0.0f, wb = !cs[i].terrain && cs[i].inv_b > 0.0f;
@@ -818,6 +878,13 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     for (uint32_t k = 0; k < njr; k++) {
         if (jr[k].wa > 0.0f && jr[k].wb > 0.0f) {
             uint32_t x = uf_find(par, jr[k].a), y = uf_find(par, jr[k].b);
+            if (x != y) par[x > y ? x : y] = x < y ? x : y;
+        }
+    }
+    for (uint32_t q = 0; q < w->nj; q++) {
+        const Joint *j = &w->joints[q];
+        if (dact[q] && w->inv_mass[j->a] > 0.0f && w->inv_mass[j->b] > 0.0f) {
+            uint32_t x = uf_find(par, j->a), y = uf_find(par, j->b);
             if (x != y) par[x > y ? x : y] = x < y ? x : y;
         }
     }
@@ -837,8 +904,17 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         jisl[k] = uf_find(par, jr[k].wa > 0.0f ? jr[k].a : jr[k].b);
         isl_rows[jisl[k]]++;
     }
+    for (uint32_t q = 0; q < w->nj; q++) {
+        if (!dact[q]) continue;
+        const Joint *j = &w->joints[q];
+        disl[q] = uf_find(par, w->inv_mass[j->a] > 0.0f ? j->a : j->b);
+        isl_rows[disl[q]]++;
+    }
     /* the sweep statistics count a joint with rows as one entry of its island, like a contact */
-    for (uint32_t q = 0; q < w->nj; q++) if (jr1[q] > jr0[q]) isl_n[jisl[jr0[q]]]++;
+    for (uint32_t q = 0; q < w->nj; q++) {
+        if (jr1[q] > jr0[q]) isl_n[jisl[jr0[q]]]++;
+        else if (dact[q]) isl_n[disl[q]]++;
+    }
 
     /* PGS iterations (solver.rs:788-880), serial in contact order == the level schedule */
     uint32_t iters = w->resid_on ? w->resid_max : w->iv;
@@ -847,19 +923,28 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     for (uint32_t it = 0; it < iters; it++) {
         int active = 0;
         for (uint32_t b = 0; b <= nb; b++) isl_r[b] = 0.0f;
-        /* A. joint rows, serial in joint order (the joint level batches, solver.rs:791-835) */
-        for (uint32_t k = 0; k < njr; k++) {
-            const JRow *r = &jr[k];
-            if (isl_done[jisl[k]]) continue;
-            active = 1;
-            float v = jrow_vel(w, r);
-            float old = racc[k];
-            float nwv = old + (0.0f - v) * r->m;
-            racc[k] = nwv;
-            if (nwv != old) jrow_apply(w, r, nwv - old);
-            if (w->resid_on) {
-                float rr = fabsf(nwv - old) / r->m;
-                if (rr > isl_r[jisl[k]]) isl_r[jisl[k]] = rr;
+        /* A. joints, serial in joint order (the joint level batches, solver.rs:791-835) */
+        for (uint32_t q = 0; q < w->nj; q++) {
+            if (dact[q]) {
+                if (isl_done[disl[q]]) continue;
+                active = 1;
+                float rr = dist_vel_row(w, &w->joints[q], dt);
+                if (w->resid_on && rr > isl_r[disl[q]]) isl_r[disl[q]] = rr;
+                continue;
+            }
+            for (uint32_t k = jr0[q]; k < jr1[q]; k++) {
+                const JRow *r = &jr[k];
+                if (isl_done[jisl[k]]) continue;
+                active = 1;
+                float v = jrow_vel(w, r);
+                float old = racc[k];
+                float nwv = old + (0.0f - v) * r->m;
+                racc[k] = nwv;
+                if (nwv != old) jrow_apply(w, r, nwv - old);
+                if (w->resid_on) {
+                    float rr = fabsf(nwv - old) / r->m;
+                    if (rr > isl_r[jisl[k]]) isl_r[jisl[k]] = rr;
+                }
             }
         }
         /* B. contacts */
@@ -908,6 +993,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     w->stats.iters_sum = isum;
     w->stats.residual = resid;
     free(par); free(isl); free(isl_n); free(isl_it); free(isl_done); free(isl_r); free(isl_rows); free(jisl); free(jw);
+    free(dact); free(disl);
     /* joints solved this step replace their warm rows (solver.rs:945-952) */
     for (uint32_t q = 0; q < w->nj; q++) {
         uint32_t s0 = jr0[q], e0 = jr1[q];
@@ -1006,7 +1092,7 @@ static int islands_nonempty(const OrWorld *w, const Contact *cts, uint32_t n) {
 }
 
 /* ------------------------------------------------------------------ sin/cos (glam from_axis_angle) */
-void or_sincosf(float x, float *s, float *c) { or_sincos_d(x, s, c); }
+void or_sincosf(float x, float *s, float *c) { or_sincos_libm(x, s, c); }
 /* Quat::from_axis_angle: (axis * sin(a/2), cos(a/2)) */
 static Q4 from_axis_angle(V3 axis, float angle) {
     float s, c;
@@ -1083,6 +1169,39 @@ static void ball_pos(OrWorld *w, const Joint *j, const V3 *spos, const Q4 *srot,
         orient_delta(w, a, dfin_v(vmul(vneg(m3v(iaw, rau)), dl)));
         orient_delta(w, b, dfin_v(vmul(m3v(ibw, rbu), dl)));
     }
+}
+
+/* bodies.rs:501-548 solve_distance_row: the XPBD row of a distance joint without anchors or
+ * damping, corrected along the start-of-step centre line (SHAKE) unless the rod turned more than
+ * SHAKE_MAX_TURN = 0.5 rad this step (cos 0.5 = 0x3f60a940 as f32), the multiplier accumulated
+ * over the step's iterations. */
+static void dist_pos(OrWorld *w, Joint *j, const V3 *spos, float dt) {
+    const float cos_max_turn = 0.87758255f;
+    uint32_t a = j->a, b = j->b;
+    float wa = w->inv_mass[a], wb = w->inv_mass[b];
+    float wsum = dfin(wa + wb);
+    if (wsum <= 0.0f) return;
+    V3 d = vsub(w->pos[b], w->pos[a]);
+    float len2 = vlen2(d);
+    if (!isfinite(len2) || len2 <= 1.0e-20f) return;
+    float len = sqrtf(len2);
+    if (len <= 1.0e-6f) return;
+    V3 ncur = normalize_fin(vdivs(d, len));
+    if (viszero(ncur)) return;
+    V3 n = ncur;
+    V3 n0 = normalize_fin(vsub(spos[b], spos[a]));
+    if (!viszero(n0) && vdot(n0, ncur) > cos_max_turn) n = n0;
+    float c = dfin(len - dfin(j->rest));
+    float dt2 = fmaxf(dt * dt, 1.0e-12f);
+    float alpha = j->compliance <= 0.0f ? 0.0f : dfin(fmaxf(dfin(j->compliance), 0.0f) / dt2);
+    float denom = dfin(wsum + alpha);
+    if (denom <= 0.0f) return;
+    float lold = dfin(j->lam);
+    float lam = dfin(-(c + dfin(alpha * lold)) / (denom * vdot(n, ncur)));
+    if (fabsf(lam) <= 1.0e-9f) return;
+    j->lam = dfin(lold + lam);
+    pos_delta(w, a, dfin_v(vmul(vneg(n), dfin(lam * wa))));
+    pos_delta(w, b, dfin_v(vmul(n, dfin(lam * wb))));
 }
 
 typedef struct { uint32_t a, b; int has_b; V3 la, lb, pbw, n; float pen0; } PosC;
@@ -1327,11 +1446,14 @@ void or_step(OrWorld *w, float dt) {
             srot[i] = live ? orot[i] : w->rot[i];
             frozen[i] = !live;
         }
+        /* Joints::begin_step (bodies.rs:308-318): the step's multipliers start at zero */
+        for (uint32_t q = 0; q < w->nj; q++) w->joints[q].lam = 0.0f;
         for (uint32_t it = 0; it < w->ip; it++) {
             for (uint32_t q = 0; q < w->nj; q++) {
-                const Joint *j = &w->joints[q];
+                Joint *j = &w->joints[q];
                 if (frozen[j->a] && frozen[j->b]) continue;
-                ball_pos(w, j, spos, srot, dt);
+                if (j->kind == J_DIST) dist_pos(w, j, spos, dt);
+                else ball_pos(w, j, spos, srot, dt);
             }
             project(w, pc, npc, slin, sang, 0);
         }

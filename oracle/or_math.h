@@ -117,11 +117,13 @@ static inline M3 m3quat(Q4 q) {
                   v3(xz + wy, yz - wx, 1.0f - (xx + yy)));
 }
 
-/* sin/cos/exp evaluated in double and rounded once to f32 (Taylor series on |r| <= pi/4 after
- * Cody-Waite reduction by pi/2; exp by 2^k * Taylor(r), |r| <= ln2/2).  The reference calls the
- * libm 0.2.15 sincosf / platform expf, which agree with a once-rounded evaluation except in rare
- * last-ulp cases; the device library uses the same definition, so oracle and device agree bit
- * for bit.  Pinned against glibc within 1 ulp (tests/test_oracle_core.py). */
+/* exp, and sin/cos beyond libm's medium range, evaluated in double and rounded once to f32
+ * (Taylor series on |r| <= pi/4 after Cody-Waite reduction by pi/2; exp by 2^k * Taylor(r),
+ * |r| <= ln2/2).  The reference's Stribeck term calls the platform expf (solver.rs:430), which
+ * no golden pins; it agrees with a once-rounded evaluation except in rare last-ulp cases, and it
+ * drops out bit for bit when mu_s == mu_k.  The device library uses the same definitions, so
+ * oracle and device agree bit for bit.  Pinned against glibc within 1 ulp
+ * (tests/test_oracle_core.py). */
 static inline double or_sin_k(double r) {
     double r2 = r * r, t = 1.0 / 51090942171709440000.0;
     t = t * r2 - 1.0 / 121645100408832000.0; t = t * r2 + 1.0 / 355687428096000.0;
@@ -139,6 +141,72 @@ static inline double or_cos_k(double r) {
     t = t * r2 - 1.0 / 720.0;              t = t * r2 + 1.0 / 24.0;
     t = t * r2 - 0.5;
     return 1.0 + r2 * t;
+}
+/* libm 0.2.15 sincosf (Cargo.lock:1476-1479; glam 0.27's Quat::from_axis_angle /
+ * from_rotation_* call it through its `libm` feature).  That crate ports musl's sincosf.c, whose
+ * kernels are FreeBSD msun's k_sinf.c / k_cosf.c: fixed polynomials in double, rounded once to
+ * f32, after reduction by multiples of pi/2 (rem_pio2f.c for |x| > 9pi/4).  Restated from that
+ * published algorithm; pinned by the distance-joint goldens (distance_bits_test.rs:47-63), which
+ * run every gyroscopic turn through it.  |x| >= 2^28 pi/2 (Payne-Hanek reduction in libm) falls
+ * back to the once-rounded Taylor evaluation below: unpinned, and unreachable from a step. */
+static inline float or_k_sinf(double x) {
+    const double S1 = -0x15555554cbac77.0p-55, S2 = 0x111110896efbb2.0p-59;
+    const double S3 = -0x1a00f9e2cae774.0p-65, S4 = 0x16cd878c3b46a7.0p-71;
+    double z = x * x, w = z * z, r = S3 + z * S4, s = z * x;
+    return (float)((x + s * (S1 + z * S2)) + s * w * r);
+}
+static inline float or_k_cosf(double x) {
+    const double C0 = -0x1ffffffd0c5e81.0p-54, C1 = 0x155553e1053a42.0p-57;
+    const double C2 = -0x16c087e80f1e27.0p-62, C3 = 0x199342e0ee5069.0p-68;
+    double z = x * x, w = z * z, r = C2 + z * C3;
+    return (float)(((1.0 + z * C0) + w * C1) + (w * z) * r);
+}
+static inline void or_sincos_d(float xf, float *s, float *c);
+static inline void or_sincos_libm(float x, float *s, float *c) {
+    const double P1 = 1.0 * 1.57079632679489661923, P2 = 2.0 * 1.57079632679489661923;
+    const double P3 = 3.0 * 1.57079632679489661923, P4 = 4.0 * 1.57079632679489661923;
+    uint32_t ix; memcpy(&ix, &x, 4);
+    int sign = (int)(ix >> 31);
+    ix &= 0x7fffffffu;
+    double xd = (double)x;
+    if (ix <= 0x3f490fdau) {                      /* |x| <= pi/4 */
+        if (ix < 0x39800000u) { *s = x; *c = 1.0f; return; } /* |x| < 2^-12 */
+        *s = or_k_sinf(xd); *c = or_k_cosf(xd); return;
+    }
+    if (ix <= 0x407b53d1u) {                      /* |x| <= 5pi/4 */
+        if (ix <= 0x4016cbe3u) {                  /* |x| <= 3pi/4 */
+            if (sign) { *s = -or_k_cosf(xd + P1); *c = or_k_sinf(xd + P1); }
+            else { *s = or_k_cosf(P1 - xd); *c = or_k_sinf(P1 - xd); }
+            return;
+        }
+        double y = sign ? xd + P2 : xd - P2;
+        *s = -or_k_sinf(y); *c = -or_k_cosf(y); return;
+    }
+    if (ix <= 0x40e231d5u) {                      /* |x| <= 9pi/4 */
+        if (ix <= 0x40afeddfu) {                  /* |x| <= 7pi/4 */
+            if (sign) { *s = or_k_cosf(xd + P3); *c = -or_k_sinf(xd + P3); }
+            else { *s = -or_k_cosf(xd - P3); *c = or_k_sinf(xd - P3); }
+            return;
+        }
+        double y = sign ? xd + P4 : xd - P4;
+        *s = or_k_sinf(y); *c = or_k_cosf(y); return;
+    }
+    if (ix >= 0x7f800000u) { *s = *c = x - x; return; }
+    if (ix >= 0x4dc90fdbu) { or_sincos_d(x, s, c); return; }
+    /* rem_pio2f, medium size: n = rint(x 2/pi) with the 1.5 2^52 trick, y = x - n (pio2_1 + pio2_1t) */
+    const double TOINT = 1.5 / 2.220446049250313080847e-16;
+    const double INV_PIO2 = 6.36619772367581382433e-01;
+    const double PIO2_1 = 1.57079631090164184570e+00, PIO2_1T = 1.58932547735281966916e-08;
+    double fn = (xd * INV_PIO2 + TOINT) - TOINT;
+    int n = (int)fn;
+    double y = xd - fn * PIO2_1 - fn * PIO2_1T;
+    float sv = or_k_sinf(y), cv = or_k_cosf(y);
+    switch (n & 3) {
+        case 0: *s = sv; *c = cv; break;
+        case 1: *s = cv; *c = -sv; break;
+        case 2: *s = -sv; *c = -cv; break;
+        default: *s = -cv; *c = sv; break;
+    }
 }
 static inline void or_sincos_d(float xf, float *s, float *c) {
     if (!isfinite(xf)) { *s = NAN; *c = NAN; return; }

@@ -88,6 +88,12 @@ void or_step_hash(const OrWorld *w, uint8_t out32[32]);
 /* world.rs:718 add_ball_joint: bodies a, b (indices), frames {px,py,pz,qx,qy,qz,qw} local to each
  * body.  Returns the joint index, or -1. */
 int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *frame_a, const float *frame_b);
+/* world.rs:537 add_distance_joint between the centres of mass of a and b: rest length (m),
+ * compliance (m/N, 0 = rigid).  Returns the joint index, or -1. */
+int32_t or_add_distance_joint(OrWorld *w, uint32_t a, uint32_t b, float rest, float compliance);
+/* world.rs:595 add_body without add_collider: the shape only sets the mass properties. */
+int32_t or_add_body_nc(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic,
+                       float density);
 uint32_t or_num_joints(const OrWorld *w);
 uint32_t or_get_joint_warm(const OrWorld *w, uint32_t q, uint8_t *tags, float *lams);
 int or_shake_ok(float qw);

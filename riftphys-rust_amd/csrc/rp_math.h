@@ -1,9 +1,9 @@
 // Device f32 arithmetic for the step path, in the operation order of glam 0.27 as the reference
 // runs it on x86_64 (Vec3 scalar; Quat / Vec4 / Mat3A on the SSE2 path).  Compiled with
 // -ffp-contract=off and IEEE division / square root so every operation rounds exactly as
-// written.  sin/cos/exp are computed here in double precision and rounded once to f32 (the
-// reference calls libm sincosf / expf; both agree with a once-rounded double evaluation except
-// in rare ties).
+// written.  sin/cos follow libm's sincosf (double-precision kernels rounded once to f32); exp is
+// a once-rounded double evaluation (the reference calls the platform expf, which agrees with it
+// except in rare ties).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -131,7 +131,7 @@ RP_HD double rp_cos_kernel(double r) {
     t = t * r2 - 0.5;                         // 1/2!
     return 1.0 + r2 * t;
 }
-RP_HD void rp_sincosf(float xf, float *s, float *c) {
+RP_HD void rp_sincosf_taylor(float xf, float *s, float *c) {
     if (!isfinite(xf)) { *s = NAN; *c = NAN; return; }
     double x = (double)xf;
     const double PIO2_HI = 1.57079632673412561417e+00;  // first 33 bits of pi/2
@@ -148,6 +148,68 @@ RP_HD void rp_sincosf(float xf, float *s, float *c) {
         default: sv = -cr; cv = sr; break;
     }
     *s = (float)sv; *c = (float)cv;
+}
+// sincosf of libm 0.2.15 (Cargo.lock:1476-1479), which glam 0.27's Quat::from_axis_angle calls
+// through its `libm` feature: musl's sincosf.c with FreeBSD msun's k_sinf / k_cosf kernels, fixed
+// polynomials in double rounded once to f32, after reduction by multiples of pi/2 (rem_pio2f's
+// medium-size form beyond 9pi/4).  Arguments of 2^28 pi/2 and more take the Taylor form above.
+RP_HD float rp_k_sinf(double x) {
+    const double S1 = -0x15555554cbac77.0p-55, S2 = 0x111110896efbb2.0p-59;
+    const double S3 = -0x1a00f9e2cae774.0p-65, S4 = 0x16cd878c3b46a7.0p-71;
+    double z = x * x, w = z * z, r = S3 + z * S4, s = z * x;
+    return (float)((x + s * (S1 + z * S2)) + s * w * r);
+}
+RP_HD float rp_k_cosf(double x) {
+    const double C0 = -0x1ffffffd0c5e81.0p-54, C1 = 0x155553e1053a42.0p-57;
+    const double C2 = -0x16c087e80f1e27.0p-62, C3 = 0x199342e0ee5069.0p-68;
+    double z = x * x, w = z * z, r = C2 + z * C3;
+    return (float)(((1.0 + z * C0) + w * C1) + (w * z) * r);
+}
+RP_HD void rp_sincosf(float x, float *s, float *c) {
+    const double P1 = 1.0 * 1.57079632679489661923, P2 = 2.0 * 1.57079632679489661923;
+    const double P3 = 3.0 * 1.57079632679489661923, P4 = 4.0 * 1.57079632679489661923;
+    uint32_t ix;
+    __builtin_memcpy(&ix, &x, 4);
+    bool neg = (ix >> 31) != 0;
+    ix &= 0x7fffffffu;
+    double xd = (double)x;
+    if (ix <= 0x3f490fdau) {                      // |x| <= pi/4
+        if (ix < 0x39800000u) { *s = x; *c = 1.0f; return; }  // |x| < 2^-12
+        *s = rp_k_sinf(xd); *c = rp_k_cosf(xd); return;
+    }
+    if (ix <= 0x407b53d1u) {                      // |x| <= 5pi/4
+        if (ix <= 0x4016cbe3u) {                  // |x| <= 3pi/4
+            if (neg) { *s = -rp_k_cosf(xd + P1); *c = rp_k_sinf(xd + P1); }
+            else { *s = rp_k_cosf(P1 - xd); *c = rp_k_sinf(P1 - xd); }
+            return;
+        }
+        double y = neg ? xd + P2 : xd - P2;
+        *s = -rp_k_sinf(y); *c = -rp_k_cosf(y); return;
+    }
+    if (ix <= 0x40e231d5u) {                      // |x| <= 9pi/4
+        if (ix <= 0x40afeddfu) {                  // |x| <= 7pi/4
+            if (neg) { *s = rp_k_cosf(xd + P3); *c = -rp_k_sinf(xd + P3); }
+            else { *s = -rp_k_cosf(xd - P3); *c = rp_k_sinf(xd - P3); }
+            return;
+        }
+        double y = neg ? xd + P4 : xd - P4;
+        *s = rp_k_sinf(y); *c = rp_k_cosf(y); return;
+    }
+    if (ix >= 0x7f800000u) { *s = *c = x - x; return; }
+    if (ix >= 0x4dc90fdbu) { rp_sincosf_taylor(x, s, c); return; }
+    const double TOINT = 1.5 / 2.220446049250313080847e-16;   // 1.5 / DBL_EPSILON
+    const double INV_PIO2 = 6.36619772367581382433e-01;
+    const double PIO2_1 = 1.57079631090164184570e+00, PIO2_1T = 1.58932547735281966916e-08;
+    double fn = (xd * INV_PIO2 + TOINT) - TOINT;
+    int n = (int)fn;
+    double y = xd - fn * PIO2_1 - fn * PIO2_1T;
+    float sv = rp_k_sinf(y), cv = rp_k_cosf(y);
+    switch (n & 3) {
+        case 0: *s = sv; *c = cv; break;
+        case 1: *s = cv; *c = -sv; break;
+        case 2: *s = -sv; *c = -cv; break;
+        default: *s = -cv; *c = sv; break;
+    }
 }
 // exp(x) = 2^k exp(r), |r| <= ln2/2, Taylor to r^14
 RP_HD float rp_expf(float xf) {

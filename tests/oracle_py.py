@@ -7,6 +7,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+OR_SPHERE, OR_BOX, OR_CAPSULE = 0, 1, 2
 
 
 class OrStats(C.Structure):
@@ -42,6 +43,10 @@ def lib():
         L.or_set_pair_props.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float]
         L.or_add_body.restype = C.c_int32
         L.or_add_body.argtypes = [P, C.c_int, FP, FP, FP, C.c_int, C.c_float]
+        L.or_add_body_nc.restype = C.c_int32
+        L.or_add_body_nc.argtypes = [P, C.c_int, FP, FP, FP, C.c_int, C.c_float]
+        L.or_add_distance_joint.restype = C.c_int32
+        L.or_add_distance_joint.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
         L.or_remove_body.argtypes = [P, C.c_uint32]
         L.or_add_ball_joint.argtypes = [P, C.c_uint32, C.c_uint32, FP, FP]
         L.or_add_ball_joint.restype = C.c_int32

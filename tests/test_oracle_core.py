@@ -42,7 +42,11 @@ def test_blake3_chunk_boundaries_distinct_and_stable():
 
 
 # ---------------------------------------------------------------- sin / cos
-def test_sincos_rounds_once_from_exact():
+def test_sincos_libm_form_within_one_ulp():
+    """or_sincosf restates libm 0.2.15's sincosf (musl / FreeBSD msun kernels: fixed double
+    polynomials, so not always correctly rounded).  Checked against the exact value over every
+    reduction branch: within 1 ulp everywhere, correctly rounded in all but a few in 10^4, and
+    sin x = x, cos x = 1 exactly below 2^-12."""
     rng = np.random.default_rng(1)
     xs = np.concatenate([rng.uniform(-4, 4, 40000), rng.uniform(-1e-3, 1e-3, 5000), rng.uniform(-60, 60, 5000)]).astype(np.float32)
     s = C.c_float(); c = C.c_float()
@@ -56,7 +60,10 @@ def test_sincos_rounds_once_from_exact():
                 mism += 1
                 ulp = abs(int(np.array([g]).view(np.int32)[0]) - int(np.array([w]).view(np.int32)[0]))
                 assert ulp <= 1, (x, g, w)
-    assert mism <= 2, mism
+    assert mism <= 0.001 * 2 * len(xs), mism
+    for x in np.float32([1e-5, -2e-4, 2.4e-4, 1e-30]):
+        O.lib().or_sincosf(float(x), C.byref(s), C.byref(c))
+        assert np.float32(s.value) == x and c.value == 1.0
 
 
 # ---------------------------------------------------------------- narrowphase (narrowphase.rs:735-776)

@@ -120,6 +120,18 @@ RPhysResult rphys_world_capsule_sweep(const RPhysWorld *world, const RPhysCapsul
 
 RPhysResult rphys_last_error_copy(char *dst, uint32_t cap, uint32_t *out_len);   /* h:225, lib.rs:561-604 */
 
+/* h:227-243, lib.rs:1580-1693: the kinematic player controller is outside this library's scope.
+ * The entries exist so that a host written against the reference links unchanged.
+ * rphys_add_player validates its arguments as the reference does (out index set to 0xFFFFFFFF
+ * first; null -> ERR_NULL; non-finite pose, radius <= 0 or height <= 0 -> ERR_BAD_ARG), then
+ * returns RPHYS_ERR_INTERNAL with a message.  With no player ever added, set_input and get_pose
+ * answer as the reference does for any index then: ERR_NULL for a null world (or out_pose),
+ * ERR_BAD_ARG "player_idx out of bounds", with the identity pose and RPHYS_FALSE written first. */
+typedef struct RPhysPlayerDesc { RPhysIsometry start_pose; float radius; float height; float speed; } RPhysPlayerDesc;
+RPhysResult rphys_add_player(RPhysWorld *world, const RPhysPlayerDesc *desc, uint32_t *out_player_idx);
+RPhysResult rphys_player_set_input(RPhysWorld *world, uint32_t player_idx, RPhysVec3 move_dir);
+RPhysResult rphys_player_get_pose(const RPhysWorld *world, uint32_t player_idx, RPhysIsometry *out_pose, RPhysBool *out_grounded);
+
 /* lib.rs:107-115: the reference uses this type in rphys_world_get_contacts but its header never
  * defines it; 36 bytes, fields in the reference's order. */
 typedef struct RPhysContactEvent {
@@ -326,6 +338,18 @@ RPhysResult rphys_mlp_forward(const float *params, const float *feat12, uint32_t
 /* One training step's loss and gradient (the first 5,187 parameters: weights and biases) over
  * rphys_mlp_gradient_rows() given rows, without updating (checking hook). */
 RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const float *targ3, float *loss, float *grad);
+
+/* Test hooks (no reference counterpart), process-wide, 0 = default.
+ * rphys_debug_set_sweep_tiers: the largest island solved by sweep tiers 0 (one wave, default 256
+ * entries), 3 (four waves, 4,096) and 1 (1024 threads with LDS flags, 131,072), larger islands by
+ * tier 2 (flags in HBM); and the width from which an island's leading level runs as a GPU-wide
+ * launch (default 1,024).  Bounds are clamped to each tier's flag storage, so islands only move
+ * to wider tiers; every setting gives the same results.
+ * rphys_debug_set_initial_capacity: pair, terrain-candidate, contact and warm-cache capacities
+ * of scenes built afterwards (the warm cache still holds at least its entries + 1,024), so the
+ * overflow-and-replay path runs on small scenes. */
+RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, uint32_t max_tier1, uint32_t wide_min);
+RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm);
 
 #ifdef __cplusplus
 }

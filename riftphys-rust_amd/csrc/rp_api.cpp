@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <chrono>
 #include <cstdio>
@@ -177,6 +178,7 @@ struct Slot {  // device-side snapshot of a scene
     uint64_t *wkey; float4 *wla; float *wjn; float4 *wjt; uint32_t *woff;
     uint32_t *wj_mask; float *wj_lam;
     uint32_t cur; uint32_t nwarm;
+    size_t cap_warm = 0;  // warm-cache entries the slot's buffers hold
     std::vector<uint64_t> ticks;
 };
 
@@ -196,6 +198,7 @@ struct Scene {
     uint32_t graph_grids[5] = {0, 0, 0, 0, 0};  // the shape the graphs were captured with
     uint32_t max_extra = 0;
     std::vector<uint32_t> body_off, col_off;
+    std::vector<uint32_t> col_body;  // scene body of each collider, as built (contacts name colliders)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     TickParams graph_tp{};  // parameters the graphs were captured with (cur aside)
     std::vector<std::unique_ptr<Slot>> slots;
@@ -295,6 +298,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         const size_t NJ = nj + 1;
         ok &= B.alloc(d.jt_a, NJ) && B.alloc(d.jt_b, NJ) && B.alloc(d.jt_fap, NJ) && B.alloc(d.jt_far, NJ) && B.alloc(d.jt_fbp, NJ) &&
               B.alloc(d.jt_fbr, NJ) && B.alloc(d.jr_u, 3 * NJ) && B.alloc(d.jr_ra, NJ) && B.alloc(d.jr_rb, NJ) && B.alloc(d.jr_mask, NJ) &&
+              B.alloc(d.bk_wj_mask, NJ) && B.alloc(d.bk_wj_lam, 3 * NJ) &&
               B.alloc(d.jr_acc, 3 * NJ) && B.alloc(d.wj_mask, NJ) && B.alloc(d.wj_lam, 3 * NJ) && B.alloc(d.jadj_off, NB + 1) &&
               B.alloc(d.jadj, 2 * NJ) && B.alloc(d.w_joint_off, nw + 1);
     }
@@ -403,6 +407,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
          up(d.w_joint_off, jwoff.data(), jwoff.size() * 4);
     if (ok) ok = hip_ok(hipMemsetAsync(d.cnt, 0, CT_COUNT * 4, S->st), "memset");
     if (ok && ws[0]->mlp.size() == RP_MLP_NPARAM) ok = upload_model(*S, ws[0]->mlp.data());
+    S->col_body = cbody;
     if (ok) ok = hip_ok(hipStreamSynchronize(S->st), "upload sync");
     if (!ok) return nullptr;
     S->cur = cur;
@@ -432,9 +437,24 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
     return ok && hip_ok(hipStreamSynchronize(S.st), "download sync");
 }
 
+// Test hooks, process-wide: the sweep tiers' island-size bounds and the wide-level minimum (so
+// small scenes can be solved by every tier and by the split form), and the initial capacities
+// of new scenes (so the overflow-and-replay path runs on small scenes).  Zero: the default.
+static std::atomic<uint32_t> g_tier_max[3] = {{0}, {0}, {0}}, g_wide_min{0};
+static std::atomic<uint32_t> g_init_caps[4] = {{0}, {0}, {0}, {0}};
+
 static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
     TickParams tp{};
     tp.dt = dt;
+    // tier bounds: each within its tier's flag storage and not below the previous tier's
+    const uint32_t storage[3] = {256u, 4096u, 131072u};
+    uint32_t lo = 0;
+    for (int k = 0; k < 3; k++) {
+        uint32_t v = g_tier_max[k].load();
+        v = v ? std::min(v, storage[k]) : storage[k];
+        tp.tier_max[k] = lo = std::max(v, lo);
+    }
+    tp.wide_min = g_wide_min.load() ? g_wide_min.load() : RP_WIDE_MIN;
     // gravity.rs:85-87 q6 of the uniform field
     tp.gx = std::isfinite(w.gx) ? q6mul(w.gx) : 0.0f;
     tp.gy = std::isfinite(w.gy) ? q6mul(w.gy) : 0.0f;
@@ -466,6 +486,10 @@ static Caps initial_caps(const std::vector<HostWorld *> &ws) {
     for (auto *w : ws) nj += w->nj();
     c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + nj + 1024, 0x7FFFFFFF);  // the island lists hold joints too
     c.warm = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
+    if (g_init_caps[0].load()) c.pairs = g_init_caps[0].load();
+    if (g_init_caps[1].load()) c.cands = g_init_caps[1].load();
+    if (g_init_caps[2].load()) c.contacts = g_init_caps[2].load();
+    if (g_init_caps[3].load()) c.warm = g_init_caps[3].load();
     return c;
 }
 
@@ -1148,6 +1172,39 @@ RPhysResult rphys_world_capsule_sweep(const RPhysWorld *world, const RPhysCapsul
     return RPHYS_ERR_INTERNAL;
 }
 
+RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, uint32_t max_tier1, uint32_t wide_min) {
+    g_tier_max[0] = max_tier0; g_tier_max[1] = max_tier3; g_tier_max[2] = max_tier1; g_wide_min = wide_min;
+    return RPHYS_OK;
+}
+RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm) {
+    g_init_caps[0] = pairs; g_init_caps[1] = cands; g_init_caps[2] = contacts; g_init_caps[3] = warm;
+    return RPHYS_OK;
+}
+
+// lib.rs:1580-1693 player controller: argument checks as the reference, then out of scope
+RPhysResult rphys_add_player(RPhysWorld *world, const RPhysPlayerDesc *desc, uint32_t *out_player_idx) {
+    if (out_player_idx) *out_player_idx = 0xFFFFFFFFu;
+    if (!world || !desc || !out_player_idx) { set_err("rphys_add_player: null arg"); return RPHYS_ERR_NULL; }
+    if (!fin_iso(desc->start_pose)) return bad_arg("rphys_add_player", "pose contains NaN/Inf");
+    if (!(hfin(desc->radius) && desc->radius > 0.0f)) return bad_arg("rphys_add_player", "radius must be > 0");
+    if (!(hfin(desc->height) && desc->height > 0.0f)) return bad_arg("rphys_add_player", "height must be > 0");
+    set_err("rphys_add_player: the kinematic player controller is not part of the step library");
+    return RPHYS_ERR_INTERNAL;
+}
+RPhysResult rphys_player_set_input(RPhysWorld *world, uint32_t player_idx, RPhysVec3 move_dir) {
+    (void)player_idx;
+    if (!world) { set_err("rphys_player_set_input: null arg"); return RPHYS_ERR_NULL; }
+    if (!fin3(move_dir)) return bad_arg("rphys_player_set_input", "move_dir NaN/Inf");
+    return bad_arg("rphys_player_set_input", "player_idx out of bounds");  // no player can exist
+}
+RPhysResult rphys_player_get_pose(const RPhysWorld *world, uint32_t player_idx, RPhysIsometry *out_pose, RPhysBool *out_grounded) {
+    (void)player_idx;
+    if (out_pose) *out_pose = RPhysIsometry{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 1.0f}};
+    if (out_grounded) *out_grounded = RPHYS_FALSE;
+    if (!world || !out_pose) { set_err("rphys_player_get_pose: null arg"); return RPHYS_ERR_NULL; }
+    return bad_arg("rphys_player_get_pose", "player_idx out of bounds");
+}
+
 // terrain.rs:24-52 from_heights, material Grit (lib.rs:1479,1536)
 static void set_hf(HostWorld &w, uint32_t nx, uint32_t nz, float sx, float sz, float ox, float oz, std::vector<float> &&h) {
     w.hnx = nx; w.hnz = nz;
@@ -1346,15 +1403,29 @@ RPhysResult rphys_world_get_contacts(const RPhysWorld *cworld, RPhysContactEvent
     *out_count = (uint32_t)D.a.size();
     if (!out_buffer) return RPHYS_OK;
     uint32_t n = std::min(*out_count, buffer_capacity);
+    // Colliders resolve to bodies through the layout the tick ran with (a removal since then
+    // compacts the host's collider list); an entry whose body is gone is skipped, as lib.rs:1841-1854
+    // skips ids it cannot resolve.  Like the reference, the count is that of the last tick's
+    // contacts, and the events written are packed at the front of the buffer.
     const HostWorld &w = world->w;
+    const std::vector<uint32_t> &cb = world->scene->col_body;
+    auto body_of = [&](uint32_t c, uint32_t &id) {
+        if (c >= cb.size()) return false;
+        uint32_t b = cb[c];
+        if (b >= w.nb() || !(w.bflags[b] & BF_ALIVE)) return false;
+        id = b + 1;
+        return true;
+    };
+    uint32_t k = 0;
     for (uint32_t i = 0; i < n; i++) {
         RPhysContactEvent e;
-        e.body_id_a = w.col_body[D.a[i]] + 1;
-        e.body_id_b = D.b[i] == RP_TERRAIN ? RPHYS_TERRAIN_BODY_ID : w.col_body[D.b[i]] + 1;
+        if (!body_of(D.a[i], e.body_id_a)) continue;
+        if (D.b[i] == RP_TERRAIN) e.body_id_b = RPHYS_TERRAIN_BODY_ID;
+        else if (!body_of(D.b[i], e.body_id_b)) continue;
         e.position = {D.p[i].x, D.p[i].y, D.p[i].z};
         e.normal = {D.n[i].x, D.n[i].y, D.n[i].z};
         e.impulse = D.imp[i];
-        out_buffer[i] = e;
+        out_buffer[k++] = e;
     }
     return RPHYS_OK;
     GUARD_END
@@ -1592,9 +1663,14 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
     Scene &S = *b->scene;
     if (S.slots.size() <= slot) S.slots.resize(slot + 1);
     const Dev &d = S.d;
+    uint32_t nwarm = 0;
+    if (!hip_ok(hipMemcpyAsync(&nwarm, d.wc_off[S.cur] + d.nw, 4, hipMemcpyDeviceToHost, S.st), "snapshot") || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
+    // A slot made before a capacity rebuild can be smaller than today's warm cache: replace it.
+    if (S.slots[slot] && S.slots[slot]->cap_warm < nwarm) S.slots[slot].reset();
     if (!S.slots[slot]) {
         std::unique_ptr<Slot> sl(new Slot());
-        size_t NB = d.nb + 1, W = d.cap_warm;
+        size_t NB = d.nb + 1, W = std::max<size_t>(d.cap_warm, nwarm);
+        sl->cap_warm = W;
         bool ok = sl->buf.alloc(sl->pos, NB) && sl->buf.alloc(sl->rot, NB) && sl->buf.alloc(sl->lin, NB) && sl->buf.alloc(sl->ang, NB) &&
                   sl->buf.alloc(sl->last, NB) && sl->buf.alloc(sl->awake, NB) && sl->buf.alloc(sl->sticks, NB) &&
                   sl->buf.alloc(sl->wkey, W) && sl->buf.alloc(sl->wla, W) && sl->buf.alloc(sl->wjn, W) && sl->buf.alloc(sl->wjt, W) &&
@@ -1603,8 +1679,6 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
         S.slots[slot] = std::move(sl);
     }
     Slot &sl = *S.slots[slot];
-    uint32_t nwarm = 0;
-    if (!hip_ok(hipMemcpyAsync(&nwarm, d.wc_off[S.cur] + d.nw, 4, hipMemcpyDeviceToHost, S.st), "snapshot") || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
     size_t n = d.nb;
     auto cp = [&](void *dst, const void *src, size_t bytes) { return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.st), "snapshot copy"); };
     bool ok = cp(sl.pos, d.pos, 16 * n) && cp(sl.rot, d.rot, 16 * n) && cp(sl.lin, d.lin, 16 * n) && cp(sl.ang, d.ang, 16 * n) &&

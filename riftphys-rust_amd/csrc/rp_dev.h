@@ -71,6 +71,8 @@ struct Dev {
     // pre-tick backup (overflow replay)
     float4 *bk_pos, *bk_rot, *bk_lin, *bk_ang, *bk_last;
     uint32_t *bk_awake, *bk_sticks;
+    uint32_t *bk_wj_mask;  // joints' carried warm rows (nj, 3 nj)
+    float *bk_wj_lam;
     // per-body tick scratch
     float4 *pre_lin, *pre_ang;
     float *iiw;  // 9 per body
@@ -206,7 +208,7 @@ enum {
 // 2 acos(min(|w|, 1)) < 0.5 for w of q_start * conj(q_now), which (acosf, rounded as glibc does)
 // holds exactly when min(|w|, 1) >= this float (0.968912482)
 #define RP_SHAKE_COS_BITS 0x3f780aa6u
-#define RP_WIDE_MIN 1024u         // ranks of a level that one workgroup cannot take in one round
+#define RP_WIDE_MIN 1024u         // default wide level: ranks one workgroup cannot take in one round
 #define RP_WIDE_MAX_LEVELS 64u    // leading wide levels recorded per island
 
 // Optional side streams for the island sweeps: the four size tiers of one stage solve disjoint
@@ -224,4 +226,6 @@ struct TickParams {
     float resid_tol;
     uint32_t ws_mode;      // warm start: 0 previous frame (the reference), 1 zero, 2 learned
     uint32_t log_on;       // append (features, converged impulses) of solved contacts to the log
+    uint32_t tier_max[3];  // largest island of sweep tiers 0, 3 and 1 (within their flag storage)
+    uint32_t wide_min;     // ranks of a leading level that make it a GPU-wide level launch
 };

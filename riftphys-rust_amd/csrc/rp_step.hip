@@ -71,6 +71,11 @@ __global__ void k_bodies_pre(Dev d, TickParams tp) {
         }
         store_m9(d.iiw + 9 * b, inv_inertia_world(d, b, LQ(q)));
     }
+    // the joints' carried warm rows: the velocity solve rewrites them in place
+    GRID_STRIDE(q, d.nj) {
+        d.bk_wj_mask[q] = d.wj_mask[q];
+        for (uint32_t i = 0; i < 3; i++) d.bk_wj_lam[3 * q + i] = d.wj_lam[3 * q + i];
+    }
 }
 
 // ------------------------------------------------------------------ S2 AABBs
@@ -913,16 +918,19 @@ template <> struct Tier<3> { static constexpr uint32_t threads = 256, flag_bytes
 template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
 template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
 template <int TIER> constexpr bool BIG = TIER == 1 || TIER == 2;
-static __device__ __forceinline__ int tier_of(uint32_t n) {
-    return n <= Tier<0>::flag_bytes ? 0 : (n <= Tier<3>::flag_bytes ? 3 : (n <= Tier<1>::flag_bytes ? 1 : 2));
+// Island sizes up to tp.tier_max[0] go to tier 0, then up to tier_max[1] to tier 3, up to
+// tier_max[2] to tier 1, larger ones to tier 2.  The host keeps each bound within its tier's flag
+// storage (defaults: the storage itself), so lowering them only moves islands to wider tiers.
+static __device__ __forceinline__ int tier_of(uint32_t n, const TickParams &tp) {
+    return n <= tp.tier_max[0] ? 0 : (n <= tp.tier_max[1] ? 3 : (n <= tp.tier_max[2] ? 1 : 2));
 }
 // The islands of each sweep tier (by contact count), so each tier's launch walks only its own.
-__global__ void k_tier_lists(Dev d) {
+__global__ void k_tier_lists(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     GRID_STRIDE(isl, nisl) {
         const uint32_t n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - d.isl_start[isl];
-        const int t = tier_of(n);
+        const int t = tier_of(n, tp);
         d.tier_list[t][atomicAdd(&d.cnt[CT_TIER_N + t], 1u)] = isl;
     }
 }
@@ -1034,7 +1042,8 @@ struct PhaseClock {
 // or nullptr (list order, always valid) should the expansion not place every rank.
 template <class FL>
 static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa, const uint32_t *pb, const uint32_t *sa, const uint32_t *sb,
-                                              uint32_t *ex, unsigned long long *rounds, uint32_t *wide_off = nullptr, uint32_t *wide_k = nullptr) {
+                                              uint32_t *ex, unsigned long long *rounds, uint32_t wide_min, uint32_t *wide_off = nullptr,
+                                              uint32_t *wide_k = nullptr) {
     __shared__ uint32_t tail;
     const uint32_t s = I.s, n = I.n;
     if (threadIdx.x == 0) tail = 0;
@@ -1048,7 +1057,7 @@ static __device__ const uint32_t *level_order(Island<FL> &I, const uint32_t *pa,
     __syncthreads();  // every thread has the frontier's end before any appends to the next
     while (fs < fe) {
         // the island's leading run of wide levels: their start slots, then the run's end
-        if (leading && fe - fs >= RP_WIDE_MIN && nw < RP_WIDE_MAX_LEVELS) {
+        if (leading && fe - fs >= wide_min && nw < RP_WIDE_MAX_LEVELS) {
             if (threadIdx.x == 0) { wide_off[nw] = fs; wide_off[nw + 1] = fe; }
             nw++;
         } else {
@@ -1465,11 +1474,11 @@ static __device__ __forceinline__ void vshock_row(const Dev &d, const V3 up, uin
 }
 // Shock order of island isl in slots ss, recording (tiers 1 and 2) its leading wide levels.
 template <int TIER, class FL>
-static __device__ void shock_slots(const Dev &d, Island<FL> &I, uint32_t isl) {
+static __device__ void shock_slots(const Dev &d, const TickParams &tp, Island<FL> &I, uint32_t isl) {
     const uint32_t s = I.s, n = I.n;
     __shared__ uint32_t wide_ks;
     const uint32_t *exs = n > blockDim.x ? level_order(I, d.sh_pa, d.sh_pb, d.sh_sa, d.sh_sb, d.ex_shock, &d.ph[RP_PH_ROUNDS_SHOCK],
-                                                       BIG<TIER> ? d.lvl_off_sh + s : nullptr, BIG<TIER> ? &wide_ks : nullptr)
+                                                       tp.wide_min, BIG<TIER> ? d.lvl_off_sh + s : nullptr, BIG<TIER> ? &wide_ks : nullptr)
                                           : nullptr;
     build_slots(d, I, d.sh_ord, d.sh_pa, d.sh_pb, exs, Slots{d.ss_pa, d.ss_pb, d.ss_c});
     if constexpr (BIG<TIER>) {
@@ -1491,7 +1500,7 @@ static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> 
         else d.jn_pre[c] = d.jn[c];
     }
     __syncthreads();
-    shock_slots<TIER>(d, I, isl);
+    shock_slots<TIER>(d, tp, I, isl);
     if constexpr (BIG<TIER>) pc.lap(d, RP_PH_VEL_SHOCK_ORDER);
     if (BIG<TIER> && klsh) return;
     run_sweep(I, Slots{d.ss_pa, d.ss_pb, d.ss_c}, [&](uint32_t c) { vshock_row(d, up, c); });
@@ -1513,7 +1522,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         PhaseClock pc;
         __shared__ uint32_t wide_k;
         const uint32_t *exb = n > blockDim.x ? level_order(I, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, d.ex_base, &d.ph[RP_PH_ROUNDS_BASE],
-                                                           BIG<TIER> ? d.lvl_off + s : nullptr, BIG<TIER> ? &wide_k : nullptr)
+                                                           tp.wide_min, BIG<TIER> ? d.lvl_off + s : nullptr, BIG<TIER> ? &wide_k : nullptr)
                                               : nullptr;
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
         build_slots(d, I, d.isl_v2, d.ch_pa, d.ch_pb, exb, sb);
@@ -2036,7 +2045,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d,
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
-        shock_slots<TIER>(d, I, isl);
+        shock_slots<TIER>(d, tp, I, isl);
         const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ_SHOCK_ORDER);
         if (BIG<TIER> && klsh) continue;  // k_level_pshock + k_sweep_pshock_tail follow
@@ -2106,6 +2115,10 @@ __global__ void k_restore_backup(Dev d) {
     GRID_STRIDE(b, d.nb) {
         d.pos[b] = d.bk_pos[b]; d.rot[b] = d.bk_rot[b]; d.lin[b] = d.bk_lin[b]; d.ang[b] = d.bk_ang[b];
         d.last_lin[b] = d.bk_last[b]; d.awake[b] = d.bk_awake[b]; d.sticks[b] = d.bk_sticks[b];
+    }
+    GRID_STRIDE(q, d.nj) {
+        d.wj_mask[q] = d.bk_wj_mask[q];
+        for (uint32_t i = 0; i < 3; i++) d.wj_lam[3 * q + i] = d.bk_wj_lam[3 * q + i];
     }
 }
 
@@ -2245,7 +2258,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     };
     mark(RP_ST_PRE);
     uint32_t *scratch_total = d.cnt + CT_SCRATCH;
-    hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
+    hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb > d.nj ? d.nb : d.nj, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_BROAD);
     hipLaunchKernelGGL(k_bp_sort, dim3(d.nw), dim3(1024), 0, st, d);
@@ -2287,7 +2300,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     scan_u32(d.isl_flag, d.cap_contacts, d.scan_tmp, d.cnt + CT_ISLANDS, st);
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     for (uint32_t t = 0; t < 4; t++) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_TIER_N + t, 0u);
-    hipLaunchKernelGGL(k_tier_lists, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_tier_lists, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_MAX, 0u);
@@ -2473,5 +2486,5 @@ extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, h
 }
 
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st) {
-    hipLaunchKernelGGL(k_restore_backup, dim3(grid_for(d.nb, 256)), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_restore_backup, dim3(grid_for(d.nb > d.nj ? d.nb : d.nj, 256)), dim3(256), 0, st, d);
 }

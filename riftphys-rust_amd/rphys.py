@@ -77,6 +77,10 @@ class ContactEvent(C.Structure):
     _fields_ = [("body_id_a", C.c_uint32), ("body_id_b", C.c_uint32), ("position", Vec3), ("normal", Vec3), ("impulse", C.c_float)]
 
 
+class PlayerDesc(C.Structure):
+    _fields_ = [("start_pose", Isometry), ("radius", C.c_float), ("height", C.c_float), ("speed", C.c_float)]
+
+
 class StepStats(C.Structure):
     _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("terrain_candidates", C.c_uint32), ("island_contacts", C.c_uint32),
                 ("islands", C.c_uint32), ("warm_entries", C.c_uint32), ("pgs_iters", C.c_uint32), ("pgs_iters_sum", C.c_uint32)]
@@ -138,6 +142,9 @@ EXPORTS = [
     ("rphys_world_set_heightfield_raw32_square", U32, [P, P, U32, F32, F32, F32, F32, F32]),
     ("rphys_world_clear_heightfield", U32, [P]),
     ("rphys_world_get_contacts", U32, [P, C.POINTER(ContactEvent), U32, PU32]),
+    ("rphys_add_player", U32, [P, C.POINTER(PlayerDesc), PU32]),
+    ("rphys_player_set_input", U32, [P, U32, Vec3]),
+    ("rphys_player_get_pose", U32, [P, U32, C.POINTER(Isometry), PU32]),
     ("rphys_world_snapshot_create", P, [P]),
     ("rphys_world_snapshot_restore", U32, [P, P]),
     ("rphys_world_snapshot_free", None, [P]),
@@ -181,6 +188,8 @@ EXPORTS = [
     ("rphys_batch_train_warm_start", U32, [P, C.POINTER(MlpTrainConfig), PF32, PF32]),
     ("rphys_mlp_forward", U32, [PF32, PF32, U32, PF32]),
     ("rphys_mlp_gradient", U32, [PF32, PF32, PF32, PF32, PF32]),
+    ("rphys_debug_set_sweep_tiers", U32, [U32, U32, U32, U32]),
+    ("rphys_debug_set_initial_capacity", U32, [U32, U32, U32, U32]),
 ]
 
 _lib = None

@@ -59,6 +59,40 @@ int main(void) {
     assert C.sizeof(rphys.BoxBodyDesc) == box
 
 
+def test_player_controller_entries_link_and_answer_as_the_reference():
+    """riftphys_c.h:227-243 / lib.rs:1580-1693: the player entries exist (a host written against
+    the reference links), validate as the reference does, and report the controller as outside
+    this library (ERR_INTERNAL), so no player index is ever valid."""
+    L = rphys.lib()
+    assert C.sizeof(rphys.PlayerDesc) == 40
+    w = L.rphys_world_create(8, 8)
+    try:
+        d = rphys.PlayerDesc()
+        d.start_pose = rphys.iso([0, 1, 0], [0, 0, 0, 1])
+        d.radius, d.height, d.speed = 0.3, 1.8, 4.0
+        idx = C.c_uint32(7)
+        assert L.rphys_add_player(w, None, C.byref(idx)) == rphys.ERR_NULL and idx.value == 0xFFFFFFFF
+        d.radius = 0.0
+        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.ERR_BAD_ARG
+        d.radius = 0.3
+        d.start_pose.pos.x = float("nan")
+        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.ERR_BAD_ARG
+        d.start_pose.pos.x = 0.0
+        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.ERR_INTERNAL
+        assert idx.value == 0xFFFFFFFF and "player" in last_error()
+        assert L.rphys_player_set_input(w, 0, rphys.Vec3(1, 0, 0)) == rphys.ERR_BAD_ARG
+        assert L.rphys_player_set_input(w, 0, rphys.Vec3(float("inf"), 0, 0)) == rphys.ERR_BAD_ARG
+        assert L.rphys_player_set_input(None, 0, rphys.Vec3(1, 0, 0)) == rphys.ERR_NULL
+        pose = rphys.Isometry()
+        pose.pos.x = 5.0
+        g = C.c_uint32(9)
+        assert L.rphys_player_get_pose(w, 0, C.byref(pose), C.byref(g)) == rphys.ERR_BAD_ARG
+        assert (pose.pos.x, pose.rot.w, g.value) == (0.0, 1.0, 0)
+        assert L.rphys_player_get_pose(w, 0, None, None) == rphys.ERR_NULL
+    finally:
+        L.rphys_world_destroy(w)
+
+
 def box_desc(**kw):
     d = rphys.BoxBodyDesc()
     d.pose = rphys.iso([0, 1, 0], [0, 0, 0, 1])

@@ -1,0 +1,90 @@
+"""Device paths that the default sizes leave idle, forced on small scenes and checked against the
+oracle bit for bit: every sweep tier (one wave, four waves, 1024 threads with LDS flags, 1024
+threads with flags in HBM), the GPU-wide level launches of the split form, and the capacity
+overflow that rolls a tick back, grows the buffers and replays it (with ball joints, whose warm
+rows the failed tick has already rewritten), including rollback slots made before the growth.
+"""
+import numpy as np
+import pytest
+
+import oracle_py
+import rphys
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def hooks():
+    L = rphys.lib()
+    yield L
+    L.rphys_debug_set_sweep_tiers(0, 0, 0, 0)
+    L.rphys_debug_set_initial_capacity(0, 0, 0, 0)
+
+
+def run_pair(scene, ticks, check_every=1):
+    o, g = oracle_py.OracleWorld(scene), rphys.World(scene)
+    for t in range(ticks):
+        o.step()
+        g.step()
+        if (t + 1) % check_every == 0 or t + 1 == ticks:
+            assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+    return g
+
+
+# (max tier 0, max tier 3, max tier 1, wide-level minimum; 0 = default): every island of two or
+# more entries in tier 2 (flags in HBM); a spread over all four tiers; the split form with narrow
+# "wide" levels in tier 1 (LDS flags) and in tier 2
+TIERS = {"all_hbm_flags": (1, 1, 1, 0), "spread": (8, 40, 200, 0), "split_lds": (1, 1, 0, 8), "split_hbm": (1, 1, 1, 8)}
+
+
+@pytest.mark.parametrize("tiers", list(TIERS))
+@pytest.mark.parametrize("scene", ["mixed", "lattice", "joints"])
+def test_sweep_tiers_and_split_form_bit_exact(hooks, tiers, scene):
+    hooks.rphys_debug_set_sweep_tiers(*TIERS[tiers])
+    if scene == "mixed":
+        s, ticks = scenes.small_mixed(n=96, seed=7), 40
+    elif scene == "lattice":
+        s, ticks = scenes.box_lattice(), 30
+    else:
+        s, ticks = scenes.ragdolls_small(), 40
+    g = run_pair(s, ticks)
+    if tiers.startswith("split") and scene == "lattice":
+        _, counters = g.step_profiled(1)
+        assert counters["wide_levels"] > 0, "the split form did not run"
+
+
+def test_residual_mode_in_hbm_flag_tier(hooks):
+    hooks.rphys_debug_set_sweep_tiers(1, 1, 1, 0)
+    s = scenes.box_lattice()
+    s["residual"] = (1.0e-4, 40)
+    run_pair(s, 20)
+
+
+def test_overflow_replay_with_joints_bit_exact(hooks):
+    """Tiny initial capacities: the first ticks with contacts overflow the pair, contact and
+    warm-cache buffers; the device rolls each failed tick back (joint warm rows included), grows,
+    and replays, bit-exact with the oracle every tick."""
+    hooks.rphys_debug_set_initial_capacity(64, 256, 64, 16)
+    s = scenes.ragdolls_small(n_towers=6, tower_h=5, n_ragdolls=8)
+    run_pair(s, 60)
+
+
+def test_rollback_slot_made_before_growth(hooks):
+    """A rollback slot taken while the warm-cache capacity is small, then re-taken after a rebuild
+    grew it: the slot is reallocated, and the restored window replays its hashes."""
+    hooks.rphys_debug_set_initial_capacity(0, 0, 0, 16)
+    worlds = [rphys.World(scenes.box_lattice(seed=scenes.seed_for(1, k))) for k in range(2)]
+    b = rphys.Batch(worlds)
+    b.snapshot(0)  # empty warm cache, capacity ~1,024 entries
+    b.step_n(40)   # the settled lattices hold several thousand warm entries: rebuilt larger
+    b.snapshot(0)
+    b.step_n(8)
+    want = b.hashes()
+    b.restore(0)
+    b.step_n(8)
+    assert b.hashes() == want
+    o = oracle_py.OracleWorld(scenes.box_lattice(seed=scenes.seed_for(1, 1)))
+    for _ in range(48):
+        o.step()
+    assert o.hash() == want[1]

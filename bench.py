@@ -383,6 +383,171 @@ def towers_bench(a):
             "roofline": roofline_of(stages, ctr, iv, ip, wl["workload"]), "cpu_baseline": base}
 
 
+def residual_pass(step_profiled, nb, ticks, chunk=10):
+    """Runs `ticks` residual-mode ticks through a profiled stepper; throughput and sweep counts."""
+    el, acc, done = 0.0, {}, 0
+    stages = {}
+    while done < ticks:
+        k = min(chunk, ticks - done)
+        t1 = time.perf_counter()
+        st, c = step_profiled(k)
+        el += time.perf_counter() - t1
+        merge_profile(stages, acc, st, c)
+        done += k
+    ic = max(acc["island_contacts"], 1.0)
+    icc = acc["island_contacts_converged"]
+    return {"body_steps_per_s": nb * ticks / el, "ms_per_tick": 1000.0 * el / ticks, "mean_sweeps_per_entry": acc["pgs_iters_sum"] / ic,
+            "converged_entry_frac": icc / ic, "mean_sweeps_converged_islands": acc["pgs_iters_sum_converged"] / icc if icc else None,
+            "islands_converged_frac": acc["islands_converged"] / max(acc["islands"], 1.0), "contacts_per_tick": acc["contacts"] / ticks,
+            "learned_contact_frac": acc["warm_start_learned"] / max(acc["contacts"], 1.0)}, stages, acc
+
+
+def atscale_bench(a):
+    """The headline comparison: learned + batched vs plain, PGS to residual 1e-4.
+
+    `--worlds` (default 64) worlds of 16,384 bodies each: 669 towers of 20 boxes (he 0.25), 4 loose
+    boxes and 200 chains of 15 capsules joined by 14 ball joints, over 256x256 fBm terrain; dt 1/60,
+    velocity iterations 10 (the sweep limit is `--max-sweeps` in residual mode), position
+    iterations 2.  Stepped untimed to tick 120 (settled towers, chains at rest) with the fixed
+    iterations, then from one snapshot:
+      learned + batched : all worlds in one launch sequence per tick, every island iterated to
+                          residual 1e-4, the learned warm start (trained first, on the device, on
+                          the converged impulses the solver logs for 4 other worlds of the same
+                          generator, islands that reached the tolerance only);
+      batched           : the same with the previous-frame (the reference's) and zero warm starts;
+      plain             : the same worlds one world at a time (one launch sequence per world per
+                          tick), previous-frame warm start, to the same residual.
+    value = learned + batched body-steps/s.  The plain leg's per-world hashes must equal the
+    batched previous-frame leg's."""
+    tol, max_it = 1.0e-4, a.max_sweeps
+    t0 = time.perf_counter()
+    worlds = [scenes.towers_ragdolls(seed=scenes.seed_for(5, w)) for w in range(a.worlds)]
+    b = rphys.Batch([rphys.World(s_) for s_ in worlds])
+    nb = b.total_bodies()
+    dt = float(worlds[0].get("dt", 1.0 / 60.0))
+    iv, ip = worlds[0].get("iters", (10, 2))
+    nj = sum(len(s_["joints"]) for s_ in worlds)
+    settle = max(a.warmup, 120)
+    print("bench: atscale, %d worlds, %d bodies, %d ball joints, built in %.1f s; stepping to tick %d" % (
+        a.worlds, nb, nj, time.perf_counter() - t0, settle), file=sys.stderr, flush=True)
+    for k in range(0, settle, 30):
+        b.step_n(min(30, settle - k), dt)
+        b.hashes()
+        print("bench: settle %d/%d ticks, %.1f s" % (min(k + 30, settle), settle, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    b.snapshot(1)
+
+    # the warm-start model: converged impulses of 4 other worlds of the generator
+    t1 = time.perf_counter()
+    tw = [scenes.towers_ragdolls(seed=scenes.seed_for(5, 100 + w)) for w in range(4)]
+    tb = rphys.Batch([rphys.World(s_) for s_ in tw])
+    for k in range(0, settle, 30):
+        tb.step_n(min(30, settle - k), dt)
+        tb.hashes()
+    tb.set_residual_mode(True, tol, max_it)
+    tb.set_contact_log(a.train_rows + 400000)
+    rows, logged = 0, 0
+    while rows < a.train_rows and logged < 400:
+        tb.step_n(5, dt)
+        logged += 5
+        rows = tb.contact_log_rows()[0]
+        if logged % 25 == 0:
+            print("bench: training data %d rows after %d ticks, %.1f s" % (rows, logged, time.perf_counter() - t1), file=sys.stderr, flush=True)
+    t2 = time.perf_counter()
+    params, losses = tb.train_warm_start(epochs=a.epochs, lr=1.0e-3, seed=20260101)
+    training = {"worlds": "4 worlds of the generator (seeds of worlds 100..103), residual 1e-4, from tick %d, %d ticks logged" % (settle, logged),
+                "rows": rows, "rows_from": "islands that reached the tolerance", "epochs": a.epochs, "lr": 1e-3,
+                "loss_first_last": [float(losses[0]), float(losses[-1])], "data_s": t2 - t1, "train_s": time.perf_counter() - t2}
+    print("bench: trained on %d rows, %d epochs in %.1f s, loss %.4f -> %.4f" % (rows, a.epochs, training["train_s"], losses[0], losses[-1]),
+          file=sys.stderr, flush=True)
+    del tb
+
+    # learned + batched: timed without events (the value), then the same ticks profiled
+    b.set_residual_mode(True, tol, max_it)
+    b.restore(1)
+    b.set_warm_start(2, params)
+    done, el = 0, 0.0
+    while done < a.steps:
+        k = min(10, a.steps - done)
+        t1 = time.perf_counter()
+        b.step_n(k, dt)  # returns once the ticks have run (the library reads their counters back)
+        el += time.perf_counter() - t1
+        done += k
+        print("bench: learned + batched %d/%d ticks, %.2f s" % (done, a.steps, el), file=sys.stderr, flush=True)
+    value = nb * a.steps / el
+    modes, stages_l, ctr_l = {}, None, None
+    for name, mode in (("learned", 2), ("previous_frame", 0), ("zero", 1)):
+        b.restore(1)
+        b.set_warm_start(mode, params if mode == 2 else None)
+        modes[name], st_, c_ = residual_pass(lambda k: b.step_profiled(k, dt), nb, a.steps)
+        if mode == 2:
+            stages_l, ctr_l = st_, c_
+        if mode == 0:
+            batched_hashes = b.hashes()
+        print("bench: batched, %s warm start: %.0f body-steps/s, %.2f sweeps per island entry, %.3f of entries converged" % (
+            name, modes[name]["body_steps_per_s"], modes[name]["mean_sweeps_per_entry"], modes[name]["converged_entry_frac"]),
+            file=sys.stderr, flush=True)
+    ctr_l["joint_entries"] = nj * ctr_l["ticks"]
+
+    # plain: the same worlds one at a time from the same tick, previous-frame warm start
+    b.restore(1)
+    b.set_residual_mode(False)
+    b.set_warm_start(0)
+    pbodies, pel, same = 0, 0.0, 0
+    acc = {}
+    for w in range(a.worlds):
+        pw = b.world(w)
+        pw.set_residual_mode(True, tol, max_it)
+        pw.set_warm_start(0)
+        pw.hash()  # device scene built, off the clock
+        t1 = time.perf_counter()
+        pw.step_n(a.steps, dt)
+        pel += time.perf_counter() - t1
+        h = pw.hash()
+        pbodies += pw.num_bodies()
+        same += int(h == batched_hashes[w])
+        pw.close()
+        if (w + 1) % 8 == 0 or w + 1 == a.worlds:
+            print("bench: plain %d/%d worlds, %.2f s" % (w + 1, a.worlds, pel), file=sys.stderr, flush=True)
+    plain = {"body_steps_per_s": pbodies * a.steps / pel, "ms_per_tick_all_worlds": 1000.0 * pel / a.steps,
+             "hashes_equal_batched_previous_frame": "%d/%d" % (same, a.worlds)}
+    if a.stages:
+        print_stages(stages_l, ctr_l)
+    base = None
+    if not a.no_cpu:
+        base = atscale_cpu_baseline(b, worlds[0], a.cpu_ticks, dt)
+    wl = {"workload": "atscale%d_towers_chains_x16k" % a.worlds, "worlds": a.worlds, "bodies": nb, "ball_joints": nj,
+          "world": "669 towers x 20 boxes (he 0.25), 4 loose boxes, 200 chains x 15 capsules (r 0.1, hh 0.15) with 14 ball joints, 256x256 fBm",
+          "dt": "1/60", "vel_iters": iv, "pos_iters": ip, "residual": tol, "max_sweeps": max_it, "ticks": "%d..%d" % (settle, settle + a.steps),
+          "launches": "stream", "learned_batched": modes["learned"], "batched_previous_frame": modes["previous_frame"],
+          "batched_zero": modes["zero"], "plain": plain, "learned_batched_over_plain": value / plain["body_steps_per_s"],
+          "training": training}
+    return {"metric": "body-steps/s on 1 GPU, learned+batched vs plain; PGS iters to residual 1e-4", "value": value, "unit": "body-steps/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": settle, "ms_per_step": 1000.0 * el / a.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 (warm-start model: bf16 weights and activations, f32 accumulation)",
+            "data": "synthetic (seeded XorShift64 scenes; the model is trained in the run)", "config": wl,
+            "roofline": roofline_of(stages_l, ctr_l, ctr_l["pgs_iters_sum"] / max(ctr_l["island_contacts"], 1.0), ip, wl["workload"]),
+            "cpu_baseline": base}
+
+
+def atscale_cpu_baseline(b, scene, ticks, dt):
+    """The oracle on world 0, continued from the batch's state at the snapshot (bodies, sleep,
+    warm cache), for `ticks` ticks with the fixed iterations (the reference's step)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+
+    b.restore(1)
+    st = b.world(0).export_state()
+    o = oracle_py.OracleWorld(scene)
+    o.load_state(st)
+    t1 = time.perf_counter()
+    for _ in range(ticks):
+        o.step(dt)
+    spent = time.perf_counter() - t1
+    return {"value": len(scene["shape"]) * ticks / spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": "world 0 (16,384 bodies, 2,800 ball joints), %d ticks from the timed window's first tick, fixed iterations "
+            "(%.1f s; %s, %d host cores, 1 used)" % (ticks, spent, cpu_model(), os.cpu_count() or 0)}
+
+
 def learned_bench(a):
     """Learned + batched vs plain, PGS to residual 1e-4 (per island, at most `--max-sweeps`).
 
@@ -494,7 +659,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["mixed16k", "boxes512", "batch", "learned", "towers"], default="mixed16k")
+    ap.add_argument("--workload", choices=["atscale", "mixed16k", "boxes512", "batch", "learned", "towers"], default="atscale")
     ap.add_argument("--train-rows", type=int, default=10_000_000, help="learned workload: contacts logged for training")
     ap.add_argument("--epochs", type=int, default=20, help="learned workload: training epochs")
     ap.add_argument("--max-sweeps", type=int, default=100, help="learned workload: PGS sweep limit per island in residual mode")
@@ -519,6 +684,17 @@ def main():
               "world across GPUs with no per-tick exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "atscale":
+        if a.steps == 600:
+            a.steps = 20
+        if a.worlds == 1024:
+            a.worlds = 64
+        if a.train_rows == 10_000_000:
+            a.train_rows = 2_000_000
+        if a.cpu_ticks == 3:
+            a.cpu_ticks = 60
+        print(json.dumps(atscale_bench(a)), flush=True)
+        return
     if a.workload == "learned":
         if a.steps == 600:
             a.steps = 40

@@ -229,6 +229,9 @@ typedef struct RPhysProfile {
     /* Leading levels of at least 1,024 rows of the largest island, list order and shock order,
      * summed per tick: the levels that run as GPU-wide launches (fixed iteration counts) */
     double wide_levels, wide_levels_shock;
+    /* Residual mode: sweeps x entries of the islands that reached the tolerance, summed per tick
+     * (over island_contacts_converged: the mean sweeps of the islands that converge) */
+    double pgs_iters_sum_converged;
 } RPhysProfile;
 RPhysResult rphys_world_step_profiled(RPhysWorld *world, float dt, uint32_t n_ticks, RPhysProfile *out);
 /* Bandwidth probe (build addition): copies `bytes` between two fresh device buffers `reps` times
@@ -269,6 +272,9 @@ typedef struct RPhysBatch RPhysBatch;
 RPhysBatch *rphys_batch_create(RPhysWorld *const *worlds, uint32_t n_worlds);
 void rphys_batch_destroy(RPhysBatch *batch);
 RPhysResult rphys_batch_step_n(RPhysBatch *batch, float dt, uint32_t n_ticks);
+/* Copies world wi of the batch (bodies, settings and carried state as the batch has stepped them,
+ * tick included) into dst, replacing dst's contents; dst then steps on its own from there. */
+RPhysResult rphys_batch_get_world(RPhysBatch *batch, uint32_t world_index, RPhysWorld *dst);
 /* out: 32 * n_worlds bytes */
 RPhysResult rphys_batch_get_hashes(const RPhysBatch *batch, uint8_t *out);
 RPhysResult rphys_batch_get_state(const RPhysBatch *batch, uint32_t world_index, float *pos3, float *rot4, float *lin3, float *ang3);

@@ -273,6 +273,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.cs_ai, C) && B.alloc(d.cs_bi, C) && B.alloc(d.cs_wa, C) && B.alloc(d.cs_wb, C) && B.alloc(d.cs_inv, C);
     ok &= B.alloc(d.cs_ra, C) && B.alloc(d.cs_rb, C) && B.alloc(d.cs_n, C) && B.alloc(d.cs_t1, C) && B.alloc(d.cs_t2, C) && B.alloc(d.cs_m, C) && B.alloc(d.cs_mu, C);
     ok &= B.alloc(d.jn, C) && B.alloc(d.jn_pre, C) && B.alloc(d.jt, C) && B.alloc(d.ct_key, C) && B.alloc(d.ct_anchor, C) && B.alloc(d.warm_idx, C);
+    ok &= B.alloc(d.rank_conv, C);
     ok &= B.alloc(d.isl_k, C) && B.alloc(d.isl_v, C) && B.alloc(d.isl_k2, C) && B.alloc(d.isl_v2, C) && B.alloc(d.isl_start, C) && B.alloc(d.isl_flag, C + 1);
     ok &= B.alloc(d.ch_pa, C) && B.alloc(d.ch_pb, C) && B.alloc(d.sh_ord, C) && B.alloc(d.sh_pa, C) && B.alloc(d.sh_pb, C) && B.alloc(d.g_done, C);
     ok &= B.alloc(d.ch_sa, C) && B.alloc(d.ch_sb, C) && B.alloc(d.sh_sa, C) && B.alloc(d.sh_sb, C) && B.alloc(d.ex_base, C) && B.alloc(d.ex_shock, C);
@@ -544,6 +545,7 @@ struct ProfileAcc {
         out->pgs_iters_max += cnt[CT_SUM_ITERS_MAX]; out->pgs_iters_sum += cnt[CT_SUM_ITERS_SUM] + 4294967296.0 * cnt[CT_SUM_ITERS_SUM_HI];
         out->islands_converged += cnt[CT_SUM_ISL_CONV]; out->island_contacts_converged += cnt[CT_SUM_ISLC_CONV];
         out->warm_start_learned += cnt[CT_SUM_WS_LEARNED];
+        out->pgs_iters_sum_converged += cnt[CT_SUM_ITERS_CONV] + 4294967296.0 * cnt[CT_SUM_ITERS_CONV_HI];
         out->wide_levels += cnt[CT_SUM_WIDE]; out->wide_levels_shock += cnt[CT_SUM_WIDE_SH];
         return true;
     }
@@ -1652,6 +1654,20 @@ RPhysResult rphys_batch_get_state(const RPhysBatch *cb, uint32_t wi, float *pos3
         if (lin3) { lin3[3 * i] = w.lin[i].x; lin3[3 * i + 1] = w.lin[i].y; lin3[3 * i + 2] = w.lin[i].z; }
         if (ang3) { ang3[3 * i] = w.ang[i].x; ang3[3 * i + 1] = w.ang[i].y; ang3[3 * i + 2] = w.ang[i].z; }
     }
+    return RPHYS_OK;
+    GUARD_END
+}
+
+RPhysResult rphys_batch_get_world(RPhysBatch *b, uint32_t wi, RPhysWorld *dst) {
+    GUARD_BEGIN
+    if (!b || !dst) { set_err("rphys_batch_get_world: null arg"); return RPHYS_ERR_NULL; }
+    if (wi >= b->worlds.size()) return bad_arg("rphys_batch_get_world", "world index out of range");
+    if (!pull_world(*b->scene, wi, b->worlds[wi])) return RPHYS_ERR_INTERNAL;
+    std::lock_guard<std::mutex> lk(dst->mu);
+    dst->w = b->worlds[wi];
+    dst->dev_newer = false;
+    dst->have_contacts = false;
+    host_changed(dst);
     return RPHYS_OK;
     GUARD_END
 }

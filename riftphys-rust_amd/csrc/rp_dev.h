@@ -46,6 +46,8 @@ enum {
     CT_SUM_ISL_CONV, CT_SUM_ISLC_CONV, CT_SUM_WS_LEARNED,
     CT_SUM_WIDE, CT_SUM_WIDE_SH,  // leading wide levels of the largest island (list, shock order)
     CT_SUM_ITERS_SUM_HI,  // high word of the CT_SUM_ITERS_SUM sum (sweeps x contacts outgrow 32 bits)
+    CT_ITERS_SUM_CONV,    // residual mode: sweeps x entries of the islands that reached the tolerance
+    CT_SUM_ITERS_CONV, CT_SUM_ITERS_CONV_HI,
     CT_COUNT
 };
 
@@ -109,6 +111,7 @@ struct Dev {
     int32_t *warm_idx;
     // islands
     uint32_t *isl_k, *isl_v, *isl_k2, *isl_v2, *isl_start, *isl_flag;
+    uint32_t *rank_conv;  // residual mode: 1 where the entry's island reached the tolerance (by island-list rank)
     // sweep orders over island-list positions: chains of the list order, the shock order (contact
     // at each shock rank) and its chains, readiness flags of the largest islands
     uint32_t *ch_pa, *ch_pb, *ch_sa, *ch_sb, *sh_ord, *sh_pa, *sh_pb, *sh_sa, *sh_sb, *g_done;

@@ -752,10 +752,13 @@ static __device__ __forceinline__ V3 warm_base(const Dev &d, uint32_t c, uint32_
 }
 // The contact entries of the island lists (joint entries are not logged), flagged for an
 // exclusive scan that gives each its log row; newflag is free until the warm-cache store.
-__global__ void k_log_flags(Dev d) {
+// The contacts the log takes: every solved contact, or in residual mode those of islands that
+// reached the tolerance (an island stopped at the sweep limit holds no converged impulses).
+__global__ void k_log_flags(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
-    GRID_STRIDE(r, d.cap_contacts) d.newflag[r] = (r < nislc && !(d.isl_v2[r] & RP_JBIT)) ? 1u : 0u;
+    GRID_STRIDE(r, d.cap_contacts)
+        d.newflag[r] = (r < nislc && !(d.isl_v2[r] & RP_JBIT) && (!tp.resid_on || d.rank_conv[r])) ? 1u : 0u;
 }
 // Appends the solved contacts of this tick (island-list order) with their converged impulses,
 // the normal impulse before the shock pass and the friction accumulators (solver.rs:917-926),
@@ -768,7 +771,7 @@ __global__ void k_log_contacts(Dev d, TickParams tp) {
     GRID_STRIDE(r, nislc) {
         if (!room) break;
         const uint32_t c = d.isl_v2[r];
-        if (c & RP_JBIT) continue;
+        if ((c & RP_JBIT) || (tp.resid_on && !d.rank_conv[r])) continue;
         const size_t o = (size_t)base + d.newflag[r];
         for (int k = 0; k < 12; k++) d.log_feat[o * 12 + k] = d.feat[(size_t)c * 12 + k];
         const float2 t = d.jt[c];
@@ -1261,7 +1264,11 @@ static __device__ void joint_warm(const Dev &d, uint32_t q) {
     }
 }
 // one PGS pass over joint q's rows (solver.rs:822-832); residual |d lambda| K
-static __device__ void joint_pgs(const Dev &d, uint32_t q, RP_LDS uint32_t *resid_bits) {
+// Where a sweep gathers its largest row residual (residual mode): an LDS word, and whether to.
+// The flag is explicit: an LDS variable can sit at address 0, which compares equal to a null
+// address_space(3) pointer (a kernel whose only LDS word is this one lost its residuals that way).
+struct ResidSink { RP_LDS uint32_t *p; bool on; };
+static __device__ void joint_pgs(const Dev &d, uint32_t q, ResidSink resid_bits) {
     const uint32_t m = d.jr_mask[q], a = d.jt_a[q], b = d.jt_b[q];
     const V3 ra = L3(d.jr_ra[q]), rb = L3(d.jr_rb[q]);
     for (uint32_t i = 0; i < 3; i++) {
@@ -1273,7 +1280,7 @@ static __device__ void joint_pgs(const Dev &d, uint32_t q, RP_LDS uint32_t *resi
         const float nw = old + (0.0f - v) * u4.w;
         d.jr_acc[3 * q + i] = nw;
         if (nw != old) joint_apply(d, a, b, ra, rb, u, nw - old);
-        if (resid_bits) __hip_atomic_fetch_max(resid_bits, __float_as_uint(fabsf(nw - old) / u4.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (resid_bits.on) __hip_atomic_fetch_max(resid_bits.p, __float_as_uint(fabsf(nw - old) / u4.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 // joint q's rows become its warm rows (a joint without rows keeps its old ones)
@@ -1416,7 +1423,7 @@ static __device__ __forceinline__ PgsIn pgs_fetch(const Dev &d, uint32_t c, uint
     in.jt = d.jt[c];
     return in;
 }
-static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, RP_LDS uint32_t *resid_bits) {
+static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, ResidSink resid_bits) {
     if (c & RP_JBIT) { joint_pgs(d, c & ~RP_JBIT, resid_bits); return; }
     const Row &x = in.x;
     Vel2 v;
@@ -1437,11 +1444,11 @@ static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const P
     d.jt[c] = make_float2(nt.x, nt.y);
     V3 dd = nt - ot;
     apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
-    if (resid_bits) {  // the oracle's row residual, same operations
+    if (resid_bits.on) {  // the oracle's row residual, same operations
         float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
         float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
         float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
-        __hip_atomic_fetch_max(resid_bits, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_max(resid_bits.p, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     // store what changed (a body the row does not write keeps its bits)
     if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
@@ -1554,7 +1561,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         __syncthreads();
         // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
         __shared__ uint32_t resid_bits;
-        RP_LDS uint32_t *rb_lds = tp.resid_on ? (RP_LDS uint32_t *)&resid_bits : nullptr;
+        const ResidSink rb_lds{(RP_LDS uint32_t *)&resid_bits, tp.resid_on != 0};
         uint32_t used = 0;
         bool conv = false;
         auto pgs_prep = [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); };
@@ -1577,8 +1584,10 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         }
         if (threadIdx.x == 0) {
             atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n);
-            if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); }
+            if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); atomicAdd(&d.cnt[CT_ITERS_SUM_CONV], used * n); }
         }
+        if (tp.resid_on)  // which entries' impulses are converged ones (the contact log keeps only those)
+            for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.rank_conv[s + r] = conv ? 1u : 0u;
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PGS);
         vel_shock<TIER>(d, tp, I, lst, isl, pc, 0u);
     ISLAND_LOOP_END
@@ -1617,7 +1626,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel_tail(Dev d, T
     ISLAND_LOOP_BEGIN
         const uint32_t k = min(d.isl_w[isl], kl);
         run_sweep_from(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, k ? d.lvl_off[s + k] : 0u,
-                       [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); }, [&](uint32_t c, const PgsIn &in) { pgs_row(d, c, in, nullptr); });
+                       [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); }, [&](uint32_t c, const PgsIn &in) { pgs_row(d, c, in, ResidSink{nullptr, false}); });
     ISLAND_LOOP_END
 }
 // ... and, after the sweeps, the pre-shock snapshot and the shock pass.
@@ -1651,7 +1660,7 @@ __global__ void __launch_bounds__(256) k_level_vel(Dev d, uint32_t L) {
     if (aborted(d)) return;
     for_wide_level(d, d.wide_list, d.cnt[CT_WIDE_N], d.isl_w, d.lvl_off, L, [&](uint32_t slot) {
         const uint32_t c = d.bs_c[slot];
-        pgs_row(d, c, pgs_fetch(d, c, slot), nullptr);
+        pgs_row(d, c, pgs_fetch(d, c, slot), ResidSink{nullptr, false});
     });
 }
 // The constant rows of every big island's slots (split form), across the GPU.
@@ -2149,6 +2158,8 @@ __global__ void k_tick_done(Dev d) {
     c[CT_SUM_ITERS_MAX] += c[CT_ITERS_MAX];
     const uint64_t isum = ((uint64_t)c[CT_SUM_ITERS_SUM_HI] << 32 | c[CT_SUM_ITERS_SUM]) + c[CT_ITERS_SUM];
     c[CT_SUM_ITERS_SUM] = (uint32_t)isum; c[CT_SUM_ITERS_SUM_HI] = (uint32_t)(isum >> 32);
+    const uint64_t csum = ((uint64_t)c[CT_SUM_ITERS_CONV_HI] << 32 | c[CT_SUM_ITERS_CONV]) + c[CT_ITERS_SUM_CONV];
+    c[CT_SUM_ITERS_CONV] = (uint32_t)csum; c[CT_SUM_ITERS_CONV_HI] = (uint32_t)(csum >> 32);
     c[CT_SUM_WIDE] += c[CT_WIDE_MAX]; c[CT_SUM_WIDE_SH] += c[CT_WSH_MAX];
     c[CT_SUM_ISL_CONV] += c[CT_ISL_CONV]; c[CT_SUM_ISLC_CONV] += c[CT_ISLC_CONV]; c[CT_SUM_WS_LEARNED] += c[CT_WS_LEARNED];
 }
@@ -2307,6 +2318,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_SUM, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISL_CONV, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_CONV, 0u);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ITERS_SUM_CONV, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WS_LEARNED, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_N, 0u);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_WIDE_MAX, 0u);
@@ -2409,7 +2421,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
     }
     if (tp.log_on) {
-        hipLaunchKernelGGL(k_log_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+        hipLaunchKernelGGL(k_log_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         scan_u32(d.newflag, d.cap_contacts, d.scan_tmp, d.cnt + CT_LOG_N, st);
         hipLaunchKernelGGL(k_log_contacts, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         hipLaunchKernelGGL(k_log_advance, dim3(1), dim3(1), 0, st, d);

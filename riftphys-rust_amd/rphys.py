@@ -96,7 +96,8 @@ class Profile(C.Structure):
                 ("terrain_candidates", C.c_double), ("contacts", C.c_double), ("island_contacts", C.c_double), ("islands", C.c_double),
                 ("island_contacts_global", C.c_double), ("pgs_iters_max", C.c_double), ("pgs_iters_sum", C.c_double),
                 ("sweep_phase", C.c_double * 11), ("islands_converged", C.c_double), ("island_contacts_converged", C.c_double),
-                ("warm_start_learned", C.c_double), ("wide_levels", C.c_double), ("wide_levels_shock", C.c_double)]
+                ("warm_start_learned", C.c_double), ("wide_levels", C.c_double), ("wide_levels_shock", C.c_double),
+                ("pgs_iters_sum_converged", C.c_double)]
 
 
 class MlpTrainConfig(C.Structure):
@@ -170,6 +171,7 @@ EXPORTS = [
     ("rphys_batch_get_hashes", U32, [P, C.POINTER(C.c_uint8)]),
     ("rphys_batch_get_state", U32, [P, U32, PF32, PF32, PF32, PF32]),
     ("rphys_batch_snapshot", U32, [P, U32]),
+    ("rphys_batch_get_world", U32, [P, U32, P]),
     ("rphys_batch_restore", U32, [P, U32]),
     ("rphys_batch_set_tuning", U32, [P, C.POINTER(Tuning)]),
     ("rphys_batch_get_tuning", U32, [P, C.POINTER(Tuning)]),
@@ -238,7 +240,8 @@ def _profile_dicts(pr):
                     island_contacts=pr.island_contacts, islands=pr.islands, island_contacts_global=pr.island_contacts_global,
                     pgs_iters_max=pr.pgs_iters_max, pgs_iters_sum=pr.pgs_iters_sum, islands_converged=pr.islands_converged,
                     island_contacts_converged=pr.island_contacts_converged, warm_start_learned=pr.warm_start_learned,
-                    wide_levels=pr.wide_levels, wide_levels_shock=pr.wide_levels_shock)
+                    wide_levels=pr.wide_levels, wide_levels_shock=pr.wide_levels_shock,
+                    pgs_iters_sum_converged=pr.pgs_iters_sum_converged)
     names = ("vel_order", "warm", "pgs", "vel_shock_order", "vel_shock", "ballistic", "projection", "proj_shock_order",
              "proj_shock", "levels_list", "levels_shock")
     counters["sweep_phase"] = {k: pr.sweep_phase[i] for i, k in enumerate(names)}
@@ -524,6 +527,12 @@ class Batch:
         loss = np.zeros(epochs, np.float32)
         check(lib().rphys_batch_train_warm_start(self.h, C.byref(cfg), params.ctypes.data_as(PF32), loss.ctypes.data_as(PF32)))
         return params, loss
+
+    def world(self, wi):
+        """A new World holding world wi of the batch as the batch has stepped it."""
+        w = World()
+        check(lib().rphys_batch_get_world(self.h, wi, w.h))
+        return w
 
     def snapshot(self, slot=0):
         check(lib().rphys_batch_snapshot(self.h, slot))

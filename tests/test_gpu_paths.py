@@ -38,9 +38,15 @@ def run_pair(scene, ticks, check_every=1):
 TIERS = {"all_hbm_flags": (1, 1, 1, 0), "spread": (8, 40, 200, 0), "split_lds": (1, 1, 0, 8), "split_hbm": (1, 1, 1, 8)}
 
 
-@pytest.mark.parametrize("tiers", list(TIERS))
+def dense_pile():
+    """1,200 mixed bodies dropped into a 12 m square: piles of a few thousand contacts whose list
+    orders open with wide levels (the 512-box lattice's chains leave its first level one rank wide)."""
+    return scenes.mixed_shapes(n=1200, hf_n=48, extent=12.0, height=10.0, margin=4.0)
+
+
+@pytest.mark.parametrize("tiers", ["all_hbm_flags", "spread"])
 @pytest.mark.parametrize("scene", ["mixed", "lattice", "joints"])
-def test_sweep_tiers_and_split_form_bit_exact(hooks, tiers, scene):
+def test_sweep_tiers_bit_exact(hooks, tiers, scene):
     hooks.rphys_debug_set_sweep_tiers(*TIERS[tiers])
     if scene == "mixed":
         s, ticks = scenes.small_mixed(n=96, seed=7), 40
@@ -48,17 +54,33 @@ def test_sweep_tiers_and_split_form_bit_exact(hooks, tiers, scene):
         s, ticks = scenes.box_lattice(), 30
     else:
         s, ticks = scenes.ragdolls_small(), 40
-    g = run_pair(s, ticks)
-    if tiers.startswith("split") and scene == "lattice":
-        _, counters = g.step_profiled(1)
-        assert counters["wide_levels"] > 0, "the split form did not run"
+    run_pair(s, ticks)
 
 
-def test_residual_mode_in_hbm_flag_tier(hooks):
-    hooks.rphys_debug_set_sweep_tiers(1, 1, 1, 0)
+@pytest.mark.parametrize("tiers", ["split_lds", "split_hbm"])
+def test_split_form_bit_exact(hooks, tiers):
+    """Big islands' leading wide levels as GPU-wide launches (fixed iterations), checked every
+    tenth tick over the fall and the settling of the pile."""
+    hooks.rphys_debug_set_sweep_tiers(*TIERS[tiers])
+    g = run_pair(dense_pile(), 100, check_every=10)
+    _, counters = g.step_profiled(1)
+    assert counters["wide_levels"] > 0 and counters["wide_levels_shock"] > 0, "the split form did not run"
+
+
+@pytest.mark.parametrize("tiers", ["all_hbm_flags", "split_lds"])
+def test_residual_mode_in_the_big_tiers(hooks, tiers):
+    """Residual mode in tiers 2 and 1 (one-kernel sweeps): the same per-island stop and sweep
+    counts as the oracle.  Tier 2's only LDS word is the sweep residual, at LDS address 0."""
+    hooks.rphys_debug_set_sweep_tiers(*TIERS[tiers])
     s = scenes.box_lattice()
     s["residual"] = (1.0e-4, 40)
-    run_pair(s, 20)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(20):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+        so, sg = o.stats(), g.stats()
+        assert (so.iters_used, so.iters_sum) == (sg.pgs_iters, sg.pgs_iters_sum), (t + 1, so.iters_used, sg.pgs_iters)
 
 
 def test_overflow_replay_with_joints_bit_exact(hooks):

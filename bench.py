@@ -478,7 +478,11 @@ def atscale_bench(a):
     for name, mode in (("learned", 2), ("previous_frame", 0), ("zero", 1)):
         b.restore(1)
         b.set_warm_start(mode, params if mode == 2 else None)
+        if mode == 2 and a.mark_window:  # a copy-kernel dispatch on each side marks the roofline's window in a kernel trace
+            rphys.copy_probe(1 << 20, 1)
         modes[name], st_, c_ = residual_pass(lambda k: b.step_profiled(k, dt), nb, a.steps)
+        if mode == 2 and a.mark_window:
+            rphys.copy_probe(1 << 20, 1)
         if mode == 2:
             stages_l, ctr_l = st_, c_
         if mode == 0:
@@ -537,15 +541,26 @@ def atscale_cpu_baseline(b, scene, ticks, dt):
 
     b.restore(1)
     st = b.world(0).export_state()
-    o = oracle_py.OracleWorld(scene)
-    o.load_state(st)
-    t1 = time.perf_counter()
-    for _ in range(ticks):
-        o.step(dt)
-    spent = time.perf_counter() - t1
-    return {"value": len(scene["shape"]) * ticks / spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": "world 0 (16,384 bodies, 2,800 ball joints), %d ticks from the timed window's first tick, fixed iterations "
-            "(%.1f s; %s, %d host cores, 1 used)" % (ticks, spent, cpu_model(), os.cpu_count() or 0)}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    spent, hashes = {}, {}
+    for nt in (1, threads):
+        oracle_py.lib().or_set_threads(nt)
+        o = oracle_py.OracleWorld(scene)
+        o.load_state(st)
+        t1 = time.perf_counter()
+        for _ in range(ticks):
+            o.step(dt)
+        spent[nt] = time.perf_counter() - t1
+        hashes[nt] = o.hash()
+        del o
+        print("bench: oracle, %d thread(s): %.1f s for %d ticks" % (nt, spent[nt], ticks), file=sys.stderr, flush=True)
+    oracle_py.lib().or_set_threads(1)
+    n = len(scene["shape"])
+    return {"value": n * ticks / spent[threads], "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "single_thread_value": n * ticks / spent[1], "threads_bit_identical": hashes[1] == hashes[threads],
+            "sample": "world 0 (16,384 bodies, 2,800 ball joints), %d ticks from the timed window's first tick, the reference's fixed "
+            "iterations; OpenMP where the reference uses rayon (narrowphase pairs, terrain colliders, PGS level batches >= 64), %d threads "
+            "%.1f s, 1 thread %.1f s (%s, %d host cores)" % (ticks, threads, spent[threads], spent[1], cpu_model(), os.cpu_count() or 0)}
 
 
 def learned_bench(a):
@@ -669,6 +684,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="step on the stream instead of graph-captured ticks (profiling runs: rocprofv3 kernel tracing of graph launches crashed the profiled process)")
     ap.add_argument("--stages", action="store_true", help="also print the per-stage split to stderr")
+    ap.add_argument("--mark-window", action="store_true",
+                    help="atscale: launch the copy kernel just before and after the profiled learned + batched pass (the roofline's "
+                         "window), so riftphys-rust_amd/kernel_window.py can cut that window out of a rocprofv3 kernel trace")
     ap.add_argument("--learned", action="store_true", help="towers workload: also train and compare the learned warm start")
     ap.add_argument("--graph", action="store_true",
                     help="towers workload: graph-captured, autotuned ticks (default: stream launches, measured faster on this workload)")

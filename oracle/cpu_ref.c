@@ -53,6 +53,13 @@
 #define TERRAIN_CI 0xFFFFFFFFu
 static const float MARGIN = 0.05f;     /* step.rs:12 MELEE_CCD_MARGIN */
 static const float POS_SLOP = 0.005f;  /* step.rs:16 */
+/* step.rs:14 / solver.rs:18 PAR_MIN: below this many items the reference runs a batch on the
+ * calling thread; at or above it rayon splits it.  The oracle's OpenMP regions (narrowphase pairs,
+ * terrain colliders, the PGS level batches) use the same threshold; each writes only its own
+ * items, so the results are the single-threaded ones bit for bit (or_set_threads). */
+#define PAR_MIN 64u
+static int g_threads = 1;
+void or_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 
 /* ------------------------------------------------------------------ quantisers */
 /* det.rs:8-13 q6: (x / 1e-6).round() * 1e-6, non-finite -> 0 */
@@ -916,64 +923,125 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         else if (dact[q]) isl_n[disl[q]]++;
     }
 
-    /* PGS iterations (solver.rs:788-880), serial in contact order == the level schedule */
+    /* PGS iterations (solver.rs:788-880) as the reference's level batches (solver.rs:621-687):
+     * joints by joint level, then contacts by contact level, each batch in index order.  Rows of
+     * one batch write pairwise distinct bodies, so the batched order gives the serial order's bits,
+     * and a batch of at least PAR_MIN rows runs in parallel as run_batch does (solver.rs:21-23). */
+    uint32_t *jlev = calloc(nb + 1, 4), *clev = calloc(nb + 1, 4);
+    uint32_t *jord = malloc(4 * (w->nj + 1)), *jlo = calloc(w->nj + 2, 4), njl = 0;
+    uint32_t *cord = malloc(4 * (nct + 1)), *clo = calloc(nct + 2, 4), ncl = 0;
+    {
+        uint32_t *lv = malloc(4 * (w->nj + 1));
+        for (uint32_t q = 0; q < w->nj; q++) {
+            lv[q] = 0xFFFFFFFFu;
+            if (!dact[q] && jr1[q] == jr0[q]) continue;
+            uint32_t a = w->joints[q].a, b = w->joints[q].b;
+            uint32_t t = jlev[a] > jlev[b] ? jlev[a] : jlev[b];
+            lv[q] = t; jlev[a] = jlev[b] = t + 1;
+            if (t + 1 > njl) njl = t + 1;
+        }
+        for (uint32_t q = 0; q < w->nj; q++) if (lv[q] != 0xFFFFFFFFu) jlo[lv[q] + 1]++;
+        for (uint32_t l = 0; l < njl; l++) jlo[l + 1] += jlo[l];
+        uint32_t *fill = calloc(njl + 1, 4);
+        for (uint32_t q = 0; q < w->nj; q++) if (lv[q] != 0xFFFFFFFFu) jord[jlo[lv[q]] + fill[lv[q]]++] = q;
+        free(fill); free(lv);
+    }
+    {
+        uint32_t *lv = malloc(4 * (nct + 1));
+        for (uint32_t i = 0; i < nct; i++) {
+            int wa = cs[i].inv_a > 0.0f, wb = !cs[i].terrain && cs[i].inv_b > 0.0f;
+            lv[i] = 0xFFFFFFFFu;
+            if (isl[i] == 0xFFFFFFFFu) continue;
+            uint32_t la = wa ? clev[cs[i].ai] : 0, lb = wb ? clev[cs[i].bi] : 0;
+            uint32_t t = la > lb ? la : lb;
+            lv[i] = t;
+            if (wa) clev[cs[i].ai] = t + 1;
+            if (wb) clev[cs[i].bi] = t + 1;
+            if (t + 1 > ncl) ncl = t + 1;
+        }
+        for (uint32_t i = 0; i < nct; i++) if (lv[i] != 0xFFFFFFFFu) clo[lv[i] + 1]++;
+        for (uint32_t l = 0; l < ncl; l++) clo[l + 1] += clo[l];
+        uint32_t *fill = calloc(ncl + 1, 4);
+        for (uint32_t i = 0; i < nct; i++) if (lv[i] != 0xFFFFFFFFu) cord[clo[lv[i]] + fill[lv[i]]++] = i;
+        free(fill); free(lv);
+    }
+    float *jres = calloc(w->nj + 1, 4), *cres = calloc(nct + 1, 4);
+    uint8_t *jran = calloc(w->nj + 1, 1), *cran = calloc(nct + 1, 1);
     uint32_t iters = w->resid_on ? w->resid_max : w->iv;
     uint32_t used = 0;
     float resid = 0;
     for (uint32_t it = 0; it < iters; it++) {
         int active = 0;
         for (uint32_t b = 0; b <= nb; b++) isl_r[b] = 0.0f;
-        /* A. joints, serial in joint order (the joint level batches, solver.rs:791-835) */
-        for (uint32_t q = 0; q < w->nj; q++) {
-            if (dact[q]) {
-                if (isl_done[disl[q]]) continue;
-                active = 1;
-                float rr = dist_vel_row(w, &w->joints[q], dt);
-                if (w->resid_on && rr > isl_r[disl[q]]) isl_r[disl[q]] = rr;
-                continue;
-            }
-            for (uint32_t k = jr0[q]; k < jr1[q]; k++) {
-                const JRow *r = &jr[k];
-                if (isl_done[jisl[k]]) continue;
-                active = 1;
-                float v = jrow_vel(w, r);
-                float old = racc[k];
-                float nwv = old + (0.0f - v) * r->m;
-                racc[k] = nwv;
-                if (nwv != old) jrow_apply(w, r, nwv - old);
-                if (w->resid_on) {
+        /* A. joints (solver.rs:789-835) */
+        for (uint32_t l = 0; l < njl; l++) {
+            uint32_t lo = jlo[l], hi = jlo[l + 1];
+#pragma omp parallel for schedule(static) if (hi - lo >= PAR_MIN) num_threads(g_threads)
+            for (uint32_t o = lo; o < hi; o++) {
+                uint32_t q = jord[o];
+                jran[q] = 0; jres[q] = 0.0f;
+                if (dact[q]) {
+                    if (isl_done[disl[q]]) continue;
+                    jran[q] = 1;
+                    jres[q] = dist_vel_row(w, &w->joints[q], dt);
+                    continue;
+                }
+                for (uint32_t k = jr0[q]; k < jr1[q]; k++) {
+                    const JRow *r = &jr[k];
+                    if (isl_done[jisl[k]]) continue;
+                    jran[q] = 1;
+                    float v = jrow_vel(w, r);
+                    float old = racc[k];
+                    float nwv = old + (0.0f - v) * r->m;
+                    racc[k] = nwv;
+                    if (nwv != old) jrow_apply(w, r, nwv - old);
                     float rr = fabsf(nwv - old) / r->m;
-                    if (rr > isl_r[jisl[k]]) isl_r[jisl[k]] = rr;
+                    if (rr > jres[q]) jres[q] = rr;
                 }
             }
         }
-        /* B. contacts */
-        for (uint32_t i = 0; i < nct; i++) {
-            const CC *k = &cs[i];
-            if (isl[i] == 0xFFFFFFFFu || isl_done[isl[i]]) continue;
+        for (uint32_t q = 0; q < w->nj; q++) {
+            if (!jran[q]) continue;
             active = 1;
-            float vrn = vdot(rel_vel(w, k), k->n);
-            float jraw = (k->depth_bias - vrn) * k->m_n;
-            float old = jn[i];
-            float nw = fmaxf(old + jraw, 0.0f);
-            jn[i] = nw;
-            apply_cimp(w, k, vmul(k->n, nw - old));
-            V3 vf = rel_vel(w, k);
-            float vt1 = vdot(vf, k->t1), vt2 = vdot(vf, k->t2);
-            float slip = sqrtf(vt1 * vt1 + vt2 * vt2);
-            float mu = mu_dyn(k, slip);
-            V3 ot = v3(jt[2 * i], jt[2 * i + 1], 0.0f);
-            V3 nt = vclamp_len_max(vsub(ot, v3(vt1 * k->m_t1, vt2 * k->m_t2, 0.0f)), mu * nw);
-            jt[2 * i] = nt.x; jt[2 * i + 1] = nt.y;
-            V3 d = vsub(nt, ot);
-            apply_cimp(w, k, vadd(vmul(k->t1, d.x), vmul(k->t2, d.y)));
-            if (w->resid_on) {
+            uint32_t b = dact[q] ? disl[q] : jisl[jr0[q]];
+            if (w->resid_on && jres[q] > isl_r[b]) isl_r[b] = jres[q];
+        }
+        /* B. contacts (solver.rs:838-880) */
+        for (uint32_t l = 0; l < ncl; l++) {
+            uint32_t lo = clo[l], hi = clo[l + 1];
+#pragma omp parallel for schedule(static) if (hi - lo >= PAR_MIN) num_threads(g_threads)
+            for (uint32_t o = lo; o < hi; o++) {
+                uint32_t i = cord[o];
+                const CC *k = &cs[i];
+                cran[i] = 0;
+                if (isl_done[isl[i]]) continue;
+                cran[i] = 1;
+                float vrn = vdot(rel_vel(w, k), k->n);
+                float jraw = (k->depth_bias - vrn) * k->m_n;
+                float old = jn[i];
+                float nw = fmaxf(old + jraw, 0.0f);
+                jn[i] = nw;
+                apply_cimp(w, k, vmul(k->n, nw - old));
+                V3 vf = rel_vel(w, k);
+                float vt1 = vdot(vf, k->t1), vt2 = vdot(vf, k->t2);
+                float slip = sqrtf(vt1 * vt1 + vt2 * vt2);
+                float mu = mu_dyn(k, slip);
+                V3 ot = v3(jt[2 * i], jt[2 * i + 1], 0.0f);
+                V3 nt = vclamp_len_max(vsub(ot, v3(vt1 * k->m_t1, vt2 * k->m_t2, 0.0f)), mu * nw);
+                jt[2 * i] = nt.x; jt[2 * i + 1] = nt.y;
+                V3 d = vsub(nt, ot);
+                apply_cimp(w, k, vadd(vmul(k->t1, d.x), vmul(k->t2, d.y)));
                 float r0 = k->m_n > 0.0f ? fabsf(nw - old) / k->m_n / (1.0f + fabsf(k->depth_bias)) : 0.0f;
                 float r1 = k->m_t1 > 0.0f ? fabsf(d.x) / k->m_t1 : 0.0f;
                 float r2 = k->m_t2 > 0.0f ? fabsf(d.y) / k->m_t2 : 0.0f;
-                float r = fmaxf(r0, fmaxf(r1, r2));
-                if (r > isl_r[isl[i]]) isl_r[isl[i]] = r;
+                cres[i] = fmaxf(r0, fmaxf(r1, r2));
             }
+        }
+        for (uint32_t o = 0; o < clo[ncl]; o++) {
+            uint32_t i = cord[o];
+            if (!cran[i]) continue;
+            active = 1;
+            if (w->resid_on && cres[i] > isl_r[isl[i]]) isl_r[isl[i]] = cres[i];
         }
         if (!active) break;
         used++;
@@ -994,6 +1062,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     w->stats.residual = resid;
     free(par); free(isl); free(isl_n); free(isl_it); free(isl_done); free(isl_r); free(isl_rows); free(jisl); free(jw);
     free(dact); free(disl);
+    free(jlev); free(clev); free(jord); free(jlo); free(cord); free(clo); free(jres); free(cres); free(jran); free(cran);
     /* joints solved this step replace their warm rows (solver.rs:945-952) */
     for (uint32_t q = 0; q < w->nj; q++) {
         uint32_t s0 = jr0[q], e0 = jr1[q];
@@ -1297,7 +1366,14 @@ void or_step(OrWorld *w, float dt) {
     }
     qsort(jkeys, w->nj, 8, cmp_u64);
     CVec raw = {0};
-    for (uint32_t p = 0; p < w->npairs; p++) {
+    /* pairs in parallel above PAR_MIN (step.rs:113-117), each into its own manifold, then appended
+     * in pair order: the output is the serial one */
+    uint32_t np = w->npairs;
+    Manifold *pm = malloc(sizeof(Manifold) * (np + 1));
+    float *preach = malloc(sizeof(float) * (np + 1));
+#pragma omp parallel for schedule(dynamic, 64) if (np >= PAR_MIN) num_threads(g_threads)
+    for (uint32_t p = 0; p < np; p++) {
+        pm[p].len = 0;
         uint32_t i = w->pairs[2 * p], j = w->pairs[2 * p + 1];
         uint32_t ba = w->cbody[i], bb = w->cbody[j];
         if (ba == bb) continue;
@@ -1305,19 +1381,32 @@ void or_step(OrWorld *w, float dt) {
             uint64_t key = ((uint64_t)(ba < bb ? ba : bb) << 32) | (ba < bb ? bb : ba);
             if (bsearch(&key, jkeys, w->nj, 8, cmp_u64)) continue;
         }
-        float reach = sweep[i] + sweep[j] - MARGIN;
-        Manifold m;
-        if (!collide_shapes(w->cshape[i], w->pos[ba], w->rot[ba], w->cshape[j], w->pos[bb], w->rot[bb], reach, &m)) continue;
-        for (int k = 0; k < m.len; k++) {
-            Contact c; c.a = i; c.b = j; c.point = m.p[k].pos; c.normal = m.p[k].n;
-            c.depth = m.p[k].pen - reach + MARGIN; c.impulse = 0; c.env_mat = 0; c.feat = m.p[k].feat; c.cell = 0xFFFFFFFFu;
+        preach[p] = sweep[i] + sweep[j] - MARGIN;
+        if (!collide_shapes(w->cshape[i], w->pos[ba], w->rot[ba], w->cshape[j], w->pos[bb], w->rot[bb], preach[p], &pm[p])) pm[p].len = 0;
+    }
+    for (uint32_t p = 0; p < np; p++) {
+        const Manifold *m = &pm[p];
+        for (int k = 0; k < m->len; k++) {
+            Contact c; c.a = w->pairs[2 * p]; c.b = w->pairs[2 * p + 1]; c.point = m->p[k].pos; c.normal = m->p[k].n;
+            c.depth = m->p[k].pen - preach[p] + MARGIN; c.impulse = 0; c.env_mat = 0; c.feat = m->p[k].feat; c.cell = 0xFFFFFFFFu;
             cpush(&raw, c);
         }
     }
+    free(pm); free(preach);
     free(jkeys);
-    /* terrain (step.rs:121-142) */
+    /* terrain (step.rs:121-142): colliders in parallel above PAR_MIN, each into its own buffer,
+     * appended in collider order */
     uint32_t tc = 0;
-    for (uint32_t c = 0; c < nc; c++) tc += terrain_contacts(w, c, sweep[c], &raw);
+    CVec *tcv = calloc(nc + 1, sizeof(CVec));
+    uint32_t *tcn = calloc(nc + 1, sizeof(uint32_t));
+#pragma omp parallel for schedule(dynamic, 16) if (nc >= PAR_MIN) num_threads(g_threads)
+    for (uint32_t c = 0; c < nc; c++) tcn[c] = terrain_contacts(w, c, sweep[c], &tcv[c]);
+    for (uint32_t c = 0; c < nc; c++) {
+        tc += tcn[c];
+        for (size_t k = 0; k < tcv[c].n; k++) cpush(&raw, tcv[c].v[k]);
+        free(tcv[c].v);
+    }
+    free(tcv); free(tcn);
     w->stats.terrain_candidates = tc;
     /* S3 (step.rs:144-150) */
     CVec cul = {0};

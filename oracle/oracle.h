@@ -84,6 +84,9 @@ void or_set_body_pose(OrWorld *w, uint32_t id, const float *pose);
 void or_set_body_vel(OrWorld *w, uint32_t id, const float *vel);
 void or_set_sleep_allowed(OrWorld *w, uint32_t id, int allowed);
 void or_step(OrWorld *w, float dt);
+/* Threads of the step's parallel regions (default 1): where the reference uses rayon (narrowphase
+ * pairs, terrain colliders, PGS level batches of at least 64 rows), bit-identical for any count. */
+void or_set_threads(int n);
 void or_step_hash(const OrWorld *w, uint8_t out32[32]);
 /* world.rs:718 add_ball_joint: bodies a, b (indices), frames {px,py,pz,qx,qy,qz,qw} local to each
  * body.  Returns the joint index, or -1. */

@@ -61,6 +61,7 @@ def lib():
         L.or_set_body_vel.argtypes = [P, C.c_uint32, FP]
         L.or_set_sleep_allowed.argtypes = [P, C.c_uint32, C.c_int]
         L.or_step.argtypes = [P, C.c_float]
+        L.or_set_threads.argtypes = [C.c_int]
         L.or_step_hash.argtypes = [P, C.c_char_p]
         L.or_num_bodies.restype = C.c_uint32
         L.or_num_bodies.argtypes = [P]

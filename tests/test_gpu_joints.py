@@ -102,20 +102,28 @@ def test_world_snapshot_restore_with_joints_replays():
 
 def test_contact_log_skips_joint_entries():
     """The contact log (training data) holds one row per solved contact, none for joint entries of
-    the island lists."""
+    the island lists; in residual mode only the contacts of islands that reached the tolerance
+    (an island stopped at the sweep limit holds no converged impulses)."""
     ss = [chains(40 + i, n_ragdolls=3 + i) for i in range(2)]
     nj = sum(len(s["joints"]) for s in ss)
     b = rphys.Batch([rphys.World(s) for s in ss])
     b.step_n(10)
-    b.set_residual_mode(True, 1.0e-4, 30)
-    b.set_contact_log(200000)
+    b.set_contact_log(400000)
     expect = 0
     for _ in range(5):
         _, c = b.step_profiled(1)
         expect += c["island_contacts"] - nj  # every joint here is active with its rows
     rows, dropped = b.contact_log_rows()
     assert dropped == 0 and rows == expect and rows > 0
-    feat, targ = b.read_contact_log(0, rows)
+    b.set_residual_mode(True, 1.0e-2, 60)
+    conv = 0
+    for _ in range(5):
+        _, c = b.step_profiled(1)
+        conv += c["island_contacts_converged"]
+    rows2, dropped = b.contact_log_rows()
+    added = rows2 - rows
+    assert dropped == 0 and 0 < added <= conv and added >= conv - 5 * nj, (added, conv)
+    feat, targ = b.read_contact_log(0, rows2)
     assert np.isfinite(feat).all() and np.isfinite(targ).all()
 
 

@@ -650,6 +650,7 @@ static V3 rel_vel(const OrWorld *w, const CC *c) {
 }
 /* solver.rs:429-431 */
 static float mu_dyn(const CC *c, float slip) {
+    if (c->mu_s == c->mu_k && slip == slip) return c->mu_k + 0.0f; /* (+0) * exp(..) = +0 exactly */
     float x = slip / c->vs;
     return c->mu_k + (c->mu_s - c->mu_k) * or_expf_d(-(x * x));
 }
@@ -674,6 +675,36 @@ static int key_cmp(const uint32_t *a, const uint32_t *b) {
     for (int i = 0; i < 4; i++) if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
     return 0;
 }
+/* solver.rs:730-745: among the unused cached entries of the contact's key (the cache is sorted by
+ * key), the one nearest to its anchor with |dla|^2 <= 0.01^2; ties go to the lowest index.  The
+ * chosen entry is marked used.  Returns its index, or -1. */
+static int warm_pick(const Warm *warm, uint32_t nwarm, const uint32_t key[4], V3 anchor, uint8_t *taken) {
+    const float tol2 = 0.01f * 0.01f;
+    uint32_t lo = 0, hi = nwarm;
+    while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (key_cmp(warm[mid].key, key) < 0) lo = mid + 1; else hi = mid; }
+    if (lo >= nwarm || key_cmp(warm[lo].key, key) != 0) return -1;
+    int best = -1; float bd = 0;
+    for (uint32_t e = lo; e < nwarm && key_cmp(warm[e].key, key) == 0; e++) {
+        if (taken[e]) continue;
+        float d = vlen2(vsub(warm[e].la, anchor));
+        if (d <= tol2 && (best < 0 || d < bd)) { best = (int)e; bd = d; }
+    }
+    if (best >= 0) taken[best] = 1;
+    return best;
+}
+/* Unit entry of warm_pick: contacts (keys, anchors) matched in order against a key-sorted cache. */
+void or_warm_pick_sequence(const uint32_t *ckeys4, const float *cla3, uint32_t nwarm, const uint32_t *qkeys4, const float *anchors3,
+                           uint32_t nq, int32_t *out) {
+    Warm *warm = calloc(nwarm + 1, sizeof(Warm));
+    uint8_t *taken = calloc(nwarm + 1, 1);
+    for (uint32_t e = 0; e < nwarm; e++) {
+        memcpy(warm[e].key, ckeys4 + 4 * e, 16);
+        warm[e].la = v3(cla3[3 * e], cla3[3 * e + 1], cla3[3 * e + 2]);
+    }
+    for (uint32_t i = 0; i < nq; i++) out[i] = warm_pick(warm, nwarm, qkeys4 + 4 * i, v3(anchors3[3 * i], anchors3[3 * i + 1], anchors3[3 * i + 2]), taken);
+    free(warm); free(taken);
+}
+
 static int is_idle(const OrWorld *w, uint32_t b) {
     if (b == 0xFFFFFFFFu) return 1;
     if (b >= w->cap || !w->alive[b]) return 0;
@@ -826,20 +857,9 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     }
     uint8_t *taken = calloc(w->nwarm + 1, 1);
     uint32_t matched = 0;
-    const float tol2 = 0.01f * 0.01f;
     for (uint32_t i = 0; i < nct; i++) {
-        /* entries of this key: binary search in the sorted cache */
-        uint32_t lo = 0, hi = w->nwarm;
-        while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (key_cmp(w->warm[mid].key, keys[i]) < 0) lo = mid + 1; else hi = mid; }
-        if (lo >= w->nwarm || key_cmp(w->warm[lo].key, keys[i]) != 0) continue;
-        int best = -1; float bd = 0;
-        for (uint32_t e = lo; e < w->nwarm && key_cmp(w->warm[e].key, keys[i]) == 0; e++) {
-            if (taken[e]) continue;
-            float d = vlen2(vsub(w->warm[e].la, anch[i]));
-            if (d <= tol2 && (best < 0 || d < bd)) { best = (int)e; bd = d; }
-        }
+        int best = warm_pick(w->warm, w->nwarm, keys[i], anch[i], taken);
         if (best < 0) continue;
-        taken[best] = 1;
         matched++;
         const Warm *e = &w->warm[best];
         const CC *k = &cs[i];
@@ -1648,6 +1668,30 @@ void or_step_hash(const OrWorld *w, uint8_t out[32]) {
     or_blake3_final(&h, out);
 }
 
+uint32_t or_export_carried(const OrWorld *w, float *last3, uint8_t *awake, uint16_t *sticks, uint32_t *wkeys4, float *wla3, float *wjn,
+                           float *wjt3, uint32_t cap, uint32_t *jq, uint8_t *jn_rows, uint8_t *jtags3, float *jlam3, uint32_t *njw) {
+    for (uint32_t i = 0; last3 && i < w->n; i++) { last3[3 * i] = w->last_lin[i].x; last3[3 * i + 1] = w->last_lin[i].y; last3[3 * i + 2] = w->last_lin[i].z; }
+    uint32_t sl = w->cap > w->n ? w->cap : w->n;
+    for (uint32_t i = 0; awake && i < sl; i++) { awake[i] = w->awake[i]; sticks[i] = w->sticks[i]; }
+    for (uint32_t e = 0; wkeys4 && e < w->nwarm && e < cap; e++) {
+        memcpy(wkeys4 + 4 * e, w->warm[e].key, 16);
+        wla3[3 * e] = w->warm[e].la.x; wla3[3 * e + 1] = w->warm[e].la.y; wla3[3 * e + 2] = w->warm[e].la.z;
+        wjn[e] = w->warm[e].jn;
+        wjt3[3 * e] = w->warm[e].jt.x; wjt3[3 * e + 1] = w->warm[e].jt.y; wjt3[3 * e + 2] = w->warm[e].jt.z;
+    }
+    uint32_t k = 0;
+    for (uint32_t q = 0; q < w->nj; q++) {
+        if (!w->wj[q].has) continue;
+        if (jq) {
+            jq[k] = q; jn_rows[k] = w->wj[q].n;
+            for (uint32_t r = 0; r < w->wj[q].n; r++) { jtags3[3 * k + r] = w->wj[q].tag[r]; jlam3[3 * k + r] = w->wj[q].lam[r]; }
+        }
+        k++;
+    }
+    if (njw) *njw = k;
+    return w->nwarm;
+}
+uint32_t or_capacity(const OrWorld *w) { return w->cap; }
 uint32_t or_num_bodies(const OrWorld *w) { return w->n; }
 uint64_t or_tick(const OrWorld *w) { return w->tick; }
 void or_get_state(const OrWorld *w, float *pos, float *rot, float *lin, float *ang) {
@@ -1759,6 +1803,14 @@ int or_collide(int sa, const float *da, const float *pa, int sb, const float *db
 int or_collide_box_triangle(const float *he, const float *pose, const float *t9, float cd, float *out) {
     Tri t = {v3(t9[0], t9[1], t9[2]), v3(t9[3], t9[4], t9[5]), v3(t9[6], t9[7], t9[8])};
     ShapeD s = {OR_BOX, {he[0], he[1], he[2]}};
+    Manifold m;
+    if (!collide_shape_tri(s, v3(pose[0], pose[1], pose[2]), q4(pose[3], pose[4], pose[5], pose[6]), t, cd, &m)) return 0;
+    write_m(&m, out);
+    return m.len;
+}
+int or_collide_shape_triangle(int shape, const float *dims, const float *pose, const float *t9, float cd, float *out) {
+    ShapeD s; s.type = shape; s.d[0] = dims[0]; s.d[1] = dims[1]; s.d[2] = dims[2];
+    Tri t = {v3(t9[0], t9[1], t9[2]), v3(t9[3], t9[4], t9[5]), v3(t9[6], t9[7], t9[8])};
     Manifold m;
     if (!collide_shape_tri(s, v3(pose[0], pose[1], pose[2]), q4(pose[3], pose[4], pose[5], pose[6]), t, cd, &m)) return 0;
     write_m(&m, out);

@@ -122,6 +122,16 @@ int or_collide_box_triangle(const float *he, const float *pose, const float *tri
 int or_collide_sphere_triangle(float r, const float *center, const float *tri9, float contact_dist,
                                float *out_points);
 void or_sincosf(float x, float *s, float *c);
+int or_collide_shape_triangle(int shape, const float *dims, const float *pose, const float *tri9, float contact_dist, float *out_points);
+void or_warm_pick_sequence(const uint32_t *cache_keys4, const float *cache_la3, uint32_t nwarm, const uint32_t *keys4,
+                           const float *anchors3, uint32_t n, int32_t *out);
+/* The carried state the step hash covers (hash_carried, world.rs:884-909): last velocities, sleep
+ * slots (capacity entries), warm-cache entries (up to cap) and stored joint rows.  Returns the
+ * warm-cache entry count. */
+uint32_t or_export_carried(const OrWorld *w, float *last3, uint8_t *awake, uint16_t *sticks, uint32_t *wkeys4, float *wla3, float *wjn,
+                           float *wjt3, uint32_t cap, uint32_t *joint_idx, uint8_t *joint_rows, uint8_t *tags3, float *lams3,
+                           uint32_t *n_joint_warm);
+uint32_t or_capacity(const OrWorld *w);
 uint32_t or_broadphase(const float *aabbs6, uint32_t n, uint32_t *out_ij, uint32_t cap);
 void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *lin, const float *ang, const float *last_lin,
                    const uint32_t *awake, const uint32_t *sticks, uint64_t tick, const uint64_t *wkeys, const float *wla,

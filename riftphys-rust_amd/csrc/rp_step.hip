@@ -54,6 +54,15 @@ static __device__ __forceinline__ void set_vel(const Dev &d, uint32_t b, V3 l, V
     if (fin3(a)) d.ang[b] = S3(fin_v(a));
 }
 
+// With mu_s == mu_k the Stribeck term is (+0) * exp(..) = +0 for every slip that is not NaN (exp
+// of a finite or infinite negative argument is finite), so mu = mu_k + 0 exactly: the exp is
+// skipped, bit-identically (mu_k + 0.0f keeps the sum's sign of zero).
+static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
+    if (mu4.x == mu4.y && slip == slip) return mu4.y + 0.0f;
+    float x = slip / mu4.z;
+    return mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
+}
+
 // ------------------------------------------------------------------ S1
 __global__ void k_bodies_pre(Dev d, TickParams tp) {
     if (aborted(d)) return;
@@ -702,9 +711,7 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
                 V3 v = vb - va;
                 V3 t1 = L3(d.cs_t1[g]), t2 = L3(d.cs_t2[g]);
                 float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
-                float4 mu4 = d.cs_mu[g];
-                float x = sqrtf(vt1 * vt1 + vt2 * vt2) / mu4.z;
-                float mu = mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
+                float mu = mu_dyn(d.cs_mu[g], sqrtf(vt1 * vt1 + vt2 * vt2));
                 V3 ejt = L3(d.wk_jt[cur][best]);
                 V3 t = clamp_len_max(mk3(dot3(ejt, t1), dot3(ejt, t2), 0.0f), mu * ejn);
                 d.jt[g] = make_float2(t.x, t.y);
@@ -1104,10 +1111,6 @@ static __device__ __forceinline__ V3 rel_vel(const Dev &d, uint32_t ai, uint32_t
     V3 va = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
     V3 vb = bi == RP_NONE ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
     return vb - va;
-}
-static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
-    float x = slip / mu4.z;
-    return mu4.y + (mu4.x - mu4.y) * rp_expf(-(x * x));
 }
 // A contact row's constants during the velocity solve: everything but the bodies' velocities
 // and the row's accumulators (world inverse inertia is fixed until integration).

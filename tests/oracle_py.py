@@ -62,6 +62,14 @@ def lib():
         L.or_set_sleep_allowed.argtypes = [P, C.c_uint32, C.c_int]
         L.or_step.argtypes = [P, C.c_float]
         L.or_set_threads.argtypes = [C.c_int]
+        P8, P16, P32 = C.POINTER(C.c_uint8), C.POINTER(C.c_uint16), C.POINTER(C.c_uint32)
+        L.or_collide_shape_triangle.restype = C.c_int
+        L.or_collide_shape_triangle.argtypes = [C.c_int, FP, FP, FP, C.c_float, FP]
+        L.or_warm_pick_sequence.argtypes = [P32, FP, C.c_uint32, P32, FP, C.c_uint32, C.POINTER(C.c_int32)]
+        L.or_export_carried.restype = C.c_uint32
+        L.or_export_carried.argtypes = [P, FP, P8, P16, P32, FP, FP, FP, C.c_uint32, P32, P8, P8, FP, P32]
+        L.or_capacity.restype = C.c_uint32
+        L.or_capacity.argtypes = [P]
         L.or_step_hash.argtypes = [P, C.c_char_p]
         L.or_num_bodies.restype = C.c_uint32
         L.or_num_bodies.argtypes = [P]

@@ -341,6 +341,21 @@ typedef struct RPhysMlpTrainConfig {
 RPhysResult rphys_batch_train_warm_start(RPhysBatch *batch, const RPhysMlpTrainConfig *cfg, float *params_out, float *epoch_loss);
 /* Device inference on host features (n rows of 12) -> n rows of 3 (checking hook). */
 RPhysResult rphys_mlp_forward(const float *params, const float *feat12, uint32_t n, float *out3);
+/* No reference counterpart: rphys_mlp_forward for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  params
+ * (RP_MLP_NPARAM floats, a valid model), feat12 (n rows of 12) and out4 (n rows of 4: the three
+ * predictions, then 1 where the row passes the model's gate, else 0) are device pointers on the
+ * current device; n <= 2^30 (else RPHYS_ERR_BAD_ARG).  The kernels run on `stream` (a
+ * hipStream_t; NULL: the null stream) and the call returns once they are enqueued.  The bf16
+ * weight tiles go to a fixed 26,624-byte device array the library holds per GPU until the process
+ * ends: no call allocates.  Calls that overlap on one GPU share that array and race; serialise them
+ * (one stream, or events between them). */
+RPhysResult rphys_mlp_forward_noalloc(const float *params, const float *feat12, uint32_t n, float *out4, void *stream);
+/* Test plumbing (no reference counterpart): device buffers and blocking copies (to_host 0:
+ * host -> device, 1: device -> host) for the device-pointer entry points above. */
+RPhysResult rphys_debug_device_alloc(uint64_t bytes, void **out);
+RPhysResult rphys_debug_device_free(void *ptr);
+RPhysResult rphys_debug_device_copy(void *dst, const void *src, uint64_t bytes, uint32_t to_host);
 /* One training step's loss and gradient (the first 5,187 parameters: weights and biases) over
  * rphys_mlp_gradient_rows() given rows, without updating (checking hook). */
 RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const float *targ3, float *loss, float *grad);

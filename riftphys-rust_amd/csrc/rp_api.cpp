@@ -44,6 +44,7 @@ extern "C" void rp_launch_hash(const Dev &d, const HashDev &h, uint32_t cur, uin
 // ------------------------------------------------------------------ errors (lib.rs:56-82)
 static thread_local std::string g_last_error;
 static void set_err(const std::string &s) { g_last_error = s; }
+extern "C" void rp_set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
 static void clear_err() { g_last_error.clear(); }
 
 struct F4 { float x, y, z, w; };
@@ -2053,6 +2054,19 @@ RPhysResult rphys_mlp_forward(const float *params, const float *feat12, uint32_t
         for (int k = 0; k < 3; k++) out3[3 * (size_t)i + k] = host[4 * (size_t)i + k];
     return RPHYS_OK;
     GUARD_END
+}
+
+// Test plumbing for device-pointer entry points: plain device buffers and copies.
+RPhysResult rphys_debug_device_alloc(uint64_t bytes, void **out) {
+    if (!out) { set_err("rphys_debug_device_alloc: null arg"); return RPHYS_ERR_NULL; }
+    *out = nullptr;
+    return hip_ok(hipMalloc(out, bytes ? bytes : 1), "rphys_debug_device_alloc") ? RPHYS_OK : RPHYS_ERR_CAPACITY;
+}
+RPhysResult rphys_debug_device_free(void *p) { return hip_ok(hipFree(p), "rphys_debug_device_free") ? RPHYS_OK : RPHYS_ERR_INTERNAL; }
+RPhysResult rphys_debug_device_copy(void *dst, const void *src, uint64_t bytes, uint32_t to_host) {
+    if (!dst || !src) { set_err("rphys_debug_device_copy: null arg"); return RPHYS_ERR_NULL; }
+    return hip_ok(hipMemcpy(dst, src, bytes, to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice), "rphys_debug_device_copy")
+               ? RPHYS_OK : RPHYS_ERR_INTERNAL;
 }
 
 RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const float *targ3, float *loss, float *grad) {

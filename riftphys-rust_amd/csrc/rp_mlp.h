@@ -45,6 +45,9 @@ struct MlpWeights {
 #define RP_MLP_W3T (64 * 32)
 #define RP_MLP_NBF16 (RP_MLP_W1P + 2 * RP_MLP_W2P + RP_MLP_W3P + RP_MLP_W3T)
 
+// rphys_mlp_forward_noalloc: the most rows one call takes
+#define RP_MLP_NOALLOC_MAX_ROWS (1u << 30)
+
 // Training: one workgroup of 4 waves, RP_MLP_TILES tiles of 32 contacts per wave per Adam step.
 #define RP_MLP_TILES 4
 #define RP_MLP_STEP_CONTACTS (4 * RP_MLP_TILES * 32)

@@ -3,6 +3,10 @@
 // backward passes.  Deterministic: fixed tile order, fixed-order cross-wave reductions.
 #include <stdint.h>
 
+#include <mutex>
+#include <string>
+
+#include "../../include/riftphys_c.h"
 #include "rp_mlp.h"
 
 #define RP_LDS __attribute__((address_space(3)))
@@ -148,6 +152,59 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *p, const __bf16 
         __syncthreads();
         if (threadIdx.x == 0 && block_trusted) atomicAdd(n_trusted, block_trusted);
     }
+}
+
+// ------------------------------------------------------------------ inference without allocation
+// The bf16 weight tiles of rphys_mlp_forward_noalloc live in this file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes `scratch` itself (on the host
+// it only names the symbol): each device's address comes from hipGetSymbolAddress, taken once per
+// device with that device current.
+static __device__ __align__(256) __bf16 scratch[RP_MLP_NBF16];
+#define RP_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static __bf16 *g_scratch_ptr[RP_MAX_DEVICES];
+
+extern "C" void rp_set_last_error(const char *msg);
+
+// The scratch tiles of the current device (nullptr, with the error set, on failure).
+static __bf16 *device_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= RP_MAX_DEVICES) {
+        rp_set_last_error("rphys_mlp_forward_noalloc: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            rp_set_last_error((std::string("rphys_mlp_forward_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e)).c_str());
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (__bf16 *)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+RPhysResult rphys_mlp_forward_noalloc(const float *params, const float *feat12, uint32_t n, float *out4, void *stream) {
+    if (!params || (n && (!feat12 || !out4))) { rp_set_last_error("rphys_mlp_forward_noalloc: null arg"); return RPHYS_ERR_NULL; }
+    if (n > RP_MLP_NOALLOC_MAX_ROWS) { rp_set_last_error("rphys_mlp_forward_noalloc: n exceeds 2^30 rows"); return RPHYS_ERR_BAD_ARG; }
+    if (n == 0) return RPHYS_OK;
+    __bf16 *s = device_scratch();
+    if (!s) return RPHYS_ERR_INTERNAL;
+    __bf16 *w1p = s, *w2p = w1p + RP_MLP_W1P, *w2t = w2p + RP_MLP_W2P, *w3p = w2t + RP_MLP_W2P, *w3t = w3p + RP_MLP_W3P;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_mlp_prep, dim3(1), dim3(256), 0, st, params, w1p, w2p, w2t, w3p, w3t);
+    uint32_t blocks = (n + 63) / 64;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, params, w1p, w2p, w3p, feat12, (const uint32_t *)nullptr, n, out4,
+                       (uint32_t *)nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        rp_set_last_error((std::string("rphys_mlp_forward_noalloc: launch: ") + hipGetErrorString(e)).c_str());
+        return RPHYS_ERR_INTERNAL;
+    }
+    return RPHYS_OK;
 }
 
 extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st) {

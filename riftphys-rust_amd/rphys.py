@@ -190,6 +190,10 @@ EXPORTS = [
     ("rphys_batch_train_warm_start", U32, [P, C.POINTER(MlpTrainConfig), PF32, PF32]),
     ("rphys_mlp_forward", U32, [PF32, PF32, U32, PF32]),
     ("rphys_mlp_gradient", U32, [PF32, PF32, PF32, PF32, PF32]),
+    ("rphys_mlp_forward_noalloc", U32, [P, P, U32, P, P]),
+    ("rphys_debug_device_alloc", U32, [U64, C.POINTER(P)]),
+    ("rphys_debug_device_free", U32, [P]),
+    ("rphys_debug_device_copy", U32, [P, P, U64, U32]),
     ("rphys_debug_set_sweep_tiers", U32, [U32, U32, U32, U32]),
     ("rphys_debug_set_initial_capacity", U32, [U32, U32, U32, U32]),
 ]

@@ -371,6 +371,13 @@ RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const f
  * overflow-and-replay path runs on small scenes. */
 RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, uint32_t max_tier1, uint32_t wide_min);
 RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm);
+/* Test hook (no reference counterpart), process-wide: the lane tier, whose velocity solve runs one
+ * island per lane.  max_entries: its largest island (0 = default: tier 0's bound; clamped to it;
+ * RPHYS_LANE_NONE: no lane tier, tier 0 solves every island a wave each).  wave_islands: how
+ * many of tier 0's largest islands are solved a wave each even when within max_entries (0 =
+ * default, 2,048; RPHYS_LANE_NONE: none).  Every setting gives the same results. */
+#define RPHYS_LANE_NONE 0xFFFFFFFFu
+RPhysResult rphys_debug_set_lane_tier(uint32_t max_entries, uint32_t wave_islands);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,8 @@ enum {
     CT_SUM_ITERS_SUM_HI,  // high word of the CT_SUM_ITERS_SUM sum (sweeps x contacts outgrow 32 bits)
     CT_ITERS_SUM_CONV,    // residual mode: sweeps x entries of the islands that reached the tolerance
     CT_SUM_ITERS_CONV, CT_SUM_ITERS_CONV_HI,
+    CT_LANE_N,            // islands of the lane tier this tick
+    CT_LANE_K,            // tier-0 islands solved a wave each while the lane tier is on
     CT_COUNT
 };
 
@@ -130,6 +132,8 @@ struct Dev {
     uint32_t *lvl_off_sh, *isl_wsh;  // as lvl_off / isl_w, for the shock order of the current shock pass
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
+    float4 *lrow;            // RP_LROW * cap_lane: rows and accumulators of the lane tier, in the lane layout
+    size_t cap_lane;         // entries of lrow: cap_contacts + RP_LANE_PAD
     // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model
     float *feat;
     float4 *pred;
@@ -206,6 +210,10 @@ enum {
 };
 
 #define RP_SROW 12u
+#define RP_LROW 13u  // lane layout: the constant row (RP_SROW components), then (jn, jt, entry)
+// lane layout room beyond the island lists' length: 64 lanes x the largest lane-tier island
+#define RP_LANE_PAD (64u * 256u)
+#define RP_LANE_WAVE 2048u  // default: tier 0's largest islands solved a wave each alongside the lane tier
 
 // The position solve's SHAKE directions need both bodies turned by less than 0.5 rad this step:
 // 2 acos(min(|w|, 1)) < 0.5 for w of q_start * conj(q_now), which (acosf, rounded as glibc does)
@@ -230,5 +238,7 @@ struct TickParams {
     uint32_t ws_mode;      // warm start: 0 previous frame (the reference), 1 zero, 2 learned
     uint32_t log_on;       // append (features, converged impulses) of solved contacts to the log
     uint32_t tier_max[3];  // largest island of sweep tiers 0, 3 and 1 (within their flag storage)
+    uint32_t lane_max;     // largest island of the lane tier (at most tier_max[0]; 0: no lane tier)
+    uint32_t lane_wave;    // tier 0's largest islands solved a wave each though within lane_max
     uint32_t wide_min;     // ranks of a leading level that make it a GPU-wide level launch
 };

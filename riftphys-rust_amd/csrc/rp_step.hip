@@ -1137,18 +1137,15 @@ static __device__ __forceinline__ Row load_row(const Dev &d, uint32_t c) {
 // Rows stored per execution slot, component-major (srow[k * cap + slot], k < RP_SROW), so the
 // lanes of a wave load consecutive addresses and a row is one round of independent loads.
 static __device__ __forceinline__ float4 f4u(uint32_t a, uint32_t b, float z, float w) { return make_float4(__uint_as_float(a), __uint_as_float(b), z, w); }
-static __device__ __forceinline__ void store_row(const Dev &d, uint32_t slot, const Row &x) {
-    float4 *o = d.srow + slot;
-    const size_t C = d.cap_contacts;
+// o: the row's first component; C: the distance between components
+static __device__ __forceinline__ void store_row_at(float4 *o, size_t C, const Row &x) {
     o[0] = S3(x.ra, x.m4.x); o[C] = S3(x.rb, x.m4.y); o[2 * C] = S3(x.n, x.m4.z); o[3 * C] = S3(x.t1, x.m4.w);
     o[4 * C] = S3(x.t2, x.mu4.x); o[5 * C] = make_float4(x.mu4.y, x.mu4.z, x.inv_a, x.inv_b); o[6 * C] = f4u(x.ai, x.bi, x.ima, x.imb);
     o[7 * C] = make_float4(x.iia.c0.x, x.iia.c0.y, x.iia.c0.z, x.iia.c1.x); o[8 * C] = make_float4(x.iia.c1.y, x.iia.c1.z, x.iia.c2.x, x.iia.c2.y);
     o[9 * C] = make_float4(x.iia.c2.z, x.iib.c0.x, x.iib.c0.y, x.iib.c0.z); o[10 * C] = make_float4(x.iib.c1.x, x.iib.c1.y, x.iib.c1.z, x.iib.c2.x);
     o[11 * C] = make_float4(x.iib.c2.y, x.iib.c2.z, 0.0f, 0.0f);
 }
-static __device__ __forceinline__ Row fetch_row(const Dev &d, uint32_t slot) {
-    const float4 *o = d.srow + slot;
-    const size_t C = d.cap_contacts;
+static __device__ __forceinline__ Row fetch_row_at(const float4 *o, size_t C) {
     float4 f0 = o[0], f1 = o[C], f2 = o[2 * C], f3 = o[3 * C], f4 = o[4 * C], f5 = o[5 * C], f6 = o[6 * C];
     float4 f7 = o[7 * C], f8 = o[8 * C], f9 = o[9 * C], f10 = o[10 * C], f11 = o[11 * C];
     Row x;
@@ -1161,6 +1158,8 @@ static __device__ __forceinline__ Row fetch_row(const Dev &d, uint32_t slot) {
     x.iib.c0 = mk3(f9.y, f9.z, f9.w); x.iib.c1 = mk3(f10.x, f10.y, f10.z); x.iib.c2 = mk3(f10.w, f11.x, f11.y);
     return x;
 }
+static __device__ __forceinline__ void store_row(const Dev &d, uint32_t slot, const Row &x) { store_row_at(d.srow + slot, d.cap_contacts, x); }
+static __device__ __forceinline__ Row fetch_row(const Dev &d, uint32_t slot) { return fetch_row_at(d.srow + slot, d.cap_contacts); }
 static __device__ __forceinline__ bool same_bits(V3 a, V3 b) {
     return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) && __float_as_uint(a.z) == __float_as_uint(b.z);
 }
@@ -1202,12 +1201,13 @@ static __device__ __forceinline__ void apply_row(const Dev &d, const Row &x, V3 
 
 // Islands of this tier, one per workgroup (grid-stride); I is the island's sweep state with its
 // readiness flags, lst its contacts in list order.
-#define ISLAND_LOOP_BEGIN                                                                          \
+#define ISLAND_LOOP_BEGIN ISLAND_LOOP_BEGIN_FROM(d.tier_list[TIER], d.cnt[CT_TIER_N + TIER])
+#define ISLAND_LOOP_BEGIN_FROM(LIST, COUNT)                                                        \
     __shared__ __attribute__((aligned(4))) uint8_t flag_lds[Tier<TIER>::flag_bytes > 0 ? Tier<TIER>::flag_bytes : 4]; \
     const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);         \
-    const uint32_t ntier = d.cnt[CT_TIER_N + TIER];                                                \
+    const uint32_t *const tlist = (LIST), ntier = (COUNT);                                         \
     for (uint32_t q = blockIdx.x; q < ntier; q += gridDim.x) {                                     \
-        const uint32_t isl = d.tier_list[TIER][q];                                                 \
+        const uint32_t isl = tlist[q];                                                             \
         const uint32_t s = d.isl_start[isl], n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - s; \
         const uint32_t *lst = d.isl_v2 + s;                                                        \
         auto I = make_island<TIER>(d, flag_lds, s, n);                                             \
@@ -1266,14 +1266,16 @@ static __device__ void joint_warm(const Dev &d, uint32_t q) {
         if (lam != 0.0f) joint_apply(d, a, b, ra, rb, L3(d.jr_u[3 * q + i]), lam);
     }
 }
-// one PGS pass over joint q's rows (solver.rs:822-832); residual |d lambda| K
 // Where a sweep gathers its largest row residual (residual mode): an LDS word, and whether to.
 // The flag is explicit: an LDS variable can sit at address 0, which compares equal to a null
 // address_space(3) pointer (a kernel whose only LDS word is this one lost its residuals that way).
 struct ResidSink { RP_LDS uint32_t *p; bool on; };
-static __device__ void joint_pgs(const Dev &d, uint32_t q, ResidSink resid_bits) {
+// One PGS pass over joint q's rows (solver.rs:822-832).  Residual mode: returns the bits of the
+// largest row residual |d lambda| K (non-negative floats order as their bits; NaN above all).
+static __device__ uint32_t joint_pgs(const Dev &d, uint32_t q, bool resid) {
     const uint32_t m = d.jr_mask[q], a = d.jt_a[q], b = d.jt_b[q];
     const V3 ra = L3(d.jr_ra[q]), rb = L3(d.jr_rb[q]);
+    uint32_t rbits = 0;
     for (uint32_t i = 0; i < 3; i++) {
         if (!((m >> i) & 1u)) continue;
         const float4 u4 = d.jr_u[3 * q + i];
@@ -1283,8 +1285,9 @@ static __device__ void joint_pgs(const Dev &d, uint32_t q, ResidSink resid_bits)
         const float nw = old + (0.0f - v) * u4.w;
         d.jr_acc[3 * q + i] = nw;
         if (nw != old) joint_apply(d, a, b, ra, rb, u, nw - old);
-        if (resid_bits.on) __hip_atomic_fetch_max(resid_bits.p, __float_as_uint(fabsf(nw - old) / u4.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (resid) rbits = max(rbits, __float_as_uint(fabsf(nw - old) / u4.w));
     }
+    return rbits;
 }
 // joint q's rows become its warm rows (a joint without rows keeps its old ones)
 static __device__ __forceinline__ void joint_store(const Dev &d, uint32_t q) {
@@ -1415,8 +1418,8 @@ static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &
 
 // PGS row of contact c (solver.rs:839-879): its constant row and accumulators are fetched, before
 // the row's wait, by pgs_fetch; pgs_row keeps the two bodies' velocities in registers between the
-// normal and the friction update and stores only the components that changed.  resid_bits (LDS,
-// may be null) gathers the row residual of residual mode.
+// normal and the friction update and stores only the components that changed.  resid_bits (LDS)
+// gathers the row residual of residual mode.
 struct PgsIn { Row x; float jn; float2 jt; };
 static __device__ __forceinline__ PgsIn pgs_fetch(const Dev &d, uint32_t c, uint32_t slot) {
     PgsIn in;
@@ -1426,40 +1429,59 @@ static __device__ __forceinline__ PgsIn pgs_fetch(const Dev &d, uint32_t c, uint
     in.jt = d.jt[c];
     return in;
 }
-static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, ResidSink resid_bits) {
-    if (c & RP_JBIT) { joint_pgs(d, c & ~RP_JBIT, resid_bits); return; }
-    const Row &x = in.x;
+static __device__ __forceinline__ Vel2 load_vel2(const Dev &d, const Row &x) {
     Vel2 v;
     v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]);
     if (x.bi != RP_NONE) { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
-    const Vel2 v0 = v;
-    float vrn = dot3(rel_vel2(x, v), x.n);
-    float jraw = (x.m4.w - vrn) * x.m4.x;
-    float old = in.jn;
-    float nw = fmaxf(old + jraw, 0.0f);
-    d.jn[c] = nw;
-    apply_row2(x, v, x.n * (nw - old));
-    V3 vf = rel_vel2(x, v);
-    float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
-    float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
-    V3 ot = mk3(in.jt.x, in.jt.y, 0.0f);
-    V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
-    d.jt[c] = make_float2(nt.x, nt.y);
-    V3 dd = nt - ot;
-    apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
-    if (resid_bits.on) {  // the oracle's row residual, same operations
-        float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
-        float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
-        float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
-        __hip_atomic_fetch_max(resid_bits.p, __float_as_uint(fmaxf(r0, fmaxf(r1, r2))), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    // store what changed (a body the row does not write keeps its bits)
+    return v;
+}
+// stores what changed (a body the row does not write keeps its bits)
+static __device__ __forceinline__ void store_vel2(const Dev &d, const Row &x, const Vel2 &v, const Vel2 &v0) {
     if (!same_bits(v.la, v0.la)) d.lin[x.ai] = S3(v.la);
     if (!same_bits(v.aa, v0.aa)) d.ang[x.ai] = S3(v.aa);
     if (x.bi != RP_NONE) {
         if (!same_bits(v.lb, v0.lb)) d.lin[x.bi] = S3(v.lb);
         if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
     }
+}
+// The row's update on register copies of its bodies' velocities v and its accumulators (jn, jt).
+// Residual mode: returns the bits of the oracle's row residual (same operations).
+static __device__ __forceinline__ uint32_t pgs_core(const Row &x, Vel2 &v, float &jn, float2 &jt, bool resid) {
+    float vrn = dot3(rel_vel2(x, v), x.n);
+    float jraw = (x.m4.w - vrn) * x.m4.x;
+    float old = jn;
+    float nw = fmaxf(old + jraw, 0.0f);
+    jn = nw;
+    apply_row2(x, v, x.n * (nw - old));
+    V3 vf = rel_vel2(x, v);
+    float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
+    float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
+    V3 ot = mk3(jt.x, jt.y, 0.0f);
+    V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
+    jt = make_float2(nt.x, nt.y);
+    V3 dd = nt - ot;
+    apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
+    if (!resid) return 0u;
+    float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
+    float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
+    float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
+    return __float_as_uint(fmaxf(r0, fmaxf(r1, r2)));
+}
+static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, ResidSink resid_bits) {
+    uint32_t r;
+    if (c & RP_JBIT) {
+        r = joint_pgs(d, c & ~RP_JBIT, resid_bits.on);
+    } else {
+        Vel2 v = load_vel2(d, in.x);
+        const Vel2 v0 = v;
+        float jn = in.jn;
+        float2 jt = in.jt;
+        r = pgs_core(in.x, v, jn, jt, resid_bits.on);
+        d.jn[c] = jn;
+        d.jt[c] = jt;
+        store_vel2(d, in.x, v, v0);
+    }
+    if (resid_bits.on) __hip_atomic_fetch_max(resid_bits.p, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Velocity shock row of contact c (solver.rs:976-993): the lower body along up is held fixed.
@@ -1520,12 +1542,14 @@ static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> 
 // Velocity solve of one island per workgroup: warm-start application and the PGS iterations in
 // list order, the pre-shock impulse snapshot (what the warm cache stores), then the shock pass in
 // shock order (built before the launch from the tick-start poses, which the solve leaves as is).
+// order (tier 0 with the lane tier on): the tier's islands by decreasing size, of which this
+// launch takes the first CT_LANE_K (the rest are k_lane_vel's).
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickParams tp, uint32_t split) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickParams tp, uint32_t split, const uint32_t *order) {
     if (aborted(d)) return;
     const uint32_t cur = tp.cur;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    ISLAND_LOOP_BEGIN
+    ISLAND_LOOP_BEGIN_FROM(order ? order : d.tier_list[TIER], order ? d.cnt[CT_LANE_K] : d.cnt[CT_TIER_N + TIER])
         if (threadIdx.x == 0 && TIER == 1) atomicAdd(&d.cnt[CT_ISLC_GLB], n);
         // islands larger than the workgroup sweep in level order (the ballistic and projection
         // sweeps reuse these slots); smaller ones give every rank its own lane in list order
@@ -1594,6 +1618,219 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PGS);
         vel_shock<TIER>(d, tp, I, lst, isl, pc, 0u);
     ISLAND_LOOP_END
+}
+
+// ------------------------------------------------------------------ lane tier
+// The velocity solve of the islands of up to tp.lane_max entries, one island per lane.  These are
+// mostly chains a row or two wide (box towers, jointed capsule chains), whose rows a wave per
+// island runs one at a time; a lane walking its island's entries in list order (a topological
+// order of the chains, so the same bits as any other) keeps every lane of the wave busy instead.
+// Waves take 64 islands of similar size (lane_list: the lane tier's islands by decreasing entry
+// count) and keep their constant rows and accumulators in the lane layout: entry j, component k
+// of lane l at lrow[base + (RP_LROW j + k) 64 + l], base = 64 RP_LROW x the largest island of
+// each earlier wave, so a wave's load of one component of its lanes' j-th entries is 1 KiB
+// contiguous.  With the islands sorted, the waves' spans sum to at most the island lists' length
+// plus 64 x the largest island (cap_lane entries).
+static __device__ __forceinline__ uint32_t island_len(const Dev &d, uint32_t isl, uint32_t nisl, uint32_t nislc) {
+    return ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - d.isl_start[isl];
+}
+static __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Sort keys of tier 0's islands: decreasing size (tier 0 holds islands of at most 256 entries).
+__global__ void k_lane_keys(Dev d, TickParams tp) {
+    if (aborted(d)) return;
+    const uint32_t m = d.cnt[CT_TIER_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    GRID_STRIDE(q, m) {
+        const uint32_t isl = d.tier_list[0][q];
+        d.sk0[q] = tp.tier_max[0] - island_len(d, isl, nisl, nislc);
+        d.sv0[q] = isl;
+    }
+}
+// The lane tier's share of tier 0's sorted islands: all but the first K, K = the number of islands
+// above tp.lane_max entries, or tp.lane_wave (at most all) if that is more: the largest islands
+// are solved a wave each, concurrently, while the lanes take the many small ones.
+__global__ void k_lane_split(Dev d, TickParams tp, const uint32_t *sorted) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || aborted(d)) return;
+    const uint32_t m = d.cnt[CT_TIER_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    uint32_t lo = 0, hi = m;  // the first island of at most lane_max entries
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (island_len(d, sorted[mid], nisl, nislc) > tp.lane_max) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t k = max(lo, min(m, tp.lane_wave));
+    d.cnt[CT_LANE_K] = k;
+    d.cnt[CT_LANE_N] = m - k;
+}
+// Entry j + 1's bodies take entry j's velocities where they are the same bodies (vn loaded before
+// entry j stored them; after entry j, memory holds exactly its registers' bits).
+static __device__ __forceinline__ void forward_vel(const Row &x, const Vel2 &v, const Row &xn, Vel2 &vn) {
+    if (xn.ai == x.ai) { vn.la = v.la; vn.aa = v.aa; }
+    else if (x.bi != RP_NONE && xn.ai == x.bi) { vn.la = v.lb; vn.aa = v.ab; }
+    if (xn.bi != RP_NONE) {
+        if (xn.bi == x.ai) { vn.lb = v.la; vn.ab = v.aa; }
+        else if (xn.bi == x.bi) { vn.lb = v.lb; vn.ab = v.ab; }
+    }
+}
+// A ball joint's rows in the layout of a contact row: lever arms and 1/K of axes 0 and 1 where
+// the contact keeps ra, rb and the normal and tangent masses, the three axes with 1/K of axis 2
+// and the row mask (bits) in n, t1, t2 and m4.z, m4.w, and the bodies, inverse masses and world
+// inverse inertias where the contact keeps its own (inv_a, inv_b = the inverse masses: a joint
+// writes a body exactly when its inverse mass is positive, joint_row_apply, solver.rs:298-307).
+static __device__ __forceinline__ Row load_jrow(const Dev &d, uint32_t q) {
+    Row x;
+    x.ai = d.jt_a[q]; x.bi = d.jt_b[q];
+    x.ima = d.inv_mass[x.ai]; x.imb = d.inv_mass[x.bi];
+    x.inv_a = x.ima; x.inv_b = x.imb;
+    x.ra = L3(d.jr_ra[q]); x.rb = L3(d.jr_rb[q]);
+    const uint32_t m = d.jr_mask[q];
+    const float4 u0 = (m & 1u) ? d.jr_u[3 * q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 u1 = (m & 2u) ? d.jr_u[3 * q + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 u2 = (m & 4u) ? d.jr_u[3 * q + 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    x.n = L3(u0); x.t1 = L3(u1); x.t2 = L3(u2);
+    x.m4 = make_float4(u0.w, u1.w, u2.w, __uint_as_float(m));
+    x.mu4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    x.iia = load_m9(d.iiw + 9 * x.ai); x.iib = load_m9(d.iiw + 9 * x.bi);
+    return x;
+}
+// joint_pgs on the register copies: the same operations (apply_row2 is joint_row_apply's pair of
+// apply_impulse + apply_angular_impulse for the impulse u * d lambda)
+static __device__ __forceinline__ uint32_t joint_core(const Row &x, Vel2 &v, float4 &acc, bool resid) {
+    const uint32_t m = __float_as_uint(x.m4.w);
+    uint32_t rbits = 0;
+    const V3 u[3] = {x.n, x.t1, x.t2};
+    const float k[3] = {x.m4.x, x.m4.y, x.m4.z};
+    float lam[3] = {acc.x, acc.y, acc.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (!((m >> i) & 1u)) continue;
+        const float vel = dot3(rel_vel2(x, v), u[i]);
+        const float old = lam[i];
+        const float nw = old + (0.0f - vel) * k[i];
+        lam[i] = nw;
+        if (nw != old) apply_row2(x, v, u[i] * (nw - old));
+        if (resid) rbits = max(rbits, __float_as_uint(fabsf(nw - old) / k[i]));
+    }
+    acc = make_float4(lam[0], lam[1], lam[2], acc.w);
+    return rbits;
+}
+// One lane-tier entry in registers: its row, accumulators (and entry) and its bodies' velocities.
+struct LaneEntry { Row x; float4 acc; Vel2 v; };
+static __device__ __forceinline__ void lane_fetch(const Dev &d, const float4 *R, uint32_t j, LaneEntry &e) {
+    const float4 *o = R + (size_t)j * (64u * RP_LROW);
+    e.acc = o[64u * RP_SROW];
+    e.x = fetch_row_at(o, 64u);
+    e.v = load_vel2(d, e.x);
+}
+// Entry j of n (in cur): fetches entry j + 1 into nxt, runs entry j, stores its accumulators and
+// changed velocities, and hands nxt the velocities entry j wrote.  Returns the residual bits.
+static __device__ __forceinline__ uint32_t lane_step(const Dev &d, float4 *R, uint32_t j, uint32_t n, LaneEntry &cur, LaneEntry &nxt,
+                                                     bool resid) {
+    const bool more = j + 1 < n;
+    if (more) lane_fetch(d, R, j + 1, nxt);
+    const Vel2 v0 = cur.v;
+    uint32_t r;
+    if (__float_as_uint(cur.acc.w) & RP_JBIT) {
+        r = joint_core(cur.x, cur.v, cur.acc, resid);
+    } else {
+        float jn = cur.acc.x;
+        float2 jt = make_float2(cur.acc.y, cur.acc.z);
+        r = pgs_core(cur.x, cur.v, jn, jt, resid);
+        cur.acc = make_float4(jn, jt.x, jt.y, cur.acc.w);
+    }
+    R[(size_t)j * (64u * RP_LROW) + 64u * RP_SROW] = cur.acc;
+    store_vel2(d, cur.x, cur.v, v0);
+    if (more) forward_vel(cur.x, cur.v, nxt.x, nxt.v);
+    return r;
+}
+__global__ void __launch_bounds__(64) k_lane_vel(Dev d, TickParams tp, const uint32_t *sorted) {
+    if (aborted(d)) return;
+    const uint32_t m = d.cnt[CT_LANE_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    const uint32_t *const lane_list = sorted + d.cnt[CT_LANE_K];
+    const uint32_t nwaves = (m + 63u) / 64u, l = threadIdx.x;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    for (uint32_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
+        uint32_t before = 0;
+        for (uint32_t v = l; v < w; v += 64u) before += island_len(d, lane_list[64u * v], nisl, nislc);
+        before = wave_sum(before);
+        const uint32_t nmax = island_len(d, lane_list[64u * w], nisl, nislc);
+        if ((size_t)(before + nmax) * 64u > d.cap_lane) {  // cannot happen with sorted islands; replay with room
+            if (l == 0) { d.cnt[CT_NEED_CONTACTS] = max(d.cnt[CT_NEED_CONTACTS], nislc + 64u * nmax); d.cnt[CT_ABORT] = 1u; }
+            return;
+        }
+        float4 *R = d.lrow + (size_t)before * 64u * RP_LROW + l;
+        const uint32_t q = 64u * w + l;
+        const uint32_t isl = q < m ? lane_list[q] : 0u;
+        const uint32_t s = q < m ? d.isl_start[isl] : 0u, n = q < m ? island_len(d, isl, nisl, nislc) : 0u;
+        const uint32_t *lst = d.isl_v2 + s;
+        // the list order's slots (the ballistic and projection sweeps run on them)
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t a = d.ch_pa[s + j], b = d.ch_pb[s + j];
+            d.bs_pa[s + j] = a == RP_NONE ? RP_NONE : a - s;
+            d.bs_pb[s + j] = b == RP_NONE ? RP_NONE : b - s;
+            d.bs_c[s + j] = lst[j];
+        }
+        if (tp.ws_mode != 1)
+            for (uint32_t j = 0; j < n; j++) warm_row(d, tp, lst[j]);
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t c = lst[j];
+            float4 *o = R + (size_t)j * (64u * RP_LROW);
+            if (c & RP_JBIT) {
+                const uint32_t q = c & ~RP_JBIT;
+                store_row_at(o, 64u, load_jrow(d, q));
+                o[64u * RP_SROW] = make_float4(d.jr_acc[3 * q], d.jr_acc[3 * q + 1], d.jr_acc[3 * q + 2], __uint_as_float(c));
+                continue;
+            }
+            store_row_at(o, 64u, load_row(d, c));
+            const float2 jt = d.jt[c];
+            o[64u * RP_SROW] = make_float4(d.jn[c], jt.x, jt.y, __uint_as_float(c));
+        }
+        // PGS iterations (solver.rs:839-879); residual mode: a lane stops at its island's tolerance
+        const bool resid = tp.resid_on != 0;
+        bool act = n > 0, conv = false;
+        uint32_t used = 0;
+        for (uint32_t it = 0; it < tp.iv; it++) {
+            if (!__any(act)) break;
+            if (!act) continue;
+            // entry j + 1's row, accumulators and bodies' velocities load while entry j runs (two
+            // buffers in turn); the velocities of bodies entry j wrote are taken from its registers
+            LaneEntry A, B;
+            lane_fetch(d, R, 0, A);
+            uint32_t rbits = 0;
+            for (uint32_t j = 0; j < n; j += 2) {
+                rbits = max(rbits, lane_step(d, R, j, n, A, B, resid));
+                if (j + 1 < n) rbits = max(rbits, lane_step(d, R, j + 1, n, B, A, resid));
+            }
+            used++;
+            if (resid && __uint_as_float(rbits) <= tp.resid_tol) { conv = true; act = false; }
+        }
+        if (n) {
+            atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n);
+            if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); atomicAdd(&d.cnt[CT_ITERS_SUM_CONV], used * n); }
+        }
+        // the accumulators, the pre-shock impulse snapshot (what the warm cache stores) and, in
+        // residual mode, which entries' impulses are converged ones
+        for (uint32_t j = 0; j < n; j++) {
+            const float4 acc = R[(size_t)j * (64u * RP_LROW) + 64u * RP_SROW];
+            const uint32_t c = __float_as_uint(acc.w);
+            if (c & RP_JBIT) {
+                const uint32_t q = c & ~RP_JBIT, m = d.jr_mask[q];
+                if (m & 1u) d.jr_acc[3 * q] = acc.x;
+                if (m & 2u) d.jr_acc[3 * q + 1] = acc.y;
+                if (m & 4u) d.jr_acc[3 * q + 2] = acc.z;
+                joint_store(d, q);
+            } else {
+                d.jn[c] = acc.x;
+                d.jt[c] = make_float2(acc.y, acc.z);
+                d.jn_pre[c] = acc.x;
+            }
+            if (resid) d.rank_conv[s + j] = conv ? 1u : 0u;
+        }
+        // the velocity shock pass in shock order (solver.rs:964-993)
+        for (uint32_t j = 0; j < n; j++) vshock_row(d, up, d.sh_ord[s + j]);
+    }
 }
 
 // Every slot of wide level L of the islands in list[0..count) whose leading wide levels
@@ -2335,6 +2572,17 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     mark(RP_ST_ORDERS);
     build_chains(d, d.isl_v2, d.ch_pa, d.ch_pb, d.ch_sa, d.ch_sb, body_bits, st);
     build_shock_order(d, tp, body_bits, st);
+    // the lane tier's islands by decreasing size (the radix buffers are free until the
+    // projection's shock order)
+    const uint32_t *sorted0 = nullptr;
+    if (tp.lane_max) {
+        hipLaunchKernelGGL(k_lane_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
+        uint32_t *lk, *lv;
+        radix_sort(d, d.sk0, d.sv0, d.sk1, d.sv1, d.cnt + CT_TIER_N, d.cap_contacts, key_bits(tp.tier_max[0] - 1u), st, &lk, &lv);
+        sorted0 = lv;
+        hipLaunchKernelGGL(k_lane_split, dim3(1), dim3(64), 0, st, d, tp, sorted0);
+    }
+    const uint32_t g_lane = (d.cap_contacts + 63u) / 64u < 8192u ? (d.cap_contacts + 63u) / 64u : 8192u;
     const uint32_t g0 = grids[0], g1 = grids[1], g2 = grids[2];
     // wide levels: kl level launches per sweep (fixed iteration counts only); 0 keeps the
     // one-kernel sweeps.  Any kl gives the same results: levels beyond it join the tail sweep.
@@ -2356,8 +2604,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         (void)hipStreamWaitEvent(st, ts->ev[2], 0);
     };
     auto big_vel = [&](hipStream_t s) {
-        hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
-        hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split);
+        hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split, (const uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split, (const uint32_t *)nullptr);
         if (!split) return;
         hipLaunchKernelGGL(k_store_rows, dim3(1024), dim3(256), 0, s, d);
         if (tp.ws_mode != 1) {
@@ -2390,8 +2638,16 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     };
     // the small islands: tier 0 (one wave each) and tier 3 (mid-sized, four waves)
     const uint32_t g3 = g0 / 4 > 256u ? g0 / 4 : 256u;
-    auto vel0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
-    auto vel3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
+    // tier 0: with the lane tier on, the lanes' islands on side[0] and the largest islands (a wave
+    // each) ahead of tier 1 on the tick's stream
+    auto vel0 = [&](hipStream_t s) {
+        if (tp.lane_max) hipLaunchKernelGGL(k_lane_vel, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
+        else hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr);
+    };
+    auto vel0w = [&](hipStream_t s) {
+        if (tp.lane_max) hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, sorted0);
+    };
+    auto vel3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr); };
     // profiled with side streams: tiers 0 and 3 run alongside on side[0] and side[1], timed as one
     // stage by events ev[RP_NMARKS + 2 * which] and ev[RP_NMARKS + 2 * which + 1] on side[0]; the
     // larger tiers take the stages of the tick's stream
@@ -2409,19 +2665,22 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     };
     if (ev) {  // profiled: a stage per tier (the split big-island solve counts as large)
         mark(RP_ST_VEL_SMALL);
-        if (ts) small_timed(vel0, vel3, 0);
-        else { vel0(st); vel3(st); }
+        if (ts) small_timed(vel0, [&](hipStream_t s) { vel0w(s); vel3(s); }, 0);
+        else { vel0(st); vel0w(st); vel3(st); }
         mark(RP_ST_VEL_LARGE);
         if (split) big_vel(st);
-        else hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
+        else hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u, (const uint32_t *)nullptr);
         mark(RP_ST_VEL_HUGE);
-        if (!split) hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
+        if (!split) hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u, (const uint32_t *)nullptr);
         if (ts) (void)hipStreamWaitEvent(st, ts->ev[1], 0);
     } else if (split) {
-        fork0(vel0, vel3, big_vel);
+        fork0(vel0, vel3, [&](hipStream_t s) { vel0w(s); big_vel(s); });
     } else {
-        tiers(vel0, vel3, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
-              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
+        tiers(vel0, vel3, [&](hipStream_t s) {
+                  vel0w(s);
+                  hipLaunchKernelGGL(k_sweep_vel<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr);
+              },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr); });
     }
     if (tp.log_on) {
         hipLaunchKernelGGL(k_log_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);

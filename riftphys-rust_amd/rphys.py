@@ -21,6 +21,7 @@ OK, ERR_NULL, ERR_INVALID_ID, ERR_CAPACITY, ERR_BAD_ARG, ERR_INTERNAL = 0, 1, 2,
 BODY_STATIC, BODY_DYNAMIC = 0, 1
 SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE = 0, 1, 2
 TERRAIN_BODY_ID = 0xFFFFFFFF
+LANE_NONE = 0xFFFFFFFF  # rphys_debug_set_lane_tier: no lane tier / no wave islands
 STEP_FLAG_SOLVER_ITERS = 1
 
 
@@ -195,6 +196,7 @@ EXPORTS = [
     ("rphys_debug_device_free", U32, [P]),
     ("rphys_debug_device_copy", U32, [P, P, U64, U32]),
     ("rphys_debug_set_sweep_tiers", U32, [U32, U32, U32, U32]),
+    ("rphys_debug_set_lane_tier", U32, [U32, U32]),
     ("rphys_debug_set_initial_capacity", U32, [U32, U32, U32, U32]),
 ]
 

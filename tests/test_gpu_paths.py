@@ -1,6 +1,6 @@
 """Device paths that the default sizes leave idle, forced on small scenes and checked against the
-oracle bit for bit: every sweep tier (one wave, four waves, 1024 threads with LDS flags, 1024
-threads with flags in HBM), the GPU-wide level launches of the split form, and the capacity
+oracle bit for bit: every sweep tier (one island per lane, one wave, four waves, 1024 threads with
+LDS flags, 1024 threads with flags in HBM), the GPU-wide level launches of the split form, and the capacity
 overflow that rolls a tick back, grows the buffers and replays it (with ball joints, whose warm
 rows the failed tick has already rewritten), including rollback slots made before the growth.
 """
@@ -19,6 +19,7 @@ def hooks():
     L = rphys.lib()
     yield L
     L.rphys_debug_set_sweep_tiers(0, 0, 0, 0)
+    L.rphys_debug_set_lane_tier(0, 0)
     L.rphys_debug_set_initial_capacity(0, 0, 0, 0)
 
 
@@ -55,6 +56,39 @@ def test_sweep_tiers_bit_exact(hooks, tiers, scene):
     else:
         s, ticks = scenes.ragdolls_small(), 40
     run_pair(s, ticks)
+
+
+# (largest island of the lane tier, tier-0 islands kept a wave each): every tier-0 island by the
+# lanes; none (tier 0 solves every small island a wave each); islands of up to 8 entries by the
+# lanes and the larger ones a wave each.  (By default the 2,048 largest are solved a wave each,
+# which on these scenes is every island.)
+LANES = {"all": (0, rphys.LANE_NONE), "off": (rphys.LANE_NONE, 0), "up_to_8": (8, rphys.LANE_NONE)}
+
+
+@pytest.mark.parametrize("lanes", ["all", "off", "up_to_8"])
+@pytest.mark.parametrize("scene", ["mixed", "joints"])
+def test_lane_tier_bounds_bit_exact(hooks, lanes, scene):
+    """The velocity solve of the small islands one per lane, a wave per island, or split between
+    the two by island size: the same bits as the oracle every tick."""
+    hooks.rphys_debug_set_lane_tier(*LANES[lanes])
+    s, ticks = (scenes.small_mixed(n=96, seed=7), 40) if scene == "mixed" else (scenes.ragdolls_small(), 40)
+    run_pair(s, ticks)
+
+
+@pytest.mark.parametrize("lanes", ["all", "off"])
+def test_residual_mode_in_the_lane_tier(hooks, lanes):
+    """Residual mode with every island in the lane tier (and, for comparison, a wave each): each
+    lane stops at its own island's tolerance, with the oracle's sweep counts and hashes."""
+    hooks.rphys_debug_set_lane_tier(*LANES[lanes])
+    s = scenes.ragdolls_small()
+    s["residual"] = (1.0e-4, 60)
+    o, g = oracle_py.OracleWorld(s), rphys.World(s)
+    for t in range(40):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "hash at tick %d" % (t + 1)
+        so, sg = o.stats(), g.stats()
+        assert (so.iters_used, so.iters_sum) == (sg.pgs_iters, sg.pgs_iters_sum), (t + 1, so.iters_used, sg.pgs_iters)
 
 
 @pytest.mark.parametrize("tiers", ["split_lds", "split_hbm"])

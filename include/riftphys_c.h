@@ -371,13 +371,14 @@ RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const f
  * overflow-and-replay path runs on small scenes. */
 RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, uint32_t max_tier1, uint32_t wide_min);
 RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm);
-/* Test hook (no reference counterpart), process-wide: the lane tier, whose velocity solve runs one
- * island per lane.  max_entries: its largest island (0 = default: tier 0's bound; clamped to it;
- * RPHYS_LANE_NONE: no lane tier, tier 0 solves every island a wave each).  wave_islands: how
- * many of tier 0's largest islands are solved a wave each even when within max_entries (0 =
- * default, 2,048; RPHYS_LANE_NONE: none).  Every setting gives the same results. */
+/* Test hook (no reference counterpart), process-wide: how sweep tier 0 solves its islands'
+ * velocities.  Islands of up to lane_max entries one island per lane (0 = default, 150;
+ * RPHYS_LANE_NONE: none, and tier 0 solves every island a wave each), then up to pair_max by a
+ * pair of lanes (at most 128), up to quad_max by a group of four lanes, the larger ones a wave
+ * each (0 = default: 150 for both, that is none).  Bounds are clamped to tier 0's and kept in
+ * order; every setting gives the same results. */
 #define RPHYS_LANE_NONE 0xFFFFFFFFu
-RPhysResult rphys_debug_set_lane_tier(uint32_t max_entries, uint32_t wave_islands);
+RPhysResult rphys_debug_set_lane_tier(uint32_t lane_max, uint32_t pair_max, uint32_t quad_max);
 
 #ifdef __cplusplus
 }

@@ -281,7 +281,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     ok &= B.alloc(d.bs_pa, C) && B.alloc(d.bs_pb, C) && B.alloc(d.bs_c, C) && B.alloc(d.ss_pa, C) && B.alloc(d.ss_pb, C) && B.alloc(d.ss_c, C);
     ok &= B.alloc(d.slot_of, C) && B.alloc(d.lvl_off, C) && B.alloc(d.isl_w, C + 1) && B.alloc(d.wide_list, C + 1) && B.alloc(d.big_list, C + 1) && B.alloc(d.lvl_off_sh, C) && B.alloc(d.isl_wsh, C + 1) && B.alloc(d.bp_fill, NC) && B.alloc(d.tier_list[0], C + 1) && B.alloc(d.tier_list[1], C + 1) && B.alloc(d.tier_list[2], C + 1) && B.alloc(d.tier_list[3], C + 1) && B.alloc(d.ph, RP_PH_COUNT) && B.alloc(d.srow, (size_t)RP_SROW * C);
     d.cap_lane = C + RP_LANE_PAD;
-    ok &= B.alloc(d.lrow, (size_t)RP_LROW * d.cap_lane);
+    ok &= B.alloc(d.lrow, (size_t)RP_LROW * d.cap_lane) && B.alloc(d.lane_order, C + 1) && B.alloc(d.lid, 2 * C);
     ok &= B.alloc(d.feat, 12 * C) && B.alloc(d.pred, C);
     {
         float *mp = nullptr;
@@ -444,7 +444,7 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
 // Test hooks, process-wide: the sweep tiers' island-size bounds and the wide-level minimum (so
 // small scenes can be solved by every tier and by the split form), and the initial capacities
 // of new scenes (so the overflow-and-replay path runs on small scenes).  Zero: the default.
-static std::atomic<uint32_t> g_tier_max[3] = {{0}, {0}, {0}}, g_wide_min{0}, g_lane_max{0}, g_lane_wave{0};
+static std::atomic<uint32_t> g_tier_max[3] = {{0}, {0}, {0}}, g_wide_min{0}, g_lane_max{0}, g_pair_max{0}, g_quad_max{0};
 static std::atomic<uint32_t> g_init_caps[4] = {{0}, {0}, {0}, {0}};
 
 static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
@@ -459,9 +459,13 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
         tp.tier_max[k] = lo = std::max(v, lo);
     }
     tp.wide_min = g_wide_min.load() ? g_wide_min.load() : RP_WIDE_MIN;
-    const uint32_t lm = g_lane_max.load(), lw = g_lane_wave.load();
-    tp.lane_max = lm == 0 ? tp.tier_max[0] : (lm == RPHYS_LANE_NONE ? 0u : std::min(lm, tp.tier_max[0]));
-    tp.lane_wave = lw == 0 ? RP_LANE_WAVE : (lw == RPHYS_LANE_NONE ? 0u : lw);
+    // tier 0 with the lane tier on: lanes up to lane_max entries, lane pairs up to pair_max (at most
+    // 128), groups of four lanes up to quad_max, a wave each above (every bound within tier 0's)
+    const uint32_t lm = g_lane_max.load(), pm = g_pair_max.load(), qm = g_quad_max.load();
+    tp.lane_max = lm == RPHYS_LANE_NONE ? 0u : std::min(lm ? lm : RP_LANE_MAX, tp.tier_max[0]);
+    tp.pair_max = std::min(std::max(tp.lane_max, pm ? pm : RP_PAIR_MAX), tp.tier_max[0]);
+    if (tp.pair_max > 128u && tp.pair_max > tp.lane_max) tp.pair_max = std::max(128u, tp.lane_max);
+    tp.quad_max = std::min(std::max(tp.pair_max, qm ? qm : RP_QUAD_MAX), tp.tier_max[0]);
     // gravity.rs:85-87 q6 of the uniform field
     tp.gx = std::isfinite(w.gx) ? q6mul(w.gx) : 0.0f;
     tp.gy = std::isfinite(w.gy) ? q6mul(w.gy) : 0.0f;
@@ -1184,9 +1188,10 @@ RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, 
     g_tier_max[0] = max_tier0; g_tier_max[1] = max_tier3; g_tier_max[2] = max_tier1; g_wide_min = wide_min;
     return RPHYS_OK;
 }
-RPhysResult rphys_debug_set_lane_tier(uint32_t max_entries, uint32_t wave_islands) {
-    g_lane_max = max_entries;
-    g_lane_wave = wave_islands;
+RPhysResult rphys_debug_set_lane_tier(uint32_t lane_max, uint32_t pair_max, uint32_t quad_max) {
+    g_lane_max = lane_max;
+    g_pair_max = pair_max;
+    g_quad_max = quad_max;
     return RPHYS_OK;
 }
 RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm) {

@@ -49,7 +49,8 @@ enum {
     CT_ITERS_SUM_CONV,    // residual mode: sweeps x entries of the islands that reached the tolerance
     CT_SUM_ITERS_CONV, CT_SUM_ITERS_CONV_HI,
     CT_LANE_N,            // islands of the lane tier this tick
-    CT_LANE_K,            // tier-0 islands solved a wave each while the lane tier is on
+    CT_WAVE_K, CT_GRP_K, CT_LANE_K,  // with the lane tier on: where tier 0's sorted islands solved by
+                                     // lane groups of four, lane pairs and single lanes begin
     CT_COUNT
 };
 
@@ -134,6 +135,8 @@ struct Dev {
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
     float4 *lrow;            // RP_LROW * cap_lane: rows and accumulators of the lane tier, in the lane layout
     size_t cap_lane;         // entries of lrow: cap_contacts + RP_LANE_PAD
+    uint32_t *lane_order;    // tier 0's islands this tick by decreasing size (lane tier on)
+    uint32_t *lid;           // 2 per island-list entry: LDS velocity slot of the body each side writes
     // learned warm start: per-contact features [cap][12] and predictions [cap][4], the model
     float *feat;
     float4 *pred;
@@ -213,7 +216,12 @@ enum {
 #define RP_LROW 13u  // lane layout: the constant row (RP_SROW components), then (jn, jt, entry)
 // lane layout room beyond the island lists' length: 64 lanes x the largest lane-tier island
 #define RP_LANE_PAD (64u * 256u)
-#define RP_LANE_WAVE 2048u  // default: tier 0's largest islands solved a wave each alongside the lane tier
+// defaults of tier 0's split with the lane tier on: islands of up to RP_LANE_MAX entries one per
+// lane, then up to RP_PAIR_MAX by lane pairs (128 at most), up to RP_QUAD_MAX by groups of four
+// lanes (256 at most), the larger ones a wave each
+#define RP_LANE_MAX 150u
+#define RP_PAIR_MAX 150u
+#define RP_QUAD_MAX 150u
 
 // The position solve's SHAKE directions need both bodies turned by less than 0.5 rad this step:
 // 2 acos(min(|w|, 1)) < 0.5 for w of q_start * conj(q_now), which (acosf, rounded as glibc does)
@@ -238,7 +246,8 @@ struct TickParams {
     uint32_t ws_mode;      // warm start: 0 previous frame (the reference), 1 zero, 2 learned
     uint32_t log_on;       // append (features, converged impulses) of solved contacts to the log
     uint32_t tier_max[3];  // largest island of sweep tiers 0, 3 and 1 (within their flag storage)
-    uint32_t lane_max;     // largest island of the lane tier (at most tier_max[0]; 0: no lane tier)
-    uint32_t lane_wave;    // tier 0's largest islands solved a wave each though within lane_max
+    uint32_t lane_max;     // largest island of the lane tier (0: no lane tier; tier 0 a wave per island)
+    uint32_t pair_max;     // largest island of the lane pairs (at least lane_max, at most 128)
+    uint32_t quad_max;     // largest island of the groups of four lanes (at least pair_max)
     uint32_t wide_min;     // ranks of a leading level that make it a GPU-wide level launch
 };

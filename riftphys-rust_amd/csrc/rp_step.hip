@@ -983,7 +983,8 @@ static __device__ void build_slots(const Dev &d, const Island<FL> &I, const uint
 // One sweep over the slots: each lane walks its slots j = threadIdx.x + k * blockDim in
 // increasing order; it loads the slot's constant row with prep(contact, slot) at once, waits until
 // both predecessor slots carry this sweep's token, then runs run(contact, row) and marks the
-// slot.  The lowest unfinished slot is always ready, so every lane finishes.
+// slot.  The lowest unfinished slot is always ready, so every lane finishes.  A wave none of whose
+// lanes is ready sleeps briefly, leaving its SIMD to the waves that run rows.
 template <class FL, class P, class R>
 static __device__ void run_sweep(Island<FL> &I, const Slots &sl, P prep, R run) {
     if (I.token >= FL::max_token) I.begin();
@@ -994,6 +995,7 @@ static __device__ void run_sweep(Island<FL> &I, const Slots &sl, P prep, R run) 
     while (j < n) {
         if (fresh) { a = sl.pa[s + j]; b = sl.pb[s + j]; c = sl.c[s + j]; x = prep(c, s + j); fresh = false; }
         bool ready = (a == RP_NONE || I.fl.load(a) >= tok) && (b == RP_NONE || I.fl.load(b) >= tok);
+        if (!__any(ready)) __builtin_amdgcn_s_sleep(1);  // the whole wave waits: yield the SIMD
         if (ready) {
             run(c, x);
             I.fl.store(j, tok);
@@ -1015,6 +1017,7 @@ static __device__ void run_sweep_from(Island<FL> &I, const Slots &sl, uint32_t j
     while (j < n) {
         if (fresh) { a = sl.pa[s + j]; b = sl.pb[s + j]; c = sl.c[s + j]; x = prep(c, s + j); fresh = false; }
         bool ready = (a == RP_NONE || a < j0 || I.fl.load(a) >= tok) && (b == RP_NONE || b < j0 || I.fl.load(b) >= tok);
+        if (!__any(ready)) __builtin_amdgcn_s_sleep(1);  // the whole wave waits: yield the SIMD
         if (ready) {
             run(c, x);
             I.fl.store(j, tok);
@@ -1116,6 +1119,7 @@ static __device__ __forceinline__ V3 rel_vel(const Dev &d, uint32_t ai, uint32_t
 // and the row's accumulators (world inverse inertia is fixed until integration).
 struct Row {
     uint32_t ai, bi;
+    uint32_t la, lb;  // LDS velocity slots of the bodies the row writes (RP_NONE: the body's own, in HBM)
     float inv_a, inv_b, ima, imb;
     V3 ra, rb, n, t1, t2;
     float4 m4, mu4;
@@ -1124,6 +1128,7 @@ struct Row {
 static __device__ __forceinline__ Row load_row(const Dev &d, uint32_t c) {
     Row x;
     x.ai = d.cs_ai[c]; x.bi = d.cs_bi[c];
+    x.la = RP_NONE; x.lb = RP_NONE;
     float2 inv = d.cs_inv[c];
     x.inv_a = inv.x; x.inv_b = inv.y;
     x.ra = L3(d.cs_ra[c]); x.rb = L3(d.cs_rb[c]); x.n = L3(d.cs_n[c]); x.t1 = L3(d.cs_t1[c]); x.t2 = L3(d.cs_t2[c]);
@@ -1143,7 +1148,7 @@ static __device__ __forceinline__ void store_row_at(float4 *o, size_t C, const R
     o[4 * C] = S3(x.t2, x.mu4.x); o[5 * C] = make_float4(x.mu4.y, x.mu4.z, x.inv_a, x.inv_b); o[6 * C] = f4u(x.ai, x.bi, x.ima, x.imb);
     o[7 * C] = make_float4(x.iia.c0.x, x.iia.c0.y, x.iia.c0.z, x.iia.c1.x); o[8 * C] = make_float4(x.iia.c1.y, x.iia.c1.z, x.iia.c2.x, x.iia.c2.y);
     o[9 * C] = make_float4(x.iia.c2.z, x.iib.c0.x, x.iib.c0.y, x.iib.c0.z); o[10 * C] = make_float4(x.iib.c1.x, x.iib.c1.y, x.iib.c1.z, x.iib.c2.x);
-    o[11 * C] = make_float4(x.iib.c2.y, x.iib.c2.z, 0.0f, 0.0f);
+    o[11 * C] = make_float4(x.iib.c2.y, x.iib.c2.z, __uint_as_float(x.la), __uint_as_float(x.lb));
 }
 static __device__ __forceinline__ Row fetch_row_at(const float4 *o, size_t C) {
     float4 f0 = o[0], f1 = o[C], f2 = o[2 * C], f3 = o[3 * C], f4 = o[4 * C], f5 = o[5 * C], f6 = o[6 * C];
@@ -1156,6 +1161,7 @@ static __device__ __forceinline__ Row fetch_row_at(const float4 *o, size_t C) {
     x.ai = __float_as_uint(f6.x); x.bi = __float_as_uint(f6.y); x.ima = f6.z; x.imb = f6.w;
     x.iia.c0 = mk3(f7.x, f7.y, f7.z); x.iia.c1 = mk3(f7.w, f8.x, f8.y); x.iia.c2 = mk3(f8.z, f8.w, f9.x);
     x.iib.c0 = mk3(f9.y, f9.z, f9.w); x.iib.c1 = mk3(f10.x, f10.y, f10.z); x.iib.c2 = mk3(f10.w, f11.x, f11.y);
+    x.la = __float_as_uint(f11.z); x.lb = __float_as_uint(f11.w);
     return x;
 }
 static __device__ __forceinline__ void store_row(const Dev &d, uint32_t slot, const Row &x) { store_row_at(d.srow + slot, d.cap_contacts, x); }
@@ -1171,18 +1177,18 @@ static __device__ __forceinline__ V3 rel_vel2(const Row &x, const Vel2 &v) {
     return vb - va;
 }
 // apply_cimp on the register copies (same operations and the same sanitised values as stored)
+static __device__ __forceinline__ V3 sel3(bool c, V3 a, V3 b) { return mk3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
+// Every candidate is computed and the kept ones selected: the values of the conditional form.
 static __device__ __forceinline__ void apply_row2(const Row &x, Vel2 &v, V3 j) {
-    if (x.inv_a > 0.0f) {
-        V3 nj = -j;
-        if (x.ima > 0.0f && fin3(nj)) v.la = fin_v(v.la + nj * x.ima);
-        V3 tau = cross3(x.ra, nj);
-        if (x.ima > 0.0f && fin3(tau)) v.aa = fin_v(v.aa + mv(x.iia, tau));
-    }
-    if (x.inv_b > 0.0f) {
-        if (x.imb > 0.0f && fin3(j)) v.lb = fin_v(v.lb + j * x.imb);
-        V3 tau = cross3(x.rb, j);
-        if (x.imb > 0.0f && fin3(tau)) v.ab = fin_v(v.ab + mv(x.iib, tau));
-    }
+    const V3 nj = -j;
+    const bool wa = x.inv_a > 0.0f && x.ima > 0.0f, wb = x.inv_b > 0.0f && x.imb > 0.0f;
+    const V3 ta = cross3(x.ra, nj), tb = cross3(x.rb, j);
+    const V3 la = fin_v(v.la + nj * x.ima), aa = fin_v(v.aa + mv(x.iia, ta));
+    const V3 lb = fin_v(v.lb + j * x.imb), ab = fin_v(v.ab + mv(x.iib, tb));
+    v.la = sel3(wa && fin3(nj), la, v.la);
+    v.aa = sel3(wa && fin3(ta), aa, v.aa);
+    v.lb = sel3(wb && fin3(j), lb, v.lb);
+    v.ab = sel3(wb && fin3(tb), ab, v.ab);
 }
 // apply_cimp with the row's prefetched masses and inertias (same operations)
 static __device__ __forceinline__ void apply_row(const Dev &d, const Row &x, V3 j) {
@@ -1416,6 +1422,103 @@ static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &
     }
 }
 
+// A ball joint's rows in the layout of a contact row: the lever arms in ra, rb; axes 0, 1, 2 in n,
+// t1, t2 with their 1/K in m4.x, m4.y, m4.z; the row mask (bits) in m4.w; the bodies, inverse
+// masses and world inverse inertias where the contact keeps its own (inv_a, inv_b = the inverse
+// masses: a joint writes a body exactly when its inverse mass is positive, joint_row_apply,
+// solver.rs:298-307).  apply_row2 with the impulse u d lambda is joint_row_apply's pair of
+// apply_impulse and apply_angular_impulse, the same operations.
+static __device__ __forceinline__ Row load_jrow(const Dev &d, uint32_t q) {
+    Row x;
+    x.ai = d.jt_a[q]; x.bi = d.jt_b[q];
+    x.la = RP_NONE; x.lb = RP_NONE;
+    x.ima = d.inv_mass[x.ai]; x.imb = d.inv_mass[x.bi];
+    x.inv_a = x.ima; x.inv_b = x.imb;
+    x.ra = L3(d.jr_ra[q]); x.rb = L3(d.jr_rb[q]);
+    const uint32_t m = d.jr_mask[q];
+    const float4 u0 = (m & 1u) ? d.jr_u[3 * q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 u1 = (m & 2u) ? d.jr_u[3 * q + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 u2 = (m & 4u) ? d.jr_u[3 * q + 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    x.n = L3(u0); x.t1 = L3(u1); x.t2 = L3(u2);
+    x.m4 = make_float4(u0.w, u1.w, u2.w, __uint_as_float(m));
+    x.mu4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    x.iia = load_m9(d.iiw + 9 * x.ai); x.iib = load_m9(d.iiw + 9 * x.bi);
+    return x;
+}
+// ------------------------------------------------------------------ island velocities in LDS
+// The one-wave and four-wave tiers keep the velocities of the bodies their island writes in LDS
+// during the PGS iterations (up to VB bodies; any beyond stay in HBM): a row's wait and its
+// velocity loads are then LDS round trips.  The warm start and the shock pass work in HBM, before
+// the velocities are loaded and after they are written back.
+template <uint32_t VB> struct LdsVel {
+    RP_LDS float *lin, *ang;  // 4 floats per slot: the HBM row's bits
+    RP_LDS uint32_t *body;
+};
+static __device__ __forceinline__ V3 lds_v3(const RP_LDS float *p) { return mk3(p[0], p[1], p[2]); }
+static __device__ __forceinline__ void lds_put4(RP_LDS float *p, float4 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w; }
+static __device__ __forceinline__ float4 lds_get4(const RP_LDS float *p) { return make_float4(p[0], p[1], p[2], p[3]); }
+// The body island-list entry pos writes on side k (the chains' key, k_chain_entries).
+static __device__ __forceinline__ uint32_t entry_body(const Dev &d, uint32_t pos, uint32_t k) {
+    const uint32_t c = d.isl_v2[pos];
+    if (c & RP_JBIT) {
+        const uint32_t b = k ? d.jt_b[c & ~RP_JBIT] : d.jt_a[c & ~RP_JBIT];
+        return d.inv_mass[b] > 0.0f ? b : RP_NONE;
+    }
+    return k ? d.cs_wb[c] : d.cs_wa[c];
+}
+// Slots for the island [s, s + n)'s written bodies: each chain head (an entry side with a written
+// body and no predecessor) takes the next slot, loads its body's velocities and walks its body's
+// chain, recording the slot at every entry side it reaches (lid[2 pos + k]; RP_NONE: HBM).
+template <uint32_t VB>
+static __device__ void lds_vel_begin(const Dev &d, uint32_t s, uint32_t n, const LdsVel<VB> &L, uint32_t *nloc) {
+    if (threadIdx.x == 0) *nloc = 0;
+    for (uint32_t e = threadIdx.x; e < 2 * n; e += blockDim.x) d.lid[2 * s + e] = RP_NONE;
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < 2 * n; e += blockDim.x) {
+        const uint32_t r = e >> 1, k = e & 1u;
+        if ((k ? d.ch_pb : d.ch_pa)[s + r] != RP_NONE) continue;
+        const uint32_t body = entry_body(d, s + r, k);
+        if (body == RP_NONE) continue;
+        const uint32_t id = atomicAdd(nloc, 1u), slot = id < VB ? id : RP_NONE;
+        if (id < VB) { L.body[id] = body; lds_put4(L.lin + 4 * id, d.lin[body]); lds_put4(L.ang + 4 * id, d.ang[body]); }
+        uint32_t cr = r, ck = k;
+        while (true) {
+            d.lid[2 * (s + cr) + ck] = slot;
+            const uint32_t q = (ck ? d.ch_sb : d.ch_sa)[s + cr];
+            if (q == RP_NONE) break;
+            cr = q - s;
+            ck = entry_body(d, q, 0) == body ? 0u : 1u;
+        }
+    }
+    __syncthreads();
+}
+template <uint32_t VB>
+static __device__ void lds_vel_end(const Dev &d, const LdsVel<VB> &L, uint32_t nloc) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < min(nloc, VB); i += blockDim.x) {
+        d.lin[L.body[i]] = lds_get4(L.lin + 4 * i);
+        d.ang[L.body[i]] = lds_get4(L.ang + 4 * i);
+    }
+    __syncthreads();
+}
+template <uint32_t VB>
+static __device__ __forceinline__ Vel2 load_vel2_l(const Dev &d, const Row &x, const LdsVel<VB> &L) {
+    Vel2 v;
+    if (x.la < VB) { v.la = lds_v3(L.lin + 4 * x.la); v.aa = lds_v3(L.ang + 4 * x.la); }
+    else { v.la = L3(d.lin[x.ai]); v.aa = L3(d.ang[x.ai]); }
+    if (x.bi == RP_NONE) { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
+    else if (x.lb < VB) { v.lb = lds_v3(L.lin + 4 * x.lb); v.ab = lds_v3(L.ang + 4 * x.lb); }
+    else { v.lb = L3(d.lin[x.bi]); v.ab = L3(d.ang[x.bi]); }
+    return v;
+}
+template <uint32_t VB>
+static __device__ __forceinline__ void store_vel2_l(const Dev &d, const Row &x, const Vel2 &v, const Vel2 &v0, const LdsVel<VB> &L) {
+    if (!same_bits(v.la, v0.la)) { if (x.la < VB) lds_put4(L.lin + 4 * x.la, S3(v.la)); else d.lin[x.ai] = S3(v.la); }
+    if (!same_bits(v.aa, v0.aa)) { if (x.la < VB) lds_put4(L.ang + 4 * x.la, S3(v.aa)); else d.ang[x.ai] = S3(v.aa); }
+    if (x.bi == RP_NONE) return;
+    if (!same_bits(v.lb, v0.lb)) { if (x.lb < VB) lds_put4(L.lin + 4 * x.lb, S3(v.lb)); else d.lin[x.bi] = S3(v.lb); }
+    if (!same_bits(v.ab, v0.ab)) { if (x.lb < VB) lds_put4(L.ang + 4 * x.lb, S3(v.ab)); else d.ang[x.bi] = S3(v.ab); }
+}
 // PGS row of contact c (solver.rs:839-879): its constant row and accumulators are fetched, before
 // the row's wait, by pgs_fetch; pgs_row keeps the two bodies' velocities in registers between the
 // normal and the friction update and stores only the components that changed.  resid_bits (LDS)
@@ -1444,28 +1547,74 @@ static __device__ __forceinline__ void store_vel2(const Dev &d, const Row &x, co
         if (!same_bits(v.ab, v0.ab)) d.ang[x.bi] = S3(v.ab);
     }
 }
-// The row's update on register copies of its bodies' velocities v and its accumulators (jn, jt).
-// Residual mode: returns the bits of the oracle's row residual (same operations).
-static __device__ __forceinline__ uint32_t pgs_core(const Row &x, Vel2 &v, float &jn, float2 &jt, bool resid) {
-    float vrn = dot3(rel_vel2(x, v), x.n);
-    float jraw = (x.m4.w - vrn) * x.m4.x;
-    float old = jn;
-    float nw = fmaxf(old + jraw, 0.0f);
-    jn = nw;
-    apply_row2(x, v, x.n * (nw - old));
-    V3 vf = rel_vel2(x, v);
-    float vt1 = dot3(vf, x.t1), vt2 = dot3(vf, x.t2);
-    float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
-    V3 ot = mk3(jt.x, jt.y, 0.0f);
-    V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw);
-    jt = make_float2(nt.x, nt.y);
-    V3 dd = nt - ot;
-    apply_row2(x, v, x.t1 * dd.x + x.t2 * dd.y);
+// One entry's update on register copies of its bodies' velocities v and its accumulators acc: a
+// contact (solver.rs:839-879; acc = jn, jt) or, with joint set, a ball joint's rows in a contact
+// row's layout (solver.rs:822-832; acc = the three rows' lambdas).  The two run as three phases
+// sharing their relative velocities and impulse applications: the normal row / axis 0, the
+// friction pair / axis 1, axis 2 (joints only), each with its own operations.  Residual mode:
+// returns the bits of the entry's residual, the oracle's operations (a contact's largest of its
+// normal and friction residuals, a joint's largest row residual |d lambda| K).
+static __device__ __forceinline__ uint32_t entry_core(const Row &x, Vel2 &v, float4 &acc, bool joint, bool resid) {
+    const uint32_t m = joint ? __float_as_uint(x.m4.w) : 7u;
+    // normal row: jn = max(jn + (bias - v.n) m, 0), applied; axis 0: lambda += (0 - v.u) K, applied if changed
+    const float vn = dot3(rel_vel2(x, v), x.n);
+    const float old0 = acc.x;
+    float nw0 = old0 + ((joint ? 0.0f : x.m4.w) - vn) * x.m4.x;
+    if (!joint) nw0 = fmaxf(nw0, 0.0f);
+    if (!(m & 1u)) nw0 = old0;
+    acc.x = nw0;
+    if (!joint || nw0 != old0) apply_row2(x, v, x.n * (nw0 - old0));
+    // friction: the tangent impulses clamped to mu jn together; axis 1
+    const V3 vf = rel_vel2(x, v);
+    const float vt1 = dot3(vf, x.t1);
+    V3 imp;
+    float c1, c2 = 0.0f;
+    bool ap1;
+    if (joint) {
+        const float old1 = acc.y;
+        float nw1 = old1 + (0.0f - vt1) * x.m4.y;
+        if (!(m & 2u)) nw1 = old1;
+        acc.y = nw1;
+        c1 = nw1 - old1;
+        imp = x.t1 * c1;
+        ap1 = nw1 != old1;
+    } else {
+        const float vt2 = dot3(vf, x.t2);
+        const float mu = mu_dyn(x.mu4, sqrtf(vt1 * vt1 + vt2 * vt2));
+        const V3 ot = mk3(acc.y, acc.z, 0.0f);
+        const V3 nt = clamp_len_max(ot - mk3(vt1 * x.m4.y, vt2 * x.m4.z, 0.0f), mu * nw0);
+        acc.y = nt.x;
+        acc.z = nt.y;
+        const V3 dd = nt - ot;
+        c1 = dd.x;
+        c2 = dd.y;
+        imp = x.t1 * dd.x + x.t2 * dd.y;
+        ap1 = true;
+    }
+    if (ap1) apply_row2(x, v, imp);
+    // axis 2
+    if (joint && (m & 4u)) {
+        const float old2 = acc.z;
+        const float nw2 = old2 + (0.0f - dot3(rel_vel2(x, v), x.t2)) * x.m4.z;
+        acc.z = nw2;
+        c2 = nw2 - old2;
+        if (nw2 != old2) apply_row2(x, v, x.t2 * c2);
+    }
     if (!resid) return 0u;
-    float r0 = x.m4.x > 0.0f ? fabsf(nw - old) / x.m4.x / (1.0f + fabsf(x.m4.w)) : 0.0f;
-    float r1 = x.m4.y > 0.0f ? fabsf(dd.x) / x.m4.y : 0.0f;
-    float r2 = x.m4.z > 0.0f ? fabsf(dd.y) / x.m4.z : 0.0f;
+    const float q0 = fabsf(nw0 - old0) / x.m4.x, q1 = fabsf(c1) / x.m4.y, q2 = fabsf(c2) / x.m4.z;
+    if (joint)
+        return max((m & 1u) ? __float_as_uint(q0) : 0u, max((m & 2u) ? __float_as_uint(q1) : 0u, (m & 4u) ? __float_as_uint(q2) : 0u));
+    const float r0 = x.m4.x > 0.0f ? q0 / (1.0f + fabsf(x.m4.w)) : 0.0f;
+    const float r1 = x.m4.y > 0.0f ? q1 : 0.0f;
+    const float r2 = x.m4.z > 0.0f ? q2 : 0.0f;
     return __float_as_uint(fmaxf(r0, fmaxf(r1, r2)));
+}
+static __device__ __forceinline__ uint32_t pgs_core(const Row &x, Vel2 &v, float &jn, float2 &jt, bool resid) {
+    float4 acc = make_float4(jn, jt.x, jt.y, 0.0f);
+    const uint32_t r = entry_core(x, v, acc, false, resid);
+    jn = acc.x;
+    jt = make_float2(acc.y, acc.z);
+    return r;
 }
 static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const PgsIn &in, ResidSink resid_bits) {
     uint32_t r;
@@ -1481,6 +1630,40 @@ static __device__ __forceinline__ void pgs_row(const Dev &d, uint32_t c, const P
         d.jt[c] = jt;
         store_vel2(d, in.x, v, v0);
     }
+    if (resid_bits.on) __hip_atomic_fetch_max(resid_bits.p, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// A PGS entry of either kind with its accumulators (a contact's jn, jt; a joint's rows' lambdas).
+struct PgsE { Row x; float4 acc; };
+static __device__ __forceinline__ PgsE pgs_fetch_e(const Dev &d, uint32_t c, uint32_t slot) {
+    PgsE e;
+    e.x = fetch_row(d, slot);
+    if (c & RP_JBIT) {
+        const uint32_t q = c & ~RP_JBIT;
+        e.acc = make_float4(d.jr_acc[3 * q], d.jr_acc[3 * q + 1], d.jr_acc[3 * q + 2], 0.0f);
+    } else {
+        const float2 jt = d.jt[c];
+        e.acc = make_float4(d.jn[c], jt.x, jt.y, 0.0f);
+    }
+    return e;
+}
+template <uint32_t VB>
+static __device__ __forceinline__ void pgs_row_l(const Dev &d, uint32_t c, const PgsE &in, const LdsVel<VB> &L, ResidSink resid_bits) {
+    Vel2 v = load_vel2_l(d, in.x, L);
+    const Vel2 v0 = v;
+    float4 acc = in.acc;
+    const bool joint = (c & RP_JBIT) != 0u;
+    const uint32_t r = entry_core(in.x, v, acc, joint, resid_bits.on);
+    if (joint) {
+        const uint32_t q = c & ~RP_JBIT, m = __float_as_uint(in.x.m4.w);
+        if (m & 1u) d.jr_acc[3 * q] = acc.x;
+        if (m & 2u) d.jr_acc[3 * q + 1] = acc.y;
+        if (m & 4u) d.jr_acc[3 * q + 2] = acc.z;
+    } else {
+        d.jn[c] = acc.x;
+        d.jt[c] = make_float2(acc.y, acc.z);
+    }
+    store_vel2_l(d, in.x, v, v0, L);
     if (resid_bits.on) __hip_atomic_fetch_max(resid_bits.p, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -1543,13 +1726,13 @@ static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> 
 // list order, the pre-shock impulse snapshot (what the warm cache stores), then the shock pass in
 // shock order (built before the launch from the tick-start poses, which the solve leaves as is).
 // order (tier 0 with the lane tier on): the tier's islands by decreasing size, of which this
-// launch takes the first CT_LANE_K (the rest are k_lane_vel's).
+// launch takes the first CT_WAVE_K.
 template <int TIER>
 __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickParams tp, uint32_t split, const uint32_t *order) {
     if (aborted(d)) return;
     const uint32_t cur = tp.cur;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    ISLAND_LOOP_BEGIN_FROM(order ? order : d.tier_list[TIER], order ? d.cnt[CT_LANE_K] : d.cnt[CT_TIER_N + TIER])
+    ISLAND_LOOP_BEGIN_FROM(order ? order : d.tier_list[TIER], order ? d.cnt[CT_WAVE_K] : d.cnt[CT_TIER_N + TIER])
         if (threadIdx.x == 0 && TIER == 1) atomicAdd(&d.cnt[CT_ISLC_GLB], n);
         // islands larger than the workgroup sweep in level order (the ballistic and projection
         // sweeps reuse these slots); smaller ones give every rank its own lane in list order
@@ -1581,9 +1764,23 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
             if (threadIdx.x == 0) { atomicMax(&d.cnt[CT_ITERS_MAX], tp.iv); atomicAdd(&d.cnt[CT_ITERS_SUM], tp.iv * n); }
             continue;
         }
+        // the rows per slot, with the LDS velocity slots of their bodies in the LDS tiers
+        constexpr uint32_t VB = TIER == 0 ? 257u : (TIER == 3 ? 512u : 1u);
+        __shared__ __attribute__((aligned(16))) float v_lin[4 * VB], v_ang[4 * VB];
+        __shared__ uint32_t v_body[VB], v_n;
+        const LdsVel<VB> L{(RP_LDS float *)v_lin, (RP_LDS float *)v_ang, (RP_LDS uint32_t *)v_body};
+        if constexpr (!BIG<TIER>) lds_vel_begin<VB>(d, s, n, L, &v_n);
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
             const uint32_t c = d.bs_c[s + j];
-            if (!(c & RP_JBIT)) store_row(d, s + j, load_row(d, c));
+            if constexpr (BIG<TIER>) {
+                if (!(c & RP_JBIT)) store_row(d, s + j, load_row(d, c));
+            } else {
+                const uint32_t r = exb ? exb[s + j] : j;
+                Row x = (c & RP_JBIT) ? load_jrow(d, c & ~RP_JBIT) : load_row(d, c);
+                x.la = d.lid[2 * (s + r)];
+                x.lb = d.lid[2 * (s + r) + 1];
+                store_row(d, s + j, x);
+            }
         }
         __syncthreads();
         // residual mode: the sweep's largest row residual (non-negative floats order as their bits)
@@ -1591,16 +1788,21 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         const ResidSink rb_lds{(RP_LDS uint32_t *)&resid_bits, tp.resid_on != 0};
         uint32_t used = 0;
         bool conv = false;
-        auto pgs_prep = [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); };
-        auto pgs_apply = [&](uint32_t c, const PgsIn &in) { pgs_row(d, c, in, rb_lds); };
+        auto sweep = [&]() {
+            if constexpr (BIG<TIER>)
+                run_sweep(I, sb, [&](uint32_t c, uint32_t slot) { return pgs_fetch(d, c, slot); }, [&](uint32_t c, const PgsIn &in) { pgs_row(d, c, in, rb_lds); });
+            else
+                run_sweep(I, sb, [&](uint32_t c, uint32_t slot) { return pgs_fetch_e(d, c, slot); },
+                          [&](uint32_t c, const PgsE &in) { pgs_row_l<VB>(d, c, in, L, rb_lds); });
+        };
         if (!tp.resid_on) {
-            for (uint32_t it = 0; it < tp.iv; it++) run_sweep(I, sb, pgs_prep, pgs_apply);
+            for (uint32_t it = 0; it < tp.iv; it++) sweep();
             used = tp.iv;
         } else {
             for (uint32_t it = 0; it < tp.iv; it++) {
                 if (threadIdx.x == 0) resid_bits = 0;
                 __syncthreads();
-                run_sweep(I, sb, pgs_prep, pgs_apply);
+                sweep();
                 // every thread takes the sweep's residual before thread 0 may clear it for the
                 // next sweep, so all threads leave the loop at the same sweep
                 const float rmax = __uint_as_float(resid_bits);
@@ -1616,6 +1818,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_vel(Dev d, TickPa
         if (tp.resid_on)  // which entries' impulses are converged ones (the contact log keeps only those)
             for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) d.rank_conv[s + r] = conv ? 1u : 0u;
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PGS);
+        if constexpr (!BIG<TIER>) lds_vel_end<VB>(d, L, v_n);
         vel_shock<TIER>(d, tp, I, lst, isl, pc, 0u);
     ISLAND_LOOP_END
 }
@@ -1648,21 +1851,27 @@ __global__ void k_lane_keys(Dev d, TickParams tp) {
         d.sv0[q] = isl;
     }
 }
-// The lane tier's share of tier 0's sorted islands: all but the first K, K = the number of islands
-// above tp.lane_max entries, or tp.lane_wave (at most all) if that is more: the largest islands
-// are solved a wave each, concurrently, while the lanes take the many small ones.
+// The split of tier 0's sorted islands: [0, CT_WAVE_K) (more than tp.quad_max entries) solved a
+// wave each, [CT_WAVE_K, CT_GRP_K) (more than tp.pair_max) by groups of four lanes, [CT_GRP_K,
+// CT_LANE_K) (more than tp.lane_max) by lane pairs, the rest (CT_LANE_N of them) one per lane.
+static __device__ uint32_t first_at_most(const Dev &d, const uint32_t *sorted, uint32_t m, uint32_t bound, uint32_t nisl, uint32_t nislc) {
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (island_len(d, sorted[mid], nisl, nislc) > bound) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
 __global__ void k_lane_split(Dev d, TickParams tp, const uint32_t *sorted) {
     if (threadIdx.x != 0 || blockIdx.x != 0 || aborted(d)) return;
     const uint32_t m = d.cnt[CT_TIER_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
-    uint32_t lo = 0, hi = m;  // the first island of at most lane_max entries
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (island_len(d, sorted[mid], nisl, nislc) > tp.lane_max) lo = mid + 1;
-        else hi = mid;
-    }
-    const uint32_t k = max(lo, min(m, tp.lane_wave));
-    d.cnt[CT_LANE_K] = k;
-    d.cnt[CT_LANE_N] = m - k;
+    const uint32_t kw = first_at_most(d, sorted, m, tp.quad_max, nisl, nislc), k4 = first_at_most(d, sorted, m, tp.pair_max, nisl, nislc);
+    const uint32_t k2 = first_at_most(d, sorted, m, tp.lane_max, nisl, nislc);
+    d.cnt[CT_WAVE_K] = kw;
+    d.cnt[CT_GRP_K] = k4;
+    d.cnt[CT_LANE_K] = k2;
+    d.cnt[CT_LANE_N] = m - k2;
 }
 // Entry j + 1's bodies take entry j's velocities where they are the same bodies (vn loaded before
 // entry j stored them; after entry j, memory holds exactly its registers' bits).
@@ -1673,48 +1882,6 @@ static __device__ __forceinline__ void forward_vel(const Row &x, const Vel2 &v, 
         if (xn.bi == x.ai) { vn.lb = v.la; vn.ab = v.aa; }
         else if (xn.bi == x.bi) { vn.lb = v.lb; vn.ab = v.ab; }
     }
-}
-// A ball joint's rows in the layout of a contact row: lever arms and 1/K of axes 0 and 1 where
-// the contact keeps ra, rb and the normal and tangent masses, the three axes with 1/K of axis 2
-// and the row mask (bits) in n, t1, t2 and m4.z, m4.w, and the bodies, inverse masses and world
-// inverse inertias where the contact keeps its own (inv_a, inv_b = the inverse masses: a joint
-// writes a body exactly when its inverse mass is positive, joint_row_apply, solver.rs:298-307).
-static __device__ __forceinline__ Row load_jrow(const Dev &d, uint32_t q) {
-    Row x;
-    x.ai = d.jt_a[q]; x.bi = d.jt_b[q];
-    x.ima = d.inv_mass[x.ai]; x.imb = d.inv_mass[x.bi];
-    x.inv_a = x.ima; x.inv_b = x.imb;
-    x.ra = L3(d.jr_ra[q]); x.rb = L3(d.jr_rb[q]);
-    const uint32_t m = d.jr_mask[q];
-    const float4 u0 = (m & 1u) ? d.jr_u[3 * q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 u1 = (m & 2u) ? d.jr_u[3 * q + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 u2 = (m & 4u) ? d.jr_u[3 * q + 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    x.n = L3(u0); x.t1 = L3(u1); x.t2 = L3(u2);
-    x.m4 = make_float4(u0.w, u1.w, u2.w, __uint_as_float(m));
-    x.mu4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    x.iia = load_m9(d.iiw + 9 * x.ai); x.iib = load_m9(d.iiw + 9 * x.bi);
-    return x;
-}
-// joint_pgs on the register copies: the same operations (apply_row2 is joint_row_apply's pair of
-// apply_impulse + apply_angular_impulse for the impulse u * d lambda)
-static __device__ __forceinline__ uint32_t joint_core(const Row &x, Vel2 &v, float4 &acc, bool resid) {
-    const uint32_t m = __float_as_uint(x.m4.w);
-    uint32_t rbits = 0;
-    const V3 u[3] = {x.n, x.t1, x.t2};
-    const float k[3] = {x.m4.x, x.m4.y, x.m4.z};
-    float lam[3] = {acc.x, acc.y, acc.z};
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        if (!((m >> i) & 1u)) continue;
-        const float vel = dot3(rel_vel2(x, v), u[i]);
-        const float old = lam[i];
-        const float nw = old + (0.0f - vel) * k[i];
-        lam[i] = nw;
-        if (nw != old) apply_row2(x, v, u[i] * (nw - old));
-        if (resid) rbits = max(rbits, __float_as_uint(fabsf(nw - old) / k[i]));
-    }
-    acc = make_float4(lam[0], lam[1], lam[2], acc.w);
-    return rbits;
 }
 // One lane-tier entry in registers: its row, accumulators (and entry) and its bodies' velocities.
 struct LaneEntry { Row x; float4 acc; Vel2 v; };
@@ -1731,27 +1898,19 @@ static __device__ __forceinline__ uint32_t lane_step(const Dev &d, float4 *R, ui
     const bool more = j + 1 < n;
     if (more) lane_fetch(d, R, j + 1, nxt);
     const Vel2 v0 = cur.v;
-    uint32_t r;
-    if (__float_as_uint(cur.acc.w) & RP_JBIT) {
-        r = joint_core(cur.x, cur.v, cur.acc, resid);
-    } else {
-        float jn = cur.acc.x;
-        float2 jt = make_float2(cur.acc.y, cur.acc.z);
-        r = pgs_core(cur.x, cur.v, jn, jt, resid);
-        cur.acc = make_float4(jn, jt.x, jt.y, cur.acc.w);
-    }
+    const uint32_t r = entry_core(cur.x, cur.v, cur.acc, (__float_as_uint(cur.acc.w) & RP_JBIT) != 0u, resid);
     R[(size_t)j * (64u * RP_LROW) + 64u * RP_SROW] = cur.acc;
     store_vel2(d, cur.x, cur.v, v0);
     if (more) forward_vel(cur.x, cur.v, nxt.x, nxt.v);
     return r;
 }
-__global__ void __launch_bounds__(64) k_lane_vel(Dev d, TickParams tp, const uint32_t *sorted) {
-    if (aborted(d)) return;
+// the lane tier's waves b0, b0 + nb, ...
+static __device__ void lane_vel(const Dev &d, const TickParams &tp, const uint32_t *sorted, uint32_t b0, uint32_t nb) {
     const uint32_t m = d.cnt[CT_LANE_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     const uint32_t *const lane_list = sorted + d.cnt[CT_LANE_K];
     const uint32_t nwaves = (m + 63u) / 64u, l = threadIdx.x;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    for (uint32_t w = blockIdx.x; w < nwaves; w += gridDim.x) {
+    for (uint32_t w = b0; w < nwaves; w += nb) {
         uint32_t before = 0;
         for (uint32_t v = l; v < w; v += 64u) before += island_len(d, lane_list[64u * v], nisl, nislc);
         before = wave_sum(before);
@@ -1765,13 +1924,6 @@ __global__ void __launch_bounds__(64) k_lane_vel(Dev d, TickParams tp, const uin
         const uint32_t isl = q < m ? lane_list[q] : 0u;
         const uint32_t s = q < m ? d.isl_start[isl] : 0u, n = q < m ? island_len(d, isl, nisl, nislc) : 0u;
         const uint32_t *lst = d.isl_v2 + s;
-        // the list order's slots (the ballistic and projection sweeps run on them)
-        for (uint32_t j = 0; j < n; j++) {
-            const uint32_t a = d.ch_pa[s + j], b = d.ch_pb[s + j];
-            d.bs_pa[s + j] = a == RP_NONE ? RP_NONE : a - s;
-            d.bs_pb[s + j] = b == RP_NONE ? RP_NONE : b - s;
-            d.bs_c[s + j] = lst[j];
-        }
         if (tp.ws_mode != 1)
             for (uint32_t j = 0; j < n; j++) warm_row(d, tp, lst[j]);
         for (uint32_t j = 0; j < n; j++) {
@@ -1831,6 +1983,169 @@ __global__ void __launch_bounds__(64) k_lane_vel(Dev d, TickParams tp, const uin
         // the velocity shock pass in shock order (solver.rs:964-993)
         for (uint32_t j = 0; j < n; j++) vshock_row(d, up, d.sh_ord[s + j]);
     }
+}
+
+// ------------------------------------------------------------------ lane groups
+// The velocity solve of tier 0's mid-sized islands by groups of G lanes, 64 / G islands per wave:
+// the dataflow sweep of one wave per island (slots in list order, lane li of the group takes
+// slots li, li + G, ...; a slot runs once both its predecessor slots carry the sweep's token),
+// within each group.  These islands are a few rows wide, so G lanes keep up with their levels
+// while a wave per island would run 64-wide instructions for those few rows.  A group's lanes are
+// one wave's: their memory operations are ordered by the wave's program order, and the readiness
+// flags (LDS, one byte per slot) need only wavefront scope.  Rows and accumulators stay in the
+// slot layout of the one-wave sweeps (srow, jn / jt, jr_acc), each slot's by its own lane.
+template <int G> struct LaneGroup;
+template <> struct LaneGroup<2> { static constexpr uint32_t slots = 128; };  // islands of up to 128 entries
+template <> struct LaneGroup<4> { static constexpr uint32_t slots = 256; };  // (tier 0's bound)
+static __device__ __forceinline__ uint32_t gflag_load(const RP_LDS uint8_t *f, uint32_t i) {
+    return __hip_atomic_load(&f[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+static __device__ __forceinline__ void gflag_store(RP_LDS uint8_t *f, uint32_t i, uint32_t t) {
+    __hip_atomic_store(&f[i], (uint8_t)t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+// One sweep of an island of n slots by lane li of its group; pred(j, a, b) gives slot j's
+// predecessor slots (RP_NONE: none, always lower slots), prep(j) its row, run(j, row) runs it.  A
+// lane's next slot is often ready as soon as this one is done, so its row loads before this one
+// runs.  The group's lowest unfinished slot is always ready.
+template <int G, class PRED, class PRE, class RUN>
+static __device__ __forceinline__ void group_sweep(RP_LDS uint8_t *fl, uint32_t tok, uint32_t n, uint32_t li, PRED pred, PRE prep, RUN run) {
+    uint32_t j = li, a = RP_NONE, b = RP_NONE;
+    decltype(prep(0u)) x{};
+    if (j < n) { pred(j, a, b); x = prep(j); }
+    while (j < n) {
+        const bool ready = (a == RP_NONE || gflag_load(fl, a) >= tok) && (b == RP_NONE || gflag_load(fl, b) >= tok);
+        if (ready) {
+            const uint32_t jn = j + G;
+            uint32_t an = RP_NONE, bn = RP_NONE;
+            decltype(prep(0u)) xn{};
+            if (jn < n) { pred(jn, an, bn); xn = prep(jn); }
+            run(j, x);
+            gflag_store(fl, j, tok);
+            j = jn; a = an; b = bn; x = xn;
+        }
+    }
+}
+// the largest residual bits over the G lanes of each group
+template <int G>
+static __device__ __forceinline__ uint32_t group_max(uint32_t v) {
+    for (int o = G / 2; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+// the waves b0, b0 + nb, ... of the islands solved by groups of G lanes
+template <int G>
+static __device__ void group_vel(const Dev &d, const TickParams &tp, const uint32_t *order, uint32_t b0, uint32_t nb) {
+    constexpr uint32_t NG = 64u / G, FC = LaneGroup<G>::slots;
+    __shared__ __attribute__((aligned(4))) uint8_t flags[NG * FC];
+    const uint32_t lo = G == 4 ? d.cnt[CT_WAVE_K] : d.cnt[CT_GRP_K], hi = G == 4 ? d.cnt[CT_GRP_K] : d.cnt[CT_LANE_K];
+    const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    const uint32_t gi = threadIdx.x / G, li = threadIdx.x % G;
+    RP_LDS uint8_t *fl = (RP_LDS uint8_t *)flags + gi * FC;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    const bool resid = tp.resid_on != 0;
+    const uint32_t nwaves = (hi - lo + NG - 1u) / NG;
+    for (uint32_t w = b0; w < nwaves; w += nb) {
+        const uint32_t q = lo + w * NG + gi;
+        const uint32_t isl = q < hi ? order[q] : 0u;
+        const uint32_t s = q < hi ? d.isl_start[isl] : 0u, n = q < hi ? min(island_len(d, isl, nisl, nislc), FC) : 0u;
+        const uint32_t *lst = d.isl_v2 + s;
+        auto list_pred = [&](uint32_t j, uint32_t &a, uint32_t &b) {
+            const uint32_t pa = d.ch_pa[s + j], pb = d.ch_pb[s + j];
+            a = pa == RP_NONE ? RP_NONE : pa - s;
+            b = pb == RP_NONE ? RP_NONE : pb - s;
+        };
+        uint32_t tok = 0;
+        auto next_token = [&]() {  // the flags hold byte tokens: cleared when they run out
+            if (tok == 255u) {
+                for (uint32_t j = li; j < n; j += G) fl[j] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the clears precede every later flag access
+                tok = 0;
+            }
+            return ++tok;
+        };
+        for (uint32_t j = li; j < n; j += G) fl[j] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // warm start in list order (solver.rs:749-760)
+        if (tp.ws_mode != 1)
+            group_sweep<G>(fl, next_token(), n, li, list_pred, [](uint32_t) { return NoRow{}; }, [&](uint32_t j, const NoRow &) { warm_row(d, tp, lst[j]); });
+        for (uint32_t j = li; j < n; j += G) {
+            const uint32_t c = lst[j];
+            store_row(d, s + j, (c & RP_JBIT) ? load_jrow(d, c & ~RP_JBIT) : load_row(d, c));
+        }
+        // PGS iterations (solver.rs:839-879); residual mode: a group stops at its island's tolerance
+        bool act = n > 0, conv = false;
+        uint32_t used = 0;
+        for (uint32_t it = 0; it < tp.iv; it++) {
+            if (!__any(act)) break;
+            const uint32_t t = next_token();
+            uint32_t rbits = 0;
+            if (act)
+                group_sweep<G>(fl, t, n, li, list_pred,
+                               [&](uint32_t j) {
+                                   LaneEntry e;
+                                   const uint32_t c = lst[j];
+                                   e.x = fetch_row(d, s + j);
+                                   if (c & RP_JBIT) {
+                                       const uint32_t jq = c & ~RP_JBIT;
+                                       e.acc = make_float4(d.jr_acc[3 * jq], d.jr_acc[3 * jq + 1], d.jr_acc[3 * jq + 2], __uint_as_float(c));
+                                   } else {
+                                       const float2 jt = d.jt[c];
+                                       e.acc = make_float4(d.jn[c], jt.x, jt.y, __uint_as_float(c));
+                                   }
+                                   return e;
+                               },
+                               [&](uint32_t, LaneEntry &e) {
+                                   const uint32_t c = __float_as_uint(e.acc.w);
+                                   Vel2 v = load_vel2(d, e.x);
+                                   const Vel2 v0 = v;
+                                   rbits = max(rbits, entry_core(e.x, v, e.acc, (c & RP_JBIT) != 0u, resid));
+                                   if (c & RP_JBIT) {
+                                       const uint32_t jq = c & ~RP_JBIT, m = __float_as_uint(e.x.m4.w);
+                                       if (m & 1u) d.jr_acc[3 * jq] = e.acc.x;
+                                       if (m & 2u) d.jr_acc[3 * jq + 1] = e.acc.y;
+                                       if (m & 4u) d.jr_acc[3 * jq + 2] = e.acc.z;
+                                   } else {
+                                       d.jn[c] = e.acc.x;
+                                       d.jt[c] = make_float2(e.acc.y, e.acc.z);
+                                   }
+                                   store_vel2(d, e.x, v, v0);
+                               });
+            const uint32_t rg = group_max<G>(rbits);
+            if (act) {
+                used++;
+                if (resid && __uint_as_float(rg) <= tp.resid_tol) { conv = true; act = false; }
+            }
+        }
+        if (n && li == 0) {
+            atomicMax(&d.cnt[CT_ITERS_MAX], used); atomicAdd(&d.cnt[CT_ITERS_SUM], used * n);
+            if (conv) { atomicAdd(&d.cnt[CT_ISL_CONV], 1u); atomicAdd(&d.cnt[CT_ISLC_CONV], n); atomicAdd(&d.cnt[CT_ITERS_SUM_CONV], used * n); }
+        }
+        // the pre-shock impulse snapshot (what the warm cache stores), converged entries
+        for (uint32_t j = li; j < n; j += G) {
+            const uint32_t c = lst[j];
+            if (c & RP_JBIT) joint_store(d, c & ~RP_JBIT);
+            else d.jn_pre[c] = d.jn[c];
+            if (resid) d.rank_conv[s + j] = conv ? 1u : 0u;
+        }
+        // the velocity shock pass in shock order (solver.rs:964-993): slots are shock ranks
+        group_sweep<G>(fl, next_token(), n, li,
+                       [&](uint32_t j, uint32_t &a, uint32_t &b) {
+                           const uint32_t pa = d.sh_pa[s + j], pb = d.sh_pb[s + j];
+                           a = pa == RP_NONE ? RP_NONE : pa - s;
+                           b = pb == RP_NONE ? RP_NONE : pb - s;
+                       },
+                       [](uint32_t) { return NoRow{}; }, [&](uint32_t j, const NoRow &) { vshock_row(d, up, d.sh_ord[s + j]); });
+    }
+}
+
+// Tier 0's velocity solve with the lane tier on, but for its largest islands (k_sweep_vel<0>):
+// one launch whose first nl workgroups run the lane tier, the next ng the groups of four and the
+// rest the lane pairs, concurrently.
+__global__ void __launch_bounds__(64) k_small_vel(Dev d, TickParams tp, const uint32_t *sorted, uint32_t nl, uint32_t ng) {
+    if (aborted(d)) return;
+    const uint32_t b = blockIdx.x;
+    if (b < nl) lane_vel(d, tp, sorted, b, nl);
+    else if (b < nl + ng) group_vel<4>(d, tp, sorted, b - nl, ng);
+    else group_vel<2>(d, tp, sorted, b - nl - ng, gridDim.x - nl - ng);
 }
 
 // Every slot of wide level L of the islands in list[0..count) whose leading wide levels
@@ -2043,32 +2358,34 @@ __global__ void k_ball_init(Dev d) {
 
 // step.rs:180-206 in contact order over each island; only awake dynamic bodies' flags can change
 // an outcome, and those are chained.
+// Ballistic classification of contact c (step.rs:154-203): bodies still approaching lose the flag.
+static __device__ __forceinline__ void ball_row(const Dev &d, const TickParams &tp, uint32_t c) {
+    if (c & RP_JBIT) return;
+    uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
+    bool hb = b != RP_NONE;
+    bool ba = d.ballf[a] != 0, bb = hb && d.ballf[b] != 0;
+    if (!ba && !bb) return;
+    V3 point = L3(d.ct_p[c]);
+    V3 nrm = L3(d.cs_n[c]);
+    V3 la = ba ? (L3(d.last_lin[a]) + L3(d.lin[a])) * 0.5f : L3(d.lin[a]);
+    V3 wa = L3(d.ang[a]);
+    V3 ra = point - L3(d.pos[a]);
+    V3 lb = splat3(0.0f), wb = splat3(0.0f), rb = splat3(0.0f), vbl = splat3(0.0f);
+    if (hb) { lb = bb ? (L3(d.last_lin[b]) + L3(d.lin[b])) * 0.5f : L3(d.lin[b]); wb = L3(d.ang[b]); rb = point - L3(d.pos[b]); vbl = L3(d.lin[b]); }
+    float approach = dot3((lb + cross3(wb, rb)) - (la + cross3(wa, ra)), nrm);
+    float vend = dot3((vbl + cross3(wb, rb)) - (L3(d.lin[a]) + cross3(wa, ra)), nrm);
+    float s0 = -(d.ct_p[c].w - 0.05f);
+    if (s0 + (2.0f * vend) * tp.dt < 0.0f || s0 + approach * tp.dt < 0.0f) {
+        if (d.cs_wa[c] != RP_NONE) d.ballf[a] = 0;
+        if (d.cs_wb[c] != RP_NONE) d.ballf[b] = 0;
+    }
+}
 template <int TIER>
 __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_ball(Dev d, TickParams tp) {
     if (aborted(d)) return;
     ISLAND_LOOP_BEGIN
         PhaseClock pc;
-        run_sweep(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, [&](uint32_t c) {
-            if (c & RP_JBIT) return;
-            uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
-            bool hb = b != RP_NONE;
-            bool ba = d.ballf[a] != 0, bb = hb && d.ballf[b] != 0;
-            if (!ba && !bb) return;
-            V3 point = L3(d.ct_p[c]);
-            V3 nrm = L3(d.cs_n[c]);
-            V3 la = ba ? (L3(d.last_lin[a]) + L3(d.lin[a])) * 0.5f : L3(d.lin[a]);
-            V3 wa = L3(d.ang[a]);
-            V3 ra = point - L3(d.pos[a]);
-            V3 lb = splat3(0.0f), wb = splat3(0.0f), rb = splat3(0.0f), vbl = splat3(0.0f);
-            if (hb) { lb = bb ? (L3(d.last_lin[b]) + L3(d.lin[b])) * 0.5f : L3(d.lin[b]); wb = L3(d.ang[b]); rb = point - L3(d.pos[b]); vbl = L3(d.lin[b]); }
-            float approach = dot3((lb + cross3(wb, rb)) - (la + cross3(wa, ra)), nrm);
-            float vend = dot3((vbl + cross3(wb, rb)) - (L3(d.lin[a]) + cross3(wa, ra)), nrm);
-            float s0 = -(d.ct_p[c].w - 0.05f);
-            if (s0 + (2.0f * vend) * tp.dt < 0.0f || s0 + approach * tp.dt < 0.0f) {
-                if (d.cs_wa[c] != RP_NONE) d.ballf[a] = 0;
-                if (d.cs_wb[c] != RP_NONE) d.ballf[b] = 0;
-            }
-        });
+        run_sweep(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, [&](uint32_t c) { ball_row(d, tp, c); });
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_BALL);
     ISLAND_LOOP_END
 }
@@ -2303,6 +2620,35 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d,
     ISLAND_LOOP_END
 }
 
+// Tier 0's ballistic pass, projection iterations and projection shock pass with the lane tier on:
+// one island per thread (order: the tier's islands), entries in list order (shock order for the
+// shock pass), the same rows as the sweeps.
+#define LANE_ISLANDS_BEGIN                                                                           \
+    const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);            \
+    GRID_STRIDE(q, d.cnt[CT_TIER_N]) {                                                               \
+        const uint32_t isl = order[q], s = d.isl_start[isl], n = island_len(d, isl, nisl, nislc);
+__global__ void __launch_bounds__(64) k_lane_ball(Dev d, TickParams tp, const uint32_t *order) {
+    if (aborted(d)) return;
+    LANE_ISLANDS_BEGIN
+        for (uint32_t j = 0; j < n; j++) ball_row(d, tp, d.isl_v2[s + j]);
+    }
+}
+__global__ void __launch_bounds__(64) k_lane_proj(Dev d, TickParams tp, const uint32_t *order) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    LANE_ISLANDS_BEGIN
+        for (uint32_t it = 0; it < tp.ip; it++)
+            for (uint32_t j = 0; j < n; j++) project_row(d, load_prow(d, d.isl_v2[s + j]), false, up);
+    }
+}
+__global__ void __launch_bounds__(64) k_lane_pshock(Dev d, TickParams tp, const uint32_t *order) {
+    if (aborted(d)) return;
+    const V3 up = mk3(tp.upx, tp.upy, tp.upz);
+    LANE_ISLANDS_BEGIN
+        for (uint32_t j = 0; j < n; j++) project_row(d, load_prow(d, d.sh_ord[s + j]), true, up);
+    }
+}
+
 // Split form of the projection shock pass: wide level L across the GPU, then the tail.
 __global__ void __launch_bounds__(256) k_level_pshock(Dev d, TickParams tp, uint32_t L) {
     if (aborted(d)) return;
@@ -2405,6 +2751,7 @@ __global__ void k_tick_done(Dev d) {
 }
 __global__ void k_set_counter(uint32_t *cnt, uint32_t idx, uint32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) cnt[idx] = v; }
 __global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n) { GRID_STRIDE(i, n) dst[i] = src[i]; }
+__global__ void k_copy_counted(const uint32_t *src, uint32_t *dst, const uint32_t *n) { GRID_STRIDE(i, *n) dst[i] = src[i]; }
 
 // ------------------------------------------------------------------ host-side launch sequence
 extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
@@ -2579,7 +2926,9 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         hipLaunchKernelGGL(k_lane_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         uint32_t *lk, *lv;
         radix_sort(d, d.sk0, d.sv0, d.sk1, d.sv1, d.cnt + CT_TIER_N, d.cap_contacts, key_bits(tp.tier_max[0] - 1u), st, &lk, &lv);
-        sorted0 = lv;
+        // kept for the projection passes (the radix buffers serve the projection's shock order)
+        hipLaunchKernelGGL(k_copy_counted, dim3(grid_for(d.cap_contacts, B, 4096)), dim3(B), 0, st, lv, d.lane_order, d.cnt + CT_TIER_N);
+        sorted0 = d.lane_order;
         hipLaunchKernelGGL(k_lane_split, dim3(1), dim3(64), 0, st, d, tp, sorted0);
     }
     const uint32_t g_lane = (d.cap_contacts + 63u) / 64u < 8192u ? (d.cap_contacts + 63u) / 64u : 8192u;
@@ -2638,10 +2987,10 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     };
     // the small islands: tier 0 (one wave each) and tier 3 (mid-sized, four waves)
     const uint32_t g3 = g0 / 4 > 256u ? g0 / 4 : 256u;
-    // tier 0: with the lane tier on, the lanes' islands on side[0] and the largest islands (a wave
-    // each) ahead of tier 1 on the tick's stream
+    // tier 0: with the lane tier on, the lanes' and lane groups' islands on side[0] and the largest
+    // islands (a wave each) ahead of tier 1 on the tick's stream
     auto vel0 = [&](hipStream_t s) {
-        if (tp.lane_max) hipLaunchKernelGGL(k_lane_vel, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
+        if (tp.lane_max) hipLaunchKernelGGL(k_small_vel, dim3(3u * g_lane), dim3(64), 0, s, d, tp, sorted0, g_lane, g_lane);
         else hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr);
     };
     auto vel0w = [&](hipStream_t s) {
@@ -2701,15 +3050,22 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
-    tiers([&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp); },
-          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp); },
+    // tier 0's share of each pass below: with the lane tier on, one island per thread (k_lane_*)
+    auto ball0 = [&](hipStream_t s) {
+        if (sorted0) hipLaunchKernelGGL(k_lane_ball, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
+        else hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp);
+    };
+    tiers(ball0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp); },
           [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
           [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
     mark(RP_ST_INTEGRATE);
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
-    auto proj0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
+    auto proj0 = [&](hipStream_t s) {
+        if (sorted0) hipLaunchKernelGGL(k_lane_proj, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
+        else hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u);
+    };
     auto proj3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
     if (ev) {
         mark(RP_ST_PROJ_SMALL);
@@ -2730,7 +3086,10 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses
     mark(RP_ST_PROJ_SHOCK);
-    auto pshock0 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u); };
+    auto pshock0 = [&](hipStream_t s) {
+        if (sorted0) hipLaunchKernelGGL(k_lane_pshock, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
+        else hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u);
+    };
     auto pshock3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
     auto big_pshock = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);

@@ -19,7 +19,7 @@ def hooks():
     L = rphys.lib()
     yield L
     L.rphys_debug_set_sweep_tiers(0, 0, 0, 0)
-    L.rphys_debug_set_lane_tier(0, 0)
+    L.rphys_debug_set_lane_tier(0, 0, 0)
     L.rphys_debug_set_initial_capacity(0, 0, 0, 0)
 
 
@@ -58,27 +58,27 @@ def test_sweep_tiers_bit_exact(hooks, tiers, scene):
     run_pair(s, ticks)
 
 
-# (largest island of the lane tier, tier-0 islands kept a wave each): every tier-0 island by the
-# lanes; none (tier 0 solves every small island a wave each); islands of up to 8 entries by the
-# lanes and the larger ones a wave each.  (By default the 2,048 largest are solved a wave each,
-# which on these scenes is every island.)
-LANES = {"all": (0, rphys.LANE_NONE), "off": (rphys.LANE_NONE, 0), "up_to_8": (8, rphys.LANE_NONE)}
+# (largest island solved one per lane, by a lane pair, by a group of four lanes; tier 0's larger
+# ones a wave each): every tier-0 island by lanes; none (a wave per island); every split, with
+# lanes up to 4 entries, pairs up to 16, groups of four up to 64; lane pairs up to 128; groups of
+# four for every island of two or more entries
+LANES = {"all": (256, 0, 0), "off": (rphys.LANE_NONE, 0, 0), "split": (4, 16, 64), "pairs": (1, 128, 128), "groups": (1, 1, 256)}
 
 
-@pytest.mark.parametrize("lanes", ["all", "off", "up_to_8"])
+@pytest.mark.parametrize("lanes", ["all", "off", "split", "pairs", "groups"])
 @pytest.mark.parametrize("scene", ["mixed", "joints"])
 def test_lane_tier_bounds_bit_exact(hooks, lanes, scene):
-    """The velocity solve of the small islands one per lane, a wave per island, or split between
-    the two by island size: the same bits as the oracle every tick."""
+    """Tier 0's velocity solve one island per lane, by lane pairs and groups of four, or a wave per
+    island: the same bits as the oracle every tick."""
     hooks.rphys_debug_set_lane_tier(*LANES[lanes])
     s, ticks = (scenes.small_mixed(n=96, seed=7), 40) if scene == "mixed" else (scenes.ragdolls_small(), 40)
     run_pair(s, ticks)
 
 
-@pytest.mark.parametrize("lanes", ["all", "off"])
+@pytest.mark.parametrize("lanes", ["all", "split", "groups", "off"])
 def test_residual_mode_in_the_lane_tier(hooks, lanes):
-    """Residual mode with every island in the lane tier (and, for comparison, a wave each): each
-    lane stops at its own island's tolerance, with the oracle's sweep counts and hashes."""
+    """Residual mode by lanes, lane groups and a wave per island: each island stops at its own
+    tolerance, with the oracle's sweep counts and hashes."""
     hooks.rphys_debug_set_lane_tier(*LANES[lanes])
     s = scenes.ragdolls_small()
     s["residual"] = (1.0e-4, 60)

@@ -362,7 +362,7 @@ RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const f
 
 /* Test hooks (no reference counterpart), process-wide, 0 = default.
  * rphys_debug_set_sweep_tiers: the largest island solved by sweep tiers 0 (one wave, default 256
- * entries), 3 (four waves, 4,096) and 1 (1024 threads with LDS flags, 131,072), larger islands by
+ * entries), 3 (one wave, 4,096) and 1 (1024 threads with LDS flags, 131,072), larger islands by
  * tier 2 (flags in HBM); and the width from which an island's leading level runs as a GPU-wide
  * launch (default 1,024).  Bounds are clamped to each tier's flag storage, so islands only move
  * to wider tiers; every setting gives the same results.
@@ -375,10 +375,11 @@ RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uin
  * velocities.  Islands of up to lane_max entries one island per lane (0 = default, 150;
  * RPHYS_LANE_NONE: none, and tier 0 solves every island a wave each), then up to pair_max by a
  * pair of lanes (at most 128), up to quad_max by a group of four lanes, the larger ones a wave
- * each (0 = default: 150 for both, that is none).  Bounds are clamped to tier 0's and kept in
+ * each (0 = default: 150 for both, that is none); but at least the wave_min largest islands a wave
+ * each (0 = default, 2,048; RPHYS_LANE_NONE: none).  Bounds are clamped to tier 0's and kept in
  * order; every setting gives the same results. */
 #define RPHYS_LANE_NONE 0xFFFFFFFFu
-RPhysResult rphys_debug_set_lane_tier(uint32_t lane_max, uint32_t pair_max, uint32_t quad_max);
+RPhysResult rphys_debug_set_lane_tier(uint32_t lane_max, uint32_t pair_max, uint32_t quad_max, uint32_t wave_min);
 
 #ifdef __cplusplus
 }

@@ -444,7 +444,7 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
 // Test hooks, process-wide: the sweep tiers' island-size bounds and the wide-level minimum (so
 // small scenes can be solved by every tier and by the split form), and the initial capacities
 // of new scenes (so the overflow-and-replay path runs on small scenes).  Zero: the default.
-static std::atomic<uint32_t> g_tier_max[3] = {{0}, {0}, {0}}, g_wide_min{0}, g_lane_max{0}, g_pair_max{0}, g_quad_max{0};
+static std::atomic<uint32_t> g_tier_max[3] = {{0}, {0}, {0}}, g_wide_min{0}, g_lane_max{0}, g_pair_max{0}, g_quad_max{0}, g_wave_min{0};
 static std::atomic<uint32_t> g_init_caps[4] = {{0}, {0}, {0}, {0}};
 
 static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
@@ -466,6 +466,8 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
     tp.pair_max = std::min(std::max(tp.lane_max, pm ? pm : RP_PAIR_MAX), tp.tier_max[0]);
     if (tp.pair_max > 128u && tp.pair_max > tp.lane_max) tp.pair_max = std::max(128u, tp.lane_max);
     tp.quad_max = std::min(std::max(tp.pair_max, qm ? qm : RP_QUAD_MAX), tp.tier_max[0]);
+    const uint32_t wm = g_wave_min.load();
+    tp.wave_min = wm == 0 ? RP_WAVE_MIN : (wm == RPHYS_LANE_NONE ? 0u : wm);
     // gravity.rs:85-87 q6 of the uniform field
     tp.gx = std::isfinite(w.gx) ? q6mul(w.gx) : 0.0f;
     tp.gy = std::isfinite(w.gy) ? q6mul(w.gy) : 0.0f;
@@ -1188,10 +1190,11 @@ RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, 
     g_tier_max[0] = max_tier0; g_tier_max[1] = max_tier3; g_tier_max[2] = max_tier1; g_wide_min = wide_min;
     return RPHYS_OK;
 }
-RPhysResult rphys_debug_set_lane_tier(uint32_t lane_max, uint32_t pair_max, uint32_t quad_max) {
+RPhysResult rphys_debug_set_lane_tier(uint32_t lane_max, uint32_t pair_max, uint32_t quad_max, uint32_t wave_min) {
     g_lane_max = lane_max;
     g_pair_max = pair_max;
     g_quad_max = quad_max;
+    g_wave_min = wave_min;
     return RPHYS_OK;
 }
 RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm) {

@@ -919,12 +919,12 @@ struct GlbFlags {
     __device__ __forceinline__ uint32_t dec(uint32_t i) const { return __hip_atomic_fetch_sub(&f[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 };
 template <int TIER> struct Tier;
-// Tier 0 is one wave per island: its islands (towers, chains, small piles) are mostly deep chains
-// a few rows wide, so a single wave keeps up with them, and four times as many of them fit on the
-// chip at once as with 256-thread workgroups.  Tier 3 takes the mid-sized islands (up to 4,096
-// entries) with four waves; tiers 1 and 2 are the big islands (wide levels, split form, phase clocks).
+// Tiers 0 and 3 are one wave per island: their islands (towers, chains, piles of up to 4,096
+// entries) are mostly deep chains a few to a few tens of rows wide, so a single wave keeps up with
+// their levels, and a level's rows run in one pass of one wave rather than a pass in each of the
+// waves they fall in.  Tiers 1 and 2 are the big islands (wide levels, split form, phase clocks).
 template <> struct Tier<0> { static constexpr uint32_t threads = 64, flag_bytes = 256; };
-template <> struct Tier<3> { static constexpr uint32_t threads = 256, flag_bytes = 4096; };
+template <> struct Tier<3> { static constexpr uint32_t threads = 64, flag_bytes = 4096; };
 template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
 template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
 template <int TIER> constexpr bool BIG = TIER == 1 || TIER == 2;
@@ -1851,9 +1851,11 @@ __global__ void k_lane_keys(Dev d, TickParams tp) {
         d.sv0[q] = isl;
     }
 }
-// The split of tier 0's sorted islands: [0, CT_WAVE_K) (more than tp.quad_max entries) solved a
-// wave each, [CT_WAVE_K, CT_GRP_K) (more than tp.pair_max) by groups of four lanes, [CT_GRP_K,
-// CT_LANE_K) (more than tp.lane_max) by lane pairs, the rest (CT_LANE_N of them) one per lane.
+// The split of tier 0's sorted islands: [0, CT_WAVE_K) (more than tp.quad_max entries, and at
+// least the tp.wave_min largest: while a wave per island still fits the chip at once, its
+// islands' levels run in parallel) solved a wave each, [CT_WAVE_K, CT_GRP_K) (more than
+// tp.pair_max) by groups of four lanes, [CT_GRP_K, CT_LANE_K) (more than tp.lane_max) by lane
+// pairs, the rest (CT_LANE_N of them) one per lane.
 static __device__ uint32_t first_at_most(const Dev &d, const uint32_t *sorted, uint32_t m, uint32_t bound, uint32_t nisl, uint32_t nislc) {
     uint32_t lo = 0, hi = m;
     while (lo < hi) {
@@ -1866,47 +1868,75 @@ static __device__ uint32_t first_at_most(const Dev &d, const uint32_t *sorted, u
 __global__ void k_lane_split(Dev d, TickParams tp, const uint32_t *sorted) {
     if (threadIdx.x != 0 || blockIdx.x != 0 || aborted(d)) return;
     const uint32_t m = d.cnt[CT_TIER_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
-    const uint32_t kw = first_at_most(d, sorted, m, tp.quad_max, nisl, nislc), k4 = first_at_most(d, sorted, m, tp.pair_max, nisl, nislc);
-    const uint32_t k2 = first_at_most(d, sorted, m, tp.lane_max, nisl, nislc);
+    const uint32_t kw = max(first_at_most(d, sorted, m, tp.quad_max, nisl, nislc), min(m, tp.wave_min));
+    const uint32_t k4 = max(first_at_most(d, sorted, m, tp.pair_max, nisl, nislc), kw);
+    const uint32_t k2 = max(first_at_most(d, sorted, m, tp.lane_max, nisl, nislc), k4);
     d.cnt[CT_WAVE_K] = kw;
     d.cnt[CT_GRP_K] = k4;
     d.cnt[CT_LANE_K] = k2;
     d.cnt[CT_LANE_N] = m - k2;
 }
-// Entry j + 1's bodies take entry j's velocities where they are the same bodies (vn loaded before
-// entry j stored them; after entry j, memory holds exactly its registers' bits).
-static __device__ __forceinline__ void forward_vel(const Row &x, const Vel2 &v, const Row &xn, Vel2 &vn) {
-    if (xn.ai == x.ai) { vn.la = v.la; vn.aa = v.aa; }
-    else if (x.bi != RP_NONE && xn.ai == x.bi) { vn.la = v.lb; vn.aa = v.ab; }
-    if (xn.bi != RP_NONE) {
-        if (xn.bi == x.ai) { vn.lb = v.la; vn.ab = v.aa; }
-        else if (xn.bi == x.bi) { vn.lb = v.lb; vn.ab = v.ab; }
-    }
+// One lane-tier entry in registers: its row, accumulators (and entry), its bodies' velocities v
+// and what HBM holds of them, m (a body the next entry also writes passes to it in registers and
+// reaches HBM after the last consecutive entry that uses it).
+struct LaneEntry { Row x; float4 acc; Vel2 v, m; };
+// In the lane layout a row's la, lb (LDS slots elsewhere) hold the next entry's bodies, so the
+// next entry's velocities load alongside its row instead of after it.
+static __device__ __forceinline__ Vel2 load_vel2_ab(const Dev &d, uint32_t ai, uint32_t bi) {
+    Vel2 v;
+    v.la = L3(d.lin[ai]); v.aa = L3(d.ang[ai]);
+    if (bi != RP_NONE) { v.lb = L3(d.lin[bi]); v.ab = L3(d.ang[bi]); } else { v.lb = splat3(0.0f); v.ab = splat3(0.0f); }
+    return v;
 }
-// One lane-tier entry in registers: its row, accumulators (and entry) and its bodies' velocities.
-struct LaneEntry { Row x; float4 acc; Vel2 v; };
-static __device__ __forceinline__ void lane_fetch(const Dev &d, const float4 *R, uint32_t j, LaneEntry &e) {
+static __device__ __forceinline__ void lane_fetch_row(const float4 *R, uint32_t j, LaneEntry &e) {
     const float4 *o = R + (size_t)j * (64u * RP_LROW);
     e.acc = o[64u * RP_SROW];
     e.x = fetch_row_at(o, 64u);
-    e.v = load_vel2(d, e.x);
+}
+// entry j (bodies ai, bi) after the entry in cur: the bodies cur also has come from cur's registers
+static __device__ __forceinline__ void lane_fetch(const Dev &d, const float4 *R, uint32_t j, LaneEntry &e, uint32_t ai, uint32_t bi,
+                                                  const LaneEntry &cur) {
+    lane_fetch_row(R, j, e);
+    if (ai != cur.x.ai && ai != cur.x.bi) { e.v.la = L3(d.lin[ai]); e.v.aa = L3(d.ang[ai]); }
+    if (bi == RP_NONE) { e.v.lb = splat3(0.0f); e.v.ab = splat3(0.0f); }
+    else if (bi != cur.x.ai && bi != cur.x.bi) { e.v.lb = L3(d.lin[bi]); e.v.ab = L3(d.ang[bi]); }
+    e.m = e.v;
+}
+// cur's bodies after its entry ran: to the next entry (bodies na, nb) in registers, or to HBM
+static __device__ __forceinline__ void lane_retire(const Dev &d, const LaneEntry &cur, LaneEntry &nxt, bool more, uint32_t na, uint32_t nb) {
+    const uint32_t a = cur.x.ai, b = cur.x.bi;
+    if (more && a == na) { nxt.v.la = cur.v.la; nxt.v.aa = cur.v.aa; nxt.m.la = cur.m.la; nxt.m.aa = cur.m.aa; }
+    else if (more && a == nb) { nxt.v.lb = cur.v.la; nxt.v.ab = cur.v.aa; nxt.m.lb = cur.m.la; nxt.m.ab = cur.m.aa; }
+    else {
+        if (!same_bits(cur.v.la, cur.m.la)) d.lin[a] = S3(cur.v.la);
+        if (!same_bits(cur.v.aa, cur.m.aa)) d.ang[a] = S3(cur.v.aa);
+    }
+    if (b == RP_NONE) return;
+    if (more && b == na) { nxt.v.la = cur.v.lb; nxt.v.aa = cur.v.ab; nxt.m.la = cur.m.lb; nxt.m.aa = cur.m.ab; }
+    else if (more && b == nb) { nxt.v.lb = cur.v.lb; nxt.v.ab = cur.v.ab; nxt.m.lb = cur.m.lb; nxt.m.ab = cur.m.ab; }
+    else {
+        if (!same_bits(cur.v.lb, cur.m.lb)) d.lin[b] = S3(cur.v.lb);
+        if (!same_bits(cur.v.ab, cur.m.ab)) d.ang[b] = S3(cur.v.ab);
+    }
 }
 // Entry j of n (in cur): fetches entry j + 1 into nxt, runs entry j, stores its accumulators and
-// changed velocities, and hands nxt the velocities entry j wrote.  Returns the residual bits.
+// hands its bodies' velocities on (lane_retire).  Returns the residual bits.
 static __device__ __forceinline__ uint32_t lane_step(const Dev &d, float4 *R, uint32_t j, uint32_t n, LaneEntry &cur, LaneEntry &nxt,
                                                      bool resid) {
     const bool more = j + 1 < n;
-    if (more) lane_fetch(d, R, j + 1, nxt);
-    const Vel2 v0 = cur.v;
+    const uint32_t na = cur.x.la, nb = cur.x.lb;  // the next entry's bodies
+    if (more) lane_fetch(d, R, j + 1, nxt, na, nb, cur);
     const uint32_t r = entry_core(cur.x, cur.v, cur.acc, (__float_as_uint(cur.acc.w) & RP_JBIT) != 0u, resid);
     R[(size_t)j * (64u * RP_LROW) + 64u * RP_SROW] = cur.acc;
-    store_vel2(d, cur.x, cur.v, v0);
-    if (more) forward_vel(cur.x, cur.v, nxt.x, nxt.v);
+    lane_retire(d, cur, nxt, more, na, nb);
     return r;
 }
 // the lane tier's waves b0, b0 + nb, ...
 static __device__ void lane_vel(const Dev &d, const TickParams &tp, const uint32_t *sorted, uint32_t b0, uint32_t nb) {
     const uint32_t m = d.cnt[CT_LANE_N], nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
+    // a lane wave's islands are the velocity solve's longest serial chains: its instructions go
+    // first on a shared SIMD
+    __builtin_amdgcn_s_setprio(2);
     const uint32_t *const lane_list = sorted + d.cnt[CT_LANE_K];
     const uint32_t nwaves = (m + 63u) / 64u, l = threadIdx.x;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
@@ -1929,13 +1959,23 @@ static __device__ void lane_vel(const Dev &d, const TickParams &tp, const uint32
         for (uint32_t j = 0; j < n; j++) {
             const uint32_t c = lst[j];
             float4 *o = R + (size_t)j * (64u * RP_LROW);
+            uint32_t na = RP_NONE, nbd = RP_NONE;  // the next entry's bodies
+            if (j + 1 < n) {
+                const uint32_t cn = lst[j + 1];
+                if (cn & RP_JBIT) { na = d.jt_a[cn & ~RP_JBIT]; nbd = d.jt_b[cn & ~RP_JBIT]; }
+                else { na = d.cs_ai[cn]; nbd = d.cs_bi[cn]; }
+            }
             if (c & RP_JBIT) {
                 const uint32_t q = c & ~RP_JBIT;
-                store_row_at(o, 64u, load_jrow(d, q));
+                Row x = load_jrow(d, q);
+                x.la = na; x.lb = nbd;
+                store_row_at(o, 64u, x);
                 o[64u * RP_SROW] = make_float4(d.jr_acc[3 * q], d.jr_acc[3 * q + 1], d.jr_acc[3 * q + 2], __uint_as_float(c));
                 continue;
             }
-            store_row_at(o, 64u, load_row(d, c));
+            Row x = load_row(d, c);
+            x.la = na; x.lb = nbd;
+            store_row_at(o, 64u, x);
             const float2 jt = d.jt[c];
             o[64u * RP_SROW] = make_float4(d.jn[c], jt.x, jt.y, __uint_as_float(c));
         }
@@ -1949,7 +1989,9 @@ static __device__ void lane_vel(const Dev &d, const TickParams &tp, const uint32
             // entry j + 1's row, accumulators and bodies' velocities load while entry j runs (two
             // buffers in turn); the velocities of bodies entry j wrote are taken from its registers
             LaneEntry A, B;
-            lane_fetch(d, R, 0, A);
+            lane_fetch_row(R, 0, A);
+            A.v = load_vel2(d, A.x);
+            A.m = A.v;
             uint32_t rbits = 0;
             for (uint32_t j = 0; j < n; j += 2) {
                 rbits = max(rbits, lane_step(d, R, j, n, A, B, resid));
@@ -2137,15 +2179,19 @@ static __device__ void group_vel(const Dev &d, const TickParams &tp, const uint3
     }
 }
 
-// Tier 0's velocity solve with the lane tier on, but for its largest islands (k_sweep_vel<0>):
-// one launch whose first nl workgroups run the lane tier, the next ng the groups of four and the
-// rest the lane pairs, concurrently.
-__global__ void __launch_bounds__(64) k_small_vel(Dev d, TickParams tp, const uint32_t *sorted, uint32_t nl, uint32_t ng) {
+// Tier 0's velocity solve with the lane tier on, but for its largest islands (k_sweep_vel<0>): the
+// lane tier, and in a launch of their own (when their bounds leave them any islands) the groups of
+// four lanes in the first ng workgroups, the lane pairs in the rest.  The launches run alongside
+// one another; keeping the lane tier's code apart keeps its loop in the instruction cache.
+__global__ void __launch_bounds__(64) k_lane_vel(Dev d, TickParams tp, const uint32_t *sorted) {
+    if (aborted(d)) return;
+    lane_vel(d, tp, sorted, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(64) k_group_vel(Dev d, TickParams tp, const uint32_t *sorted, uint32_t ng) {
     if (aborted(d)) return;
     const uint32_t b = blockIdx.x;
-    if (b < nl) lane_vel(d, tp, sorted, b, nl);
-    else if (b < nl + ng) group_vel<4>(d, tp, sorted, b - nl, ng);
-    else group_vel<2>(d, tp, sorted, b - nl - ng, gridDim.x - nl - ng);
+    if (b < ng) group_vel<4>(d, tp, sorted, b, ng);
+    else group_vel<2>(d, tp, sorted, b - ng, gridDim.x - ng);
 }
 
 // Every slot of wide level L of the islands in list[0..count) whose leading wide levels
@@ -2985,16 +3031,18 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
             hipLaunchKernelGGL(k_sweep_proj_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, kl);
         }
     };
-    // the small islands: tier 0 (one wave each) and tier 3 (mid-sized, four waves)
-    const uint32_t g3 = g0 / 4 > 256u ? g0 / 4 : 256u;
+    // the small islands: tier 0 and tier 3 (mid-sized), one wave each
+    const uint32_t g3 = g0 > 1024u ? g0 : 1024u;
     // tier 0: with the lane tier on, the lanes' and lane groups' islands on side[0] and the largest
     // islands (a wave each) ahead of tier 1 on the tick's stream
     auto vel0 = [&](hipStream_t s) {
-        if (tp.lane_max) hipLaunchKernelGGL(k_small_vel, dim3(3u * g_lane), dim3(64), 0, s, d, tp, sorted0, g_lane, g_lane);
+        if (tp.lane_max) hipLaunchKernelGGL(k_lane_vel, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
         else hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr);
     };
     auto vel0w = [&](hipStream_t s) {
-        if (tp.lane_max) hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, sorted0);
+        if (!tp.lane_max) return;
+        if (tp.quad_max > tp.lane_max) hipLaunchKernelGGL(k_group_vel, dim3(2u * g_lane), dim3(64), 0, s, d, tp, sorted0, g_lane);
+        hipLaunchKernelGGL(k_sweep_vel<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, sorted0);
     };
     auto vel3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr); };
     // profiled with side streams: tiers 0 and 3 run alongside on side[0] and side[1], timed as one

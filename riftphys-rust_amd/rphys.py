@@ -196,7 +196,7 @@ EXPORTS = [
     ("rphys_debug_device_free", U32, [P]),
     ("rphys_debug_device_copy", U32, [P, P, U64, U32]),
     ("rphys_debug_set_sweep_tiers", U32, [U32, U32, U32, U32]),
-    ("rphys_debug_set_lane_tier", U32, [U32, U32, U32]),
+    ("rphys_debug_set_lane_tier", U32, [U32, U32, U32, U32]),
     ("rphys_debug_set_initial_capacity", U32, [U32, U32, U32, U32]),
 ]
 

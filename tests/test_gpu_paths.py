@@ -1,6 +1,7 @@
 """Device paths that the default sizes leave idle, forced on small scenes and checked against the
-oracle bit for bit: every sweep tier (one island per lane, one wave, four waves, 1024 threads with
-LDS flags, 1024 threads with flags in HBM), the GPU-wide level launches of the split form, and the capacity
+oracle bit for bit: every sweep tier (one island per lane, lane groups, one wave for islands of up
+to 256 and up to 4,096 entries, 1024 threads with LDS flags, 1024 threads with flags in HBM), the
+GPU-wide level launches of the split form, and the capacity
 overflow that rolls a tick back, grows the buffers and replays it (with ball joints, whose warm
 rows the failed tick has already rewritten), including rollback slots made before the growth.
 """
@@ -19,7 +20,7 @@ def hooks():
     L = rphys.lib()
     yield L
     L.rphys_debug_set_sweep_tiers(0, 0, 0, 0)
-    L.rphys_debug_set_lane_tier(0, 0, 0)
+    L.rphys_debug_set_lane_tier(0, 0, 0, 0)
     L.rphys_debug_set_initial_capacity(0, 0, 0, 0)
 
 
@@ -59,13 +60,16 @@ def test_sweep_tiers_bit_exact(hooks, tiers, scene):
 
 
 # (largest island solved one per lane, by a lane pair, by a group of four lanes; tier 0's larger
-# ones a wave each): every tier-0 island by lanes; none (a wave per island); every split, with
-# lanes up to 4 entries, pairs up to 16, groups of four up to 64; lane pairs up to 128; groups of
-# four for every island of two or more entries
-LANES = {"all": (256, 0, 0), "off": (rphys.LANE_NONE, 0, 0), "split": (4, 16, 64), "pairs": (1, 128, 128), "groups": (1, 1, 256)}
+# ones a wave each; least number of the largest islands a wave each): every tier-0 island by
+# lanes; none (a wave per island); every split, with lanes up to 4 entries, pairs up to 16, groups
+# of four up to 64; lane pairs up to 128; groups of four for every island of two or more entries;
+# the default bounds, with the 8 largest islands a wave each
+N = rphys.LANE_NONE
+LANES = {"all": (256, 0, 0, N), "off": (N, 0, 0, 0), "split": (4, 16, 64, N), "pairs": (1, 128, 128, N), "groups": (1, 1, 256, N),
+         "wave8": (0, 0, 0, 8)}
 
 
-@pytest.mark.parametrize("lanes", ["all", "off", "split", "pairs", "groups"])
+@pytest.mark.parametrize("lanes", ["all", "off", "split", "pairs", "groups", "wave8"])
 @pytest.mark.parametrize("scene", ["mixed", "joints"])
 def test_lane_tier_bounds_bit_exact(hooks, lanes, scene):
     """Tier 0's velocity solve one island per lane, by lane pairs and groups of four, or a wave per

@@ -59,13 +59,15 @@ def stage_bytes(stage, ctr, iv, ip):
     return table.get(stage)
 
 
-# The kernels of each stage.  A big island's velocity solve and projection run as one launch
-# sequence per tick (a setup kernel, kl wide-level launches and a tail sweep per iteration, and
-# for the velocity solve a shock kernel), so their roofline unit is the stage, per tick.
+# The kernels of each stage.  The small islands' velocity solve and projection run as concurrent
+# launches (the lane tier on one stream, tier 0's largest islands and tier 3 on another), and a
+# big island's as one launch sequence per tick (a setup kernel, kl wide-level launches and a tail
+# sweep per iteration, and for the velocity solve a shock kernel), so the roofline unit of these
+# stages is the stage, per tick.
 KERNEL_OF_STAGE = {
-    "vel_solve_small": "k_sweep_vel<0> + k_sweep_vel<3>",
+    "vel_solve_small": "k_lane_vel + k_sweep_vel<0> + k_sweep_vel<3>",
     "vel_solve_large": "k_sweep_vel<1> + k_level_vel + k_sweep_vel_tail<1> + k_sweep_vel_post<1>",
-    "projection_small": "k_sweep_proj<0> + k_sweep_proj<3>",
+    "projection_small": "k_lane_proj + k_sweep_proj<0> + k_sweep_proj<3>",
     "projection_large": "k_sweep_proj<1> + k_level_proj + k_sweep_proj_tail<1>",
     "narrow_pairs": "k_narrow_pairs",
     "narrow_terrain": "k_narrow_terrain",

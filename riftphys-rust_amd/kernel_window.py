@@ -7,8 +7,12 @@ the bench reports).  Input: a kernel-trace CSV (durations) or a counter-collecti
 `rocprofv3 --pmc` (counter values, summed per kernel over the window's dispatches).  Prints a
 markdown table.
 
-    python3 riftphys-rust_amd/kernel_window.py <prefix>_kernel_trace.csv [--ticks N]
+    python3 riftphys-rust_amd/kernel_window.py <prefix>_kernel_trace.csv [--ticks N] [--group K1,K2,...]
     python3 riftphys-rust_amd/kernel_window.py <prefix>_counter_collection.csv [--ticks N]
+
+--group names kernels that run concurrently (on side streams) within one stage of a tick: the
+trace summary then also gives the stage's wall span, first start to last end of each cluster of
+overlapping dispatches of those kernels (one cluster per tick).
 """
 import argparse
 import csv
@@ -42,6 +46,7 @@ def main():
     ap.add_argument("trace")
     ap.add_argument("--marker", default="k_copy_probe")
     ap.add_argument("--ticks", type=int, default=0, help="ticks in the window (adds per-tick columns)")
+    ap.add_argument("--group", default="", help="comma-separated kernels of one concurrent stage (adds its wall span)")
     a = ap.parse_args()
     with open(a.trace) as f:
         rd = csv.DictReader(f)
@@ -90,6 +95,22 @@ def main():
         if a.ticks:
             line += " %.3f |" % (t * 1e-6 / a.ticks)
         print(line + " %.2f |" % (100.0 * t / tot))
+    if a.group:
+        names = set(a.group.split(","))
+        clusters = []
+        for st, en, n in win:
+            if n not in names:
+                continue
+            if clusters and st <= clusters[-1][1]:
+                clusters[-1][1] = max(clusters[-1][1], en)
+                clusters[-1][2] += 1
+            else:
+                clusters.append([st, en, 1])
+        spans = [(e - s_) * 1e-6 for s_, e, _ in clusters]
+        if spans:
+            print()
+            print("Stage of %s: %d clusters of overlapping dispatches, wall span %.3f ms mean (min %.3f, max %.3f), %.3f ms total" % (
+                " + ".join("`%s`" % n for n in sorted(names)), len(spans), sum(spans) / len(spans), min(spans), max(spans), sum(spans)))
 
 
 if __name__ == "__main__":

@@ -481,6 +481,9 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
     tp.iv = iv; tp.ip = w.ip;
     if (w.resid_on) { tp.resid_on = 1; tp.resid_tol = w.resid_tol; tp.iv = w.resid_max; }
     tp.ws_mode = w.ws_mode;
+    // one island per thread serialises its rows over every projection iteration: worth it for a
+    // few iterations only
+    tp.lane_pos = (tp.lane_max && tp.ip <= RP_LANE_POS_ITERS) ? 1u : 0u;
     return tp;
 }
 

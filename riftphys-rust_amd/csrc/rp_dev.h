@@ -223,6 +223,7 @@ enum {
 #define RP_PAIR_MAX 150u
 #define RP_QUAD_MAX 150u
 #define RP_WAVE_MIN 2048u  // default: at least this many of tier 0's largest islands a wave each
+#define RP_LANE_POS_ITERS 4u  // the lane tier runs the projection too up to this many position iterations
 
 // The position solve's SHAKE directions need both bodies turned by less than 0.5 rad this step:
 // 2 acos(min(|w|, 1)) < 0.5 for w of q_start * conj(q_now), which (acosf, rounded as glibc does)
@@ -251,5 +252,6 @@ struct TickParams {
     uint32_t pair_max;     // largest island of the lane pairs (at least lane_max, at most 128)
     uint32_t quad_max;     // largest island of the groups of four lanes (at least pair_max)
     uint32_t wave_min;     // at least this many of tier 0's largest islands solved a wave each
+    uint32_t lane_pos;     // the lane tier's islands also run the ballistic and projection passes one per thread
     uint32_t wide_min;     // ranks of a leading level that make it a GPU-wide level launch
 };

@@ -1954,6 +1954,13 @@ static __device__ void lane_vel(const Dev &d, const TickParams &tp, const uint32
         const uint32_t isl = q < m ? lane_list[q] : 0u;
         const uint32_t s = q < m ? d.isl_start[isl] : 0u, n = q < m ? island_len(d, isl, nisl, nislc) : 0u;
         const uint32_t *lst = d.isl_v2 + s;
+        if (!tp.lane_pos)  // the list order's slots, for the one-wave ballistic and projection sweeps
+            for (uint32_t j = 0; j < n; j++) {
+                const uint32_t a = d.ch_pa[s + j], b = d.ch_pb[s + j];
+                d.bs_pa[s + j] = a == RP_NONE ? RP_NONE : a - s;
+                d.bs_pb[s + j] = b == RP_NONE ? RP_NONE : b - s;
+                d.bs_c[s + j] = lst[j];
+            }
         if (tp.ws_mode != 1)
             for (uint32_t j = 0; j < n; j++) warm_row(d, tp, lst[j]);
         for (uint32_t j = 0; j < n; j++) {
@@ -2104,7 +2111,14 @@ static __device__ void group_vel(const Dev &d, const TickParams &tp, const uint3
             }
             return ++tok;
         };
-        for (uint32_t j = li; j < n; j += G) fl[j] = 0;
+        for (uint32_t j = li; j < n; j += G) {
+            fl[j] = 0;
+            // the list order's slots, for the one-wave ballistic and projection sweeps
+            const uint32_t a = d.ch_pa[s + j], b = d.ch_pb[s + j];
+            d.bs_pa[s + j] = a == RP_NONE ? RP_NONE : a - s;
+            d.bs_pb[s + j] = b == RP_NONE ? RP_NONE : b - s;
+            d.bs_c[s + j] = lst[j];
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // warm start in list order (solver.rs:749-760)
         if (tp.ws_mode != 1)
@@ -2426,10 +2440,12 @@ static __device__ __forceinline__ void ball_row(const Dev &d, const TickParams &
         if (d.cs_wb[c] != RP_NONE) d.ballf[b] = 0;
     }
 }
+// order (tier 0 with the lane tier's projection on): the tier's islands by decreasing size, of
+// which this launch takes the first CT_LANE_K (the rest are the lane tier's, k_lane_ball).
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_ball(Dev d, TickParams tp) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_ball(Dev d, TickParams tp, const uint32_t *order) {
     if (aborted(d)) return;
-    ISLAND_LOOP_BEGIN
+    ISLAND_LOOP_BEGIN_FROM(order ? order : d.tier_list[TIER], order ? d.cnt[CT_LANE_K] : d.cnt[CT_TIER_N + TIER])
         PhaseClock pc;
         run_sweep(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, [&](uint32_t c) { ball_row(d, tp, c); });
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_BALL);
@@ -2612,10 +2628,10 @@ static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 u
 // Position projection iterations in list order (step.rs:297-310); the shock pass follows in its
 // own launch once the shock order of the projected poses is built.
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickParams tp, uint32_t split) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickParams tp, uint32_t split, const uint32_t *order) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    ISLAND_LOOP_BEGIN
+    ISLAND_LOOP_BEGIN_FROM(order ? order : d.tier_list[TIER], order ? d.cnt[CT_LANE_K] : d.cnt[CT_TIER_N + TIER])
         PhaseClock pc;
         const Slots sb{d.bs_pa, d.bs_pb, d.bs_c};
         if (BIG<TIER> && split) continue;  // k_store_prows, then k_level_proj + k_sweep_proj_tail
@@ -2652,10 +2668,10 @@ __global__ void __launch_bounds__(256) k_level_proj(Dev d, TickParams tp, uint32
 }
 // Projection shock pass (step.rs:299-301, 447-497 with shock = true) in shock order.
 template <int TIER>
-__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d, TickParams tp, uint32_t klsh) {
+__global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d, TickParams tp, uint32_t klsh, const uint32_t *order) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    ISLAND_LOOP_BEGIN
+    ISLAND_LOOP_BEGIN_FROM(order ? order : d.tier_list[TIER], order ? d.cnt[CT_LANE_K] : d.cnt[CT_TIER_N + TIER])
         PhaseClock pc;
         shock_slots<TIER>(d, tp, I, isl);
         const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
@@ -2666,13 +2682,14 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d,
     ISLAND_LOOP_END
 }
 
-// Tier 0's ballistic pass, projection iterations and projection shock pass with the lane tier on:
-// one island per thread (order: the tier's islands), entries in list order (shock order for the
-// shock pass), the same rows as the sweeps.
+// The lane tier's ballistic pass, projection iterations and projection shock pass (when the
+// projection iterations are few, tp.lane_pos): one island per thread, entries in list order (shock
+// order for the shock pass), the same rows as the sweeps.
 #define LANE_ISLANDS_BEGIN                                                                           \
     const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);            \
-    GRID_STRIDE(q, d.cnt[CT_TIER_N]) {                                                               \
-        const uint32_t isl = order[q], s = d.isl_start[isl], n = island_len(d, isl, nisl, nislc);
+    const uint32_t *const lane_list = order + d.cnt[CT_LANE_K];                                      \
+    GRID_STRIDE(q, d.cnt[CT_LANE_N]) {                                                               \
+        const uint32_t isl = lane_list[q], s = d.isl_start[isl], n = island_len(d, isl, nisl, nislc);
 __global__ void __launch_bounds__(64) k_lane_ball(Dev d, TickParams tp, const uint32_t *order) {
     if (aborted(d)) return;
     LANE_ISLANDS_BEGIN
@@ -3021,8 +3038,8 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         hipLaunchKernelGGL(k_sweep_vshock_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
     };
     auto big_proj = [&](hipStream_t s) {
-        hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split);
-        hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split);
+        hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, split, (const uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, split, (const uint32_t *)nullptr);
         if (!split) return;
         hipLaunchKernelGGL(k_store_prows, dim3(1024), dim3(256), 0, s, d);
         for (uint32_t it = 0; it < tp.ip; it++) {
@@ -3098,58 +3115,71 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
-    // tier 0's share of each pass below: with the lane tier on, one island per thread (k_lane_*)
+    // tier 0's share of each pass below: with few projection iterations (tp.lane_pos), the lane
+    // tier's islands one per thread (k_lane_*) and the rest a wave each ahead of tier 1
+    const uint32_t *const NO = nullptr, *const lp = tp.lane_pos ? sorted0 : nullptr;
     auto ball0 = [&](hipStream_t s) {
-        if (sorted0) hipLaunchKernelGGL(k_lane_ball, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
-        else hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp);
+        if (lp) hipLaunchKernelGGL(k_lane_ball, dim3(g_lane), dim3(64), 0, s, d, tp, lp);
+        else hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, NO);
     };
-    tiers(ball0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp); },
-          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp); },
-          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp); });
+    tiers(ball0, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, NO); },
+          [&](hipStream_t s) {
+              if (lp) hipLaunchKernelGGL(k_sweep_ball<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, lp);
+              hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, NO);
+          },
+          [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, NO); });
     mark(RP_ST_INTEGRATE);
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     auto proj0 = [&](hipStream_t s) {
-        if (sorted0) hipLaunchKernelGGL(k_lane_proj, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
-        else hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u);
+        if (lp) hipLaunchKernelGGL(k_lane_proj, dim3(g_lane), dim3(64), 0, s, d, tp, lp);
+        else hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, NO);
     };
-    auto proj3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
+    auto proj0w = [&](hipStream_t s) { if (lp) hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, lp); };
+    auto proj3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u, NO); };
     if (ev) {
         mark(RP_ST_PROJ_SMALL);
-        if (ts) small_timed(proj0, proj3, 1);
-        else { proj0(st); proj3(st); }
+        if (ts) small_timed(proj0, [&](hipStream_t s) { proj0w(s); proj3(s); }, 1);
+        else { proj0(st); proj0w(st); proj3(st); }
         mark(RP_ST_PROJ_LARGE);
         if (split) big_proj(st);
-        else hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u);
+        else hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, st, d, tp, 0u, NO);
         mark(RP_ST_PROJ_HUGE);
-        if (!split) hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u);
+        if (!split) hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, st, d, tp, 0u, NO);
         if (ts) (void)hipStreamWaitEvent(st, ts->ev[1], 0);
     } else if (split) {
-        fork0(proj0, proj3, big_proj);
+        fork0(proj0, proj3, [&](hipStream_t s) { proj0w(s); big_proj(s); });
     } else {
-        tiers(proj0, proj3, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
-              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
+        tiers(proj0, proj3, [&](hipStream_t s) {
+                  proj0w(s);
+                  hipLaunchKernelGGL(k_sweep_proj<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u, NO);
+              },
+              [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u, NO); });
     }
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses
     mark(RP_ST_PROJ_SHOCK);
     auto pshock0 = [&](hipStream_t s) {
-        if (sorted0) hipLaunchKernelGGL(k_lane_pshock, dim3(g_lane), dim3(64), 0, s, d, tp, sorted0);
-        else hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u);
+        if (lp) hipLaunchKernelGGL(k_lane_pshock, dim3(g_lane), dim3(64), 0, s, d, tp, lp);
+        else hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, NO);
     };
-    auto pshock3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u); };
+    auto pshock0w = [&](hipStream_t s) { if (lp) hipLaunchKernelGGL(k_sweep_proj_shock<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, lp); };
+    auto pshock3 = [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<3>, dim3(g3), dim3(Tier<3>::threads), 0, s, d, tp, 0u, NO); };
     auto big_pshock = [&](hipStream_t s) {
-        hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
-        hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
+        hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh, NO);
+        hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh, NO);
         if (!klsh) return;
         for (uint32_t L = 0; L < klsh; L++) hipLaunchKernelGGL(k_level_pshock, dim3(GL), dim3(256), 0, s, d, tp, L);
         hipLaunchKernelGGL(k_sweep_pshock_tail<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, klsh);
         hipLaunchKernelGGL(k_sweep_pshock_tail<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, klsh);
     };
-    if (klsh) fork0(pshock0, pshock3, big_pshock);
-    else tiers(pshock0, pshock3, [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u); },
-               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u); });
+    if (klsh) fork0(pshock0, pshock3, [&](hipStream_t s) { pshock0w(s); big_pshock(s); });
+    else tiers(pshock0, pshock3, [&](hipStream_t s) {
+                   pshock0w(s);
+                   hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u, NO);
+               },
+               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u, NO); });
     mark(RP_ST_S6);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);

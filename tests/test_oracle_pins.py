@@ -236,3 +236,34 @@ def test_step_hash_byte_stream_restated(kind):
         o.step()
         if t in (0, 29, 59):
             assert o.hash() == restated_step_hash(o.h, 0x1122334455667788, 0xABCDEF | 1), "tick %d" % (t + 1)
+
+
+def _on_flat_terrain(shape, dims, y, n=17, cell=0.5):
+    s = scenes._empty(1)
+    s["shape"][0] = shape
+    s["dims"][0] = dims
+    s["pose"][0] = (4.0, y, 4.0, 0.0, 0.0, 0.0, 1.0)
+    s["hf"] = dict(nx=n, nz=n, sx=np.float32(cell), sz=np.float32(cell), ox=0.0, oz=0.0, heights=np.zeros(n * n, np.float32))
+    s["pairs"] = list(scenes.CONFIG_PAIRS)
+    return s
+
+
+@pytest.mark.parametrize("name, shape, dims, y, ny", [("sphere", O.OR_SPHERE, (0.3, 0.0, 0.0), 0.31, 1.0),
+                                                       ("capsule", O.OR_CAPSULE, (0.2, 0.3, 0.0), 0.51, 1.0),
+                                                       ("box", O.OR_BOX, (0.3, 0.3, 0.3), 0.31, -1.0)])
+def test_terrain_contact_normal_conventions(name, shape, dims, y, ny):
+    """The reference orients a sphere's (and so a capsule's) terrain contact from the triangle to
+    the shape (narrowphase.rs:683-693: normal = (center - closest) / dist), a box's from the box to
+    the terrain (narrowphase.rs:402), and the solver takes every terrain contact's normal the same
+    way (body a the shape, b the terrain).  So a resting sphere's or capsule's terrain rows push it
+    toward the terrain in the velocity solve, and the position projection lifts it back: in islands
+    where such a row must also hold up other bodies the rows have no common solution, which is the
+    residual floor of the capsule-chain worlds.  This pins the two conventions the oracle and the
+    device reproduce (a flipped sign fails it)."""
+    o = O.OracleWorld(_on_flat_terrain(shape, dims, y))
+    for _ in range(20):
+        o.step()
+    c = o.contacts()
+    assert len(c) > 0
+    assert all(x["b_col"] == 0xFFFFFFFF for x in c)
+    assert np.all(c["normal"][:, 1] == np.float32(ny)), (name, c["normal"])

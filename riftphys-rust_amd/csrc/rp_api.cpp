@@ -240,7 +240,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     S->body_off.push_back(0); S->col_off.push_back(0);
     for (auto *w : ws) nj += w->nj();
     for (auto *w : ws) {
-        if (w->nc() > RP_MAX_WORLD_COLLIDERS) { set_err("a world holds more than 16384 colliders (the broadphase's per-world LDS sort width)"); return nullptr; }
+        if (w->nc() > RP_MAX_WORLD_COLLIDERS) d.bp_radix = 1;  // the broadphase orders this scene by radix sorts
         nb += w->nb(); nc += w->nc(); nh += w->has_hf ? (uint32_t)w->heights.size() : 0u; nwarm += (uint32_t)w->wkey.size();
         S->body_off.push_back(nb); S->col_off.push_back(nc);
         uint32_t cap = std::max(w->max_bodies, w->nb());
@@ -290,7 +290,8 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         d.mlp.p = mp;
     }
     ok &= B.alloc(d.sk0, 2 * C) && B.alloc(d.sv0, 2 * C) && B.alloc(d.sk1, 2 * C) && B.alloc(d.sv1, 2 * C) && B.alloc(d.sh_hi, C);
-    d.rs_blocks = (uint32_t)((2 * C + 1023) / 1024);
+    d.rs_blocks = (uint32_t)((std::max<size_t>(2 * C, NC) + 1023) / 1024);
+    if (d.bp_radix) ok &= B.alloc(d.bp_k0, NC) && B.alloc(d.bp_v0, NC) && B.alloc(d.bp_k1, NC) && B.alloc(d.bp_v1, NC);
     ok &= B.alloc(d.rs_hist, 256 * (size_t)d.rs_blocks + 1);
     ok &= B.alloc(d.pc_la, C) && B.alloc(d.pc_lb, C);
     for (int k = 0; k < 2; k++) ok &= B.alloc(d.wk_key[k], W) && B.alloc(d.wk_la[k], W) && B.alloc(d.wk_jn[k], W) && B.alloc(d.wk_jt[k], W) && B.alloc(d.wc_off[k], nw + 1);

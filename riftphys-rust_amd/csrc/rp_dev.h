@@ -10,7 +10,7 @@
 
 #define RP_NONE 0xFFFFFFFFu
 #define RP_TERRAIN 0xFFFFFFFFu
-#define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase
+#define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase's one-workgroup sort (larger worlds: radix sorts)
 
 enum { BF_DYN = 1u, BF_ALIVE = 2u, BF_SLEEP_OK = 4u };
 enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u, FL_JOINTED = 8u, FL_JW = 16u };
@@ -49,6 +49,7 @@ enum {
     CT_ITERS_SUM_CONV,    // residual mode: sweeps x entries of the islands that reached the tolerance
     CT_SUM_ITERS_CONV, CT_SUM_ITERS_CONV_HI,
     CT_LANE_N,            // islands of the lane tier this tick
+    CT_NCOL,              // the scene's collider count (the radix broadphase order's length)
     CT_WAVE_K, CT_GRP_K, CT_LANE_K,  // with the lane tier on: where tier 0's sorted islands solved by
                                      // lane groups of four, lane pairs and single lanes begin
     CT_COUNT
@@ -87,6 +88,8 @@ struct Dev {
     float4 *amin, *amax, *smin, *smax;
     float *sweep;
     uint32_t *ccls, *bp_ord, *bp_rank;
+    uint32_t bp_radix;                         // a world exceeds RP_MAX_WORLD_COLLIDERS: order by radix sorts
+    uint32_t *bp_k0, *bp_v0, *bp_k1, *bp_v1;   // their buffers (nc each; allocated only then)
     uint32_t *w_nsap, *w_nover, *w_solve;
     float *w_maxext;
     uint32_t *out_off, *cand_off, *c_off;  // nc + 1 each (counts, then exclusive scans)

@@ -219,6 +219,43 @@ __global__ void __launch_bounds__(1024) k_bp_sort(Dev d) {
     }
 }
 
+// The same order for scenes with a world too large for k_bp_sort's LDS (more than
+// RP_MAX_WORLD_COLLIDERS colliders): every collider's class key (k_bp_sort's), then two stable
+// radix sorts over the whole scene, by key and then by world, which leave index order on equal
+// keys, as the (key, index) pairs of the bitonic sort do.
+__global__ void k_bp_keys(Dev d, uint32_t *keys, uint32_t *vals) {
+    if (aborted(d)) return;
+    GRID_STRIDE(c, d.nc) {
+        const uint32_t cls = d.ccls[c], w = d.col_world[c];
+        uint32_t k;
+        if (cls == CLS_SAP) {
+            k = sortable(d.smin[c].x);
+            if (k == 0xFFFFFFFFu || k == 0xFFFFFFFEu) k = 0xFFFFFFFDu;
+            atomicAdd(&d.w_nsap[w], 1u);
+        } else if (cls == CLS_OVER) {
+            k = 0xFFFFFFFEu;
+            atomicAdd(&d.w_nover[w], 1u);
+        } else {
+            k = 0xFFFFFFFFu;
+        }
+        keys[c] = k;
+        vals[c] = c;
+    }
+}
+__global__ void k_bp_world_keys(Dev d, const uint32_t *vals, uint32_t *keys) {
+    if (aborted(d)) return;
+    GRID_STRIDE(p, d.nc) keys[p] = d.col_world[vals[p]];
+}
+// sorted position p holds collider vals[p]; world w's colliders take positions [c0, c0 + n)
+__global__ void k_bp_ranks(Dev d, const uint32_t *vals) {
+    if (aborted(d)) return;
+    GRID_STRIDE(p, d.nc) {
+        const uint32_t c = vals[p], c0 = d.w_col_off[d.col_world[c]];
+        d.bp_ord[p] = c - c0;
+        d.bp_rank[c] = p - c0;
+    }
+}
+
 // aabb.rs:35-40 inclusive overlap (symmetric)
 static __device__ __forceinline__ bool overlaps(float4 amn, float4 amx, float4 bmn, float4 bmx) {
     return !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z);
@@ -2922,7 +2959,20 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb > d.nj ? d.nb : d.nj, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_BROAD);
-    hipLaunchKernelGGL(k_bp_sort, dim3(d.nw), dim3(1024), 0, st, d);
+    if (!d.bp_radix) {
+        hipLaunchKernelGGL(k_bp_sort, dim3(d.nw), dim3(1024), 0, st, d);
+    } else {
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_nsap, d.nw);
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_nover, d.nw);
+        hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_NCOL, d.nc);
+        hipLaunchKernelGGL(k_bp_keys, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, d.bp_k0, d.bp_v0);
+        uint32_t *ka, *va, *kb, *vb;
+        radix_sort(d, d.bp_k0, d.bp_v0, d.bp_k1, d.bp_v1, d.cnt + CT_NCOL, d.nc, 32, st, &ka, &va);
+        uint32_t *kf = ka == d.bp_k0 ? d.bp_k1 : d.bp_k0, *vf = va == d.bp_v0 ? d.bp_v1 : d.bp_v0;
+        hipLaunchKernelGGL(k_bp_world_keys, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, va, kf);
+        radix_sort(d, kf, va, ka, vf, d.cnt + CT_NCOL, d.nc, key_bits(d.nw - 1u), st, &kb, &vb);
+        hipLaunchKernelGGL(k_bp_ranks, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, vb);
+    }
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d.out_off, d.nc);
     hipLaunchKernelGGL(k_bp_enum<0>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     scan_u32(d.out_off, d.nc, d.scan_tmp, d.cnt + CT_PAIRS, st);

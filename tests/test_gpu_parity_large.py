@@ -64,3 +64,32 @@ def test_step_profiled_equals_step_n():
     stages, ctr = b.step_profiled(40)
     assert ctr["ticks"] == 40 and ctr["island_contacts_global"] > 0, ctr
     assert a.hash() == b.hash()
+
+
+def test_world_of_65536_bodies_bit_exact():
+    """A world of 65,536 mixed bodies (four times the broadphase's one-workgroup sort width):
+    ordered by the radix-sort broadphase path, bit-exact with the oracle from spawn (pairs,
+    contacts, state, hash) over its first ticks, where the dense spawn already overlaps."""
+    s = scenes.mixed_shapes(n=65536, hf_n=512, extent=200.0)
+    g, o = rphys.World(s), oracle_py.OracleWorld(s)
+    oracle_py.lib().or_set_threads(8)
+    try:
+        for t in range(1, 4):
+            ctr = compare_tick(g, o, t)
+        assert ctr["pairs"] > 0, ctr
+    finally:
+        oracle_py.lib().or_set_threads(1)
+
+
+def test_batch_with_one_world_above_the_sort_width_matches_single_worlds():
+    """A batch holding one world above the one-workgroup sort width and two small ones (the whole
+    scene then takes the radix-sort order): the same per-world hashes as the worlds stepped
+    alone, every tick."""
+    scs = [scenes.small_mixed(n=96, seed=3), scenes.mixed_shapes(n=20000, hf_n=256, extent=100.0), scenes.box_lattice()]
+    b = rphys.Batch([rphys.World(x) for x in scs])
+    singles = [rphys.World(x) for x in scs]
+    for t in range(12):
+        b.step_n(1)
+        for w in singles:
+            w.step()
+        assert b.hashes() == [w.hash() for w in singles], "tick %d" % (t + 1)

@@ -14,10 +14,13 @@
  *   cull_and_orient_contacts          step.rs:499-542
  *   build_islands (non-empty test)    islands.rs:27-92
  *   jointed-pair exclusion            step.rs:85-93, 102
- *   solve_islands_parallel            solver.rs:477-1008, 1247-1271 (e=0: W6 inert); ball-joint
+ *   solve_islands_parallel            solver.rs:477-1008, 1247-1271; ball-joint
  *                                     rows, warm start and store solver.rs:272-307, 336-424,
  *                                     644-715, 788-835, 945-952 (articulation bodies.rs:326-331)
- *   S4 integrate + gyroscopic turn    step.rs:154-280
+ *   restitution relax + W6 bounce     solver.rs:569-588 (time-of-impact rows), 995-1245 (relax,
+ *                                     velocity pass with the interior-point LCP, pseudo pass),
+ *                                     solver.rs:121-212 and bounce_lcp.rs (or_bounce.h)
+ *   S4 integrate + gyroscopic turn    step.rs:154-280 (with W6 pseudo-velocities, step.rs:271-277)
  *   S5 joints + contact projection    step.rs:284-310, 423-497; ball joints' solve_d6 translational
  *                                     rows with SHAKE directions bodies.rs:422-460, 562-573, 616-739,
  *                                     dynamics.rs:71-89, 207-210
@@ -30,17 +33,23 @@
  * (solver.rs:621-642) is proven by its own construction to give the same result (constraints
  * in one level share no written body), and the level count is computed for the stats.
  *
- * Parity pins (tests/test_oracle_*.py): BLAKE3 published digests; XorShift64 closed form;
+ * Parity pins (tests/test_oracle_*.py): the reference's golden bits G1 (six distance-joint state
+ * hashes, distance_bits_test.rs:47-63) and G2/G3 (six W6 scene step hashes, restitution_toi_test.rs
+ * :596-619 / docs/PARITY_SPEC.md:2120-2126) reproduced bit for bit; the W6 closed forms of
+ * restitution_toi_test.rs (e^2 h apexes, the zero-g 1-D collision, a resting box not lifted, side-hit
+ * momentum) and the LCP numerics against dense algebra and a brute-force LCP; BLAKE3 published digests; XorShift64 closed form;
  * broadphase vs brute-force O(N^2) overlap; narrowphase unit tests restated from
  * narrowphase.rs:735-776 and collision/tests/box_sphere_inside_test.rs; terrain query yields
  * (terrain.rs:360-394); closed-form free fall and resting heights (contact_resting_test.rs:30-45,
  * reality/src/analytic.rs free fall); invariants (manifold <= 4 points, n.(pb-pa) >= 0,
  * acc_jn >= 0, |jt| <= mu jn, snapshot/restore hash equality); ball joints (test_oracle_joints.py):
  * a rigid pendulum's large-amplitude half period and energy, anchor drift, a hanging body's joint
- * impulse m g dt, jointed pairs without contacts, the SHAKE test's acosf boundary.  glam's SSE2 operation order and
- * the platform libm are third-party (glam 0.27.0, libm 0.2.15): no reference test pins the
- * contact path's or a ball joint's bits, so bit-level agreement with the Rust build is "parity
- * unpinned" for the contact and joint scenes; the oracle is pinned by the checks above.
+ * impulse m g dt, jointed pairs without contacts, the SHAKE test's acosf boundary.  glam's SSE2
+ * operation order and the platform libm are third-party (glam 0.27.0, libm 0.2.15); G1-G3 pin them
+ * on the paths they cover (box-box contacts on a static box, distance joints, W6).  No reference
+ * golden holds a heightfield contact, a sphere or capsule contact or a ball joint, so bit-level
+ * agreement with the Rust build there is "parity unpinned": those paths are pinned by the closed
+ * forms and invariants above.
  */
 #include <math.h>
 #include <stdint.h>
@@ -49,6 +58,7 @@
 #include "oracle.h"
 #include "or_math.h"
 #include "or_narrow.h"
+#include "or_bounce.h"
 
 #define TERRAIN_CI 0xFFFFFFFFu
 static const float MARGIN = 0.05f;     /* step.rs:12 MELEE_CCD_MARGIN */
@@ -147,6 +157,8 @@ struct OrWorld {
     uint32_t *pairs; uint32_t npairs;
     Contact *contacts; uint32_t ncontacts;
     uint8_t *loaded; int loaded_valid;
+    /* W6 pseudo-velocities of the last solve (linear, angular), used by S4 when has_pseudo */
+    V3 *pl, *pa; int has_pseudo;
     OrStats stats;
 };
 
@@ -164,6 +176,7 @@ OrWorld *or_world_create(uint32_t cap) {
     w->cbody = calloc(cap + 1, 4); w->cshape = calloc(cap + 1, sizeof(ShapeD));
     w->cmat = calloc(cap + 1, 4); w->caabb = calloc(cap + 1, sizeof(Aabb));
     w->loaded = calloc(cap + 1, 1);
+    w->pl = calloc(cap + 1, sizeof(V3)); w->pa = calloc(cap + 1, sizeof(V3));
     w->gravity = v3(0.0f, -9.81f, 0.0f);
     w->rng = 0xC0FFEEull | 1ull; /* world.rs:413 XorShift64::new(0xC0FFEE) */
     w->iv = 20; w->ip = 20;      /* world.rs:425 */
@@ -174,7 +187,7 @@ void or_world_destroy(OrWorld *w) {
     if (!w) return;
     free(w->joints); free(w->wj);
     free(w->pos); free(w->rot); free(w->lin); free(w->ang); free(w->inv_mass); free(w->dyn); free(w->iil);
-    free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow);
+    free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow); free(w->pl); free(w->pa);
     free(w->cbody); free(w->cshape); free(w->cmat); free(w->caabb); free(w->loaded);
     free(w->hts); free(w->warm); free(w->pairs); free(w->contacts);
     free(w);
@@ -446,6 +459,11 @@ void or_set_body_vel(OrWorld *w, uint32_t id, const float *vel) {
     b_set_vel(w, id, v3(vel[0], vel[1], vel[2]), v3(vel[3], vel[4], vel[5]));
     w->last_lin[id] = v3(vel[0], vel[1], vel[2]);
 }
+/* world.rs:614-622 add_collider's material: the body's collider takes material id `mat`
+ * (materials.rs core ids 0..15) */
+void or_set_body_material(OrWorld *w, uint32_t id, uint32_t mat) {
+    for (uint32_t c = 0; c < w->nc; c++) if (w->cbody[c] == id) w->cmat[c] = mat;
+}
 void or_set_sleep_allowed(OrWorld *w, uint32_t id, int allowed) {
     w->sallow[id] = (uint8_t)(allowed != 0);
     if (!allowed) { w->awake[id] = 1; w->sticks[id] = 0; }
@@ -633,6 +651,9 @@ typedef struct {
     float inv_a, inv_b; M3 ii_a, ii_b; V3 ra, rb;
     float m_n, m_t1, m_t2; V3 n, t1, t2;
     float mu_s, mu_k, vs, depth_bias, rest_bias, approach;
+    /* W6 time of impact (solver.rs:569-588): the normal velocities of each side at the step's
+     * start and after this step's forces, the start gap, the pair's restitution */
+    int toi; float toi_s0, toi_e, toi_va0, toi_va1, toi_vb0, toi_vb1;
 } CC;
 
 /* world/src/world/geom.rs:30-38 */
@@ -774,10 +795,274 @@ static float dist_vel_row(OrWorld *w, const Joint *j, float dt) {
     return fabsf(jm) * isum;
 }
 
+/* ---- W6 bounce passes (solver.rs:1009-1245) ---- */
+static int loaded_at(const float *jn, const float *jt, uint32_t i) { return jn[i] > 0.0f || jt[2 * i] != 0.0f || jt[2 * i + 1] != 0.0f; }
+/* grow (solver.rs:1018-1030): involved spreads through rigid loaded contacts to a fixed point,
+ * never onto static, sleeping or jointed bodies (the fixed point is the closure, so the scan order
+ * does not matter) */
+static void w6_grow(const CC *cs, uint32_t nct, const float *jn, const float *jt, const uint8_t *jointed, uint8_t *inv) {
+    for (;;) {
+        int changed = 0;
+        for (uint32_t i = 0; i < nct; i++) {
+            const CC *c = &cs[i];
+            if (c->terrain || !loaded_at(jn, jt, i)) continue;
+            if (inv[c->ai] && !inv[c->bi] && c->inv_b > 0.0f && !jointed[c->bi]) { inv[c->bi] = 1; changed = 1; }
+            if (inv[c->bi] && !inv[c->ai] && c->inv_a > 0.0f && !jointed[c->ai]) { inv[c->ai] = 1; changed = 1; }
+        }
+        if (!changed) break;
+    }
+}
+/* free_only (solver.rs:1034-1041): jointed sides immovable, the normal effective mass again;
+ * 0 if neither side can move */
+static int w6_free_only(const CC *c, const uint8_t *jointed, CC *out) {
+    CC k = *c;
+    if (jointed[k.ai]) { k.inv_a = 0.0f; k.ii_a = m3zero(); }
+    if (k.terrain || jointed[k.bi]) { k.inv_b = 0.0f; k.ii_b = m3zero(); }
+    V3 ca = vcross(k.ra, k.n), cb = vcross(k.rb, k.n);
+    float kk = ((k.inv_a + k.inv_b) + vdot(ca, m3v(k.ii_a, ca))) + vdot(cb, m3v(k.ii_b, cb));
+    if (!(kk > 1.0e-9f)) return 0;
+    k.m_n = 1.0f / kk;
+    *out = k;
+    return 1;
+}
+static void w6_sides(const CC *c, BSide *sd) {
+    sd[0] = or_bside(c->ai, -1.0, c->ra, c->inv_a, c->ii_a, c->n);
+    sd[1] = or_bside(c->terrain ? BN_NONE : c->bi, 1.0, c->rb, c->inv_b, c->ii_b, c->n);
+}
+typedef struct { CC c; float target; int32_t cidx; } W6Row;
+/* rows of each component, ascending, components in first-row order */
+static uint32_t w6_comps(const BSide (*sides)[2], uint32_t n, uint32_t **order, uint32_t **start) {
+    uint32_t *cof = malloc(4 * (n + 1));
+    uint32_t nc = bn_components(sides, n, cof);
+    *start = calloc(nc + 2, 4);
+    *order = malloc(4 * (n + 1));
+    for (uint32_t i = 0; i < n; i++) (*start)[cof[i] + 1]++;
+    for (uint32_t c = 0; c < nc; c++) (*start)[c + 1] += (*start)[c];
+    uint32_t *fill = calloc(nc + 1, 4);
+    for (uint32_t i = 0; i < n; i++) (*order)[(*start)[cof[i]] + fill[cof[i]]++] = i;
+    free(fill); free(cof);
+    return nc;
+}
+static V3 pseudo_rel(const OrWorld *w, const CC *c) {
+    V3 la = w->pl[c->ai], wa = w->pa[c->ai];
+    V3 lb = c->terrain ? v3s(0) : w->pl[c->bi], wb = c->terrain ? v3s(0) : w->pa[c->bi];
+    return v3(vdot(vsub(vadd(lb, vcross(wb, c->rb)), vadd(la, vcross(wa, c->ra))), c->n), 0.0f, 0.0f);
+}
+static void w6_passes(OrWorld *w, const CC *cs, uint32_t nct, float *jn, const float *jt, const uint8_t *jointed, float dt) {
+    uint32_t nb = w->n;
+    uint8_t *toi_set = calloc(nct + 1, 1);
+    int any = 0;
+    for (uint32_t i = 0; i < nct; i++) {
+        toi_set[i] = cs[i].toi && cs[i].rest_bias > 0.0f && jn[i] > 0.0f;
+        any |= toi_set[i];
+    }
+    W6Row *prow_src = malloc(sizeof(W6Row) * (nct + 1));  /* (constraint, pseudo u) */
+    uint32_t npseudo = 0;
+    if (any) {
+        /* loaded contacts outside the bounce, per body: who each one is with (solver.rs:1051-1057);
+         * only "is there a partner other than p" is asked, so a body keeps its first partner and
+         * whether a second, different one exists */
+        uint32_t *p0 = malloc(4 * (nb + 1));
+        uint8_t *pc = calloc(nb + 1, 1), *pm = calloc(nb + 1, 1);
+#define W6_REST(id, p) do { uint32_t _i = (id), _p = (p); if (!pc[_i]) { pc[_i] = 1; p0[_i] = _p; } else if (p0[_i] != _p) pm[_i] = 1; } while (0)
+        for (uint32_t i = 0; i < nct; i++) {
+            if (toi_set[i] || !loaded_at(jn, jt, i)) continue;
+            W6_REST(cs[i].ai, cs[i].terrain ? TERRAIN_CI : cs[i].bi);
+            if (!cs[i].terrain) W6_REST(cs[i].bi, cs[i].ai);
+        }
+#undef W6_REST
+        W6Row *rows = malloc(sizeof(W6Row) * (nct + 1));
+        uint32_t nr = 0;
+        for (uint32_t i = 0; i < nct; i++) {
+            if (!toi_set[i]) continue;
+            const CC *c = &cs[i];
+            /* a side resting on something other than its partner here is struck from its start
+             * velocity (solver.rs:1063-1068) */
+            uint32_t oa = c->terrain ? TERRAIN_CI : c->bi;
+            int sup_a = pc[c->ai] && (pm[c->ai] || p0[c->ai] != oa);
+            float a0 = c->toi_va0, a1 = sup_a ? c->toi_va0 : c->toi_va1;
+            float b0 = 0.0f, b1 = 0.0f;
+            if (!c->terrain) {
+                int sup_b = pc[c->bi] && (pm[c->bi] || p0[c->bi] != c->ai);
+                b0 = c->toi_vb0; b1 = sup_b ? c->toi_vb0 : c->toi_vb1;
+            }
+            float v0 = b0 - a0;
+            float acc = ((b1 - a1) - v0) / dt;
+            float u; int hp;
+            float v_end = bn_toi_bounce(c->toi_s0, v0, acc, c->toi_e, dt, &u, &hp);
+            rows[nr].c = *c; rows[nr].target = v_end; rows[nr].cidx = (int32_t)i; nr++;
+            if (hp) { prow_src[npseudo].c = *c; prow_src[npseudo].target = u; prow_src[npseudo].cidx = (int32_t)i; npseudo++; }
+        }
+        free(p0); free(pc); free(pm);
+        uint32_t ntoi = nr;
+        uint8_t *inv = calloc(nb + 1, 1);
+        for (uint32_t r = 0; r < ntoi; r++) { inv[rows[r].c.ai] = 1; if (!rows[r].c.terrain) inv[rows[r].c.bi] = 1; }
+        w6_grow(cs, nct, jn, jt, jointed, inv);
+        /* the bounced bodies' other loaded contacts: push-only support rows with fresh
+         * accumulators and target min(depth_bias, 0) (solver.rs:1081-1092) */
+        for (uint32_t i = 0; i < nct; i++) {
+            if (toi_set[i] || jn[i] <= 0.0f) continue;
+            const CC *c = &cs[i];
+            if (!inv[c->ai] && (c->terrain || !inv[c->bi])) continue;
+            CC k;
+            if (!w6_free_only(c, jointed, &k)) continue;
+            rows[nr].c = k; rows[nr].target = fminf(c->depth_bias, 0.0f); rows[nr].cidx = -1; nr++;
+        }
+        free(inv);
+        float *sup = calloc(nr + 1, 4);
+        BSide (*sides)[2] = malloc(sizeof(BSide) * 2 * (nr + 1));
+        for (uint32_t r = 0; r < nr; r++) w6_sides(&rows[r].c, sides[r]);
+        uint32_t *ord, *cst;
+        uint32_t ncomp = w6_comps((const BSide(*)[2])sides, nr, &ord, &cst);
+        w->stats.toi_rows = ntoi;
+        w->stats.support_rows = nr - ntoi;
+        w->stats.bounce_components = ncomp;
+        for (uint32_t cc = 0; cc < ncomp; cc++) {
+            const uint32_t *comp = ord + cst[cc];
+            uint32_t n = cst[cc + 1] - cst[cc];
+            int sweep = n > BN_DIRECT_MAX;
+            if (!sweep) {
+                BSide (*cs2)[2] = malloc(sizeof(BSide) * 2 * (n + 1));
+                double *b = malloc(sizeof(double) * (n + 1)), *acc0 = malloc(sizeof(double) * (n + 1)), *delta = malloc(sizeof(double) * (n + 1));
+                for (uint32_t q = 0; q < n; q++) {
+                    const W6Row *r = &rows[comp[q]];
+                    cs2[q][0] = sides[comp[q]][0]; cs2[q][1] = sides[comp[q]][1];
+                    b[q] = (double)(r->target - vdot(rel_vel(w, &r->c), r->c.n));
+                    acc0[q] = (double)(r->cidx >= 0 ? jn[r->cidx] : sup[comp[q]]);
+                }
+                BCsr K = bn_contact_csr((const BSide(*)[2])cs2, n);
+                uint32_t solves = 0;
+                int conv = bn_bounce_qp(&K, b, acc0, BN_SOLVES, delta, &solves);
+                if (solves > w->stats.bounce_solves) w->stats.bounce_solves = solves;
+                sweep = !conv;
+                for (uint32_t q = 0; q < n; q++) {
+                    const W6Row *r = &rows[comp[q]];
+                    float *slot = r->cidx >= 0 ? &jn[r->cidx] : &sup[comp[q]];
+                    float old = *slot;
+                    float nv = fmaxf((float)((double)old + delta[q]), 0.0f);
+                    *slot = nv;
+                    apply_cimp(w, &r->c, vmul(r->c.n, nv - old));
+                }
+                bn_csr_free(&K);
+                free(cs2); free(b); free(acc0); free(delta);
+            }
+            if (!sweep) continue;
+            w->stats.bounce_fallbacks++;
+            /* above the direct limit, or not converged: Gauss-Seidel from where it got to,
+             * alternating direction (solver.rs:1135-1154) */
+            for (uint32_t it = 0; it < BN_SWEEPS; it++) {
+                float change = 0.0f;
+                for (uint32_t m = 0; m < n; m++) {
+                    uint32_t ri = comp[it % 2 == 0 ? m : n - 1 - m];
+                    const W6Row *r = &rows[ri];
+                    float v = vdot(rel_vel(w, &r->c), r->c.n);
+                    float *slot = r->cidx >= 0 ? &jn[r->cidx] : &sup[ri];
+                    float old = *slot;
+                    float nv = fmaxf(old + (r->target - v) * r->c.m_n, 0.0f);
+                    *slot = nv;
+                    apply_cimp(w, &r->c, vmul(r->c.n, nv - old));
+                    change = fmaxf(change, fabsf(nv - old) / r->c.m_n / (1.0f + fabsf(r->target)));
+                }
+                if (change < BN_TOL) break;
+            }
+        }
+        free(ord); free(cst); free(sides); free(sup); free(rows);
+    }
+    /* the bounce's position: pseudo-velocities along the bounced contacts' normals, the involved
+     * bodies' other loaded contacts holding their gaps both ways (solver.rs:1163-1243) */
+    if (npseudo > 0) {
+        uint8_t *in_set = calloc(nct + 1, 1), *inv = calloc(nb + 1, 1);
+        for (uint32_t r = 0; r < npseudo; r++) {
+            const CC *c = &prow_src[r].c;
+            in_set[prow_src[r].cidx] = 1;
+            inv[c->ai] = 1;
+            if (!c->terrain) inv[c->bi] = 1;
+        }
+        w6_grow(cs, nct, jn, jt, jointed, inv);
+        w->stats.pseudo_rows = npseudo;
+        W6Row *rows = malloc(sizeof(W6Row) * (nct + 1));
+        uint32_t nr = 0;
+        for (uint32_t r = 0; r < npseudo; r++) {
+            CC k;
+            if (w6_free_only(&cs[prow_src[r].cidx], jointed, &k)) { rows[nr].c = k; rows[nr].target = prow_src[r].target; rows[nr].cidx = -1; nr++; }
+        }
+        for (uint32_t i = 0; i < nct; i++) {
+            if (in_set[i] || !loaded_at(jn, jt, i)) continue;
+            const CC *c = &cs[i];
+            if (!inv[c->ai] && (c->terrain || !inv[c->bi])) continue;
+            CC k;
+            if (w6_free_only(c, jointed, &k)) { rows[nr].c = k; rows[nr].target = 0.0f; rows[nr].cidx = -1; nr++; }
+        }
+        for (uint32_t b = 0; b < nb; b++) { w->pl[b] = v3s(0); w->pa[b] = v3s(0); }
+        BSide (*sides)[2] = malloc(sizeof(BSide) * 2 * (nr + 1));
+        for (uint32_t r = 0; r < nr; r++) w6_sides(&rows[r].c, sides[r]);
+        D3 *accl = calloc(nb + 1, sizeof(D3)), *acca = calloc(nb + 1, sizeof(D3));
+        uint8_t *direct = calloc(nb + 1, 1);
+        uint32_t *ord, *cst;
+        uint32_t ncomp = w6_comps((const BSide(*)[2])sides, nr, &ord, &cst);
+        for (uint32_t cc = 0; cc < ncomp; cc++) {
+            const uint32_t *comp = ord + cst[cc];
+            uint32_t n = cst[cc + 1] - cst[cc];
+            if (n <= BN_DIRECT_MAX) {
+                BSide (*cs2)[2] = malloc(sizeof(BSide) * 2 * (n + 1));
+                double *u = malloc(sizeof(double) * (n + 1)), *x = malloc(sizeof(double) * (n + 1));
+                for (uint32_t q = 0; q < n; q++) {
+                    cs2[q][0] = sides[comp[q]][0]; cs2[q][1] = sides[comp[q]][1];
+                    u[q] = (double)rows[comp[q]].target;
+                }
+                BCsr K = bn_contact_csr((const BSide(*)[2])cs2, n);
+                bn_pseudo_solve(&K, u, x);
+                for (uint32_t q = 0; q < n; q++)
+                    for (int t = 0; t < 2; t++) {
+                        const BSide *sd = &cs2[q][t];
+                        if (sd->body == BN_NONE) continue;
+                        accl[sd->body] = d3add(accl[sd->body], d3mul(sd->ml, x[q]));
+                        acca[sd->body] = d3add(acca[sd->body], d3mul(sd->ma, x[q]));
+                        direct[sd->body] = 1;
+                    }
+                bn_csr_free(&K);
+                free(cs2); free(u); free(x);
+                continue;
+            }
+            w->stats.bounce_fallbacks++;
+            for (uint32_t it = 0; it < BN_SWEEPS; it++) {
+                float change = 0.0f;
+                for (uint32_t m = 0; m < n; m++) {
+                    const W6Row *r = &rows[comp[it % 2 == 0 ? m : n - 1 - m]];
+                    const CC *c = &r->c;
+                    float d = r->target - pseudo_rel(w, c).x;
+                    change = fmaxf(change, fabsf(d) / (1.0f + fabsf(r->target)));
+                    V3 j = vmul(c->n, d * c->m_n);
+                    if (c->inv_a > 0.0f) {
+                        w->pl[c->ai] = vsub(w->pl[c->ai], vmul(j, c->inv_a));
+                        w->pa[c->ai] = vsub(w->pa[c->ai], m3v(c->ii_a, vcross(c->ra, j)));
+                    }
+                    if (c->inv_b > 0.0f) {
+                        w->pl[c->bi] = vadd(w->pl[c->bi], vmul(j, c->inv_b));
+                        w->pa[c->bi] = vadd(w->pa[c->bi], m3v(c->ii_b, vcross(c->rb, j)));
+                    }
+                }
+                if (change < BN_TOL) break;
+            }
+        }
+        for (uint32_t b = 0; b < nb; b++)
+            if (direct[b]) {
+                w->pl[b] = v3((float)accl[b].x, (float)accl[b].y, (float)accl[b].z);
+                w->pa[b] = v3((float)acca[b].x, (float)acca[b].y, (float)acca[b].z);
+            }
+        w->has_pseudo = 1;
+        free(ord); free(cst); free(sides); free(accl); free(acca); free(direct); free(rows); free(in_set); free(inv);
+    }
+    free(prow_src); free(toi_set);
+}
+
 static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, const V3 *pre_ang, float dt) {
     const float beta = 0.05f, slop = 0.005f;
     uint32_t nb = w->n;
     CC *cs = malloc(sizeof(CC) * (nct + 1));
+    /* bodies in any joint (solver.rs:483-491): W6 leaves their contacts as they were */
+    uint8_t *jointed = calloc(nb + 1, 1);
+    for (uint32_t q = 0; q < w->nj; q++) { jointed[w->joints[q].a] = 1; jointed[w->joints[q].b] = 1; }
     for (uint32_t i = 0; i < nct; i++) {
         const Contact *c = &cts[i];
         CC *k = &cs[i];
@@ -805,7 +1090,23 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         float vrn = 0.5f * (v_pre + vdot(rel_vel(w, k), k->n));
         k->approach = vrn;
         k->rest_bias = 0.0f;
-        if (vrn < -1.0f) k->rest_bias = -pp.rest * vrn; /* W6 (solver.rs:1009-1245) not built */
+        k->toi = 0; k->toi_s0 = k->toi_e = 0.0f; k->toi_va0 = k->toi_va1 = k->toi_vb0 = k->toi_vb1 = 0.0f;
+        if (vrn < -1.0f) {
+            k->rest_bias = -pp.rest * vrn;
+            /* W6: free bodies bounce at their time of impact (solver.rs:572-587); a side's start
+             * velocity is last step's end linear velocity with the pre-step angular one */
+            if (!jointed[k->ai] && (k->terrain || !jointed[k->bi])) {
+                k->toi = 1;
+                k->toi_s0 = -pen;
+                k->toi_e = pp.rest;
+                k->toi_va0 = vdot(vadd(w->last_lin[k->ai], vcross(pre_ang[k->ai], k->ra)), k->n);
+                k->toi_va1 = vdot(vadd(w->lin[k->ai], vcross(w->ang[k->ai], k->ra)), k->n);
+                if (!k->terrain) {
+                    k->toi_vb0 = vdot(vadd(w->last_lin[k->bi], vcross(pre_ang[k->bi], k->rb)), k->n);
+                    k->toi_vb1 = vdot(vadd(w->lin[k->bi], vcross(w->ang[k->bi], k->rb)), k->n);
+                }
+            }
+        }
     }
     /* level schedule, for the stats (solver.rs:621-642) */
     uint32_t *blev = calloc(nb + 1, 4);
@@ -1152,7 +1453,19 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         apply_cimp(w, &k, vmul(k.n, nwv - old));
     }
     free(ord);
-    /* restitution relax (solver.rs:1001-1008): toi contacts go to W6 (not built); at e=0 nothing runs */
+    /* restitution relax (solver.rs:995-1008): contacts that carried load and bounce, other than
+     * the time-of-impact ones, one sweep in index order */
+    for (uint32_t i = 0; i < nct; i++) {
+        const CC *k = &cs[i];
+        if (k->toi || k->rest_bias <= 0.0f || jn[i] <= 0.0f) continue;
+        float v = vdot(rel_vel(w, k), k->n);
+        float old = jn[i];
+        float nwv = fmaxf(old + (k->rest_bias - v) * k->m_n, 0.0f);
+        jn[i] = nwv;
+        apply_cimp(w, k, vmul(k->n, nwv - old));
+    }
+    w6_passes(w, cs, nct, jn, jt, jointed, dt);
+    free(jointed);
     /* loaded flags + impulses (solver.rs:1247-1270) */
     memset(w->loaded, 0, w->cap + 1);
     w->loaded_valid = 1;
@@ -1353,6 +1666,7 @@ void or_step(OrWorld *w, float dt) {
     w->tick += 1;
     memset(&w->stats, 0, sizeof(w->stats));
     w->loaded_valid = 0;
+    w->has_pseudo = 0;
     /* S1 (step.rs:53-68) */
     V3 *pre_lin = malloc(sizeof(V3) * (count + 1)), *pre_ang = malloc(sizeof(V3) * (count + 1));
     for (uint32_t i = 0; i < count; i++) { pre_lin[i] = w->lin[i]; pre_ang[i] = w->ang[i]; }
@@ -1516,6 +1830,12 @@ void or_step(OrWorld *w, float dt) {
                 va = wv;
                 b_set_vel(w, i, vl, va);
             }
+        }
+        /* a bounced free body's pseudo-velocity moves it and never enters its velocity
+         * (step.rs:271-277) */
+        if (w->has_pseudo) {
+            p = vadd(p, vmul(w->pl[i], dt));
+            if (vlen2(w->pa[i]) > 0.0f) q = qnormalize(qmul(from_axis_angle(vnormalize(w->pa[i]), vlen(w->pa[i]) * dt), q));
         }
         b_set_pose(w, i, p, q);
         ppos[i] = w->pos[i]; prot[i] = w->rot[i];
@@ -1862,4 +2182,44 @@ void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *
         e->jt = v3(wjt[3 * k], wjt[3 * k + 1], wjt[3 * k + 2]);
     }
     w->nwarm = nwarm;
+}
+
+/* The W6 numerics alone, for the oracle's own pin tests: a symmetric CSR matrix (both triangles,
+ * columns ascending, diagonal present). */
+static BCsr bn_wrap(uint32_t n, const uint32_t *ptr, const uint32_t *idx, const double *val) {
+    BCsr k;
+    k.n = n;
+    k.ptr = malloc(4 * (n + 1)); memcpy(k.ptr, ptr, 4 * (n + 1));
+    k.idx = malloc(4 * (ptr[n] + 1)); memcpy(k.idx, idx, 4 * ptr[n]);
+    k.val = malloc(8 * (ptr[n] + 1)); memcpy(k.val, val, 8 * ptr[n]);
+    return k;
+}
+uint32_t or_bn_min_degree(uint32_t n, const uint32_t *ptr, const uint32_t *idx, uint32_t *order) {
+    if (n > BN_DIRECT_MAX) return 0xFFFFFFFFu;
+    double *val = calloc(ptr[n] + 1, 8);
+    BCsr k = bn_wrap(n, ptr, idx, val);
+    BSym s;
+    bn_min_degree(&k, &s);
+    memcpy(order, s.order, 4 * n);
+    uint32_t l = s.cp[n];
+    bn_sym_free(&s); bn_csr_free(&k); free(val);
+    return l;
+}
+int or_bn_pseudo_solve(uint32_t n, const uint32_t *ptr, const uint32_t *idx, const double *val, const double *u, double *x) {
+    if (n > BN_DIRECT_MAX) return -1;
+    BCsr k = bn_wrap(n, ptr, idx, val);
+    bn_pseudo_solve(&k, u, x);
+    bn_csr_free(&k);
+    return 0;
+}
+int or_bn_bounce_qp(uint32_t n, const uint32_t *ptr, const uint32_t *idx, const double *val, const double *b, const double *acc0,
+                    double *x, uint32_t *solves) {
+    if (n > BN_DIRECT_MAX) return -1;
+    BCsr k = bn_wrap(n, ptr, idx, val);
+    int c = bn_bounce_qp(&k, b, acc0, BN_SOLVES, x, solves);
+    bn_csr_free(&k);
+    return c;
+}
+float or_toi_bounce(float s0, float v0, float a, float e, float dt, float *pseudo, int *has_pseudo) {
+    return bn_toi_bounce(s0, v0, a, e, dt, pseudo, has_pseudo);
 }

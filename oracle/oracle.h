@@ -57,6 +57,10 @@ typedef struct {
     uint32_t iters_used;   /* velocity iterations run by the island that ran the most */
     float residual;        /* largest final residual over the islands (m/s), residual mode only */
     uint32_t iters_sum;    /* sum over islands of (iterations run x the island's contacts) */
+    /* W6 bounce passes of the last step (BounceStats, world.rs): time-of-impact rows, support rows,
+     * pseudo rows, velocity-pass components, components swept instead of solved directly, and the
+     * most interior-point solves any component took */
+    uint32_t toi_rows, support_rows, pseudo_rows, bounce_components, bounce_fallbacks, bounce_solves;
 } OrStats;
 
 OrWorld *or_world_create(uint32_t max_bodies);
@@ -83,6 +87,8 @@ void or_set_heightfield(OrWorld *w, uint32_t nx, uint32_t nz, float sx, float sz
 void or_set_body_pose(OrWorld *w, uint32_t id, const float *pose);
 void or_set_body_vel(OrWorld *w, uint32_t id, const float *vel);
 void or_set_sleep_allowed(OrWorld *w, uint32_t id, int allowed);
+/* The material id (0..15, materials.rs core table) of body id's collider. */
+void or_set_body_material(OrWorld *w, uint32_t id, uint32_t mat);
 void or_step(OrWorld *w, float dt);
 /* Threads of the step's parallel regions (default 1): where the reference uses rayon (narrowphase
  * pairs, terrain colliders, PGS level batches of at least 64 rows), bit-identical for any count. */
@@ -136,6 +142,15 @@ uint32_t or_broadphase(const float *aabbs6, uint32_t n, uint32_t *out_ij, uint32
 void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *lin, const float *ang, const float *last_lin,
                    const uint32_t *awake, const uint32_t *sticks, uint64_t tick, const uint64_t *wkeys, const float *wla,
                    const float *wjn, const float *wjt, uint32_t nwarm);
+
+/* W6 numerics (bounce_lcp.rs), for the pin tests: K symmetric CSR, n <= 250.  min_degree writes
+ * the elimination order and returns the factor's entries below the diagonal; pseudo_solve solves
+ * (K + CFM diag K) x = u; bounce_qp solves the push-only LCP and returns 1 if converged. */
+uint32_t or_bn_min_degree(uint32_t n, const uint32_t *ptr, const uint32_t *idx, uint32_t *order);
+int or_bn_pseudo_solve(uint32_t n, const uint32_t *ptr, const uint32_t *idx, const double *val, const double *u, double *x);
+int or_bn_bounce_qp(uint32_t n, const uint32_t *ptr, const uint32_t *idx, const double *val, const double *b, const double *acc0,
+                    double *x, uint32_t *solves);
+float or_toi_bounce(float s0, float v0, float a, float e, float dt, float *pseudo, int *has_pseudo);
 
 #ifdef __cplusplus
 }

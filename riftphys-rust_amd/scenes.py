@@ -277,3 +277,83 @@ def ragdolls_small(seed=11, n_towers=4, tower_h=5, n_ragdolls=6, chain=15, hf_n=
     """A few towers and capsule chains over fBm terrain: the quick joint parity case."""
     return towers_ragdolls(n_towers=n_towers, tower_h=tower_h, n_loose=2, n_ragdolls=n_ragdolls, chain=chain, seed=seed,
                            hf_n=hf_n, cell=0.5, extent=14.0, margin=4.0, drop=3.0)
+
+
+# ---- bounce scenes: the restitution cases of crates/riftphys-world/tests/restitution_toi_test.rs
+# (static box floor with its top at y = 0, boxes and balls of given mass and material id, sleep
+# disabled on the moving bodies, gravity -9.81 or none).  Material ids are the reference's core
+# table (materials.rs:11-12).
+MAT_ICE, MAT_RUBBER, MAT_CONCRETE, MAT_ALUMINUM = 1, 2, 6, 13
+
+
+def bounce_body(shape, dims, pos, mass, mat, rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
+    """One moving body of the restitution tests: a box (dims = half extents) of density
+    mass / (8 hx hy hz), or a sphere (dims[0] = r) of density mass / (4/3 pi r^3), each in f32 in
+    the tests' order (restitution_toi_test.rs:28-44)."""
+    f = np.float32
+    d = np.array(dims, f)
+    if shape == 1:
+        dens = f(mass) / (((f(8.0) * d[0]) * d[1]) * d[2])
+    else:
+        r = d[0]
+        dens = f(mass) / ((((f(4.0) / f(3.0)) * f(math.pi)) * r) * r * r)
+    return dict(shape=shape, dims=d, pose=np.array([*pos, *rot], f), vel=np.array(vel, f), dynamic=1, density=f(dens),
+                material=mat, sleep_allowed=0)
+
+
+def bounce_floor(mat):
+    """A static 100 x 1 x 100 m box whose top is at y = 0 (restitution_toi_test.rs:15-19)."""
+    return dict(shape=1, dims=np.array([50.0, 0.5, 50.0], np.float32), pose=np.array([0, -0.5, 0, 0, 0, 0, 1], np.float32),
+                vel=np.zeros(6, np.float32), dynamic=0, density=np.float32(0.0), material=mat, sleep_allowed=1)
+
+
+def bounce_scene_of(bodies, gravity=True, hz=60.0, name="bounce"):
+    n = len(bodies)
+    s = dict(shape=np.array([b["shape"] for b in bodies], np.int32), dims=np.zeros((n, 3), np.float32),
+             pose=np.array([b["pose"] for b in bodies], np.float32), vel=np.array([b["vel"] for b in bodies], np.float32),
+             dynamic=np.array([b["dynamic"] for b in bodies], np.int32),
+             density=np.array([b["density"] for b in bodies], np.float32),
+             material=np.array([b["material"] for b in bodies], np.int32),
+             sleep_allowed=np.array([b["sleep_allowed"] for b in bodies], np.int32))
+    for i, b in enumerate(bodies):
+        s["dims"][i, :len(b["dims"])] = b["dims"]
+    s["gravity"] = (0.0, -9.81, 0.0) if gravity else (0.0, 0.0, 0.0)
+    s["hf"] = None
+    s["iters"] = (20, 20)  # World::with_capacity's solver iterations (world.rs:425)
+    s["dt"] = np.float32(1.0) / np.float32(hz)
+    s["name"] = name
+    return s
+
+
+BOUNCE_HALF = (0.2, 0.1, 0.2)
+
+
+def bounce_pile(hz=60.0):
+    """The 20-box pyramid (layers of 6, 5, 4, 3, 2 aluminium boxes on concrete) of the W6 pile
+    golden, before its drop; the box dropped after 1 s is bounce_pile_drop()
+    (restitution_toi_test.rs:532-553)."""
+    f = np.float32
+    bodies = [bounce_floor(MAT_CONCRETE)]
+    for layer, n in enumerate([6, 5, 4, 3, 2]):
+        for i in range(n):
+            x = (f(i) - (f(n) - f(1.0)) * f(0.5)) * f(0.4)
+            y = f(0.1) + f(0.2) * f(layer)
+            bodies.append(bounce_body(1, BOUNCE_HALF, (x, y, 0.0), 2.0, MAT_ALUMINUM))
+    return bounce_scene_of(bodies, hz=hz, name="bounce_pile")
+
+
+def bounce_pile_drop():
+    return bounce_body(1, BOUNCE_HALF, (-0.2, 2.0, 0.0), 2.0, MAT_ALUMINUM)
+
+
+def bounce_stack(hz=60.0):
+    """A two-box aluminium stack with a third box 1.2 m above (restitution_toi_test.rs:548-550)."""
+    bodies = [bounce_floor(MAT_CONCRETE)] + [bounce_body(1, BOUNCE_HALF, (0.0, y, 0.0), 2.0, MAT_ALUMINUM) for y in (0.1, 0.3, 1.5)]
+    return bounce_scene_of(bodies, hz=hz, name="bounce_stack")
+
+
+def bounce_drone(hz=60.0):
+    """The drone-like aluminium box (0.4 x 0.1 x 0.4 m, 1.5 kg) flat from 2 m over concrete
+    (restitution_toi_test.rs:612-614)."""
+    bodies = [bounce_floor(MAT_CONCRETE), bounce_body(1, (0.2, 0.05, 0.2), (0.0, 2.0, 0.0), 1.5, MAT_ALUMINUM)]
+    return bounce_scene_of(bodies, hz=hz, name="bounce_drone")

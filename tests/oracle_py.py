@@ -13,7 +13,9 @@ OR_SPHERE, OR_BOX, OR_CAPSULE = 0, 1, 2
 class OrStats(C.Structure):
     _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("islands", C.c_uint32),
                 ("terrain_candidates", C.c_uint32), ("levels", C.c_uint32), ("warm_matched", C.c_uint32),
-                ("iters_used", C.c_uint32), ("residual", C.c_float), ("iters_sum", C.c_uint32)]
+                ("iters_used", C.c_uint32), ("residual", C.c_float), ("iters_sum", C.c_uint32),
+                ("toi_rows", C.c_uint32), ("support_rows", C.c_uint32), ("pseudo_rows", C.c_uint32),
+                ("bounce_components", C.c_uint32), ("bounce_fallbacks", C.c_uint32), ("bounce_solves", C.c_uint32)]
 
 
 class OrContact(C.Structure):
@@ -32,6 +34,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         FP = C.POINTER(C.c_float)
+        P32_ = C.POINTER(C.c_uint32)
         L.or_world_create.restype = P
         L.or_world_create.argtypes = [C.c_uint32]
         L.or_world_destroy.argtypes = [P]
@@ -60,6 +63,16 @@ def lib():
         L.or_set_body_pose.argtypes = [P, C.c_uint32, FP]
         L.or_set_body_vel.argtypes = [P, C.c_uint32, FP]
         L.or_set_sleep_allowed.argtypes = [P, C.c_uint32, C.c_int]
+        L.or_set_body_material.argtypes = [P, C.c_uint32, C.c_uint32]
+        DP = C.POINTER(C.c_double)
+        L.or_bn_min_degree.restype = C.c_uint32
+        L.or_bn_min_degree.argtypes = [C.c_uint32, P32_, P32_, P32_]
+        L.or_bn_pseudo_solve.restype = C.c_int
+        L.or_bn_pseudo_solve.argtypes = [C.c_uint32, P32_, P32_, DP, DP, DP]
+        L.or_bn_bounce_qp.restype = C.c_int
+        L.or_bn_bounce_qp.argtypes = [C.c_uint32, P32_, P32_, DP, DP, DP, DP, P32_]
+        L.or_toi_bounce.restype = C.c_float
+        L.or_toi_bounce.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, FP, C.POINTER(C.c_int)]
         L.or_step.argtypes = [P, C.c_float]
         L.or_set_threads.argtypes = [C.c_int]
         P8, P16, P32 = C.POINTER(C.c_uint8), C.POINTER(C.c_uint16), C.POINTER(C.c_uint32)
@@ -133,11 +146,17 @@ class OracleWorld:
         if s.get("residual"):
             tol, mx = s["residual"]
             L.or_set_residual_mode(self.h, 1, tol, mx)
+        if s.get("gravity") is not None:
+            L.or_set_gravity(self.h, *[float(x) for x in s["gravity"]])
         for i in range(len(s["shape"])):
             d = np.ascontiguousarray(s["dims"][i], np.float32)
             p = np.ascontiguousarray(s["pose"][i], np.float32)
             v = np.ascontiguousarray(s["vel"][i], np.float32)
-            L.or_add_body(self.h, int(s["shape"][i]), fptr(d), fptr(p), fptr(v), int(s["dynamic"][i]), float(s["density"][i]))
+            b = L.or_add_body(self.h, int(s["shape"][i]), fptr(d), fptr(p), fptr(v), int(s["dynamic"][i]), float(s["density"][i]))
+            if s.get("material") is not None:
+                L.or_set_body_material(self.h, b, int(s["material"][i]))
+            if s.get("sleep_allowed") is not None and not s["sleep_allowed"][i]:
+                L.or_set_sleep_allowed(self.h, b, 0)
         for a, b, fa, fb in s.get("joints", []):
             fa = np.ascontiguousarray(fa, np.float32)
             fb = np.ascontiguousarray(fb, np.float32)
@@ -157,6 +176,21 @@ class OracleWorld:
             lib().or_world_destroy(self.h)
         except Exception:
             pass
+
+    def add_body_spec(self, b):
+        """One body from a dict of the scene fields (shape, dims, pose, vel, dynamic, density,
+        optional material and sleep_allowed); returns its index."""
+        L = lib()
+        d = np.zeros(3, np.float32)
+        d[:len(b["dims"])] = b["dims"]
+        p = np.ascontiguousarray(b["pose"], np.float32)
+        v = np.ascontiguousarray(b["vel"], np.float32)
+        i = L.or_add_body(self.h, int(b["shape"]), fptr(d), fptr(p), fptr(v), int(b["dynamic"]), float(b["density"]))
+        if "material" in b:
+            L.or_set_body_material(self.h, i, int(b["material"]))
+        if not b.get("sleep_allowed", 1):
+            L.or_set_sleep_allowed(self.h, i, 0)
+        return i
 
     def step(self, dt=None):
         lib().or_step(self.h, self.dt if dt is None else dt)

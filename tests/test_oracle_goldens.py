@@ -93,3 +93,53 @@ def distance_scene(kind, hz):
 def test_distance_joint_goldens_g1(kind, hz, want):
     got = distance_scene(kind, hz)
     assert got == want, "kind %d at %g Hz: %016x, reference golden %016x" % (kind, hz, got, want)
+
+
+# G2/G3: restitution_toi_test.rs:596-619 (`w6_scene_hashes`, recorded in docs/PARITY_SPEC.md
+# :2120-2126) prints the first 8 bytes of World::step_hash after three W6 bounce scenes at 60 and
+# 240 Hz.  They pin, bit for bit: box-box SAT contacts and manifolds, the warm start and its cache,
+# the shock pass, the W6 time-of-impact bounce rows, the interior-point LCP on the pile's
+# support component, the pseudo pass, the integration with pseudo-velocities, sleep bookkeeping
+# and the step-hash byte stream (World::with_capacity(32|16) fixes the hash's sleep-slot count).
+W6_GOLDEN = [("pile", 60.0, "b6be817421a5936a"), ("pile", 240.0, "1a15ce00411301fa"),
+             ("stack", 60.0, "56a889bb6e73d42c"), ("stack", 240.0, "90eac03965c85039"),
+             ("drone", 60.0, "75f5412b43fa9cc0"), ("drone", 240.0, "a88414ba7682a6ed")]
+
+
+def w6_golden_run(kind, hz, world_cls, **kw):
+    """Runs one golden scene in world_cls (the oracle or the device binding); returns the world
+    and the largest (toi rows, support rows, pseudo rows) seen by its stats, if it has them."""
+    import scenes
+    if kind == "pile":
+        s = scenes.bounce_pile(hz)
+        w = world_cls(s, **kw)
+        for _ in range(int(hz)):
+            w.step()
+        w.add_body_spec(scenes.bounce_pile_drop())
+        ticks = int(hz)
+    elif kind == "stack":
+        w = world_cls(scenes.bounce_stack(hz), **kw)
+        ticks = int(3 * hz)
+    else:
+        w = world_cls(scenes.bounce_drone(hz), **kw)
+        ticks = int(3 * hz)
+    seen = [0, 0, 0, 0]
+    for _ in range(ticks):
+        w.step()
+        st = w.stats() if hasattr(w, "stats") else None
+        if st is not None and hasattr(st, "toi_rows"):
+            seen = [max(seen[0], st.toi_rows), max(seen[1], st.support_rows), max(seen[2], st.pseudo_rows),
+                    seen[3] + st.bounce_fallbacks]
+    return w, seen
+
+
+@pytest.mark.parametrize("kind,hz,want", W6_GOLDEN)
+def test_w6_scene_goldens_g2_g3(kind, hz, want):
+    cap = 16 if kind == "drone" else 32
+    w, seen = w6_golden_run(kind, hz, lambda s: O.OracleWorld(s, capacity=cap))
+    got = w.hash()[:8].hex()
+    assert got == want, "%s at %g Hz: %s, reference golden %s" % (kind, hz, got, want)
+    assert seen[0] > 0, "no time-of-impact row fired"
+    assert seen[3] == 0, "a bounce component fell back to sweeping"
+    if kind == "pile":
+        assert seen[1] > 100, "the pile's support component is the large interior-point system"

@@ -188,6 +188,10 @@ RPhysResult rphys_world_set_solver_iterations(RPhysWorld *world, uint32_t veloci
 RPhysResult rphys_world_set_residual_mode(RPhysWorld *world, uint32_t on, float tol, uint32_t max_iters);
 RPhysResult rphys_world_set_gravity(RPhysWorld *world, RPhysVec3 g);          /* World::set_gravity (world.rs:560-563) */
 RPhysResult rphys_body_set_sleep_allowed(RPhysWorld *world, uint32_t body_id, RPhysBool allowed);  /* world.rs:565-570 */
+/* World::add_collider's material (world.rs:614-622, materials.rs:11-12 core ids 0..15): the pair
+ * table entry (friction, restitution) the body's contacts take.  The reference's FFI gives every
+ * body MaterialId::Default (lib.rs:544-557); this sets the id a World-level host would pass. */
+RPhysResult rphys_body_set_material_id(RPhysWorld *world, uint32_t body_id, uint32_t material_id);
 /* n ticks of dt enqueued back to back with one host synchronisation at the end */
 RPhysResult rphys_world_step_n(RPhysWorld *world, float dt, uint32_t n_ticks);
 
@@ -200,6 +204,11 @@ typedef struct RPhysStepStats {
     uint32_t warm_entries;       /* warm-start cache entries after the tick */
     uint32_t pgs_iters;          /* PGS sweeps run by the island that ran the most */
     uint32_t pgs_iters_sum;      /* sum over islands of (sweeps x the island's contacts) */
+    /* W6 restitution over the ticks of the last step call (solver.rs:1009-1245): time-of-impact
+     * bounce rows, their support rows, pseudo rows, and components swept instead of solved
+     * directly */
+    uint32_t bounce_rows, bounce_support_rows, bounce_pseudo_rows, bounce_fallbacks;
+    uint32_t relax_rows;         /* contacts on jointed bodies bounced by the relax pass (solver.rs:1001-1008) */
 } RPhysStepStats;
 RPhysResult rphys_world_get_stats(const RPhysWorld *world, RPhysStepStats *out);
 

@@ -1458,6 +1458,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     for (uint32_t i = 0; i < nct; i++) {
         const CC *k = &cs[i];
         if (k->toi || k->rest_bias <= 0.0f || jn[i] <= 0.0f) continue;
+        w->stats.relax_rows++;
         float v = vdot(rel_vel(w, k), k->n);
         float old = jn[i];
         float nwv = fmaxf(old + (k->rest_bias - v) * k->m_n, 0.0f);

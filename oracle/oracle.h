@@ -61,6 +61,7 @@ typedef struct {
      * pseudo rows, velocity-pass components, components swept instead of solved directly, and the
      * most interior-point solves any component took */
     uint32_t toi_rows, support_rows, pseudo_rows, bounce_components, bounce_fallbacks, bounce_solves;
+    uint32_t relax_rows;   /* contacts bounced by the restitution relax pass */
 } OrStats;
 
 OrWorld *or_world_create(uint32_t max_bodies);

@@ -34,6 +34,7 @@ struct HashDev {
 extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const uint32_t *grids, hipStream_t st, hipEvent_t *ev,
                                      const TierStreams *ts);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
+extern "C" size_t rp_w6_ws_bytes();
 extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st);
 extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
@@ -230,6 +231,29 @@ static bool upload_model(Scene &S, const float *params) {
     return hip_ok(hipGetLastError(), "model prep");
 }
 
+// Whether a contact of these worlds can carry a restitution bias: some pair of the materials in
+// use (colliders with colliders, colliders with terrain) has restitution > 0 in the scene's pair
+// table (the first world's overrides, as the uploaded table).  Without one the W6 passes never
+// run and their buffers are not allocated.
+static bool scene_bounces(const std::vector<HostWorld *> &ws) {
+    bool used[16] = {}, env[16] = {};
+    for (auto *w : ws) {
+        for (uint32_t m : w->col_mat) used[m & 15u] = true;
+        if (w->has_hf) env[w->hmat & 15u] = true;
+    }
+    for (uint32_t a = 0; a < 16; a++) {
+        if (!used[a]) continue;
+        for (uint32_t b = 0; b < 16; b++) {
+            if (!used[b] && !env[b]) continue;
+            MatP p = table_pair(a, b);
+            uint32_t x = std::min(a, b), y = std::max(a, b);
+            for (auto &o : ws[0]->ovr) if (o.a == x && o.b == y) p = o.p;
+            if (p.rest > 0.0f) return true;
+        }
+    }
+    return false;
+}
+
 // Builds a device scene for the given worlds (host mirrors) with the given capacities and
 // uploads everything.  Returns nullptr (with the error set) on failure.
 static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_t cur) {
@@ -294,6 +318,17 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     if (d.bp_radix) ok &= B.alloc(d.bp_k0, NC) && B.alloc(d.bp_v0, NC) && B.alloc(d.bp_k1, NC) && B.alloc(d.bp_v1, NC);
     ok &= B.alloc(d.rs_hist, 256 * (size_t)d.rs_blocks + 1);
     ok &= B.alloc(d.pc_la, C) && B.alloc(d.pc_lb, C);
+    if (scene_bounces(ws)) {
+        d.w6_on = 1;
+        d.w6_slots = std::min<uint32_t>(nw, 32u);
+        ok &= B.alloc(d.w_w6, 2 * (size_t)nw) && B.alloc(d.w6_toi, C) && B.alloc(d.w6_tv, C) && B.alloc(d.w6_mark, C) &&
+              B.alloc(d.w6_rl, C) && B.alloc(d.w6_rc, C) && B.alloc(d.w6_pc, C) && B.alloc(d.w6_grp, C) && B.alloc(d.w6_comp, C) &&
+              B.alloc(d.w6_cst, C + nw + 1) && B.alloc(d.w6_rt, C) && B.alloc(d.w6_racc, C) && B.alloc(d.w6_pu, C) &&
+              B.alloc(d.w6_b, 2 * NB) && B.alloc(d.w6_par, NB) && B.alloc(d.w6_pl, NB) && B.alloc(d.w6_pa, NB) && B.alloc(d.w6_acc, 6 * NB);
+        char *wsp = nullptr;
+        ok &= B.alloc(wsp, rp_w6_ws_bytes() * d.w6_slots);
+        d.w6_ws = wsp;
+    }
     for (int k = 0; k < 2; k++) ok &= B.alloc(d.wk_key[k], W) && B.alloc(d.wk_la[k], W) && B.alloc(d.wk_jn[k], W) && B.alloc(d.wk_jt[k], W) && B.alloc(d.wc_off[k], nw + 1);
     ok &= B.alloc(d.newflag, C + 1) && B.alloc(d.keepflag, W + 1);
     size_t scan_elems = std::max<size_t>({NB, NC, C + 1, W + 1, 256 * (size_t)d.rs_blocks + 1, (size_t)nw + 8});
@@ -1008,6 +1043,22 @@ RPhysResult rphys_body_set_sleep_allowed(RPhysWorld *world, uint32_t body_id, RP
     GUARD_END
 }
 
+RPhysResult rphys_body_set_material_id(RPhysWorld *world, uint32_t body_id, uint32_t material_id) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_body_set_material_id: world is null"); return RPHYS_ERR_NULL; }
+    if (material_id > 15) return bad_arg("rphys_body_set_material_id", "material ids above 15 are not reachable through this ABI");
+    std::lock_guard<std::mutex> lk(world->mu);
+    uint32_t b;
+    RPhysResult r = live_id(world, body_id, "rphys_body_set_material_id", b);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    HostWorld &w = world->w;
+    for (uint32_t c = 0; c < w.nc(); c++) if (w.col_body[c] == b) w.col_mat[c] = material_id;
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+
 static RPhysResult step_impl(RPhysWorld *world, float dt, uint32_t substeps, uint32_t iv_override, uint32_t repeat, bool graph) {
     std::lock_guard<std::mutex> lk(world->mu);
     float sub = dt / (float)substeps;
@@ -1393,6 +1444,9 @@ RPhysResult rphys_world_get_stats(const RPhysWorld *cworld, RPhysStepStats *out)
     out->pairs = c[CT_PAIRS]; out->contacts = c[CT_CONTACTS]; out->terrain_candidates = c[CT_CANDS];
     out->island_contacts = c[CT_ISLC]; out->islands = c[CT_ISLANDS]; out->warm_entries = wt;
     out->pgs_iters = c[CT_ITERS_MAX]; out->pgs_iters_sum = c[CT_ITERS_SUM];
+    out->bounce_rows = c[CT_W6_TOI]; out->bounce_support_rows = c[CT_W6_SUPPORT];
+    out->bounce_pseudo_rows = c[CT_W6_PSEUDO]; out->bounce_fallbacks = c[CT_W6_FALLBACK];
+    out->relax_rows = c[CT_W6_RELAX];
     return RPHYS_OK;
     GUARD_END
 }

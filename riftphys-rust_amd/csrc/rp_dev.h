@@ -52,6 +52,9 @@ enum {
     CT_NCOL,              // the scene's collider count (the radix broadphase order's length)
     CT_WAVE_K, CT_GRP_K, CT_LANE_K,  // with the lane tier on: where tier 0's sorted islands solved by
                                      // lane groups of four, lane pairs and single lanes begin
+    CT_W6_TOI, CT_W6_SUPPORT, CT_W6_PSEUDO, CT_W6_FALLBACK,  // W6 this tick: bounce rows, support rows,
+                                                             // pseudo rows, components swept
+    CT_W6_RELAX,                                             // contacts bounced by the relax pass
     CT_COUNT
 };
 
@@ -180,6 +183,22 @@ struct Dev {
     float *wj_lam;
     uint32_t *jadj_off, *jadj;
     uint32_t *w_joint_off;
+    // W6 restitution (solver.rs:569-588, 995-1245), allocated when a pair of the scene's materials
+    // has restitution > 0 (w6_on): per world, a contact with a restitution bias ([w]) and
+    // pseudo-velocities ([nw + w]); per contact with a bias (written by the prep), (bias, start gap,
+    // e, time-of-impact row) and the sides' normal velocities at the step's start and after the
+    // forces; the bounce passes' scratch over each world's contact segment (marks, lists, rows,
+    // components); per body, a partner or component word and bits, the union-find parent, the
+    // pseudo-velocities and their f64 sums; the component systems' workspace slots.
+    uint32_t w6_on, w6_slots;
+    uint32_t *w_w6;
+    float4 *w6_toi, *w6_tv;
+    uint32_t *w6_mark, *w6_rl, *w6_rc, *w6_pc, *w6_grp, *w6_comp, *w6_cst;
+    float *w6_rt, *w6_racc, *w6_pu;
+    uint32_t *w6_b, *w6_par;
+    float4 *w6_pl, *w6_pa;
+    double *w6_acc;
+    void *w6_ws;
 };
 
 // Stage boundaries of one tick (events recorded by the profiled step; stage k spans marks k..k+1).

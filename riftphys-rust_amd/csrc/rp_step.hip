@@ -592,6 +592,28 @@ __global__ void k_prep(Dev d, TickParams tp) {
         V3 cva = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
         V3 cvb = terr ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
         float approach = 0.5f * (v_pre + dot3(cvb - cva, n));
+        // restitution bias (solver.rs:569-588): a contact approaching faster than 1 m/s bounces
+        // after the solve; for free bodies at its time of impact (W6), from the sides' normal
+        // velocities at the step's start (last step's end linear velocity, the pre-step angular
+        // one) and after this step's forces
+        if (d.w6_on && approach < -1.0f) {
+            const float rbias = -pp.z * approach;
+            if (!(rbias <= 0.0f)) {
+                const bool toi = !(d.flags[ai] & FL_JOINTED) && (terr || !(d.flags[bi] & FL_JOINTED));
+                float4 tv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (toi) {
+                    tv.x = dot3(L3(d.last_lin[ai]) + cross3(L3(d.pre_ang[ai]), ra), n);
+                    tv.y = dot3(cva, n);
+                    if (!terr) {
+                        tv.z = dot3(L3(d.last_lin[bi]) + cross3(L3(d.pre_ang[bi]), rb), n);
+                        tv.w = dot3(cvb, n);
+                    }
+                }
+                d.w6_toi[c] = make_float4(rbias, toi ? -pen : 0.0f, toi ? pp.z : 0.0f, toi ? 1.0f : 0.0f);
+                d.w6_tv[c] = tv;
+                if (!d.w_w6[w]) atomicOr(&d.w_w6[w], 1u);
+            }
+        }
         d.cs_ai[c] = ai; d.cs_bi[c] = bi;
         d.cs_inv[c] = make_float2(inv_a, inv_b);
         d.cs_ra[c] = S3(ra); d.cs_rb[c] = S3(rb);
@@ -2444,6 +2466,9 @@ __global__ void k_warm_merge(Dev d, TickParams tp) {
     }
 }
 
+#include "rp_bounce.h"
+
+
 // ------------------------------------------------------------------ S4
 __global__ void k_ball_init(Dev d) {
     if (aborted(d)) return;
@@ -2545,6 +2570,12 @@ __global__ void k_integrate(Dev d, TickParams tp) {
                 q = qq;
                 set_vel(d, i, vl, wv);
             }
+        }
+        // a bounced free body's pseudo-velocity moves it and never enters its velocity (step.rs:271-277)
+        if (d.w6_on && d.w_w6[d.nw + d.bworld[i]]) {
+            p = p + L3(d.w6_pl[i]) * dt;
+            const V3 pa = L3(d.w6_pa[i]);
+            if (len2(pa) > 0.0f) q = qnormalize(qmul(from_axis_angle(normalize3(pa), len3(pa) * dt), q));
         }
         set_pose(d, i, p, q);
         d.ppos[i] = d.pos[i]; d.prot[i] = d.rot[i];
@@ -2996,6 +3027,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_contact_write, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     mark(RP_ST_PREP);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_solve, d.nw);
+    if (d.w6_on) hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(2 * d.nw, B)), dim3(B), 0, st, d.w_w6, 2 * d.nw);
     if (d.nj) hipLaunchKernelGGL(k_joint_prep, dim3(grid_for(d.nj, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_island_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
@@ -3162,6 +3194,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.keepflag, d.cap_warm, d.cnt + CT_KEEP_TOTAL);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 2u);
     hipLaunchKernelGGL(k_warm_merge, dim3(grid_for(d.cap_contacts > d.cap_warm ? d.cap_contacts : d.cap_warm, B)), dim3(B), 0, st, d, tp);
+    if (d.w6_on) hipLaunchKernelGGL(k_w6, dim3(d.w6_slots), dim3(64), 0, st, d, tp);
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
@@ -3246,6 +3279,7 @@ extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, h
     hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, st, src, dst, n);
 }
 
+extern "C" size_t rp_w6_ws_bytes() { return w6_ws_bytes(); }
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st) {
     hipLaunchKernelGGL(k_restore_backup, dim3(grid_for(d.nb > d.nj ? d.nb : d.nj, 256)), dim3(256), 0, st, d);
 }

@@ -84,7 +84,9 @@ class PlayerDesc(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("terrain_candidates", C.c_uint32), ("island_contacts", C.c_uint32),
-                ("islands", C.c_uint32), ("warm_entries", C.c_uint32), ("pgs_iters", C.c_uint32), ("pgs_iters_sum", C.c_uint32)]
+                ("islands", C.c_uint32), ("warm_entries", C.c_uint32), ("pgs_iters", C.c_uint32), ("pgs_iters_sum", C.c_uint32),
+                ("bounce_rows", C.c_uint32), ("bounce_support_rows", C.c_uint32), ("bounce_pseudo_rows", C.c_uint32),
+                ("bounce_fallbacks", C.c_uint32), ("relax_rows", C.c_uint32)]
 
 
 class ContactEx(C.Structure):
@@ -158,6 +160,7 @@ EXPORTS = [
     ("rphys_world_set_gravity", U32, [P, Vec3]),
     ("rphys_world_set_residual_mode", U32, [P, U32, F32, U32]),
     ("rphys_body_set_sleep_allowed", U32, [P, U32, U32]),
+    ("rphys_body_set_material_id", U32, [P, U32, U32]),
     ("rphys_world_step_n", U32, [P, F32, U32]),
     ("rphys_world_get_stats", U32, [P, C.POINTER(StepStats)]),
     ("rphys_world_step_profiled", U32, [P, F32, U32, C.POINTER(Profile)]),
@@ -323,8 +326,15 @@ class World:
         if s.get("residual"):  # (tol, max_iters): each island iterates to the residual
             tol, mx = s["residual"]
             check(L.rphys_world_set_residual_mode(self.h, 1, tol, mx))
+        if s.get("gravity") is not None:
+            check(L.rphys_world_set_gravity(self.h, Vec3(*[float(x) for x in s["gravity"]])))
         ids = [self.add_body(int(s["shape"][i]), s["dims"][i], s["pose"][i], s["vel"][i], int(s["dynamic"][i]), float(s["density"][i]))
                for i in range(len(s["shape"]))]
+        for i, b in enumerate(ids):
+            if s.get("material") is not None:
+                check(L.rphys_body_set_material_id(self.h, b, int(s["material"][i])))
+            if s.get("sleep_allowed") is not None and not s["sleep_allowed"][i]:
+                check(L.rphys_body_set_sleep_allowed(self.h, b, 0))
         for a, b, fa, fb in s.get("joints", []):  # scene joints name bodies by scene index
             self.add_ball_joint(ids[a], ids[b], fa, fb)
         hf = s.get("hf")
@@ -338,6 +348,18 @@ class World:
         check(lib().rphys_add_body_density(self.h, shape, d.ctypes.data_as(PF32), C.byref(iso(pose[:3], pose[3:])),
                                            C.byref(vel(v[:3], v[3:])), BODY_DYNAMIC if dynamic else BODY_STATIC, density, C.byref(out)))
         return out.value
+
+    def add_body_spec(self, b):
+        """One body from a dict of the scene fields (shape, dims, pose, vel, dynamic, density,
+        optional material and sleep_allowed); returns its id."""
+        d = np.zeros(3, np.float32)
+        d[:len(b["dims"])] = b["dims"]
+        i = self.add_body(int(b["shape"]), d, b["pose"], b["vel"], int(b["dynamic"]), float(b["density"]))
+        if "material" in b:
+            check(lib().rphys_body_set_material_id(self.h, i, int(b["material"])))
+        if not b.get("sleep_allowed", 1):
+            check(lib().rphys_body_set_sleep_allowed(self.h, i, 0))
+        return i
 
     def add_ball_joint(self, body_a, body_b, frame_a, frame_b):
         """Ball joint between two body ids, frames (px, py, pz, qx, qy, qz, qw) in each body's space;

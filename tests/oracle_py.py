@@ -15,7 +15,8 @@ class OrStats(C.Structure):
                 ("terrain_candidates", C.c_uint32), ("levels", C.c_uint32), ("warm_matched", C.c_uint32),
                 ("iters_used", C.c_uint32), ("residual", C.c_float), ("iters_sum", C.c_uint32),
                 ("toi_rows", C.c_uint32), ("support_rows", C.c_uint32), ("pseudo_rows", C.c_uint32),
-                ("bounce_components", C.c_uint32), ("bounce_fallbacks", C.c_uint32), ("bounce_solves", C.c_uint32)]
+                ("bounce_components", C.c_uint32), ("bounce_fallbacks", C.c_uint32), ("bounce_solves", C.c_uint32),
+                ("relax_rows", C.c_uint32)]
 
 
 class OrContact(C.Structure):

@@ -113,10 +113,16 @@ RPhysResult rphys_world_num_bodies(const RPhysWorld *world, uint32_t *out_num);
 RPhysResult rphys_world_get_step_hash(const RPhysWorld *world, RPhysHash32 *out_hash);
 RPhysResult rphys_world_step_and_hash(RPhysWorld *world, float dt, RPhysHash32 *out_hash);
 
-/* h:221-222: queries are outside this library's scope; they write a miss and return
- * RPHYS_ERR_INTERNAL with a message */
+/* h:221-222, lib.rs:1268-1410: World::raycast (collider bounding boxes, terrain triangles) and
+ * World::sweep_capsule (queries.rs:50-261), answered on the device from the world's current state */
 RPhysResult rphys_world_raycast(const RPhysWorld *world, const RPhysRayQuery *query, RPhysRayHit *out_hit);
 RPhysResult rphys_world_capsule_sweep(const RPhysWorld *world, const RPhysCapsuleSweepQuery *query, RPhysCapsuleSweepHit *out_hit);
+/* Batched queries (build addition): n rays or capsule sweeps answered in one device launch, each as
+ * rphys_world_raycast / rphys_world_capsule_sweep would (same checks; the first invalid query's
+ * error is returned with every output a miss). */
+RPhysResult rphys_world_raycast_n(const RPhysWorld *world, const RPhysRayQuery *queries, uint32_t n, RPhysRayHit *out_hits);
+RPhysResult rphys_world_capsule_sweep_n(const RPhysWorld *world, const RPhysCapsuleSweepQuery *queries, uint32_t n,
+                                        RPhysCapsuleSweepHit *out_hits);
 
 RPhysResult rphys_last_error_copy(char *dst, uint32_t cap, uint32_t *out_len);   /* h:225, lib.rs:561-604 */
 

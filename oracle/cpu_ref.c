@@ -2185,6 +2185,261 @@ void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *
     w->nwarm = nwarm;
 }
 
+/* ------------------------------------------------------------------ queries
+ * world/src/world/queries.rs:50-261 (World::raycast against collider AABBs and terrain triangles;
+ * World::sweep_capsule: overlap at the start against true shapes, then tip and base spheres swept
+ * against AABBs), with the geometry they call: ray_aabb / normal_from_aabb_point
+ * (world/src/world/geom.rs:40-93), Triangle::intersect_ray / barycentric / normal
+ * (geom/src/triangle.rs:25-232, plane.rs:15-86), the query crate's sphere sweeps
+ * (query/src/lib.rs: ray_aabb_slab_unit, sweep_sphere_aabb, sweep_two_spheres_aabbs, pick_better),
+ * HeightField::query_aabb (terrain.rs:300-357).  Body indices; TERRAIN_CI for the terrain. */
+static V3 q_normal_from_aabb_point(Aabb bb, V3 p) {
+    const float eps = 1.0e-4f;
+    if (fabsf(p.x - bb.min.x) < eps) return v3(-1, 0, 0);
+    if (fabsf(p.x - bb.max.x) < eps) return v3(1, 0, 0);
+    if (fabsf(p.y - bb.min.y) < eps) return v3(0, -1, 0);
+    if (fabsf(p.y - bb.max.y) < eps) return v3(0, 1, 0);
+    if (fabsf(p.z - bb.min.z) < eps) return v3(0, 0, -1);
+    if (fabsf(p.z - bb.max.z) < eps) return v3(0, 0, 1);
+    V3 c = vmul(vadd(bb.min, bb.max), 0.5f);
+    V3 n = vsub(p, c);
+    float l2 = vlen2(n);
+    if (l2 > 1.0e-20f) return vdivs(n, sqrtf(l2));
+    return v3(0, 1, 0);
+}
+static int q_ray_aabb(V3 o, V3 d, Aabb bb, float max_dist, float *t_out, V3 *p_out, V3 *n_out) {
+    const float eps = 1.0e-8f;
+    float tmin = 0.0f, tmax = max_dist;
+    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    float mn[3] = {bb.min.x, bb.min.y, bb.min.z}, mx[3] = {bb.max.x, bb.max.y, bb.max.z};
+    for (int a = 0; a < 3; a++) {
+        if (fabsf(dd[a]) < eps) {
+            if (oo[a] < mn[a] || oo[a] > mx[a]) return 0;
+        } else {
+            float inv = 1.0f / dd[a];
+            float t1 = (mn[a] - oo[a]) * inv, t2 = (mx[a] - oo[a]) * inv;
+            if (t1 > t2) { float x = t1; t1 = t2; t2 = x; }
+            if (t1 > tmin) tmin = t1;
+            if (t2 < tmax) tmax = t2;
+            if (tmin > tmax) return 0;
+        }
+    }
+    if (tmin < 0.0f || tmin > max_dist) return 0;
+    *t_out = tmin;
+    *p_out = vadd(o, vmul(d, tmin));
+    *n_out = q_normal_from_aabb_point(bb, *p_out);
+    return 1;
+}
+static int q_finite3(V3 v) { return vfinite(v); }
+/* Triangle::intersect_ray(ray, 0, t_max): plane through the triangle, then barycentric inside test */
+static int q_tri_ray(Tri t, V3 o, V3 d, float t_max, float *t_out) {
+    if (0.0f > t_max) return 0;
+    V3 nu = vcross(vsub(t.b, t.a), vsub(t.c, t.a));
+    float a2 = vdot(nu, nu);
+    if (!isfinite(a2) || a2 <= 1.0e-16f) return 0;                       /* is_degenerate */
+    if (!(q_finite3(t.a) && q_finite3(t.b) && q_finite3(t.c))) return 0;  /* Plane::from_points */
+    float len2 = vlen2(nu);
+    if (!isfinite(len2) || len2 <= 1.0e-6f * 1.0e-6f) return 0;
+    float inv_len = 1.0f / sqrtf(len2);
+    V3 pn = vmul(nu, inv_len);
+    float pd = -vdot(pn, t.a);
+    if (!(q_finite3(pn) && isfinite(pd))) return 0;
+    if (!(q_finite3(o) && q_finite3(d))) return 0;                       /* Plane::intersect_ray */
+    float denom = vdot(pn, d);
+    if (!isfinite(denom) || fabsf(denom) < 1.0e-6f) return 0;
+    float numer = -(vdot(pn, o) + pd);
+    if (!isfinite(numer)) return 0;
+    float tt = numer / denom;
+    if (!isfinite(tt)) return 0;
+    if (!(tt >= 0.0f && tt <= t_max)) return 0;
+    V3 p = vadd(o, vmul(d, tt));                                         /* Ray::point_at */
+    /* barycentric */
+    if (!(q_finite3(t.a) && q_finite3(t.b) && q_finite3(t.c) && q_finite3(p))) return 0;
+    V3 v0 = vsub(t.b, t.a), v1 = vsub(t.c, t.a), v2 = vsub(p, t.a);
+    float d00 = vdot(v0, v0), d01 = vdot(v0, v1), d11 = vdot(v1, v1), d20 = vdot(v2, v0), d21 = vdot(v2, v1);
+    float den = d00 * d11 - d01 * d01;
+    float scale = fmaxf(fabsf(d00 * d11), 1.0f);
+    if (!isfinite(den) || fabsf(den) <= 1.0e-12f * scale) return 0;
+    float id = 1.0f / den;
+    float bv = (d11 * d20 - d01 * d21) * id, bw = (d00 * d21 - d01 * d20) * id;
+    float bu = 1.0f - bv - bw;
+    if (!(isfinite(bu) && isfinite(bv) && isfinite(bw))) return 0;
+    if (!(bu >= -1.0e-5f && bv >= -1.0e-5f && bw >= -1.0e-5f)) return 0;  /* contains_barycentric */
+    *t_out = tt;
+    return 1;
+}
+static V3 q_tri_normal(Tri t) {
+    V3 n = vcross(vsub(t.b, t.a), vsub(t.c, t.a));
+    if (!vfinite(n)) return v3s(0);
+    return vnormalize_or_zero(n);
+}
+/* HeightField::query_aabb's cell range (terrain.rs:300-319); 0 when nothing is emitted */
+static int q_hf_range(const OrWorld *w, Aabb bb, int *gx0, int *gx1, int *gz0, int *gz1) {
+    if (!w->has_hf) return 0;
+    int nx = (int)w->hnx, nz = (int)w->hnz;
+    if (nx < 2 || nz < 2) return 0;
+    float lminx = bb.min.x - w->hox, lmaxx = bb.max.x - w->hox;
+    float lminz = bb.min.z - w->hoz, lmaxz = bb.max.z - w->hoz;
+    *gx0 = ICLAMP((int)floorf(lminx / w->hsx) - 1, 0, nx - 2); *gx1 = ICLAMP((int)ceilf(lmaxx / w->hsx) + 1, 0, nx - 2);
+    *gz0 = ICLAMP((int)floorf(lminz / w->hsz) - 1, 0, nz - 2); *gz1 = ICLAMP((int)ceilf(lmaxz / w->hsz) + 1, 0, nz - 2);
+    if (bb.min.y > w->hmax + 1.0f || bb.max.y < w->hmin - 1.0f) return 0;
+    return 1;
+}
+static void q_hf_tris(const OrWorld *w, int gx, int gz, Tri out[2]) {
+    int nx = (int)w->hnx;
+    float h00 = w->hts[gx + gz * nx], h10 = w->hts[gx + 1 + gz * nx];
+    float h01 = w->hts[gx + (gz + 1) * nx], h11 = w->hts[gx + 1 + (gz + 1) * nx];
+    float x0 = (float)gx * w->hsx + w->hox, x1 = x0 + w->hsx;
+    float z0 = (float)gz * w->hsz + w->hoz, z1 = z0 + w->hsz;
+    V3 p00 = v3(x0, h00, z0), p10 = v3(x1, h10, z0), p01 = v3(x0, h01, z1), p11 = v3(x1, h11, z1);
+    Tri a = {p00, p01, p10}, b = {p10, p01, p11};
+    out[0] = a; out[1] = b;
+}
+static void q_put(float *o3, V3 v) { o3[0] = v.x; o3[1] = v.y; o3[2] = v.z; }
+static Aabb q_aabb_of_body(const OrWorld *w, uint32_t c) { uint32_t b = w->cbody[c]; return aabb_of(w->cshape[c], w->pos[b], w->rot[b]); }
+
+/* World::raycast (queries.rs:50-116); dir is normalised by the caller as the FFI does */
+int or_raycast(const OrWorld *w, const float *origin3, const float *dir3, float max_dist, int32_t ignore, OrQueryHit *out) {
+    memset(out, 0, sizeof(*out));
+    V3 o = v3(origin3[0], origin3[1], origin3[2]), dr = v3(dir3[0], dir3[1], dir3[2]);
+    float max_d = fmaxf(max_dist, 0.0f);
+    if (max_d <= 0.0f) return 0;
+    float len = vlen(dr);
+    if (!isfinite(len) || len < 1.0e-12f) return 0;
+    V3 d = vdivs(dr, len);
+    int have = 0; float bt = 0; uint32_t bi = 0; V3 bp = v3s(0), bn = v3s(0);
+    for (uint32_t c = 0; c < w->nc; c++) {
+        if (ignore >= 0 && w->cbody[c] == (uint32_t)ignore) continue;
+        float t; V3 p, n;
+        if (!q_ray_aabb(o, d, q_aabb_of_body(w, c), max_d, &t, &p, &n)) continue;
+        if (!have || (t < bt - 1.0e-9f) || (fabsf(t - bt) <= 1.0e-9f && c < bi)) { have = 1; bt = t; bi = c; bp = p; bn = n; }
+    }
+    V3 end = vadd(o, vmul(d, max_d));
+    Aabb qb = aabb_new(vmin(o, end), vmax(o, end));
+    int gx0, gx1, gz0, gz1;
+    if (q_hf_range(w, qb, &gx0, &gx1, &gz0, &gz1))
+        for (int gz = gz0; gz <= gz1; gz++)
+            for (int gx = gx0; gx <= gx1; gx++) {
+                Tri tr[2];
+                q_hf_tris(w, gx, gz, tr);
+                for (int k = 0; k < 2; k++) {
+                    float t;
+                    if (!q_tri_ray(tr[k], o, d, max_d, &t)) continue;
+                    V3 p = isfinite(t) ? vadd(o, vmul(d, t)) : o;
+                    if (!have || t < bt - 1.0e-9f) { have = 1; bt = t; bi = TERRAIN_CI; bp = p; bn = q_tri_normal(tr[k]); }
+                }
+            }
+    if (!have) return 0;
+    out->hit = 1;
+    out->body = bi == TERRAIN_CI ? TERRAIN_CI : w->cbody[bi];
+    out->fraction = fclampf(bt / max_d, 0.0f, 1.0f);
+    q_put(out->point, bp); q_put(out->normal, bn);
+    return 1;
+}
+
+/* query/src/lib.rs ray_aabb_slab_unit on the AABB grown by r (sweep_sphere_aabb with dt = 1 and
+ * a non-degenerate sweep) */
+static int q_sweep_sphere_aabb(V3 p0, V3 dir, float r, Aabb bb, float *t_out, V3 *n_out) {
+    V3 mn = vsub(bb.min, v3s(r)), mx = vadd(bb.max, v3s(r));
+    V3 inv = v3(p0.x == p0.x && fabsf(dir.x) > 1.0e-9f ? 1.0f / dir.x : 1.0e9f,
+                p0.y == p0.y && fabsf(dir.y) > 1.0e-9f ? 1.0f / dir.y : 1.0e9f,
+                p0.z == p0.z && fabsf(dir.z) > 1.0e-9f ? 1.0f / dir.z : 1.0e9f);
+    V3 t1 = v3((mn.x - p0.x) * inv.x, (mn.y - p0.y) * inv.y, (mn.z - p0.z) * inv.z);
+    V3 t2 = v3((mx.x - p0.x) * inv.x, (mx.y - p0.y) * inv.y, (mx.z - p0.z) * inv.z);
+    V3 tmin = vmin(t1, t2), tmax = vmax(t1, t2);
+    float te = tmin.x;
+    V3 n = v3(t1.x > t2.x ? 1.0f : -1.0f, 0, 0);
+    if (tmin.y > te) { te = tmin.y; n = v3(0, t1.y > t2.y ? 1.0f : -1.0f, 0); }
+    if (tmin.z > te) { te = tmin.z; n = v3(0, 0, t1.z > t2.z ? 1.0f : -1.0f); }
+    float tx = fminf(fminf(tmax.x, tmax.y), tmax.z);
+    if (te <= tx && tx >= 0.0f && te >= 0.0f && te <= 1.0f) { *t_out = te; *n_out = n; return 1; }
+    return 0;
+}
+/* sweep_two_spheres_aabbs over one AABB: the tip's hit, replaced by the base's when earlier by
+ * more than 1e-9 (pick_better) */
+static int q_sweep_two(V3 tip0, V3 base0, V3 v, float r, Aabb bb, float *t_out, V3 *n_out, int *kind) {
+    float t; V3 n;
+    int have = 0;
+    if (q_sweep_sphere_aabb(tip0, v, r, bb, &t, &n)) { have = 1; *t_out = t; *n_out = n; *kind = 0; }
+    if (q_sweep_sphere_aabb(base0, v, r, bb, &t, &n)) {
+        if (!have || t < *t_out - 1.0e-9f) { have = 1; *t_out = t; *n_out = n; *kind = 1; }
+    }
+    return have;
+}
+/* World::sweep_capsule (queries.rs:118-261) */
+int or_capsule_sweep(const OrWorld *w, const float *from3, const float *to3, float radius, float half_height, int32_t ignore,
+                     OrQueryHit *out) {
+    memset(out, 0, sizeof(*out));
+    V3 from = v3(from3[0], from3[1], from3[2]), to = v3(to3[0], to3[1], to3[2]);
+    V3 v = vsub(to, from);
+    float dist_sq = vlen2(v);
+    ShapeD cap; cap.type = OR_CAPSULE; cap.d[0] = radius; cap.d[1] = half_height; cap.d[2] = 0.0f;
+    Q4 qi = qident();
+    for (uint32_t c = 0; c < w->nc; c++) {
+        uint32_t b = w->cbody[c];
+        if (ignore >= 0 && b == (uint32_t)ignore) continue;
+        Manifold m;
+        if (collide_shapes(cap, from, qi, w->cshape[c], w->pos[b], w->rot[b], 0.0f, &m) && m.len > 0) {
+            out->hit = 1; out->body = b; out->fraction = 0.0f; out->started = 1; q_put(out->point, m.p[0].pos); q_put(out->normal, m.normal);
+            return 1;
+        }
+    }
+    Aabb sb = aabb_of(cap, from, qi);
+    int gx0, gx1, gz0, gz1;
+    if (q_hf_range(w, sb, &gx0, &gx1, &gz0, &gz1))
+        for (int gz = gz0; gz <= gz1; gz++)
+            for (int gx = gx0; gx <= gx1; gx++) {
+                Tri tr[2];
+                q_hf_tris(w, gx, gz, tr);
+                for (int k = 0; k < 2; k++) {
+                    Manifold m;
+                    if (collide_shape_tri(cap, from, qi, tr[k], 0.0f, &m) && m.len > 0) {
+                        out->hit = 1; out->body = TERRAIN_CI; out->fraction = 0.0f; out->started = 1; q_put(out->point, m.p[0].pos);
+                        q_put(out->normal, m.normal);
+                        return 1;
+                    }
+                }
+            }
+    if (dist_sq < 1.0e-12f) return 0;
+    V3 tip0 = vadd(from, v3(0.0f, half_height, 0.0f)), base0 = vadd(from, v3(0.0f, -half_height, 0.0f));
+    const float eps = 1.0e-9f;
+    int have = 0; float bt = 0; uint32_t bi = 0; V3 bp = v3s(0), bn = v3s(0);
+    for (uint32_t c = 0; c < w->nc; c++) {
+        uint32_t b = w->cbody[c];
+        if (ignore >= 0 && b == (uint32_t)ignore) continue;
+        float t; V3 n; int kind;
+        if (!q_sweep_two(tip0, base0, v, radius, q_aabb_of_body(w, c), &t, &n, &kind)) continue;
+        t = fclampf(t, 0.0f, 1.0f);
+        V3 p = vsub(vadd(kind == 0 ? tip0 : base0, vmul(v, t)), vmul(n, radius));
+        if (!have || (t < bt - eps) || (fabsf(t - bt) <= eps && c < bi)) { have = 1; bt = t; bi = c; bp = p; bn = n; }
+    }
+    V3 ext = v3s(radius + half_height);
+    Aabb swb = aabb_new(vsub(vmin(from, to), ext), vadd(vmax(from, to), ext));
+    if (q_hf_range(w, swb, &gx0, &gx1, &gz0, &gz1))
+        for (int gz = gz0; gz <= gz1; gz++)
+            for (int gx = gx0; gx <= gx1; gx++) {
+                Tri tr[2];
+                q_hf_tris(w, gx, gz, tr);
+                for (int k = 0; k < 2; k++) {
+                    /* aabb_of(Triangle, identity): the vertices through the identity pose */
+                    V3 wa = vadd(v3s(0), qrot(qi, tr[k].a)), wb = vadd(v3s(0), qrot(qi, tr[k].b)), wc = vadd(v3s(0), qrot(qi, tr[k].c));
+                    Aabb eb = aabb_new(vmin(vmin(wa, wb), wc), vmax(vmax(wa, wb), wc));
+                    float t; V3 n; int kind;
+                    if (!q_sweep_two(tip0, base0, v, radius, eb, &t, &n, &kind)) continue;
+                    t = fclampf(t, 0.0f, 1.0f);
+                    V3 p = vsub(vadd(kind == 0 ? tip0 : base0, vmul(v, t)), vmul(n, radius));
+                    if (!have || t < bt - eps) { have = 1; bt = t; bi = TERRAIN_CI; bp = p; bn = n; }
+                }
+            }
+    if (!have) return 0;
+    out->hit = 1;
+    out->body = bi == TERRAIN_CI ? TERRAIN_CI : w->cbody[bi];
+    out->fraction = fclampf(bt, 0.0f, 1.0f);
+    out->started = 0;
+    q_put(out->point, bp); q_put(out->normal, bn);
+    return 1;
+}
+
 /* The W6 numerics alone, for the oracle's own pin tests: a symmetric CSR matrix (both triangles,
  * columns ascending, diagonal present). */
 static BCsr bn_wrap(uint32_t n, const uint32_t *ptr, const uint32_t *idx, const double *val) {

@@ -144,6 +144,14 @@ void or_load_state(OrWorld *w, const float *pos, const float *rot, const float *
                    const uint32_t *awake, const uint32_t *sticks, uint64_t tick, const uint64_t *wkeys, const float *wla,
                    const float *wjn, const float *wjt, uint32_t nwarm);
 
+/* Queries (world/src/world/queries.rs): body index of the hit (0xFFFFFFFF terrain), fraction of the
+ * ray / sweep, point and normal; started = the capsule overlapped at its start.  ignore = a body
+ * index to skip, or -1. */
+typedef struct { int hit; uint32_t body; float fraction; float point[3], normal[3]; int started; } OrQueryHit;
+int or_raycast(const OrWorld *w, const float *origin3, const float *dir3, float max_dist, int32_t ignore, OrQueryHit *out);
+int or_capsule_sweep(const OrWorld *w, const float *from3, const float *to3, float radius, float half_height, int32_t ignore,
+                     OrQueryHit *out);
+
 /* W6 numerics (bounce_lcp.rs), for the pin tests: K symmetric CSR, n <= 250.  min_degree writes
  * the elimination order and returns the factor's entries below the diagonal; pseudo_solve solves
  * (K + CFM diag K) x = u; bounce_qp solves the push-only LCP and returns 1 if converged. */

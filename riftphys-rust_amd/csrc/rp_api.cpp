@@ -35,6 +35,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
                                      const TierStreams *ts);
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st);
 extern "C" size_t rp_w6_ws_bytes();
+extern "C" hipError_t rp_launch_query(const Dev &d, const RpQuery *qs, RpQueryHit *out, uint32_t n, hipStream_t st);
 extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, hipStream_t st);
 extern "C" void rp_launch_mlp_prep(const float *p, __bf16 *w1p, __bf16 *w2p, __bf16 *w2t, __bf16 *w3p, __bf16 *w3t, hipStream_t st);
 extern "C" void rp_launch_mlp_infer(const float *p, const __bf16 *w1p, const __bf16 *w2p, const __bf16 *w3p, const float *feat, const uint32_t *n_ptr,
@@ -205,6 +206,9 @@ struct Scene {
     TickParams graph_tp{};  // parameters the graphs were captured with (cur aside)
     std::vector<std::unique_ptr<Slot>> slots;
     TierStreams ts{{nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // island-sweep tiers run concurrently
+    RpQuery *q_buf = nullptr;  // spatial queries in flight and their answers (q_cap of each)
+    RpQueryHit *h_buf = nullptr;
+    uint32_t q_cap = 0;
     ~Scene() {
         for (int i = 0; i < 2; i++) if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
         for (int i = 0; i < 3; i++) if (ts.ev[i]) (void)hipEventDestroy(ts.ev[i]);
@@ -1228,17 +1232,150 @@ RPhysResult rphys_world_step_and_hash(RPhysWorld *world, float dt, RPhysHash32 *
     GUARD_END
 }
 
+// ---- spatial queries (ffi/src/lib.rs:1268-1410 over world/src/world/queries.rs): the device
+// answers them from the world's current state (k_query, one workgroup per query)
+static void ray_miss(RPhysRayHit *h) {  // lib.rs:470-479
+    h->hit = RPHYS_FALSE; h->body_id = RPHYS_INVALID_BODY_ID; h->fraction = 1.0f;
+    h->point = {0.0f, 0.0f, 0.0f}; h->normal = {0.0f, 1.0f, 0.0f};
+}
+static void sweep_miss(RPhysCapsuleSweepHit *h) {  // lib.rs:482-492
+    h->hit = RPHYS_FALSE; h->body_id = RPHYS_INVALID_BODY_ID; h->fraction = 1.0f; h->started_overlapping = RPHYS_FALSE;
+    h->point = {0.0f, 0.0f, 0.0f}; h->normal = {0.0f, 1.0f, 0.0f};
+}
+static bool fin3h(RPhysVec3 v) { return hfin(v.x) && hfin(v.y) && hfin(v.z); }
+// checked_optional_live_body_id (lib.rs:412-421): 0 is none, else a live body
+static RPhysResult ignore_index(RPhysWorld *world, uint32_t id, const char *ctx, uint32_t &out) {
+    out = RP_NONE;
+    if (id == 0) return RPHYS_OK;
+    uint32_t b;
+    RPhysResult r = live_id(world, id, ctx, b);
+    if (r != RPHYS_OK) return r;
+    out = b;
+    return RPHYS_OK;
+}
+// Runs n queries on the device against the world's current state.
+static bool run_queries(RPhysWorld *world, std::vector<RpQuery> &qs, std::vector<RpQueryHit> &hits) {
+    if (world->need_build || !world->scene) { if (!pull(world) || !push(world)) return false; }
+    Scene &S = *world->scene;
+    const uint32_t n = (uint32_t)qs.size();
+    if (n > S.q_cap) {
+        uint32_t cap = std::max<uint32_t>(n, 64u);
+        if (!S.buf.alloc(S.q_buf, cap) || !S.buf.alloc(S.h_buf, cap)) { set_err("query buffers: device allocation failed"); return false; }
+        S.q_cap = cap;
+    }
+    hits.resize(n);
+    return hip_ok(hipMemcpyAsync(S.q_buf, qs.data(), n * sizeof(RpQuery), hipMemcpyHostToDevice, S.st), "query upload") &&
+           hip_ok(rp_launch_query(S.d, S.q_buf, S.h_buf, n, S.st), "query launch") &&
+           hip_ok(hipMemcpyAsync(hits.data(), S.h_buf, n * sizeof(RpQueryHit), hipMemcpyDeviceToHost, S.st), "query download") &&
+           hip_ok(hipStreamSynchronize(S.st), "query sync");
+}
+static uint32_t hit_body_id(const RpQueryHit &h) { return h.body == RP_TERRAIN ? RPHYS_TERRAIN_BODY_ID : h.body + 1; }
+
+// The argument checks of rphys_world_raycast (lib.rs:1277-1323); returns OK with skip set when the
+// query misses by definition (max_distance <= 0 or a zero direction).
+static RPhysResult ray_query(RPhysWorld *world, const RPhysRayQuery &q, RpQuery &out, bool &skip) {
+    skip = false;
+    if (!fin3h(q.origin)) return bad_arg("rphys_world_raycast", "origin contains NaN/Inf");
+    if (!fin3h(q.dir)) return bad_arg("rphys_world_raycast", "dir contains NaN/Inf");
+    if (!hfin(q.max_distance)) return bad_arg("rphys_world_raycast", "max_distance is not finite");
+    uint32_t ig;
+    RPhysResult r = ignore_index(world, q.ignore_body, "rphys_world_raycast.ignore_body", ig);
+    if (r != RPHYS_OK) return r;
+    if (q.max_distance <= 0.0f) { skip = true; return RPHYS_OK; }
+    const float len2 = (q.dir.x * q.dir.x + q.dir.y * q.dir.y) + q.dir.z * q.dir.z;
+    if (!hfin(len2)) return bad_arg("rphys_world_raycast", "dir length overflow/invalid");
+    if (len2 <= 1.0e-20f) { skip = true; return RPHYS_OK; }
+    const float l = std::sqrt(len2);
+    float d[3] = {q.dir.x / l, q.dir.y / l, q.dir.z / l};
+    // World::raycast normalises again (queries.rs:55-57)
+    const float len = std::sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    if (!hfin(len) || len < 1.0e-12f) { skip = true; return RPHYS_OK; }
+    out = RpQuery{};
+    out.kind = 0; out.world = 0; out.ignore = ig;
+    out.a[0] = q.origin.x; out.a[1] = q.origin.y; out.a[2] = q.origin.z;
+    out.b[0] = d[0] / len; out.b[1] = d[1] / len; out.b[2] = d[2] / len;
+    out.max_d = q.max_distance;
+    return RPHYS_OK;
+}
+static RPhysResult sweep_query(RPhysWorld *world, const RPhysCapsuleSweepQuery &q, RpQuery &out) {
+    if (!fin3h(q.from)) return bad_arg("rphys_world_capsule_sweep", "from contains NaN/Inf");
+    if (!fin3h(q.to)) return bad_arg("rphys_world_capsule_sweep", "to contains NaN/Inf");
+    if (!(hfin(q.radius) && q.radius > 0.0f)) return bad_arg("rphys_world_capsule_sweep", "radius must be finite and > 0");
+    if (!(hfin(q.half_height) && q.half_height >= 0.0f)) return bad_arg("rphys_world_capsule_sweep", "half_height must be finite and >= 0");
+    uint32_t ig;
+    RPhysResult r = ignore_index(world, q.ignore_body, "rphys_world_capsule_sweep.ignore_body", ig);
+    if (r != RPHYS_OK) return r;
+    out = RpQuery{};
+    out.kind = 1; out.world = 0; out.ignore = ig;
+    out.a[0] = q.from.x; out.a[1] = q.from.y; out.a[2] = q.from.z;
+    out.b[0] = q.to.x; out.b[1] = q.to.y; out.b[2] = q.to.z;
+    out.radius = q.radius; out.half_height = q.half_height;
+    return RPHYS_OK;
+}
+
+RPhysResult rphys_world_raycast_n(const RPhysWorld *cworld, const RPhysRayQuery *queries, uint32_t n, RPhysRayHit *out_hits) {
+    GUARD_BEGIN
+    if (out_hits) for (uint32_t i = 0; i < n; i++) ray_miss(&out_hits[i]);
+    if (!cworld || (n && (!queries || !out_hits))) { set_err("rphys_world_raycast: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    std::vector<RpQuery> qs;
+    std::vector<uint32_t> slot;
+    for (uint32_t i = 0; i < n; i++) {
+        RpQuery q;
+        bool skip;
+        RPhysResult r = ray_query(world, queries[i], q, skip);
+        if (r != RPHYS_OK) return r;
+        if (!skip) { qs.push_back(q); slot.push_back(i); }
+    }
+    if (qs.empty() || (world->w.nb() == 0 && !world->w.has_hf)) return RPHYS_OK;
+    std::vector<RpQueryHit> hits;
+    if (!run_queries(world, qs, hits)) return RPHYS_ERR_INTERNAL;
+    for (size_t k = 0; k < qs.size(); k++) {
+        const RpQueryHit &h = hits[k];
+        if (!h.hit) continue;
+        RPhysRayHit &o = out_hits[slot[k]];
+        o.hit = RPHYS_TRUE; o.body_id = hit_body_id(h); o.fraction = std::min(std::max(h.fraction, 0.0f), 1.0f);
+        o.point = {h.point[0], h.point[1], h.point[2]}; o.normal = {h.normal[0], h.normal[1], h.normal[2]};
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
 RPhysResult rphys_world_raycast(const RPhysWorld *world, const RPhysRayQuery *query, RPhysRayHit *out_hit) {
-    if (out_hit) std::memset(out_hit, 0, sizeof(*out_hit));
+    if (out_hit) ray_miss(out_hit);
     if (!world || !query || !out_hit) { set_err("rphys_world_raycast: null arg"); return RPHYS_ERR_NULL; }
-    set_err("rphys_world_raycast: queries are not part of the step library");
-    return RPHYS_ERR_INTERNAL;
+    return rphys_world_raycast_n(world, query, 1, out_hit);
+}
+RPhysResult rphys_world_capsule_sweep_n(const RPhysWorld *cworld, const RPhysCapsuleSweepQuery *queries, uint32_t n,
+                                        RPhysCapsuleSweepHit *out_hits) {
+    GUARD_BEGIN
+    if (out_hits) for (uint32_t i = 0; i < n; i++) sweep_miss(&out_hits[i]);
+    if (!cworld || (n && (!queries || !out_hits))) { set_err("rphys_world_capsule_sweep: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    std::vector<RpQuery> qs(n);
+    for (uint32_t i = 0; i < n; i++) {
+        RPhysResult r = sweep_query(world, queries[i], qs[i]);
+        if (r != RPHYS_OK) return r;
+    }
+    if (qs.empty() || (world->w.nb() == 0 && !world->w.has_hf)) return RPHYS_OK;
+    std::vector<RpQueryHit> hits;
+    if (!run_queries(world, qs, hits)) return RPHYS_ERR_INTERNAL;
+    for (uint32_t k = 0; k < n; k++) {
+        const RpQueryHit &h = hits[k];
+        if (!h.hit) continue;
+        RPhysCapsuleSweepHit &o = out_hits[k];
+        o.hit = RPHYS_TRUE; o.body_id = hit_body_id(h); o.fraction = std::min(std::max(h.fraction, 0.0f), 1.0f);
+        o.started_overlapping = h.started ? RPHYS_TRUE : RPHYS_FALSE;
+        o.point = {h.point[0], h.point[1], h.point[2]}; o.normal = {h.normal[0], h.normal[1], h.normal[2]};
+    }
+    return RPHYS_OK;
+    GUARD_END
 }
 RPhysResult rphys_world_capsule_sweep(const RPhysWorld *world, const RPhysCapsuleSweepQuery *query, RPhysCapsuleSweepHit *out_hit) {
-    if (out_hit) std::memset(out_hit, 0, sizeof(*out_hit));
+    if (out_hit) sweep_miss(out_hit);
     if (!world || !query || !out_hit) { set_err("rphys_world_capsule_sweep: null arg"); return RPHYS_ERR_NULL; }
-    set_err("rphys_world_capsule_sweep: queries are not part of the step library");
-    return RPHYS_ERR_INTERNAL;
+    return rphys_world_capsule_sweep_n(world, query, 1, out_hit);
 }
 
 RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, uint32_t max_tier1, uint32_t wide_min) {

@@ -201,6 +201,21 @@ struct Dev {
     void *w6_ws;
 };
 
+// A spatial query (k_query) and its answer.
+struct RpQuery {
+    uint32_t kind;    // 0 ray, 1 capsule sweep
+    uint32_t world;   // world index in the scene
+    uint32_t ignore;  // scene body index skipped, or RP_NONE
+    uint32_t pad;
+    float a[4], b[4];  // ray: origin, unit direction; sweep: from, to
+    float max_d, radius, half_height, pad2;
+};
+struct RpQueryHit {
+    uint32_t hit, body, started, pad;  // body: scene index, or RP_TERRAIN
+    float fraction, pad2[3];
+    float point[4], normal[4];
+};
+
 // Stage boundaries of one tick (events recorded by the profiled step; stage k spans marks k..k+1).
 enum {
     RP_ST_PRE = 0,          // S1 forces, AABBs, swept bounds

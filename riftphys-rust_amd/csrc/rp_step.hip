@@ -2467,6 +2467,7 @@ __global__ void k_warm_merge(Dev d, TickParams tp) {
 }
 
 #include "rp_bounce.h"
+#include "rp_query.h"
 
 
 // ------------------------------------------------------------------ S4
@@ -3280,6 +3281,10 @@ extern "C" void rp_launch_copy_probe(const float4 *src, float4 *dst, size_t n, h
 }
 
 extern "C" size_t rp_w6_ws_bytes() { return w6_ws_bytes(); }
+extern "C" hipError_t rp_launch_query(const Dev &d, const RpQuery *qs, RpQueryHit *out, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_query, dim3(n), dim3(RPQ_T), 0, st, d, qs, out);
+    return hipGetLastError();
+}
 extern "C" void rp_launch_restore_backup(const Dev &d, hipStream_t st) {
     hipLaunchKernelGGL(k_restore_backup, dim3(grid_for(d.nb > d.nj ? d.nb : d.nj, 256)), dim3(256), 0, st, d);
 }

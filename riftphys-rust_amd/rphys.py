@@ -82,6 +82,23 @@ class PlayerDesc(C.Structure):
     _fields_ = [("start_pose", Isometry), ("radius", C.c_float), ("height", C.c_float), ("speed", C.c_float)]
 
 
+class RayQuery(C.Structure):
+    _fields_ = [("origin", Vec3), ("dir", Vec3), ("max_distance", C.c_float), ("ignore_body", C.c_uint32)]
+
+
+class RayHit(C.Structure):
+    _fields_ = [("hit", C.c_uint32), ("body_id", C.c_uint32), ("fraction", C.c_float), ("point", Vec3), ("normal", Vec3)]
+
+
+class CapsuleSweepQuery(C.Structure):
+    _fields_ = [("from_", Vec3), ("to", Vec3), ("radius", C.c_float), ("half_height", C.c_float), ("ignore_body", C.c_uint32)]
+
+
+class CapsuleSweepHit(C.Structure):
+    _fields_ = [("hit", C.c_uint32), ("body_id", C.c_uint32), ("fraction", C.c_float), ("started_overlapping", C.c_uint32),
+                ("point", Vec3), ("normal", Vec3)]
+
+
 class StepStats(C.Structure):
     _fields_ = [("pairs", C.c_uint32), ("contacts", C.c_uint32), ("terrain_candidates", C.c_uint32), ("island_contacts", C.c_uint32),
                 ("islands", C.c_uint32), ("warm_entries", C.c_uint32), ("pgs_iters", C.c_uint32), ("pgs_iters_sum", C.c_uint32),
@@ -141,6 +158,9 @@ EXPORTS = [
     ("rphys_world_get_step_hash", U32, [P, C.POINTER(Hash32)]),
     ("rphys_world_step_and_hash", U32, [P, F32, C.POINTER(Hash32)]),
     ("rphys_world_raycast", U32, [P, P, P]),
+    ("rphys_world_capsule_sweep", U32, [P, P, P]),
+    ("rphys_world_raycast_n", U32, [P, P, U32, P]),
+    ("rphys_world_capsule_sweep_n", U32, [P, P, U32, P]),
     ("rphys_world_capsule_sweep", U32, [P, P, P]),
     ("rphys_world_set_heightfield_i16", U32, [P, C.POINTER(HeightfieldDesc), F32, F32, F32]),
     ("rphys_world_set_heightfield_raw32_square", U32, [P, P, U32, F32, F32, F32, F32, F32]),
@@ -348,6 +368,36 @@ class World:
         check(lib().rphys_add_body_density(self.h, shape, d.ctypes.data_as(PF32), C.byref(iso(pose[:3], pose[3:])),
                                            C.byref(vel(v[:3], v[3:])), BODY_DYNAMIC if dynamic else BODY_STATIC, density, C.byref(out)))
         return out.value
+
+    def raycast(self, origin, direction, max_distance, ignore_body=0):
+        """rphys_world_raycast: (hit, body_id, fraction, point, normal)."""
+        q = RayQuery(Vec3(*[float(x) for x in origin]), Vec3(*[float(x) for x in direction]), float(max_distance), int(ignore_body))
+        h = RayHit()
+        check(lib().rphys_world_raycast(self.h, C.byref(q), C.byref(h)))
+        return h
+
+    def raycast_n(self, queries):
+        """rphys_world_raycast_n over [(origin, direction, max_distance, ignore_body)]."""
+        n = len(queries)
+        qa = (RayQuery * n)(*[RayQuery(Vec3(*map(float, o)), Vec3(*map(float, d)), float(m), int(i)) for o, d, m, i in queries])
+        ha = (RayHit * n)()
+        check(lib().rphys_world_raycast_n(self.h, qa, n, ha))
+        return list(ha)
+
+    def capsule_sweep(self, frm, to, radius, half_height, ignore_body=0):
+        """rphys_world_capsule_sweep: (hit, body_id, fraction, started_overlapping, point, normal)."""
+        q = CapsuleSweepQuery(Vec3(*[float(x) for x in frm]), Vec3(*[float(x) for x in to]), float(radius), float(half_height), int(ignore_body))
+        h = CapsuleSweepHit()
+        check(lib().rphys_world_capsule_sweep(self.h, C.byref(q), C.byref(h)))
+        return h
+
+    def capsule_sweep_n(self, queries):
+        n = len(queries)
+        qa = (CapsuleSweepQuery * n)(*[CapsuleSweepQuery(Vec3(*map(float, f)), Vec3(*map(float, t)), float(r), float(hh), int(i))
+                                       for f, t, r, hh, i in queries])
+        ha = (CapsuleSweepHit * n)()
+        check(lib().rphys_world_capsule_sweep_n(self.h, qa, n, ha))
+        return list(ha)
 
     def add_body_spec(self, b):
         """One body from a dict of the scene fields (shape, dims, pose, vel, dynamic, density,

@@ -24,6 +24,11 @@ class OrContact(C.Structure):
                 ("depth", C.c_float), ("impulse", C.c_float), ("feat", C.c_uint32), ("cell", C.c_uint32)]
 
 
+class OrQueryHit(C.Structure):
+    _fields_ = [("hit", C.c_int), ("body", C.c_uint32), ("fraction", C.c_float), ("point", C.c_float * 3), ("normal", C.c_float * 3),
+                ("started", C.c_int)]
+
+
 _lib = None
 
 
@@ -65,6 +70,10 @@ def lib():
         L.or_set_body_vel.argtypes = [P, C.c_uint32, FP]
         L.or_set_sleep_allowed.argtypes = [P, C.c_uint32, C.c_int]
         L.or_set_body_material.argtypes = [P, C.c_uint32, C.c_uint32]
+        L.or_raycast.argtypes = [P, FP, FP, C.c_float, C.c_int32, C.POINTER(OrQueryHit)]
+        L.or_raycast.restype = C.c_int
+        L.or_capsule_sweep.argtypes = [P, FP, FP, C.c_float, C.c_float, C.c_int32, C.POINTER(OrQueryHit)]
+        L.or_capsule_sweep.restype = C.c_int
         DP = C.POINTER(C.c_double)
         L.or_bn_min_degree.restype = C.c_uint32
         L.or_bn_min_degree.argtypes = [C.c_uint32, P32_, P32_, P32_]
@@ -192,6 +201,25 @@ class OracleWorld:
         if not b.get("sleep_allowed", 1):
             L.or_set_sleep_allowed(self.h, i, 0)
         return i
+
+    def raycast(self, origin, direction, max_distance, ignore=-1):
+        """World::raycast with the direction normalised as rphys_world_raycast does (ignore: a body
+        index or -1)."""
+        d = np.asarray(direction, np.float32)
+        l2 = np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+        h = OrQueryHit()
+        if not max_distance > 0 or not l2 > np.float32(1e-20):
+            return h
+        d = d / np.sqrt(l2)
+        lib().or_raycast(self.h, fptr(np.asarray(origin, np.float32)), fptr(np.ascontiguousarray(d, np.float32)), float(max_distance),
+                         int(ignore), C.byref(h))
+        return h
+
+    def capsule_sweep(self, frm, to, radius, half_height, ignore=-1):
+        h = OrQueryHit()
+        lib().or_capsule_sweep(self.h, fptr(np.asarray(frm, np.float32)), fptr(np.asarray(to, np.float32)), float(radius), float(half_height),
+                               int(ignore), C.byref(h))
+        return h
 
     def step(self, dt=None):
         lib().or_step(self.h, self.dt if dt is None else dt)

@@ -201,14 +201,50 @@ def test_last_error_is_thread_local():
     assert buf.value == last_error().encode()[:7]
 
 
-def test_queries_are_declared_out_of_scope():
+def test_query_argument_checks():
+    """lib.rs:1270-1410: null -> ERR_NULL, non-finite input -> ERR_BAD_ARG, a dead or sentinel
+    ignore_body -> ERR_INVALID_ID, max_distance <= 0 or a zero direction -> a miss; outputs are the
+    miss record on every failure.  (None of these reach the device.)"""
     L = rphys.lib()
-    w = L.rphys_world_create(1, 1)
-    hit = (C.c_uint8 * 40)()
-    q = (C.c_uint8 * 32)()
-    assert L.rphys_world_raycast(w, q, hit) == rphys.ERR_INTERNAL
-    assert "not part" in last_error()
-    L.rphys_world_destroy(w)
+    w = L.rphys_world_create(4, 4)
+    try:
+        nan = float("nan")
+        hit = rphys.RayHit()
+        q = rphys.RayQuery(rphys.Vec3(0, 1, 0), rphys.Vec3(0, -1, 0), 5.0, 0)
+        assert L.rphys_world_raycast(None, C.byref(q), C.byref(hit)) == rphys.ERR_NULL
+        assert L.rphys_world_raycast(w, None, C.byref(hit)) == rphys.ERR_NULL
+        assert (hit.hit, hit.body_id, hit.fraction, hit.normal.y) == (0, 0, 1.0, 1.0)
+        for bad in (rphys.RayQuery(rphys.Vec3(nan, 1, 0), rphys.Vec3(0, -1, 0), 5.0, 0),
+                    rphys.RayQuery(rphys.Vec3(0, 1, 0), rphys.Vec3(0, float("inf"), 0), 5.0, 0),
+                    rphys.RayQuery(rphys.Vec3(0, 1, 0), rphys.Vec3(0, -1, 0), nan, 0)):
+            assert L.rphys_world_raycast(w, C.byref(bad), C.byref(hit)) == rphys.ERR_BAD_ARG
+            assert hit.hit == 0 and hit.fraction == 1.0
+        for ig in (7, rphys.TERRAIN_BODY_ID):
+            q.ignore_body = ig
+            assert L.rphys_world_raycast(w, C.byref(q), C.byref(hit)) == rphys.ERR_INVALID_ID
+        q.ignore_body = 0
+        q.max_distance = 0.0
+        assert L.rphys_world_raycast(w, C.byref(q), C.byref(hit)) == rphys.OK and hit.hit == 0
+        q.max_distance, q.dir = 5.0, rphys.Vec3(0, 0, 0)
+        assert L.rphys_world_raycast(w, C.byref(q), C.byref(hit)) == rphys.OK and hit.hit == 0
+        sh = rphys.CapsuleSweepHit()
+        sq = rphys.CapsuleSweepQuery(rphys.Vec3(0, 1, 0), rphys.Vec3(0, 0, 0), 0.3, 0.5, 0)
+        assert L.rphys_world_capsule_sweep(w, None, C.byref(sh)) == rphys.ERR_NULL
+        for r, hh in ((0.0, 0.5), (-1.0, 0.5), (nan, 0.5), (0.3, -0.1), (0.3, nan)):
+            sq.radius, sq.half_height = r, hh
+            assert L.rphys_world_capsule_sweep(w, C.byref(sq), C.byref(sh)) == rphys.ERR_BAD_ARG
+            assert (sh.hit, sh.fraction, sh.started_overlapping) == (0, 1.0, 0)
+        sq.radius, sq.half_height, sq.ignore_body = 0.3, 0.5, 3
+        assert L.rphys_world_capsule_sweep(w, C.byref(sq), C.byref(sh)) == rphys.ERR_INVALID_ID
+        sq.ignore_body = 0
+        # an empty world with no terrain: nothing to hit, answered without the device
+        assert L.rphys_world_capsule_sweep(w, C.byref(sq), C.byref(sh)) == rphys.OK and sh.hit == 0
+        qs = (rphys.RayQuery * 2)(q, rphys.RayQuery(rphys.Vec3(0, 1, 0), rphys.Vec3(nan, 0, 0), 1.0, 0))
+        hs = (rphys.RayHit * 2)()
+        assert L.rphys_world_raycast_n(w, qs, 2, hs) == rphys.ERR_BAD_ARG
+        assert hs[0].hit == 0 and hs[1].hit == 0
+    finally:
+        L.rphys_world_destroy(w)
 
 
 def test_ball_joint_export_validates_its_arguments():

@@ -137,6 +137,7 @@ enum { J_BALL = 0, J_DIST = 1 };
 typedef struct { uint32_t a, b; V3 fap, fbp; Q4 fra, frb; int kind; float rest, compliance, lam; } Joint;
 /* A joint's warm rows (world.rs:250 warm_joints): row tags and impulses of its last solve. */
 typedef struct { uint8_t has, n, tag[3]; float lam[3]; } WarmJ;
+typedef struct { uint32_t body; float radius, height, speed, vvel; int grounded; V3 input; } OrPlayer;
 
 struct OrWorld {
     uint32_t cap, n;
@@ -159,6 +160,8 @@ struct OrWorld {
     uint8_t *loaded; int loaded_valid;
     /* W6 pseudo-velocities of the last solve (linear, angular), used by S4 when has_pseudo */
     V3 *pl, *pa; int has_pseudo;
+    /* kinematic players (world.rs:44-69 PlayerController) */
+    OrPlayer *players; uint32_t np, capp;
     OrStats stats;
 };
 
@@ -187,7 +190,7 @@ void or_world_destroy(OrWorld *w) {
     if (!w) return;
     free(w->joints); free(w->wj);
     free(w->pos); free(w->rot); free(w->lin); free(w->ang); free(w->inv_mass); free(w->dyn); free(w->iil);
-    free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow); free(w->pl); free(w->pa);
+    free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow); free(w->pl); free(w->pa); free(w->players);
     free(w->cbody); free(w->cshape); free(w->cmat); free(w->caabb); free(w->loaded);
     free(w->hts); free(w->warm); free(w->pairs); free(w->contacts);
     free(w);
@@ -1662,12 +1665,100 @@ static void project(OrWorld *w, const PosC *pc, uint32_t npc, V3 *slin, V3 *sang
 }
 
 /* ------------------------------------------------------------------ the step */
+/* ---- kinematic players: World::update_players (step.rs:544-639), run first in World::step
+ * (step.rs:51).  Each player in order: gravity or the sticky floor, the input direction at its
+ * speed, then up to four resolution passes of the capsule (radius, height / 2) at the proposed
+ * position: the first terrain triangle within 0.05 m, else the first collider (other than its own
+ * body) whose stored box meets the capsule's box and whose shape is within 0.05 m; push out along
+ * the contact normal by the penetration + 1 mm, drop the part of the move into the normal, kick a
+ * dynamic body hit (wake, impulse -10 n); a normal with y > 0.6 grounds the player.  The stored
+ * boxes are those before any player moved (the step refreshes them only at its end). */
+static int q_hf_range(const OrWorld *w, Aabb bb, int *gx0, int *gx1, int *gz0, int *gz1);
+static void q_hf_tris(const OrWorld *w, int gx, int gz, Tri out[2]);
+static void update_players(OrWorld *w, float dt) {
+    if (!w->np) return;
+    for (uint32_t c = 0; c < w->nc; c++) { uint32_t b = w->cbody[c]; w->caabb[c] = aabb_of(w->cshape[c], w->pos[b], w->rot[b]); }
+    for (uint32_t pi = 0; pi < w->np; pi++) {
+        OrPlayer *pl = &w->players[pi];
+        V3 pos = w->pos[pl->body];
+        if (!pl->grounded) pl->vvel -= 9.81f * dt;
+        else if (pl->vvel <= 0.0f) pl->vvel = -0.5f;
+        float m2 = vlen2(pl->input);
+        V3 dir = m2 > 1.0e-6f ? vdivs(pl->input, sqrtf(m2)) : v3s(0);
+        V3 desired = vadd(vmul(vmul(dir, pl->speed), dt), v3(0.0f, pl->vvel * dt, 0.0f));
+        pl->grounded = 0;
+        ShapeD cap; cap.type = OR_CAPSULE; cap.d[0] = pl->radius; cap.d[1] = pl->height * 0.5f; cap.d[2] = 0.0f;
+        Q4 qi = qident();
+        for (int it = 0; it < 4; it++) {
+            if (vlen2(desired) < 1.0e-6f) break;
+            V3 xp = vadd(pos, desired);
+            Aabb bb = aabb_of(cap, xp, qi);
+            int hit = 0; V3 n = v3s(0); float pen = 0.0f; int32_t hb = -1;
+            int gx0, gx1, gz0, gz1;
+            if (q_hf_range(w, bb, &gx0, &gx1, &gz0, &gz1))
+                for (int gz = gz0; gz <= gz1 && !hit; gz++)
+                    for (int gx = gx0; gx <= gx1 && !hit; gx++) {
+                        Tri tr[2];
+                        q_hf_tris(w, gx, gz, tr);
+                        for (int k = 0; k < 2 && !hit; k++) {
+                            Manifold m;
+                            if (collide_shape_tri(cap, xp, qi, tr[k], 0.05f, &m)) { hit = 1; n = m.p[0].n; pen = m.p[0].pen; }
+                        }
+                    }
+            if (!hit)
+                for (uint32_t c = 0; c < w->nc && !hit; c++) {
+                    uint32_t b = w->cbody[c];
+                    const Aabb *cb = &w->caabb[c];  /* Aabb::intersects (aabb.rs:75-79) */
+                    if (b == pl->body || !(bb.min.x <= cb->max.x && bb.max.x >= cb->min.x && bb.min.y <= cb->max.y &&
+                                           bb.max.y >= cb->min.y && bb.min.z <= cb->max.z && bb.max.z >= cb->min.z)) continue;
+                    Manifold m;
+                    if (collide_shapes(cap, xp, qi, w->cshape[c], w->pos[b], w->rot[b], 0.05f, &m)) { hit = 1; n = m.p[0].n; pen = m.p[0].pen; hb = (int32_t)b; }
+                }
+            if (!hit) break;
+            if (n.y > 0.6f) { pl->grounded = 1; pl->vvel = 0.0f; }
+            desired = vadd(desired, vmul(n, pen + 0.001f));
+            float dd = vdot(desired, n);
+            if (dd < 0.0f) desired = vsub(desired, vmul(n, dd));
+            if (hb >= 0 && w->dyn[hb]) {
+                w->awake[hb] = 1; w->sticks[hb] = 0;
+                apply_impulse(w, (uint32_t)hb, vmul(vneg(n), 10.0f));
+            }
+        }
+        pos = vadd(pos, desired);
+        b_set_pose(w, pl->body, pos, qident());
+    }
+}
+
+/* world.rs:466-475 add_player with the FFI's dimensions (lib.rs:1580-1626): a static body with the
+ * controller's default capsule collider (r 0.4, half height 0.9, material Default); the controller
+ * then takes the given radius, height and speed.  Returns the player index, or -1. */
+int32_t or_add_player(OrWorld *w, const float *pose, float radius, float height, float speed) {
+    float dims[3] = {0.4f, 1.8f * 0.5f, 0.0f}, vel[6] = {0, 0, 0, 0, 0, 0};
+    int32_t b = add_body(w, OR_CAPSULE, dims, pose, vel, 0, 0.0f, 1);
+    if (b < 0) return -1;
+    if (w->np == w->capp) { w->capp = w->capp ? 2 * w->capp : 4; w->players = xrealloc(w->players, sizeof(OrPlayer) * w->capp); }
+    OrPlayer *pl = &w->players[w->np];
+    memset(pl, 0, sizeof(*pl));
+    pl->body = (uint32_t)b; pl->radius = radius; pl->height = height; pl->speed = speed;
+    return (int32_t)w->np++;
+}
+void or_player_set_input(OrWorld *w, uint32_t idx, const float *dir3) { if (idx < w->np) w->players[idx].input = v3(dir3[0], dir3[1], dir3[2]); }
+/* pose {p, q}, grounded; returns the player's body index */
+uint32_t or_player_get(const OrWorld *w, uint32_t idx, float *pose7, int *grounded, float *vvel) {
+    const OrPlayer *pl = &w->players[idx];
+    V3 p = w->pos[pl->body]; Q4 q = w->rot[pl->body];
+    pose7[0] = p.x; pose7[1] = p.y; pose7[2] = p.z; pose7[3] = q.x; pose7[4] = q.y; pose7[5] = q.z; pose7[6] = q.w;
+    *grounded = pl->grounded; *vvel = pl->vvel;
+    return pl->body;
+}
+
 void or_step(OrWorld *w, float dt) {
     uint32_t count = w->n;
     w->tick += 1;
     memset(&w->stats, 0, sizeof(w->stats));
     w->loaded_valid = 0;
     w->has_pseudo = 0;
+    update_players(w, dt);
     /* S1 (step.rs:53-68) */
     V3 *pre_lin = malloc(sizeof(V3) * (count + 1)), *pre_ang = malloc(sizeof(V3) * (count + 1));
     for (uint32_t i = 0; i < count; i++) { pre_lin[i] = w->lin[i]; pre_ang[i] = w->ang[i]; }
@@ -2053,6 +2144,7 @@ typedef struct {
     Warm *warm; uint32_t nwarm;
     V3 *last_lin; uint8_t *awake; uint16_t *sticks;
     WarmJ *wj; uint32_t nj;
+    OrPlayer *players; uint32_t np;
 } Snap;
 void *or_snapshot(const OrWorld *w) {
     Snap *s = calloc(1, sizeof(Snap));
@@ -2068,11 +2160,15 @@ void *or_snapshot(const OrWorld *w) {
     s->awake = malloc(w->cap + 1); memcpy(s->awake, w->awake, w->cap);
     s->sticks = malloc(2 * (w->cap + 1)); memcpy(s->sticks, w->sticks, 2 * w->cap);
     s->nj = w->nj; s->wj = malloc(sizeof(WarmJ) * (w->nj + 1)); memcpy(s->wj, w->wj, sizeof(WarmJ) * w->nj);
+    s->np = w->np; s->players = malloc(sizeof(OrPlayer) * (w->np + 1)); memcpy(s->players, w->players, sizeof(OrPlayer) * w->np);
     return s;
 }
 void or_restore(OrWorld *w, const void *sp) {
     const Snap *s = sp;
     w->tick = s->tick;
+    /* lib.rs:1760-1761: the players as snapshot (players added since are dropped) */
+    w->np = s->np < w->np ? s->np : w->np;
+    memcpy(w->players, s->players, sizeof(OrPlayer) * w->np);
     for (uint32_t i = 0; i < s->n && i < w->n; i++) {
         if (!w->alive[i]) continue;
         float pose[7] = {s->pos[i].x, s->pos[i].y, s->pos[i].z, s->rot[i].x, s->rot[i].y, s->rot[i].z, s->rot[i].w};
@@ -2102,6 +2198,7 @@ void or_snapshot_free(void *sp) {
     Snap *s = sp;
     if (!s) return;
     free(s->pos); free(s->rot); free(s->lin); free(s->ang); free(s->warm); free(s->last_lin); free(s->awake); free(s->sticks); free(s->wj);
+    free(s->players);
     free(s);
 }
 

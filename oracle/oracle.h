@@ -152,6 +152,12 @@ int or_raycast(const OrWorld *w, const float *origin3, const float *dir3, float 
 int or_capsule_sweep(const OrWorld *w, const float *from3, const float *to3, float radius, float half_height, int32_t ignore,
                      OrQueryHit *out);
 
+/* Kinematic players (world.rs:466-475, step.rs:544-639, lib.rs:1580-1693): add returns the player
+ * index (-1 when the world is full); get writes the pose and grounded flag and returns the body. */
+int32_t or_add_player(OrWorld *w, const float *pose, float radius, float height, float speed);
+void or_player_set_input(OrWorld *w, uint32_t idx, const float *dir3);
+uint32_t or_player_get(const OrWorld *w, uint32_t idx, float *pose7, int *grounded, float *vvel);
+
 /* W6 numerics (bounce_lcp.rs), for the pin tests: K symmetric CSR, n <= 250.  min_degree writes
  * the elimination order and returns the factor's entries below the diagonal; pseudo_solve solves
  * (K + CFM diag K) x = u; bounce_qp solves the push-only LCP and returns 1 if converged. */

@@ -143,6 +143,10 @@ struct HostWorld {
     std::vector<F4> jt_fap, jt_far, jt_fbp, jt_fbr;
     std::vector<uint32_t> wj_mask;
     std::vector<float> wj_lam;  // 3 per joint
+    // kinematic players (world.rs:44-69): body index, radius, height, speed, vertical velocity,
+    // grounded, input direction
+    struct Player { uint32_t body; float radius, height, speed, vvel; uint32_t grounded; float input[3]; };
+    std::vector<Player> players;
     uint32_t nb() const { return (uint32_t)pos.size(); }
     uint32_t nc() const { return (uint32_t)col_body.size(); }
     uint32_t nj() const { return (uint32_t)jt_a.size(); }
@@ -180,6 +184,7 @@ struct Slot {  // device-side snapshot of a scene
     uint32_t *awake, *sticks;
     uint64_t *wkey; float4 *wla; float *wjn; float4 *wjt; uint32_t *woff;
     uint32_t *wj_mask; float *wj_lam;
+    float4 *pl_state, *pl_input; uint32_t *pl_grounded;
     uint32_t cur; uint32_t nwarm;
     size_t cap_warm = 0;  // warm-cache entries the slot's buffers hold
     std::vector<uint64_t> ticks;
@@ -322,6 +327,14 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     if (d.bp_radix) ok &= B.alloc(d.bp_k0, NC) && B.alloc(d.bp_v0, NC) && B.alloc(d.bp_k1, NC) && B.alloc(d.bp_v1, NC);
     ok &= B.alloc(d.rs_hist, 256 * (size_t)d.rs_blocks + 1);
     ok &= B.alloc(d.pc_la, C) && B.alloc(d.pc_lb, C);
+    uint32_t np = 0;
+    for (auto *w : ws) np += (uint32_t)w->players.size();
+    d.np = np;
+    ok &= B.alloc(d.w_player_off, nw + 1);
+    if (np) {
+        ok &= B.alloc(d.pl_body, np) && B.alloc(d.pl_grounded, np) && B.alloc(d.bk_pl_grounded, np) && B.alloc(d.pl_of_body, NB) &&
+              B.alloc(d.pl_state, np) && B.alloc(d.bk_pl_state, np) && B.alloc(d.pl_input, np) && B.alloc(d.pl_bmin, np) && B.alloc(d.pl_bmax, np);
+    }
     if (scene_bounces(ws)) {
         d.w6_on = 1;
         d.w6_slots = std::min<uint32_t>(nw, 32u);
@@ -449,6 +462,23 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
          up(d.wj_mask, jmask.data(), jmask.size() * 4) && up(d.wj_lam, jlam.data(), jlam.size() * 4) &&
          up(d.jadj_off, jadj_off.data(), jadj_off.size() * 4) && up(d.jadj, jadj.data(), jadj.size() * 4) &&
          up(d.w_joint_off, jwoff.data(), jwoff.size() * 4);
+    {
+        std::vector<uint32_t> poff{0}, pbody, pgnd, pof(nb + 1, RP_NONE);
+        std::vector<F4> pst, pin;
+        for (size_t wi = 0; wi < ws.size(); wi++) {
+            for (auto &pl : ws[wi]->players) {
+                const uint32_t gb = S->body_off[wi] + pl.body;
+                pof[gb] = (uint32_t)pbody.size();
+                pbody.push_back(gb); pgnd.push_back(pl.grounded);
+                pst.push_back(f4(pl.radius, pl.height, pl.speed, pl.vvel));
+                pin.push_back(f4(pl.input[0], pl.input[1], pl.input[2]));
+            }
+            poff.push_back((uint32_t)pbody.size());
+        }
+        ok = ok && up(d.w_player_off, poff.data(), poff.size() * 4);
+        if (np) ok = ok && up(d.pl_body, pbody.data(), np * 4) && up(d.pl_grounded, pgnd.data(), np * 4) && up(d.pl_of_body, pof.data(), pof.size() * 4) &&
+                     up(d.pl_state, pst.data(), np * 16) && up(d.pl_input, pin.data(), np * 16);
+    }
     if (ok) ok = hip_ok(hipMemsetAsync(d.cnt, 0, CT_COUNT * 4, S->st), "memset");
     if (ok && ws[0]->mlp.size() == RP_MLP_NPARAM) ok = upload_model(*S, ws[0]->mlp.data());
     S->col_body = cbody;
@@ -477,6 +507,16 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
         uint32_t joff[2];
         ok = dn(joff, d.w_joint_off + wi, 8) && hip_ok(hipStreamSynchronize(S.st), "download sync") &&
              dn(w.wj_mask.data(), d.wj_mask + joff[0], w.nj() * 4) && dn(w.wj_lam.data(), d.wj_lam + 3 * (size_t)joff[0], 3 * (size_t)w.nj() * 4);
+    }
+    if (ok && !w.players.empty() && d.np) {
+        uint32_t poff = 0;
+        const size_t np = w.players.size();
+        std::vector<F4> st(np);
+        std::vector<uint32_t> gnd(np);
+        ok = dn(&poff, d.w_player_off + wi, 4) && hip_ok(hipStreamSynchronize(S.st), "download sync") &&
+             dn(st.data(), d.pl_state + poff, np * 16) && dn(gnd.data(), d.pl_grounded + poff, np * 4) &&
+             hip_ok(hipStreamSynchronize(S.st), "download sync");
+        for (size_t k = 0; ok && k < np; k++) { w.players[k].vvel = st[k].w; w.players[k].grounded = gnd[k]; }
     }
     return ok && hip_ok(hipStreamSynchronize(S.st), "download sync");
 }
@@ -1395,27 +1435,78 @@ RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uin
 }
 
 // lib.rs:1580-1693 player controller: argument checks as the reference, then out of scope
+// Kinematic players (lib.rs:1580-1693).  add_player (world.rs:466-475): a static body with the
+// controller's default capsule (r 0.4, half height 0.9, material Default) consuming one body and
+// one collider; the controller then takes the described radius, height and speed.  The device
+// moves players at the start of each tick (k_players); input changes go straight to the device.
 RPhysResult rphys_add_player(RPhysWorld *world, const RPhysPlayerDesc *desc, uint32_t *out_player_idx) {
+    GUARD_BEGIN
     if (out_player_idx) *out_player_idx = 0xFFFFFFFFu;
     if (!world || !desc || !out_player_idx) { set_err("rphys_add_player: null arg"); return RPHYS_ERR_NULL; }
     if (!fin_iso(desc->start_pose)) return bad_arg("rphys_add_player", "pose contains NaN/Inf");
     if (!(hfin(desc->radius) && desc->radius > 0.0f)) return bad_arg("rphys_add_player", "radius must be > 0");
     if (!(hfin(desc->height) && desc->height > 0.0f)) return bad_arg("rphys_add_player", "height must be > 0");
-    set_err("rphys_add_player: the kinematic player controller is not part of the step library");
-    return RPHYS_ERR_INTERNAL;
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (world->w.bodies_created + 1 > world->w.max_bodies || world->w.nc() + 1 > world->w.max_colliders) {
+        set_err("rphys_add_player: body capacity exceeded");
+        return RPHYS_ERR_CAPACITY;
+    }
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    HostWorld &w = world->w;
+    const float dims[3] = {0.4f, 1.8f * 0.5f, 0.0f};
+    const RPhysVelocity zero{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    add_body_host(w, 2u, dims, desc->start_pose, zero, false, 0.0f);
+    HostWorld::Player pl{};
+    pl.body = w.nb() - 1; pl.radius = desc->radius; pl.height = desc->height; pl.speed = desc->speed;
+    w.players.push_back(pl);
+    host_changed(world);
+    *out_player_idx = (uint32_t)w.players.size() - 1;
+    return RPHYS_OK;
+    GUARD_END
 }
 RPhysResult rphys_player_set_input(RPhysWorld *world, uint32_t player_idx, RPhysVec3 move_dir) {
-    (void)player_idx;
+    GUARD_BEGIN
     if (!world) { set_err("rphys_player_set_input: null arg"); return RPHYS_ERR_NULL; }
     if (!fin3(move_dir)) return bad_arg("rphys_player_set_input", "move_dir NaN/Inf");
-    return bad_arg("rphys_player_set_input", "player_idx out of bounds");  // no player can exist
+    std::lock_guard<std::mutex> lk(world->mu);
+    HostWorld &w = world->w;
+    if (player_idx >= w.players.size()) return bad_arg("rphys_player_set_input", "player_idx out of bounds");
+    HostWorld::Player &pl = w.players[player_idx];
+    pl.input[0] = move_dir.x; pl.input[1] = move_dir.y; pl.input[2] = move_dir.z;
+    if (world->scene && !world->need_build && world->scene->d.np > player_idx) {
+        const F4 in = f4(move_dir.x, move_dir.y, move_dir.z);
+        Scene &S = *world->scene;
+        if (!hip_ok(hipMemcpyAsync(S.d.pl_input + player_idx, &in, 16, hipMemcpyHostToDevice, S.st), "player input") ||
+            !hip_ok(hipStreamSynchronize(S.st), "player input"))
+            return RPHYS_ERR_INTERNAL;
+    }
+    return RPHYS_OK;
+    GUARD_END
 }
-RPhysResult rphys_player_get_pose(const RPhysWorld *world, uint32_t player_idx, RPhysIsometry *out_pose, RPhysBool *out_grounded) {
-    (void)player_idx;
+RPhysResult rphys_player_get_pose(const RPhysWorld *cworld, uint32_t player_idx, RPhysIsometry *out_pose, RPhysBool *out_grounded) {
+    GUARD_BEGIN
     if (out_pose) *out_pose = RPhysIsometry{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 1.0f}};
     if (out_grounded) *out_grounded = RPHYS_FALSE;
-    if (!world || !out_pose) { set_err("rphys_player_get_pose: null arg"); return RPHYS_ERR_NULL; }
-    return bad_arg("rphys_player_get_pose", "player_idx out of bounds");
+    if (!cworld || !out_pose) { set_err("rphys_player_get_pose: null arg"); return RPHYS_ERR_NULL; }
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    const HostWorld &w = world->w;
+    if (player_idx >= w.players.size()) return bad_arg("rphys_player_get_pose", "player_idx out of bounds");
+    const uint32_t b = w.players[player_idx].body;
+    F4 p = w.pos[b], q = w.rot[b];
+    uint32_t gnd = w.players[player_idx].grounded;
+    if (world->dev_newer && world->scene) {  // the device is ahead of the host mirror: read this player alone
+        Scene &S = *world->scene;
+        if (!hip_ok(hipMemcpyAsync(&p, S.d.pos + b, 16, hipMemcpyDeviceToHost, S.st), "player pose") ||
+            !hip_ok(hipMemcpyAsync(&q, S.d.rot + b, 16, hipMemcpyDeviceToHost, S.st), "player pose") ||
+            !hip_ok(hipMemcpyAsync(&gnd, S.d.pl_grounded + player_idx, 4, hipMemcpyDeviceToHost, S.st), "player pose") ||
+            !hip_ok(hipStreamSynchronize(S.st), "player pose"))
+            return RPHYS_ERR_INTERNAL;
+    }
+    *out_pose = RPhysIsometry{{p.x, p.y, p.z}, {q.x, q.y, q.z, q.w}};
+    if (out_grounded) *out_grounded = gnd ? RPHYS_TRUE : RPHYS_FALSE;
+    return RPHYS_OK;
+    GUARD_END
 }
 
 // terrain.rs:24-52 from_heights, material Grit (lib.rs:1479,1536)
@@ -1731,6 +1822,7 @@ struct RPhysWorldSnapshot {
     std::vector<float> wjn;
     std::vector<uint32_t> wj_mask;
     std::vector<float> wj_lam;
+    std::vector<HostWorld::Player> players;
 };
 RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
     clear_err();
@@ -1748,6 +1840,7 @@ RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
         s->last_lin = w.last_lin; s->awake = w.awake; s->sticks = w.sticks;
         s->wkey = w.wkey; s->wla = w.wla; s->wjn = w.wjn; s->wjt = w.wjt;
         s->wj_mask = w.wj_mask; s->wj_lam = w.wj_lam;
+        s->players = w.players;
         return s;
     } catch (...) {
         set_err("panic in rphys_world_snapshot_create");
@@ -1761,6 +1854,7 @@ RPhysResult rphys_world_snapshot_restore(RPhysWorld *world, const RPhysWorldSnap
     if (!pull(world)) return RPHYS_ERR_INTERNAL;
     HostWorld &w = world->w;
     w.tick = s->tick;
+    w.players = s->players;  // lib.rs:1761: the players as snapshot
     for (uint32_t i = 0; i < s->pos.size() && i < w.nb(); i++) {
         if (!(w.bflags[i] & BF_ALIVE)) continue;
         // set_body_pose (world.rs:501-506) then set_body_vel (world.rs:674-676)
@@ -1904,7 +1998,8 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
         bool ok = sl->buf.alloc(sl->pos, NB) && sl->buf.alloc(sl->rot, NB) && sl->buf.alloc(sl->lin, NB) && sl->buf.alloc(sl->ang, NB) &&
                   sl->buf.alloc(sl->last, NB) && sl->buf.alloc(sl->awake, NB) && sl->buf.alloc(sl->sticks, NB) &&
                   sl->buf.alloc(sl->wkey, W) && sl->buf.alloc(sl->wla, W) && sl->buf.alloc(sl->wjn, W) && sl->buf.alloc(sl->wjt, W) &&
-                  sl->buf.alloc(sl->woff, d.nw + 1) && sl->buf.alloc(sl->wj_mask, d.nj + 1) && sl->buf.alloc(sl->wj_lam, 3 * ((size_t)d.nj + 1));
+                  sl->buf.alloc(sl->woff, d.nw + 1) && sl->buf.alloc(sl->wj_mask, d.nj + 1) && sl->buf.alloc(sl->wj_lam, 3 * ((size_t)d.nj + 1)) &&
+                  sl->buf.alloc(sl->pl_state, d.np + 1) && sl->buf.alloc(sl->pl_input, d.np + 1) && sl->buf.alloc(sl->pl_grounded, d.np + 1);
         if (!ok) { set_err("rphys_batch_snapshot: device allocation failed"); return RPHYS_ERR_INTERNAL; }
         S.slots[slot] = std::move(sl);
     }
@@ -1916,7 +2011,8 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
               cp(sl.wkey, d.wk_key[S.cur], 8 * (size_t)nwarm) && cp(sl.wla, d.wk_la[S.cur], 16 * (size_t)nwarm) &&
               cp(sl.wjn, d.wk_jn[S.cur], 4 * (size_t)nwarm) && cp(sl.wjt, d.wk_jt[S.cur], 16 * (size_t)nwarm) &&
               cp(sl.woff, d.wc_off[S.cur], 4 * ((size_t)d.nw + 1)) && cp(sl.wj_mask, d.wj_mask, 4 * (size_t)d.nj) &&
-              cp(sl.wj_lam, d.wj_lam, 12 * (size_t)d.nj);
+              cp(sl.wj_lam, d.wj_lam, 12 * (size_t)d.nj) && cp(sl.pl_state, d.pl_state, 16 * (size_t)d.np) &&
+              cp(sl.pl_input, d.pl_input, 16 * (size_t)d.np) && cp(sl.pl_grounded, d.pl_grounded, 4 * (size_t)d.np);
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
     sl.cur = S.cur; sl.nwarm = nwarm; sl.valid = true;
     sl.ticks.clear();
@@ -1945,7 +2041,8 @@ static RPhysResult batch_restore(RPhysBatch *b, uint32_t slot) {
               cp(d.wk_key[sl.cur], sl.wkey, 8 * (size_t)sl.nwarm) && cp(d.wk_la[sl.cur], sl.wla, 16 * (size_t)sl.nwarm) &&
               cp(d.wk_jn[sl.cur], sl.wjn, 4 * (size_t)sl.nwarm) && cp(d.wk_jt[sl.cur], sl.wjt, 16 * (size_t)sl.nwarm) &&
               cp(d.wc_off[sl.cur], sl.woff, 4 * ((size_t)d.nw + 1)) && cp(d.wj_mask, sl.wj_mask, 4 * (size_t)d.nj) &&
-              cp(d.wj_lam, sl.wj_lam, 12 * (size_t)d.nj);
+              cp(d.wj_lam, sl.wj_lam, 12 * (size_t)d.nj) && cp(d.pl_state, sl.pl_state, 16 * (size_t)d.np) &&
+              cp(d.pl_input, sl.pl_input, 16 * (size_t)d.np) && cp(d.pl_grounded, sl.pl_grounded, 4 * (size_t)d.np);
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "restore")) return RPHYS_ERR_INTERNAL;
     S.cur = sl.cur;
     for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];

@@ -199,6 +199,13 @@ struct Dev {
     float4 *w6_pl, *w6_pa;
     double *w6_acc;
     void *w6_ws;
+    // kinematic players (world.rs:44-69 PlayerController), world-major: each player's body,
+    // (radius, height, speed, vertical velocity), input direction and grounded flag, the backups
+    // of the state a tick changes (replayed ticks start again from them), each world's first
+    // player, and per body the player it carries (RP_NONE); allocated when the scene has players
+    uint32_t np;
+    uint32_t *pl_body, *pl_grounded, *bk_pl_grounded, *w_player_off, *pl_of_body;
+    float4 *pl_state, *bk_pl_state, *pl_input, *pl_bmin, *pl_bmax;
 };
 
 // A spatial query (k_query) and its answer.

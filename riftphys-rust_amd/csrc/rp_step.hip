@@ -68,8 +68,10 @@ __global__ void k_bodies_pre(Dev d, TickParams tp) {
     if (aborted(d)) return;
     GRID_STRIDE(b, d.nb) {
         float4 p = d.pos[b], q = d.rot[b], l = d.lin[b], a = d.ang[b];
-        d.bk_pos[b] = p; d.bk_rot[b] = q; d.bk_lin[b] = l; d.bk_ang[b] = a;
-        d.bk_last[b] = d.last_lin[b]; d.bk_awake[b] = d.awake[b]; d.bk_sticks[b] = d.sticks[b];
+        if (!d.np) {  // with players the backup is taken before they move (k_player_backup)
+            d.bk_pos[b] = p; d.bk_rot[b] = q; d.bk_lin[b] = l; d.bk_ang[b] = a;
+            d.bk_last[b] = d.last_lin[b]; d.bk_awake[b] = d.awake[b]; d.bk_sticks[b] = d.sticks[b];
+        }
         d.pre_lin[b] = l; d.pre_ang[b] = a;
         d.flags[b] = 0; d.root[b] = b; d.ballf[b] = 0;
         d.slin[b] = make_float4(0, 0, 0, 0); d.sang[b] = make_float4(0, 0, 0, 0);
@@ -2468,6 +2470,7 @@ __global__ void k_warm_merge(Dev d, TickParams tp) {
 
 #include "rp_bounce.h"
 #include "rp_query.h"
+#include "rp_player.h"
 
 
 // ------------------------------------------------------------------ S4
@@ -2847,6 +2850,7 @@ __global__ void k_restore_backup(Dev d) {
         d.wj_mask[q] = d.bk_wj_mask[q];
         for (uint32_t i = 0; i < 3; i++) d.wj_lam[3 * q + i] = d.bk_wj_lam[3 * q + i];
     }
+    GRID_STRIDE(p, d.np) { d.pl_state[p] = d.bk_pl_state[p]; d.pl_grounded[p] = d.bk_pl_grounded[p]; }
 }
 
 // Capacity checks after each sizing scan; sets the abort flag so later kernels return at once.
@@ -2988,6 +2992,10 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     };
     mark(RP_ST_PRE);
     uint32_t *scratch_total = d.cnt + CT_SCRATCH;
+    if (d.np) {  // World::update_players runs first (step.rs:51)
+        hipLaunchKernelGGL(k_player_backup, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
+        hipLaunchKernelGGL(k_players, dim3(d.nw), dim3(RPP_T), 0, st, d, tp);
+    }
     hipLaunchKernelGGL(k_bodies_pre, dim3(grid_for(d.nb > d.nj ? d.nb : d.nj, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_col_aabb, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_BROAD);

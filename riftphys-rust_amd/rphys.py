@@ -369,6 +369,25 @@ class World:
                                            C.byref(vel(v[:3], v[3:])), BODY_DYNAMIC if dynamic else BODY_STATIC, density, C.byref(out)))
         return out.value
 
+    def add_player(self, pose, radius, height, speed):
+        """rphys_add_player: returns the player index."""
+        d = PlayerDesc()
+        d.start_pose = iso(pose[:3], pose[3:])
+        d.radius, d.height, d.speed = float(radius), float(height), float(speed)
+        out = C.c_uint32()
+        check(lib().rphys_add_player(self.h, C.byref(d), C.byref(out)))
+        return out.value
+
+    def player_set_input(self, idx, d):
+        check(lib().rphys_player_set_input(self.h, idx, Vec3(*[float(x) for x in d])))
+
+    def player(self, idx):
+        """rphys_player_get_pose: (pose7, grounded)."""
+        p = Isometry()
+        g = C.c_uint32()
+        check(lib().rphys_player_get_pose(self.h, idx, C.byref(p), C.byref(g)))
+        return np.array([p.pos.x, p.pos.y, p.pos.z, p.rot.x, p.rot.y, p.rot.z, p.rot.w], np.float32), bool(g.value)
+
     def raycast(self, origin, direction, max_distance, ignore_body=0):
         """rphys_world_raycast: (hit, body_id, fraction, point, normal)."""
         q = RayQuery(Vec3(*[float(x) for x in origin]), Vec3(*[float(x) for x in direction]), float(max_distance), int(ignore_body))

@@ -70,6 +70,11 @@ def lib():
         L.or_set_body_vel.argtypes = [P, C.c_uint32, FP]
         L.or_set_sleep_allowed.argtypes = [P, C.c_uint32, C.c_int]
         L.or_set_body_material.argtypes = [P, C.c_uint32, C.c_uint32]
+        L.or_add_player.argtypes = [P, FP, C.c_float, C.c_float, C.c_float]
+        L.or_add_player.restype = C.c_int32
+        L.or_player_set_input.argtypes = [P, C.c_uint32, FP]
+        L.or_player_get.argtypes = [P, C.c_uint32, FP, C.POINTER(C.c_int), FP]
+        L.or_player_get.restype = C.c_uint32
         L.or_raycast.argtypes = [P, FP, FP, C.c_float, C.c_int32, C.POINTER(OrQueryHit)]
         L.or_raycast.restype = C.c_int
         L.or_capsule_sweep.argtypes = [P, FP, FP, C.c_float, C.c_float, C.c_int32, C.POINTER(OrQueryHit)]
@@ -201,6 +206,20 @@ class OracleWorld:
         if not b.get("sleep_allowed", 1):
             L.or_set_sleep_allowed(self.h, i, 0)
         return i
+
+    def add_player(self, pose, radius, height, speed):
+        return lib().or_add_player(self.h, fptr(np.asarray(pose, np.float32)), float(radius), float(height), float(speed))
+
+    def player_set_input(self, idx, d):
+        lib().or_player_set_input(self.h, idx, fptr(np.asarray(d, np.float32)))
+
+    def player(self, idx):
+        """(pose7, grounded, vertical velocity, body index)"""
+        pose = np.zeros(7, np.float32)
+        g = C.c_int()
+        vv = np.zeros(1, np.float32)
+        b = lib().or_player_get(self.h, idx, fptr(pose), C.byref(g), fptr(vv))
+        return pose, bool(g.value), float(vv[0]), b
 
     def raycast(self, origin, direction, max_distance, ignore=-1):
         """World::raycast with the direction normalised as rphys_world_raycast does (ignore: a body

@@ -59,13 +59,13 @@ int main(void) {
     assert C.sizeof(rphys.BoxBodyDesc) == box
 
 
-def test_player_controller_entries_link_and_answer_as_the_reference():
-    """riftphys_c.h:227-243 / lib.rs:1580-1693: the player entries exist (a host written against
-    the reference links), validate as the reference does, and report the controller as outside
-    this library (ERR_INTERNAL), so no player index is ever valid."""
+def test_player_controller_entries_validate_as_the_reference():
+    """riftphys_c.h:227-243 / lib.rs:1580-1693: argument checks, the player index counting from 0,
+    one body and one collider consumed (capacity), get_pose answering from the host mirror before
+    any step.  (No call here reaches the device.)"""
     L = rphys.lib()
     assert C.sizeof(rphys.PlayerDesc) == 40
-    w = L.rphys_world_create(8, 8)
+    w = L.rphys_world_create(2, 2)
     try:
         d = rphys.PlayerDesc()
         d.start_pose = rphys.iso([0, 1, 0], [0, 0, 0, 1])
@@ -78,15 +78,23 @@ def test_player_controller_entries_link_and_answer_as_the_reference():
         d.start_pose.pos.x = float("nan")
         assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.ERR_BAD_ARG
         d.start_pose.pos.x = 0.0
-        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.ERR_INTERNAL
-        assert idx.value == 0xFFFFFFFF and "player" in last_error()
-        assert L.rphys_player_set_input(w, 0, rphys.Vec3(1, 0, 0)) == rphys.ERR_BAD_ARG
+        assert L.rphys_player_set_input(w, 0, rphys.Vec3(1, 0, 0)) == rphys.ERR_BAD_ARG  # no player yet
+        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.OK and idx.value == 0
+        d.start_pose.pos.x = 3.0
+        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.OK and idx.value == 1
+        assert L.rphys_add_player(w, C.byref(d), C.byref(idx)) == rphys.ERR_CAPACITY  # two bodies at most
+        n = C.c_uint32()
+        assert L.rphys_world_num_bodies(w, C.byref(n)) == rphys.OK and n.value == 2
+        assert L.rphys_player_set_input(w, 1, rphys.Vec3(1, 0, 0)) == rphys.OK
+        assert L.rphys_player_set_input(w, 2, rphys.Vec3(1, 0, 0)) == rphys.ERR_BAD_ARG
         assert L.rphys_player_set_input(w, 0, rphys.Vec3(float("inf"), 0, 0)) == rphys.ERR_BAD_ARG
         assert L.rphys_player_set_input(None, 0, rphys.Vec3(1, 0, 0)) == rphys.ERR_NULL
         pose = rphys.Isometry()
-        pose.pos.x = 5.0
         g = C.c_uint32(9)
-        assert L.rphys_player_get_pose(w, 0, C.byref(pose), C.byref(g)) == rphys.ERR_BAD_ARG
+        assert L.rphys_player_get_pose(w, 1, C.byref(pose), C.byref(g)) == rphys.OK
+        assert (pose.pos.x, pose.pos.y, pose.rot.w, g.value) == (3.0, 1.0, 1.0, 0)
+        pose.pos.x = 5.0
+        assert L.rphys_player_get_pose(w, 2, C.byref(pose), C.byref(g)) == rphys.ERR_BAD_ARG
         assert (pose.pos.x, pose.rot.w, g.value) == (0.0, 1.0, 0)
         assert L.rphys_player_get_pose(w, 0, None, None) == rphys.ERR_NULL
     finally:

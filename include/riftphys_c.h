@@ -126,13 +126,15 @@ RPhysResult rphys_world_capsule_sweep_n(const RPhysWorld *world, const RPhysCaps
 
 RPhysResult rphys_last_error_copy(char *dst, uint32_t cap, uint32_t *out_len);   /* h:225, lib.rs:561-604 */
 
-/* h:227-243, lib.rs:1580-1693: the kinematic player controller is outside this library's scope.
- * The entries exist so that a host written against the reference links unchanged.
+/* h:227-243, lib.rs:1580-1693: the kinematic player controller (World::add_player,
+ * World::update_players; the players move on the device as the first stage of every tick).
  * rphys_add_player validates its arguments as the reference does (out index set to 0xFFFFFFFF
- * first; null -> ERR_NULL; non-finite pose, radius <= 0 or height <= 0 -> ERR_BAD_ARG), then
- * returns RPHYS_ERR_INTERNAL with a message.  With no player ever added, set_input and get_pose
- * answer as the reference does for any index then: ERR_NULL for a null world (or out_pose),
- * ERR_BAD_ARG "player_idx out of bounds", with the identity pose and RPHYS_FALSE written first. */
+ * first; null -> ERR_NULL; non-finite pose, radius <= 0 or height <= 0 -> ERR_BAD_ARG) and adds
+ * the player's body (infinite mass, not dynamic, default capsule collider) and controller; players are indexed
+ * 0, 1, ... per world.  set_input stores the move direction (the tick normalises it when its
+ * squared length exceeds 1e-6, else uses zero); get_pose returns the controller's pose and grounded flag.  Both answer an
+ * unknown index with ERR_BAD_ARG "player_idx out of bounds", the identity pose and RPHYS_FALSE
+ * written first. */
 typedef struct RPhysPlayerDesc { RPhysIsometry start_pose; float radius; float height; float speed; } RPhysPlayerDesc;
 RPhysResult rphys_add_player(RPhysWorld *world, const RPhysPlayerDesc *desc, uint32_t *out_player_idx);
 RPhysResult rphys_player_set_input(RPhysWorld *world, uint32_t player_idx, RPhysVec3 move_dir);
@@ -386,6 +388,12 @@ RPhysResult rphys_mlp_gradient(const float *params, const float *feat12, const f
  * overflow-and-replay path runs on small scenes. */
 RPhysResult rphys_debug_set_sweep_tiers(uint32_t max_tier0, uint32_t max_tier3, uint32_t max_tier1, uint32_t wide_min);
 RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uint32_t contacts, uint32_t warm);
+/* Test hook (no reference counterpart), process-wide: the next n device allocations the library
+ * makes for scenes and snapshot slots fail as out of memory (0 clears), so a host can force the
+ * device-failure path: the call that needed them returns RPHYS_ERR_INTERNAL with a message, and
+ * the world or batch keeps its last good state.  Returns the allocations still to fail through
+ * out_pending (may be null). */
+RPhysResult rphys_debug_fail_device_allocs(uint32_t n, uint32_t *out_pending);
 /* Test hook (no reference counterpart), process-wide: how sweep tier 0 solves its islands'
  * velocities.  Islands of up to lane_max entries one island per lane (0 = default, 150;
  * RPHYS_LANE_NONE: none, and tier 0 solves every island a wave each), then up to pair_max by a

@@ -165,12 +165,20 @@ static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
 }
 
 // ------------------------------------------------------------------ device scene (batch of worlds)
+// Device allocations still to fail (rphys_debug_fail_device_allocs): forces the device-failure
+// path from the host.
+static std::atomic<uint32_t> g_fail_allocs{0};
+static bool take_forced_failure() {
+    uint32_t v = g_fail_allocs.load();
+    while (v && !g_fail_allocs.compare_exchange_weak(v, v - 1)) {}
+    return v != 0;
+}
 struct DeviceBuffers {
     std::vector<void *> ptrs;
     ~DeviceBuffers() { for (void *p : ptrs) (void)hipFree(p); }
     template <class T> bool alloc(T *&p, size_t n) {
         void *q = nullptr;
-        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { p = nullptr; return false; }
+        if (take_forced_failure() || hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { p = nullptr; return false; }
         ptrs.push_back(q);
         p = (T *)q;
         return true;
@@ -1434,7 +1442,12 @@ RPhysResult rphys_debug_set_initial_capacity(uint32_t pairs, uint32_t cands, uin
     return RPHYS_OK;
 }
 
-// lib.rs:1580-1693 player controller: argument checks as the reference, then out of scope
+RPhysResult rphys_debug_fail_device_allocs(uint32_t n, uint32_t *out_pending) {
+    uint32_t prev = g_fail_allocs.exchange(n);
+    if (out_pending) *out_pending = prev;
+    return RPHYS_OK;
+}
+
 // Kinematic players (lib.rs:1580-1693).  add_player (world.rs:466-475): a static body with the
 // controller's default capsule (r 0.4, half height 0.9, material Default) consuming one body and
 // one collider; the controller then takes the described radius, height and speed.  The device

@@ -221,6 +221,7 @@ EXPORTS = [
     ("rphys_debug_set_sweep_tiers", U32, [U32, U32, U32, U32]),
     ("rphys_debug_set_lane_tier", U32, [U32, U32, U32, U32]),
     ("rphys_debug_set_initial_capacity", U32, [U32, U32, U32, U32]),
+    ("rphys_debug_fail_device_allocs", U32, [U32, PU32]),
 ]
 
 _lib = None

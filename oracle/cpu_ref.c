@@ -17,6 +17,12 @@
  *   solve_islands_parallel            solver.rs:477-1008, 1247-1271; ball-joint
  *                                     rows, warm start and store solver.rs:272-307, 336-424,
  *                                     644-715, 788-835, 945-952 (articulation bodies.rs:326-331)
+ *   compliant (Hunt-Crossley) rows   solver.rs:545-567, 848-856; left out of the warm start and
+ *                                     store, shock, W6 supports, projection and grounding
+ *                                     (solver.rs:736, 922, 973, 1020, 1087, 1175; step.rs:340, 433)
+ *   friction patches                  solver.rs:219-260 (patch_radius), 592-619, 763-782, 865,
+ *                                     882-915, 927-944, 1257-1264; warm_patches in the hash and
+ *                                     snapshots (world.rs:249, 818-821, 896-897, 935, 952)
  *   restitution relax + W6 bounce     solver.rs:569-588 (time-of-impact rows), 995-1245 (relax,
  *                                     velocity pass with the interior-point LCP, pseudo pass),
  *                                     solver.rs:121-212 and bounce_lcp.rs (or_bounce.h)
@@ -44,7 +50,13 @@
  * reality/src/analytic.rs free fall); invariants (manifold <= 4 points, n.(pb-pa) >= 0,
  * acc_jn >= 0, |jt| <= mu jn, snapshot/restore hash equality); ball joints (test_oracle_joints.py):
  * a rigid pendulum's large-amplitude half period and energy, anchor drift, a hanging body's joint
- * impulse m g dt, jointed pairs without contacts, the SHAKE test's acosf boundary.  glam's SSE2
+ * impulse m g dt, jointed pairs without contacts, the SHAKE test's acosf boundary; contact models
+ * (test_oracle_contact_models.py): compliant_contact_test.rs's resting sinks m g / (4 k) and F / k,
+ * bounces against Hunt-Crossley's ODE, Coulomb sliding and a slope hold on compliant rows;
+ * patch_friction_test.rs's spin-down angles with the face's mean radius and the corners' radius,
+ * Coulomb sliding, the patch radius against quadrature.  The patch point order's atan2 is a
+ * once-rounded double evaluation (or_math.h), within 1 ulp of the platform's atan2f; only the order
+ * of the angles' keys reaches the results.  glam's SSE2
  * operation order and the platform libm are third-party (glam 0.27.0, libm 0.2.15); G1-G3 pin them
  * on the paths they cover (box-box contacts on a static box, distance joints, W6).  No reference
  * golden holds a heightfield contact, a sphere or capsule contact or a ball joint, so bit-level
@@ -99,11 +111,15 @@ static const MatP MATS[16] = {
     {0.20f, 0.15f, 0.85f, 0.04f}};
 enum { MAT_DEFAULT = 0, MAT_ICE = 1, MAT_RUBBER = 2, MAT_STEEL = 3, MAT_GRIT = 4, MAT_CONCRETE = 6, MAT_ASPHALT = 7 };
 
-typedef struct { float mu_s, mu_k, rest, vs; } PairP;
+/* The solver's pair properties (materials.rs:75-90 MatPairProps): friction, restitution, Stribeck
+ * speed, and the contact model: stiffness k > 0 makes the pair compliant (Hunt-Crossley, with
+ * dissipation d), patch solves a manifold's friction as one patch.  The table's pairs are rigid. */
+typedef struct { float mu_s, mu_k, rest, vs, stiff, diss; int patch; } PairP;
 /* materials.rs:278-304 mix_pair + pair_props + override_for */
 static PairP table_pair(uint32_t a, uint32_t b) {
     MatP pa = MATS[a & 15], pb = MATS[b & 15];
     PairP r;
+    r.stiff = 0.0f; r.diss = 0.0f; r.patch = 0;
     r.mu_s = mquant(sqrtf(fabsf(pa.mu_s * pb.mu_s)));
     r.mu_k = mquant(sqrtf(fabsf(pa.mu_k * pb.mu_k)));
     r.rest = mquant(fmaxf(pa.rest, pb.rest));
@@ -127,6 +143,8 @@ typedef struct {
     uint32_t env_mat, feat, cell;
 } Contact;
 typedef struct { uint32_t key[4]; V3 la; float jn; V3 jt; } Warm;
+/* A friction patch's carried impulses (solver.rs WarmPatch): sliding jt (world frame), torsion tw */
+typedef struct { uint32_t key[4]; V3 jt; float tw; } WarmP;
 typedef struct { uint32_t a, b; PairP p; } PairOverride;
 /* A ball joint (articulation/src/bodies.rs:326-331 add_ball: the three translational axes of frame
  * A locked, the rotational ones free) between bodies a and b, with body-local frames (pos, rot);
@@ -152,7 +170,9 @@ struct OrWorld {
     uint32_t iv, ip;
     int resid_on; float resid_tol; uint32_t resid_max;
     PairOverride ovr[32]; uint32_t novr;
+    int damp_at_start;  /* world.rs:992 set_compliant_damping_at_start */
     Warm *warm; uint32_t nwarm;
+    WarmP *wpatch; uint32_t nwpatch;  /* world.rs:249 warm_patches, sorted by key */
     Joint *joints; WarmJ *wj; uint32_t nj, capj;
     /* per-step outputs */
     uint32_t *pairs; uint32_t npairs;
@@ -192,7 +212,7 @@ void or_world_destroy(OrWorld *w) {
     free(w->pos); free(w->rot); free(w->lin); free(w->ang); free(w->inv_mass); free(w->dyn); free(w->iil);
     free(w->alive); free(w->last_lin); free(w->awake); free(w->sticks); free(w->sallow); free(w->pl); free(w->pa); free(w->players);
     free(w->cbody); free(w->cshape); free(w->cmat); free(w->caabb); free(w->loaded);
-    free(w->hts); free(w->warm); free(w->pairs); free(w->contacts);
+    free(w->hts); free(w->warm); free(w->wpatch); free(w->pairs); free(w->contacts);
     free(w);
 }
 
@@ -203,14 +223,31 @@ void or_set_gravity(OrWorld *w, float x, float y, float z) { w->gravity = v3(x, 
 void or_set_iterations(OrWorld *w, uint32_t v, uint32_t p) { w->iv = v ? v : 1; w->ip = p ? p : 1; }
 void or_set_residual_mode(OrWorld *w, int on, float tol, uint32_t max_iters) { w->resid_on = on; w->resid_tol = tol; w->resid_max = max_iters; }
 /* world.rs:975-986 */
-void or_set_pair_props(OrWorld *w, uint32_t a, uint32_t b, float mu_s, float mu_k, float rest, float vs) {
+static PairP pair_props(const OrWorld *w, uint32_t a, uint32_t b);
+/* The pair's override, created from its current properties when absent (NULL when full). */
+static PairP *pair_override(OrWorld *w, uint32_t a, uint32_t b) {
     uint32_t x = a < b ? a : b, y = a < b ? b : a;
-    for (uint32_t i = 0; i < w->novr; i++) if (w->ovr[i].a == x && w->ovr[i].b == y) { w->ovr[i] = w->ovr[--w->novr]; break; }
-    if (w->novr >= 32) return;
-    PairOverride o; o.a = x; o.b = y; o.p.mu_s = mu_s; o.p.mu_k = mu_k; o.p.rest = rest; o.p.vs = vs;
+    for (uint32_t i = 0; i < w->novr; i++) if (w->ovr[i].a == x && w->ovr[i].b == y) return &w->ovr[i].p;
+    if (w->novr >= 32) return NULL;
     /* the reference keeps overrides in a Vec and finds the first match; order of distinct keys is irrelevant */
+    PairOverride o; o.a = x; o.b = y; o.p = pair_props(w, a, b);
     w->ovr[w->novr++] = o;
+    return &w->ovr[w->novr - 1].p;
 }
+/* friction and restitution of a pair (its contact model stays) */
+void or_set_pair_props(OrWorld *w, uint32_t a, uint32_t b, float mu_s, float mu_k, float rest, float vs) {
+    PairP *p = pair_override(w, a, b);
+    if (!p) return;
+    p->mu_s = mu_s; p->mu_k = mu_k; p->rest = rest; p->vs = vs;
+}
+/* the pair's contact model (materials.rs:75-90 stiffness, dissipation, patch_friction; its friction
+ * and restitution stay) */
+void or_set_pair_contact_model(OrWorld *w, uint32_t a, uint32_t b, float stiffness, float dissipation, int patch) {
+    PairP *p = pair_override(w, a, b);
+    if (!p) return;
+    p->stiff = stiffness; p->diss = dissipation; p->patch = patch != 0;
+}
+void or_set_compliant_damping_at_start(OrWorld *w, int on) { w->damp_at_start = on != 0; }
 static PairP pair_props(const OrWorld *w, uint32_t a, uint32_t b) {
     uint32_t x = a < b ? a : b, y = a < b ? b : a;
     for (uint32_t i = 0; i < w->novr; i++) if (w->ovr[i].a == x && w->ovr[i].b == y) return w->ovr[i].p;
@@ -374,6 +411,10 @@ static void drop_warm_of(OrWorld *w, uint32_t b) {
     for (uint32_t i = 0; i < w->nwarm; i++)
         if (w->warm[i].key[0] != b && w->warm[i].key[1] != b) w->warm[k++] = w->warm[i];
     w->nwarm = k;
+    k = 0;
+    for (uint32_t i = 0; i < w->nwpatch; i++)
+        if (w->wpatch[i].key[0] != b && w->wpatch[i].key[1] != b) w->wpatch[k++] = w->wpatch[i];
+    w->nwpatch = k;
     for (uint32_t q = 0; q < w->nj; q++)
         if (w->joints[q].a == b || w->joints[q].b == b) w->wj[q].has = 0;
 }
@@ -657,7 +698,15 @@ typedef struct {
     /* W6 time of impact (solver.rs:569-588): the normal velocities of each side at the step's
      * start and after this step's forces, the start gap, the pair's restitution */
     int toi; float toi_s0, toi_e, toi_va0, toi_va1, toi_vb0, toi_vb1;
+    /* compliant pair (solver.rs:64-77, 545-567): no rigid normal row, shock or restitution; the
+     * implicit soft row dlam = soft_m ((soft_bias - v_n) - soft_gamma lam), lam >= 0, or none (a
+     * row whose effective mass or spring is zero keeps lam at 0) */
+    int soft, soft_implicit; float soft_gamma, soft_bias, soft_m;
+    int patch;  /* friction solved by the manifold's patch, not by this point */
 } CC;
+/* One manifold's friction patch (solver.rs:219-229, 592-619): a constraint at the points' centroid,
+ * the points (contact indices, ascending), the mean radius of the patch and the torsional mass. */
+typedef struct { CC at; uint32_t p0, np; float radius, m_tw; } Patch;
 
 /* world/src/world/geom.rs:30-38 */
 static void orthonormal_basis(V3 n, V3 *t1, V3 *t2) {
@@ -695,9 +744,164 @@ static float eff_mass(const CC *c, V3 d) {
     float k = ((c->inv_a + c->inv_b) + vdot(ca, m3v(c->ii_a, ca))) + vdot(cb, m3v(c->ii_b, cb));
     return k > 1.0e-9f ? 1.0f / k : 0.0f;
 }
+/* solver.rs:233-260 patch_radius: the mean distance from the origin over the convex polygon of the
+ * plane points (sorted by angle; each fan triangle integrated on a 6 x 6 subdivision by its
+ * centroids); collinear points: a quarter of their longest distance */
+typedef struct { float x, y; } P2;
+static float patch_radius(P2 *p, uint32_t n) {
+    if (n < 2) return 0.0f;
+    int64_t *key = malloc(sizeof(int64_t) * n);
+    for (uint32_t i = 0; i < n; i++) key[i] = sort_key6(or_atan2f(p[i].y, p[i].x));
+    for (uint32_t x = 1; x < n; x++) {  /* stable */
+        P2 v = p[x]; int64_t kv = key[x]; uint32_t y = x;
+        while (y > 0 && key[y - 1] > kv) { p[y] = p[y - 1]; key[y] = key[y - 1]; y--; }
+        p[y] = v; key[y] = kv;
+    }
+    free(key);
+    float area = 0.0f, moment = 0.0f;
+    const int N = 6;
+    for (uint32_t i = 0; i < n; i++) {
+        P2 a = p[i], b = p[(i + 1) % n];
+        float tri = 0.5f * fabsf(a.x * b.y - a.y * b.x);
+        if (tri <= 0.0f) continue;
+        float sub = tri / (float)(N * N);
+        for (int r = 0; r < N; r++)
+            for (int c = 0; c < N - r; c++)
+                for (int up = 1; up >= 0; up--) {
+                    if (!up && c + r + 1 >= N) continue;
+                    float u = up ? (float)r + 1.0f / 3.0f : (float)r + 2.0f / 3.0f;
+                    float v = up ? (float)c + 1.0f / 3.0f : (float)c + 2.0f / 3.0f;
+                    u = u / (float)N; v = v / (float)N;
+                    float x0 = a.x * u + b.x * v, x1 = a.y * u + b.y * v;
+                    moment += sqrtf(x0 * x0 + x1 * x1) * sub;
+                }
+        area += tri;
+    }
+    if (area > 1.0e-12f) return moment / area;
+    float len = 0.0f;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < n; j++) {
+            float dx = p[i].x - p[j].x, dy = p[i].y - p[j].y;
+            len = fmaxf(len, sqrtf(dx * dx + dy * dy));
+        }
+    return 0.25f * len;
+}
+/* solver.rs:596-617: the patch of the manifold whose points (contact indices, ascending) are idx:
+ * a constraint at their centroid along their mean normal, with the first point's bodies, masses and
+ * pair properties; marks the points.  0 when the normals cancel. */
+static int build_patch(const OrWorld *w, const Contact *cts, CC *cs, const uint32_t *idx, uint32_t n, Patch *out) {
+    const CC first = cs[idx[0]];
+    V3 sum = v3s(0), nsum = v3s(0);
+    for (uint32_t k = 0; k < n; k++) sum = vadd(sum, cts[idx[k]].point);
+    V3 centroid = v3(sum.x / (float)n, sum.y / (float)n, sum.z / (float)n);
+    for (uint32_t k = 0; k < n; k++) nsum = vadd(nsum, cts[idx[k]].normal);
+    V3 normal = vnormalize_or_zero(nsum);
+    if (normal.x == 0.0f && normal.y == 0.0f && normal.z == 0.0f) return 0;
+    V3 t1, t2;
+    orthonormal_basis(normal, &t1, &t2);
+    P2 *plane = malloc(sizeof(P2) * n);
+    for (uint32_t k = 0; k < n; k++) {
+        V3 d = vsub(cts[idx[k]].point, centroid);
+        plane[k].x = vdot(d, t1); plane[k].y = vdot(d, t2);
+    }
+    CC at = first;
+    at.ra = vsub(centroid, w->pos[first.ai]);
+    at.rb = first.terrain ? v3s(0) : vsub(centroid, w->pos[first.bi]);
+    at.n = normal; at.t1 = t1; at.t2 = t2;
+    at.m_n = eff_mass(&at, normal); at.m_t1 = eff_mass(&at, t1); at.m_t2 = eff_mass(&at, t2);
+    float kt = vdot(normal, m3v(first.ii_a, normal)) + vdot(normal, m3v(first.ii_b, normal));
+    for (uint32_t k = 0; k < n; k++) cs[idx[k]].patch = 1;
+    out->at = at; out->np = n;
+    out->radius = patch_radius(plane, n);
+    free(plane);
+    out->m_tw = kt > 1.0e-9f ? 1.0f / kt : 0.0f;
+    return 1;
+}
+/* slip_speed (solver.rs:435-439) */
+static float slip_speed(const OrWorld *w, const CC *c) {
+    V3 v = rel_vel(w, c);
+    float vt1 = vdot(v, c->t1), vt2 = vdot(v, c->t2);
+    return sqrtf(vt1 * vt1 + vt2 * vt2);
+}
+/* f32::clamp */
+static float clampf_rs(float x, float lo, float hi) { if (x < lo) x = lo; if (x > hi) x = hi; return x; }
+/* sum of the patch's points' normal impulses (Iterator::sum, from -0) */
+static float patch_jn(const float *jn, const uint32_t *pts, uint32_t n) {
+    float s = -0.0f;
+    for (uint32_t k = 0; k < n; k++) s += jn[pts[k]];
+    return s;
+}
+/* torsion impulse about the normal on both sides (solver.rs:778-781, 908-911) */
+static void patch_twist(OrWorld *w, const CC *c, float tw) {
+    if (c->inv_a > 0.0f) apply_ang_impulse(w, c->ai, vmul(vneg(c->n), tw));
+    if (!c->terrain && c->inv_b > 0.0f) apply_ang_impulse(w, c->bi, vmul(c->n, tw));
+}
 static int key_cmp(const uint32_t *a, const uint32_t *b) {
     for (int i = 0; i < 4; i++) if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
     return 0;
+}
+/* One pass of a friction patch (solver.rs:887-913): sliding at the centroid, then torsion about the
+ * normal, both limited by the points' current normal impulse; acc = (t1, t2, torsion).  Returns the
+ * residual-mode residual: the largest of |d jt| / m_t and |d tw| / m_tw. */
+static float patch_row(OrWorld *w, const Patch *P, const uint32_t *pp, const float *jn, float *acc) {
+    const CC *c = &P->at;
+    float jsum = patch_jn(jn, pp, P->np);
+    V3 v = rel_vel(w, c);
+    float vt1 = vdot(v, c->t1), vt2 = vdot(v, c->t2);
+    float mu = mu_dyn(c, sqrtf(vt1 * vt1 + vt2 * vt2));
+    V3 ot = v3(acc[0], acc[1], 0.0f);
+    V3 nt = vclamp_len_max(vsub(ot, v3(vt1 * c->m_t1, vt2 * c->m_t2, 0.0f)), mu * jsum);
+    acc[0] = nt.x; acc[1] = nt.y;
+    V3 d = vsub(nt, ot);
+    apply_cimp(w, c, vadd(vmul(c->t1, d.x), vmul(c->t2, d.y)));
+    V3 wa = c->inv_a > 0.0f ? w->ang[c->ai] : v3s(0);
+    V3 wb = (c->terrain || c->inv_b <= 0.0f) ? v3s(0) : w->ang[c->bi];
+    float w_rel = vdot(vsub(wb, wa), c->n);
+    float old_tw = acc[2];
+    float lim = (mu * jsum) * P->radius;
+    float new_tw = clampf_rs(old_tw - w_rel * P->m_tw, -lim, lim);
+    acc[2] = new_tw;
+    float dtw = new_tw - old_tw;
+    if (dtw != 0.0f) patch_twist(w, c, dtw);
+    float r1 = c->m_t1 > 0.0f ? fabsf(d.x) / c->m_t1 : 0.0f;
+    float r2 = c->m_t2 > 0.0f ? fabsf(d.y) / c->m_t2 : 0.0f;
+    float r3 = P->m_tw > 0.0f ? fabsf(dtw) / P->m_tw : 0.0f;
+    return fmaxf(r1, fmaxf(r2, r3));
+}
+static int wpatch_find(const OrWorld *w, const uint32_t key[4]) {
+    uint32_t lo = 0, hi = w->nwpatch;
+    while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (key_cmp(w->wpatch[mid].key, key) < 0) lo = mid + 1; else hi = mid; }
+    return (lo < w->nwpatch && key_cmp(w->wpatch[lo].key, key) == 0) ? (int)lo : -1;
+}
+static int is_idle(const OrWorld *w, uint32_t b);
+/* warm_patches store (solver.rs:927-944): this step's rigid patches by their first point's key,
+ * plus the entries of pairs neither of whose bodies is awake that were not solved */
+static void store_patches(OrWorld *w, const Patch *pt, uint32_t npt, const uint32_t *ppts, const uint32_t (*keys)[4], const float *pacc) {
+    WarmP *nw = malloc(sizeof(WarmP) * (npt + w->nwpatch + 1));
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < npt; k++) {
+        if (pt[k].at.soft) continue;
+        WarmP e;
+        memcpy(e.key, keys[ppts[pt[k].p0]], 16);
+        e.jt = vadd(vmul(pt[k].at.t1, pacc[3 * k]), vmul(pt[k].at.t2, pacc[3 * k + 1]));
+        e.tw = pacc[3 * k + 2];
+        uint32_t y = n;  /* key order (a map insert) */
+        while (y > 0 && key_cmp(nw[y - 1].key, e.key) > 0) { nw[y] = nw[y - 1]; y--; }
+        if (y > 0 && key_cmp(nw[y - 1].key, e.key) == 0) { nw[y - 1] = e; memmove(nw + y, nw + y + 1, sizeof(WarmP) * (n - y)); continue; }
+        nw[y] = e; n++;
+    }
+    WarmP *merged = malloc(sizeof(WarmP) * (n + w->nwpatch + 1));
+    uint32_t m = 0, p = 0;
+    for (uint32_t q = 0; q < w->nwpatch; q++) {
+        const WarmP *o = &w->wpatch[q];
+        if (!(is_idle(w, o->key[0]) && is_idle(w, o->key[1]))) continue;
+        while (p < n && key_cmp(nw[p].key, o->key) < 0) merged[m++] = nw[p++];
+        if (p < n && key_cmp(nw[p].key, o->key) == 0) continue;
+        merged[m++] = *o;
+    }
+    while (p < n) merged[m++] = nw[p++];
+    free(nw); free(w->wpatch);
+    w->wpatch = merged; w->nwpatch = m;
 }
 /* solver.rs:730-745: among the unused cached entries of the contact's key (the cache is sorted by
  * key), the one nearest to its anchor with |dla|^2 <= 0.01^2; ties go to the lowest index.  The
@@ -808,7 +1012,7 @@ static void w6_grow(const CC *cs, uint32_t nct, const float *jn, const float *jt
         int changed = 0;
         for (uint32_t i = 0; i < nct; i++) {
             const CC *c = &cs[i];
-            if (c->terrain || !loaded_at(jn, jt, i)) continue;
+            if (c->soft || c->terrain || !loaded_at(jn, jt, i)) continue;
             if (inv[c->ai] && !inv[c->bi] && c->inv_b > 0.0f && !jointed[c->bi]) { inv[c->bi] = 1; changed = 1; }
             if (inv[c->bi] && !inv[c->ai] && c->inv_a > 0.0f && !jointed[c->ai]) { inv[c->ai] = 1; changed = 1; }
         }
@@ -904,7 +1108,7 @@ static void w6_passes(OrWorld *w, const CC *cs, uint32_t nct, float *jn, const f
         /* the bounced bodies' other loaded contacts: push-only support rows with fresh
          * accumulators and target min(depth_bias, 0) (solver.rs:1081-1092) */
         for (uint32_t i = 0; i < nct; i++) {
-            if (toi_set[i] || jn[i] <= 0.0f) continue;
+            if (toi_set[i] || cs[i].soft || jn[i] <= 0.0f) continue;
             const CC *c = &cs[i];
             if (!inv[c->ai] && (c->terrain || !inv[c->bi])) continue;
             CC k;
@@ -990,7 +1194,7 @@ static void w6_passes(OrWorld *w, const CC *cs, uint32_t nct, float *jn, const f
             if (w6_free_only(&cs[prow_src[r].cidx], jointed, &k)) { rows[nr].c = k; rows[nr].target = prow_src[r].target; rows[nr].cidx = -1; nr++; }
         }
         for (uint32_t i = 0; i < nct; i++) {
-            if (in_set[i] || !loaded_at(jn, jt, i)) continue;
+            if (in_set[i] || cs[i].soft || !loaded_at(jn, jt, i)) continue;
             const CC *c = &cs[i];
             if (!inv[c->ai] && (c->terrain || !inv[c->bi])) continue;
             CC k;
@@ -1094,6 +1298,25 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         k->approach = vrn;
         k->rest_bias = 0.0f;
         k->toi = 0; k->toi_s0 = k->toi_e = 0.0f; k->toi_va0 = k->toi_va1 = k->toi_vb0 = k->toi_vb1 = 0.0f;
+        k->soft = pp.stiff > 0.0f; k->soft_implicit = 0; k->soft_gamma = k->soft_bias = k->soft_m = 0.0f; k->patch = 0;
+        if (k->soft) {
+            /* implicit Hunt-Crossley (solver.rs:545-566): the spring acts at the end-of-step
+             * penetration, the damping linearised at the predicted mid-step penetration (or the
+             * step-start one); no Baumgarte bias */
+            k->depth_bias = 0.0f;
+            float kk = pp.stiff;
+            float vn0 = vdot(rel_vel(w, k), k->n);
+            float pen_d = w->damp_at_start ? pen : pen - (0.5f * dt) * vn0;
+            float cdamp = (kk * fmaxf(pp.diss, 0.0f)) * fmaxf(pen_d, 0.0f);
+            float denom = cdamp + dt * kk;
+            if (k->m_n > 0.0f && denom > 0.0f) {
+                k->soft_implicit = 1;
+                k->soft_gamma = 1.0f / (dt * denom);
+                k->soft_bias = (((dt * kk) / denom) * pen) / dt;
+                k->soft_m = 1.0f / (1.0f / k->m_n + k->soft_gamma);
+            }
+            continue;
+        }
         if (vrn < -1.0f) {
             k->rest_bias = -pp.rest * vrn;
             /* W6: free bodies bounce at their time of impact (solver.rs:572-587); a side's start
@@ -1110,6 +1333,23 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
                 }
             }
         }
+    }
+    /* friction patches (solver.rs:592-619): manifolds of two or more points on patch-friction pairs,
+     * in (collider a, collider b) order */
+    uint32_t *ppts = malloc(4 * (nct + 1)), npp = 0;
+    for (uint32_t i = 0; i < nct; i++) if (contact_props(w, &cts[i]).patch) ppts[npp++] = i;
+    for (uint32_t x = 1; x < npp; x++) {  /* stable insertion sort by the collider pair */
+        uint32_t v = ppts[x], y = x;
+        while (y > 0 && (cts[ppts[y - 1]].a > cts[v].a || (cts[ppts[y - 1]].a == cts[v].a && cts[ppts[y - 1]].b > cts[v].b))) { ppts[y] = ppts[y - 1]; y--; }
+        ppts[y] = v;
+    }
+    Patch *pt = malloc(sizeof(Patch) * (npp / 2 + 1));
+    uint32_t npt = 0;
+    for (uint32_t g = 0; g < npp;) {
+        uint32_t e = g;
+        while (e < npp && cts[ppts[e]].a == cts[ppts[g]].a && cts[ppts[e]].b == cts[ppts[g]].b) e++;
+        if (e - g >= 2 && build_patch(w, cts, cs, ppts + g, e - g, &pt[npt])) { pt[npt].p0 = g; npt++; }
+        g = e;
     }
     /* level schedule, for the stats (solver.rs:621-642) */
     uint32_t *blev = calloc(nb + 1, 4);
@@ -1162,6 +1402,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     uint8_t *taken = calloc(w->nwarm + 1, 1);
     uint32_t matched = 0;
     for (uint32_t i = 0; i < nct; i++) {
+        if (cs[i].soft) continue;  /* compliant rows start cold */
         int best = warm_pick(w->warm, w->nwarm, keys[i], anch[i], taken);
         if (best < 0) continue;
         matched++;
@@ -1169,15 +1410,37 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         const CC *k = &cs[i];
         jn[i] = e->jn;
         V3 imp = vmul(k->n, e->jn);
-        V3 v = rel_vel(w, k);
-        float vt1 = vdot(v, k->t1), vt2 = vdot(v, k->t2);
-        float mu = mu_dyn(k, sqrtf(vt1 * vt1 + vt2 * vt2));
-        V3 t = vclamp_len_max(v3(vdot(e->jt, k->t1), vdot(e->jt, k->t2), 0.0f), mu * e->jn);
-        jt[2 * i] = t.x; jt[2 * i + 1] = t.y;
-        imp = vadd(imp, vadd(vmul(k->t1, t.x), vmul(k->t2, t.y)));
+        if (!k->patch) {
+            V3 v = rel_vel(w, k);
+            float vt1 = vdot(v, k->t1), vt2 = vdot(v, k->t2);
+            float mu = mu_dyn(k, sqrtf(vt1 * vt1 + vt2 * vt2));
+            V3 t = vclamp_len_max(v3(vdot(e->jt, k->t1), vdot(e->jt, k->t2), 0.0f), mu * e->jn);
+            jt[2 * i] = t.x; jt[2 * i + 1] = t.y;
+            imp = vadd(imp, vadd(vmul(k->t1, t.x), vmul(k->t2, t.y)));
+        }
         apply_cimp(w, k, imp);
     }
     free(taken);
+    /* patches (solver.rs:763-782): sliding and torsion from the carried patch impulses, limited by
+     * the points' warm normal load; compliant pairs start cold */
+    float *pacc = calloc(3 * npt + 3, 4);
+    for (uint32_t k = 0; k < npt; k++) {
+        const Patch *P = &pt[k];
+        if (P->at.soft) continue;
+        const uint32_t *pp = ppts + P->p0;
+        int e = wpatch_find(w, keys[pp[0]]);
+        if (e < 0) continue;
+        const WarmP *wp = &w->wpatch[e];
+        const CC *c = &P->at;
+        float jsum = patch_jn(jn, pp, P->np);
+        float mu = mu_dyn(c, slip_speed(w, c));
+        V3 t = vclamp_len_max(v3(vdot(wp->jt, c->t1), vdot(wp->jt, c->t2), 0.0f), mu * jsum);
+        float lim = (mu * jsum) * P->radius;
+        float tw = clampf_rs(wp->tw, -lim, lim);
+        pacc[3 * k] = t.x; pacc[3 * k + 1] = t.y; pacc[3 * k + 2] = tw;
+        apply_cimp(w, c, vadd(vmul(c->t1, t.x), vmul(c->t2, t.y)));
+        if (tw != 0.0f) patch_twist(w, c, tw);
+    }
     w->stats.warm_matched = matched;
 
     /* islands of the written bodies (residual mode stops each island on its own residual).  A joint
@@ -1340,12 +1603,22 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
                 cran[i] = 0;
                 if (isl_done[isl[i]]) continue;
                 cran[i] = 1;
-                float vrn = vdot(rel_vel(w, k), k->n);
-                float jraw = (k->depth_bias - vrn) * k->m_n;
-                float old = jn[i];
-                float nw = fmaxf(old + jraw, 0.0f);
-                jn[i] = nw;
-                apply_cimp(w, k, vmul(k->n, nw - old));
+                float old = jn[i], nw = old;
+                if (k->soft_implicit) {  /* the implicit compliant row (solver.rs:848-855) */
+                    float vrn = vdot(rel_vel(w, k), k->n);
+                    nw = fmaxf(old + k->soft_m * ((k->soft_bias - vrn) - k->soft_gamma * old), 0.0f);
+                    jn[i] = nw;
+                    if (nw != old) apply_cimp(w, k, vmul(k->n, nw - old));
+                } else if (!k->soft) {
+                    float vrn = vdot(rel_vel(w, k), k->n);
+                    float jraw = (k->depth_bias - vrn) * k->m_n;
+                    nw = fmaxf(old + jraw, 0.0f);
+                    jn[i] = nw;
+                    apply_cimp(w, k, vmul(k->n, nw - old));
+                }
+                const float rm = k->soft_implicit ? k->soft_m : k->m_n, rt = k->soft_implicit ? k->soft_bias : k->depth_bias;
+                float r0 = rm > 0.0f ? fabsf(nw - old) / rm / (1.0f + fabsf(rt)) : 0.0f;
+                if (k->patch) { cres[i] = r0; continue; }  /* friction by the patch (solver.rs:865) */
                 V3 vf = rel_vel(w, k);
                 float vt1 = vdot(vf, k->t1), vt2 = vdot(vf, k->t2);
                 float slip = sqrtf(vt1 * vt1 + vt2 * vt2);
@@ -1355,7 +1628,6 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
                 jt[2 * i] = nt.x; jt[2 * i + 1] = nt.y;
                 V3 d = vsub(nt, ot);
                 apply_cimp(w, k, vadd(vmul(k->t1, d.x), vmul(k->t2, d.y)));
-                float r0 = k->m_n > 0.0f ? fabsf(nw - old) / k->m_n / (1.0f + fabsf(k->depth_bias)) : 0.0f;
                 float r1 = k->m_t1 > 0.0f ? fabsf(d.x) / k->m_t1 : 0.0f;
                 float r2 = k->m_t2 > 0.0f ? fabsf(d.y) / k->m_t2 : 0.0f;
                 cres[i] = fmaxf(r0, fmaxf(r1, r2));
@@ -1366,6 +1638,15 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
             if (!cran[i]) continue;
             active = 1;
             if (w->resid_on && cres[i] > isl_r[isl[i]]) isl_r[isl[i]] = cres[i];
+        }
+        /* C. friction patches (solver.rs:882-915), in patch order; a patch with no writable body
+         * runs after the loop */
+        for (uint32_t k = 0; k < npt; k++) {
+            uint32_t pis = isl[ppts[pt[k].p0]];
+            if (pis == 0xFFFFFFFFu || isl_done[pis]) continue;
+            float r = patch_row(w, &pt[k], ppts + pt[k].p0, jn, pacc + 3 * k);
+            active = 1;
+            if (w->resid_on && r > isl_r[pis]) isl_r[pis] = r;
         }
         if (!active) break;
         used++;
@@ -1381,6 +1662,9 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         isum += isl_it[b] * isl_n[b];
         if (isl_rows[b] && isl_r[b] > resid) resid = isl_r[b];
     }
+    for (uint32_t it = 0; it < iters; it++)
+        for (uint32_t k = 0; k < npt; k++)
+            if (isl[ppts[pt[k].p0]] == 0xFFFFFFFFu) (void)patch_row(w, &pt[k], ppts + pt[k].p0, jn, pacc + 3 * k);
     w->stats.iters_used = used;
     w->stats.iters_sum = isum;
     w->stats.residual = resid;
@@ -1401,6 +1685,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     Warm *nw = malloc(sizeof(Warm) * (nct + w->nwarm + 1));
     uint32_t nn = 0;
     for (uint32_t i = 0; i < nct; i++) {
+        if (cs[i].soft) continue;
         V3 j = vadd(vmul(cs[i].t1, jt[2 * i]), vmul(cs[i].t2, jt[2 * i + 1]));
         if (jn[i] == 0.0f && viszero(j)) continue;
         Warm e; memcpy(e.key, keys[i], 16); e.la = anch[i]; e.jn = jn[i]; e.jt = j;
@@ -1429,13 +1714,14 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     free(nw);
     free(w->warm);
     w->warm = merged; w->nwarm = mcount;
+    store_patches(w, pt, npt, ppts, keys, pacc);
 
     /* shock propagation (solver.rs:964-993) */
     V3 up = vlen2(w->gravity) > 0.0f ? vneg(vnormalize(w->gravity)) : v3(0, 1, 0);
     SK *ord = malloc(sizeof(SK) * (nct + 1));
     uint32_t no = 0;
     for (uint32_t i = 0; i < nct; i++) {
-        if (!(cs[i].inv_a > 0.0f || cs[i].inv_b > 0.0f)) continue;
+        if (cs[i].soft || !(cs[i].inv_a > 0.0f || cs[i].inv_b > 0.0f)) continue;
         float ha = vdot(w->pos[cs[i].ai], up);
         float h = cs[i].terrain ? -INFINITY : fminf(ha, vdot(w->pos[cs[i].bi], up));
         ord[no].k = sort_key6(h); ord[no].idx = i; no++;
@@ -1480,6 +1766,16 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         }
         cts[i].impulse = jn[i];
     }
+    for (uint32_t k = 0; k < npt; k++) {
+        if (pacc[3 * k] == 0.0f && pacc[3 * k + 1] == 0.0f && pacc[3 * k + 2] == 0.0f) continue;
+        for (uint32_t m = 0; m < pt[k].np; m++) {
+            const CC *c = &cs[ppts[pt[k].p0 + m]];
+            w->loaded[c->ai] = 1;
+            if (!c->terrain) w->loaded[c->bi] = 1;
+        }
+    }
+    w->stats.patches = npt;
+    free(pacc); free(pt); free(ppts);
     free(jn); free(jt); free(keys); free(anch); free(cs);
 }
 
@@ -1943,6 +2239,7 @@ void or_step(OrWorld *w, float dt) {
         int ma = w->inv_mass[a] > 0.0f && w->awake[a];
         int mb = hb && w->inv_mass[b] > 0.0f && w->awake[b];
         if (!ma && !mb) continue;
+        if (contact_props(w, c).stiff > 0.0f) continue;  /* compliant pairs penetrate by design (step.rs:433) */
         PosC x; x.a = a; x.b = b; x.has_b = hb; x.n = c->normal; x.pen0 = c->depth - MARGIN;
         V3 pap = (w->dyn[a] && w->awake[a]) ? opos[a] : w->pos[a];
         Q4 paq = (w->dyn[a] && w->awake[a]) ? orot[a] : w->rot[a];
@@ -1986,7 +2283,7 @@ void or_step(OrWorld *w, float dt) {
     uint8_t *ground = calloc(count + 1, 1);
     for (uint32_t k = 0; k < cul.n; k++) {
         const Contact *c = &cul.v[k];
-        if (c->b == TERRAIN_CI && c->depth - MARGIN > -POS_SLOP) ground[w->cbody[c->a]] = 1;
+        if (c->b == TERRAIN_CI && c->depth - MARGIN > -POS_SLOP && contact_props(w, c).stiff <= 0.0f) ground[w->cbody[c->a]] = 1;
     }
     for (uint32_t i = 0; i < count; i++) {
         if (!w->dyn[i] || !w->awake[i]) continue;
@@ -2059,7 +2356,11 @@ void or_step_hash(const OrWorld *w, uint8_t out[32]) {
         for (uint32_t j = i; j < e; j++) { hv3(&h, w->warm[j].la); hf(&h, w->warm[j].jn); hv3(&h, w->warm[j].jt); }
         i = e;
     }
-    hu64(&h, 0); /* warm_patches */
+    hu64(&h, w->nwpatch); /* warm_patches, in key order */
+    for (uint32_t i = 0; i < w->nwpatch; i++) {
+        for (int k = 0; k < 4; k++) hu32(&h, w->wpatch[i].key[k]);
+        hv3(&h, w->wpatch[i].jt); hf(&h, w->wpatch[i].tw);
+    }
     uint64_t nwj = 0;
     for (uint32_t q = 0; q < w->nj; q++) nwj += w->wj[q].has;
     hu64(&h, nwj); /* warm_joints, in joint order */
@@ -2142,6 +2443,7 @@ typedef struct {
     uint64_t tick; uint32_t n, cap;
     V3 *pos; Q4 *rot; V3 *lin, *ang;
     Warm *warm; uint32_t nwarm;
+    WarmP *wpatch; uint32_t nwpatch;
     V3 *last_lin; uint8_t *awake; uint16_t *sticks;
     WarmJ *wj; uint32_t nj;
     OrPlayer *players; uint32_t np;
@@ -2156,6 +2458,7 @@ void *or_snapshot(const OrWorld *w) {
         else { s->pos[i] = v3s(0); s->rot[i] = qident(); s->lin[i] = v3s(0); s->ang[i] = v3s(0); }
     }
     s->warm = malloc(sizeof(Warm) * (w->nwarm + 1)); memcpy(s->warm, w->warm, sizeof(Warm) * w->nwarm); s->nwarm = w->nwarm;
+    s->wpatch = malloc(sizeof(WarmP) * (w->nwpatch + 1)); memcpy(s->wpatch, w->wpatch, sizeof(WarmP) * w->nwpatch); s->nwpatch = w->nwpatch;
     s->last_lin = malloc(sizeof(V3) * (w->n + 1)); memcpy(s->last_lin, w->last_lin, sizeof(V3) * w->n);
     s->awake = malloc(w->cap + 1); memcpy(s->awake, w->awake, w->cap);
     s->sticks = malloc(2 * (w->cap + 1)); memcpy(s->sticks, w->sticks, 2 * w->cap);
@@ -2188,6 +2491,16 @@ void or_restore(OrWorld *w, const void *sp) {
         if (la && lb) w->warm[k++] = w->warm[i];
     }
     w->nwarm = k;
+    free(w->wpatch);
+    w->wpatch = malloc(sizeof(WarmP) * (s->nwpatch + 1));
+    k = 0;
+    for (uint32_t i = 0; i < s->nwpatch; i++) {
+        uint32_t a = s->wpatch[i].key[0], b = s->wpatch[i].key[1];
+        int la = a == 0xFFFFFFFFu || (a < w->cap && w->alive[a]);
+        int lb = b == 0xFFFFFFFFu || (b < w->cap && w->alive[b]);
+        if (la && lb) w->wpatch[k++] = s->wpatch[i];
+    }
+    w->nwpatch = k;
     /* warm joint rows as snapshot (joints added since start cold); a dead body's joints drop theirs */
     for (uint32_t q = 0; q < w->nj; q++) {
         if (q < s->nj) w->wj[q] = s->wj[q]; else memset(&w->wj[q], 0, sizeof(WarmJ));
@@ -2197,12 +2510,21 @@ void or_restore(OrWorld *w, const void *sp) {
 void or_snapshot_free(void *sp) {
     Snap *s = sp;
     if (!s) return;
-    free(s->pos); free(s->rot); free(s->lin); free(s->ang); free(s->warm); free(s->last_lin); free(s->awake); free(s->sticks); free(s->wj);
+    free(s->pos); free(s->rot); free(s->lin); free(s->ang); free(s->warm); free(s->wpatch); free(s->last_lin); free(s->awake); free(s->sticks); free(s->wj);
     free(s->players);
     free(s);
 }
 
 /* ------------------------------------------------------------------ unit entry points */
+/* patch_radius of n plane points (x, y pairs) */
+float or_patch_radius(const float *xy, uint32_t n) {
+    P2 *p = malloc(sizeof(P2) * (n + 1));
+    for (uint32_t i = 0; i < n; i++) { p[i].x = xy[2 * i]; p[i].y = xy[2 * i + 1]; }
+    float r = patch_radius(p, n);
+    free(p);
+    return r;
+}
+float or_atan2f_unit(float y, float x) { return or_atan2f(y, x); }
 static void write_m(const Manifold *m, float *out) {
     for (int i = 0; i < m->len; i++) {
         out[7 * i] = m->p[i].pos.x; out[7 * i + 1] = m->p[i].pos.y; out[7 * i + 2] = m->p[i].pos.z;

@@ -235,4 +235,31 @@ static inline float or_expf_d(float xf) {
     return (float)ldexp(t, (int)k);
 }
 
+/* atan2 for ordering a friction patch's points by angle (solver.rs:233-236 sorts by
+ * sort_key6(y.atan2(x))): a once-rounded double evaluation, the same operations on the device.
+ * Only the order of the keys matters there, and a correctly rounded angle orders as any accurate
+ * f32 atan2 does unless two points' angles lie within a few 1e-7 rad.  Finite inputs.
+ * atan on [0, 1]: tan(pi/8) reduction, then the odd series to t^49 in Horner form (|t| <= 0.4143,
+ * truncation below 1e-20). */
+static inline double or_atan_unit(double t) {
+    double base = 0.0;
+    if (t > 0.41421356237309503) { t = (t - 1.0) / (t + 1.0); base = 0.78539816339744828; }
+    const double t2 = t * t;
+    double p = 1.0 / 49.0;
+    for (int k = 23; k >= 0; k--) p = 1.0 / (double)(2 * k + 1) - t2 * p;
+    return base + t * p;
+}
+static inline float or_atan2f(float y, float x) {
+    if (x != x || y != y) return x + y;
+    const double ay = fabs((double)y), ax = fabs((double)x);
+    double a;
+    if (ay == 0.0 && ax == 0.0) a = signbit(x) ? 3.141592653589793 : 0.0;
+    else {
+        a = ay <= ax ? or_atan_unit(ay / ax) : 1.5707963267948966 - or_atan_unit(ax / ay);
+        if (signbit(x)) a = 3.141592653589793 - a;
+    }
+    const float r = (float)a;
+    return signbit(y) ? -r : r;
+}
+
 #endif

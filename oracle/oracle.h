@@ -62,6 +62,7 @@ typedef struct {
      * most interior-point solves any component took */
     uint32_t toi_rows, support_rows, pseudo_rows, bounce_components, bounce_fallbacks, bounce_solves;
     uint32_t relax_rows;   /* contacts bounced by the restitution relax pass */
+    uint32_t patches;      /* friction patches of the last step */
 } OrStats;
 
 OrWorld *or_world_create(uint32_t max_bodies);
@@ -77,6 +78,15 @@ void or_set_iterations(OrWorld *w, uint32_t vel, uint32_t pos);
 void or_set_residual_mode(OrWorld *w, int on, float tol, uint32_t max_iters);
 void or_set_pair_props(OrWorld *w, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
                        float restitution, float stribeck_v);
+/* materials.rs:75-90: the pair's contact model (stiffness > 0: compliant Hunt-Crossley contact with
+ * that dissipation; patch: a manifold's friction solved as one patch); world.rs:992: damping
+ * linearised at the step-start penetration instead of the predicted mid-step one */
+void or_set_pair_contact_model(OrWorld *w, uint32_t mat_a, uint32_t mat_b, float stiffness, float dissipation, int patch);
+void or_set_compliant_damping_at_start(OrWorld *w, int on);
+/* unit entry points: a friction patch's mean radius over n plane points (x, y pairs), and the
+ * patch point order's atan2 */
+float or_patch_radius(const float *xy, uint32_t n);
+float or_atan2f_unit(float y, float x);
 /* shape: OR_*; dims: sphere {r}, box {hx,hy,hz}, capsule {r,hh}; pose {px,py,pz,qx,qy,qz,qw};
  * vel {lx,ly,lz,ax,ay,az}; density <= 0 means the reference default (1500 kg/m^3). Returns the
  * 0-based body id. */

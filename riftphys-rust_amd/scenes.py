@@ -357,3 +357,29 @@ def bounce_drone(hz=60.0):
     (restitution_toi_test.rs:612-614)."""
     bodies = [bounce_floor(MAT_CONCRETE), bounce_body(1, (0.2, 0.05, 0.2), (0.0, 2.0, 0.0), 1.5, MAT_ALUMINUM)]
     return bounce_scene_of(bodies, hz=hz, name="bounce_drone")
+
+
+# Contact-model scenes: a 40 x 1 x 40 m static box whose top is at y = 0 (tilted about z by
+# ground_tilt radians around the origin) and one dynamic body of mass 1 kg (box dims = half
+# extents, or a sphere of radius dims[0]) that may not sleep, on a pair whose contact model is set
+# (compliant_contact_test.rs:10-27, patch_friction_test.rs:16-33).  Friction mu on both
+# coefficients; the restitution and Stribeck speed given are unused by compliant rows and by equal
+# coefficients.
+MAT_STEEL, MAT_PMMA, MAT_WOOD = 3, 5, 9
+
+
+def contact_model_scene(k, d, mu, shape, dims, pos, vel=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0), hz=1000.0, gravity=True,
+                        patch=False, mass=1.0, ground_tilt=0.0, rot=(0.0, 0.0, 0.0, 1.0), rest=0.0, mats=(MAT_STEEL, MAT_CONCRETE),
+                        name="contact_model"):
+    f = np.float32
+    q = (0.0, 0.0, math.sin(ground_tilt / 2.0), math.cos(ground_tilt / 2.0))
+    # the ground's centre: the rotation applied to (0, -0.5, 0)
+    c, s_ = math.cos(ground_tilt), math.sin(ground_tilt)
+    gpos = (0.5 * s_, -0.5 * c, 0.0)
+    ground = dict(shape=1, dims=np.array([20.0, 0.5, 20.0], f), pose=np.array([*gpos, *q], f), vel=np.zeros(6, f), dynamic=0,
+                  density=f(0.0), material=mats[1], sleep_allowed=1)
+    body = bounce_body(shape, dims, pos, mass, mats[0], rot=rot, vel=(*vel, *ang))
+    s = bounce_scene_of([ground, body], gravity=gravity, hz=hz, name=name)
+    s["pairs"] = [(mats[0], mats[1], f(mu), f(mu), f(rest), f(0.1))]
+    s["contact_models"] = [(mats[0], mats[1], f(k), f(d), bool(patch))]
+    return s

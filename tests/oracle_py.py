@@ -16,7 +16,7 @@ class OrStats(C.Structure):
                 ("iters_used", C.c_uint32), ("residual", C.c_float), ("iters_sum", C.c_uint32),
                 ("toi_rows", C.c_uint32), ("support_rows", C.c_uint32), ("pseudo_rows", C.c_uint32),
                 ("bounce_components", C.c_uint32), ("bounce_fallbacks", C.c_uint32), ("bounce_solves", C.c_uint32),
-                ("relax_rows", C.c_uint32)]
+                ("relax_rows", C.c_uint32), ("patches", C.c_uint32)]
 
 
 class OrContact(C.Structure):
@@ -123,6 +123,12 @@ def lib():
         L.or_collide_sphere_triangle.restype = C.c_int
         L.or_collide_sphere_triangle.argtypes = [C.c_float, FP, FP, C.c_float, FP]
         L.or_sincosf.argtypes = [C.c_float, FP, FP]
+        L.or_set_pair_contact_model.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_int]
+        L.or_set_compliant_damping_at_start.argtypes = [P, C.c_int]
+        L.or_patch_radius.restype = C.c_float
+        L.or_patch_radius.argtypes = [FP, C.c_uint32]
+        L.or_atan2f_unit.restype = C.c_float
+        L.or_atan2f_unit.argtypes = [C.c_float, C.c_float]
         U = C.POINTER(C.c_uint32)
         L.or_load_state.argtypes = [P, FP, FP, FP, FP, FP, U, U, C.c_uint64, C.POINTER(C.c_uint64), FP, FP, FP, C.c_uint32]
         L.or_broadphase.restype = C.c_uint32
@@ -156,6 +162,10 @@ class OracleWorld:
         L = lib()
         for a, b, ms, mk, e, vs in s.get("pairs", []):
             L.or_set_pair_props(self.h, a, b, ms, mk, e, vs)
+        for a, b, k, d, patch in s.get("contact_models", []):
+            L.or_set_pair_contact_model(self.h, a, b, k, d, int(patch))
+        if s.get("damping_at_start"):
+            L.or_set_compliant_damping_at_start(self.h, 1)
         vi, pi = s.get("iters", (20, 20))
         L.or_set_iterations(self.h, vi, pi)
         if s.get("residual"):

@@ -187,6 +187,21 @@ RPhysResult rphys_world_add_ball_joint(RPhysWorld *world, uint32_t body_a, uint3
 uint32_t rphys_world_num_joints(const RPhysWorld *world);
 RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
                                        float restitution, float stribeck_v);
+/* World::set_pair_props' contact model (materials.rs:75-90 MatPairProps stiffness, dissipation,
+ * patch_friction; the pair's friction and restitution stay): stiffness > 0 (N/m per contact point)
+ * makes the pair compliant, an implicit Hunt-Crossley row with that dissipation (s/m) in place of
+ * the rigid normal row, with no restitution, shock, projection or warm start (solver.rs:545-567,
+ * 848-856); patch_friction solves each manifold's friction as one patch at the points' centroid,
+ * sliding and torsion limited by the manifold's normal impulse (solver.rs:592-619, 882-915).  Like
+ * the other pair properties, a batch uses its first world's.  0 and RPHYS_FALSE restore the rigid
+ * model.  Errors: ids above 15 or a negative or non-finite stiffness, non-finite dissipation ->
+ * ERR_BAD_ARG. */
+RPhysResult rphys_world_set_pair_contact_model(RPhysWorld *world, uint32_t mat_a, uint32_t mat_b, float stiffness, float dissipation,
+                                               RPhysBool patch_friction);
+/* World::set_compliant_damping_at_start (world.rs:992): the compliant rows' damping linearised at the
+ * step-start penetration instead of the predicted mid-step one (default off; a batch uses its
+ * first world's). */
+RPhysResult rphys_world_set_compliant_damping_at_start(RPhysWorld *world, RPhysBool on);
 /* World::set_solver_iterations (world.rs:994-996), (velocity, position), default (20, 20) */
 RPhysResult rphys_world_set_solver_iterations(RPhysWorld *world, uint32_t velocity, uint32_t position);
 /* Residual mode (build addition; the reference runs a fixed count): each island (component of the

@@ -1504,11 +1504,14 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         disl[q] = uf_find(par, w->inv_mass[j->a] > 0.0f ? j->a : j->b);
         isl_rows[disl[q]]++;
     }
-    /* the sweep statistics count a joint with rows as one entry of its island, like a contact */
+    /* the sweep statistics count a joint with rows as one entry of its island, like a contact, and
+     * so a friction patch */
     for (uint32_t q = 0; q < w->nj; q++) {
         if (jr1[q] > jr0[q]) isl_n[jisl[jr0[q]]]++;
         else if (dact[q]) isl_n[disl[q]]++;
     }
+    for (uint32_t k = 0; k < npt; k++)
+        if (isl[ppts[pt[k].p0]] != 0xFFFFFFFFu) isl_n[isl[ppts[pt[k].p0]]]++;
 
     /* PGS iterations (solver.rs:788-880) as the reference's level batches (solver.rs:621-687):
      * joints by joint level, then contacts by contact level, each batch in index order.  Rows of

@@ -110,7 +110,9 @@ static MatP table_pair(uint32_t a, uint32_t b) {
 }
 
 // ------------------------------------------------------------------ host world mirror
-struct PairOverride { uint32_t a, b; MatP p; };
+// a pair's properties: friction and restitution, and its contact model (materials.rs:75-90:
+// stiffness > 0 compliant with that dissipation; patch friction)
+struct PairOverride { uint32_t a, b; MatP p; float stiff = 0.0f, diss = 0.0f; uint32_t patch = 0; };
 struct HostWorld {
     uint32_t max_bodies = 0, max_colliders = 0, bodies_created = 0;
     // bodies
@@ -132,11 +134,15 @@ struct HostWorld {
     uint32_t ws_mode = 0;                    // warm start: 0 previous frame, 1 zero, 2 learned
     std::vector<float> mlp;                  // the learned model (RP_MLP_NPARAM floats) for mode 2
     std::vector<PairOverride> ovr;
+    uint32_t damp_start = 0;  // compliant rows' damping at the step-start penetration (world.rs:992)
     uint64_t epoch = 0, rng = 0xC0FFEEull | 1ull, tick = 0;
     // carried warm cache
     std::vector<uint64_t> wkey;
     std::vector<F4> wla, wjt;
     std::vector<float> wjn;
+    // carried friction-patch impulses (world.rs:249 warm_patches): keys, (jt, tw)
+    std::vector<uint64_t> pkey;
+    std::vector<F4> pjt;
     // ball joints (world.rs:718 add_ball_joint): body indices, frames (position, rotation) in each
     // body's space, and the carried warm rows (mask: bit 31 stored, bits 0-2 the rows' axes)
     std::vector<uint32_t> jt_a, jt_b;
@@ -152,6 +158,14 @@ struct HostWorld {
     uint32_t nj() const { return (uint32_t)jt_a.size(); }
 };
 
+// The override of pair (a, b), made from the pair's table properties (rigid) when absent.
+static PairOverride &pair_override(HostWorld &w, uint32_t a, uint32_t b) {
+    const uint32_t x = std::min(a, b), y = std::max(a, b);
+    for (auto &o : w.ovr) if (o.a == x && o.b == y) return o;
+    w.ovr.push_back(PairOverride{x, y, table_pair(x, y)});
+    return w.ovr.back();
+}
+
 static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
     size_t k = 0;
     for (size_t i = 0; i < w.wkey.size(); i++) {
@@ -160,6 +174,13 @@ static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
         w.wkey[k] = w.wkey[i]; w.wla[k] = w.wla[i]; w.wjn[k] = w.wjn[i]; w.wjt[k] = w.wjt[i]; k++;
     }
     w.wkey.resize(k); w.wla.resize(k); w.wjn.resize(k); w.wjt.resize(k);
+    k = 0;
+    for (size_t i = 0; i < w.pkey.size(); i++) {  // its patches' (world.rs:820)
+        uint32_t a = (uint32_t)(w.pkey[i] >> 32), c = (uint32_t)w.pkey[i];
+        if (a == b || c == b) continue;
+        w.pkey[k] = w.pkey[i]; w.pjt[k] = w.pjt[i]; k++;
+    }
+    w.pkey.resize(k); w.pjt.resize(k);
     for (uint32_t q = 0; q < w.nj(); q++)  // and its joints' warm rows (world.rs:822-826)
         if (w.jt_a[q] == b || w.jt_b[q] == b) w.wj_mask[q] = 0;
 }
@@ -193,7 +214,8 @@ struct Slot {  // device-side snapshot of a scene
     uint64_t *wkey; float4 *wla; float *wjn; float4 *wjt; uint32_t *woff;
     uint32_t *wj_mask; float *wj_lam;
     float4 *pl_state, *pl_input; uint32_t *pl_grounded;
-    uint32_t cur; uint32_t nwarm;
+    uint64_t *pkey; float4 *pjt; uint32_t *poff;  // carried patch impulses (contact-model scenes)
+    uint32_t cur; uint32_t nwarm, npatch;
     size_t cap_warm = 0;  // warm-cache entries the slot's buffers hold
     std::vector<uint64_t> ticks;
 };
@@ -271,6 +293,14 @@ static bool scene_bounces(const std::vector<HostWorld *> &ws) {
     return false;
 }
 
+// Whether the scene needs the contact models: a pair override of the first world (the uploaded
+// table) that is compliant or patch friction, or carried patch impulses in any world.
+static bool scene_cm(const std::vector<HostWorld *> &ws) {
+    for (auto &o : ws[0]->ovr) if (o.stiff > 0.0f || o.patch) return true;
+    for (auto *w : ws) if (!w->pkey.empty()) return true;
+    return false;
+}
+
 // Builds a device scene for the given worlds (host mirrors) with the given capacities and
 // uploads everything.  Returns nullptr (with the error set) on failure.
 static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_t cur) {
@@ -287,7 +317,10 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         uint32_t cap = std::max(w->max_bodies, w->nb());
         S->max_extra = std::max(S->max_extra, cap - w->nb());
     }
-    caps.warm = std::max(caps.warm, nwarm + 1024u);
+    uint32_t npk = 0;
+    for (auto *w : ws) npk += (uint32_t)w->pkey.size();
+    caps.warm = std::max(caps.warm, std::max(nwarm, npk) + 1024u);
+    d.cm_on = scene_cm(ws) ? 1u : 0u;
     d.nb = nb; d.nc = nc; d.nw = nw; d.nj = nj;
     d.cap_pairs = caps.pairs; d.cap_cands = caps.cands; d.cap_contacts = caps.contacts; d.cap_warm = caps.warm;
     DeviceBuffers &B = S->buf;
@@ -355,6 +388,17 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         d.w6_ws = wsp;
     }
     for (int k = 0; k < 2; k++) ok &= B.alloc(d.wk_key[k], W) && B.alloc(d.wk_la[k], W) && B.alloc(d.wk_jn[k], W) && B.alloc(d.wk_jt[k], W) && B.alloc(d.wc_off[k], nw + 1);
+    ok &= B.alloc(d.tier_list[4], d.cm_on ? C + 1 : 1);
+    if (d.cm_on) {  // contact models: per pair, per contact, per patch (one per two contacts), the patch cache
+        d.cap_patch = (uint32_t)(C / 2 + 1);
+        const size_t PT = d.cap_patch + 1;
+        ok &= B.alloc(d.pair_cm, 256) && B.alloc(d.cs_soft, C) && B.alloc(d.ct_patch, C) && B.alloc(d.pt_head, C + 1) && B.alloc(d.isl_cm, C + 1);
+        ok &= B.alloc(d.pt_first, PT) && B.alloc(d.pt_n, PT) && B.alloc(d.pt_ai, PT) && B.alloc(d.pt_bi, PT) && B.alloc(d.pt_inv, PT) &&
+              B.alloc(d.pt_ra, PT) && B.alloc(d.pt_rb, PT) && B.alloc(d.pt_nn, PT) && B.alloc(d.pt_t1, PT) && B.alloc(d.pt_t2, PT) &&
+              B.alloc(d.pt_m, PT) && B.alloc(d.pt_mu, PT) && B.alloc(d.pt_acc, PT) && B.alloc(d.pt_key, PT) && B.alloc(d.pt_soft, PT);
+        for (int k = 0; k < 2; k++) ok &= B.alloc(d.wp_key[k], W) && B.alloc(d.wp_jt[k], W) && B.alloc(d.wpc_off[k], nw + 1);
+        ok &= B.alloc(d.wp_new, PT) && B.alloc(d.wp_keep, W + 1);
+    }
     ok &= B.alloc(d.newflag, C + 1) && B.alloc(d.keepflag, W + 1);
     size_t scan_elems = std::max<size_t>({NB, NC, C + 1, W + 1, 256 * (size_t)d.rs_blocks + 1, (size_t)nw + 8});
     ok &= B.alloc(d.scan_tmp, scan_elems / 1024 + 2 * nw + 64);
@@ -375,6 +419,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         bytes += 48 + 56 * n + 8 + 24 + 8 + 12 * n + 8 + cap + 8 + 2 * cap + 1024 + 23ull * w->nj();
     }
     bytes += 48ull * W;
+    if (d.cm_on) bytes += 32ull * W;  // carried patch entries
     h.cap_chunks = (uint32_t)(bytes / 1024 + nw + 1);
     ok &= B.alloc(h.w_jbytes, nw + 1) && B.alloc(h.j_off, nj + 1);
     ok &= B.alloc(h.keyflag, W + 1) && B.alloc(h.w_chunk, nw + 1) && B.alloc(h.w_len, nw) && B.alloc(h.w_epoch, nw) && B.alloc(h.w_rng, nw);
@@ -393,8 +438,10 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     std::vector<uint64_t> wkey, ep, rn;
     std::vector<F4> wla, wjt;
     std::vector<float> wjn;
-    std::vector<uint32_t> woff;
-    woff.push_back(0);
+    std::vector<uint32_t> woff, poff_w;
+    std::vector<uint64_t> pkey;
+    std::vector<F4> pjt;
+    woff.push_back(0); poff_w.push_back(0);
     for (uint32_t wi = 0; wi < nw; wi++) {
         HostWorld *w = ws[wi];
         uint32_t bo = S->body_off[wi];
@@ -417,6 +464,8 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         wkey.insert(wkey.end(), w->wkey.begin(), w->wkey.end()); wla.insert(wla.end(), w->wla.begin(), w->wla.end());
         wjn.insert(wjn.end(), w->wjn.begin(), w->wjn.end()); wjt.insert(wjt.end(), w->wjt.begin(), w->wjt.end());
         woff.push_back((uint32_t)wkey.size());
+        pkey.insert(pkey.end(), w->pkey.begin(), w->pkey.end()); pjt.insert(pjt.end(), w->pjt.begin(), w->pjt.end());
+        poff_w.push_back((uint32_t)pkey.size());
         ep.push_back(w->epoch); rn.push_back(w->rng);
     }
     // joints (global body indices), their warm rows, each body's jointed partners, world offsets
@@ -442,14 +491,15 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         std::vector<uint32_t> fill(jadj_off.begin(), jadj_off.end() - 1);
         for (size_t q = 0; q < ja.size(); q++) { jadj[fill[ja[q]]++] = jb[q]; jadj[fill[jb[q]]++] = ja[q]; }
     }
-    std::vector<F4> ptab(256);
+    std::vector<F4> ptab(256), pcm(256, f4(0, 0, 0, 0));
     {
         HostWorld *w0 = ws[0];
         for (uint32_t a = 0; a < 16; a++)
             for (uint32_t b = 0; b < 16; b++) {
                 MatP p = table_pair(a, b);
                 uint32_t x = std::min(a, b), y = std::max(a, b);
-                for (auto &o : w0->ovr) if (o.a == x && o.b == y) p = o.p;
+                for (auto &o : w0->ovr)
+                    if (o.a == x && o.b == y) { p = o.p; pcm[a * 16 + b] = f4(o.stiff, o.diss, o.patch ? 1.0f : 0.0f); }
                 ptab[a * 16 + b] = f4(p.mu_s, p.mu_k, p.rest, p.vs);
             }
     }
@@ -487,6 +537,9 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         if (np) ok = ok && up(d.pl_body, pbody.data(), np * 4) && up(d.pl_grounded, pgnd.data(), np * 4) && up(d.pl_of_body, pof.data(), pof.size() * 4) &&
                      up(d.pl_state, pst.data(), np * 16) && up(d.pl_input, pin.data(), np * 16);
     }
+    if (ok && d.cm_on)
+        ok = up(d.pair_cm, pcm.data(), pcm.size() * 16) && up(d.wp_key[cur], pkey.data(), pkey.size() * 8) &&
+             up(d.wp_jt[cur], pjt.data(), pjt.size() * 16) && up(d.wpc_off[cur], poff_w.data(), poff_w.size() * 4);
     if (ok) ok = hip_ok(hipMemsetAsync(d.cnt, 0, CT_COUNT * 4, S->st), "memset");
     if (ok && ws[0]->mlp.size() == RP_MLP_NPARAM) ok = upload_model(*S, ws[0]->mlp.data());
     S->col_body = cbody;
@@ -515,6 +568,15 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
         uint32_t joff[2];
         ok = dn(joff, d.w_joint_off + wi, 8) && hip_ok(hipStreamSynchronize(S.st), "download sync") &&
              dn(w.wj_mask.data(), d.wj_mask + joff[0], w.nj() * 4) && dn(w.wj_lam.data(), d.wj_lam + 3 * (size_t)joff[0], 3 * (size_t)w.nj() * 4);
+    }
+    if (ok && d.cm_on) {
+        uint32_t po[2];
+        ok = dn(po, d.wpc_off[S.cur] + wi, 8) && hip_ok(hipStreamSynchronize(S.st), "download sync");
+        if (ok) {
+            const uint32_t np = po[1] - po[0];
+            w.pkey.resize(np); w.pjt.resize(np);
+            ok = dn(w.pkey.data(), d.wp_key[S.cur] + po[0], (size_t)np * 8) && dn(w.pjt.data(), d.wp_jt[S.cur] + po[0], (size_t)np * 16);
+        }
     }
     if (ok && !w.players.empty() && d.np) {
         uint32_t poff = 0;
@@ -569,6 +631,7 @@ static TickParams tick_params(const HostWorld &w, float dt, uint32_t iv) {
     tp.iv = iv; tp.ip = w.ip;
     if (w.resid_on) { tp.resid_on = 1; tp.resid_tol = w.resid_tol; tp.iv = w.resid_max; }
     tp.ws_mode = w.ws_mode;
+    tp.damp_start = w.damp_start;
     // one island per thread serialises its rows over every projection iteration: worth it for a
     // few iterations only
     tp.lane_pos = (tp.lane_max && tp.ip <= RP_LANE_POS_ITERS) ? 1u : 0u;
@@ -588,7 +651,7 @@ static Caps initial_caps(const std::vector<HostWorld *> &ws) {
     c.cands = (uint32_t)std::min<uint64_t>(96 * nc + 4096, 0x7FFFFFFF);
     uint64_t nj = 0;
     for (auto *w : ws) nj += w->nj();
-    c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + nj + 1024, 0x7FFFFFFF);  // the island lists hold joints too
+    c.contacts = (uint32_t)std::min<uint64_t>(12 * nc + nj + 1024, RP_PBIT - 1);  // the island lists hold joints too; entries stay below RP_PBIT
     c.warm = (uint32_t)std::min<uint64_t>(12 * nc + 1024, 0x7FFFFFFF);
     if (g_init_caps[0].load()) c.pairs = g_init_caps[0].load();
     if (g_init_caps[1].load()) c.cands = g_init_caps[1].load();
@@ -705,7 +768,7 @@ static bool run_ticks(Scene *&S, std::vector<HostWorld *> &ws, float dt, uint32_
         if (!hip_ok(hipStreamSynchronize(S->st), "restore")) return false;
         auto grow = [](uint32_t cap, uint32_t need) -> uint32_t {
             uint64_t v = std::max<uint64_t>((uint64_t)need + need / 2 + 1024, (uint64_t)cap * 2);
-            return (uint32_t)std::min<uint64_t>(v, 0x7FFFFFFF);
+            return (uint32_t)std::min<uint64_t>(v, RP_PBIT - 1);  // island entries stay below RP_PBIT
         };
         if (cnt[CT_NEED_PAIRS]) caps.pairs = grow(caps.pairs, cnt[CT_NEED_PAIRS]);
         if (cnt[CT_NEED_CANDS]) caps.cands = grow(caps.cands, cnt[CT_NEED_CANDS]);
@@ -1634,11 +1697,30 @@ RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t a, uint32_t b
     if (!(hfin(mu_s) && hfin(mu_k) && hfin(rest) && hfin(vs))) return bad_arg("rphys_world_set_pair_props", "non-finite property");
     std::lock_guard<std::mutex> lk(world->mu);
     if (!pull(world)) return RPHYS_ERR_INTERNAL;
-    uint32_t x = std::min(a, b), y = std::max(a, b);
-    auto &o = world->w.ovr;
-    o.erase(std::remove_if(o.begin(), o.end(), [&](const PairOverride &p) { return p.a == x && p.b == y; }), o.end());
-    o.push_back(PairOverride{x, y, MatP{mu_s, mu_k, rest, vs}});
+    pair_override(world->w, a, b).p = MatP{mu_s, mu_k, rest, vs};  // the pair's contact model stays
     host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_set_pair_contact_model(RPhysWorld *world, uint32_t a, uint32_t b, float stiffness, float dissipation, RPhysBool patch_friction) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_pair_contact_model: world is null"); return RPHYS_ERR_NULL; }
+    if (a > 15 || b > 15) return bad_arg("rphys_world_set_pair_contact_model", "material ids above 15 are not reachable through this ABI");
+    if (!(hfin(stiffness) && stiffness >= 0.0f)) return bad_arg("rphys_world_set_pair_contact_model", "stiffness must be finite and >= 0");
+    if (!hfin(dissipation)) return bad_arg("rphys_world_set_pair_contact_model", "dissipation must be finite");
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    PairOverride &o = pair_override(world->w, a, b);  // its friction and restitution stay
+    o.stiff = stiffness; o.diss = dissipation; o.patch = patch_friction ? 1u : 0u;
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_set_compliant_damping_at_start(RPhysWorld *world, RPhysBool on) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_compliant_damping_at_start: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    world->w.damp_start = on ? 1u : 0u;
     return RPHYS_OK;
     GUARD_END
 }
@@ -1836,6 +1918,8 @@ struct RPhysWorldSnapshot {
     std::vector<uint32_t> wj_mask;
     std::vector<float> wj_lam;
     std::vector<HostWorld::Player> players;
+    std::vector<uint64_t> pkey;
+    std::vector<F4> pjt;
 };
 RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
     clear_err();
@@ -1854,6 +1938,7 @@ RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
         s->wkey = w.wkey; s->wla = w.wla; s->wjn = w.wjn; s->wjt = w.wjt;
         s->wj_mask = w.wj_mask; s->wj_lam = w.wj_lam;
         s->players = w.players;
+        s->pkey = w.pkey; s->pjt = w.pjt;
         return s;
     } catch (...) {
         set_err("panic in rphys_world_snapshot_create");
@@ -1881,6 +1966,11 @@ RPhysResult rphys_world_snapshot_restore(RPhysWorld *world, const RPhysWorldSnap
     for (size_t k = 0; k < s->wkey.size(); k++) {
         if (!alive((uint32_t)(s->wkey[k] >> 32)) || !alive((uint32_t)s->wkey[k])) continue;
         w.wkey.push_back(s->wkey[k]); w.wla.push_back(s->wla[k]); w.wjn.push_back(s->wjn[k]); w.wjt.push_back(s->wjt[k]);
+    }
+    w.pkey.clear(); w.pjt.clear();
+    for (size_t k = 0; k < s->pkey.size(); k++) {
+        if (!alive((uint32_t)(s->pkey[k] >> 32)) || !alive((uint32_t)s->pkey[k])) continue;
+        w.pkey.push_back(s->pkey[k]); w.pjt.push_back(s->pjt[k]);
     }
     for (uint32_t i = 0; i < s->last_lin.size() && i < w.nb(); i++) w.last_lin[i] = s->last_lin[i];
     for (uint32_t i = 0; i < s->awake.size() && i < w.nb(); i++) { w.awake[i] = s->awake[i]; w.sticks[i] = s->sticks[i]; }
@@ -2000,19 +2090,23 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
     Scene &S = *b->scene;
     if (S.slots.size() <= slot) S.slots.resize(slot + 1);
     const Dev &d = S.d;
-    uint32_t nwarm = 0;
-    if (!hip_ok(hipMemcpyAsync(&nwarm, d.wc_off[S.cur] + d.nw, 4, hipMemcpyDeviceToHost, S.st), "snapshot") || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
-    // A slot made before a capacity rebuild can be smaller than today's warm cache: replace it.
-    if (S.slots[slot] && S.slots[slot]->cap_warm < nwarm) S.slots[slot].reset();
+    uint32_t nwarm = 0, npatch = 0;
+    if (!hip_ok(hipMemcpyAsync(&nwarm, d.wc_off[S.cur] + d.nw, 4, hipMemcpyDeviceToHost, S.st), "snapshot") ||
+        (d.cm_on && !hip_ok(hipMemcpyAsync(&npatch, d.wpc_off[S.cur] + d.nw, 4, hipMemcpyDeviceToHost, S.st), "snapshot")) ||
+        !hip_ok(hipStreamSynchronize(S.st), "snapshot"))
+        return RPHYS_ERR_INTERNAL;
+    // A slot made before a capacity rebuild can be smaller than today's caches: replace it.
+    if (S.slots[slot] && S.slots[slot]->cap_warm < std::max(nwarm, npatch)) S.slots[slot].reset();
     if (!S.slots[slot]) {
         std::unique_ptr<Slot> sl(new Slot());
-        size_t NB = d.nb + 1, W = std::max<size_t>(d.cap_warm, nwarm);
+        size_t NB = d.nb + 1, W = std::max<size_t>(d.cap_warm, std::max(nwarm, npatch));
         sl->cap_warm = W;
         bool ok = sl->buf.alloc(sl->pos, NB) && sl->buf.alloc(sl->rot, NB) && sl->buf.alloc(sl->lin, NB) && sl->buf.alloc(sl->ang, NB) &&
                   sl->buf.alloc(sl->last, NB) && sl->buf.alloc(sl->awake, NB) && sl->buf.alloc(sl->sticks, NB) &&
                   sl->buf.alloc(sl->wkey, W) && sl->buf.alloc(sl->wla, W) && sl->buf.alloc(sl->wjn, W) && sl->buf.alloc(sl->wjt, W) &&
                   sl->buf.alloc(sl->woff, d.nw + 1) && sl->buf.alloc(sl->wj_mask, d.nj + 1) && sl->buf.alloc(sl->wj_lam, 3 * ((size_t)d.nj + 1)) &&
-                  sl->buf.alloc(sl->pl_state, d.np + 1) && sl->buf.alloc(sl->pl_input, d.np + 1) && sl->buf.alloc(sl->pl_grounded, d.np + 1);
+                  sl->buf.alloc(sl->pl_state, d.np + 1) && sl->buf.alloc(sl->pl_input, d.np + 1) && sl->buf.alloc(sl->pl_grounded, d.np + 1) &&
+                  sl->buf.alloc(sl->pkey, d.cm_on ? W : 1) && sl->buf.alloc(sl->pjt, d.cm_on ? W : 1) && sl->buf.alloc(sl->poff, d.nw + 1);
         if (!ok) { set_err("rphys_batch_snapshot: device allocation failed"); return RPHYS_ERR_INTERNAL; }
         S.slots[slot] = std::move(sl);
     }
@@ -2026,8 +2120,11 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
               cp(sl.woff, d.wc_off[S.cur], 4 * ((size_t)d.nw + 1)) && cp(sl.wj_mask, d.wj_mask, 4 * (size_t)d.nj) &&
               cp(sl.wj_lam, d.wj_lam, 12 * (size_t)d.nj) && cp(sl.pl_state, d.pl_state, 16 * (size_t)d.np) &&
               cp(sl.pl_input, d.pl_input, 16 * (size_t)d.np) && cp(sl.pl_grounded, d.pl_grounded, 4 * (size_t)d.np);
+    if (ok && d.cm_on)
+        ok = cp(sl.pkey, d.wp_key[S.cur], 8 * (size_t)npatch) && cp(sl.pjt, d.wp_jt[S.cur], 16 * (size_t)npatch) &&
+             cp(sl.poff, d.wpc_off[S.cur], 4 * ((size_t)d.nw + 1));
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
-    sl.cur = S.cur; sl.nwarm = nwarm; sl.valid = true;
+    sl.cur = S.cur; sl.nwarm = nwarm; sl.npatch = npatch; sl.valid = true;
     sl.ticks.clear();
     for (auto &w : b->worlds) sl.ticks.push_back(w.tick);
     return RPHYS_OK;
@@ -2044,7 +2141,7 @@ static RPhysResult batch_restore(RPhysBatch *b, uint32_t slot) {
     if (slot >= S.slots.size() || !S.slots[slot] || !S.slots[slot]->valid) return bad_arg("rphys_batch_restore", "empty slot");
     Slot &sl = *S.slots[slot];
     const Dev &d = S.d;
-    if (sl.nwarm > d.cap_warm) { set_err("rphys_batch_restore: warm cache larger than the scene's capacity"); return RPHYS_ERR_INTERNAL; }
+    if (sl.nwarm > d.cap_warm || sl.npatch > d.cap_warm) { set_err("rphys_batch_restore: warm cache larger than the scene's capacity"); return RPHYS_ERR_INTERNAL; }
     size_t n = d.nb;
     auto cp = [&](void *dst, const void *src, size_t bytes) { return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.st), "restore copy"); };
     // Snapshot poses are already sanitised (set_pose's normalisation maps them to themselves for
@@ -2056,6 +2153,9 @@ static RPhysResult batch_restore(RPhysBatch *b, uint32_t slot) {
               cp(d.wc_off[sl.cur], sl.woff, 4 * ((size_t)d.nw + 1)) && cp(d.wj_mask, sl.wj_mask, 4 * (size_t)d.nj) &&
               cp(d.wj_lam, sl.wj_lam, 12 * (size_t)d.nj) && cp(d.pl_state, sl.pl_state, 16 * (size_t)d.np) &&
               cp(d.pl_input, sl.pl_input, 16 * (size_t)d.np) && cp(d.pl_grounded, sl.pl_grounded, 4 * (size_t)d.np);
+    if (ok && d.cm_on)
+        ok = cp(d.wp_key[sl.cur], sl.pkey, 8 * (size_t)sl.npatch) && cp(d.wp_jt[sl.cur], sl.pjt, 16 * (size_t)sl.npatch) &&
+             cp(d.wpc_off[sl.cur], sl.poff, 4 * ((size_t)d.nw + 1));
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "restore")) return RPHYS_ERR_INTERNAL;
     S.cur = sl.cur;
     for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];

@@ -147,6 +147,9 @@ static __device__ __forceinline__ float w6_vn(const Dev &d, const W6C &k) { retu
 static __device__ __forceinline__ void w6_apply_n(const Dev &d, const W6C &k, float dj) {
     apply_cimp(d, k.ai, k.bi, k.inv_a, k.inv_b, k.ra, k.rb, k.n * dj);
 }
+// a compliant pair's contact: no support row in either pass, no growth through it (solver.rs:1020,
+// 1087, 1175)
+static __device__ __forceinline__ bool w6_soft(const Dev &d, uint32_t c) { return (cm_flags(d, c) & CM_SOFT) != 0u; }
 static __device__ __forceinline__ bool w6_loaded(const Dev &d, uint32_t c) {
     const float2 t = d.jt[c];
     return d.jn[c] > 0.0f || t.x != 0.0f || t.y != 0.0f;
@@ -506,7 +509,7 @@ static __device__ void w6_grow(const Dev &d, uint32_t c0, uint32_t c1, uint32_t 
         bool ch = false;
         for (uint32_t c = c0 + w6_lane(); c < c1; c += 64) {
             const uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
-            if (bi == RP_NONE || !w6_loaded(d, c)) continue;
+            if (bi == RP_NONE || w6_soft(d, c) || !w6_loaded(d, c)) continue;
             const float2 inv = d.cs_inv[c];
             const bool ia = (d.w6_b[2 * ai + 1] & W6B_INV) != 0, ib = (d.w6_b[2 * bi + 1] & W6B_INV) != 0;
             if (ia && !ib && inv.y > 0.0f && !w6_jointed(d, bi)) { atomicOr(&d.w6_b[2 * bi + 1], W6B_INV); ch = true; }
@@ -729,7 +732,7 @@ static __device__ void w6_world(const Dev &d, const TickParams &tp, const W6Ws &
     w6_grow(d, c0, c1, shu);
     // support rows (solver.rs:1081-1092): fresh push-only rows with target min(bias, 0)
     const uint32_t nsup = w6_compact(c0, c1, rc + ntoi, [&](uint32_t c) {
-        if ((d.w6_mark[c] & W6M_T) || d.jn[c] <= 0.0f) return false;
+        if ((d.w6_mark[c] & W6M_T) || w6_soft(d, c) || d.jn[c] <= 0.0f) return false;
         if (!w6_inv(d, d.cs_ai[c]) && !w6_inv(d, d.cs_bi[c])) return false;
         return w6_free_ok(d, c);
     });
@@ -794,7 +797,7 @@ static __device__ void w6_world(const Dev &d, const TickParams &tp, const W6Ws &
     for (uint32_t r = lane; r < np2; r += 64) { rc[r] = pc[grp[r]]; rt[r] = pu[grp[r]]; }
     w6_sync();
     const uint32_t nex = w6_compact(c0, c1, rc + np2, [&](uint32_t c) {
-        if ((d.w6_mark[c] & W6M_P) || !w6_loaded(d, c)) return false;
+        if ((d.w6_mark[c] & W6M_P) || w6_soft(d, c) || !w6_loaded(d, c)) return false;
         if (!w6_inv(d, d.cs_ai[c]) && !w6_inv(d, d.cs_bi[c])) return false;
         return w6_free_ok(d, c);
     });

@@ -17,8 +17,14 @@ enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u, FL_JOINTED = 8u, FL_
 // FL_JOINTED: in a joint (step.rs:164-173); FL_JW: written by a joint row this tick (a dynamic
 // body of an active joint, asleep or not)
 
-// Island-list entries are contact indices, or RP_JBIT | joint index for a ball joint's rows.
+// Island-list entries are contact indices, or RP_JBIT | joint index for a ball joint's rows, or
+// RP_PBIT | patch index for a friction patch (scenes with a contact-model pair: contact indices
+// stay below RP_PBIT there).
 #define RP_JBIT 0x80000000u
+#define RP_PBIT 0x40000000u
+// Contact-model flags of a contact (cs_soft[c].w bits): a compliant pair's contact, with an
+// implicit soft row, a point of a friction patch
+enum { CM_SOFT = 1u, CM_IMPLICIT = 2u, CM_PATCH_PT = 4u };
 enum { CLS_SAP = 0u, CLS_OVER = 1u, CLS_INVALID = 2u };
 
 struct HfDesc {
@@ -39,6 +45,7 @@ enum {
     CT_BIG_N,                    // islands of the 1024-thread tiers
     CT_WSH_MAX,                  // most leading wide levels of one island's shock order
     CT_TIER_N, CT_TIER_N1, CT_TIER_N2, CT_TIER_N3,  // islands per sweep tier
+    CT_TIER_N4,                  // islands of the contact-model tier (compliant rows or patches)
     CT_ROWS,                     // island-list entries before the sort: joints + contacts
     CT_LOG_N,                    // contacts appended to the contact log this tick
     // sums over the completed ticks since the counters were last cleared
@@ -55,6 +62,8 @@ enum {
     CT_W6_TOI, CT_W6_SUPPORT, CT_W6_PSEUDO, CT_W6_FALLBACK,  // W6 this tick: bounce rows, support rows,
                                                              // pseudo rows, components swept
     CT_W6_RELAX,                                             // contacts bounced by the relax pass
+    CT_NPATCH,            // friction patches this tick
+    CT_WP_NEW, CT_WP_KEEP,  // carried patch entries stored: new ones, kept ones
     CT_COUNT
 };
 
@@ -135,7 +144,7 @@ struct Dev {
     uint32_t *wide_list;  // islands with wide levels this tick (CT_WIDE_N of them)
     uint32_t *big_list;   // islands of the 1024-thread tiers this tick (CT_BIG_N of them)
     uint32_t *bp_fill;    // per collider: pairs appended to its list (broadphase write pass)
-    uint32_t *tier_list[4];  // island indices of each sweep tier this tick (CT_TIER_N + tier of them)
+    uint32_t *tier_list[5];  // island indices of each sweep tier this tick (CT_TIER_N + tier of them)
     uint32_t *lvl_off_sh, *isl_wsh;  // as lvl_off / isl_w, for the shock order of the current shock pass
     unsigned long long *ph;  // RP_PH_COUNT sweep-phase clocks of the large-island tiers
     float4 *srow;            // RP_SROW * cap_contacts: constant rows of the current sweep order, per slot
@@ -206,6 +215,25 @@ struct Dev {
     uint32_t np;
     uint32_t *pl_body, *pl_grounded, *bk_pl_grounded, *w_player_off, *pl_of_body;
     float4 *pl_state, *bk_pl_state, *pl_input, *pl_bmin, *pl_bmax;
+    // contact models (materials.rs:75-90), allocated when a pair of the scene has a compliant or a
+    // patch-friction model (cm_on): per material pair (stiffness, dissipation, patch 0/1, 0); per
+    // contact the implicit row's (gamma, bias, mass, CM_* bits) and its patch (RP_NONE: none); per
+    // patch (at most one per two contacts) its first point and point count, the centroid
+    // constraint (bodies, inverse masses, lever arms, normal, tangents, (m_n, m_t1, m_t2, m_tw),
+    // (mu_s, mu_k, Stribeck speed, radius)), its accumulators (t1, t2, torsion) and its first
+    // point's carried-cache key; per island the contact-model tier flag; the carried patch cache
+    // (double buffer as the warm cache: keys, (jt, tw), per-world offsets) and its store's flags.
+    uint32_t cm_on, cap_patch;
+    float4 *pair_cm, *cs_soft;
+    uint32_t *ct_patch, *pt_head, *pt_first, *pt_n, *pt_ai, *pt_bi, *isl_cm;
+    float2 *pt_inv;
+    float4 *pt_ra, *pt_rb, *pt_nn, *pt_t1, *pt_t2, *pt_m, *pt_mu, *pt_acc;
+    uint64_t *pt_key;
+    uint32_t *pt_soft;
+    uint64_t *wp_key[2];
+    float4 *wp_jt[2];
+    uint32_t *wpc_off[2];
+    uint32_t *wp_new, *wp_keep;
 };
 
 // A spatial query (k_query) and its answer.
@@ -298,4 +326,5 @@ struct TickParams {
     uint32_t wave_min;     // at least this many of tier 0's largest islands solved a wave each
     uint32_t lane_pos;     // the lane tier's islands also run the ballistic and projection passes one per thread
     uint32_t wide_min;     // ranks of a leading level that make it a GPU-wide level launch
+    uint32_t damp_start;   // compliant rows: damping linearised at the step-start penetration (world.rs:992)
 };

@@ -74,13 +74,18 @@ __global__ void k_hash_keyflags(Dev d, HashDev h, uint32_t cur) {
     }
 }
 
-// stream layout per world: header 48, bodies 56 n, warm 8 + 20 K + 28 E, warm patches 8 (none),
+// stream layout per world: header 48, bodies 56 n, warm 8 + 20 K + 28 E, warm patches 8 + 32 P,
 // warm joints 8 + JB, rotors 8 (none), last_lin 8 + 12 n, awake 8 + cap, sleep ticks 8 + 2 cap
+// carried patch entries of world w (world.rs:896-897: key, jt, tw)
+static __device__ __forceinline__ uint32_t world_patches_n(const Dev &d, uint32_t w, uint32_t cur) {
+    return d.cm_on ? d.wpc_off[cur][w + 1] - d.wpc_off[cur][w] : 0u;
+}
 static __device__ __forceinline__ uint32_t world_len(const Dev &d, const HashDev &h, uint32_t w, uint32_t cur) {
     uint32_t n = d.w_body_off[w + 1] - d.w_body_off[w], cap = d.w_cap[w];
     uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
     uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
-    return 48u + 56u * n + 8u + 20u * K + 28u * E + 24u + h.w_jbytes[w] + 8u + 12u * n + 8u + cap + 8u + 2u * cap;
+    return 48u + 56u * n + 8u + 20u * K + 28u * E + 24u + 32u * world_patches_n(d, w, cur) + h.w_jbytes[w] + 8u + 12u * n + 8u + cap + 8u +
+           2u * cap;
 }
 // A stored joint's entry (world.rs:896-900): index u32, row count u32, then (tag u8, impulse f32)
 // per row, rows in axis order with tags axis << 3.
@@ -116,14 +121,14 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
             uint32_t o = 48u + 56u * n;
             put_u64(S + o, K);
             uint32_t z = o + 8u + 20u * K + 28u * E;
-            const uint32_t JB = h.w_jbytes[w];
-            put_u64(S + z, 0);
+            const uint32_t JB = h.w_jbytes[w], P = world_patches_n(d, w, cur), PB = 32u * P;
+            put_u64(S + z, P);  // the patch entries themselves are written per entry below
             uint32_t nstored = 0;  // the entries themselves are written per joint below
             for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) nstored += d.wj_mask[q] >> 31;
-            put_u64(S + z + 8, nstored);
-            put_u64(S + z + 16 + JB, 0);
-            put_u64(S + z + 24 + JB, n);
-            uint32_t ao = z + 32u + JB + 12u * n;
+            put_u64(S + z + 8 + PB, nstored);
+            put_u64(S + z + 16 + PB + JB, 0);
+            put_u64(S + z + 24 + PB + JB, n);
+            uint32_t ao = z + 32u + PB + JB + 12u * n;
             put_u64(S + ao, cap);
             put_u64(S + ao + 8u + cap, cap);
         }
@@ -134,7 +139,7 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
         put_f(p + 16, q.x); put_f(p + 20, q.y); put_f(p + 24, q.z); put_f(p + 28, q.w);
         put_f(p + 32, l.x); put_f(p + 36, l.y); put_f(p + 40, l.z);
         put_f(p + 44, a.x); put_f(p + 48, a.y); put_f(p + 52, a.z);
-        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E + h.w_jbytes[w];
+        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E + 32u * world_patches_n(d, w, cur) + h.w_jbytes[w];
         uint8_t *ll = S + z + 32u + 12u * i;
         float4 lv = d.last_lin[b];
         put_f(ll, lv.x); put_f(ll + 4, lv.y); put_f(ll + 8, lv.z);
@@ -152,7 +157,8 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
         const uint32_t n = d.w_body_off[w + 1] - d.w_body_off[w];
         const uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
         const uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
-        uint8_t *jp = h.stream + (size_t)h.w_chunk[w] * 1024u + 48u + 56u * n + 8u + 20u * K + 28u * E + 16u + (h.j_off[q] - h.j_off[q0]);
+        uint8_t *jp = h.stream + (size_t)h.w_chunk[w] * 1024u + 48u + 56u * n + 8u + 20u * K + 28u * E + 16u + 32u * world_patches_n(d, w, cur) +
+                      (h.j_off[q] - h.j_off[q0]);
         put_u32(jp, q - q0); put_u32(jp + 4, (uint32_t)__popc(m & 7u));
         jp += 8;
         for (uint32_t i = 0; i < 3; i++) {
@@ -160,6 +166,22 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
             jp[0] = (uint8_t)(i << 3);
             put_u32(jp + 1, canon(d.wj_lam[3 * q + i]));
             jp += 5;
+        }
+    }
+    // carried patch entries, key order: key (4 u32), jt (3 f32), tw
+    if (d.cm_on) {
+        GS(e, d.wpc_off[cur][d.nw]) {
+            uint32_t lo = 0, hi = d.nw;
+            while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (d.wpc_off[cur][mid] <= e) lo = mid + 1; else hi = mid; }
+            const uint32_t w = lo - 1;
+            const uint32_t n = d.w_body_off[w + 1] - d.w_body_off[w];
+            const uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
+            const uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
+            uint8_t *p = h.stream + (size_t)h.w_chunk[w] * 1024u + 48u + 56u * n + 8u + 20u * K + 28u * E + 8u + 32u * (e - d.wpc_off[cur][w]);
+            const uint64_t k = d.wp_key[cur][e];
+            const float4 jt = d.wp_jt[cur][e];
+            put_u32(p, (uint32_t)(k >> 32)); put_u32(p + 4, (uint32_t)k); put_u32(p + 8, 0); put_u32(p + 12, 0);
+            put_f(p + 16, jt.x); put_f(p + 20, jt.y); put_f(p + 24, jt.z); put_f(p + 28, jt.w);
         }
     }
     // warm entries (key headers written by the group's first entry)

@@ -261,3 +261,26 @@ RP_HD Q4 fin_q(Q4 q) {
     if (!finq(n)) return qid();
     return n.w < 0.0f ? qneg(n) : n;
 }
+// y.atan2(x) for ordering a friction patch's points by angle (solver.rs:233-236): a once-rounded
+// double evaluation (atan on [0, 1] by the tan(pi/8) reduction and the odd series to t^49 in
+// Horner form), the oracle's operations; finite inputs.
+RP_HD double rp_atan_unit(double t) {
+    double base = 0.0;
+    if (t > 0.41421356237309503) { t = (t - 1.0) / (t + 1.0); base = 0.78539816339744828; }
+    const double t2 = t * t;
+    double p = 1.0 / 49.0;
+    for (int k = 23; k >= 0; k--) p = 1.0 / (double)(2 * k + 1) - t2 * p;
+    return base + t * p;
+}
+RP_HD float rp_atan2f(float y, float x) {
+    if (x != x || y != y) return x + y;
+    const double ay = fabs((double)y), ax = fabs((double)x);
+    double a;
+    if (ay == 0.0 && ax == 0.0) a = signbit(x) ? 3.141592653589793 : 0.0;
+    else {
+        a = ay <= ax ? rp_atan_unit(ay / ax) : 1.5707963267948966 - rp_atan_unit(ax / ay);
+        if (signbit(x)) a = 3.141592653589793 - a;
+    }
+    const float r = (float)a;
+    return signbit(y) ? -r : r;
+}

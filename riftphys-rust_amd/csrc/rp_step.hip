@@ -57,6 +57,8 @@ static __device__ __forceinline__ void set_vel(const Dev &d, uint32_t b, V3 l, V
 // With mu_s == mu_k the Stribeck term is (+0) * exp(..) = +0 for every slip that is not NaN (exp
 // of a finite or infinite negative argument is finite), so mu = mu_k + 0 exactly: the exp is
 // skipped, bit-identically (mu_k + 0.0f keeps the sum's sign of zero).
+// A contact's contact-model bits (CM_*): 0 in scenes without a contact-model pair.
+static __device__ __forceinline__ uint32_t cm_flags(const Dev &d, uint32_t c) { return d.cm_on ? __float_as_uint(d.cs_soft[c].w) : 0u; }
 static __device__ __forceinline__ float mu_dyn(float4 mu4, float slip) {
     if (mu4.x == mu4.y && slip == slip) return mu4.y + 0.0f;
     float x = slip / mu4.z;
@@ -594,11 +596,35 @@ __global__ void k_prep(Dev d, TickParams tp) {
         V3 cva = L3(d.lin[ai]) + cross3(L3(d.ang[ai]), ra);
         V3 cvb = terr ? splat3(0.0f) : L3(d.lin[bi]) + cross3(L3(d.ang[bi]), rb);
         float approach = 0.5f * (v_pre + dot3(cvb - cva, n));
+        // compliant pair (solver.rs:545-567): no Baumgarte bias or restitution; the implicit
+        // Hunt-Crossley row's constants when its effective mass and spring are positive
+        uint32_t cmf = 0;
+        if (d.cm_on) {
+            const float4 cm = d.pair_cm[(ma & 15u) * 16u + (mb & 15u)];
+            float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (cm.x > 0.0f) {
+                cmf = CM_SOFT;
+                bias = 0.0f;
+                const float vn0 = dot3(cvb - cva, n);
+                const float pen_d = tp.damp_start ? pen : pen - (0.5f * tp.dt) * vn0;
+                const float cdamp = (cm.x * fmaxf(cm.y, 0.0f)) * fmaxf(pen_d, 0.0f);
+                const float denom = cdamp + tp.dt * cm.x;
+                if (mn > 0.0f && denom > 0.0f) {
+                    cmf |= CM_IMPLICIT;
+                    sp.x = 1.0f / (tp.dt * denom);
+                    sp.y = (((tp.dt * cm.x) / denom) * pen) / tp.dt;
+                    sp.z = 1.0f / (1.0f / mn + sp.x);
+                }
+            }
+            if (d.ct_patch[c] != RP_NONE) cmf |= CM_PATCH_PT;
+            sp.w = __uint_as_float(cmf);
+            d.cs_soft[c] = sp;
+        }
         // restitution bias (solver.rs:569-588): a contact approaching faster than 1 m/s bounces
         // after the solve; for free bodies at its time of impact (W6), from the sides' normal
         // velocities at the step's start (last step's end linear velocity, the pre-step angular
         // one) and after this step's forces
-        if (d.w6_on && approach < -1.0f) {
+        if (d.w6_on && approach < -1.0f && !(cmf & CM_SOFT)) {
             const float rbias = -pp.z * approach;
             if (!(rbias <= 0.0f)) {
                 const bool toi = !(d.flags[ai] & FL_JOINTED) && (terr || !(d.flags[bi] & FL_JOINTED));
@@ -638,7 +664,8 @@ __global__ void k_prep(Dev d, TickParams tp) {
         // idempotent ORs: skipped once the bit is seen set (a world's contacts all mark one word)
         if (!(d.flags[ai] & FL_CONSTRAINED)) atomicOr(&d.flags[ai], FL_CONSTRAINED);
         if (!terr && !(d.flags[bi] & FL_CONSTRAINED)) atomicOr(&d.flags[bi], FL_CONSTRAINED);
-        if (terr && depth - 0.05f > -0.005f && !(d.flags[ai] & FL_GROUND)) atomicOr(&d.flags[ai], FL_GROUND);
+        // grounding (step.rs:337-342): touching terrain contacts of rigid pairs
+        if (terr && depth - 0.05f > -0.005f && !(cmf & CM_SOFT) && !(d.flags[ai] & FL_GROUND)) atomicOr(&d.flags[ai], FL_GROUND);
         if ((d.inv_mass[ai] > 0.0f || (!terr && d.inv_mass[bi] > 0.0f)) && !d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
         if (wa != RP_NONE && wb != RP_NONE) uf_union(d.root, wa, wb);
     }
@@ -686,9 +713,10 @@ __global__ void k_joint_prep(Dev d) {
     }
 }
 
-// Island key of each list entry (joints first, then contacts, so a stable sort keeps every joint
-// ahead of the contacts of its island, as the reference's joint batches run before the contact
-// batches): the component root (lowest body index) of its written bodies.
+// Island key of each list entry (joints first, then contacts, then friction patches, so a stable
+// sort keeps every joint ahead of the contacts of its island and every patch after them, as the
+// reference runs joint batches, contact batches, then the patches in each iteration): the
+// component root (lowest body index) of its written bodies.
 __global__ void k_island_keys(Dev d) {
     if (aborted(d)) return;
     const uint32_t nrows = min(d.cnt[CT_ROWS], d.cap_contacts), nj = d.nj;
@@ -698,6 +726,14 @@ __global__ void k_island_keys(Dev d) {
             const uint32_t x = d.inv_mass[a] > 0.0f ? a : (d.inv_mass[b] > 0.0f ? b : RP_NONE);
             d.isl_k[r] = (d.jr_mask[r] && x != RP_NONE) ? uf_find(d.root, x) : d.nb;
             d.isl_v[r] = RP_JBIT | r;
+            continue;
+        }
+        const uint32_t nct = min(d.cnt[CT_CONTACTS], d.cap_contacts);
+        if (r - nj >= nct) {  // a friction patch: its first point's island, after its contacts
+            const uint32_t p = r - nj - nct, c = d.pt_first[p];
+            const uint32_t x = d.cs_wa[c] != RP_NONE ? d.cs_wa[c] : d.cs_wb[c];
+            d.isl_k[r] = x == RP_NONE ? d.nb : uf_find(d.root, x);
+            d.isl_v[r] = RP_PBIT | p;
             continue;
         }
         const uint32_t c = r - nj;
@@ -748,6 +784,8 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
         if (es == ee) continue;
         uint32_t taken = 0;  // entries per key stay well under 32
         for (uint32_t g = c; g < nct && d.ct_key[g] == key && d.col_world[d.ct_a[g]] == w; g++) {
+            const uint32_t cmf = cm_flags(d, g);
+            if (cmf & CM_SOFT) continue;  // compliant rows start cold (solver.rs:736)
             V3 an = L3(d.ct_anchor[g]);
             int best = -1; float bd = 0.0f;
             for (uint32_t e = es; e < ee && e - es < 32; e++) {
@@ -763,7 +801,7 @@ __global__ void k_warm_match(Dev d, TickParams tp) {
             float ejn = d.wk_jn[cur][best];
             d.jn[g] = ejn;
             d.jn_pre[g] = ejn;
-            if (d.cs_wa[g] == RP_NONE && d.cs_wb[g] == RP_NONE) {
+            if (d.cs_wa[g] == RP_NONE && d.cs_wb[g] == RP_NONE && !(cmf & CM_PATCH_PT)) {
                 // slip speed from constant velocities; clamp as the iterations do
                 uint32_t ai = d.cs_ai[g], bi = d.cs_bi[g];
                 V3 ra = L3(d.cs_ra[g]), rb = L3(d.cs_rb[g]);
@@ -826,7 +864,7 @@ __global__ void k_log_flags(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     GRID_STRIDE(r, d.cap_contacts)
-        d.newflag[r] = (r < nislc && !(d.isl_v2[r] & RP_JBIT) && (!tp.resid_on || d.rank_conv[r])) ? 1u : 0u;
+        d.newflag[r] = (r < nislc && !(d.isl_v2[r] & (RP_JBIT | RP_PBIT)) && (!tp.resid_on || d.rank_conv[r])) ? 1u : 0u;
 }
 // Appends the solved contacts of this tick (island-list order) with their converged impulses,
 // the normal impulse before the shock pass and the friction accumulators (solver.rs:917-926),
@@ -839,7 +877,7 @@ __global__ void k_log_contacts(Dev d, TickParams tp) {
     GRID_STRIDE(r, nislc) {
         if (!room) break;
         const uint32_t c = d.isl_v2[r];
-        if ((c & RP_JBIT) || (tp.resid_on && !d.rank_conv[r])) continue;
+        if ((c & (RP_JBIT | RP_PBIT)) || (tp.resid_on && !d.rank_conv[r])) continue;
         const size_t o = (size_t)base + d.newflag[r];
         for (int k = 0; k < 12; k++) d.log_feat[o * 12 + k] = d.feat[(size_t)c * 12 + k];
         const float2 t = d.jt[c];
@@ -876,6 +914,9 @@ __global__ void k_chain_entries(Dev d, const uint32_t *ord, uint32_t *keys, uint
         if (c & RP_JBIT) {  // a joint writes each of its dynamic bodies
             const uint32_t x = (e & 1u) ? d.jt_b[c & ~RP_JBIT] : d.jt_a[c & ~RP_JBIT];
             b = d.inv_mass[x] > 0.0f ? x : RP_NONE;
+        } else if (c & RP_PBIT) {  // a patch writes its points' bodies
+            const uint32_t f = d.pt_first[c & ~RP_PBIT];
+            b = (e & 1u) ? d.cs_wb[f] : d.cs_wa[f];
         } else {
             b = (e & 1u) ? d.cs_wb[c] : d.cs_wa[c];
         }
@@ -923,8 +964,8 @@ __global__ void k_shock_keys(Dev d, TickParams tp, uint32_t *klo, uint32_t *khi,
     GRID_STRIDE(r, nislc) {
         uint32_t c = d.isl_v2[r];
         uint64_t k;
-        if (c & RP_JBIT) {
-            k = ((uint64_t)1 << 44) - 1;  // joints take no part in the shock passes: last, no-ops
+        if (c & (RP_JBIT | RP_PBIT)) {
+            k = ((uint64_t)1 << 44) - 1;  // joints and patches take no part in the shock passes: last, no-ops
         } else {
             uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
             float h = b == RP_NONE ? -INFINITY : fminf(dot3(L3(d.pos[a]), up), dot3(L3(d.pos[b]), up));
@@ -988,6 +1029,9 @@ template <> struct Tier<0> { static constexpr uint32_t threads = 64, flag_bytes 
 template <> struct Tier<3> { static constexpr uint32_t threads = 64, flag_bytes = 4096; };
 template <> struct Tier<1> { static constexpr uint32_t threads = 1024, flag_bytes = 131072; };
 template <> struct Tier<2> { static constexpr uint32_t threads = 1024, flag_bytes = 0; };
+// Tier 4: islands with a compliant row or a friction patch (any size), one workgroup each, flags
+// in HBM, velocities in HBM
+template <> struct Tier<4> { static constexpr uint32_t threads = 256, flag_bytes = 0; };
 template <int TIER> constexpr bool BIG = TIER == 1 || TIER == 2;
 // Island sizes up to tp.tier_max[0] go to tier 0, then up to tier_max[1] to tier 3, up to
 // tier_max[2] to tier 1, larger ones to tier 2.  The host keeps each bound within its tier's flag
@@ -995,13 +1039,14 @@ template <int TIER> constexpr bool BIG = TIER == 1 || TIER == 2;
 static __device__ __forceinline__ int tier_of(uint32_t n, const TickParams &tp) {
     return n <= tp.tier_max[0] ? 0 : (n <= tp.tier_max[1] ? 3 : (n <= tp.tier_max[2] ? 1 : 2));
 }
-// The islands of each sweep tier (by contact count), so each tier's launch walks only its own.
+// The islands of each sweep tier (by contact count; islands with a compliant row or a patch to
+// tier 4), so each tier's launch walks only its own.
 __global__ void k_tier_lists(Dev d, TickParams tp) {
     if (aborted(d)) return;
     const uint32_t nisl = d.cnt[CT_ISLANDS], nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     GRID_STRIDE(isl, nisl) {
         const uint32_t n = ((isl + 1 < nisl) ? d.isl_start[isl + 1] : nislc) - d.isl_start[isl];
-        const int t = tier_of(n, tp);
+        const int t = (d.cm_on && d.isl_cm[isl]) ? 4 : tier_of(n, tp);
         d.tier_list[t][atomicAdd(&d.cnt[CT_TIER_N + t], 1u)] = isl;
     }
 }
@@ -1283,7 +1328,7 @@ static __device__ __forceinline__ void apply_row(const Dev &d, const Row &x, V3 
 
 template <int TIER>
 static __device__ __forceinline__ auto make_island(const Dev &d, uint8_t *flag_lds, uint32_t s, uint32_t n) {
-    if constexpr (TIER != 2) return Island<LdsFlags>{LdsFlags{(RP_LDS uint8_t *)flag_lds}, s, n, 0};
+    if constexpr (TIER != 2 && TIER != 4) return Island<LdsFlags>{LdsFlags{(RP_LDS uint8_t *)flag_lds}, s, n, 0};
     else return Island<GlbFlags>{GlbFlags{d.g_done + s}, s, n, 0};
 }
 
@@ -1435,7 +1480,11 @@ static __device__ void joint_pos(const Dev &d, uint32_t q) {
 static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &tp, uint32_t c) {
     if (c & RP_JBIT) { joint_warm(d, c & ~RP_JBIT); return; }
     const uint32_t cur = tp.cur;
-    if (tp.ws_mode == 2) {
+    // a patch's point takes its carried normal impulse only (its friction is the patch's,
+    // solver.rs:752); compliant rows were matched to nothing.  Both start as the reference's in
+    // the learned mode too.
+    const uint32_t cmf = cm_flags(d, c);
+    if (tp.ws_mode == 2 && !cmf) {
         const int wi = d.warm_idx[c];
         const float4 pr = d.pred[c];
         const bool use = pr.w != 0.0f;
@@ -1464,7 +1513,7 @@ static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &
         V3 imp = n3 * jn;
         imp = imp + (t1 * t.x + t2 * t.y);
         apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
-    } else if (tp.ws_mode == 0) {
+    } else if (tp.ws_mode != 1) {
         int wi = d.warm_idx[c];
         if (wi < 0) return;
         uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
@@ -1472,6 +1521,11 @@ static __device__ __forceinline__ void warm_row(const Dev &d, const TickParams &
         float2 inv = d.cs_inv[c];
         float ejn = d.wk_jn[cur][wi];
         V3 imp = n3 * ejn;
+        if (cmf & CM_PATCH_PT) {
+            d.jn[c] = ejn;
+            apply_cimp(d, ai, bi, inv.x, inv.y, ra, rb, imp);
+            return;
+        }
         V3 v = rel_vel(d, ai, bi, ra, rb);
         float vt1 = dot3(v, t1), vt2 = dot3(v, t2);
         float mu = mu_dyn(d.cs_mu[c], sqrtf(vt1 * vt1 + vt2 * vt2));
@@ -1730,7 +1784,8 @@ static __device__ __forceinline__ void pgs_row_l(const Dev &d, uint32_t c, const
 
 // Velocity shock row of contact c (solver.rs:976-993): the lower body along up is held fixed.
 static __device__ __forceinline__ void vshock_row(const Dev &d, const V3 up, uint32_t c) {
-    if (c & RP_JBIT) return;
+    if (c & (RP_JBIT | RP_PBIT)) return;
+    if (cm_flags(d, c) & CM_SOFT) return;  // compliant rows have no shock pass (solver.rs:973)
     uint32_t ai = d.cs_ai[c], bi = d.cs_bi[c];
     V3 ra = L3(d.cs_ra[c]), rb = L3(d.cs_rb[c]), n3 = L3(d.cs_n[c]);
     float2 inv = d.cs_inv[c];
@@ -1773,7 +1828,7 @@ static __device__ void vel_shock(const Dev &d, const TickParams &tp, Island<FL> 
     for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
         const uint32_t c = lst[r];
         if (c & RP_JBIT) joint_store(d, c & ~RP_JBIT);
-        else d.jn_pre[c] = d.jn[c];
+        else if (!(c & RP_PBIT)) d.jn_pre[c] = d.jn[c];
     }
     __syncthreads();
     shock_slots<TIER>(d, tp, I, isl);
@@ -2389,6 +2444,7 @@ static __device__ __forceinline__ bool idle_body(const Dev &d, uint32_t w, uint3
     return !(d.bflags[b] & BF_DYN) || !d.awake[b];
 }
 static __device__ __forceinline__ bool new_entry(const Dev &d, uint32_t c) {
+    if (cm_flags(d, c) & CM_SOFT) return false;  // compliant rows are not carried (solver.rs:922)
     float2 t = d.jt[c];
     V3 jt = L3(d.cs_t1[c]) * t.x + L3(d.cs_t2[c]) * t.y;
     return !(d.jn_pre[c] == 0.0f && zero3(jt));
@@ -2468,6 +2524,7 @@ __global__ void k_warm_merge(Dev d, TickParams tp) {
     }
 }
 
+#include "rp_cm.h"
 #include "rp_bounce.h"
 #include "rp_query.h"
 #include "rp_player.h"
@@ -2486,7 +2543,7 @@ __global__ void k_ball_init(Dev d) {
 // an outcome, and those are chained.
 // Ballistic classification of contact c (step.rs:154-203): bodies still approaching lose the flag.
 static __device__ __forceinline__ void ball_row(const Dev &d, const TickParams &tp, uint32_t c) {
-    if (c & RP_JBIT) return;
+    if (c & (RP_JBIT | RP_PBIT)) return;
     uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
     bool hb = b != RP_NONE;
     bool ba = d.ballf[a] != 0, bb = hb && d.ballf[b] != 0;
@@ -2593,12 +2650,15 @@ __global__ void k_posc_prep(Dev d) {
     const uint32_t nislc = d.cnt[CT_ISLC];
     GRID_STRIDE(r, nislc) {
         uint32_t c = d.isl_v2[r];
-        if (c & RP_JBIT) continue;
+        if (c & (RP_JBIT | RP_PBIT)) continue;
         uint32_t a = d.cs_ai[c], b = d.cs_bi[c];
         V3 point = L3(d.ct_p[c]);
         V3 la = qrot(qconj(LQ(d.orot[a])), point - L3(d.opos[a]));
         V3 lb = b == RP_NONE ? splat3(0.0f) : qrot(qconj(LQ(d.orot[b])), point - L3(d.opos[b]));
-        d.pc_la[c] = S3(la, d.ct_p[c].w - 0.05f);
+        // a compliant pair is left out of the projection (step.rs:433): its depth never exceeds
+        // the slop, so its item does nothing
+        const float pen0 = (cm_flags(d, c) & CM_SOFT) ? -INFINITY : d.ct_p[c].w - 0.05f;
+        d.pc_la[c] = S3(la, pen0);
         d.pc_lb[c] = S3(lb);
     }
 }
@@ -2615,7 +2675,7 @@ struct PRow {
 };
 static __device__ __forceinline__ PRow load_prow(const Dev &d, uint32_t c) {
     PRow x;
-    if (c & RP_JBIT) {  // a joint's position pass: only the entry travels
+    if (c & (RP_JBIT | RP_PBIT)) {  // a joint's position pass (a patch has none): only the entry travels
         x.a = c; x.b = RP_NONE; x.am = x.bm = false; x.la4 = make_float4(0, 0, 0, 0); x.lb = x.n = x.pt = splat3(0.0f);
         x.ima = x.imb = 0.0f; x.ila = x.ilb = mzero();
         return x;
@@ -2658,6 +2718,7 @@ static __device__ __forceinline__ PRow fetch_prow(const Dev &d, uint32_t slot) {
 }
 static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 up) {
     if (x.a & RP_JBIT) { if (!shock) joint_pos(d, x.a & ~RP_JBIT); return; }
+    if (x.a & RP_PBIT) return;
     const uint32_t a = x.a, b = x.b;
     const bool hb = b != RP_NONE;
     bool am = x.am, bm = x.bm;
@@ -2862,11 +2923,14 @@ __global__ void k_check(Dev d, uint32_t which) {
         if (d.cnt[CT_PAIRS] > d.cap_pairs) { d.cnt[CT_NEED_PAIRS] = d.cnt[CT_PAIRS]; d.cnt[CT_ABORT] = 1; }
         if (d.cnt[CT_CANDS] > d.cap_cands) { d.cnt[CT_NEED_CANDS] = d.cnt[CT_CANDS]; d.cnt[CT_ABORT] = 1; }
     } else if (which == 1) {
-        // the island lists hold the joints too
+        // the island lists hold the joints too, and the friction patches (at most one per two
+        // contacts; their count is added once they are found)
         d.cnt[CT_ROWS] = d.cnt[CT_CONTACTS] + d.nj;
-        if (d.cnt[CT_ROWS] > d.cap_contacts) { d.cnt[CT_NEED_CONTACTS] = d.cnt[CT_ROWS]; d.cnt[CT_ABORT] = 1; }
+        const uint32_t need = d.cnt[CT_ROWS] + (d.cm_on ? d.cnt[CT_CONTACTS] / 2u : 0u);
+        if (need > d.cap_contacts) { d.cnt[CT_NEED_CONTACTS] = need; d.cnt[CT_ABORT] = 1; }
     } else if (which == 2) {
         uint32_t wt = d.cnt[CT_NEW_TOTAL] + d.cnt[CT_KEEP_TOTAL];
+        if (d.cm_on) wt = max(wt, d.cnt[CT_WP_NEW] + d.cnt[CT_WP_KEEP]);  // the patch cache shares the capacity
         if (wt > d.cap_warm) { d.cnt[CT_NEED_WARM] = wt; d.cnt[CT_ABORT] = 1; }
     }
 }
@@ -3034,11 +3098,19 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.c_off, d.nc, d.cnt + CT_CONTACTS);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 1u);
     hipLaunchKernelGGL(k_contact_write, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    if (d.cm_on) {  // friction patches: runs of a patch-friction pair's contacts
+        hipLaunchKernelGGL(k_patch_heads, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+        scan_u32(d.pt_head, d.cap_contacts, d.scan_tmp, d.cnt + CT_NPATCH, st);
+        hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.pt_head, d.cap_contacts, d.cnt + CT_NPATCH);
+        hipLaunchKernelGGL(k_patch_fill, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+        hipLaunchKernelGGL(k_rows_patches, dim3(1), dim3(1), 0, st, d);
+    }
     mark(RP_ST_PREP);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_solve, d.nw);
     if (d.w6_on) hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(2 * d.nw, B)), dim3(B), 0, st, d.w_w6, 2 * d.nw);
     if (d.nj) hipLaunchKernelGGL(k_joint_prep, dim3(grid_for(d.nj, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
+    if (d.cm_on) hipLaunchKernelGGL(k_patch_prep, dim3(grid_for(d.cap_patch, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_island_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
     const uint32_t body_bits = key_bits(d.nb);  // body ids and the nb sentinel
     uint32_t *ks, *vs;
@@ -3051,7 +3123,11 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_island_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
     scan_u32(d.isl_flag, d.cap_contacts, d.scan_tmp, d.cnt + CT_ISLANDS, st);
     hipLaunchKernelGGL(k_island_compact, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
-    for (uint32_t t = 0; t < 4; t++) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_TIER_N + t, 0u);
+    for (uint32_t t = 0; t < 5; t++) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_TIER_N + t, 0u);
+    if (d.cm_on) {  // islands with a compliant row or a patch go to tier 4
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.cap_contacts + 1, B)), dim3(B), 0, st, d.isl_cm, d.cap_contacts + 1);
+        hipLaunchKernelGGL(k_isl_cm, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, d.isl_k2);
+    }
     hipLaunchKernelGGL(k_tier_lists, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_warm_match, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, d.cnt, (uint32_t)CT_ISLC_GLB, 0u);
@@ -3187,6 +3263,11 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
               },
               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_vel<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u, (const uint32_t *)nullptr); });
     }
+    const uint32_t g4 = 512;
+    if (d.cm_on) {  // islands with compliant rows or patches, then the patches in no island
+        hipLaunchKernelGGL(k_sweep_cm, dim3(g4), dim3(Tier<4>::threads), 0, st, d, tp);
+        hipLaunchKernelGGL(k_free_patches, dim3(grid_for(d.cap_patch, B)), dim3(B), 0, st, d, tp);
+    }
     if (tp.log_on) {
         hipLaunchKernelGGL(k_log_flags, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
         scan_u32(d.newflag, d.cap_contacts, d.scan_tmp, d.cnt + CT_LOG_N, st);
@@ -3201,10 +3282,20 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_warm_keepflags, dim3(grid_for(d.cap_warm, B)), dim3(B), 0, st, d, tp);
     scan_u32(d.keepflag, d.cap_warm, d.scan_tmp, d.cnt + CT_KEEP_TOTAL, st);
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.keepflag, d.cap_warm, d.cnt + CT_KEEP_TOTAL);
+    if (d.cm_on) {  // the carried patch impulses (into buffer cur ^ 1)
+        hipLaunchKernelGGL(k_wp_newflags, dim3(grid_for(d.cap_patch, B)), dim3(B), 0, st, d);
+        scan_u32(d.wp_new, d.cap_patch, d.scan_tmp, d.cnt + CT_WP_NEW, st);
+        hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.wp_new, d.cap_patch, d.cnt + CT_WP_NEW);
+        hipLaunchKernelGGL(k_wp_keepflags, dim3(grid_for(d.cap_warm, B)), dim3(B), 0, st, d, tp);
+        scan_u32(d.wp_keep, d.cap_warm, d.scan_tmp, d.cnt + CT_WP_KEEP, st);
+        hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.wp_keep, d.cap_warm, d.cnt + CT_WP_KEEP);
+    }
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 2u);
     hipLaunchKernelGGL(k_warm_merge, dim3(grid_for(d.cap_contacts > d.cap_warm ? d.cap_contacts : d.cap_warm, B)), dim3(B), 0, st, d, tp);
+    if (d.cm_on) hipLaunchKernelGGL(k_wp_merge, dim3(grid_for(d.cap_warm > d.cap_patch ? d.cap_warm : d.cap_patch, B)), dim3(B), 0, st, d, tp);
     if (d.w6_on) hipLaunchKernelGGL(k_w6, dim3(d.w6_slots), dim3(64), 0, st, d, tp);
     hipLaunchKernelGGL(k_loaded, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    if (d.cm_on) hipLaunchKernelGGL(k_patch_loaded, dim3(grid_for(d.cap_patch, B)), dim3(B), 0, st, d);
     mark(RP_ST_BALLISTIC);
     hipLaunchKernelGGL(k_ball_init, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d);
     // tier 0's share of each pass below: with few projection iterations (tp.lane_pos), the lane
@@ -3220,6 +3311,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
               hipLaunchKernelGGL(k_sweep_ball<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, NO);
           },
           [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_ball<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, NO); });
+    if (d.cm_on) hipLaunchKernelGGL(k_sweep_ball<4>, dim3(g4), dim3(Tier<4>::threads), 0, st, d, tp, NO);
     mark(RP_ST_INTEGRATE);
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_POS_PREP);
@@ -3249,6 +3341,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
               },
               [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u, NO); });
     }
+    if (d.cm_on) hipLaunchKernelGGL(k_sweep_proj<4>, dim3(g4), dim3(Tier<4>::threads), 0, st, d, tp, 0u, NO);
     mark(RP_ST_PROJ_ORDER);
     build_shock_order(d, tp, body_bits, st);  // from the projected poses
     mark(RP_ST_PROJ_SHOCK);
@@ -3272,6 +3365,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
                    hipLaunchKernelGGL(k_sweep_proj_shock<1>, dim3(g1), dim3(Tier<1>::threads), 0, s, d, tp, 0u, NO);
                },
                [&](hipStream_t s) { hipLaunchKernelGGL(k_sweep_proj_shock<2>, dim3(g2), dim3(Tier<2>::threads), 0, s, d, tp, 0u, NO); });
+    if (d.cm_on) hipLaunchKernelGGL(k_sweep_proj_shock<4>, dim3(g4), dim3(Tier<4>::threads), 0, st, d, tp, 0u, NO);
     mark(RP_ST_S6);
     hipLaunchKernelGGL(k_s6, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_tick_done, dim3(1), dim3(64), 0, st, d);

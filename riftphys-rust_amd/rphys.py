@@ -174,6 +174,8 @@ EXPORTS = [
     ("rphys_world_snapshot_free", None, [P]),
     ("rphys_add_body_density", U32, [P, U32, PF32, C.POINTER(Isometry), C.POINTER(Velocity), U32, F32, PU32]),
     ("rphys_world_set_pair_props", U32, [P, U32, U32, F32, F32, F32, F32]),
+    ("rphys_world_set_pair_contact_model", U32, [P, U32, U32, F32, F32, U32]),
+    ("rphys_world_set_compliant_damping_at_start", U32, [P, U32]),
     ("rphys_world_add_ball_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), PU32]),
     ("rphys_world_num_joints", U32, [P]),
     ("rphys_world_set_solver_iterations", U32, [P, U32, U32]),
@@ -342,6 +344,10 @@ class World:
         L = lib()
         for a, b, ms, mk, e, vs in s.get("pairs", []):
             check(L.rphys_world_set_pair_props(self.h, a, b, ms, mk, e, vs))
+        for a, b, k, d, patch in s.get("contact_models", []):
+            check(L.rphys_world_set_pair_contact_model(self.h, a, b, k, d, 1 if patch else 0))
+        if s.get("damping_at_start"):
+            check(L.rphys_world_set_compliant_damping_at_start(self.h, 1))
         vi, pi = s.get("iters", (20, 20))
         check(L.rphys_world_set_solver_iterations(self.h, vi, pi))
         if s.get("residual"):  # (tol, max_iters): each island iterates to the residual

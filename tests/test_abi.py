@@ -179,6 +179,15 @@ def test_bad_arguments_are_rejected():
         assert L.rphys_add_box_body(w, C.byref(box_desc(mass=0.0, body_type=rphys.BODY_STATIC)), C.byref(out)) == rphys.OK
         bad = np.zeros(3, np.float32)
         assert L.rphys_world_set_heightfield_raw32_square(w, bad.ctypes.data, 12, 1, 1, 0, 0, 0) == rphys.ERR_BAD_ARG
+        # contact models: material ids within the table, a finite non-negative stiffness, a finite dissipation
+        assert L.rphys_world_set_pair_contact_model(None, 0, 0, 1.0, 0.0, 0) == rphys.ERR_NULL
+        assert L.rphys_world_set_pair_contact_model(w, 16, 0, 1.0, 0.0, 0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_pair_contact_model(w, 0, 0, -1.0, 0.0, 0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_pair_contact_model(w, 0, 0, float("nan"), 0.0, 0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_pair_contact_model(w, 0, 0, 1.0e4, float("inf"), 0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_pair_contact_model(w, 3, 6, 1.0e4, 0.5, 1) == rphys.OK
+        assert L.rphys_world_set_compliant_damping_at_start(None, 1) == rphys.ERR_NULL
+        assert L.rphys_world_set_compliant_damping_at_start(w, 1) == rphys.OK
         # stepping an empty-of-dynamics world is allowed without a device only when it has no bodies
         L2 = L.rphys_world_create(4, 4)
         assert L.rphys_world_step(L2, 0.016) == rphys.OK

@@ -20,6 +20,9 @@
  *   compliant (Hunt-Crossley) rows   solver.rs:545-567, 848-856; left out of the warm start and
  *                                     store, shock, W6 supports, projection and grounding
  *                                     (solver.rs:736, 922, 973, 1020, 1087, 1175; step.rs:340, 433)
+ *   D6 joints (no armature)           solver.rs:262-307, 336-424 (d6_rows), 700-715, 822-832,
+ *                                     945-952; bodies.rs:50-53, 321-390, 616-873 (solve_d6),
+ *                                     895-922, 1010-1030; step.rs:353-365 (velocity motors)
  *   friction patches                  solver.rs:219-260 (patch_radius), 592-619, 763-782, 865,
  *                                     882-915, 927-944, 1257-1264; warm_patches in the hash and
  *                                     snapshots (world.rs:249, 818-821, 896-897, 935, 952)
@@ -54,7 +57,11 @@
  * (test_oracle_contact_models.py): compliant_contact_test.rs's resting sinks m g / (4 k) and F / k,
  * bounces against Hunt-Crossley's ODE, Coulomb sliding and a slope hold on compliant rows;
  * patch_friction_test.rs's spin-down angles with the face's mean radius and the corners' radius,
- * Coulomb sliding, the patch radius against quadrature.  The patch point order's atan2 is a
+ * Coulomb sliding, the patch radius against quadrature; D6 joints (test_oracle_d6.py):
+ * hinge_angle_test.rs's stops at -1 and -2.2 rad, joint_damping_test.rs's discrete damper decay
+ * rate and friction stop angle, joint_energy_test.rs's pendulum period (RK4) and bounded energy, a
+ * velocity motor's impulse cap, a position drive's hold, a sprung hinge's static sag.  The hinge
+ * twist's atan2 is the same double evaluation.  The patch point order's atan2 is a
  * once-rounded double evaluation (or_math.h), within 1 ulp of the platform's atan2f; only the order
  * of the angles' keys reaches the results.  glam's SSE2
  * operation order and the platform libm are third-party (glam 0.27.0, libm 0.2.15); G1-G3 pin them
@@ -151,10 +158,24 @@ typedef struct { uint32_t a, b; PairP p; } PairOverride;
  * or a distance joint between the two centres of mass (bodies.rs:29-50, 273-285: no anchors and
  * no damping, the rows the reference keeps bit for bit), with rest length, compliance (m/N, 0 =
  * rigid) and the step's XPBD multiplier. */
-enum { J_BALL = 0, J_DIST = 1 };
-typedef struct { uint32_t a, b; V3 fap, fbp; Q4 fra, frb; int kind; float rest, compliance, lam; } Joint;
-/* A joint's warm rows (world.rs:250 warm_joints): row tags and impulses of its last solve. */
-typedef struct { uint8_t has, n, tag[3]; float lam[3]; } WarmJ;
+enum { J_BALL = 0, J_DIST = 1, J_D6 = 2 };
+/* One axis of a D6 joint (bodies.rs:52-104 D6Axis): enabled (constrained), compliance (XPBD, 0 =
+ * rigid), an optional drive (target, kp, kd, max impulse, velocity mode) and limit (min, max),
+ * damping and Coulomb friction, and the step's XPBD multiplier.  No armature. */
+typedef struct {
+    uint8_t enabled, has_drive, vel_mode, has_limit;
+    float compliance, damping, friction;
+    float d_target, d_kp, d_kd, d_max;
+    float lim_min, lim_max;
+    float lam;
+} D6Ax;
+/* A D6 joint (bodies.rs:107-136 Generic6Dof) keeps t[3] (translational axes of frame A) and r[3]
+ * (rotational), and its stored hinge axis (-1: derived from the rows, d6_hinge_axis). */
+typedef struct { uint32_t a, b; V3 fap, fbp; Q4 fra, frb; int kind; float rest, compliance, lam; D6Ax t[3], r[3]; int hinge; } Joint;
+/* A joint's warm rows (world.rs:250 warm_joints): row tags and impulses of its last solve (a D6
+ * joint has at most 5 rows per axis). */
+#define OR_JROWS_MAX 32
+typedef struct { uint8_t has, n, tag[OR_JROWS_MAX]; float lam[OR_JROWS_MAX]; } WarmJ;
 typedef struct { uint32_t body; float radius, height, speed, vvel; int grounded; V3 input; } OrPlayer;
 
 struct OrWorld {
@@ -428,8 +449,11 @@ int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *fa, c
         w->joints = xrealloc(w->joints, sizeof(Joint) * w->capj);
         w->wj = xrealloc(w->wj, sizeof(WarmJ) * w->capj);
     }
-    Joint j = {a, b, v3(fa[0], fa[1], fa[2]), v3(fb[0], fb[1], fb[2]), q4(fa[3], fa[4], fa[5], fa[6]), q4(fb[3], fb[4], fb[5], fb[6]),
-               J_BALL, 0.0f, 0.0f, 0.0f};
+    Joint j;
+    memset(&j, 0, sizeof j);
+    j.a = a; j.b = b; j.fap = v3(fa[0], fa[1], fa[2]); j.fbp = v3(fb[0], fb[1], fb[2]);
+    j.fra = q4(fa[3], fa[4], fa[5], fa[6]); j.frb = q4(fb[3], fb[4], fb[5], fb[6]);
+    j.kind = J_BALL; j.hinge = -1;
     w->joints[w->nj] = j;
     memset(&w->wj[w->nj], 0, sizeof(WarmJ));
     return (int32_t)w->nj++;
@@ -447,7 +471,51 @@ int32_t or_add_distance_joint(OrWorld *w, uint32_t a, uint32_t b, float rest, fl
     return q;
 }
 uint32_t or_num_joints(const OrWorld *w) { return w->nj; }
-/* Joint q's warm rows: returns their count (0: none stored), tags and impulses into the arrays (3 each). */
+/* bodies.rs:321-345 add_d6 / add_hinge: a D6 joint with the translational axes of t_mask and the
+ * rotational axes of r_mask constrained (bit i: axis i of frame A), and its stored hinge axis
+ * (0..2, or -1: none).  add_hinge(axis) is t_mask 7, r_mask 7 & ~(1 << axis), hinge axis. */
+int32_t or_add_d6_joint(OrWorld *w, uint32_t a, uint32_t b, const float *fa, const float *fb, uint32_t t_mask, uint32_t r_mask, int32_t hinge) {
+    int32_t q = or_add_ball_joint(w, a, b, fa, fb);
+    if (q < 0) return q;
+    Joint *j = &w->joints[q];
+    j->kind = J_D6;
+    j->hinge = hinge >= 0 && hinge < 3 ? hinge : -1;
+    for (int i = 0; i < 3; i++) { j->t[i].enabled = (t_mask >> i) & 1u; j->r[i].enabled = (r_mask >> i) & 1u; }
+    return q;
+}
+static D6Ax *d6_axis(OrWorld *w, uint32_t q, int angular, uint32_t axis) {
+    if (q >= w->nj || axis > 2 || w->joints[q].kind != J_D6) return NULL;
+    return angular ? &w->joints[q].r[axis] : &w->joints[q].t[axis];
+}
+/* leave_hinge (bodies.rs:878-880): a limit or drive on another rotational axis forgets the stored one */
+static void d6_leave_hinge(Joint *j, uint32_t axis) { if (j->hinge >= 0 && (uint32_t)j->hinge != axis) j->hinge = -1; }
+/* bodies.rs:347-390 set_d6_limit_t / set_d6_limit_r: a limit enables its axis */
+void or_set_d6_limit(OrWorld *w, uint32_t q, int angular, uint32_t axis, int on, float lo, float hi) {
+    D6Ax *x = d6_axis(w, q, angular, axis);
+    if (!x) return;
+    x->has_limit = on != 0; x->lim_min = lo; x->lim_max = hi;
+    if (on) { x->enabled = 1; if (angular) d6_leave_hinge(&w->joints[q], axis); }
+}
+/* set_d6_drive_t / set_d6_drive_r: a drive enables its axis */
+void or_set_d6_drive(OrWorld *w, uint32_t q, int angular, uint32_t axis, int on, float target, float kp, float kd, float max_impulse, int vel_mode) {
+    D6Ax *x = d6_axis(w, q, angular, axis);
+    if (!x) return;
+    x->has_drive = on != 0; x->d_target = target; x->d_kp = kp; x->d_kd = kd; x->d_max = max_impulse; x->vel_mode = vel_mode != 0;
+    if (on) { x->enabled = 1; if (angular) d6_leave_hinge(&w->joints[q], axis); }
+}
+/* set_d6_damping (bodies.rs:370-378): negative or non-finite values become 0 */
+void or_set_d6_damping(OrWorld *w, uint32_t q, int angular, uint32_t axis, float damping, float friction) {
+    D6Ax *x = d6_axis(w, q, angular, axis);
+    if (!x) return;
+    x->damping = fmaxf(dfin(damping), 0.0f); x->friction = fmaxf(dfin(friction), 0.0f);
+}
+/* an axis's compliance (D6Axis::compliance, m/N or rad/(N m); 0 = rigid) */
+void or_set_d6_compliance(OrWorld *w, uint32_t q, int angular, uint32_t axis, float compliance) {
+    D6Ax *x = d6_axis(w, q, angular, axis);
+    if (!x) return;
+    x->compliance = compliance;
+}
+/* Joint q's warm rows: returns their count (0: none stored), tags and impulses into the arrays (room for 32). */
 uint32_t or_get_joint_warm(const OrWorld *w, uint32_t q, uint8_t *tags, float *lams) {
     if (q >= w->nj || !w->wj[q].has) return 0;
     for (uint32_t k = 0; k < w->wj[q].n; k++) { tags[k] = w->wj[q].tag[k]; lams[k] = w->wj[q].lam[k]; }
@@ -948,16 +1016,28 @@ static uint32_t uf_find(uint32_t *p, uint32_t x) {
 /* ---- ball joints at velocity level: JointRow (solver.rs:272-289), joint_row_vel / joint_row_apply
  * (291-307), d6_rows for three locked translational axes (336-424: no limits, drives, damping,
  * friction or armature on a ball joint, so each axis gives one equality row, target 0, unbounded) */
-typedef struct { uint32_t a, b; V3 u, ra, rb; float wa, wb, m; uint8_t tag; } JRow;
+/* JointRow (solver.rs:262-289): bodies, angular or linear, direction, lever arms, inverse masses,
+ * 1 / (K + gamma), target velocity, accumulated-impulse bounds, softness gamma, tag */
+typedef struct { uint32_t a, b; int angular; V3 u, ra, rb; float wa, wb, m, target, lo, hi, gamma; uint8_t tag; } JRow;
 static float jrow_vel(const OrWorld *w, const JRow *r) {
+    if (r->angular) return vdot(vsub(w->ang[r->b], w->ang[r->a]), r->u);
     V3 va = vadd(w->lin[r->a], vcross(w->ang[r->a], r->ra));
     V3 vb = vadd(w->lin[r->b], vcross(w->ang[r->b], r->rb));
     return vdot(vsub(vb, va), r->u);
 }
 static void jrow_apply(OrWorld *w, const JRow *r, float lam) {
     V3 p = vmul(r->u, lam);
+    if (r->angular) {
+        if (r->wa > 0.0f) apply_ang_impulse(w, r->a, vneg(p));
+        if (r->wb > 0.0f) apply_ang_impulse(w, r->b, p);
+        return;
+    }
     if (r->wa > 0.0f) { apply_impulse(w, r->a, vneg(p)); apply_ang_impulse(w, r->a, vcross(r->ra, vneg(p))); }
     if (r->wb > 0.0f) { apply_impulse(w, r->b, p); apply_ang_impulse(w, r->b, vcross(r->rb, p)); }
+}
+static JRow jrow_make(uint32_t a, uint32_t b, int angular, V3 u, V3 ra, V3 rb, float wa, float wb, float m, uint8_t tag) {
+    JRow r = {a, b, angular, u, ra, rb, wa, wb, m, 0.0f, -INFINITY, INFINITY, 0.0f, tag};
+    return r;
 }
 static uint32_t ball_rows(const OrWorld *w, const Joint *j, JRow *out) {
     uint32_t a = j->a, b = j->b;
@@ -975,9 +1055,109 @@ static uint32_t ball_rows(const OrWorld *w, const Joint *j, JRow *out) {
         V3 ca = vcross(ra, u), cb = vcross(rb, u);
         float k = ((wa + wb) + vdot(ca, m3v(ia, ca))) + vdot(cb, m3v(ib, cb));
         if (k <= 1.0e-9f) continue;
-        JRow r = {a, b, u, ra, rb, wa, wb, 1.0f / k, (uint8_t)(i << 3)};
-        out[n++] = r;
+        out[n++] = jrow_make(a, b, 0, u, ra, rb, wa, wb, 1.0f / k, (uint8_t)(i << 3));
     }
+    return n;
+}
+
+/* bodies.rs:50-53 position_damped: a compliant damped axis with no limit or velocity drive keeps
+ * its damping at position level */
+static int d6_position_damped(const D6Ax *x) {
+    return x->enabled && x->compliance > 0.0f && x->damping > 0.0f && !x->has_limit && (!x->has_drive || !x->vel_mode);
+}
+/* bodies.rs:895-906 d6_hinge_axis: the stored axis, else the one rotational axis that is free,
+ * limited or driven, else the one compliant axis at least 100x softer than the others */
+static int d6_hinge_axis(const Joint *j) {
+    if (j->hinge >= 0) return j->hinge < 3 ? j->hinge : -1;
+    int freeax = -1, n = 0;
+    for (int i = 0; i < 3; i++) {
+        const D6Ax *r = &j->r[i];
+        if (!(r->enabled && !r->has_limit && !r->has_drive)) { freeax = i; n++; }
+    }
+    if (n == 1) return freeax;
+    float c[3];
+    for (int i = 0; i < 3; i++) c[i] = j->r[i].enabled ? fmaxf(dfin(j->r[i].compliance), 0.0f) : 0.0f;
+    for (int i = 0; i < 3; i++) {
+        if (!(j->r[i].enabled && c[i] > 0.0f)) continue;
+        int all = 1;
+        for (int k = 0; k < 3; k++) if (k != i && !(c[i] >= 100.0f * c[k])) all = 0;
+        if (all) return i;
+    }
+    return -1;
+}
+static V3 unit3(int i) { return v3(i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f); }
+static float v3at(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+/* bodies.rs:913-922 d6_row_angle: the hinge's own axis measures its twist (2 atan2, radians);
+ * another rotational axis the small-angle measure (rot_a (2 q.xyz)) . (rot_a e_i).  The atan2 is
+ * the once-rounded double evaluation (or_atan2f). */
+static float d6_row_angle(const Joint *j, Q4 rot_a, Q4 rot_b, int i) {
+    Q4 q = qmul(qconj(rot_a), rot_b);
+    if (q.w < 0.0f) q = q4(-q.x, -q.y, -q.z, -q.w);
+    V3 v = v3(q.x, q.y, q.z);
+    if (d6_hinge_axis(j) == i) return 2.0f * or_atan2f(v3at(v, i), q.w);
+    return vdot(qrot(rot_a, vmul(v, 2.0f)), qrot(rot_a, unit3(i)));
+}
+/* solver.rs:336-424 d6_rows (no armature): per axis, a damper (v + gamma lam = 0) and a Coulomb
+ * friction row, then a locked axis's equality, a velocity drive's motor row, or a limit's two
+ * speculative rows; position drives and compliant axes stay at position level */
+static uint32_t d6_rows(const OrWorld *w, const Joint *j, float dt, JRow *out) {
+    uint32_t a = j->a, b = j->b;
+    V3 pa = w->pos[a], pb = w->pos[b];
+    Q4 qa = w->rot[a], qb = w->rot[b];
+    Q4 rot_a = qmul(qa, j->fra), rot_b = qmul(qb, j->frb);
+    V3 xa = vadd(pa, qrot(qa, j->fap)), xb = vadd(pb, qrot(qb, j->fbp));
+    V3 axes[3] = {qrot(rot_a, unit3(0)), qrot(rot_a, unit3(1)), qrot(rot_a, unit3(2))};
+    float wa = w->inv_mass[a], wb = w->inv_mass[b];
+    if (wa == 0.0f && wb == 0.0f) return 0;
+    M3 ia = inv_inertia_world(w, a), ib = inv_inertia_world(w, b);
+    V3 ra = vsub(xa, pa), rb = vsub(xb, pb), d = vsub(xb, xa);
+    uint32_t n = 0;
+    for (int angular = 0; angular < 2; angular++)
+        for (int i = 0; i < 3; i++) {
+            const D6Ax *row = angular ? &j->r[i] : &j->t[i];
+            V3 u = axes[i];
+            float kk;
+            if (angular) kk = vdot(u, m3v(ia, u)) + vdot(u, m3v(ib, u));
+            else { V3 ca = vcross(ra, u), cb = vcross(rb, u); kk = ((wa + wb) + vdot(ca, m3v(ia, ca))) + vdot(cb, m3v(ib, cb)); }
+            uint8_t tag = (uint8_t)((angular << 5) | (i << 3));
+            float damp = d6_position_damped(row) ? 0.0f : row->damping;
+            if ((damp > 0.0f || row->friction > 0.0f) && kk > 1.0e-9f) {
+                JRow base = jrow_make(a, b, angular, u, ra, rb, wa, wb, 1.0f / kk, tag);
+                if (damp > 0.0f) {
+                    JRow r = base;
+                    r.gamma = 1.0f / (damp * dt);
+                    r.m = 1.0f / (kk + r.gamma);
+                    r.tag = tag | 4;
+                    out[n++] = r;
+                }
+                if (row->friction > 0.0f) {
+                    JRow r = base;
+                    float cap = row->friction * dt;
+                    r.lo = -cap; r.hi = cap; r.tag = tag | 5;
+                    out[n++] = r;
+                }
+            }
+            if (!row->enabled || row->compliance > 0.0f) continue;
+            if (row->has_drive && !row->vel_mode) continue;
+            if (kk <= 1.0e-9f) continue;
+            JRow base = jrow_make(a, b, angular, u, ra, rb, wa, wb, 1.0f / kk, tag);
+            if (row->has_drive) {
+                JRow r = base;
+                float cap = fabsf(row->d_max);
+                r.target = row->d_target; r.lo = -cap; r.hi = cap; r.tag = tag | 1;
+                out[n++] = r;
+            }
+            if (!row->has_limit) {
+                if (!row->has_drive) out[n++] = base;
+            } else {
+                float x = angular ? d6_row_angle(j, rot_a, rot_b, i) : vdot(d, u);
+                JRow lo = base, hi = base;
+                lo.target = fminf((row->lim_min - x) / dt, 0.0f); lo.lo = 0.0f; lo.tag = tag | 2;
+                hi.target = fmaxf((row->lim_max - x) / dt, 0.0f); hi.hi = 0.0f; hi.lo = -INFINITY; hi.tag = tag | 3;
+                out[n++] = lo;
+                out[n++] = hi;
+            }
+        }
     return n;
 }
 
@@ -1369,12 +1549,13 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
     /* joint rows of the active joints (a dynamic body on either side, islands.rs:96-101;
      * solver.rs:647-665), then their warm start from last step's rows when the tags agree, serial in
      * joint order (solver.rs:700-715) */
-    JRow *jr = malloc(sizeof(JRow) * (3 * w->nj + 1));
+    JRow *jr = malloc(sizeof(JRow) * (OR_JROWS_MAX * w->nj + 1));
     uint32_t *jr0 = malloc(4 * (w->nj + 1)), *jr1 = malloc(4 * (w->nj + 1)), njr = 0;
     for (uint32_t q = 0; q < w->nj; q++) {
         const Joint *j = &w->joints[q];
         jr0[q] = njr;
         if (j->kind == J_BALL && (w->inv_mass[j->a] > 0.0f || w->inv_mass[j->b] > 0.0f)) njr += ball_rows(w, j, jr + njr);
+        if (j->kind == J_D6 && (w->inv_mass[j->a] > 0.0f || w->inv_mass[j->b] > 0.0f)) njr += d6_rows(w, j, dt, jr + njr);
         jr1[q] = njr;
     }
     float *racc = calloc(njr + 1, 4);
@@ -1385,7 +1566,7 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
         for (uint32_t k = s0; k < e0; k++) if (w->wj[q].tag[k - s0] != jr[k].tag) same = 0;
         if (!same) continue;
         for (uint32_t k = s0; k < e0; k++) {
-            float lam = w->wj[q].lam[k - s0]; /* clamped to the row's (-inf, inf): unchanged */
+            float lam = clampf_rs(w->wj[q].lam[k - s0], jr[k].lo, jr[k].hi);
             racc[k] = lam;
             if (lam != 0.0f) jrow_apply(w, &jr[k], lam);
         }
@@ -1582,7 +1763,9 @@ static void solve(OrWorld *w, Contact *cts, uint32_t nct, const V3 *pre_lin, con
                     jran[q] = 1;
                     float v = jrow_vel(w, r);
                     float old = racc[k];
-                    float nwv = old + (0.0f - v) * r->m;
+                    /* rigid rows old + (target - v) m; soft rows (dampers) add gamma lam */
+                    float nwv = r->gamma == 0.0f ? old + (r->target - v) * r->m : old + ((r->target - v) - r->gamma * old) * r->m;
+                    nwv = clampf_rs(nwv, r->lo, r->hi);
                     racc[k] = nwv;
                     if (nwv != old) jrow_apply(w, r, nwv - old);
                     float rr = fabsf(nwv - old) / r->m;
@@ -1873,6 +2056,126 @@ static void ball_pos(OrWorld *w, const Joint *j, const V3 *spos, const Q4 *srot,
         pos_delta(w, b, dfin_v(vmul(un, dfin(dl * wb))));
         orient_delta(w, a, dfin_v(vmul(vneg(m3v(iaw, rau)), dl)));
         orient_delta(w, b, dfin_v(vmul(m3v(ibw, rbu), dl)));
+    }
+}
+
+/* bodies.rs:1010-1030 row_error: locked at 0, a limit only while violated, a position drive's target
+ * (clamped into the limit), a velocity drive free at position level.  Returns 0 for no row. */
+static int d6_row_error(const D6Ax *x, float coord, float *c) {
+    int has = 1;
+    *c = dfin(coord - 0.0f);
+    if (x->has_limit) {
+        float lo = dfin(x->lim_min), hi = dfin(x->lim_max);
+        if (coord < lo) *c = dfin(coord - lo);
+        else if (coord > hi) *c = dfin(coord - hi);
+        else has = 0;
+    }
+    if (x->has_drive) {
+        if (!x->vel_mode) {
+            float t = dfin(x->d_target);
+            if (x->has_limit) t = clampf_rs(t, dfin(x->lim_min), dfin(x->lim_max));
+            *c = dfin(coord - t);
+            has = 1;
+        } else if (!x->has_limit) has = 0;
+    }
+    return has;
+}
+/* bodies.rs:433-438 alpha~ = compliance / dt^2 (0 for rigid rows) */
+static float d6_alpha(float c, float dt2) { c = fmaxf(dfin(c), 0.0f); return c <= 0.0f ? 0.0f : dfin(c / dt2); }
+/* the angle a rotational row measures at frame rotations ra, rb (bodies.rs:786-795) */
+static float d6_pos_angle(int hinge, int i, Q4 ra, Q4 rb, V3 u) {
+    Q4 q = qmul(qconj(ra), rb);
+    if (q.w < 0.0f) q = q4(-q.x, -q.y, -q.z, -q.w);
+    V3 v = v3(q.x, q.y, q.z);
+    if (hinge == i) return dfin(2.0f * or_atan2f(v3at(v, i), q.w));
+    if (hinge >= 0) return dfin(vdot(dfin_v(vcross(normalize_fin(qrot(ra, unit3(hinge))), normalize_fin(qrot(rb, unit3(hinge))))), u));
+    return dfin(vdot(dfin_v(qrot(ra, dfin_v(vmul(v, 2.0f)))), u));
+}
+/* bodies.rs:616-873 solve_d6 (no rotor rows): each translational row, then each rotational row,
+ * measured afresh from the current poses and corrected as an XPBD row along the step-start
+ * directions (SHAKE) unless a body turned 0.5 rad or more; a damped compliant row's change since
+ * the step's start joins as XPBD's damping term. */
+static void d6_pos(OrWorld *w, Joint *j, const V3 *spos, const Q4 *srot, float dt) {
+    uint32_t a = j->a, b = j->b;
+    float wa = dfin(w->inv_mass[a]), wb = dfin(w->inv_mass[b]);
+    float dt2 = fmaxf(dt * dt, 1.0e-12f);
+    V3 sap = spos[a], sbp = spos[b];
+    Q4 sa = srot[a], sb = srot[b];
+    int shake = turned(sa, w->rot[a]) < 0.5f && turned(sb, w->rot[b]) < 0.5f;
+    V3 ax0[3] = {v3s(0), v3s(0), v3s(0)}, la0 = v3s(0), lb0 = v3s(0);
+    M3 ia0 = m3zero(), ib0 = m3zero();
+    if (shake) {
+        Q4 ra0 = qmul(sa, j->fra);
+        for (int i = 0; i < 3; i++) ax0[i] = normalize_fin(qrot(ra0, unit3(i)));
+        la0 = dfin_v(qrot(sa, j->fap)); lb0 = dfin_v(qrot(sb, j->fbp));
+        ia0 = inv_inertia_at(w, a, sa); ib0 = inv_inertia_at(w, b, sb);
+    }
+    for (int i = 0; i < 3; i++) {
+        D6Ax *row = &j->t[i];
+        if (!row->enabled) continue;
+        V3 pa = w->pos[a], pb = w->pos[b];
+        Q4 qa = w->rot[a], qb = w->rot[b];
+        V3 u = normalize_fin(qrot(qmul(qa, j->fra), unit3(i)));
+        if (viszero(u)) continue;
+        V3 xa = dfin_v(vadd(pa, qrot(qa, j->fap)));
+        V3 xb = dfin_v(vadd(pb, qrot(qb, j->fbp)));
+        V3 d = dfin_v(vsub(xb, xa));
+        V3 un, lva, lvb; M3 iaw, ibw;
+        if (shake) { un = ax0[i]; lva = la0; lvb = lb0; iaw = ia0; ibw = ib0; }
+        else { un = u; lva = vsub(xa, pa); lvb = vsub(xb, pb); iaw = inv_inertia_world(w, a); ibw = inv_inertia_world(w, b); }
+        float coord = dfin(vdot(d, u)), c;
+        if (!d6_row_error(row, coord, &c)) continue;
+        V3 rau = dfin_v(vcross(lva, un)), rbu = dfin_v(vcross(lvb, un));
+        float wra = dfin(vdot(rau, m3v(iaw, rau))), wrb = dfin(vdot(rbu, m3v(ibw, rbu)));
+        float wsum = dfin(((wa + wb) + wra) + wrb);
+        float alpha = d6_alpha(row->compliance, dt2);
+        float denom = dfin(wsum + alpha);
+        if (denom <= 0.0f) continue;
+        c = dfin(c + dfin(0.0f * dt));
+        float lold = dfin(row->lam), dl;
+        if (d6_position_damped(row)) {
+            V3 u0 = normalize_fin(qrot(qmul(sa, j->fra), unit3(i)));
+            V3 d0 = dfin_v(vsub(vadd(sbp, qrot(sb, j->fbp)), vadd(sap, qrot(sa, j->fap))));
+            float gamma = dfin(row->compliance * row->damping / dt);
+            float dc = dfin(coord - vdot(d0, u0));
+            dl = dfin(-((c + dfin(alpha * lold)) + dfin(gamma * dc)) / dfin((1.0f + gamma) * wsum + alpha));
+        } else dl = dfin(-(c + dfin(alpha * lold)) / denom);
+        row->lam = dfin(lold + dl);
+        pos_delta(w, a, dfin_v(vmul(vneg(un), dfin(dl * wa))));
+        pos_delta(w, b, dfin_v(vmul(un, dfin(dl * wb))));
+        orient_delta(w, a, dfin_v(vmul(vneg(m3v(iaw, rau)), dl)));
+        orient_delta(w, b, dfin_v(vmul(m3v(ibw, rbu), dl)));
+    }
+    int hinge = d6_hinge_axis(j);
+    for (int i = 0; i < 3; i++) {
+        D6Ax *row = &j->r[i];
+        if (!row->enabled) continue;
+        Q4 ra = qmul(w->rot[a], j->fra), rb = qmul(w->rot[b], j->frb);
+        V3 u = normalize_fin(qrot(ra, unit3(i)));
+        if (viszero(u)) continue;
+        float angle = d6_pos_angle(hinge, i, ra, rb, u);
+        V3 un; M3 iaw, ibw;
+        if (shake) { un = ax0[i]; iaw = ia0; ibw = ib0; }
+        else { un = u; iaw = inv_inertia_world(w, a); ibw = inv_inertia_world(w, b); }
+        float c;
+        if (!d6_row_error(row, angle, &c)) continue;
+        float wsum = dfin(vdot(un, m3v(iaw, un)) + vdot(un, m3v(ibw, un)));
+        float alpha = d6_alpha(row->compliance, dt2);
+        float denom = dfin(wsum + alpha);
+        if (denom <= 0.0f) continue;
+        c = dfin(c + dfin(0.0f * dt));
+        float lold = dfin(row->lam), dl;
+        if (d6_position_damped(row)) {
+            Q4 ra0 = qmul(sa, j->fra), rb0 = qmul(sb, j->frb);
+            V3 u0 = normalize_fin(qrot(ra0, unit3(i)));
+            float angle0 = d6_pos_angle(hinge, i, ra0, rb0, u0);
+            float gamma = dfin(row->compliance * row->damping / dt);
+            float dc = dfin(angle - angle0);
+            dl = dfin(-((c + dfin(alpha * lold)) + dfin(gamma * dc)) / dfin((1.0f + gamma) * wsum + alpha));
+        } else dl = dfin(-(c + dfin(alpha * lold)) / denom);
+        row->lam = dfin(lold + dl);
+        orient_delta(w, a, dfin_v(vmul(vneg(m3v(iaw, un)), dl)));
+        orient_delta(w, b, dfin_v(vmul(m3v(ibw, un), dl)));
     }
 }
 
@@ -2268,12 +2571,16 @@ void or_step(OrWorld *w, float dt) {
             frozen[i] = !live;
         }
         /* Joints::begin_step (bodies.rs:308-318): the step's multipliers start at zero */
-        for (uint32_t q = 0; q < w->nj; q++) w->joints[q].lam = 0.0f;
+        for (uint32_t q = 0; q < w->nj; q++) {
+            w->joints[q].lam = 0.0f;
+            for (int i = 0; i < 3; i++) { w->joints[q].t[i].lam = 0.0f; w->joints[q].r[i].lam = 0.0f; }
+        }
         for (uint32_t it = 0; it < w->ip; it++) {
             for (uint32_t q = 0; q < w->nj; q++) {
                 Joint *j = &w->joints[q];
                 if (frozen[j->a] && frozen[j->b]) continue;
                 if (j->kind == J_DIST) dist_pos(w, j, spos, dt);
+                else if (j->kind == J_D6) d6_pos(w, j, spos, srot, dt);
                 else ball_pos(w, j, spos, srot, dt);
             }
             project(w, pc, npc, slin, sang, 0);
@@ -2288,6 +2595,16 @@ void or_step(OrWorld *w, float dt) {
         const Contact *c = &cul.v[k];
         if (c->b == TERRAIN_CI && c->depth - MARGIN > -POS_SLOP && contact_props(w, c).stiff <= 0.0f) ground[w->cbody[c->a]] = 1;
     }
+    /* step.rs:353-365: a body of a D6 joint with a rotational velocity drive keeps the velocity
+     * solve's angular velocity (no derivation from its rotation change) */
+    uint8_t *driven = calloc(count + 1, 1);
+    for (uint32_t q = 0; q < w->nj; q++) {
+        const Joint *j = &w->joints[q];
+        if (j->kind != J_D6) continue;
+        int vd = 0;
+        for (int k = 0; k < 3; k++) if (j->r[k].has_drive && j->r[k].vel_mode) vd = 1;
+        if (vd) { driven[j->a] = 1; driven[j->b] = 1; }
+    }
     for (uint32_t i = 0; i < count; i++) {
         if (!w->dyn[i] || !w->awake[i]) continue;
         V3 vl = w->lin[i], va = w->ang[i];
@@ -2300,7 +2617,7 @@ void or_step(OrWorld *w, float dt) {
             np.y = opos[i].y;
             b_set_pose(w, i, np, nq);
         }
-        if (!qeq(nq, prot[i])) {
+        if (!driven[i] && !qeq(nq, prot[i])) {
             Q4 dq = qmul(nq, qconj(prot[i]));
             V3 da = vmul(v3(dq.x, dq.y, dq.z), 2.0f);
             if (dq.w < 0.0f) da = vneg(da);
@@ -2308,7 +2625,7 @@ void or_step(OrWorld *w, float dt) {
         }
         b_set_vel(w, i, vl, va);
     }
-    free(ground);
+    free(ground); free(driven);
     for (uint32_t i = 0; i < count; i++) w->last_lin[i] = w->lin[i];
     /* S7 (sleep.rs:59-80, step.rs:381-384) */
     for (uint32_t i = 0; i < count; i++) {

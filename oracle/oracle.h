@@ -111,6 +111,16 @@ int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *frame
 /* world.rs:537 add_distance_joint between the centres of mass of a and b: rest length (m),
  * compliance (m/N, 0 = rigid).  Returns the joint index, or -1. */
 int32_t or_add_distance_joint(OrWorld *w, uint32_t a, uint32_t b, float rest, float compliance);
+/* bodies.rs:321-390 (no armature): a D6 joint with the translational axes of t_mask and the
+ * rotational axes of r_mask constrained (bit i: axis i of frame A) and a stored hinge axis (0..2,
+ * -1 none); a hinge is t_mask 7, r_mask 7 minus its axis.  Per axis (angular: rotational):
+ * a limit (on = 0 removes it), a drive (position or velocity mode), damping and friction, and
+ * compliance. */
+int32_t or_add_d6_joint(OrWorld *w, uint32_t a, uint32_t b, const float *fa, const float *fb, uint32_t t_mask, uint32_t r_mask, int32_t hinge);
+void or_set_d6_limit(OrWorld *w, uint32_t q, int angular, uint32_t axis, int on, float lo, float hi);
+void or_set_d6_drive(OrWorld *w, uint32_t q, int angular, uint32_t axis, int on, float target, float kp, float kd, float max_impulse, int vel_mode);
+void or_set_d6_damping(OrWorld *w, uint32_t q, int angular, uint32_t axis, float damping, float friction);
+void or_set_d6_compliance(OrWorld *w, uint32_t q, int angular, uint32_t axis, float compliance);
 /* world.rs:595 add_body without add_collider: the shape only sets the mass properties. */
 int32_t or_add_body_nc(OrWorld *w, int shape, const float *dims, const float *pose, const float *vel, int dynamic,
                        float density);

@@ -383,3 +383,79 @@ def contact_model_scene(k, d, mu, shape, dims, pos, vel=(0.0, 0.0, 0.0), ang=(0.
     s["pairs"] = [(mats[0], mats[1], f(mu), f(mu), f(rest), f(0.1))]
     s["contact_models"] = [(mats[0], mats[1], f(k), f(d), bool(patch))]
     return s
+
+
+# D6 joint scenes (hinge_angle_test.rs, joint_damping_test.rs, joint_energy_test.rs,
+# joint_bits_probe.rs): static pivots carry a 1 cm sphere collider; rods are 1 m x 5 cm x 5 cm
+# boxes of density 1000 (2.5 kg) lying along +x.  A "d6" entry names bodies by scene index, its
+# frames {p, q} in each body's space, the constrained axes (t_mask, r_mask), a stored hinge axis,
+# and per axis (angular flag, axis, ...) limits, drives, damping and friction, compliance.
+ROD_HALF = (0.5, 0.025, 0.025)
+
+
+def _static_pin(pos):
+    return dict(shape=0, dims=np.array([0.01, 0.0, 0.0], np.float32), pose=np.array([*pos, 0, 0, 0, 1], np.float32),
+                vel=np.zeros(6, np.float32), dynamic=0, density=np.float32(0.0), material=0, sleep_allowed=1)
+
+
+def _box_body(half, pos, density, vel=(0, 0, 0, 0, 0, 0), sleep=1):
+    return dict(shape=1, dims=np.array(half, np.float32), pose=np.array([*pos, 0, 0, 0, 1], np.float32), vel=np.array(vel, np.float32),
+                dynamic=1, density=np.float32(density), material=0, sleep_allowed=sleep)
+
+
+def _iso(p):
+    return np.array([*p, 0.0, 0.0, 0.0, 1.0], np.float32)
+
+
+def _joint_scene(bodies, joints, gravity=True, hz=240.0, name="d6"):
+    s = bounce_scene_of(bodies, gravity=gravity, hz=hz, name=name)
+    s["d6"] = joints
+    return s
+
+
+def hinge_rod(limit=None, hz=240.0, drive=None, damping=None, compliance=None, gravity=True):
+    """A rod hinged at its end about z to a static pivot at (0, 10, 0), lying along +x."""
+    j = dict(a=0, b=1, fa=_iso((0, 0, 0)), fb=_iso((-0.5, 0, 0)), t_mask=7, r_mask=3, hinge=2)
+    if limit is not None:
+        j["limits"] = [(True, 2, limit[0], limit[1])]
+    if drive is not None:
+        j["drives"] = [(True, 2, *drive)]
+    if damping is not None:
+        j["damping"] = [(True, 2, *damping)]
+    if compliance is not None:  # a sprung hinge: the twist row enabled, compliant
+        j["compliance"] = [(True, 2, compliance)]
+        j["r_mask"] = 7
+    return _joint_scene([_static_pin((0, 10, 0)), _box_body(ROD_HALF, (0.5, 10, 0), 1000.0, sleep=0)], [j], gravity, hz, "hinge_rod")
+
+
+SPIN_HALF, SPIN_MASS = (0.5, 0.1, 0.1), 2.0
+
+
+def hinge_spinner(w0, damping=0.0, friction=0.0, hz=60.0):
+    """A 2 kg bar hinged about z through its centre to a static anchor, spinning at w0, no gravity."""
+    d = SPIN_MASS / (8.0 * SPIN_HALF[0] * SPIN_HALF[1] * SPIN_HALF[2])
+    j = dict(a=0, b=1, fa=_iso((0, 0, 0)), fb=_iso((0, 0, 0)), t_mask=7, r_mask=3, hinge=2, damping=[(True, 2, damping, friction)])
+    bar = _box_body(SPIN_HALF, (0, 0, 0), d, vel=(0, 0, 0, 0, 0, w0), sleep=0)
+    return _joint_scene([_static_pin((0, 0, 0)), bar], [j], False, hz, "hinge_spinner")
+
+
+def cube_pendulum(hz=62.5):
+    """An 80 kg, 2 m cube on a 5 m arm hinged about z at (0, 20, 0), released horizontal."""
+    j = dict(a=0, b=1, fa=_iso((0, 0, 0)), fb=_iso((-5.0, 0, 0)), t_mask=7, r_mask=3, hinge=2)
+    return _joint_scene([_static_pin((0, 20, 0)), _box_body((1.0, 1.0, 1.0), (5.0, 20, 0), 10.0, sleep=0)], [j], True, hz, "cube_pendulum")
+
+
+def hinge_chain(hz=240.0):
+    """Four rods hinged end to end about z from a static pivot: the first limited to +-1 rad, the
+    second a velocity motor (2 rad/s, at most 0.05 N s a step), the rest free
+    (joint_bits_probe.rs)."""
+    bodies = [_static_pin((0, 10, 0))] + [_box_body(ROD_HALF, (0.5 + k, 10, 0), 1000.0, sleep=0) for k in range(4)]
+    joints = []
+    for k in range(4):
+        j = dict(a=k, b=k + 1, fa=_iso((0, 0, 0) if k == 0 else (0.5, 0, 0)), fb=_iso((-0.5, 0, 0)), t_mask=7, r_mask=3, hinge=2)
+        if k == 0:
+            j["limits"] = [(True, 2, -1.0, 1.0)]
+        if k == 1:
+            j["drives"] = [(True, 2, 2.0, 0.0, 0.0, 0.05, True)]
+        joints.append(j)
+    return _joint_scene(bodies, joints, True, hz, "hinge_chain")

@@ -125,6 +125,12 @@ def lib():
         L.or_sincosf.argtypes = [C.c_float, FP, FP]
         L.or_set_pair_contact_model.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_int]
         L.or_set_compliant_damping_at_start.argtypes = [P, C.c_int]
+        L.or_add_d6_joint.restype = C.c_int32
+        L.or_add_d6_joint.argtypes = [P, C.c_uint32, C.c_uint32, FP, FP, C.c_uint32, C.c_uint32, C.c_int32]
+        L.or_set_d6_limit.argtypes = [P, C.c_uint32, C.c_int, C.c_uint32, C.c_int, C.c_float, C.c_float]
+        L.or_set_d6_drive.argtypes = [P, C.c_uint32, C.c_int, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
+        L.or_set_d6_damping.argtypes = [P, C.c_uint32, C.c_int, C.c_uint32, C.c_float, C.c_float]
+        L.or_set_d6_compliance.argtypes = [P, C.c_uint32, C.c_int, C.c_uint32, C.c_float]
         L.or_patch_radius.restype = C.c_float
         L.or_patch_radius.argtypes = [FP, C.c_uint32]
         L.or_atan2f_unit.restype = C.c_float
@@ -186,6 +192,16 @@ class OracleWorld:
             fa = np.ascontiguousarray(fa, np.float32)
             fb = np.ascontiguousarray(fb, np.float32)
             assert L.or_add_ball_joint(self.h, int(a), int(b), fptr(fa), fptr(fb)) >= 0
+        for j in s.get("d6", []):  # after the ball joints: joint indices in scene order
+            q = self.add_d6_joint(int(j["a"]), int(j["b"]), j["fa"], j["fb"], j.get("t_mask", 7), j.get("r_mask", 0), j.get("hinge", -1))
+            for ang, ax, lo, hi in j.get("limits", []):
+                L.or_set_d6_limit(self.h, q, int(ang), ax, 1, lo, hi)
+            for ang, ax, target, kp, kd, mx, vel in j.get("drives", []):
+                L.or_set_d6_drive(self.h, q, int(ang), ax, 1, target, kp, kd, mx, int(vel))
+            for ang, ax, c, fr in j.get("damping", []):
+                L.or_set_d6_damping(self.h, q, int(ang), ax, c, fr)
+            for ang, ax, c in j.get("compliance", []):
+                L.or_set_d6_compliance(self.h, q, int(ang), ax, c)
         hf = s.get("hf")
         if hf is not None:
             hts = np.ascontiguousarray(hf["heights"], np.float32)
@@ -216,6 +232,17 @@ class OracleWorld:
         if not b.get("sleep_allowed", 1):
             L.or_set_sleep_allowed(self.h, i, 0)
         return i
+
+    def add_d6_joint(self, a, b, fa, fb, t_mask=7, r_mask=0, hinge=-1):
+        """A D6 joint between bodies a and b (0-based), frames {p, q} in each body's space."""
+        fa = np.ascontiguousarray(fa, np.float32)
+        fb = np.ascontiguousarray(fb, np.float32)
+        q = lib().or_add_d6_joint(self.h, a, b, fptr(fa), fptr(fb), t_mask, r_mask, hinge)
+        assert q >= 0
+        return q
+
+    def add_hinge_joint(self, a, b, fa, fb, axis):
+        return self.add_d6_joint(a, b, fa, fb, 7, 7 & ~(1 << axis), axis)
 
     def add_player(self, pose, radius, height, speed):
         return lib().or_add_player(self.h, fptr(np.asarray(pose, np.float32)), float(radius), float(height), float(speed))

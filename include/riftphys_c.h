@@ -184,6 +184,45 @@ RPhysResult rphys_add_body_density(RPhysWorld *world, RPhysShape shape, const fl
  * ERR_BAD_ARG.  A removed body's joints stay, attached to a static body, as in the reference. */
 RPhysResult rphys_world_add_ball_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,
                                        const RPhysIsometry *frame_b, uint32_t *out_joint_id);
+/* replaces World::add_d6_joint / add_hinge_joint (crates/riftphys-world/src/world/world.rs:719-720;
+ * crates/riftphys-articulation/src/bodies.rs:321-338 add_d6 / add_hinge): a D6 joint whose
+ * translational axes (t_mask) and rotational axes (r_mask) of frame A are constrained (bit i: axis
+ * i), with a stored hinge axis (0..2, or -1: none, derived each step as bodies.rs:895-906 does).
+ * Limits, drives, damping, friction and compliance per axis are set by the calls below.  Solved as
+ * the reference solves a D6 joint with no armature: velocity rows of dampers, friction, locked axes,
+ * velocity motors and speculative limits (solver.rs:336-424), warm-started by row kind, and the
+ * position solve of solve_d6 (bodies.rs:616-873: rows of every constrained axis, the hinge's twist
+ * in radians, limits, position drives, XPBD compliance).  Islands holding a D6 joint are solved by
+ * the workgroup-per-island tier.  Errors as rphys_world_add_ball_joint, and ERR_BAD_ARG for a mask
+ * above 7 or a hinge axis outside -1..2. */
+RPhysResult rphys_world_add_d6_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,
+                                     const RPhysIsometry *frame_b, uint32_t t_mask, uint32_t r_mask, int32_t hinge_axis,
+                                     uint32_t *out_joint_id);
+/* bodies.rs:333-338 add_hinge: all translational axes and the two rotational axes other than
+ * `axis` locked, `axis` stored as the hinge axis.  Errors: axis above 2 -> ERR_BAD_ARG. */
+RPhysResult rphys_world_add_hinge_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,
+                                        const RPhysIsometry *frame_b, uint32_t axis, uint32_t *out_joint_id);
+/* replaces World::set_d6_limit_r and JointSet::set_d6_limit_t (world.rs:968; bodies.rs:361-390):
+ * on = RPHYS_FALSE removes axis `axis`'s limit; a limit [lo, hi] (m, or rad of the axis's twist
+ * angle) constrains its axis, and on a rotational axis other than the stored hinge axis forgets
+ * the stored axis (bodies.rs:878-880).  Errors: unknown joint -> ERR_INVALID_ID; a ball joint, an
+ * axis above 2 or a non-finite bound -> ERR_BAD_ARG. */
+RPhysResult rphys_world_set_joint_limit(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, RPhysBool on, float lo, float hi);
+/* replaces World::set_d6_drive_r / set_d6_drive_t (world.rs:966-967; bodies.rs:347-359): a drive
+ * toward target constrains its axis; on = RPHYS_FALSE removes it.  A position drive (m or rad) is
+ * a position row toward its target (rigid, or with the axis's compliance); a velocity drive (m/s or
+ * rad/s) is a motor row whose impulse per step is capped at |max_impulse|.  kp and kd are stored as
+ * the reference's Drive stores them (bodies.rs:65-71); neither solve reads them.
+ * Errors as rphys_world_set_joint_limit (a non-finite drive value -> ERR_BAD_ARG). */
+RPhysResult rphys_world_set_joint_drive(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, RPhysBool on, float target,
+                                        float kp, float kd, float max_impulse, RPhysBool velocity_mode);
+/* replaces World::set_d6_damping (world.rs:971; bodies.rs:370-378): viscous damping (N s/m or
+ * N m s/rad) and Coulomb friction (N or N m) on one axis; negative or non-finite values become 0;
+ * neither constrains the axis.  Errors as rphys_world_set_joint_limit. */
+RPhysResult rphys_world_set_joint_damping(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, float damping, float friction);
+/* D6Axis::compliance (bodies.rs:52-104) of one axis: 0 rigid, else XPBD compliance (m/N or
+ * rad/(N m)) of its position row.  Errors as rphys_world_set_joint_limit. */
+RPhysResult rphys_world_set_joint_compliance(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, float compliance);
 uint32_t rphys_world_num_joints(const RPhysWorld *world);
 RPhysResult rphys_world_set_pair_props(RPhysWorld *world, uint32_t mat_a, uint32_t mat_b, float mu_s, float mu_k,
                                        float restitution, float stribeck_v);
@@ -287,6 +326,17 @@ typedef struct RPhysContactEx {
 RPhysResult rphys_debug_get_contacts_ex(const RPhysWorld *world, RPhysContactEx *out, uint32_t cap, uint32_t *out_count);
 /* Broadphase pairs (collider indices i < j) of the last tick, in the reference's sorted order. */
 RPhysResult rphys_debug_get_pairs(const RPhysWorld *world, uint32_t *out_ij, uint32_t cap_pairs, uint32_t *out_count);
+
+/* The bytes the world's step hash digests (its BLAKE3 input, world.rs:882-909): up to cap of them
+ * into out; *out_len receives their count.  For locating where two hashes of one state differ.
+ * Errors: null world or out_len -> ERR_NULL. */
+RPhysResult rphys_debug_get_hash_input(const RPhysWorld *world, uint8_t *out, uint64_t cap, uint64_t *out_len);
+
+/* Joint `joint`'s stored velocity rows (what next step's warm start reads, world.rs:896-900): up to
+ * cap (tag, impulse) pairs in row order, tag = angular << 5 | axis << 3 | kind (0 locked, 1 motor,
+ * 2 / 3 lower / upper limit, 4 damper, 5 friction); *out_n receives the row count (0: none stored).
+ * Errors: null world or out_n -> ERR_NULL; unknown joint -> ERR_INVALID_ID. */
+RPhysResult rphys_debug_get_joint_rows(const RPhysWorld *world, uint32_t joint, uint8_t *tags, float *impulses, uint32_t cap, uint32_t *out_n);
 
 /* Full carried state of a world: per body pos/rot/lin/ang/last linear velocity/awake/sleep
  * ticks, and the warm-start cache (keys (body_a << 32 | body_b or 0xFFFFFFFF), body-A-frame

@@ -2645,19 +2645,25 @@ void or_step(OrWorld *w, float dt) {
 }
 
 /* ------------------------------------------------------------------ hash (provenance.rs:6-24, world.rs:882-909) */
-static void hu32(OrBlake3 *h, uint32_t v) { uint8_t b[4] = {(uint8_t)v, (uint8_t)(v >> 8), (uint8_t)(v >> 16), (uint8_t)(v >> 24)}; or_blake3_update(h, b, 4); }
-static void hu64(OrBlake3 *h, uint64_t v) { hu32(h, (uint32_t)v); hu32(h, (uint32_t)(v >> 32)); }
-static void hf(OrBlake3 *h, float x) { hu32(h, canon_bits(x)); }
-static void hv3(OrBlake3 *h, V3 v) { hf(h, v.x); hf(h, v.y); hf(h, v.z); }
+/* The hash input goes to a BLAKE3 state, or (for diagnosing a differing hash) into a byte buffer. */
+typedef struct { OrBlake3 *b3; uint8_t *buf; size_t n, cap; } HSink;
+static void hput(HSink *h, const void *p, size_t n) {
+    if (h->b3) { or_blake3_update(h->b3, p, n); return; }
+    if (h->buf && h->n + n <= h->cap) memcpy(h->buf + h->n, p, n);
+    h->n += n;
+}
+static void hu32(HSink *h, uint32_t v) { uint8_t b[4] = {(uint8_t)v, (uint8_t)(v >> 8), (uint8_t)(v >> 16), (uint8_t)(v >> 24)}; hput(h, b, 4); }
+static void hu64(HSink *h, uint64_t v) { hu32(h, (uint32_t)v); hu32(h, (uint32_t)(v >> 32)); }
+static void hf(HSink *h, float x) { hu32(h, canon_bits(x)); }
+static void hv3(HSink *h, V3 v) { hf(h, v.x); hf(h, v.y); hf(h, v.z); }
 
-void or_step_hash(const OrWorld *w, uint8_t out[32]) {
-    OrBlake3 h;
-    or_blake3_init(&h);
+static void step_hash_feed(const OrWorld *w, HSink *hs) {
+#define h (*hs)
     hu64(&h, w->epoch);
     hu64(&h, w->rng);
     uint8_t empty[32];
     or_blake3(NULL, 0, empty); /* ScheduleRecorder::digest of no stages */
-    or_blake3_update(&h, empty, 32);
+    hput(&h, empty, 32);
     for (uint32_t i = 0; i < w->n; i++) {
         hu32(&h, i);
         hv3(&h, w->pos[i]);
@@ -2688,17 +2694,30 @@ void or_step_hash(const OrWorld *w, uint8_t out[32]) {
         if (!w->wj[q].has) continue;
         hu32(&h, q);
         hu32(&h, w->wj[q].n);
-        for (uint32_t k = 0; k < w->wj[q].n; k++) { or_blake3_update(&h, &w->wj[q].tag[k], 1); hf(&h, w->wj[q].lam[k]); }
+        for (uint32_t k = 0; k < w->wj[q].n; k++) { hput(&h, &w->wj[q].tag[k], 1); hf(&h, w->wj[q].lam[k]); }
     }
     hu64(&h, 0); /* rotor */
     hu64(&h, w->n);
     for (uint32_t i = 0; i < w->n; i++) hv3(&h, w->last_lin[i]);
     uint32_t sl = w->cap > w->n ? w->cap : w->n;
     hu64(&h, sl);
-    or_blake3_update(&h, w->awake, sl);
+    hput(&h, w->awake, sl);
     hu64(&h, sl);
-    for (uint32_t i = 0; i < sl; i++) { uint8_t b[2] = {(uint8_t)w->sticks[i], (uint8_t)(w->sticks[i] >> 8)}; or_blake3_update(&h, b, 2); }
-    or_blake3_final(&h, out);
+    for (uint32_t i = 0; i < sl; i++) { uint8_t b[2] = {(uint8_t)w->sticks[i], (uint8_t)(w->sticks[i] >> 8)}; hput(&h, b, 2); }
+#undef h
+}
+void or_step_hash(const OrWorld *w, uint8_t out[32]) {
+    OrBlake3 b3;
+    or_blake3_init(&b3);
+    HSink hs = {&b3, NULL, 0, 0};
+    step_hash_feed(w, &hs);
+    or_blake3_final(&b3, out);
+}
+/* The step hash's input bytes (up to cap of them into out); returns their count. */
+size_t or_step_hash_preimage(const OrWorld *w, uint8_t *out, size_t cap) {
+    HSink hs = {NULL, out, 0, cap};
+    step_hash_feed(w, &hs);
+    return hs.n;
 }
 
 uint32_t or_export_carried(const OrWorld *w, float *last3, uint8_t *awake, uint16_t *sticks, uint32_t *wkeys4, float *wla3, float *wjn,

@@ -105,6 +105,7 @@ void or_step(OrWorld *w, float dt);
  * pairs, terrain colliders, PGS level batches of at least 64 rows), bit-identical for any count. */
 void or_set_threads(int n);
 void or_step_hash(const OrWorld *w, uint8_t out32[32]);
+size_t or_step_hash_preimage(const OrWorld *w, uint8_t *out, size_t cap);
 /* world.rs:718 add_ball_joint: bodies a, b (indices), frames {px,py,pz,qx,qy,qz,qw} local to each
  * body.  Returns the joint index, or -1. */
 int32_t or_add_ball_joint(OrWorld *w, uint32_t a, uint32_t b, const float *frame_a, const float *frame_b);

@@ -67,6 +67,10 @@ static HM hinv(HM m) {  // Mat3A::inverse
     return r;
 }
 static inline bool hfin(float x) { return std::isfinite(x); }
+static inline uint32_t f2u(float x) { uint32_t u; std::memcpy(&u, &x, 4); return u; }
+static inline float u2f(uint32_t u) { float x; std::memcpy(&x, &u, 4); return x; }
+// D6 axis bits (rp_d6.h D6F_*)
+enum { D6F_ENABLED_BIT = 1u, D6F_DRIVE_BIT = 2u, D6F_VEL_BIT = 4u, D6F_LIMIT_BIT = 8u };
 // SSE2 Vec4 normalize (division by sqrt of the pairwise dot4)
 static F4 qnorm(F4 q) {
     float l = std::sqrt((q.x * q.x + q.z * q.z) + (q.y * q.y + q.w * q.w));
@@ -149,6 +153,11 @@ struct HostWorld {
     std::vector<F4> jt_fap, jt_far, jt_fbp, jt_fbr;
     std::vector<uint32_t> wj_mask;
     std::vector<float> wj_lam;  // 3 per joint
+    // D6 joints (bodies.rs:107-136): kind (0 ball, RP_JK_D6), stored hinge axis (RP_NONE: none),
+    // 18 axis words (rp_d6.h), and the carried rows (count word, 32 tags and impulses per joint)
+    std::vector<uint32_t> jt_kind, jt_hinge, d6_wn, d6_wtag;
+    std::vector<F4> jt_ax;
+    std::vector<float> d6_wlam;
     // kinematic players (world.rs:44-69): body index, radius, height, speed, vertical velocity,
     // grounded, input direction
     struct Player { uint32_t body; float radius, height, speed, vvel; uint32_t grounded; float input[3]; };
@@ -182,7 +191,7 @@ static void drop_warm_of(HostWorld &w, uint32_t b) {  // world.rs:818-820
     }
     w.pkey.resize(k); w.pjt.resize(k);
     for (uint32_t q = 0; q < w.nj(); q++)  // and its joints' warm rows (world.rs:822-826)
-        if (w.jt_a[q] == b || w.jt_b[q] == b) w.wj_mask[q] = 0;
+        if (w.jt_a[q] == b || w.jt_b[q] == b) { w.wj_mask[q] = 0; w.d6_wn[q] = 0; }
 }
 
 // ------------------------------------------------------------------ device scene (batch of worlds)
@@ -215,6 +224,7 @@ struct Slot {  // device-side snapshot of a scene
     uint32_t *wj_mask; float *wj_lam;
     float4 *pl_state, *pl_input; uint32_t *pl_grounded;
     uint64_t *pkey; float4 *pjt; uint32_t *poff;  // carried patch impulses (contact-model scenes)
+    uint32_t *d6_wn, *d6_wtag; float *d6_wlam;    // carried D6 rows (scenes with D6 joints)
     uint32_t cur; uint32_t nwarm, npatch;
     size_t cap_warm = 0;  // warm-cache entries the slot's buffers hold
     std::vector<uint64_t> ticks;
@@ -320,7 +330,10 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     uint32_t npk = 0;
     for (auto *w : ws) npk += (uint32_t)w->pkey.size();
     caps.warm = std::max(caps.warm, std::max(nwarm, npk) + 1024u);
-    d.cm_on = scene_cm(ws) ? 1u : 0u;
+    bool any_d6 = false;
+    for (auto *w : ws) for (uint32_t k : w->jt_kind) any_d6 |= k == RP_JK_D6;
+    d.d6_on = any_d6 ? 1u : 0u;
+    d.cm_on = (scene_cm(ws) || any_d6) ? 1u : 0u;  // D6 joints are solved by the contact-model tier
     d.nb = nb; d.nc = nc; d.nw = nw; d.nj = nj;
     d.cap_pairs = caps.pairs; d.cap_cands = caps.cands; d.cap_contacts = caps.contacts; d.cap_warm = caps.warm;
     DeviceBuffers &B = S->buf;
@@ -410,13 +423,18 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
               B.alloc(d.bk_wj_mask, NJ) && B.alloc(d.bk_wj_lam, 3 * NJ) &&
               B.alloc(d.jr_acc, 3 * NJ) && B.alloc(d.wj_mask, NJ) && B.alloc(d.wj_lam, 3 * NJ) && B.alloc(d.jadj_off, NB + 1) &&
               B.alloc(d.jadj, 2 * NJ) && B.alloc(d.w_joint_off, nw + 1);
+        ok &= B.alloc(d.jt_kind, NJ);
+        if (d.d6_on)
+            ok &= B.alloc(d.jt_hinge, NJ) && B.alloc(d.jt_ax, 18 * NJ) && B.alloc(d.d6_row, 4 * 32 * NJ) && B.alloc(d.d6_nrows, NJ) &&
+                  B.alloc(d.d6_racc, 32 * NJ) && B.alloc(d.d6_wn, NJ) && B.alloc(d.d6_wtag, 32 * NJ) && B.alloc(d.d6_wlam, 32 * NJ) &&
+                  B.alloc(d.bk_d6_wn, NJ) && B.alloc(d.bk_d6_wtag, 32 * NJ) && B.alloc(d.bk_d6_wlam, 32 * NJ) && B.alloc(d.d6_plam, 6 * NJ);
     }
     // hash buffers
     HashDev &h = S->h;
     uint64_t bytes = 0;
     for (auto *w : ws) {
         uint64_t n = w->nb(), cap = std::max(w->max_bodies, w->nb());
-        bytes += 48 + 56 * n + 8 + 24 + 8 + 12 * n + 8 + cap + 8 + 2 * cap + 1024 + 23ull * w->nj();
+        bytes += 48 + 56 * n + 8 + 24 + 8 + 12 * n + 8 + cap + 8 + 2 * cap + 1024 + (d.d6_on ? 168ull : 23ull) * w->nj();
     }
     bytes += 48ull * W;
     if (d.cm_on) bytes += 32ull * W;  // carried patch entries
@@ -469,7 +487,9 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         ep.push_back(w->epoch); rn.push_back(w->rng);
     }
     // joints (global body indices), their warm rows, each body's jointed partners, world offsets
-    std::vector<uint32_t> ja, jb, jmask, jadj_off(nb + 1, 0), jadj, jwoff;
+    std::vector<uint32_t> ja, jb, jmask, jadj_off(nb + 1, 0), jadj, jwoff, jkind, jhinge, jwn, jwtag;
+    std::vector<F4> jax;
+    std::vector<float> jwlam;
     std::vector<F4> jfap, jfar, jfbp, jfbr;
     std::vector<float> jlam;
     jwoff.push_back(0);
@@ -483,6 +503,9 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
         jfap.insert(jfap.end(), w->jt_fap.begin(), w->jt_fap.end()); jfar.insert(jfar.end(), w->jt_far.begin(), w->jt_far.end());
         jfbp.insert(jfbp.end(), w->jt_fbp.begin(), w->jt_fbp.end()); jfbr.insert(jfbr.end(), w->jt_fbr.begin(), w->jt_fbr.end());
         jmask.insert(jmask.end(), w->wj_mask.begin(), w->wj_mask.end()); jlam.insert(jlam.end(), w->wj_lam.begin(), w->wj_lam.end());
+        jkind.insert(jkind.end(), w->jt_kind.begin(), w->jt_kind.end()); jhinge.insert(jhinge.end(), w->jt_hinge.begin(), w->jt_hinge.end());
+        jax.insert(jax.end(), w->jt_ax.begin(), w->jt_ax.end()); jwn.insert(jwn.end(), w->d6_wn.begin(), w->d6_wn.end());
+        jwtag.insert(jwtag.end(), w->d6_wtag.begin(), w->d6_wtag.end()); jwlam.insert(jwlam.end(), w->d6_wlam.begin(), w->d6_wlam.end());
         jwoff.push_back((uint32_t)ja.size());
     }
     for (uint32_t b = 0; b < nb; b++) jadj_off[b + 1] += jadj_off[b];
@@ -519,7 +542,10 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
          up(d.jt_far, jfar.data(), jfar.size() * 16) && up(d.jt_fbp, jfbp.data(), jfbp.size() * 16) && up(d.jt_fbr, jfbr.data(), jfbr.size() * 16) &&
          up(d.wj_mask, jmask.data(), jmask.size() * 4) && up(d.wj_lam, jlam.data(), jlam.size() * 4) &&
          up(d.jadj_off, jadj_off.data(), jadj_off.size() * 4) && up(d.jadj, jadj.data(), jadj.size() * 4) &&
-         up(d.w_joint_off, jwoff.data(), jwoff.size() * 4);
+         up(d.w_joint_off, jwoff.data(), jwoff.size() * 4) && up(d.jt_kind, jkind.data(), jkind.size() * 4);
+    if (ok && d.d6_on)
+        ok = up(d.jt_hinge, jhinge.data(), jhinge.size() * 4) && up(d.jt_ax, jax.data(), jax.size() * 16) && up(d.d6_wn, jwn.data(), jwn.size() * 4) &&
+             up(d.d6_wtag, jwtag.data(), jwtag.size() * 4) && up(d.d6_wlam, jwlam.data(), jwlam.size() * 4);
     {
         std::vector<uint32_t> poff{0}, pbody, pgnd, pof(nb + 1, RP_NONE);
         std::vector<F4> pst, pin;
@@ -568,6 +594,9 @@ static bool pull_world(Scene &S, uint32_t wi, HostWorld &w) {
         uint32_t joff[2];
         ok = dn(joff, d.w_joint_off + wi, 8) && hip_ok(hipStreamSynchronize(S.st), "download sync") &&
              dn(w.wj_mask.data(), d.wj_mask + joff[0], w.nj() * 4) && dn(w.wj_lam.data(), d.wj_lam + 3 * (size_t)joff[0], 3 * (size_t)w.nj() * 4);
+        if (ok && d.d6_on)
+            ok = dn(w.d6_wn.data(), d.d6_wn + joff[0], w.nj() * 4) && dn(w.d6_wtag.data(), d.d6_wtag + 32 * (size_t)joff[0], 32 * (size_t)w.nj() * 4) &&
+                 dn(w.d6_wlam.data(), d.d6_wlam + 32 * (size_t)joff[0], 32 * (size_t)w.nj() * 4);
     }
     if (ok && d.cm_on) {
         uint32_t po[2];
@@ -1333,6 +1362,28 @@ RPhysResult rphys_world_get_step_hash(const RPhysWorld *cworld, RPhysHash32 *out
     return RPHYS_OK;
     GUARD_END
 }
+RPhysResult rphys_debug_get_hash_input(const RPhysWorld *cworld, uint8_t *out, uint64_t cap, uint64_t *out_len) {
+    GUARD_BEGIN
+    if (!cworld || !out_len) { set_err("rphys_debug_get_hash_input: null arg"); return RPHYS_ERR_NULL; }
+    *out_len = 0;
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (world->need_build) { if (!pull(world) || !push(world)) return RPHYS_ERR_INTERNAL; }
+    if (!world->scene && !push(world)) return RPHYS_ERR_INTERNAL;
+    Scene &S = *world->scene;
+    rp_launch_hash(S.d, S.h, S.cur, S.max_extra, S.st);
+    uint32_t chunk = 0, len = 0;
+    if (!hip_ok(hipMemcpyAsync(&chunk, S.h.w_chunk, 4, hipMemcpyDeviceToHost, S.st), "hash input") ||
+        !hip_ok(hipMemcpyAsync(&len, S.h.w_len, 4, hipMemcpyDeviceToHost, S.st), "hash input") || !hip_ok(hipStreamSynchronize(S.st), "hash input"))
+        return RPHYS_ERR_INTERNAL;
+    *out_len = len;
+    if (out && cap && len) {
+        const size_t n = std::min<uint64_t>(cap, len);
+        if (!hip_ok(hipMemcpy(out, S.h.stream + (size_t)chunk * 1024u, n, hipMemcpyDeviceToHost), "hash input")) return RPHYS_ERR_INTERNAL;
+    }
+    return RPHYS_OK;
+    GUARD_END
+}
 RPhysResult rphys_world_step_and_hash(RPhysWorld *world, float dt, RPhysHash32 *out_hash) {
     GUARD_BEGIN
     if (!out_hash) { set_err("rphys_world_step_and_hash: out_hash is null"); return RPHYS_ERR_NULL; }
@@ -1680,6 +1731,120 @@ RPhysResult rphys_world_add_ball_joint(RPhysWorld *world, uint32_t body_a, uint3
     w.jt_fap.push_back(f4(v[0], v[1], v[2])); w.jt_far.push_back(f4(v[3], v[4], v[5], v[6]));
     w.jt_fbp.push_back(f4(v[7], v[8], v[9])); w.jt_fbr.push_back(f4(v[10], v[11], v[12], v[13]));
     w.wj_mask.push_back(0); w.wj_lam.insert(w.wj_lam.end(), 3, 0.0f);
+    w.jt_kind.push_back(0); w.jt_hinge.push_back(RP_NONE); w.jt_ax.insert(w.jt_ax.end(), 18, f4(0, 0, 0));
+    w.d6_wn.push_back(0); w.d6_wtag.insert(w.d6_wtag.end(), 32, 0u); w.d6_wlam.insert(w.d6_wlam.end(), 32, 0.0f);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+// bodies.rs:321-345 add_d6 / add_hinge (world.rs:719-720): a ball joint's frames and checks, then
+// the constrained axes of t_mask / r_mask and the stored hinge axis
+RPhysResult rphys_world_add_d6_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a, const RPhysIsometry *frame_b,
+                                     uint32_t t_mask, uint32_t r_mask, int32_t hinge_axis, uint32_t *out_joint_id) {
+    GUARD_BEGIN
+    if (out_joint_id) *out_joint_id = 0xFFFFFFFFu;
+    if ((t_mask | r_mask) > 7u) return bad_arg("rphys_world_add_d6_joint", "axis masks hold bits 0..2");
+    if (hinge_axis < -1 || hinge_axis > 2) return bad_arg("rphys_world_add_d6_joint", "hinge axis must be -1 (none), 0, 1 or 2");
+    RPhysResult r = rphys_world_add_ball_joint(world, body_a, body_b, frame_a, frame_b, out_joint_id);
+    if (r != RPHYS_OK) return r;
+    std::lock_guard<std::mutex> lk(world->mu);
+    HostWorld &w = world->w;
+    const uint32_t q = *out_joint_id;
+    w.jt_kind[q] = RP_JK_D6;
+    w.jt_hinge[q] = hinge_axis < 0 ? RP_NONE : (uint32_t)hinge_axis;
+    for (uint32_t k = 0; k < 6; k++) {
+        const uint32_t en = ((k < 3 ? t_mask >> k : r_mask >> (k - 3)) & 1u) ? D6F_ENABLED_BIT : 0u;
+        w.jt_ax[18 * q + 3 * k].w = u2f(en);
+    }
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+RPhysResult rphys_world_add_hinge_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a, const RPhysIsometry *frame_b,
+                                        uint32_t axis, uint32_t *out_joint_id) {
+    if (out_joint_id) *out_joint_id = 0xFFFFFFFFu;
+    if (axis > 2) { set_err("rphys_world_add_hinge_joint: axis must be 0, 1 or 2"); return RPHYS_ERR_BAD_ARG; }
+    return rphys_world_add_d6_joint(world, body_a, body_b, frame_a, frame_b, 7u, 7u & ~(1u << axis), (int32_t)axis, out_joint_id);
+}
+// The axis words of D6 joint `joint` (ERR_INVALID_ID for an unknown joint, ERR_BAD_ARG for a ball
+// joint or an axis above 2); the world's lock is held.
+static RPhysResult d6_axis_of(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, const char *fn, F4 *&ax) {
+    HostWorld &w = world->w;
+    if (joint >= w.nj()) { set_err(std::string(fn) + ": unknown joint"); return RPHYS_ERR_INVALID_ID; }
+    if (w.jt_kind[joint] != RP_JK_D6) return bad_arg(fn, "not a D6 joint");
+    if (axis > 2) return bad_arg(fn, "axis must be 0, 1 or 2");
+    ax = &w.jt_ax[18 * joint + 3 * (angular ? 3 + axis : axis)];
+    return RPHYS_OK;
+}
+// a limit or drive on a rotational axis other than the stored hinge axis forgets it (bodies.rs:878-880)
+static void d6_leave_hinge(HostWorld &w, uint32_t joint, RPhysBool angular, uint32_t axis) {
+    if (angular && w.jt_hinge[joint] != RP_NONE && w.jt_hinge[joint] != axis) w.jt_hinge[joint] = RP_NONE;
+}
+// bodies.rs:361-390 set_d6_limit_t / set_d6_limit_r: on = 0 removes the limit; a limit enables its axis
+RPhysResult rphys_world_set_joint_limit(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, RPhysBool on, float lo, float hi) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_joint_limit: world is null"); return RPHYS_ERR_NULL; }
+    if (on && !(hfin(lo) && hfin(hi))) return bad_arg("rphys_world_set_joint_limit", "non-finite limit");
+    std::lock_guard<std::mutex> lk(world->mu);
+    F4 *ax;
+    RPhysResult r = d6_axis_of(world, joint, angular, axis, "rphys_world_set_joint_limit", ax);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    uint32_t f = f2u(ax[0].w);
+    f = on ? (f | D6F_LIMIT_BIT | D6F_ENABLED_BIT) : (f & ~D6F_LIMIT_BIT);
+    ax[0].w = u2f(f);
+    ax[2] = f4(lo, hi, 0.0f);
+    if (on) d6_leave_hinge(world->w, joint, angular, axis);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+// set_d6_drive_t / set_d6_drive_r: on = 0 removes the drive; a drive enables its axis
+RPhysResult rphys_world_set_joint_drive(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, RPhysBool on, float target, float kp,
+                                        float kd, float max_impulse, RPhysBool velocity_mode) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_joint_drive: world is null"); return RPHYS_ERR_NULL; }
+    if (on && !(hfin(target) && hfin(kp) && hfin(kd) && hfin(max_impulse))) return bad_arg("rphys_world_set_joint_drive", "non-finite drive");
+    std::lock_guard<std::mutex> lk(world->mu);
+    F4 *ax;
+    RPhysResult r = d6_axis_of(world, joint, angular, axis, "rphys_world_set_joint_drive", ax);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    uint32_t f = f2u(ax[0].w) & ~(D6F_DRIVE_BIT | D6F_VEL_BIT);
+    if (on) f |= D6F_DRIVE_BIT | D6F_ENABLED_BIT | (velocity_mode ? D6F_VEL_BIT : 0u);
+    ax[0].w = u2f(f);
+    ax[1] = f4(target, kp, kd, max_impulse);
+    if (on) d6_leave_hinge(world->w, joint, angular, axis);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+// set_d6_damping (bodies.rs:370-378): negative or non-finite values become 0
+RPhysResult rphys_world_set_joint_damping(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, float damping, float friction) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_joint_damping: world is null"); return RPHYS_ERR_NULL; }
+    std::lock_guard<std::mutex> lk(world->mu);
+    F4 *ax;
+    RPhysResult r = d6_axis_of(world, joint, angular, axis, "rphys_world_set_joint_damping", ax);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    ax[0].y = std::fmax(hfin(damping) ? damping : 0.0f, 0.0f);
+    ax[0].z = std::fmax(hfin(friction) ? friction : 0.0f, 0.0f);
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+// D6Axis::compliance of one axis (0 = rigid; m/N or rad/(N m))
+RPhysResult rphys_world_set_joint_compliance(RPhysWorld *world, uint32_t joint, RPhysBool angular, uint32_t axis, float compliance) {
+    GUARD_BEGIN
+    if (!world) { set_err("rphys_world_set_joint_compliance: world is null"); return RPHYS_ERR_NULL; }
+    if (!hfin(compliance)) return bad_arg("rphys_world_set_joint_compliance", "non-finite compliance");
+    std::lock_guard<std::mutex> lk(world->mu);
+    F4 *ax;
+    RPhysResult r = d6_axis_of(world, joint, angular, axis, "rphys_world_set_joint_compliance", ax);
+    if (r != RPHYS_OK) return r;
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    ax[0].x = compliance;
     host_changed(world);
     return RPHYS_OK;
     GUARD_END
@@ -1907,6 +2072,33 @@ RPhysResult rphys_debug_export_state(const RPhysWorld *cworld, float *pos3, floa
     GUARD_END
 }
 
+RPhysResult rphys_debug_get_joint_rows(const RPhysWorld *cworld, uint32_t joint, uint8_t *tags, float *impulses, uint32_t cap, uint32_t *out_n) {
+    GUARD_BEGIN
+    if (!cworld || !out_n) { set_err("rphys_debug_get_joint_rows: null arg"); return RPHYS_ERR_NULL; }
+    *out_n = 0;
+    RPhysWorld *world = const_cast<RPhysWorld *>(cworld);
+    std::lock_guard<std::mutex> lk(world->mu);
+    if (joint >= world->w.nj()) { set_err("rphys_debug_get_joint_rows: unknown joint"); return RPHYS_ERR_INVALID_ID; }
+    if (!pull(world)) return RPHYS_ERR_INTERNAL;
+    const HostWorld &w = world->w;
+    uint32_t n = 0;
+    if (w.jt_kind[joint] == RP_JK_D6) {
+        const uint32_t wn = w.d6_wn[joint];
+        if (wn >> 31)
+            for (uint32_t r = 0; r < (wn & 0xFFu); r++, n++)
+                if (n < cap) { if (tags) tags[n] = (uint8_t)w.d6_wtag[32 * joint + r]; if (impulses) impulses[n] = w.d6_wlam[32 * joint + r]; }
+    } else if (w.wj_mask[joint] >> 31) {  // a ball joint: its locked axes' rows, tags axis << 3
+        for (uint32_t i = 0; i < 3; i++) {
+            if (!((w.wj_mask[joint] >> i) & 1u)) continue;
+            if (n < cap) { if (tags) tags[n] = (uint8_t)(i << 3); if (impulses) impulses[n] = w.wj_lam[3 * joint + i]; }
+            n++;
+        }
+    }
+    *out_n = n;
+    return RPHYS_OK;
+    GUARD_END
+}
+
 // ------------------------------------------------------------------ snapshots (lib.rs:1696-1800)
 struct RPhysWorldSnapshot {
     uint64_t tick;
@@ -1920,6 +2112,8 @@ struct RPhysWorldSnapshot {
     std::vector<HostWorld::Player> players;
     std::vector<uint64_t> pkey;
     std::vector<F4> pjt;
+    std::vector<uint32_t> d6_wn, d6_wtag;
+    std::vector<float> d6_wlam;
 };
 RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
     clear_err();
@@ -1939,6 +2133,7 @@ RPhysWorldSnapshot *rphys_world_snapshot_create(const RPhysWorld *cworld) {
         s->wj_mask = w.wj_mask; s->wj_lam = w.wj_lam;
         s->players = w.players;
         s->pkey = w.pkey; s->pjt = w.pjt;
+        s->d6_wn = w.d6_wn; s->d6_wtag = w.d6_wtag; s->d6_wlam = w.d6_wlam;
         return s;
     } catch (...) {
         set_err("panic in rphys_world_snapshot_create");
@@ -1979,7 +2174,9 @@ RPhysResult rphys_world_snapshot_restore(RPhysWorld *world, const RPhysWorldSnap
         const bool had = q < s->wj_mask.size();
         w.wj_mask[q] = had ? s->wj_mask[q] : 0u;
         for (int i = 0; i < 3; i++) w.wj_lam[3 * q + i] = had ? s->wj_lam[3 * q + i] : 0.0f;
-        if (!alive(w.jt_a[q]) || !alive(w.jt_b[q])) w.wj_mask[q] = 0;
+        w.d6_wn[q] = had ? s->d6_wn[q] : 0u;
+        for (int i = 0; i < 32; i++) { w.d6_wtag[32 * q + i] = had ? s->d6_wtag[32 * q + i] : 0u; w.d6_wlam[32 * q + i] = had ? s->d6_wlam[32 * q + i] : 0.0f; }
+        if (!alive(w.jt_a[q]) || !alive(w.jt_b[q])) { w.wj_mask[q] = 0; w.d6_wn[q] = 0; }
     }
     host_changed(world);
     return RPHYS_OK;
@@ -2106,7 +2303,9 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
                   sl->buf.alloc(sl->wkey, W) && sl->buf.alloc(sl->wla, W) && sl->buf.alloc(sl->wjn, W) && sl->buf.alloc(sl->wjt, W) &&
                   sl->buf.alloc(sl->woff, d.nw + 1) && sl->buf.alloc(sl->wj_mask, d.nj + 1) && sl->buf.alloc(sl->wj_lam, 3 * ((size_t)d.nj + 1)) &&
                   sl->buf.alloc(sl->pl_state, d.np + 1) && sl->buf.alloc(sl->pl_input, d.np + 1) && sl->buf.alloc(sl->pl_grounded, d.np + 1) &&
-                  sl->buf.alloc(sl->pkey, d.cm_on ? W : 1) && sl->buf.alloc(sl->pjt, d.cm_on ? W : 1) && sl->buf.alloc(sl->poff, d.nw + 1);
+                  sl->buf.alloc(sl->pkey, d.cm_on ? W : 1) && sl->buf.alloc(sl->pjt, d.cm_on ? W : 1) && sl->buf.alloc(sl->poff, d.nw + 1) &&
+                  sl->buf.alloc(sl->d6_wn, d.d6_on ? d.nj : 1) && sl->buf.alloc(sl->d6_wtag, d.d6_on ? 32 * (size_t)d.nj : 1) &&
+                  sl->buf.alloc(sl->d6_wlam, d.d6_on ? 32 * (size_t)d.nj : 1);
         if (!ok) { set_err("rphys_batch_snapshot: device allocation failed"); return RPHYS_ERR_INTERNAL; }
         S.slots[slot] = std::move(sl);
     }
@@ -2123,6 +2322,8 @@ static RPhysResult batch_snapshot(RPhysBatch *b, uint32_t slot) {
     if (ok && d.cm_on)
         ok = cp(sl.pkey, d.wp_key[S.cur], 8 * (size_t)npatch) && cp(sl.pjt, d.wp_jt[S.cur], 16 * (size_t)npatch) &&
              cp(sl.poff, d.wpc_off[S.cur], 4 * ((size_t)d.nw + 1));
+    if (ok && d.d6_on)
+        ok = cp(sl.d6_wn, d.d6_wn, 4 * (size_t)d.nj) && cp(sl.d6_wtag, d.d6_wtag, 128 * (size_t)d.nj) && cp(sl.d6_wlam, d.d6_wlam, 128 * (size_t)d.nj);
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "snapshot")) return RPHYS_ERR_INTERNAL;
     sl.cur = S.cur; sl.nwarm = nwarm; sl.npatch = npatch; sl.valid = true;
     sl.ticks.clear();
@@ -2156,6 +2357,8 @@ static RPhysResult batch_restore(RPhysBatch *b, uint32_t slot) {
     if (ok && d.cm_on)
         ok = cp(d.wp_key[sl.cur], sl.pkey, 8 * (size_t)sl.npatch) && cp(d.wp_jt[sl.cur], sl.pjt, 16 * (size_t)sl.npatch) &&
              cp(d.wpc_off[sl.cur], sl.poff, 4 * ((size_t)d.nw + 1));
+    if (ok && d.d6_on)
+        ok = cp(d.d6_wn, sl.d6_wn, 4 * (size_t)d.nj) && cp(d.d6_wtag, sl.d6_wtag, 128 * (size_t)d.nj) && cp(d.d6_wlam, sl.d6_wlam, 128 * (size_t)d.nj);
     if (!ok || !hip_ok(hipStreamSynchronize(S.st), "restore")) return RPHYS_ERR_INTERNAL;
     S.cur = sl.cur;
     for (size_t i = 0; i < b->worlds.size() && i < sl.ticks.size(); i++) b->worlds[i].tick = sl.ticks[i];

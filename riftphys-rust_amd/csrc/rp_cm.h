@@ -143,13 +143,13 @@ __global__ void k_patch_prep(Dev d) {
         d.pt_soft[p] = (cm_flags(d, f) & CM_SOFT) ? 1u : 0u;
     }
 }
-// Islands with a compliant row or a patch (tier 4): flagged from their sorted entries.
+// Islands with a compliant row, a patch or a D6 joint (tier 4): flagged from their sorted entries.
 __global__ void k_isl_cm(Dev d, const uint32_t *keys) {
     if (aborted(d)) return;
     const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     GRID_STRIDE(r, nislc) {
         const uint32_t e = d.isl_v2[r];
-        const bool cm = (e & RP_PBIT) || (!(e & RP_JBIT) && (cm_flags(d, e) & CM_SOFT));
+        const bool cm = (e & RP_PBIT) || (!(e & RP_JBIT) && (cm_flags(d, e) & CM_SOFT)) || ((e & RP_JBIT) && d6_is(d, e & ~RP_JBIT));
         if (!cm) continue;
         const bool start = r == 0 || keys[r - 1] != keys[r];
         const uint32_t isl = start ? d.isl_flag[r] : d.isl_flag[r] - 1u;  // exclusive scan of the starts
@@ -281,11 +281,13 @@ static __device__ uint32_t cm_contact_pgs(const Dev &d, uint32_t c) {
 }
 static __device__ __forceinline__ void cm_warm(const Dev &d, const TickParams &tp, uint32_t e) {
     if (e & RP_PBIT) patch_warm(d, tp, e & ~RP_PBIT);
+    else if ((e & RP_JBIT) && d6_is(d, e & ~RP_JBIT)) d6_warm(d, e & ~RP_JBIT);
     else warm_row(d, tp, e);
 }
 static __device__ __forceinline__ void cm_pgs(const Dev &d, uint32_t e, ResidSink sink) {
     uint32_t r;
-    if (e & RP_JBIT) r = joint_pgs(d, e & ~RP_JBIT, sink.on);
+    if ((e & RP_JBIT) && d6_is(d, e & ~RP_JBIT)) r = d6_pgs(d, e & ~RP_JBIT, sink.on);
+    else if (e & RP_JBIT) r = joint_pgs(d, e & ~RP_JBIT, sink.on);
     else if (e & RP_PBIT) r = patch_pgs(d, e & ~RP_PBIT);
     else r = cm_contact_pgs(d, e);
     if (sink.on) __hip_atomic_fetch_max(sink.p, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

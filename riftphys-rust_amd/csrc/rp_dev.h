@@ -13,9 +13,11 @@
 #define RP_MAX_WORLD_COLLIDERS 16384u  // LDS sort width of the broadphase's one-workgroup sort (larger worlds: radix sorts)
 
 enum { BF_DYN = 1u, BF_ALIVE = 2u, BF_SLEEP_OK = 4u };
-enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u, FL_JOINTED = 8u, FL_JW = 16u };
+enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u, FL_JOINTED = 8u, FL_JW = 16u, FL_DRIVEN = 32u };
 // FL_JOINTED: in a joint (step.rs:164-173); FL_JW: written by a joint row this tick (a dynamic
-// body of an active joint, asleep or not)
+// body of an active joint, asleep or not); FL_DRIVEN: in a D6 joint with a rotational velocity
+// drive (its angular velocity is the velocity solve's, step.rs:353-365)
+#define RP_JK_D6 2u  // jt_kind of a D6 joint (0: ball)
 
 // Island-list entries are contact indices, or RP_JBIT | joint index for a ball joint's rows, or
 // RP_PBIT | patch index for a friction patch (scenes with a contact-model pair: contact indices
@@ -192,6 +194,13 @@ struct Dev {
     float *wj_lam;
     uint32_t *jadj_off, *jadj;
     uint32_t *w_joint_off;
+    // D6 joints (allocated when the scene has one, d6_on; layout in rp_d6.h): kind and stored hinge
+    // axis per joint, 18 axis words per joint, 32 rows per joint and their count and accumulators,
+    // the carried rows (count word, tags, impulses) and their backups, 6 position multipliers
+    uint32_t d6_on;
+    uint32_t *jt_kind, *jt_hinge, *d6_nrows, *d6_wn, *d6_wtag, *bk_d6_wn, *bk_d6_wtag;
+    float4 *jt_ax, *d6_row;
+    float *d6_racc, *d6_wlam, *bk_d6_wlam, *d6_plam;
     // W6 restitution (solver.rs:569-588, 995-1245), allocated when a pair of the scene's materials
     // has restitution > 0 (w6_on): per world, a contact with a restitution bias ([w]) and
     // pseudo-velocities ([nw + w]); per contact with a bias (written by the prep), (bias, start gap,

@@ -90,8 +90,23 @@ static __device__ __forceinline__ uint32_t world_len(const Dev &d, const HashDev
 // A stored joint's entry (world.rs:896-900): index u32, row count u32, then (tag u8, impulse f32)
 // per row, rows in axis order with tags axis << 3.
 static __device__ __forceinline__ uint32_t joint_entry_bytes(uint32_t m) { return (m >> 31) ? 8u + 5u * (uint32_t)__popc(m & 7u) : 0u; }
+// whether joint q has stored rows, and how many (a ball joint's mask bits, a D6 joint's count word).
+// Both words are loaded and the values selected: a branch between the two loads has been compiled
+// into one load through a base pointer chosen for the whole wavefront.
+static __device__ __forceinline__ bool joint_stored(const Dev &d, uint32_t q, uint32_t &n) {
+    const uint32_t m = d.wj_mask[q];
+    const uint32_t wn = d.d6_on ? d.d6_wn[q] : 0u;
+    const bool d6 = d.d6_on && d.jt_kind[q] == RP_JK_D6;
+    n = d6 ? (wn & 0xFFu) : (uint32_t)__popc(m & 7u);
+    return ((d6 ? wn : m) >> 31) != 0u;
+}
 
-__global__ void k_hash_jbytes(Dev d, HashDev h) { GS(q, d.nj) h.j_off[q] = joint_entry_bytes(d.wj_mask[q]); }
+__global__ void k_hash_jbytes(Dev d, HashDev h) {
+    GS(q, d.nj) {
+        uint32_t n;
+        h.j_off[q] = joint_stored(d, q, n) ? 8u + 5u * n : 0u;
+    }
+}
 
 __global__ void k_hash_sizes(Dev d, HashDev h, uint32_t cur) {
     GS(w, d.nw) {
@@ -123,8 +138,8 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
             uint32_t z = o + 8u + 20u * K + 28u * E;
             const uint32_t JB = h.w_jbytes[w], P = world_patches_n(d, w, cur), PB = 32u * P;
             put_u64(S + z, P);  // the patch entries themselves are written per entry below
-            uint32_t nstored = 0;  // the entries themselves are written per joint below
-            for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) nstored += d.wj_mask[q] >> 31;
+            uint32_t nstored = 0;  // joints with an entry (8 bytes or more); the entries are written per joint below
+            for (uint32_t q = d.w_joint_off[w]; q < d.w_joint_off[w + 1]; q++) nstored += h.j_off[q + 1] != h.j_off[q] ? 1u : 0u;
             put_u64(S + z + 8 + PB, nstored);
             put_u64(S + z + 16 + PB + JB, 0);
             put_u64(S + z + 24 + PB + JB, n);
@@ -152,15 +167,24 @@ __global__ void k_hash_write(Dev d, HashDev h, uint32_t cur) {
     // warm-joint entries, one lane per joint, at the joint's offset within its world's entries
     GS(q, d.nj) {
         const uint32_t m = d.wj_mask[q];
-        if (!(m >> 31)) continue;
+        uint32_t nrows;
+        if (!joint_stored(d, q, nrows)) continue;
         const uint32_t w = d.bworld[d.jt_a[q]], q0 = d.w_joint_off[w];
         const uint32_t n = d.w_body_off[w + 1] - d.w_body_off[w];
         const uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
         const uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
         uint8_t *jp = h.stream + (size_t)h.w_chunk[w] * 1024u + 48u + 56u * n + 8u + 20u * K + 28u * E + 16u + 32u * world_patches_n(d, w, cur) +
                       (h.j_off[q] - h.j_off[q0]);
-        put_u32(jp, q - q0); put_u32(jp + 4, (uint32_t)__popc(m & 7u));
+        put_u32(jp, q - q0); put_u32(jp + 4, nrows);
         jp += 8;
+        if (d.d6_on && d.jt_kind[q] == RP_JK_D6) {
+            for (uint32_t r = 0; r < nrows; r++) {
+                jp[0] = (uint8_t)d.d6_wtag[32 * q + r];
+                put_u32(jp + 1, canon(d.d6_wlam[32 * q + r]));
+                jp += 5;
+            }
+            continue;
+        }
         for (uint32_t i = 0; i < 3; i++) {
             if (!((m >> i) & 1u)) continue;
             jp[0] = (uint8_t)(i << 3);
@@ -222,7 +246,7 @@ __global__ void k_hash_tail(Dev d, HashDev h, uint32_t cur, uint32_t max_extra) 
         uint8_t *S = h.stream + (size_t)h.w_chunk[w] * 1024u;
         uint32_t e0 = d.wc_off[cur][w], e1 = d.wc_off[cur][w + 1];
         uint32_t K = h.keyflag[e1] - h.keyflag[e0], E = e1 - e0;
-        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E + h.w_jbytes[w];
+        uint32_t z = 48u + 56u * n + 8u + 20u * K + 28u * E + 32u * world_patches_n(d, w, cur) + h.w_jbytes[w];
         uint32_t ao = z + 32u + 12u * n;
         uint32_t i = n + k;
         S[ao + 8u + i] = 1;

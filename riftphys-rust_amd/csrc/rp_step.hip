@@ -88,6 +88,10 @@ __global__ void k_bodies_pre(Dev d, TickParams tp) {
     GRID_STRIDE(q, d.nj) {
         d.bk_wj_mask[q] = d.wj_mask[q];
         for (uint32_t i = 0; i < 3; i++) d.bk_wj_lam[3 * q + i] = d.wj_lam[3 * q + i];
+        if (d.d6_on) {
+            d.bk_d6_wn[q] = d.d6_wn[q];
+            for (uint32_t r = 0; r < 32; r++) { d.bk_d6_wtag[32 * q + r] = d.d6_wtag[32 * q + r]; d.bk_d6_wlam[32 * q + r] = d.d6_wlam[32 * q + r]; }
+        }
     }
 }
 
@@ -676,7 +680,12 @@ __global__ void k_prep(Dev d, TickParams tp) {
 // rows from the tick-start poses (solver.rs:336-424: axis i of frame A, lever arms, 1 / K where
 // K = wa + wb + (ra x u) Ia (ra x u) + (rb x u) Ib (rb x u) > 1e-9), the world's solve flag, the
 // written-by-a-joint flags, and the union of its two bodies when both are dynamic.
-__global__ void k_joint_prep(Dev d) {
+// D6 joints (rp_d6.h, included below with the position solve)
+static __device__ __forceinline__ void d6_store(const Dev &d, uint32_t q);
+static __device__ __forceinline__ bool d6_is(const Dev &d, uint32_t q);
+static __device__ __forceinline__ bool d6_motor(const Dev &d, uint32_t q);
+static __device__ uint32_t d6_rows(const Dev &d, uint32_t q, float dt);
+__global__ void k_joint_prep(Dev d, TickParams tp) {
     if (aborted(d)) return;
     GRID_STRIDE(q, d.nj) {
         const uint32_t a = d.jt_a[q], b = d.jt_b[q];
@@ -684,6 +693,23 @@ __global__ void k_joint_prep(Dev d) {
         if ((d.flags[b] & (FL_CONSTRAINED | FL_JOINTED)) != (FL_CONSTRAINED | FL_JOINTED)) atomicOr(&d.flags[b], FL_CONSTRAINED | FL_JOINTED);
         const float wa = d.inv_mass[a], wb = d.inv_mass[b];
         uint32_t m = 0;
+        if (d6_is(d, q)) {  // a D6 joint: its rows (rp_d6.h); velocity motors mark their bodies
+            if (d6_motor(d, q)) { atomicOr(&d.flags[a], FL_DRIVEN); atomicOr(&d.flags[b], FL_DRIVEN); }
+            uint32_t n = 0;
+            if (wa > 0.0f || wb > 0.0f) {
+                const uint32_t w = d.bworld[a];
+                if (!d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
+                n = d6_rows(d, q, tp.dt);
+                if (n) {
+                    if (wa > 0.0f && !(d.flags[a] & FL_JW)) atomicOr(&d.flags[a], FL_JW);
+                    if (wb > 0.0f && !(d.flags[b] & FL_JW)) atomicOr(&d.flags[b], FL_JW);
+                    if (wa > 0.0f && wb > 0.0f) uf_union(d.root, a, b);
+                }
+            }
+            d.d6_nrows[q] = n;
+            d.jr_mask[q] = n ? 1u : 0u;
+            continue;
+        }
         if (wa > 0.0f || wb > 0.0f) {
             const uint32_t w = d.bworld[a];
             if (!d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
@@ -1403,6 +1429,7 @@ static __device__ uint32_t joint_pgs(const Dev &d, uint32_t q, bool resid) {
 }
 // joint q's rows become its warm rows (a joint without rows keeps its old ones)
 static __device__ __forceinline__ void joint_store(const Dev &d, uint32_t q) {
+    if (d6_is(d, q)) { d6_store(d, q); return; }
     const uint32_t m = d.jr_mask[q];
     if (!m) return;
     d.wj_mask[q] = 0x80000000u | m;
@@ -1425,13 +1452,15 @@ static __device__ __forceinline__ void joint_orient_delta(const Dev &d, uint32_t
     const Q4 dq = qnormalize(mkq(x.x * 0.5f, x.y * 0.5f, x.z * 0.5f, 1.0f));
     d.rot[b] = SQ(fin_q(qnormalize(qmul(dq, LQ(d.rot[b])))));
 }
+#include "rp_d6.h"
 // One position pass over joint q (solve_step_with, bodies.rs:422-460, one iteration): skipped when
 // both bodies are static or asleep; SHAKE directions from the step's start poses (the poses before
-// integration) unless a body turned by 0.5 rad or more since.
-static __device__ void joint_pos(const Dev &d, uint32_t q) {
+// integration) unless a body turned by 0.5 rad or more since.  A D6 joint runs solve_d6 (rp_d6.h).
+static __device__ void joint_pos(const Dev &d, uint32_t q, float dt) {
     const uint32_t a = d.jt_a[q], b = d.jt_b[q];
     const bool fa = !((d.bflags[a] & BF_DYN) && d.awake[a]), fb = !((d.bflags[b] & BF_DYN) && d.awake[b]);
     if (fa && fb) return;
+    if (d6_is(d, q)) { d6_pos(d, q, dt); return; }
     const float wa = jfin(d.inv_mass[a]), wb = jfin(d.inv_mass[b]);
     const Q4 sa = LQ(d.orot[a]), sb = LQ(d.orot[b]);
     const float cmin = __uint_as_float(RP_SHAKE_COS_BITS);
@@ -2716,8 +2745,8 @@ static __device__ __forceinline__ PRow fetch_prow(const Dev &d, uint32_t slot) {
     x.ilb.c0 = mk3(f6.w, f7.x, f7.y); x.ilb.c1 = mk3(f7.z, f7.w, f8.x); x.ilb.c2 = mk3(f8.y, f8.z, f8.w);
     return x;
 }
-static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 up) {
-    if (x.a & RP_JBIT) { if (!shock) joint_pos(d, x.a & ~RP_JBIT); return; }
+static __device__ void project_row(const Dev &d, const PRow &x, bool shock, V3 up, float dt) {
+    if (x.a & RP_JBIT) { if (!shock) joint_pos(d, x.a & ~RP_JBIT, dt); return; }
     if (x.a & RP_PBIT) return;
     const uint32_t a = x.a, b = x.b;
     const bool hb = b != RP_NONE;
@@ -2771,7 +2800,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj(Dev d, TickP
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) store_prow(d, s + j, load_prow(d, d.bs_c[s + j]));
         __syncthreads();
         for (uint32_t it = 0; it < tp.ip; it++)
-            run_sweep(I, sb, [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
+            run_sweep(I, sb, [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up, tp.dt); });
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ);
     ISLAND_LOOP_END
 }
@@ -2784,7 +2813,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_tail(Dev d, 
         PhaseClock pc;
         const uint32_t k = min(d.isl_w[isl], kl);
         run_sweep_from(I, Slots{d.bs_pa, d.bs_pb, d.bs_c}, k ? d.lvl_off[s + k] : 0u,
-                       [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up); });
+                       [&](uint32_t, uint32_t slot) { return fetch_prow(d, slot); }, [&](uint32_t, const PRow &x) { project_row(d, x, false, up, tp.dt); });
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ);
     ISLAND_LOOP_END
 }
@@ -2797,7 +2826,7 @@ __global__ void __launch_bounds__(256) k_store_prows(Dev d) {
 __global__ void __launch_bounds__(256) k_level_proj(Dev d, TickParams tp, uint32_t L) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
-    for_wide_level(d, d.wide_list, d.cnt[CT_WIDE_N], d.isl_w, d.lvl_off, L, [&](uint32_t slot) { project_row(d, fetch_prow(d, slot), false, up); });
+    for_wide_level(d, d.wide_list, d.cnt[CT_WIDE_N], d.isl_w, d.lvl_off, L, [&](uint32_t slot) { project_row(d, fetch_prow(d, slot), false, up, tp.dt); });
 }
 // Projection shock pass (step.rs:299-301, 447-497 with shock = true) in shock order.
 template <int TIER>
@@ -2810,7 +2839,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_proj_shock(Dev d,
         const Slots ss{d.ss_pa, d.ss_pb, d.ss_c};
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ_SHOCK_ORDER);
         if (BIG<TIER> && klsh) continue;  // k_level_pshock + k_sweep_pshock_tail follow
-        run_sweep(I, ss, [&](uint32_t c, uint32_t) { return load_prow(d, c); }, [&](uint32_t, const PRow &x) { project_row(d, x, true, up); });
+        run_sweep(I, ss, [&](uint32_t c, uint32_t) { return load_prow(d, c); }, [&](uint32_t, const PRow &x) { project_row(d, x, true, up, tp.dt); });
         if constexpr (BIG<TIER>) pc.lap(d, RP_PH_PROJ_SHOCK);
     ISLAND_LOOP_END
 }
@@ -2834,14 +2863,14 @@ __global__ void __launch_bounds__(64) k_lane_proj(Dev d, TickParams tp, const ui
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     LANE_ISLANDS_BEGIN
         for (uint32_t it = 0; it < tp.ip; it++)
-            for (uint32_t j = 0; j < n; j++) project_row(d, load_prow(d, d.isl_v2[s + j]), false, up);
+            for (uint32_t j = 0; j < n; j++) project_row(d, load_prow(d, d.isl_v2[s + j]), false, up, tp.dt);
     }
 }
 __global__ void __launch_bounds__(64) k_lane_pshock(Dev d, TickParams tp, const uint32_t *order) {
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     LANE_ISLANDS_BEGIN
-        for (uint32_t j = 0; j < n; j++) project_row(d, load_prow(d, d.sh_ord[s + j]), true, up);
+        for (uint32_t j = 0; j < n; j++) project_row(d, load_prow(d, d.sh_ord[s + j]), true, up, tp.dt);
     }
 }
 
@@ -2850,7 +2879,7 @@ __global__ void __launch_bounds__(256) k_level_pshock(Dev d, TickParams tp, uint
     if (aborted(d)) return;
     const V3 up = mk3(tp.upx, tp.upy, tp.upz);
     for_wide_level(d, d.big_list, d.cnt[CT_BIG_N], d.isl_wsh, d.lvl_off_sh, L,
-                   [&](uint32_t slot) { project_row(d, load_prow(d, d.ss_c[slot]), true, up); });
+                   [&](uint32_t slot) { project_row(d, load_prow(d, d.ss_c[slot]), true, up, tp.dt); });
 }
 template <int TIER>
 __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_pshock_tail(Dev d, TickParams tp, uint32_t klsh) {
@@ -2859,7 +2888,7 @@ __global__ void __launch_bounds__(Tier<TIER>::threads) k_sweep_pshock_tail(Dev d
     ISLAND_LOOP_BEGIN
         const uint32_t k = min(d.isl_wsh[isl], klsh);
         run_sweep_from(I, Slots{d.ss_pa, d.ss_pb, d.ss_c}, k ? d.lvl_off_sh[s + k] : 0u, [&](uint32_t c, uint32_t) { return load_prow(d, c); },
-                       [&](uint32_t, const PRow &x) { project_row(d, x, true, up); });
+                       [&](uint32_t, const PRow &x) { project_row(d, x, true, up, tp.dt); });
     ISLAND_LOOP_END
 }
 
@@ -2880,7 +2909,7 @@ __global__ void k_s6(Dev d, TickParams tp) {
                 np.y = d.opos[i].y;
                 set_pose(d, i, np, nq);
             }
-            if (!eqq(nq, pq)) {
+            if (!(d.flags[i] & FL_DRIVEN) && !eqq(nq, pq)) {
                 Q4 dq = qmul(nq, qconj(pq));
                 V3 da = mk3(dq.x, dq.y, dq.z) * 2.0f;
                 if (dq.w < 0.0f) da = -da;
@@ -2910,6 +2939,10 @@ __global__ void k_restore_backup(Dev d) {
     GRID_STRIDE(q, d.nj) {
         d.wj_mask[q] = d.bk_wj_mask[q];
         for (uint32_t i = 0; i < 3; i++) d.wj_lam[3 * q + i] = d.bk_wj_lam[3 * q + i];
+        if (d.d6_on) {
+            d.d6_wn[q] = d.bk_d6_wn[q];
+            for (uint32_t r = 0; r < 32; r++) { d.d6_wtag[32 * q + r] = d.bk_d6_wtag[32 * q + r]; d.d6_wlam[32 * q + r] = d.bk_d6_wlam[32 * q + r]; }
+        }
     }
     GRID_STRIDE(p, d.np) { d.pl_state[p] = d.bk_pl_state[p]; d.pl_grounded[p] = d.bk_pl_grounded[p]; }
 }
@@ -3108,7 +3141,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     mark(RP_ST_PREP);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nw, B)), dim3(B), 0, st, d.w_solve, d.nw);
     if (d.w6_on) hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(2 * d.nw, B)), dim3(B), 0, st, d.w_w6, 2 * d.nw);
-    if (d.nj) hipLaunchKernelGGL(k_joint_prep, dim3(grid_for(d.nj, B)), dim3(B), 0, st, d);
+    if (d.nj) hipLaunchKernelGGL(k_joint_prep, dim3(grid_for(d.nj, B)), dim3(B), 0, st, d, tp);
     hipLaunchKernelGGL(k_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d, tp);
     if (d.cm_on) hipLaunchKernelGGL(k_patch_prep, dim3(grid_for(d.cap_patch, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_island_keys, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
@@ -3316,6 +3349,7 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
     hipLaunchKernelGGL(k_integrate, dim3(grid_for(d.nb, B)), dim3(B), 0, st, d, tp);
     mark(RP_ST_POS_PREP);
     hipLaunchKernelGGL(k_posc_prep, dim3(grid_for(d.cap_contacts, B)), dim3(B), 0, st, d);
+    if (d.d6_on) hipLaunchKernelGGL(k_d6_begin, dim3(grid_for(6 * d.nj, B)), dim3(B), 0, st, d);
     auto proj0 = [&](hipStream_t s) {
         if (lp) hipLaunchKernelGGL(k_lane_proj, dim3(g_lane), dim3(64), 0, s, d, tp, lp);
         else hipLaunchKernelGGL(k_sweep_proj<0>, dim3(g0), dim3(Tier<0>::threads), 0, s, d, tp, 0u, NO);

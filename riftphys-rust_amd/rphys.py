@@ -177,6 +177,12 @@ EXPORTS = [
     ("rphys_world_set_pair_contact_model", U32, [P, U32, U32, F32, F32, U32]),
     ("rphys_world_set_compliant_damping_at_start", U32, [P, U32]),
     ("rphys_world_add_ball_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), PU32]),
+    ("rphys_world_add_d6_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), U32, U32, C.c_int32, PU32]),
+    ("rphys_world_add_hinge_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), U32, PU32]),
+    ("rphys_world_set_joint_limit", U32, [P, U32, U32, U32, U32, C.c_float, C.c_float]),
+    ("rphys_world_set_joint_drive", U32, [P, U32, U32, U32, U32, C.c_float, C.c_float, C.c_float, C.c_float, U32]),
+    ("rphys_world_set_joint_damping", U32, [P, U32, U32, U32, C.c_float, C.c_float]),
+    ("rphys_world_set_joint_compliance", U32, [P, U32, U32, U32, C.c_float]),
     ("rphys_world_num_joints", U32, [P]),
     ("rphys_world_set_solver_iterations", U32, [P, U32, U32]),
     ("rphys_world_set_gravity", U32, [P, Vec3]),
@@ -190,6 +196,8 @@ EXPORTS = [
     ("rphys_debug_copy_probe", U32, [U64, U32, C.POINTER(C.c_double)]),
     ("rphys_debug_get_contacts_ex", U32, [P, C.POINTER(ContactEx), U32, PU32]),
     ("rphys_debug_get_pairs", U32, [P, PU32, U32, PU32]),
+    ("rphys_debug_get_joint_rows", U32, [P, U32, C.POINTER(C.c_uint8), PF32, U32, PU32]),
+    ("rphys_debug_get_hash_input", U32, [P, C.POINTER(C.c_uint8), U64, C.POINTER(U64)]),
     ("rphys_debug_export_state", U32, [P, PF32, PF32, PF32, PF32, PF32, PU32, PU32, C.POINTER(C.c_uint64), PF32, PF32, PF32, U32, PU32]),
     ("rphys_batch_create", P, [C.POINTER(P), U32]),
     ("rphys_batch_destroy", None, [P]),
@@ -364,6 +372,16 @@ class World:
                 check(L.rphys_body_set_sleep_allowed(self.h, b, 0))
         for a, b, fa, fb in s.get("joints", []):  # scene joints name bodies by scene index
             self.add_ball_joint(ids[a], ids[b], fa, fb)
+        for j in s.get("d6", []):  # after the ball joints: joint indices in scene order
+            q = self.add_d6_joint(ids[int(j["a"])], ids[int(j["b"])], j["fa"], j["fb"], j.get("t_mask", 7), j.get("r_mask", 0), j.get("hinge", -1))
+            for ang, ax, lo, hi in j.get("limits", []):
+                check(L.rphys_world_set_joint_limit(self.h, q, int(ang), ax, 1, lo, hi))
+            for ang, ax, target, kp, kd, mx, vel in j.get("drives", []):
+                check(L.rphys_world_set_joint_drive(self.h, q, int(ang), ax, 1, target, kp, kd, mx, int(vel)))
+            for ang, ax, c, fr in j.get("damping", []):
+                check(L.rphys_world_set_joint_damping(self.h, q, int(ang), ax, c, fr))
+            for ang, ax, c in j.get("compliance", []):
+                check(L.rphys_world_set_joint_compliance(self.h, q, int(ang), ax, c))
         hf = s.get("hf")
         if hf is not None:
             self.set_heightfield(hf)
@@ -445,6 +463,20 @@ class World:
                                                C.byref(iso(frame_b[:3], frame_b[3:])), C.byref(out)))
         return out.value
 
+    def add_d6_joint(self, body_a, body_b, frame_a, frame_b, t_mask=7, r_mask=0, hinge=-1):
+        """D6 joint constraining the translational axes of t_mask and rotational axes of r_mask
+        (bit i: axis i of frame A), with stored hinge axis `hinge` (-1: none); returns its index."""
+        out = C.c_uint32(0)
+        check(lib().rphys_world_add_d6_joint(self.h, body_a, body_b, C.byref(iso(frame_a[:3], frame_a[3:])),
+                                             C.byref(iso(frame_b[:3], frame_b[3:])), t_mask, r_mask, hinge, C.byref(out)))
+        return out.value
+
+    def add_hinge_joint(self, body_a, body_b, frame_a, frame_b, axis):
+        out = C.c_uint32(0)
+        check(lib().rphys_world_add_hinge_joint(self.h, body_a, body_b, C.byref(iso(frame_a[:3], frame_a[3:])),
+                                                C.byref(iso(frame_b[:3], frame_b[3:])), axis, C.byref(out)))
+        return out.value
+
     def set_heightfield(self, hf):
         """terrain.rs via rphys_world_set_heightfield_i16 is lossy for float heights, so the
         raw32 square entry is used when the grid is square and the cell sizes agree."""
@@ -514,6 +546,22 @@ class World:
         arr = (ContactEx * max(n.value, 1))()
         check(lib().rphys_debug_get_contacts_ex(self.h, arr, n.value, C.byref(n)))
         return np.ctypeslib.as_array(arr)[: n.value].copy()
+
+    def hash_input(self):
+        """The bytes the step hash digests (rphys_debug_get_hash_input)."""
+        n = C.c_uint64(0)
+        check(lib().rphys_debug_get_hash_input(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint8)
+        check(lib().rphys_debug_get_hash_input(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8)), n.value, C.byref(n)))
+        return bytes(out[:n.value])
+
+    def joint_rows(self, q):
+        """Joint q's stored rows as (tags, impulses) (rphys_debug_get_joint_rows)."""
+        tags = (C.c_uint8 * 32)()
+        lams = np.zeros(32, np.float32)
+        n = C.c_uint32(0)
+        check(lib().rphys_debug_get_joint_rows(self.h, q, tags, lams.ctypes.data_as(PF32), 32, C.byref(n)))
+        return list(tags)[:n.value], lams[:n.value].copy()
 
     def export_state(self):
         """Bodies, sleep state, last velocities and the warm cache (rphys_debug_export_state)."""

@@ -279,6 +279,26 @@ def ragdolls_small(seed=11, n_towers=4, tower_h=5, n_ragdolls=6, chain=15, hf_n=
                            hf_n=hf_n, cell=0.5, extent=14.0, margin=4.0, drop=3.0)
 
 
+def d6_ragdolls(seed=11, n_towers=3, tower_h=4, n_ragdolls=4):
+    """ragdolls_small with two of every three chain joints made D6 joints: hinges about z limited to
+    +-1 rad with a damper, and twist-free swing joints limited to +-0.5 rad about x and y with
+    friction on the twist; the rest stay ball joints, all in the same islands as the contacts."""
+    s = ragdolls_small(seed=seed, n_towers=n_towers, tower_h=tower_h, n_ragdolls=n_ragdolls)
+    balls, d6 = [], []
+    for k, (a, b, fa, fb) in enumerate(s["joints"]):
+        fa, fb = np.asarray(fa, np.float32), np.asarray(fb, np.float32)
+        if k % 3 == 0:
+            d6.append(dict(a=a, b=b, fa=fa, fb=fb, t_mask=7, r_mask=3, hinge=2, limits=[(True, 2, -1.0, 1.0)], damping=[(True, 2, 0.5, 0.0)]))
+        elif k % 3 == 1:
+            d6.append(dict(a=a, b=b, fa=fa, fb=fb, t_mask=7, r_mask=0, hinge=-1, limits=[(True, 0, -0.5, 0.5), (True, 1, -0.5, 0.5)],
+                           damping=[(True, 2, 0.0, 0.05)]))
+        else:
+            balls.append((a, b, fa, fb))
+    s["joints"] = balls
+    s["d6"] = d6
+    return s
+
+
 # ---- bounce scenes: the restitution cases of crates/riftphys-world/tests/restitution_toi_test.rs
 # (static box floor with its top at y = 0, boxes and balls of given mass and material id, sleep
 # disabled on the moving bodies, gravity -9.81 or none).  Material ids are the reference's core

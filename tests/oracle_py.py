@@ -99,6 +99,8 @@ def lib():
         L.or_capacity.restype = C.c_uint32
         L.or_capacity.argtypes = [P]
         L.or_step_hash.argtypes = [P, C.c_char_p]
+        L.or_step_hash_preimage.restype = C.c_size_t
+        L.or_step_hash_preimage.argtypes = [P, C.c_void_p, C.c_size_t]
         L.or_num_bodies.restype = C.c_uint32
         L.or_num_bodies.argtypes = [P]
         L.or_tick.restype = C.c_uint64
@@ -285,6 +287,13 @@ class OracleWorld:
         lib().or_step_hash(self.h, out)
         return out.raw
 
+    def hash_input(self):
+        """The bytes the step hash digests."""
+        n = lib().or_step_hash_preimage(self.h, None, 0)
+        out = C.create_string_buffer(max(n, 1))
+        lib().or_step_hash_preimage(self.h, out, n)
+        return out.raw[:n]
+
     def state(self):
         n = lib().or_num_bodies(self.h)
         pos = np.zeros((n, 3), np.float32); rot = np.zeros((n, 4), np.float32)
@@ -322,8 +331,8 @@ class OracleWorld:
 
     def joint_warm(self, q):
         """Joint q's stored rows as (tags, impulses)."""
-        tags = (C.c_uint8 * 3)()
-        lams = np.zeros(3, np.float32)
+        tags = (C.c_uint8 * 32)()  # a D6 joint holds up to 30 rows
+        lams = np.zeros(32, np.float32)
         n = lib().or_get_joint_warm(self.h, q, tags, fptr(lams))
         return list(tags)[:n], lams[:n].copy()
 

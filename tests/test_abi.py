@@ -289,3 +289,42 @@ def test_ball_joint_export_validates_its_arguments():
         assert L.rphys_world_add_ball_joint(w, a, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.ERR_INVALID_ID
     finally:
         L.rphys_world_destroy(w)
+
+
+def test_d6_joint_exports_validate_their_arguments():
+    L = rphys.lib()
+    w = L.rphys_world_create(4, 4)
+    try:
+        a, b = C.c_uint32(), C.c_uint32()
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(a)) == rphys.OK
+        assert L.rphys_add_box_body(w, C.byref(box_desc()), C.byref(b)) == rphys.OK
+        fa, fb = rphys.iso((0, 0.5, 0), (0, 0, 0, 1)), rphys.iso((0, -0.5, 0), (0, 0, 0, 1))
+        j = C.c_uint32(7)
+        assert L.rphys_world_add_d6_joint(w, a, b, C.byref(fa), C.byref(fb), 8, 0, -1, C.byref(j)) == rphys.ERR_BAD_ARG
+        assert j.value == 0xFFFFFFFF
+        assert L.rphys_world_add_d6_joint(w, a, b, C.byref(fa), C.byref(fb), 7, 0, 3, C.byref(j)) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_add_d6_joint(w, a, a, C.byref(fa), C.byref(fb), 7, 0, -1, C.byref(j)) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_add_d6_joint(None, a, b, C.byref(fa), C.byref(fb), 7, 0, -1, C.byref(j)) == rphys.ERR_NULL
+        assert L.rphys_world_add_hinge_joint(w, a, b, C.byref(fa), C.byref(fb), 3, C.byref(j)) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_num_joints(w) == 0
+        assert L.rphys_world_add_ball_joint(w, a, b, C.byref(fa), C.byref(fb), C.byref(j)) == rphys.OK and j.value == 0
+        assert L.rphys_world_add_hinge_joint(w, a, b, C.byref(fa), C.byref(fb), 2, C.byref(j)) == rphys.OK and j.value == 1
+        assert L.rphys_world_add_d6_joint(w, b, a, C.byref(fb), C.byref(fa), 7, 5, -1, C.byref(j)) == rphys.OK and j.value == 2
+        assert L.rphys_world_num_joints(w) == 3
+        # setters: a ball joint has no axes; unknown joints; axes 0..2; finite values
+        assert L.rphys_world_set_joint_limit(w, 0, 1, 2, 1, -1.0, 1.0) == rphys.ERR_BAD_ARG and "D6" in last_error()
+        assert L.rphys_world_set_joint_limit(w, 9, 1, 2, 1, -1.0, 1.0) == rphys.ERR_INVALID_ID
+        assert L.rphys_world_set_joint_limit(w, 1, 1, 3, 1, -1.0, 1.0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_joint_limit(w, 1, 1, 2, 1, float("nan"), 1.0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_joint_limit(w, 1, 1, 2, 0, float("nan"), 1.0) == rphys.OK  # removing needs no bounds
+        assert L.rphys_world_set_joint_limit(w, 1, 1, 2, 1, -1.0, 1.0) == rphys.OK
+        assert L.rphys_world_set_joint_drive(w, 1, 1, 2, 1, float("inf"), 0.0, 0.0, 1.0, 1) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_joint_drive(w, 2, 0, 1, 1, 0.5, 0.0, 0.0, 1.0, 0) == rphys.OK
+        assert L.rphys_world_set_joint_drive(None, 2, 0, 1, 1, 0.5, 0.0, 0.0, 1.0, 0) == rphys.ERR_NULL
+        assert L.rphys_world_set_joint_damping(w, 1, 1, 2, -3.0, float("nan")) == rphys.OK  # clamped to 0, as the reference does
+        assert L.rphys_world_set_joint_damping(w, 0, 1, 2, 1.0, 1.0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_joint_compliance(w, 1, 1, 2, float("inf")) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_set_joint_compliance(w, 1, 1, 2, 0.01) == rphys.OK
+        assert L.rphys_world_set_joint_compliance(w, 7, 1, 2, 0.01) == rphys.ERR_INVALID_ID
+    finally:
+        L.rphys_world_destroy(w)

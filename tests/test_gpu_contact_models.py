@@ -161,3 +161,12 @@ def test_world_snapshot_carries_the_patch_impulses():
         b.append(g.hash())
     g.free_snapshot(snap)
     assert a == b
+
+
+def test_patch_world_with_spare_capacity_matches_the_oracle():
+    """A world whose capacity exceeds its bodies hashes sleep slots past the live bodies after the
+    carried patch entries: hashes agree while patches are carried."""
+    s = patch_pile(seed=11, n=24)
+    n = len(s["shape"])
+    o, g = run_pair(s, 60, cap=n + 9, every=3)
+    assert o.stats().patches > 0

@@ -198,6 +198,14 @@ RPhysResult rphys_world_add_ball_joint(RPhysWorld *world, uint32_t body_a, uint3
 RPhysResult rphys_world_add_d6_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,
                                      const RPhysIsometry *frame_b, uint32_t t_mask, uint32_t r_mask, int32_t hinge_axis,
                                      uint32_t *out_joint_id);
+/* replaces World::add_distance_joint (crates/riftphys-world/src/world/world.rs:537-539;
+ * articulation/src/bodies.rs:273-285): the two bodies' centres held `rest` apart (a non-finite
+ * rest becomes 0), rigid (compliance 0: the legacy Baumgarte velocity row of solver.rs:795-820 in
+ * every velocity sweep, not warm-started, and the XPBD position row of bodies.rs:501-548) or
+ * compliant (compliance m/N: the position row only; negative or non-finite becomes 0).  Islands
+ * holding one are solved by the workgroup-per-island tier.  Errors as rphys_world_add_ball_joint. */
+RPhysResult rphys_world_add_distance_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, float rest, float compliance,
+                                           uint32_t *out_joint_id);
 /* bodies.rs:333-338 add_hinge: all translational axes and the two rotational axes other than
  * `axis` locked, `axis` stored as the hinge axis.  Errors: axis above 2 -> ERR_BAD_ARG. */
 RPhysResult rphys_world_add_hinge_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, const RPhysIsometry *frame_a,

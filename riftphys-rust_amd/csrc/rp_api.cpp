@@ -331,7 +331,7 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     for (auto *w : ws) npk += (uint32_t)w->pkey.size();
     caps.warm = std::max(caps.warm, std::max(nwarm, npk) + 1024u);
     bool any_d6 = false;
-    for (auto *w : ws) for (uint32_t k : w->jt_kind) any_d6 |= k == RP_JK_D6;
+    for (auto *w : ws) for (uint32_t k : w->jt_kind) any_d6 |= k != 0;  // D6 or distance joints
     d.d6_on = any_d6 ? 1u : 0u;
     d.cm_on = (scene_cm(ws) || any_d6) ? 1u : 0u;  // D6 joints are solved by the contact-model tier
     d.nb = nb; d.nc = nc; d.nw = nw; d.nj = nj;
@@ -1756,6 +1756,23 @@ RPhysResult rphys_world_add_d6_joint(RPhysWorld *world, uint32_t body_a, uint32_
         const uint32_t en = ((k < 3 ? t_mask >> k : r_mask >> (k - 3)) & 1u) ? D6F_ENABLED_BIT : 0u;
         w.jt_ax[18 * q + 3 * k].w = u2f(en);
     }
+    host_changed(world);
+    return RPHYS_OK;
+    GUARD_END
+}
+// world.rs:537-539 add_distance_joint -> bodies.rs:273-285: bodies' centres held at rest (rest =
+// fin(rest), compliance = max(fin(compliance), 0)); frames are the bodies' own
+RPhysResult rphys_world_add_distance_joint(RPhysWorld *world, uint32_t body_a, uint32_t body_b, float rest, float compliance, uint32_t *out_joint_id) {
+    GUARD_BEGIN
+    if (out_joint_id) *out_joint_id = 0xFFFFFFFFu;
+    const RPhysIsometry id = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 1.0f}};
+    RPhysResult r = rphys_world_add_ball_joint(world, body_a, body_b, &id, &id, out_joint_id);
+    if (r != RPHYS_OK) return r;
+    std::lock_guard<std::mutex> lk(world->mu);
+    HostWorld &w = world->w;
+    const uint32_t q = *out_joint_id;
+    w.jt_kind[q] = RP_JK_DIST;
+    w.jt_ax[18 * q] = f4(hfin(rest) ? rest : 0.0f, std::fmax(hfin(compliance) ? compliance : 0.0f, 0.0f), 0.0f);
     host_changed(world);
     return RPHYS_OK;
     GUARD_END

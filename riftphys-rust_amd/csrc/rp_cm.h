@@ -149,7 +149,7 @@ __global__ void k_isl_cm(Dev d, const uint32_t *keys) {
     const uint32_t nislc = min(d.cnt[CT_ISLC], d.cap_contacts);
     GRID_STRIDE(r, nislc) {
         const uint32_t e = d.isl_v2[r];
-        const bool cm = (e & RP_PBIT) || (!(e & RP_JBIT) && (cm_flags(d, e) & CM_SOFT)) || ((e & RP_JBIT) && d6_is(d, e & ~RP_JBIT));
+        const bool cm = (e & RP_PBIT) || (!(e & RP_JBIT) && (cm_flags(d, e) & CM_SOFT)) || ((e & RP_JBIT) && jx_is(d, e & ~RP_JBIT));
         if (!cm) continue;
         const bool start = r == 0 || keys[r - 1] != keys[r];
         const uint32_t isl = start ? d.isl_flag[r] : d.isl_flag[r] - 1u;  // exclusive scan of the starts
@@ -282,11 +282,13 @@ static __device__ uint32_t cm_contact_pgs(const Dev &d, uint32_t c) {
 static __device__ __forceinline__ void cm_warm(const Dev &d, const TickParams &tp, uint32_t e) {
     if (e & RP_PBIT) patch_warm(d, tp, e & ~RP_PBIT);
     else if ((e & RP_JBIT) && d6_is(d, e & ~RP_JBIT)) d6_warm(d, e & ~RP_JBIT);
+    else if ((e & RP_JBIT) && dist_is(d, e & ~RP_JBIT)) return;  // a distance row starts cold
     else warm_row(d, tp, e);
 }
-static __device__ __forceinline__ void cm_pgs(const Dev &d, uint32_t e, ResidSink sink) {
+static __device__ __forceinline__ void cm_pgs(const Dev &d, const TickParams &tp, uint32_t e, ResidSink sink) {
     uint32_t r;
     if ((e & RP_JBIT) && d6_is(d, e & ~RP_JBIT)) r = d6_pgs(d, e & ~RP_JBIT, sink.on);
+    else if ((e & RP_JBIT) && dist_is(d, e & ~RP_JBIT)) r = dist_vel(d, e & ~RP_JBIT, tp.dt);
     else if (e & RP_JBIT) r = joint_pgs(d, e & ~RP_JBIT, sink.on);
     else if (e & RP_PBIT) r = patch_pgs(d, e & ~RP_PBIT);
     else r = cm_contact_pgs(d, e);
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(256) k_sweep_cm(Dev d, TickParams tp) {
                 if (threadIdx.x == 0) resid_bits = 0;
                 __syncthreads();
             }
-            run_sweep(I, sb, [&](uint32_t e) { cm_pgs(d, e, rb_lds); });
+            run_sweep(I, sb, [&](uint32_t e) { cm_pgs(d, tp, e, rb_lds); });
             used++;
             if (tp.resid_on) {
                 const float rmax = __uint_as_float(resid_bits);

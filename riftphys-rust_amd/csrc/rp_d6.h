@@ -27,6 +27,13 @@ static __device__ __forceinline__ D6AxD d6_ax(const Dev &d, uint32_t q, int k) {
     return x;
 }
 static __device__ __forceinline__ bool d6_is(const Dev &d, uint32_t q) { return d.d6_on && d.jt_kind[q] == RP_JK_D6; }
+// a distance joint (the joint arrays above exist when a scene has D6 or distance joints)
+static __device__ __forceinline__ bool dist_is(const Dev &d, uint32_t q) { return d.d6_on && d.jt_kind[q] == RP_JK_DIST; }
+// a D6 or distance joint: its islands go to the contact-model tier
+static __device__ __forceinline__ bool jx_is(const Dev &d, uint32_t q) { return d.d6_on && d.jt_kind[q] != 0u; }
+// a distance joint's rest length and compliance (bodies.rs:273-285), in jt_ax[18 q]
+static __device__ __forceinline__ float dist_rest(const Dev &d, uint32_t q) { return d.jt_ax[18 * q].x; }
+static __device__ __forceinline__ float dist_compliance(const Dev &d, uint32_t q) { return d.jt_ax[18 * q].y; }
 // a rotational velocity drive: its bodies keep the velocity solve's angular velocity (step.rs:353-365)
 static __device__ __forceinline__ bool d6_motor(const Dev &d, uint32_t q) {
     for (int k = 3; k < 6; k++)
@@ -309,6 +316,58 @@ static __device__ void d6_pos(const Dev &d, uint32_t q, float dt) {
         joint_orient_delta(d, a, fin_v((-mv(iaw, un)) * dl));
         joint_orient_delta(d, b, fin_v(mv(ibw, un) * dl));
     }
+}
+// solver.rs:795-820: a rigid distance joint's legacy row, run with the joint rows in every velocity
+// sweep: a Baumgarte velocity row along the centre line, neither warm-started nor accumulated.
+// Returns the residual's bits, |j| (wa + wb).
+static __device__ uint32_t dist_vel(const Dev &d, uint32_t q, float dt) {
+    if (dist_compliance(d, q) > 0.0f) return 0u;  // a compliant joint acts at position level only
+    const uint32_t a = d.jt_a[q], b = d.jt_b[q];
+    const V3 dir = L3(d.pos[b]) - L3(d.pos[a]);
+    const float len = len3(dir);
+    if (len < 1.0e-6f) return 0u;
+    const V3 n = dir / len;
+    const float ia = d.inv_mass[a], ib = d.inv_mass[b];
+    const float isum = ia + ib;
+    if (isum < 1.0e-9f) return 0u;
+    const float vrel = dot3(L3(d.lin[b]) - L3(d.lin[a]), n);
+    const float bias = (len - dist_rest(d, q)) * (0.1f / dt);
+    const float jm = -(vrel + bias) / isum;
+    const V3 pa = (-n) * jm, pb = n * jm;
+    if (ia > 0.0f && fin3(pa)) d.lin[a] = S3(fin_v(L3(d.lin[a]) + pa * ia));
+    if (ib > 0.0f && fin3(pb)) d.lin[b] = S3(fin_v(L3(d.lin[b]) + pb * ib));
+    return __float_as_uint(fabsf(jm) * isum);
+}
+// bodies.rs:501-548 solve_distance_row: the XPBD row along the step-start centre line (SHAKE)
+// unless the rod turned by 0.5 rad or more, its multiplier accumulated over the step's passes in
+// d6_plam[6 q]
+static __device__ void dist_pos(const Dev &d, uint32_t q, float dt) {
+    const uint32_t a = d.jt_a[q], b = d.jt_b[q];
+    const float wa = d.inv_mass[a], wb = d.inv_mass[b];
+    const float wsum = jfin(wa + wb);
+    if (wsum <= 0.0f) return;
+    const V3 dd = L3(d.pos[b]) - L3(d.pos[a]);
+    const float l2 = len2(dd);
+    if (!isfinite(l2) || l2 <= 1.0e-20f) return;
+    const float len = sqrtf(l2);
+    if (len <= 1.0e-6f) return;
+    const V3 ncur = normalize_fin(dd / len);
+    if (zero3(ncur)) return;
+    V3 n = ncur;
+    const V3 n0 = normalize_fin(L3(d.opos[b]) - L3(d.opos[a]));
+    if (!zero3(n0) && dot3(n0, ncur) > __uint_as_float(0x3f60a940u)) n = n0;  // cos 0.5 as f32
+    const float c = jfin(len - jfin(dist_rest(d, q)));
+    const float dt2 = fmaxf(dt * dt, 1.0e-12f);
+    const float comp = dist_compliance(d, q);
+    const float alpha = comp <= 0.0f ? 0.0f : jfin(fmaxf(jfin(comp), 0.0f) / dt2);
+    const float denom = jfin(wsum + alpha);
+    if (denom <= 0.0f) return;
+    const float lold = jfin(d.d6_plam[6 * q]);
+    const float lam = jfin(-(c + jfin(alpha * lold)) / (denom * dot3(n, ncur)));
+    if (fabsf(lam) <= 1.0e-9f) return;
+    d.d6_plam[6 * q] = jfin(lold + lam);
+    joint_pos_delta(d, a, fin_v((-n) * jfin(lam * wa)));
+    joint_pos_delta(d, b, fin_v(n * jfin(lam * wb)));
 }
 // Joints::begin_step (bodies.rs:308-318): the step's position multipliers start at zero
 __global__ void k_d6_begin(Dev d) {

@@ -17,7 +17,8 @@ enum { FL_CONSTRAINED = 1u, FL_LOADED = 2u, FL_GROUND = 4u, FL_JOINTED = 8u, FL_
 // FL_JOINTED: in a joint (step.rs:164-173); FL_JW: written by a joint row this tick (a dynamic
 // body of an active joint, asleep or not); FL_DRIVEN: in a D6 joint with a rotational velocity
 // drive (its angular velocity is the velocity solve's, step.rs:353-365)
-#define RP_JK_D6 2u  // jt_kind of a D6 joint (0: ball)
+#define RP_JK_D6 2u    // jt_kind of a D6 joint (0: ball)
+#define RP_JK_DIST 1u  // jt_kind of a distance joint
 
 // Island-list entries are contact indices, or RP_JBIT | joint index for a ball joint's rows, or
 // RP_PBIT | patch index for a friction patch (scenes with a contact-model pair: contact indices

@@ -685,6 +685,8 @@ static __device__ __forceinline__ void d6_store(const Dev &d, uint32_t q);
 static __device__ __forceinline__ bool d6_is(const Dev &d, uint32_t q);
 static __device__ __forceinline__ bool d6_motor(const Dev &d, uint32_t q);
 static __device__ uint32_t d6_rows(const Dev &d, uint32_t q, float dt);
+static __device__ __forceinline__ bool dist_is(const Dev &d, uint32_t q);
+static __device__ __forceinline__ float dist_compliance(const Dev &d, uint32_t q);
 __global__ void k_joint_prep(Dev d, TickParams tp) {
     if (aborted(d)) return;
     GRID_STRIDE(q, d.nj) {
@@ -693,21 +695,39 @@ __global__ void k_joint_prep(Dev d, TickParams tp) {
         if ((d.flags[b] & (FL_CONSTRAINED | FL_JOINTED)) != (FL_CONSTRAINED | FL_JOINTED)) atomicOr(&d.flags[b], FL_CONSTRAINED | FL_JOINTED);
         const float wa = d.inv_mass[a], wb = d.inv_mass[b];
         uint32_t m = 0;
+        // A D6 or distance joint with a dynamic side is an island entry even without velocity rows
+        // (its position row runs in its island's passes), joining its two bodies' islands.
         if (d6_is(d, q)) {  // a D6 joint: its rows (rp_d6.h); velocity motors mark their bodies
             if (d6_motor(d, q)) { atomicOr(&d.flags[a], FL_DRIVEN); atomicOr(&d.flags[b], FL_DRIVEN); }
             uint32_t n = 0;
-            if (wa > 0.0f || wb > 0.0f) {
+            const bool act = wa > 0.0f || wb > 0.0f;
+            if (act) {
                 const uint32_t w = d.bworld[a];
                 if (!d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
                 n = d6_rows(d, q, tp.dt);
                 if (n) {
                     if (wa > 0.0f && !(d.flags[a] & FL_JW)) atomicOr(&d.flags[a], FL_JW);
                     if (wb > 0.0f && !(d.flags[b] & FL_JW)) atomicOr(&d.flags[b], FL_JW);
-                    if (wa > 0.0f && wb > 0.0f) uf_union(d.root, a, b);
                 }
+                if (wa > 0.0f && wb > 0.0f) uf_union(d.root, a, b);
             }
             d.d6_nrows[q] = n;
-            d.jr_mask[q] = n ? 1u : 0u;
+            d.jr_mask[q] = act ? 1u : 0u;
+            continue;
+        }
+        if (dist_is(d, q)) {  // a distance joint: a rigid one's legacy row writes each side with mass
+            const bool act = wa > 0.0f || wb > 0.0f;
+            if (act) {
+                const uint32_t w = d.bworld[a];
+                if (!d.w_solve[w]) atomicOr(&d.w_solve[w], 1u);
+                if (!(dist_compliance(d, q) > 0.0f)) {
+                    if (wa > 0.0f && !(d.flags[a] & FL_JW)) atomicOr(&d.flags[a], FL_JW);
+                    if (wb > 0.0f && !(d.flags[b] & FL_JW)) atomicOr(&d.flags[b], FL_JW);
+                }
+                if (wa > 0.0f && wb > 0.0f) uf_union(d.root, a, b);
+            }
+            d.d6_nrows[q] = 0;
+            d.jr_mask[q] = act ? 1u : 0u;
             continue;
         }
         if (wa > 0.0f || wb > 0.0f) {
@@ -1429,6 +1449,7 @@ static __device__ uint32_t joint_pgs(const Dev &d, uint32_t q, bool resid) {
 }
 // joint q's rows become its warm rows (a joint without rows keeps its old ones)
 static __device__ __forceinline__ void joint_store(const Dev &d, uint32_t q) {
+    if (dist_is(d, q)) return;  // a distance row is not carried
     if (d6_is(d, q)) { d6_store(d, q); return; }
     const uint32_t m = d.jr_mask[q];
     if (!m) return;
@@ -1461,6 +1482,7 @@ static __device__ void joint_pos(const Dev &d, uint32_t q, float dt) {
     const bool fa = !((d.bflags[a] & BF_DYN) && d.awake[a]), fb = !((d.bflags[b] & BF_DYN) && d.awake[b]);
     if (fa && fb) return;
     if (d6_is(d, q)) { d6_pos(d, q, dt); return; }
+    if (dist_is(d, q)) { dist_pos(d, q, dt); return; }
     const float wa = jfin(d.inv_mass[a]), wb = jfin(d.inv_mass[b]);
     const Q4 sa = LQ(d.orot[a]), sb = LQ(d.orot[b]);
     const float cmin = __uint_as_float(RP_SHAKE_COS_BITS);

@@ -178,6 +178,7 @@ EXPORTS = [
     ("rphys_world_set_compliant_damping_at_start", U32, [P, U32]),
     ("rphys_world_add_ball_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), PU32]),
     ("rphys_world_add_d6_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), U32, U32, C.c_int32, PU32]),
+    ("rphys_world_add_distance_joint", U32, [P, U32, U32, C.c_float, C.c_float, PU32]),
     ("rphys_world_add_hinge_joint", U32, [P, U32, U32, C.POINTER(Isometry), C.POINTER(Isometry), U32, PU32]),
     ("rphys_world_set_joint_limit", U32, [P, U32, U32, U32, U32, C.c_float, C.c_float]),
     ("rphys_world_set_joint_drive", U32, [P, U32, U32, U32, U32, C.c_float, C.c_float, C.c_float, C.c_float, U32]),
@@ -372,7 +373,9 @@ class World:
                 check(L.rphys_body_set_sleep_allowed(self.h, b, 0))
         for a, b, fa, fb in s.get("joints", []):  # scene joints name bodies by scene index
             self.add_ball_joint(ids[a], ids[b], fa, fb)
-        for j in s.get("d6", []):  # after the ball joints: joint indices in scene order
+        for a, b, rest, comp in s.get("dist", []):  # after the ball joints, before the D6 joints
+            self.add_distance_joint(ids[int(a)], ids[int(b)], rest, comp)
+        for j in s.get("d6", []):  # after the ball and distance joints: joint indices in scene order
             q = self.add_d6_joint(ids[int(j["a"])], ids[int(j["b"])], j["fa"], j["fb"], j.get("t_mask", 7), j.get("r_mask", 0), j.get("hinge", -1))
             for ang, ax, lo, hi in j.get("limits", []):
                 check(L.rphys_world_set_joint_limit(self.h, q, int(ang), ax, 1, lo, hi))
@@ -469,6 +472,11 @@ class World:
         out = C.c_uint32(0)
         check(lib().rphys_world_add_d6_joint(self.h, body_a, body_b, C.byref(iso(frame_a[:3], frame_a[3:])),
                                              C.byref(iso(frame_b[:3], frame_b[3:])), t_mask, r_mask, hinge, C.byref(out)))
+        return out.value
+
+    def add_distance_joint(self, body_a, body_b, rest, compliance=0.0):
+        out = C.c_uint32(0)
+        check(lib().rphys_world_add_distance_joint(self.h, body_a, body_b, float(rest), float(compliance), C.byref(out)))
         return out.value
 
     def add_hinge_joint(self, body_a, body_b, frame_a, frame_b, axis):

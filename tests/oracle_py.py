@@ -194,7 +194,9 @@ class OracleWorld:
             fa = np.ascontiguousarray(fa, np.float32)
             fb = np.ascontiguousarray(fb, np.float32)
             assert L.or_add_ball_joint(self.h, int(a), int(b), fptr(fa), fptr(fb)) >= 0
-        for j in s.get("d6", []):  # after the ball joints: joint indices in scene order
+        for a, b, rest, comp in s.get("dist", []):  # after the ball joints, before the D6 joints
+            assert L.or_add_distance_joint(self.h, int(a), int(b), float(rest), float(comp)) >= 0
+        for j in s.get("d6", []):  # after the ball and distance joints: joint indices in scene order
             q = self.add_d6_joint(int(j["a"]), int(j["b"]), j["fa"], j["fb"], j.get("t_mask", 7), j.get("r_mask", 0), j.get("hinge", -1))
             for ang, ax, lo, hi in j.get("limits", []):
                 L.or_set_d6_limit(self.h, q, int(ang), ax, 1, lo, hi)

@@ -326,5 +326,12 @@ def test_d6_joint_exports_validate_their_arguments():
         assert L.rphys_world_set_joint_compliance(w, 1, 1, 2, float("inf")) == rphys.ERR_BAD_ARG
         assert L.rphys_world_set_joint_compliance(w, 1, 1, 2, 0.01) == rphys.OK
         assert L.rphys_world_set_joint_compliance(w, 7, 1, 2, 0.01) == rphys.ERR_INVALID_ID
+        # distance joints: the ball joint's body checks; a distance joint has no D6 axes
+        assert L.rphys_world_add_distance_joint(w, a, a, 1.0, 0.0, C.byref(j)) == rphys.ERR_BAD_ARG and j.value == 0xFFFFFFFF
+        assert L.rphys_world_add_distance_joint(w, 9, b, 1.0, 0.0, C.byref(j)) == rphys.ERR_INVALID_ID
+        assert L.rphys_world_add_distance_joint(None, a, b, 1.0, 0.0, C.byref(j)) == rphys.ERR_NULL
+        assert L.rphys_world_add_distance_joint(w, a, b, float("nan"), -1.0, C.byref(j)) == rphys.OK and j.value == 3
+        assert L.rphys_world_set_joint_limit(w, 3, 0, 0, 1, -1.0, 1.0) == rphys.ERR_BAD_ARG
+        assert L.rphys_world_num_joints(w) == 4
     finally:
         L.rphys_world_destroy(w)

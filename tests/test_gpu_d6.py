@@ -159,3 +159,31 @@ def test_world_snapshot_carries_the_d6_rows():
         b.append(g.hash())
     g.free_snapshot(snap)
     assert a == b
+
+
+@pytest.fixture
+def caps():
+    L = rphys.lib()
+    yield L
+    L.rphys_debug_set_initial_capacity(0, 0, 0, 0)
+
+
+def test_overflow_replay_with_d6_joints_bit_exact(caps):
+    """Tiny initial capacities on the D6 ragdolls: the first ticks with contacts overflow and replay,
+    the D6 joints' carried rows restored from their backup; bit-exact with the oracle every tick."""
+    caps.rphys_debug_set_initial_capacity(64, 256, 64, 16)
+    s = scenes.d6_ragdolls(seed=13, n_towers=4, tower_h=5, n_ragdolls=6)
+    run_pair(s, 60)
+
+
+def test_residual_mode_with_d6_joints_matches_the_oracle():
+    s = scenes.d6_ragdolls(seed=17)
+    s["residual"] = (1.0e-4, 60)
+    o = oracle_py.OracleWorld(s)
+    g = rphys.World(s)
+    for t in range(80):
+        o.step()
+        g.step()
+        assert o.hash() == g.hash(), "tick %d" % (t + 1)
+        so, sg = o.stats(), g.stats()
+        assert (so.iters_used, so.iters_sum) == (sg.pgs_iters, sg.pgs_iters_sum), (t + 1, so.iters_used, sg.pgs_iters)

@@ -334,6 +334,8 @@ static Scene *build_scene(const std::vector<HostWorld *> &ws, Caps caps, uint32_
     for (auto *w : ws) for (uint32_t k : w->jt_kind) any_d6 |= k != 0;  // D6 or distance joints
     d.d6_on = any_d6 ? 1u : 0u;
     d.cm_on = (scene_cm(ws) || any_d6) ? 1u : 0u;  // D6 joints are solved by the contact-model tier
+    // the broadphase runs 32 lanes per collider over a 32-bit index space
+    if (nc > (1u << 27)) { set_err("scene exceeds 134,217,728 colliders"); return nullptr; }
     d.nb = nb; d.nc = nc; d.nw = nw; d.nj = nj;
     d.cap_pairs = caps.pairs; d.cap_cands = caps.cands; d.cap_contacts = caps.contacts; d.cap_warm = caps.warm;
     DeviceBuffers &B = S->buf;

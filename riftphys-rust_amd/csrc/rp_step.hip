@@ -277,6 +277,12 @@ static __device__ __forceinline__ bool overlaps(float4 amn, float4 amx, float4 b
 // list's length (out_off, zeroed before), mode 1 appends through per-list cursors (bp_fill,
 // zeroed before); k_bp_sort_lists then puts each list in ascending order, so the lists do not
 // depend on which lane appended first.
+//
+// RP_BP_G lanes scan one collider's run together, RP_BP_G positions a step: a ballot finds the
+// step's first position past max.x, and only the lanes before it test and emit, so the pairs are
+// exactly the serial scan's.  Groups are aligned within a wavefront (blockDim is a multiple of 64),
+// and a group's lanes leave the loop together.
+#define RP_BP_G 32u
 template <int MODE>
 __global__ void k_bp_enum(Dev d) {
     if (aborted(d)) return;
@@ -289,25 +295,36 @@ __global__ void k_bp_enum(Dev d) {
             if (o < cap) { d.pair_i[o] = owner; d.pair_j[o] = other; }
         }
     };
-    GRID_STRIDE(c, d.nc) {
+    const uint32_t lane = threadIdx.x % RP_BP_G, shift = (threadIdx.x % 64u) - lane;
+    GRID_STRIDE(t, d.nc * RP_BP_G) {
+        const uint32_t c = t / RP_BP_G;
         const uint32_t w = d.col_world[c], c0 = d.w_col_off[w], n = d.w_col_off[w + 1] - c0, i = c - c0;
         const uint32_t nsap = d.w_nsap[w], nover = d.w_nover[w];
         const float4 mn = d.smin[c], mx = d.smax[c];
         const uint32_t cls = d.ccls[c];
         if (cls == CLS_OVER) {
-            for (uint32_t j = i + 1; j < n; j++)
+            for (uint32_t j = i + 1 + lane; j < n; j += RP_BP_G)
                 if (overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(c, c0 + j);
         } else {
             if (cls == CLS_SAP) {
                 const uint32_t r = d.bp_rank[c];
-                for (uint32_t q = r + 1; q < nsap; q++) {
-                    const uint32_t j = d.bp_ord[c0 + q];
-                    const float4 jmn = d.smin[c0 + j];
-                    if (jmn.x > mx.x) break;
-                    if (overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(min(c, c0 + j), max(c, c0 + j));
+                for (uint32_t q0 = r + 1; q0 < nsap; q0 += RP_BP_G) {
+                    const uint32_t q = q0 + lane;
+                    uint32_t j = 0;
+                    float4 jmn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    bool brk = q >= nsap;
+                    if (!brk) {
+                        j = d.bp_ord[c0 + q];
+                        jmn = d.smin[c0 + j];
+                        brk = jmn.x > mx.x;
+                    }
+                    const uint32_t gm = (uint32_t)(__ballot(brk) >> shift) & (uint32_t)((1ull << RP_BP_G) - 1ull);
+                    const uint32_t first = gm ? (uint32_t)__ffs(gm) - 1u : RP_BP_G;  // the serial scan's stop
+                    if (lane < first && overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(min(c, c0 + j), max(c, c0 + j));
+                    if (gm) break;
                 }
             }
-            for (uint32_t q = nsap; q < nsap + nover; q++) {
+            for (uint32_t q = nsap + lane; q < nsap + nover; q += RP_BP_G) {
                 const uint32_t j = d.bp_ord[c0 + q];
                 if (j > i && overlaps(mn, mx, d.smin[c0 + j], d.smax[c0 + j])) emit(c, c0 + j);
             }
@@ -3133,14 +3150,14 @@ extern "C" hipError_t rp_launch_tick(const Dev &d, const TickParams &tp, const u
         hipLaunchKernelGGL(k_bp_ranks, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d, vb);
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d.out_off, d.nc);
-    hipLaunchKernelGGL(k_bp_enum<0>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_bp_enum<0>, dim3(grid_for(d.nc * RP_BP_G, B)), dim3(B), 0, st, d);
     scan_u32(d.out_off, d.nc, d.scan_tmp, d.cnt + CT_PAIRS, st);
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.out_off, d.nc, d.cnt + CT_PAIRS);
     scan_u32(d.cand_off, d.nc, d.scan_tmp, d.cnt + CT_CANDS, st);
     hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, st, d.cand_off, d.nc, d.cnt + CT_CANDS);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, st, d, 0u);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d.bp_fill, d.nc);
-    hipLaunchKernelGGL(k_bp_enum<1>, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
+    hipLaunchKernelGGL(k_bp_enum<1>, dim3(grid_for(d.nc * RP_BP_G, B)), dim3(B), 0, st, d);
     hipLaunchKernelGGL(k_bp_sort_lists, dim3(grid_for(d.nc, B)), dim3(B), 0, st, d);
     mark(RP_ST_NARROW_PAIRS);
     hipLaunchKernelGGL(k_narrow_pairs, dim3(grid_for(d.cap_pairs, B)), dim3(B), 0, st, d);

@@ -279,6 +279,26 @@ def ragdolls_small(seed=11, n_towers=4, tower_h=5, n_ragdolls=6, chain=15, hf_n=
                            hf_n=hf_n, cell=0.5, extent=14.0, margin=4.0, drop=3.0)
 
 
+def jointed_bounce_pile(hz=60.0):
+    """The W6 pile golden's pyramid with its dropped box, struck by a box on a hinge (a D6 joint
+    about z to a static pin at (0, 1.6, 0), released level) and by a box on a rigid distance joint
+    swinging in from the side: restitution scenes with jointed bodies (immovable to W6)."""
+    s_bodies = [bounce_floor(MAT_CONCRETE)]
+    f = np.float32
+    for layer, n in enumerate([6, 5, 4, 3, 2]):
+        for i in range(n):
+            x = (f(i) - (f(n) - f(1.0)) * f(0.5)) * f(0.4)
+            s_bodies.append(bounce_body(1, BOUNCE_HALF, (x, f(0.1) + f(0.2) * f(layer), 0.0), 2.0, MAT_ALUMINUM))
+    s_bodies.append(bounce_pile_drop())
+    k = len(s_bodies)
+    s_bodies += [_static_pin((0.0, 1.6, 0.0)), bounce_body(1, BOUNCE_HALF, (0.8, 1.6, 0.0), 2.0, MAT_ALUMINUM),
+                 _static_pin((-1.4, 1.7, 0.0)), bounce_body(1, BOUNCE_HALF, (-1.4, 0.9, 0.0), 2.0, MAT_ALUMINUM, vel=(3.0, 0, 0, 0, 0, 0))]
+    s = bounce_scene_of(s_bodies, hz=hz, name="jointed_bounce_pile")
+    s["d6"] = [dict(a=k, b=k + 1, fa=_iso((0, 0, 0)), fb=_iso((-0.8, 0, 0)), t_mask=7, r_mask=3, hinge=2)]
+    s["dist"] = [(k + 2, k + 3, 0.8, 0.0)]
+    return s
+
+
 def d6_ragdolls(seed=11, n_towers=3, tower_h=4, n_ragdolls=4):
     """ragdolls_small with two of every three chain joints made D6 joints: hinges about z limited to
     +-1 rad with a damper, and twist-free swing joints limited to +-0.5 rad about x and y with

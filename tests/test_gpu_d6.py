@@ -187,3 +187,13 @@ def test_residual_mode_with_d6_joints_matches_the_oracle():
         assert o.hash() == g.hash(), "tick %d" % (t + 1)
         so, sg = o.stats(), g.stats()
         assert (so.iters_used, so.iters_sum) == (sg.pgs_iters, sg.pgs_iters_sum), (t + 1, so.iters_used, sg.pgs_iters)
+
+
+@pytest.mark.parametrize("hz,ticks", [(60.0, 180), (240.0, 480)])
+def test_jointed_bodies_in_a_restitution_pile_match_the_oracle(hz, ticks):
+    """The W6 pile with a hinged box falling onto it and a box on a distance joint swinging in:
+    restitution's time-of-impact rows and LCP treat both jointed boxes as immovable."""
+    s = scenes.jointed_bounce_pile(hz=hz)
+    s["name"] = "jointed_bounce_pile_%g" % hz
+    o, g = run_pair(s, ticks)
+    assert o.stats().contacts > 0

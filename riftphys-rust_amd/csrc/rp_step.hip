@@ -318,8 +318,8 @@ __global__ void k_bp_enum(Dev d) {
                         jmn = d.smin[c0 + j];
                         brk = jmn.x > mx.x;
                     }
-                    const uint32_t gm = (uint32_t)(__ballot(brk) >> shift) & (uint32_t)((1ull << RP_BP_G) - 1ull);
-                    const uint32_t first = gm ? (uint32_t)__ffs(gm) - 1u : RP_BP_G;  // the serial scan's stop
+                    const uint64_t gm = ((uint64_t)__ballot(brk) >> shift) & (RP_BP_G == 64u ? ~0ull : ((1ull << (RP_BP_G % 64u)) - 1ull));
+                    const uint32_t first = gm ? (uint32_t)__ffsll((unsigned long long)gm) - 1u : RP_BP_G;  // the serial scan's stop
                     if (lane < first && overlaps(mn, mx, jmn, d.smax[c0 + j])) emit(min(c, c0 + j), max(c, c0 + j));
                     if (gm) break;
                 }
